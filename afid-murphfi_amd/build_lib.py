@@ -1,0 +1,43 @@
+"""Build step for the HIP library libafid_gpu.so (gfx950 only).
+
+Used by __graft_entry__.build(); can also be run directly:
+    python afid-murphfi_amd/build_lib.py
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libafid_gpu.so")
+SRC = os.path.join(HERE, "csrc", "afid_gpu.hip")
+
+
+def hipcc_cmd(out=LIB, extra=()):
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    return [hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wall", "-Wno-unused-function",
+            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
+            *extra, "-o", out, SRC]
+
+
+def build(verbose=True):
+    cmd = hipcc_cmd(out=LIB + ".tmp")
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libafid_gpu.so")
+    if verbose and r.stderr.strip():
+        # only the harmless packed-fp32 host-pass note is expected here
+        lines = [l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l]
+        if lines:
+            sys.stderr.write("\n".join(lines) + "\n")
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,1020 @@
+// afid_gpu.hip -- context, set-up and C ABI of the MI355X RK3 substep.
+//
+// Build (gfx950 only):
+//   hipcc --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared
+//         -Xclang -target-feature -Xclang -packed-fp32-ops
+//         -I include -o afid-murphfi_amd/libafid_gpu.so afid-murphfi_amd/csrc/afid_gpu.hip
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "afid_gpu.h"
+#include "afid_setup.hpp"
+#include "afid_params.hpp"
+#include "afid_stencils.hpp"
+#include "afid_kernels.hpp"
+#include "afid_poisson.hpp"
+
+using namespace afid;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct TimerRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct afid_ctx {
+    afid_config c{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // sizes
+    int nxm, nx, ny, nz, ldx, ldt, nymh;
+    int nxmr = 0, nxr = 0, nyr = 0, nzr = 0, ldxr = 0;
+    bool sal = false;
+    // physics
+    double ren, pect, pecs, byct, bycs, Gy, rrho;
+    double dx, dy, dz, dxr = 0, dyr = 0, dzr = 0;
+    double gam[3], rom[3], alm[3];
+    // substep state
+    double dt = 0, al = 0, ga = 0, ro = 0, beta = 0;
+    int ns = 0;
+    // device memory
+    std::vector<DevBuf> bufs;
+    int64_t bytes = 0;
+    double *f[AFID_NFIELDS] = {nullptr};
+    double *rhs4[4] = {nullptr};       // rhs for vx, vy, vz, temp
+    double *tpdv = nullptr;
+    double2 *spec = nullptr;
+    // metrics
+    std::map<std::string, std::vector<double>> host1d;   // for afid_grid_array
+    const double *d_udx3m, *d_udx3c, *d_rudx3m, *d_ap3ck, *d_ac3ck, *d_am3ck, *d_ap3sk, *d_ac3sk, *d_am3sk;
+    const double *d_ap3ssk, *d_ac3ssk, *d_am3ssk, *d_amphk, *d_acphk, *d_apphk, *d_g3rc;
+    const double *d_udx3mr = nullptr, *d_ap3sskr = nullptr, *d_ac3sskr = nullptr, *d_am3sskr = nullptr, *d_lxa = nullptr;
+    const int *d_xs_src = nullptr, *d_ys_src = nullptr, *d_zs_src = nullptr, *d_xb_src = nullptr, *d_yb_src = nullptr, *d_zb_src = nullptr;
+    const double *d_xs_w = nullptr, *d_ys_w = nullptr, *d_zs_w = nullptr, *d_xb_w = nullptr, *d_yb_w = nullptr, *d_zb_w = nullptr;
+    // host copies of the 1-D coefficients for the tridiagonal factors
+    std::vector<double> h_ap3ck, h_ac3ck, h_am3ck, h_ap3sk, h_ac3sk, h_am3sk, h_ap3ssk, h_ac3ssk, h_am3ssk;
+    std::vector<double> h_ap3sskr, h_ac3sskr, h_am3sskr;
+    // tridiagonal factors: [ns][kind] -> 4 arrays of npad
+    int chunk[MAT_NKIND] = {0}, nrows[MAT_NKIND] = {0};
+    double *tri = nullptr;          // device, 3*4*4*npadmax doubles
+    int npadmax = 0;
+    double fact_dt = -1.0;
+    std::vector<double> tri_host;
+    // Poisson
+    FftPlan plan_y, plan_z;
+    int Ly = 1, Lz = 1, lds_y = 0, lds_z = 0;
+    PoissonFactors pf{};
+    int pchunk = 1;
+    // reductions
+    unsigned long long *redmax = nullptr;
+    double *redsum = nullptr, *redout = nullptr;
+    long long nred = 0;
+    // timing
+    bool timing = false;
+    std::vector<TimerRec> timers;
+    std::vector<std::pair<std::string, double>> last_timing;
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+#define HIPCHK(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);           \
+            return AFID_E_HIP;                                                         \
+        }                                                                              \
+    } while (0)
+
+static int set_err(afid_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+static int dalloc(afid_ctx *ctx, void **p, size_t bytes) {
+    bytes = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        ctx->err = "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e);
+        return AFID_E_MEM;
+    }
+    ctx->bufs.push_back({*p, bytes});
+    ctx->bytes += int64_t(bytes);
+    e = hipMemsetAsync(*p, 0, bytes, ctx->stream);
+    if (e != hipSuccess) { ctx->err = "hipMemset failed"; return AFID_E_HIP; }
+    return AFID_OK;
+}
+
+template <typename T>
+static int upload_vec(afid_ctx *ctx, const std::vector<T> &v, const T **out) {
+    void *p = nullptr;
+    int rc = dalloc(ctx, &p, sizeof(T) * v.size());
+    if (rc) return rc;
+    HIPCHK(ctx, hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    *out = static_cast<const T *>(p);
+    return AFID_OK;
+}
+
+static std::vector<double> farr_range(const FArr &a, int lo, int hi) {
+    std::vector<double> v;
+    for (int i = lo; i <= hi; ++i) v.push_back(a(i));
+    return v;
+}
+
+static bool is_refined(int f) {
+    return (f >= AFID_SAL && f <= AFID_VZR) || f == AFID_SALBP || f == AFID_SALTP;
+}
+static bool is_plane(int f) {
+    return f == AFID_TEMPBP || f == AFID_TEMPTP || f == AFID_SALBP || f == AFID_SALTP;
+}
+
+static KP make_kp(const afid_ctx *c) {
+    KP p{};
+    p.nxm = c->nxm; p.nx = c->nx; p.ny = c->ny; p.nz = c->nz; p.ldx = c->ldx;
+    p.plane = (long long)c->ny * c->ldx; p.nymh = c->nymh; p.ldt = c->ldt;
+    p.nxmr = c->nxmr; p.nxr = c->nxr; p.nyr = c->nyr; p.nzr = c->nzr; p.ldxr = c->ldxr;
+    p.planer = (long long)c->nyr * c->ldxr;
+    p.dt = c->dt; p.al = c->al; p.ga = c->ga; p.ro = c->ro; p.beta = c->beta;
+    p.ren = c->ren; p.pect = c->pect; p.pecs = c->pecs; p.byct = c->byct; p.bycs = c->bycs; p.Gy = c->Gy;
+    p.dx = c->dx; p.dy = c->dy; p.dz = c->dz; p.dxq = c->dx * c->dx; p.dyq = c->dy * c->dy; p.dzq = c->dz * c->dz;
+    p.dxr = c->dxr; p.dyr = c->dyr; p.dzr = c->dzr; p.dxqr = c->dxr * c->dxr; p.dyqr = c->dyr * c->dyr; p.dzqr = c->dzr * c->dzr;
+    p.inslws = c->c.inslws; p.inslwn = c->c.inslwn; p.tfixs = c->c.tfixs; p.tfixn = c->c.tfixn;
+    p.sfixs = c->c.sfixs; p.sfixn = c->c.sfixn; p.active_t = c->c.active_t; p.active_s = c->c.active_s; p.gaxis = c->c.gaxis;
+    p.xplusu = c->c.xplusu; p.xminusu = c->c.xminusu;
+    p.vx = c->f[AFID_VX]; p.vy = c->f[AFID_VY]; p.vz = c->f[AFID_VZ]; p.temp = c->f[AFID_TEMP]; p.pr = c->f[AFID_PR];
+    p.rux = c->f[AFID_RUX]; p.ruy = c->f[AFID_RUY]; p.ruz = c->f[AFID_RUZ]; p.rut = c->f[AFID_RUTEMP];
+    p.qcap = c->f[AFID_QCAP]; p.dph = c->f[AFID_DPH]; p.dq = c->f[AFID_DQ]; p.hro = c->f[AFID_HRO];
+    p.rhsx = c->rhs4[0]; p.rhsy = c->rhs4[1]; p.rhsz = c->rhs4[2]; p.rhst = c->rhs4[3];
+    p.salc = c->f[AFID_SALC]; p.tpdv = c->tpdv;
+    p.tempbp = c->f[AFID_TEMPBP]; p.temptp = c->f[AFID_TEMPTP];
+    p.sal = c->f[AFID_SAL]; p.rusal = c->f[AFID_RUSAL]; p.hsal = c->f[AFID_HSAL]; p.rhsr = c->f[AFID_RHSR];
+    p.vxr = c->f[AFID_VXR]; p.vyr = c->f[AFID_VYR]; p.vzr = c->f[AFID_VZR];
+    p.salbp = c->f[AFID_SALBP]; p.saltp = c->f[AFID_SALTP];
+    p.udx3m = c->d_udx3m; p.udx3c = c->d_udx3c; p.rudx3m = c->d_rudx3m;
+    p.ap3ck = c->d_ap3ck; p.ac3ck = c->d_ac3ck; p.am3ck = c->d_am3ck;
+    p.ap3sk = c->d_ap3sk; p.ac3sk = c->d_ac3sk; p.am3sk = c->d_am3sk;
+    p.ap3ssk = c->d_ap3ssk; p.ac3ssk = c->d_ac3ssk; p.am3ssk = c->d_am3ssk;
+    p.amphk = c->d_amphk; p.acphk = c->d_acphk; p.apphk = c->d_apphk; p.g3rc = c->d_g3rc;
+    p.udx3mr = c->d_udx3mr; p.ap3sskr = c->d_ap3sskr; p.ac3sskr = c->d_ac3sskr; p.am3sskr = c->d_am3sskr; p.lxa = c->d_lxa;
+    p.xs_src = c->d_xs_src; p.ys_src = c->d_ys_src; p.zs_src = c->d_zs_src;
+    p.xs_w = c->d_xs_w; p.ys_w = c->d_ys_w; p.zs_w = c->d_zs_w;
+    p.xb_src = c->d_xb_src; p.yb_src = c->d_yb_src; p.zb_src = c->d_zb_src;
+    p.xb_w = c->d_xb_w; p.yb_w = c->d_yb_w; p.zb_w = c->d_zb_w;
+    p.redmax = c->redmax; p.redsum = c->redsum;
+    return p;
+}
+
+static dim3 point_grid(int nxl, int ny, int nz) { return dim3((nxl + 63) / 64, (ny + 3) / 4, nz); }
+static const dim3 kPointBlock(64, 4, 1);
+
+static void tstart(afid_ctx *c, const char *name) {
+    if (!c->timing) return;
+    TimerRec r;
+    r.name = name;
+    (void)hipEventCreate(&r.a);
+    (void)hipEventCreate(&r.b);
+    (void)hipEventRecord(r.a, c->stream);
+    c->timers.push_back(r);
+}
+static void tstop(afid_ctx *c) {
+    if (!c->timing || c->timers.empty()) return;
+    (void)hipEventRecord(c->timers.back().b, c->stream);
+}
+
+static int check_launch(afid_ctx *c, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        c->err = std::string("launch of ") + what + " failed: " + hipGetErrorString(e);
+        return AFID_E_HIP;
+    }
+    return AFID_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tridiagonal factors (SolveImpEqnUpdate_X/YZ/Temp/Sal matrices, computed
+// on the host for each (dt, substep) and cached while dt is unchanged)
+// ---------------------------------------------------------------------------
+static void lu_rows(const std::vector<double> &am, const std::vector<double> &ac, const std::vector<double> &ap,
+                    double bd, int n, int r0, int r1, int npad, double *L, double *E, double *G, double *S) {
+    // rows r (0-based) in [r0, r1) carry coefficients; others are identity rows with zero RHS
+    std::vector<double> a(size_t(npad), 0.0), cc(size_t(npad), 0.0), s(size_t(npad), 0.0);
+    for (int r = r0; r < r1; ++r) {
+        const double sb = 1.0 / (1.0 - ac[size_t(r)] * bd);
+        a[size_t(r)] = -am[size_t(r)] * bd * sb;
+        cc[size_t(r)] = -ap[size_t(r)] * bd * sb;
+        s[size_t(r)] = 1.0 / (1.0 - ac[size_t(r)] * bd);
+    }
+    double u_prev = 1.0;
+    for (int r = 0; r < npad; ++r) {
+        if (r >= n) { L[r] = 0; E[r] = 0; G[r] = 0; S[r] = 0; continue; }
+        const double l = r > 0 ? a[size_t(r)] / u_prev : 0.0;
+        const double u = 1.0 - l * (r > 0 ? cc[size_t(r - 1)] : 0.0);
+        L[r] = l; E[r] = cc[size_t(r)] / u; G[r] = 1.0 / u; S[r] = s[size_t(r)];
+        u_prev = u;
+    }
+}
+
+static int ensure_factors(afid_ctx *c) {
+    if (c->fact_dt == c->dt) return AFID_OK;
+    const int np = c->npadmax;
+    c->tri_host.assign(size_t(3 * MAT_NKIND * 4 * np), 0.0);
+    for (int s = 0; s < c->c.nsst && s < 3; ++s) {
+        const double al = c->alm[s], beta = c->dt / c->ren * 0.5;
+        for (int kind = 0; kind < MAT_NKIND; ++kind) {
+            if (kind == MAT_S && !c->sal) continue;
+            double *base = c->tri_host.data() + size_t((s * MAT_NKIND + kind) * 4 * np);
+            double *L = base, *E = base + np, *G = base + 2 * np, *S = base + 3 * np;
+            switch (kind) {
+            case MAT_X:   // SolveImpEqnUpdate_X.F90:22-54: identity rows 1 and nx
+                lu_rows(c->h_am3ck, c->h_ac3ck, c->h_ap3ck, beta * al, c->nx, 1, c->nxm, np, L, E, G, S);
+                break;
+            case MAT_YZ:  // SolveImpEqnUpdate_YZ.F90:25-44
+                lu_rows(c->h_am3sk, c->h_ac3sk, c->h_ap3sk, beta * al, c->nxm, 0, c->nxm, np, L, E, G, S);
+                break;
+            case MAT_T:   // SolveImpEqnUpdate_Temp.F90:25-46
+                lu_rows(c->h_am3ssk, c->h_ac3ssk, c->h_ap3ssk, 0.5 * al * c->dt / c->pect, c->nxm, 0, c->nxm, np, L, E, G, S);
+                break;
+            case MAT_S:   // salinity.F90:418-440
+                lu_rows(c->h_am3sskr, c->h_ac3sskr, c->h_ap3sskr, 0.5 * al * c->dt / c->pecs, c->nxmr, 0, c->nxmr, np, L, E, G, S);
+                break;
+            }
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->tri, c->tri_host.data(), sizeof(double) * c->tri_host.size(), hipMemcpyHostToDevice));
+    c->fact_dt = c->dt;
+    return AFID_OK;
+}
+
+static TriFactors factors(const afid_ctx *c, int kind) {
+    const int np = c->npadmax;
+    const double *base = c->tri + size_t(((c->ns - 1) * MAT_NKIND + kind) * 4 * np);
+    TriFactors f;
+    f.l = base; f.e = base + np; f.g = base + 2 * np; f.s = base + 3 * np;
+    f.n = c->nrows[kind]; f.chunk = c->chunk[kind];
+    return f;
+}
+
+template <int CH>
+static void launch_tri(afid_ctx *c, const TriFactors &f, double *rhs, double *q, int ldx, long long plane, int ny,
+                       long long ncols, int u0, int u1, int z0, int z1) {
+    const unsigned nb = unsigned((ncols + 3) / 4);
+    hipLaunchKernelGGL(k_tri_update<CH>, dim3(nb), dim3(256), 0, c->stream, f, rhs, q, ldx, plane, ny, ncols, u0, u1, z0, z1);
+}
+
+static int run_tri(afid_ctx *c, int kind, double *rhs, double *q) {
+    TriFactors f = factors(c, kind);
+    const bool ref = kind == MAT_S;
+    const int ldx = ref ? c->ldxr : c->ldx, ny = ref ? c->nyr : c->ny, nz = ref ? c->nzr : c->nz;
+    const long long plane = (long long)ny * ldx, ncols = (long long)ny * nz;
+    int u0 = 0, u1 = f.n, z0 = -1, z1 = -1;
+    if (kind == MAT_X) { u0 = 1; u1 = c->nxm; z0 = 0; z1 = c->nx - 1; }
+    switch (f.chunk) {
+#define AFID_TRI_CASE(CH) case CH: launch_tri<CH>(c, f, rhs, q, ldx, plane, ny, ncols, u0, u1, z0, z1); break;
+        AFID_TRI_CASE(1) AFID_TRI_CASE(2) AFID_TRI_CASE(3) AFID_TRI_CASE(4) AFID_TRI_CASE(5) AFID_TRI_CASE(6)
+        AFID_TRI_CASE(7) AFID_TRI_CASE(8) AFID_TRI_CASE(9) AFID_TRI_CASE(10) AFID_TRI_CASE(11) AFID_TRI_CASE(12)
+        AFID_TRI_CASE(13) AFID_TRI_CASE(14) AFID_TRI_CASE(15) AFID_TRI_CASE(16)
+#undef AFID_TRI_CASE
+    default: return set_err(c, AFID_E_ARG, "wall-normal size above 1024 points is not supported");
+    }
+    return check_launch(c, "k_tri_update");
+}
+
+// ---------------------------------------------------------------------------
+// set-up
+// ---------------------------------------------------------------------------
+static int build_plan(afid_ctx *c, int n, FftPlan &pl) {
+    pl = FftPlan{};
+    pl.n = n;
+    int m = n;
+    while (m % 8 == 0 && m > 1) { pl.rad[pl.nst++] = 8; m /= 8; }
+    while (m % 4 == 0 && m > 1) { pl.rad[pl.nst++] = 4; m /= 4; }
+    while (m % 2 == 0 && m > 1) { pl.rad[pl.nst++] = 2; m /= 2; }
+    while (m % 3 == 0 && m > 1) { pl.rad[pl.nst++] = 3; m /= 3; }
+    if (m != 1 || pl.nst > 12) return set_err(c, AFID_E_ARG, "periodic size " + std::to_string(n) + " must factor into 2 and 3");
+    std::vector<double2> tw((size_t)(n));
+    for (int t = 0; t < n; ++t) tw[size_t(t)] = make_double2(std::cos(2.0 * kPi * t / n), -std::sin(2.0 * kPi * t / n));
+    const double2 *d = nullptr;
+    int rc = upload_vec(c, tw, &d);
+    pl.tw = d;
+    return rc;
+}
+
+static int pick_lines(int N, int maxlds, int cap) {
+    int L = 1;
+    while (2 * L * N * 16 <= maxlds && 2 * L * N <= 32 * 256 && 2 * L <= cap) L *= 2;
+    return L;
+}
+
+static int setup(afid_ctx *c) {
+    const afid_config &g = c->c;
+    if (g.nxm < 2 || g.nym < 1 || g.nzm < 1) return set_err(c, AFID_E_ARG, "grid sizes must be nxm >= 2, nym >= 1, nzm >= 1");
+    if (g.phasefield) return set_err(c, AFID_E_ARG, "phase-field configurations are not available in this build");
+    if (g.nsst != 1 && g.nsst != 3) return set_err(c, AFID_E_ARG, "nsst must be 1 (Adams-Bashforth) or 3 (RK3)");
+    if (g.precision != AFID_FP64) return set_err(c, AFID_E_ARG, "only the FP64 form is available in this build");
+    c->sal = g.salinity != 0;
+    if (c->sal && !g.multires) return set_err(c, AFID_E_ARG, "salinity requires the refined grid (MULTIRES=1)");
+    c->nxm = g.nxm; c->nx = g.nxm + 1; c->ny = g.nym; c->nz = g.nzm;
+    c->ldx = (c->nx + 7) / 8 * 8;
+    c->ldt = (c->nx + 3 + 7) / 8 * 8;
+    c->nymh = c->ny / 2 + 1;
+    if (c->sal) {
+        c->nxmr = g.nxmr; c->nxr = g.nxmr + 1; c->nyr = g.nymr; c->nzr = g.nzmr;
+        c->ldxr = (c->nxr + 7) / 8 * 8;
+        if (c->nxmr < 2 || c->nyr < 2 || c->nzr < 2) return set_err(c, AFID_E_ARG, "refined grid too small");
+    }
+    // ReadInputFile.F90:101-122
+    c->rrho = std::fabs(g.rayt) * g.pras / (std::fabs(g.rays) * g.prat);
+    if (c->sal && g.ffscales == 1) { c->ren = std::sqrt(std::fabs(g.rays) / g.pras); c->byct = c->rrho; c->bycs = 1.0; }
+    else { c->ren = std::sqrt(std::fabs(g.rayt) / g.prat); c->byct = 1.0; c->bycs = 1.0 / c->rrho; }
+    c->pect = c->ren * g.prat;
+    c->pecs = c->ren * g.pras;
+    // ExplicitTermsVY.F90:30-34
+    c->Gy = (g.inslwn == 0 || g.inslws == 0) ? g.dpdy * g.dpdy / (c->ren * c->ren) : 8.0 * g.dpdy * g.dpdy / (c->ren * c->ren);
+    // InitTimeMarchScheme.F90:17-49
+    if (g.nsst > 1) {
+        c->gam[0] = 8.0 / 15.0; c->gam[1] = 5.0 / 12.0; c->gam[2] = 3.0 / 4.0;
+        c->rom[0] = 0.0; c->rom[1] = -17.0 / 60.0; c->rom[2] = -5.0 / 12.0;
+    } else {
+        c->gam[0] = 1.5; c->gam[1] = 0.0; c->gam[2] = 0.0;
+        c->rom[0] = -0.5; c->rom[1] = 0.0; c->rom[2] = 0.0;
+    }
+    for (int s = 0; s < 3; ++s) c->alm[s] = c->gam[s] + c->rom[s];
+
+    const int nx = c->nx, nxm = c->nxm;
+    // CreateGrid.F90:87-176
+    FArr xc, xm;
+    make_wall_grid(g.istr3, nxm, g.alx3, g.str3, g.istr3 == 5 ? g.dpdy : 1.0, xc, xm);
+    c->dx = double(nxm) / g.alx3; c->dy = double(c->ny) / g.ylen; c->dz = double(c->nz) / g.zlen;
+    xm(nx) = 2 * xc(nx) - xm(nxm);
+    FArr g3rc(1, nx), g3rm(1, nx), d3xc(1, nx), d3xm(1, nx), udx3m(1, nx), udx3c(1, nx);
+    for (int k = 2; k <= nxm; ++k) {
+        g3rc(k) = (xc(k + 1) - xc(k - 1)) * c->dx * 0.5;
+        g3rm(k) = (xm(k + 1) - xm(k - 1)) * c->dx * 0.5;
+    }
+    g3rm(1) = (xm(2) - (2 * xc(1) - xm(1))) * c->dx * 0.5;
+    g3rc(1) = (xc(2) - xc(1)) * c->dx;
+    g3rc(nx) = (xc(nx) - xc(nxm)) * c->dx;
+    for (int k = 1; k <= nxm; ++k) { d3xc(k) = (xc(k + 1) - xc(k)) * c->dx; d3xm(k) = (xm(k + 1) - xm(k)) * c->dx; }
+    for (int k = 1; k <= nxm; ++k) { udx3m(k) = c->dx / d3xc(k); udx3c(k) = c->dx / g3rc(k); }
+    udx3c(nx) = c->dx / g3rc(nx);
+    FArr ap3ck, ac3ck, am3ck, ap3sk, ac3sk, am3sk, ap3ssk, ac3ssk, am3ssk;
+    second_derivative_coeff(ap3ck, ac3ck, am3ck, xc, nx, g.alx3, 1, 1);
+    FArr xmm(1, nxm);
+    for (int k = 1; k <= nxm; ++k) xmm(k) = xm(k);
+    second_derivative_coeff(ap3sk, ac3sk, am3sk, xmm, nxm, g.alx3, g.inslwn, g.inslws);
+    second_derivative_coeff(ap3ssk, ac3ssk, am3ssk, xmm, nxm, g.alx3, g.tfixn, g.tfixs);
+    // InitPressureSolver, pressure.F90:78-114
+    std::vector<double> ak1((size_t)(c->nz)), ak2((size_t)(c->ny));
+    {
+        const int nzm = c->nz, nym = c->ny;
+        for (int i = 1; i <= nzm; ++i) {
+            double ao = i <= nzm / 2 + 1 ? (i - 1) * 2.0 * kPi : -(nzm - i + 1) * 2.0 * kPi;
+            ak1[size_t(i - 1)] = 2.0 * (1.0 - std::cos(ao / nzm)) * std::pow(double(nzm) / g.zlen, 2);
+        }
+        for (int j = 1; j <= nym; ++j) {
+            double ap = j <= nym / 2 + 1 ? (j - 1) * 2.0 * kPi : -(nym - j + 1) * 2.0 * kPi;
+            ak2[size_t(j - 1)] = 2.0 * (1.0 - std::cos(ap / nym)) * std::pow(double(nym) / g.ylen, 2);
+        }
+    }
+    std::vector<double> amphk((size_t)(nxm)), acphk((size_t)(nxm)), apphk((size_t)(nxm));
+    for (int k = 1; k <= nxm; ++k) {
+        const int km = k == 1 ? 1 : k - 1, kp = k == nxm ? k : k + 1;
+        const int kmc = k - km, kpc = kp - k;
+        const double a33icc = kmc * c->dx * c->dx / d3xm(km), a33icp = kpc * c->dx * c->dx / d3xm(k);
+        const double ug = 1.0 / d3xc(k);
+        amphk[size_t(k - 1)] = a33icc * ug;
+        apphk[size_t(k - 1)] = a33icp * ug;
+        acphk[size_t(k - 1)] = -(amphk[size_t(k - 1)] + apphk[size_t(k - 1)]);
+    }
+    std::vector<double> rudx3m;
+    for (int k = 1; k <= nxm; ++k) rudx3m.push_back(1.0 / udx3m(k));   // ExplicitTermsVX.F90:30-32
+
+    c->h_ap3ck = farr_range(ap3ck, 1, nx); c->h_ac3ck = farr_range(ac3ck, 1, nx); c->h_am3ck = farr_range(am3ck, 1, nx);
+    c->h_ap3sk = farr_range(ap3sk, 1, nxm); c->h_ac3sk = farr_range(ac3sk, 1, nxm); c->h_am3sk = farr_range(am3sk, 1, nxm);
+    c->h_ap3ssk = farr_range(ap3ssk, 1, nxm); c->h_ac3ssk = farr_range(ac3ssk, 1, nxm); c->h_am3ssk = farr_range(am3ssk, 1, nxm);
+    auto &H = c->host1d;
+    H["xc"] = farr_range(xc, 1, nx); H["xm"] = farr_range(xm, 1, nx);
+    H["g3rc"] = farr_range(g3rc, 1, nx); H["g3rm"] = farr_range(g3rm, 1, nx);
+    H["d3xc"] = farr_range(d3xc, 1, nxm); H["d3xm"] = farr_range(d3xm, 1, nxm);
+    H["udx3c"] = farr_range(udx3c, 1, nx); H["udx3m"] = farr_range(udx3m, 1, nxm);
+    H["ap3ck"] = c->h_ap3ck; H["ac3ck"] = c->h_ac3ck; H["am3ck"] = c->h_am3ck;
+    H["ap3sk"] = c->h_ap3sk; H["ac3sk"] = c->h_ac3sk; H["am3sk"] = c->h_am3sk;
+    H["ap3ssk"] = c->h_ap3ssk; H["ac3ssk"] = c->h_ac3ssk; H["am3ssk"] = c->h_am3ssk;
+    H["ak1"] = ak1; H["ak2"] = ak2; H["amphk"] = amphk; H["acphk"] = acphk; H["apphk"] = apphk;
+
+    int rc;
+    auto up = [&](const std::vector<double> &v, const double **d) { return rc ? rc : (rc = upload_vec(c, v, d)); };
+    rc = 0;
+    std::vector<double> udx3m_v = farr_range(udx3m, 1, nxm), udx3c_v = farr_range(udx3c, 1, nx), g3rc_v = farr_range(g3rc, 1, nx);
+    up(udx3m_v, &c->d_udx3m); up(udx3c_v, &c->d_udx3c); up(rudx3m, &c->d_rudx3m); up(g3rc_v, &c->d_g3rc);
+    up(c->h_ap3ck, &c->d_ap3ck); up(c->h_ac3ck, &c->d_ac3ck); up(c->h_am3ck, &c->d_am3ck);
+    up(c->h_ap3sk, &c->d_ap3sk); up(c->h_ac3sk, &c->d_ac3sk); up(c->h_am3sk, &c->d_am3sk);
+    up(c->h_ap3ssk, &c->d_ap3ssk); up(c->h_ac3ssk, &c->d_ac3ssk); up(c->h_am3ssk, &c->d_am3ssk);
+    up(amphk, &c->d_amphk); up(acphk, &c->d_acphk); up(apphk, &c->d_apphk);
+    if (rc) return rc;
+
+    // fields
+    const size_t nb = size_t(c->nz) * c->ny * c->ldx;
+    for (int f = 0; f < AFID_NFIELDS; ++f) {
+        if (is_refined(f) || f == AFID_RHS || is_plane(f)) continue;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * nb))) return rc;
+    }
+    for (int q = 0; q < 4; ++q)
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->rhs4[q]), sizeof(double) * nb))) return rc;
+    c->f[AFID_RHS] = c->rhs4[0];
+    for (int f : {AFID_TEMPBP, AFID_TEMPTP})
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * size_t(c->ny) * c->nz))) return rc;
+    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->spec), sizeof(double2) * size_t(c->nz) * c->nymh * c->nxm))) return rc;
+
+    // Poisson transforms and factors
+    if ((rc = build_plan(c, c->ny, c->plan_y))) return rc;
+    if ((rc = build_plan(c, c->nz, c->plan_z))) return rc;
+    int maxlds = 65536;
+    {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && v > 0)
+            maxlds = std::min(v, 160 * 1024);
+    }
+    c->Ly = pick_lines(c->ny, maxlds, 16);      // complex lines; 2*Ly real x-values per block
+    c->Lz = pick_lines(c->nz, maxlds, 16);
+    c->lds_y = c->Ly * c->ny * 16;
+    c->lds_z = c->Lz * c->nz * 16;
+    if (c->lds_y > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_div_fft_y), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_y);
+    if (c->lds_y > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_ifft_y), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_y);
+    if (c->lds_z > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_fft_z), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_z);
+    if (c->ny * 16 > maxlds || c->nz * 16 > maxlds || c->ny > 8192 || c->nz > 8192)
+        return set_err(c, AFID_E_ARG, "periodic size too large for one workgroup transform");
+    {
+        c->pchunk = (nxm + 63) / 64;
+        if (c->pchunk > 16) return set_err(c, AFID_E_ARG, "nxm above 1024 is not supported");
+        const int npad = 64 * c->pchunk;
+        const long long nmodes = (long long)c->nymh * (c->nz / 2 + 1);
+        double *fm, *fg, *fe;
+        const double *dak1, *dak2;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&fm), sizeof(double) * size_t(nmodes) * npad))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&fg), sizeof(double) * size_t(nmodes) * npad))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&fe), sizeof(double) * size_t(nmodes) * npad))) return rc;
+        if ((rc = upload_vec(c, ak1, &dak1))) return rc;
+        if ((rc = upload_vec(c, ak2, &dak2))) return rc;
+        c->pf.m = fm; c->pf.g = fg; c->pf.e = fe; c->pf.npad = npad;
+        KP p = make_kp(c);
+        hipLaunchKernelGGL(k_poisson_factors, dim3(unsigned((nmodes + 127) / 128)), dim3(128), 0, c->stream, p, dak1, dak2, fm, fg, fe, npad);
+        if ((rc = check_launch(c, "k_poisson_factors"))) return rc;
+    }
+
+    // tridiagonal factor storage
+    c->nrows[MAT_X] = nx; c->nrows[MAT_YZ] = nxm; c->nrows[MAT_T] = nxm; c->nrows[MAT_S] = c->sal ? c->nxmr : 0;
+    int chmax = 1;
+    for (int k = 0; k < MAT_NKIND; ++k) { c->chunk[k] = std::max(1, (c->nrows[k] + 63) / 64); chmax = std::max(chmax, c->chunk[k]); }
+    if (chmax > 16) return set_err(c, AFID_E_ARG, "wall-normal size above 1024 points is not supported");
+    c->npadmax = 64 * chmax;
+    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->tri), sizeof(double) * size_t(3 * MAT_NKIND * 4) * c->npadmax))) return rc;
+
+    // reductions
+    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->redmax), sizeof(unsigned long long) * 4))) return rc;
+    {
+        dim3 gr = point_grid(nxm, c->ny, c->nz);
+        c->nred = (long long)gr.x * gr.y * gr.z;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->redsum), sizeof(double) * 3 * size_t(c->nred)))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->redout), sizeof(double) * 4))) return rc;
+    }
+
+    if (!c->sal) return AFID_OK;
+
+    // ---- refined grid: CreateMgrdGrid.F90:38-161 ----
+    const int nxr = c->nxr, nxmr = c->nxmr, nyr = c->nyr, nzr = c->nzr;
+    FArr xcr, xmr;
+    const int istr = g.istr3 == 5 ? 5 : g.istr3r;   // CreateMgrdGrid.F90:77 tests istr3 for option 5
+    if (istr == 2) return set_err(c, AFID_E_ARG, "refined grid option 2 does not exist in the reference");
+    make_wall_grid(istr, nxmr, g.alx3, g.str3, istr == 5 ? g.dpdy : g.pras, xcr, xmr);
+    c->dxr = double(nxmr) / g.alx3; c->dyr = double(nyr) / g.ylen; c->dzr = double(nzr) / g.zlen;
+    xmr(0) = 2.0 * xcr(1) - xmr(1);
+    xmr(nxr) = 2.0 * xcr(nxr) - xmr(nxmr);
+    FArr udx3mr(1, nxmr), g3rmr(1, nxr);
+    for (int k = 2; k <= nxmr; ++k) g3rmr(k) = (xmr(k + 1) - xmr(k - 1)) * c->dxr * 0.5;
+    g3rmr(1) = (xmr(2) - (2 * xcr(1) - xmr(1))) * c->dxr * 0.5;
+    for (int k = 1; k <= nxmr; ++k) udx3mr(k) = c->dxr / ((xcr(k + 1) - xcr(k)) * c->dxr);
+    FArr xmrr(1, nxmr);
+    for (int k = 1; k <= nxmr; ++k) xmrr(k) = xmr(k);
+    FArr ap3sskr, ac3sskr, am3sskr;
+    second_derivative_coeff(ap3sskr, ac3sskr, am3sskr, xmrr, nxmr, g.alx3, g.sfixn, g.sfixs);
+    c->h_ap3sskr = farr_range(ap3sskr, 1, nxmr); c->h_ac3sskr = farr_range(ac3sskr, 1, nxmr); c->h_am3sskr = farr_range(am3sskr, 1, nxmr);
+    std::vector<double> lxa;
+    for (int k = 1; k <= nxmr; ++k) lxa.push_back(xcr(k + 1) - xcr(k));   // InterpVelMgrd.F90:62
+    H["xcr"] = farr_range(xcr, 1, nxr); H["xmr"] = farr_range(xmr, 1, nxr);
+    H["udx3mr"] = farr_range(udx3mr, 1, nxmr); H["g3rmr"] = farr_range(g3rmr, 1, nxr);
+    H["ap3sskr"] = c->h_ap3sskr; H["ac3sskr"] = c->h_ac3sskr; H["am3sskr"] = c->h_am3sskr;
+
+    // uniform periodic grids (uniform_grid + end extensions, CreateGrid.F90:32-40, CreateMgrdGrid.F90:38-46)
+    auto uni = [](int n, double L, FArr &cc, FArr &mm) {
+        cc = FArr(1, n + 1); mm = FArr(0, n + 1);
+        for (int i = 1; i <= n + 1; ++i) cc(i) = L * double(i - 1) / double(n);
+        for (int i = 1; i <= n; ++i) mm(i) = 0.5 * (cc(i) + cc(i + 1));
+    };
+    FArr yc, ym, zc, zm, ycr, ymr, zcr, zmr;
+    uni(c->ny, g.ylen, yc, ym); uni(c->nz, g.zlen, zc, zm); uni(nyr, g.ylen, ycr, ymr); uni(nzr, g.zlen, zcr, zmr);
+    FArr xmc(1, nxm);
+    for (int k = 1; k <= nxm; ++k) xmc(k) = xm(k);
+
+    // CreateMgrdStencil.F90:17-78: coarse cell centres -> refined cell centres
+    IArr irangs = interpolation_indices(xmc, nxm, xmrr, nxmr, g.alx3);
+    Stencil sx = construct_stencil(xmc, nxm, xmrr, nxmr, g.alx3, irangs, 'x');
+    IArr jrangs = interpolation_indices(ym, c->ny, ymr, nyr, g.ylen);
+    Stencil sy = construct_stencil(ym, c->ny, ymr, nyr, g.ylen, jrangs, 'y');
+    IArr krangs = interpolation_indices(zm, c->nz, zmr, nzr, g.zlen);
+    Stencil sz = construct_stencil(zm, c->nz, zmr, nzr, g.zlen, krangs, 'z');
+    // refined -> coarse (InterpSalMultigrid fast path)
+    if (!(xmr(1) < xm(1) && xmr(nxmr) > xm(nxm)))
+        return set_err(c, AFID_E_ARG, "InterpSalMultigrid: only the bracketing (fast) path is implemented");
+    Stencil bx, by, bz;
+    std::vector<int> xb_src((size_t)(nx)), yb_src((size_t)(c->ny)), zb_src((size_t)(c->nz));
+    {
+        IArr jrangr = interpolation_indices(ymr, nyr, ym, c->ny, g.ylen);
+        by = construct_stencil(ymr, nyr, ym, c->ny, g.ylen, jrangr, 'y');
+        IArr krangr = interpolation_indices(zmr, nzr, zm, c->nz, g.zlen);
+        bz = construct_stencil(zmr, nzr, zm, c->nz, g.zlen, krangr, 'z');
+        IArr irangb;
+        if (g.gaxis == 1) {
+            FArr xcc(1, nx);
+            for (int k = 1; k <= nx; ++k) xcc(k) = xc(k);
+            IArr irange = interpolation_indices(xmrr, nxmr, xcc, nx, g.alx3);
+            bx = construct_stencil(xmrr, nxmr, xcc, nx, g.alx3, irange, 'x');
+            irangb = interpolation_indices(xcc, nx, xmrr, nxmr, g.alx3);
+        } else {
+            IArr irangr = interpolation_indices(xmrr, nxmr, xmc, nxm, g.alx3);
+            bx = construct_stencil(xmrr, nxmr, xmc, nxm, g.alx3, irangr, 'x');
+            irangb = irangs;
+        }
+        // first refined point of each coarse target: irang(kc)-2 (HermiteInterpolations.F90:292-298)
+        for (int k = 1; k <= nx; ++k) xb_src[size_t(k - 1)] = k <= nxm ? irangb(k) - 2 : 0;
+        for (int j = 1; j <= c->ny; ++j) yb_src[size_t(j - 1)] = jrangs(j) - 2;
+        for (int i = 1; i <= c->nz; ++i) zb_src[size_t(i - 1)] = krangs(i) - 2;
+        bx.w.resize(size_t(4 * nx), 0.0);
+    }
+    H["cxrs"] = sx.w; H["cyrs"] = sy.w; H["czrs"] = sz.w; H["cxsalc"] = bx.w; H["cysalc"] = by.w; H["czsalc"] = bz.w;
+
+    const int *pi;
+    const double *pd;
+    if ((rc = upload_vec(c, sx.src, &pi))) return rc; c->d_xs_src = pi;
+    if ((rc = upload_vec(c, sy.src, &pi))) return rc; c->d_ys_src = pi;
+    if ((rc = upload_vec(c, sz.src, &pi))) return rc; c->d_zs_src = pi;
+    if ((rc = upload_vec(c, sx.w, &pd))) return rc; c->d_xs_w = pd;
+    if ((rc = upload_vec(c, sy.w, &pd))) return rc; c->d_ys_w = pd;
+    if ((rc = upload_vec(c, sz.w, &pd))) return rc; c->d_zs_w = pd;
+    if ((rc = upload_vec(c, xb_src, &pi))) return rc; c->d_xb_src = pi;
+    if ((rc = upload_vec(c, yb_src, &pi))) return rc; c->d_yb_src = pi;
+    if ((rc = upload_vec(c, zb_src, &pi))) return rc; c->d_zb_src = pi;
+    if ((rc = upload_vec(c, bx.w, &pd))) return rc; c->d_xb_w = pd;
+    if ((rc = upload_vec(c, by.w, &pd))) return rc; c->d_yb_w = pd;
+    if ((rc = upload_vec(c, bz.w, &pd))) return rc; c->d_zb_w = pd;
+    std::vector<double> udx3mr_v = farr_range(udx3mr, 1, nxmr);
+    up(udx3mr_v, &c->d_udx3mr); up(c->h_ap3sskr, &c->d_ap3sskr); up(c->h_ac3sskr, &c->d_ac3sskr); up(c->h_am3sskr, &c->d_am3sskr);
+    up(lxa, &c->d_lxa);
+    if (rc) return rc;
+
+    const size_t nr = size_t(nzr) * nyr * c->ldxr;
+    for (int f = AFID_SAL; f <= AFID_VZR; ++f)
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * nr))) return rc;
+    for (int f : {AFID_SALBP, AFID_SALTP})
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * size_t(nyr) * nzr))) return rc;
+    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->tpdv), sizeof(double) * size_t(c->nz) * c->ny * c->ldt))) return rc;
+    if (nxmr > 1024) return set_err(c, AFID_E_ARG, "refined wall-normal size above 1024 is not supported");
+    if (4 * (nx + 3) * 8 > 65536) return set_err(c, AFID_E_ARG, "base wall-normal size too large for the vxr transfer");
+    return AFID_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int afid_gpu_init(const afid_config *cfg, afid_ctx **out) {
+    if (!cfg || !out) return AFID_E_ARG;
+    *out = nullptr;
+    afid_ctx *c = new afid_ctx();
+    c->c = *cfg;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        fprintf(stderr, "afid_gpu_init: no HIP device\n");
+        delete c;
+        return AFID_E_HIP;
+    }
+    c->device = dev;
+    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return AFID_E_HIP; }
+    int rc;
+    try {
+        rc = setup(c);
+    } catch (const std::exception &e) {
+        rc = set_err(c, AFID_E_ARG, e.what());
+    }
+    if (rc == AFID_OK) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = set_err(c, AFID_E_HIP, std::string("set-up: ") + hipGetErrorString(e));
+    }
+    *out = c;   // returned even on failure so afid_last_error can report; caller finalizes
+    return rc;
+}
+
+int afid_gpu_finalize(afid_ctx *c) {
+    if (!c) return AFID_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    for (auto &b : c->bufs) (void)hipFree(b.p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return AFID_OK;
+}
+
+const char *afid_last_error(const afid_ctx *c) { return c ? c->err.c_str() : "null context"; }
+int64_t afid_device_bytes(const afid_ctx *c) { return c ? c->bytes : 0; }
+
+static int field_dims(const afid_ctx *c, int f, int &nxf, int &ny, int &nz, int &ld) {
+    if (f < 0 || f >= AFID_NFIELDS || !c->f[f]) return AFID_E_ARG;
+    const bool r = is_refined(f);
+    ny = r ? c->nyr : c->ny;
+    nz = r ? c->nzr : c->nz;
+    if (is_plane(f)) { nxf = 1; ld = 1; }
+    else { nxf = r ? c->nxr : c->nx; ld = r ? c->ldxr : c->ldx; }
+    return AFID_OK;
+}
+
+int afid_upload(afid_ctx *c, int f, const double *host, const int64_t lb[3], const int64_t ext[3]) {
+    int nxf, ny, nz, ld;
+    if (!c || !host || field_dims(c, f, nxf, ny, nz, ld)) return set_err(c, AFID_E_ARG, "afid_upload: bad field");
+    if (lb[1] > 1 || lb[2] > 1 || lb[1] + ext[1] - 1 < ny || lb[2] + ext[2] - 1 < nz || lb[0] > 1)
+        return set_err(c, AFID_E_ARG, "afid_upload: host array does not cover the interior");
+    const int kmax = int(std::min<int64_t>(nxf, lb[0] + ext[0] - 1));
+    std::vector<double> slab(size_t(ny) * ld, 0.0);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 1; i <= nz; ++i) {
+        for (int j = 1; j <= ny; ++j)
+            for (int k = 1; k <= kmax; ++k)
+                slab[size_t(j - 1) * ld + (k - 1)] = host[(k - lb[0]) + ext[0] * ((j - lb[1]) + ext[1] * (int64_t)(i - lb[2]))];
+        HIPCHK(c, hipMemcpy(c->f[f] + size_t(i - 1) * ny * ld, slab.data(), sizeof(double) * slab.size(), hipMemcpyHostToDevice));
+    }
+    return AFID_OK;
+}
+
+int afid_download(afid_ctx *c, int f, double *host, const int64_t lb[3], const int64_t ext[3]) {
+    int nxf, ny, nz, ld;
+    if (!c || !host || field_dims(c, f, nxf, ny, nz, ld)) return set_err(c, AFID_E_ARG, "afid_download: bad field");
+    if (lb[1] > 1 || lb[2] > 1 || lb[1] + ext[1] - 1 < ny || lb[2] + ext[2] - 1 < nz || lb[0] > 1)
+        return set_err(c, AFID_E_ARG, "afid_download: host array does not cover the interior");
+    const int kmax = int(std::min<int64_t>(nxf, lb[0] + ext[0] - 1));
+    std::vector<double> slab(size_t(ny) * ld);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t ih = lb[2]; ih < lb[2] + ext[2]; ++ih) {
+        const int i = int(((ih - 1) % nz + nz) % nz);    // periodic halo (halo_common.F90:66-71 at one rank)
+        HIPCHK(c, hipMemcpy(slab.data(), c->f[f] + size_t(i) * ny * ld, sizeof(double) * slab.size(), hipMemcpyDeviceToHost));
+        for (int64_t jh = lb[1]; jh < lb[1] + ext[1]; ++jh) {
+            const int j = int(((jh - 1) % ny + ny) % ny);
+            for (int k = 1; k <= kmax; ++k)
+                host[(k - lb[0]) + ext[0] * ((jh - lb[1]) + ext[1] * (ih - lb[2]))] = slab[size_t(j) * ld + (k - 1)];
+        }
+    }
+    return AFID_OK;
+}
+
+int afid_set_substep(afid_ctx *c, double dt, int ns) {
+    if (!c || ns < 1 || ns > c->c.nsst || !(dt > 0.0)) return set_err(c, AFID_E_ARG, "afid_set_substep: bad dt or substep");
+    c->dt = dt;
+    c->beta = dt / c->ren * 0.5;
+    c->al = c->alm[ns - 1]; c->ga = c->gam[ns - 1]; c->ro = c->rom[ns - 1];
+    c->ns = ns;
+    return ensure_factors(c);
+}
+
+#define NEED_SUBSTEP(c) if (!(c) || (c)->ns == 0) return set_err((c), AFID_E_STATE, "no substep set (afid_set_substep)");
+#define NEED_SAL(c) if (!(c)->sal) return set_err((c), AFID_E_STATE, "salinity is not enabled");
+
+static int launch_point(afid_ctx *c, void (*kern)(KP), bool refined, const char *name) {
+    KP p = make_kp(c);
+    dim3 gr = refined ? point_grid(c->nxmr, c->nyr, c->nzr) : point_grid(c->nxm, c->ny, c->nz);
+    hipLaunchKernelGGL(kern, gr, kPointBlock, 0, c->stream, p);
+    return check_launch(c, name);
+}
+
+int afid_explicit_terms_vx(afid_ctx *c) { NEED_SUBSTEP(c) return launch_point(c, k_explicit_vx, false, "k_explicit_vx"); }
+int afid_explicit_terms_vy(afid_ctx *c) { NEED_SUBSTEP(c) return launch_point(c, k_explicit_vy, false, "k_explicit_vy"); }
+int afid_explicit_terms_vz(afid_ctx *c) { NEED_SUBSTEP(c) return launch_point(c, k_explicit_vz, false, "k_explicit_vz"); }
+int afid_explicit_terms_temp(afid_ctx *c) { NEED_SUBSTEP(c) return launch_point(c, k_explicit_temp, false, "k_explicit_temp"); }
+int afid_explicit_salinity(afid_ctx *c) { NEED_SUBSTEP(c) NEED_SAL(c) return launch_point(c, k_explicit_sal, true, "k_explicit_sal"); }
+
+int afid_add_sal_buoyancy(afid_ctx *c) {
+    NEED_SUBSTEP(c) NEED_SAL(c)
+    double *rkv = c->c.gaxis == 1 ? c->f[AFID_QCAP] : c->c.gaxis == 2 ? c->f[AFID_DPH] : c->f[AFID_DQ];
+    KP p = make_kp(c);
+    hipLaunchKernelGGL(k_add_sal_buoyancy, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, rkv);
+    return check_launch(c, "k_add_sal_buoyancy");
+}
+
+int afid_implicit_and_update_vx(afid_ctx *c) {
+    NEED_SUBSTEP(c)
+    int rc = launch_point(c, k_rhs_vx, false, "k_rhs_vx");
+    if (!rc) rc = run_tri(c, MAT_X, c->rhs4[0], c->f[AFID_VX]);
+    c->f[AFID_RHS] = c->rhs4[0];
+    return rc;
+}
+int afid_implicit_and_update_vy(afid_ctx *c) {
+    NEED_SUBSTEP(c)
+    int rc = launch_point(c, k_rhs_vy, false, "k_rhs_vy");
+    if (!rc) rc = run_tri(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]);
+    c->f[AFID_RHS] = c->rhs4[1];
+    return rc;
+}
+int afid_implicit_and_update_vz(afid_ctx *c) {
+    NEED_SUBSTEP(c)
+    int rc = launch_point(c, k_rhs_vz, false, "k_rhs_vz");
+    if (!rc) rc = run_tri(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]);
+    c->f[AFID_RHS] = c->rhs4[2];
+    return rc;
+}
+int afid_implicit_and_update_temp(afid_ctx *c) {
+    NEED_SUBSTEP(c)
+    int rc = launch_point(c, k_rhs_temp, false, "k_rhs_temp");
+    if (!rc) rc = run_tri(c, MAT_T, c->rhs4[3], c->f[AFID_TEMP]);
+    c->f[AFID_RHS] = c->rhs4[3];
+    return rc;
+}
+int afid_implicit_salinity(afid_ctx *c) {
+    NEED_SUBSTEP(c) NEED_SAL(c)
+    int rc = launch_point(c, k_rhs_sal, true, "k_rhs_sal");
+    if (!rc) rc = run_tri(c, MAT_S, c->f[AFID_RHSR], c->f[AFID_SAL]);
+    return rc;
+}
+
+int afid_calc_local_divergence(afid_ctx *c) { NEED_SUBSTEP(c) return launch_point(c, k_local_divergence, false, "k_local_divergence"); }
+
+// forward transforms from dph (from_dph = 1) or fused with the divergence (0)
+static int poisson(afid_ctx *c, int from_dph) {
+    KP p = make_kp(c);
+    const int TXy = 2 * c->Ly;
+    tstart(c, "poisson_fwd_y");
+    hipLaunchKernelGGL(k_div_fft_y, dim3(unsigned((c->nxm + TXy - 1) / TXy), unsigned(c->nz)), dim3(256), c->lds_y, c->stream,
+                       p, c->plan_y, c->spec, c->Ly, from_dph);
+    tstop(c);
+    int rc = check_launch(c, "k_div_fft_y");
+    if (rc) return rc;
+    tstart(c, "poisson_fft_z");
+    hipLaunchKernelGGL(k_fft_z, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(256), c->lds_z, c->stream,
+                       p, c->plan_z, c->spec, c->Lz, -1);
+    tstop(c);
+    if ((rc = check_launch(c, "k_fft_z"))) return rc;
+    tstart(c, "poisson_tridiag");
+    switch (c->pchunk) {
+#define AFID_P_CASE(CH) case CH: hipLaunchKernelGGL(k_poisson_tri<CH>, dim3(unsigned(c->nymh), unsigned(c->nz / 2 + 1)), dim3(128), 0, c->stream, p, c->pf, c->spec); break;
+        AFID_P_CASE(1) AFID_P_CASE(2) AFID_P_CASE(3) AFID_P_CASE(4) AFID_P_CASE(5) AFID_P_CASE(6) AFID_P_CASE(7) AFID_P_CASE(8)
+        AFID_P_CASE(9) AFID_P_CASE(10) AFID_P_CASE(11) AFID_P_CASE(12) AFID_P_CASE(13) AFID_P_CASE(14) AFID_P_CASE(15) AFID_P_CASE(16)
+#undef AFID_P_CASE
+    default: return set_err(c, AFID_E_ARG, "nxm too large");
+    }
+    tstop(c);
+    if ((rc = check_launch(c, "k_poisson_tri"))) return rc;
+    tstart(c, "poisson_fft_z");
+    hipLaunchKernelGGL(k_fft_z, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(256), c->lds_z, c->stream,
+                       p, c->plan_z, c->spec, c->Lz, +1);
+    tstop(c);
+    if ((rc = check_launch(c, "k_fft_z"))) return rc;
+    tstart(c, "poisson_inv_y");
+    hipLaunchKernelGGL(k_ifft_y, dim3(unsigned((c->nxm + TXy - 1) / TXy), unsigned(c->nz)), dim3(256), c->lds_y, c->stream,
+                       p, c->plan_y, c->spec, c->Ly);
+    tstop(c);
+    return check_launch(c, "k_ifft_y");
+}
+
+int afid_solve_pressure_correction(afid_ctx *c) { NEED_SUBSTEP(c) return poisson(c, 1); }
+
+static int project(afid_ctx *c, int vel, int pr) {
+    KP p = make_kp(c);
+    hipLaunchKernelGGL(k_project, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, vel, pr);
+    return check_launch(c, "k_project");
+}
+int afid_correct_velocity(afid_ctx *c) { NEED_SUBSTEP(c) return project(c, 1, 0); }
+int afid_correct_pressure(afid_ctx *c) { NEED_SUBSTEP(c) return project(c, 0, 1); }
+
+static int interp_vel(afid_ctx *c) {
+    KP p = make_kp(c);
+    const dim3 gg((c->nx + 3 + 63) / 64, (c->ny + 3) / 4, c->nz);
+    int rc;
+    // vx
+    tstart(c, "mgrd_gradient");
+    hipLaunchKernelGGL(k_mgrd_gradient, gg, kPointBlock, 0, c->stream, p, 0);
+    tstop(c);
+    if ((rc = check_launch(c, "k_mgrd_gradient"))) return rc;
+    {
+        const int ch = (c->nxmr + 63) / 64;
+        const long long ncol = (long long)c->nyr * c->nzr;
+        const dim3 gr(unsigned((ncol + 3) / 4));
+        const size_t lds = sizeof(double) * 4 * size_t(c->nx + 3);
+        tstart(c, "mgrd_vxr");
+        switch (ch) {
+#define AFID_VXR_CASE(CH) case CH: hipLaunchKernelGGL(k_vxr<CH>, gr, dim3(256), lds, c->stream, p); break;
+            AFID_VXR_CASE(1) AFID_VXR_CASE(2) AFID_VXR_CASE(3) AFID_VXR_CASE(4) AFID_VXR_CASE(5) AFID_VXR_CASE(6)
+            AFID_VXR_CASE(7) AFID_VXR_CASE(8) AFID_VXR_CASE(9) AFID_VXR_CASE(10) AFID_VXR_CASE(11) AFID_VXR_CASE(12)
+            AFID_VXR_CASE(13) AFID_VXR_CASE(14) AFID_VXR_CASE(15) AFID_VXR_CASE(16)
+#undef AFID_VXR_CASE
+        default: return set_err(c, AFID_E_ARG, "nxmr too large");
+        }
+        tstop(c);
+        if ((rc = check_launch(c, "k_vxr"))) return rc;
+    }
+    // vy
+    tstart(c, "mgrd_gradient");
+    hipLaunchKernelGGL(k_mgrd_gradient, gg, kPointBlock, 0, c->stream, p, 1);
+    tstop(c);
+    if ((rc = check_launch(c, "k_mgrd_gradient"))) return rc;
+    tstart(c, "mgrd_vyr");
+    hipLaunchKernelGGL(k_vyr, dim3(unsigned((c->nxr + 63) / 64), unsigned(c->nzr)), dim3(64), 0, c->stream, p);
+    tstop(c);
+    if ((rc = check_launch(c, "k_vyr"))) return rc;
+    // vz
+    tstart(c, "mgrd_gradient");
+    hipLaunchKernelGGL(k_mgrd_gradient, gg, kPointBlock, 0, c->stream, p, 2);
+    tstop(c);
+    if ((rc = check_launch(c, "k_mgrd_gradient"))) return rc;
+    tstart(c, "mgrd_vzr");
+    hipLaunchKernelGGL(k_vzr, dim3(unsigned((c->nxr + 63) / 64), unsigned(c->nyr)), dim3(64), 0, c->stream, p);
+    tstop(c);
+    return check_launch(c, "k_vzr");
+}
+int afid_interp_vel_mgrd(afid_ctx *c) { NEED_SAL(c) return interp_vel(c); }
+
+int afid_interp_sal_multigrid(afid_ctx *c) {
+    NEED_SAL(c)
+    KP p = make_kp(c);
+    tstart(c, "mgrd_interp_sal");
+    hipLaunchKernelGGL(k_interp_sal, point_grid(c->nx, c->ny, c->nz), kPointBlock, 0, c->stream, p);
+    tstop(c);
+    return check_launch(c, "k_interp_sal");
+}
+
+// TimeMarcher.F90:13-165
+int afid_time_marcher(afid_ctx *c, double dt) {
+    if (!c) return AFID_E_ARG;
+    for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    c->timers.clear();
+    int rc;
+    for (int ns = 1; ns <= c->c.nsst; ++ns) {
+        if ((rc = afid_set_substep(c, dt, ns))) return rc;
+        KP p = make_kp(c);
+        tstart(c, "explicit_rhs");
+        hipLaunchKernelGGL(k_explicit_rhs_all, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p);
+        tstop(c);
+        if ((rc = check_launch(c, "k_explicit_rhs_all"))) return rc;
+        if (c->sal) {
+            tstart(c, "explicit_rhs_sal");
+            hipLaunchKernelGGL(k_explicit_rhs_sal, point_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p);
+            tstop(c);
+            if ((rc = check_launch(c, "k_explicit_rhs_sal"))) return rc;
+        }
+        tstart(c, "tridiag_base");
+        if ((rc = run_tri(c, MAT_X, c->rhs4[0], c->f[AFID_VX]))) return rc;
+        if ((rc = run_tri(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]))) return rc;
+        if ((rc = run_tri(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]))) return rc;
+        if ((rc = run_tri(c, MAT_T, c->rhs4[3], c->f[AFID_TEMP]))) return rc;
+        tstop(c);
+        if (c->sal) {
+            tstart(c, "tridiag_sal");
+            if ((rc = run_tri(c, MAT_S, c->f[AFID_RHSR], c->f[AFID_SAL]))) return rc;
+            tstop(c);
+        }
+        if ((rc = poisson(c, 0))) return rc;
+        tstart(c, "project");
+        if ((rc = project(c, 1, 1))) return rc;
+        tstop(c);
+        if (c->sal) {
+            if ((rc = interp_vel(c))) return rc;
+            if ((rc = afid_interp_sal_multigrid(c))) return rc;
+        }
+    }
+    return AFID_OK;
+}
+
+int afid_synchronize(afid_ctx *c) {
+    if (!c) return AFID_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AFID_OK;
+}
+
+static int read_max(afid_ctx *c, double *a, double *b) {
+    unsigned long long h[4];
+    HIPCHK(c, hipMemcpyAsync(h, c->redmax, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double v[4];
+    std::memcpy(v, h, sizeof(v));
+    *a = v[0]; *b = v[1];
+    return AFID_OK;
+}
+
+int afid_calc_max_cfl(afid_ctx *c, double *cflm, double *cflmr) {
+    if (!c || !cflm || !cflmr) return AFID_E_ARG;
+    HIPCHK(c, hipMemsetAsync(c->redmax, 0, sizeof(unsigned long long) * 4, c->stream));
+    KP p = make_kp(c);
+    hipLaunchKernelGGL(k_cfl_div, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, 0);
+    int rc = check_launch(c, "k_cfl_div");
+    if (rc) return rc;
+    if (c->sal) {
+        hipLaunchKernelGGL(k_cfl_div_r, point_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 0);
+        if ((rc = check_launch(c, "k_cfl_div_r"))) return rc;
+    }
+    if ((rc = read_max(c, cflm, cflmr))) return rc;
+    *cflm = std::max(*cflm, 0.00000001);          // CalcMaxCFL.F90:24
+    *cflmr = c->sal ? std::max(*cflmr, 1.0e-8) : *cflm;
+    return AFID_OK;
+}
+
+int afid_check_divergence(afid_ctx *c, double *qmax, double *qmaxr) {
+    if (!c || !qmax || !qmaxr) return AFID_E_ARG;
+    HIPCHK(c, hipMemsetAsync(c->redmax, 0, sizeof(unsigned long long) * 4, c->stream));
+    KP p = make_kp(c);
+    hipLaunchKernelGGL(k_cfl_div, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, 1);
+    int rc = check_launch(c, "k_cfl_div");
+    if (rc) return rc;
+    if (c->sal) {
+        hipLaunchKernelGGL(k_cfl_div_r, point_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 1);
+        if ((rc = check_launch(c, "k_cfl_div_r"))) return rc;
+    }
+    unsigned long long h[4];
+    HIPCHK(c, hipMemcpyAsync(h, c->redmax, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double v[4];
+    std::memcpy(v, h, sizeof(v));
+    *qmax = v[2];
+    *qmaxr = c->sal ? v[3] : v[2];
+    return AFID_OK;
+}
+
+int afid_global_quantities(afid_ctx *c, double out[4]) {
+    if (!c || !out) return AFID_E_ARG;
+    KP p = make_kp(c);
+    hipLaunchKernelGGL(k_global_sums, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p);
+    int rc = check_launch(c, "k_global_sums");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->redsum, c->nred, c->redout);
+    if ((rc = check_launch(c, "k_sum_partials"))) return rc;
+    double s[4];
+    HIPCHK(c, hipMemcpyAsync(s, c->redout, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double vol = 1.0 / (c->c.alx3 * c->dx * double(c->nz) * double(c->ny));   // GlobalQuantities.F90:35
+    out[0] = 1.0 + std::sqrt(c->c.prat * std::fabs(c->c.rayt)) * s[0] * vol;       // :79
+    out[1] = 0.5 * s[2] * vol;
+    out[2] = std::sqrt(s[2] * vol) * std::sqrt(std::fabs(c->c.rayt) / c->c.prat);  // :91
+    out[3] = s[1] * vol;                                                            // :89
+    return AFID_OK;
+}
+
+int afid_set_timing(afid_ctx *c, int on) {
+    if (!c) return AFID_E_ARG;
+    c->timing = on != 0;
+    return AFID_OK;
+}
+
+int afid_get_timing(afid_ctx *c, const char **names, double *ms, int max, int *count) {
+    if (!c || !count) return AFID_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<std::pair<std::string, double>> acc;
+    for (auto &t : c->timers) {
+        float v = 0.f;
+        if (hipEventElapsedTime(&v, t.a, t.b) != hipSuccess) continue;
+        bool found = false;
+        for (auto &a : acc) if (a.first == t.name) { a.second += v; found = true; }
+        if (!found) acc.push_back({t.name, double(v)});
+    }
+    c->last_timing = acc;
+    *count = int(acc.size());
+    for (int q = 0; q < int(acc.size()) && q < max; ++q) {
+        if (names) names[q] = c->last_timing[size_t(q)].first.c_str();
+        if (ms) ms[q] = acc[size_t(q)].second;
+    }
+    return AFID_OK;
+}
+
+int afid_grid_array(afid_ctx *c, const char *name, double *out, int64_t n, int64_t *count) {
+    if (!c || !name || !count) return AFID_E_ARG;
+    auto it = c->host1d.find(name);
+    if (it == c->host1d.end()) return set_err(c, AFID_E_ARG, std::string("unknown grid array ") + name);
+    *count = int64_t(it->second.size());
+    if (out) std::memcpy(out, it->second.data(), sizeof(double) * size_t(std::min<int64_t>(n, *count)));
+    return AFID_OK;
+}
+
+}  // extern "C"

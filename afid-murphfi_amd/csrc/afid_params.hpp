@@ -1,0 +1,69 @@
+// afid_params.hpp -- the kernel parameter block.
+//
+// One plain struct, passed by value to every kernel: grid sizes, the RK3
+// substep scalars and device pointers.  Device layout of a base field is
+// [nz][ny][ldx] doubles with x fastest (x index k = kc-1 for the reference's
+// kc), y/z periodic without halo cells; a refined field is [nzr][nyr][ldxr].
+// The wall-normal stride ldx is nx rounded up to 8 (64-byte rows).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace afid {
+
+enum MatKind { MAT_X = 0, MAT_YZ = 1, MAT_T = 2, MAT_S = 3, MAT_NKIND = 4 };
+
+// LU factors of one constant wall-normal tridiagonal system, in the form the
+// wave solver consumes (row r, 0-based, padded with zeros to 64*chunk rows):
+//   forward   y_r = s_r d_r - l_r y_{r-1}
+//   backward  x_r = g_r y_r - e_r x_{r+1}
+struct TriFactors {
+    const double *l, *e, *g, *s;
+    int n, chunk;
+};
+
+struct KP {
+    // base grid (nx = nxm+1 faces; ny = nym, nz = nzm periodic cells)
+    int nxm, nx, ny, nz, ldx;
+    long long plane;            // ny*ldx
+    int nymh;                   // ny/2+1 spectral modes in y
+    int ldt;                    // x stride of the gradient scratch (x index -1..nx+1)
+    // refined grid
+    int nxmr, nxr, nyr, nzr, ldxr;
+    long long planer;
+    // substep scalars (TimeMarcher.F90:28,38-40)
+    double dt, al, ga, ro, beta;
+    // physics (ReadInputFile.F90:101-122)
+    double ren, pect, pecs, byct, bycs, Gy;
+    double dx, dy, dz, dxq, dyq, dzq;
+    double dxr, dyr, dzr, dxqr, dyqr, dzqr;
+    int inslws, inslwn, tfixs, tfixn, sfixs, sfixn, active_t, active_s, gaxis;
+    double xplusu, xminusu;
+    // base fields
+    double *vx, *vy, *vz, *temp, *pr;
+    double *rux, *ruy, *ruz, *rut;
+    double *qcap, *dph, *dq, *hro;
+    double *rhsx, *rhsy, *rhsz, *rhst;
+    double *salc, *tpdv;
+    const double *tempbp, *temptp;      // [nz][ny]
+    // refined fields
+    double *sal, *rusal, *hsal, *rhsr, *vxr, *vyr, *vzr;
+    const double *salbp, *saltp;        // [nzr][nyr]
+    // base metrics (0-based by kc-1)
+    const double *udx3m, *udx3c, *rudx3m;
+    const double *ap3ck, *ac3ck, *am3ck, *ap3sk, *ac3sk, *am3sk, *ap3ssk, *ac3ssk, *am3ssk;
+    const double *amphk, *acphk, *apphk, *g3rc;
+    // refined metrics
+    const double *udx3mr, *ap3sskr, *ac3sskr, *am3sskr, *lxa;
+    // Hermite tables coarse -> refined (per refined index): source + 4 weights
+    const int *xs_src, *ys_src, *zs_src;
+    const double *xs_w, *ys_w, *zs_w;
+    // refined -> coarse (InterpSalMultigrid fast path), per coarse index
+    const int *xb_src, *yb_src, *zb_src;
+    const double *xb_w, *yb_w, *zb_w;
+    // reductions
+    unsigned long long *redmax;         // [4] max slots (bit patterns of non-negative doubles)
+    double *redsum;                     // per-block partial sums
+};
+
+}  // namespace afid

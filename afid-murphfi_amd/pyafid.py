@@ -1,0 +1,233 @@
+"""Python binding of libafid_gpu.so (the C ABI declared in include/afid_gpu.h).
+
+The reference's host loop (src/main.F90:281-422) drives TimeMarcher and the
+CFL / divergence reductions; `Solver` exposes the same calls with the same
+names so a host written in Python reads like the Fortran one.  There is no
+CPU fallback: constructing a Solver without the built library, or without a
+GPU, raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libafid_gpu.so")
+
+FIELDS = ["vx", "vy", "vz", "temp", "pr", "rux", "ruy", "ruz", "rutemp",
+          "qcap", "dph", "dq", "hro", "rhs",
+          "sal", "rusal", "hsal", "rhsr", "vxr", "vyr", "vzr", "salc",
+          "tempbp", "temptp", "salbp", "saltp"]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+REFINED = {"sal", "rusal", "hsal", "rhsr", "vxr", "vyr", "vzr", "salbp", "saltp"}
+PLANES = {"tempbp", "temptp", "salbp", "saltp"}
+
+# every entry point declared in include/afid_gpu.h
+EXPORTS = [
+    "afid_gpu_init", "afid_gpu_finalize", "afid_last_error", "afid_device_bytes",
+    "afid_upload", "afid_download", "afid_set_substep",
+    "afid_explicit_terms_vx", "afid_explicit_terms_vy", "afid_explicit_terms_vz", "afid_explicit_terms_temp",
+    "afid_explicit_salinity", "afid_add_sal_buoyancy",
+    "afid_implicit_and_update_vx", "afid_implicit_and_update_vy", "afid_implicit_and_update_vz",
+    "afid_implicit_and_update_temp", "afid_implicit_salinity",
+    "afid_calc_local_divergence", "afid_solve_pressure_correction", "afid_correct_velocity", "afid_correct_pressure",
+    "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher",
+    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_synchronize",
+    "afid_set_timing", "afid_get_timing", "afid_grid_array",
+]
+
+ROUTINES = ["explicit_terms_vx", "explicit_terms_vy", "explicit_terms_vz", "explicit_terms_temp",
+            "explicit_salinity", "add_sal_buoyancy",
+            "implicit_and_update_vx", "implicit_and_update_vy", "implicit_and_update_vz",
+            "implicit_and_update_temp", "implicit_salinity",
+            "calc_local_divergence", "solve_pressure_correction", "correct_velocity", "correct_pressure",
+            "interp_vel_mgrd", "interp_sal_multigrid"]
+
+FP64, MIXED = 0, 1
+
+
+class AfidConfig(C.Structure):
+    """Mirror of afid_config: the bou.in value lines (ReadInputFile.F90:20-76)."""
+    _fields_ = [
+        ("nxm", C.c_int32), ("nym", C.c_int32), ("nzm", C.c_int32), ("nsst", C.c_int32),
+        ("multires", C.c_int32), ("nxmr", C.c_int32), ("nymr", C.c_int32), ("nzmr", C.c_int32),
+        ("salinity", C.c_int32), ("phasefield", C.c_int32),
+        ("alx3", C.c_double), ("ylen", C.c_double), ("zlen", C.c_double),
+        ("istr3", C.c_int32), ("str3", C.c_double), ("istr3r", C.c_int32),
+        ("rayt", C.c_double), ("prat", C.c_double), ("rays", C.c_double), ("pras", C.c_double),
+        ("ffscales", C.c_int32),
+        ("inslws", C.c_int32), ("inslwn", C.c_int32), ("tfixs", C.c_int32), ("tfixn", C.c_int32),
+        ("sfixs", C.c_int32), ("sfixn", C.c_int32),
+        ("active_t", C.c_int32), ("active_s", C.c_int32), ("gaxis", C.c_int32),
+        ("xplusu", C.c_double), ("xminusu", C.c_double), ("dpdy", C.c_double), ("dpdz", C.c_double),
+        ("precision", C.c_int32),
+    ]
+
+
+DEFAULTS = dict(nxm=16, nym=16, nzm=16, nsst=3, multires=0, nxmr=0, nymr=0, nzmr=0,
+                salinity=0, phasefield=0, alx3=1.0, ylen=1.0, zlen=1.0,
+                istr3=8, str3=0.0, istr3r=8, rayt=1e5, prat=1.0, rays=-1e5, pras=1.0, ffscales=0,
+                inslws=1, inslwn=1, tfixs=1, tfixn=1, sfixs=1, sfixn=1,
+                active_t=1, active_s=0, gaxis=1, xplusu=0.0, xminusu=0.0, dpdy=0.0, dpdz=0.0,
+                precision=FP64)
+
+
+def make_config(d):
+    v = dict(DEFAULTS)
+    v.update(d)
+    c = AfidConfig()
+    for k, _ in AfidConfig._fields_:
+        setattr(c, k, v[k])
+    return c
+
+
+_lib = None
+
+
+def lib():
+    """Load libafid_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libafid_gpu.so is not built: run __graft_entry__.build() "
+                               "or python afid-murphfi_amd/build_lib.py")
+        L = C.CDLL(LIB_PATH)
+        vp, i64p, dp = C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)
+        L.afid_gpu_init.argtypes = [C.POINTER(AfidConfig), C.POINTER(vp)]
+        L.afid_gpu_finalize.argtypes = [vp]
+        L.afid_last_error.argtypes = [vp]
+        L.afid_last_error.restype = C.c_char_p
+        L.afid_device_bytes.argtypes = [vp]
+        L.afid_device_bytes.restype = C.c_int64
+        L.afid_upload.argtypes = [vp, C.c_int, dp, i64p, i64p]
+        L.afid_download.argtypes = [vp, C.c_int, dp, i64p, i64p]
+        L.afid_set_substep.argtypes = [vp, C.c_double, C.c_int]
+        L.afid_time_marcher.argtypes = [vp, C.c_double]
+        for r in ROUTINES:
+            getattr(L, "afid_" + r).argtypes = [vp]
+        L.afid_calc_max_cfl.argtypes = [vp, dp, dp]
+        L.afid_check_divergence.argtypes = [vp, dp, dp]
+        L.afid_global_quantities.argtypes = [vp, dp]
+        L.afid_synchronize.argtypes = [vp]
+        L.afid_set_timing.argtypes = [vp, C.c_int]
+        L.afid_get_timing.argtypes = [vp, C.POINTER(C.c_char_p), dp, C.c_int, C.POINTER(C.c_int)]
+        L.afid_grid_array.argtypes = [vp, C.c_char_p, dp, C.c_int64, i64p]
+        _lib = L
+    return _lib
+
+
+class AfidError(RuntimeError):
+    pass
+
+
+class Solver:
+    """One device context: the module state of the reference on the GPU."""
+
+    def __init__(self, cfg: dict):
+        self.cfgd = dict(DEFAULTS)
+        self.cfgd.update(cfg)
+        self.cfg = make_config(self.cfgd)
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.afid_gpu_init(C.byref(self.cfg), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.L.afid_last_error(h).decode() if h else "no context"
+            if h:
+                self.L.afid_gpu_finalize(h)
+            self.h = None
+            raise AfidError(f"afid_gpu_init failed ({rc}): {msg}")
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.afid_gpu_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise AfidError(f"{what} failed ({rc}): {self.L.afid_last_error(self.h).decode()}")
+
+    # -- shapes
+    def shape(self, name):
+        c = self.cfgd
+        ref = name in REFINED
+        ny, nz = (c["nymr"], c["nzmr"]) if ref else (c["nym"], c["nzm"])
+        if name in PLANES:
+            return (nz, ny, 1)
+        nx = (c["nxmr"] + 1) if ref else (c["nxm"] + 1)
+        return (nz, ny, nx)
+
+    # -- data movement
+    def upload(self, name, arr, lbound=(1, 1, 1)):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        ext = (C.c_int64 * 3)(a.shape[2], a.shape[1], a.shape[0])
+        lb = (C.c_int64 * 3)(*lbound)
+        self._check(self.L.afid_upload(self.h, FIELD_ID[name], a.ctypes.data_as(C.POINTER(C.c_double)), lb, ext),
+                    "afid_upload " + name)
+
+    def download(self, name, shape=None, lbound=(1, 1, 1)):
+        shape = shape or self.shape(name)
+        a = np.zeros(shape, dtype=np.float64)
+        ext = (C.c_int64 * 3)(shape[2], shape[1], shape[0])
+        lb = (C.c_int64 * 3)(*lbound)
+        self._check(self.L.afid_download(self.h, FIELD_ID[name], a.ctypes.data_as(C.POINTER(C.c_double)), lb, ext),
+                    "afid_download " + name)
+        return a
+
+    # -- the TimeMarcher call tree
+    def set_substep(self, dt, ns):
+        self._check(self.L.afid_set_substep(self.h, dt, ns), "afid_set_substep")
+
+    def call(self, routine):
+        self._check(getattr(self.L, "afid_" + routine)(self.h), "afid_" + routine)
+
+    def time_marcher(self, dt):
+        self._check(self.L.afid_time_marcher(self.h, dt), "afid_time_marcher")
+
+    def synchronize(self):
+        self._check(self.L.afid_synchronize(self.h), "afid_synchronize")
+
+    # -- reductions
+    def calc_max_cfl(self):
+        a, b = C.c_double(), C.c_double()
+        self._check(self.L.afid_calc_max_cfl(self.h, C.byref(a), C.byref(b)), "afid_calc_max_cfl")
+        return a.value, b.value
+
+    def check_divergence(self):
+        a, b = C.c_double(), C.c_double()
+        self._check(self.L.afid_check_divergence(self.h, C.byref(a), C.byref(b)), "afid_check_divergence")
+        return a.value, b.value
+
+    def global_quantities(self):
+        out = (C.c_double * 4)()
+        self._check(self.L.afid_global_quantities(self.h, out), "afid_global_quantities")
+        return dict(zip(["nu", "ke", "vrms", "tmean"], list(out)))
+
+    # -- diagnostics
+    def device_bytes(self):
+        return int(self.L.afid_device_bytes(self.h))
+
+    def set_timing(self, on=True):
+        self._check(self.L.afid_set_timing(self.h, 1 if on else 0), "afid_set_timing")
+
+    def get_timing(self):
+        names = (C.c_char_p * 64)()
+        ms = (C.c_double * 64)()
+        n = C.c_int()
+        self._check(self.L.afid_get_timing(self.h, names, ms, 64, C.byref(n)), "afid_get_timing")
+        return {names[q].decode(): ms[q] for q in range(min(n.value, 64))}
+
+    def grid_array(self, name):
+        n = C.c_int64()
+        self._check(self.L.afid_grid_array(self.h, name.encode(), None, 0, C.byref(n)), "afid_grid_array")
+        out = np.zeros(n.value)
+        self._check(self.L.afid_grid_array(self.h, name.encode(), out.ctypes.data_as(C.POINTER(C.c_double)),
+                                           n.value, C.byref(n)), "afid_grid_array")
+        return out

@@ -1,0 +1,138 @@
+/*
+ * afid_gpu.h -- C ABI of the MI355X implementation of the AFiD-MuRPhFi RK3
+ * substep (TimeMarcher and its callees).
+ *
+ * The reference has no plugin or FFI layer: its boundary is the Fortran call
+ * tree under TimeMarcher (src/flow_solver/TimeMarcher.F90:13-165), made of
+ * argument-less subroutines acting on module arrays.  Each entry point below
+ * replaces one of those subroutines (cited beside it); the Fortran host binds
+ * them with iso_c_binding (afid-murphfi_amd/fortran/afid_gpu_mod.F90).
+ *
+ * Conventions
+ *  - every function returns 0 on success, nonzero on error (AFID_E_*);
+ *    afid_last_error() gives a message;
+ *  - calls are made from one host thread; work is enqueued on the context's
+ *    HIP stream and runs asynchronously; functions that return scalars and
+ *    afid_download synchronise;
+ *  - host arrays are Fortran column-major (x fastest) with explicit lower
+ *    bounds and extents per dimension, exactly as the reference allocates them
+ *    (AuxiliaryRoutines.F90:79-99): e.g. vx(1:nx, -1:nym+2, -1:nzm+2) is
+ *    lbound = {1,-1,-1}, extent = {nx, nym+4, nzm+4}.  Compact arrays use
+ *    lbound = {1,1,1}.  Only the interior is read on upload; download writes
+ *    the interior and fills any periodic y/z halo the host array has.
+ */
+#ifndef AFID_GPU_H
+#define AFID_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AFID_OK 0
+#define AFID_E_ARG 1      /* invalid argument or unsupported configuration */
+#define AFID_E_HIP 2      /* HIP runtime error */
+#define AFID_E_MEM 3      /* device allocation failed */
+#define AFID_E_STATE 4    /* call out of order (e.g. no substep set) */
+
+/* Arithmetic form of the context. */
+#define AFID_FP64 0       /* every field and every operation in FP64 */
+#define AFID_MIXED 1      /* FP32 refined-grid transfer fields, FP64 state */
+
+/* The bou.in value lines (ReadInputFile.F90:20-76) plus the arithmetic form. */
+typedef struct afid_config {
+    int32_t nxm, nym, nzm, nsst;
+    int32_t multires, nxmr, nymr, nzmr;
+    int32_t salinity, phasefield;
+    double alx3, ylen, zlen;
+    int32_t istr3;  double str3; int32_t istr3r;   /* istr3 8 = x_k = L/2 (1-cos(pi k/nxm)) */
+    double rayt, prat, rays, pras; int32_t ffscales;
+    int32_t inslws, inslwn, tfixs, tfixn, sfixs, sfixn;
+    int32_t active_t, active_s, gaxis;
+    double xplusu, xminusu, dpdy, dpdz;
+    int32_t precision;                             /* AFID_FP64 or AFID_MIXED */
+} afid_config;
+
+typedef struct afid_ctx afid_ctx;
+
+/* Field identifiers (module arrays of param.F90:100-126, salinity.F90:14-35). */
+enum afid_field {
+    AFID_VX = 0, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR,
+    AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP,
+    AFID_QCAP, AFID_DPH, AFID_DQ, AFID_HRO, AFID_RHS,
+    AFID_SAL, AFID_RUSAL, AFID_HSAL, AFID_RHSR, AFID_VXR, AFID_VYR, AFID_VZR, AFID_SALC,
+    AFID_TEMPBP, AFID_TEMPTP, AFID_SALBP, AFID_SALTP,
+    AFID_NFIELDS
+};
+
+/* Set-up: ReadInputFile derived constants (ReadInputFile.F90:101-122),
+ * InitTimeMarchScheme (InitTimeMarchScheme.F90:11-52), CreateGrid
+ * (CreateGrid.F90:11-179), InitPressureSolver (pressure.F90:61-136),
+ * CreateMgrdGrid (CreateMgrdGrid.F90:11-166), CreateMgrdStencil
+ * (CreateMgrdStencil.F90:1-81) and the allocation of InitVariables /
+ * InitMgrdVariables / InitSalVariables.  All device memory is allocated here. */
+int afid_gpu_init(const afid_config *cfg, afid_ctx **out);
+/* QuitRoutine.F90:46-59 deallocation. */
+int afid_gpu_finalize(afid_ctx *ctx);
+const char *afid_last_error(const afid_ctx *ctx);
+/* Device bytes held by the context. */
+int64_t afid_device_bytes(const afid_ctx *ctx);
+
+/* Host <-> device copies of a module array (HdfReadContinua.F90:11-107 /
+ * HdfWriteRealHalo3D, HdfRoutines.F90:428-520, read and write these arrays). */
+int afid_upload(afid_ctx *ctx, int field, const double *host, const int64_t lbound[3], const int64_t extent[3]);
+int afid_download(afid_ctx *ctx, int field, double *host, const int64_t lbound[3], const int64_t extent[3]);
+
+/* TimeMarcher.F90:28,38-40: beta = dt/ren/2 and (al, ga, ro) of substep ns (1-based). */
+int afid_set_substep(afid_ctx *ctx, double dt, int ns);
+
+/* Explicit terms (stage 1 of a substep). */
+int afid_explicit_terms_vx(afid_ctx *ctx);      /* ExplicitTermsVX.F90:11-131   -> qcap */
+int afid_explicit_terms_vy(afid_ctx *ctx);      /* ExplicitTermsVY.F90:11-126   -> dph  */
+int afid_explicit_terms_vz(afid_ctx *ctx);      /* ExplicitTermsVZ.F90:11-121   -> dq   */
+int afid_explicit_terms_temp(afid_ctx *ctx);    /* ExplicitTermsTemp.F90:11-85  -> hro  */
+int afid_explicit_salinity(afid_ctx *ctx);      /* salinity.F90:268-366         -> hsal */
+int afid_add_sal_buoyancy(afid_ctx *ctx);       /* salinity.F90:495-506, TimeMarcher.F90:54-62 */
+
+/* Implicit update (RHS assembly + wall-normal tridiagonal solve + update). */
+int afid_implicit_and_update_vx(afid_ctx *ctx);   /* ImplicitAndUpdateVX.F90:14-89 + SolveImpEqnUpdate_X.F90:12-69 */
+int afid_implicit_and_update_vy(afid_ctx *ctx);   /* ImplicitAndUpdateVY.F90:14-95 + SolveImpEqnUpdate_YZ.F90:12-60 */
+int afid_implicit_and_update_vz(afid_ctx *ctx);   /* ImplicitAndUpdateVZ.F90:14-80 + SolveImpEqnUpdate_YZ.F90:12-60 */
+int afid_implicit_and_update_temp(afid_ctx *ctx); /* ImplicitAndUpdateTemp.F90:13-77 + SolveImpEqnUpdate_Temp.F90:11-63 */
+int afid_implicit_salinity(afid_ctx *ctx);        /* salinity.F90:369-454 */
+
+/* Pressure projection. */
+int afid_calc_local_divergence(afid_ctx *ctx);     /* pressure.F90:141-162 */
+int afid_solve_pressure_correction(afid_ctx *ctx); /* pressure.F90:165-236 */
+int afid_correct_velocity(afid_ctx *ctx);          /* CorrectVelocity.F90:24 -> RemoveDivergence, pressure.F90:240-267 */
+int afid_correct_pressure(afid_ctx *ctx);          /* pressure.F90:270-301 */
+
+/* Multi-resolution transfers. */
+int afid_interp_vel_mgrd(afid_ctx *ctx);       /* InterpVelMgrd.F90:1-243 */
+int afid_interp_sal_multigrid(afid_ctx *ctx);  /* salinity.F90:458-491 (fast path, HermiteInterpolations.F90:256-312) */
+
+/* One full RK3 step, the whole call tree of TimeMarcher.F90:13-165. */
+int afid_time_marcher(afid_ctx *ctx, double dt);
+
+/* Reductions read back to the host. */
+int afid_calc_max_cfl(afid_ctx *ctx, double *cflm, double *cflmr);      /* CalcMaxCFL.F90:11-72 */
+int afid_check_divergence(afid_ctx *ctx, double *qmax, double *qmaxr);  /* CheckDivergence.F90:11-87 */
+/* GlobalQuantities.F90:11-102: out = {Nu_vol, mean kinetic energy, rms velocity, mean T}. */
+int afid_global_quantities(afid_ctx *ctx, double out[4]);
+
+/* Wait for all work enqueued on the context. */
+int afid_synchronize(afid_ctx *ctx);
+
+/* Kernel timing of the last afid_time_marcher call, for the roofline: per
+ * kernel group, accumulated device milliseconds measured with HIP events on
+ * the context's stream.  Enabled by afid_set_timing(ctx, 1). */
+int afid_set_timing(afid_ctx *ctx, int on);
+int afid_get_timing(afid_ctx *ctx, const char **names, double *ms, int max, int *count);
+
+/* 1-D set-up arrays, for the set-up tests (copied into out, at most n). */
+int afid_grid_array(afid_ctx *ctx, const char *name, double *out, int64_t n, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

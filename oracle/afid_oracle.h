@@ -1,0 +1,102 @@
+/*
+ * afid_oracle.h -- FP64 CPU restatement of the AFiD-MuRPhFi RK3 substep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py (its cpu_baseline
+ * leg) may load it.  Nothing in the product links it or falls back to it.
+ *
+ * Each routine restates the arithmetic of the reference Fortran routine cited
+ * beside it (paths relative to the reference repository root).  Arrays are
+ * stored x-fastest, without halos: periodic y/z halos of the reference
+ * (src/flow_solver/2decomp/halo_common.F90:66-71, a memcpy at one rank) become
+ * index wraps.  Base arrays hold nx = nxm+1 x-entries (index 1..nx),
+ * refined arrays nxr = nxmr+1.
+ */
+#ifndef AFID_ORACLE_H
+#define AFID_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirrors the bou.in value lines read by src/flow_solver/ReadInputFile.F90:20-76. */
+typedef struct orc_config {
+    int32_t nxm, nym, nzm, nsst;
+    int32_t multires, nxmr, nymr, nzmr;
+    int32_t salinity, phasefield;
+    double alx3, ylen, zlen;
+    int32_t istr3;  double str3; int32_t istr3r;
+    double rayt, prat, rays, pras; int32_t ffscales;
+    int32_t inslws, inslwn, tfixs, tfixn, sfixs, sfixn;
+    int32_t active_t, active_s, gaxis;
+    double xplusu, xminusu, dpdy, dpdz;
+} orc_config;
+
+typedef struct orc_state orc_state;
+
+/* field ids (shared numbering with the product's C ABI) */
+enum {
+    ORC_VX = 0, ORC_VY, ORC_VZ, ORC_TEMP, ORC_PR,
+    ORC_RUX, ORC_RUY, ORC_RUZ, ORC_RUTEMP,
+    ORC_QCAP, ORC_DPH, ORC_DQ, ORC_HRO, ORC_RHS,
+    ORC_SAL, ORC_RUSAL, ORC_HSAL, ORC_RHSR, ORC_VXR, ORC_VYR, ORC_VZR, ORC_SALC,
+    ORC_TEMPBP, ORC_TEMPTP, ORC_SALBP, ORC_SALTP,
+    ORC_NFIELDS
+};
+
+/* 1-D grid arrays exposed for the grid tests */
+enum {
+    ORC_G_XC = 0, ORC_G_XM, ORC_G_G3RC, ORC_G_G3RM, ORC_G_D3XC, ORC_G_D3XM,
+    ORC_G_UDX3C, ORC_G_UDX3M,
+    ORC_G_AP3CK, ORC_G_AC3CK, ORC_G_AM3CK,
+    ORC_G_AP3SK, ORC_G_AC3SK, ORC_G_AM3SK,
+    ORC_G_AP3SSK, ORC_G_AC3SSK, ORC_G_AM3SSK,
+    ORC_G_AK1, ORC_G_AK2, ORC_G_AMPHK, ORC_G_ACPHK, ORC_G_APPHK,
+    ORC_G_XCR, ORC_G_XMR, ORC_G_UDX3MR, ORC_G_G3RMR,
+    ORC_G_AP3SSKR, ORC_G_AC3SSKR, ORC_G_AM3SSKR,
+    ORC_G_CXRS, ORC_G_CYRS, ORC_G_CZRS, ORC_G_CXSALC, ORC_G_CYSALC, ORC_G_CZSALC,
+    ORC_G_NGRID
+};
+
+orc_state *orc_create(const orc_config *cfg);
+void orc_destroy(orc_state *s);
+double *orc_field(orc_state *s, int id, int64_t dims[3]);  /* dims = (nz, ny, nx) */
+double *orc_grid(orc_state *s, int id, int64_t *n);
+int orc_int_grid(orc_state *s, int id, int64_t *n, int32_t **out); /* 0 irangs 1 jrangs 2 krangs 3 irangb */
+void orc_scalars(orc_state *s, double out[8]);  /* ren, pect, pecs, byct, bycs, dx, dy, dz */
+
+void orc_set_substep(orc_state *s, double dt, int ns);
+void orc_explicit_terms_vx(orc_state *s);
+void orc_explicit_terms_vy(orc_state *s);
+void orc_explicit_terms_vz(orc_state *s);
+void orc_explicit_terms_temp(orc_state *s);
+void orc_implicit_and_update_vx(orc_state *s);
+void orc_implicit_and_update_vy(orc_state *s);
+void orc_implicit_and_update_vz(orc_state *s);
+void orc_implicit_and_update_temp(orc_state *s);
+void orc_calc_local_divergence(orc_state *s);
+void orc_solve_pressure_correction(orc_state *s);
+void orc_correct_velocity(orc_state *s);
+void orc_correct_pressure(orc_state *s);
+void orc_explicit_salinity(orc_state *s);
+void orc_add_sal_buoyancy(orc_state *s);
+void orc_implicit_salinity(orc_state *s);
+void orc_interp_vel_mgrd(orc_state *s);
+void orc_interp_sal_multigrid(orc_state *s);
+void orc_time_marcher(orc_state *s, double dt);
+
+void orc_calc_max_cfl(orc_state *s, double *cflm, double *cflmr);
+void orc_check_divergence(orc_state *s, double *qmax, double *qmaxr);
+void orc_global_quantities(orc_state *s, double out[4]); /* Nu_vol, KE, vrms, Tmean */
+
+/* free-standing numerics used by the pins */
+void orc_fft(double *re, double *im, int n, int sign);      /* in place, unnormalised */
+void orc_dft_naive(const double *re, const double *im, double *ore, double *oim, int n, int sign);
+void orc_tridiag_solve(int n, const double *a, const double *b, const double *c, double *d);
+void orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
