@@ -63,6 +63,7 @@ struct afid_ctx {
     double2 *spec = nullptr;
     // metrics
     std::map<std::string, std::vector<double>> host1d;   // for afid_grid_array
+    const double *d_xm = nullptr, *d_xmr = nullptr;
     const double *d_udx3m, *d_udx3c, *d_rudx3m, *d_ap3ck, *d_ac3ck, *d_am3ck, *d_ap3sk, *d_ac3sk, *d_am3sk;
     const double *d_ap3ssk, *d_ac3ssk, *d_am3ssk, *d_amphk, *d_acphk, *d_apphk, *d_g3rc;
     const double *d_udx3mr = nullptr, *d_ap3sskr = nullptr, *d_ac3sskr = nullptr, *d_am3sskr = nullptr, *d_lxa = nullptr;
@@ -275,12 +276,13 @@ static TriFactors factors(const afid_ctx *c, int kind) {
 
 template <int CH>
 static void launch_tri(afid_ctx *c, const TriFactors &f, double *rhs, double *q, int ldx, long long plane, int ny,
-                       long long ncols, int u0, int u1, int z0, int z1) {
+                       long long ncols, int u0, int u1, int z0, int z1, int keep) {
     const unsigned nb = unsigned((ncols + 3) / 4);
-    hipLaunchKernelGGL(k_tri_update<CH>, dim3(nb), dim3(256), 0, c->stream, f, rhs, q, ldx, plane, ny, ncols, u0, u1, z0, z1);
+    hipLaunchKernelGGL(k_tri_update<CH>, dim3(nb), dim3(256), 0, c->stream, f, rhs, q, ldx, plane, ny, ncols, u0, u1, z0, z1, keep);
 }
 
-static int run_tri(afid_ctx *c, int kind, double *rhs, double *q) {
+// keep = 1: leave the increment in rhs, as the reference's rhs array holds it
+static int run_tri(afid_ctx *c, int kind, double *rhs, double *q, int keep = 1) {
     TriFactors f = factors(c, kind);
     const bool ref = kind == MAT_S;
     const int ldx = ref ? c->ldxr : c->ldx, ny = ref ? c->nyr : c->ny, nz = ref ? c->nzr : c->nz;
@@ -288,7 +290,7 @@ static int run_tri(afid_ctx *c, int kind, double *rhs, double *q) {
     int u0 = 0, u1 = f.n, z0 = -1, z1 = -1;
     if (kind == MAT_X) { u0 = 1; u1 = c->nxm; z0 = 0; z1 = c->nx - 1; }
     switch (f.chunk) {
-#define AFID_TRI_CASE(CH) case CH: launch_tri<CH>(c, f, rhs, q, ldx, plane, ny, ncols, u0, u1, z0, z1); break;
+#define AFID_TRI_CASE(CH) case CH: launch_tri<CH>(c, f, rhs, q, ldx, plane, ny, ncols, u0, u1, z0, z1, keep); break;
         AFID_TRI_CASE(1) AFID_TRI_CASE(2) AFID_TRI_CASE(3) AFID_TRI_CASE(4) AFID_TRI_CASE(5) AFID_TRI_CASE(6)
         AFID_TRI_CASE(7) AFID_TRI_CASE(8) AFID_TRI_CASE(9) AFID_TRI_CASE(10) AFID_TRI_CASE(11) AFID_TRI_CASE(12)
         AFID_TRI_CASE(13) AFID_TRI_CASE(14) AFID_TRI_CASE(15) AFID_TRI_CASE(16)
@@ -425,6 +427,8 @@ static int setup(afid_ctx *c) {
     auto up = [&](const std::vector<double> &v, const double **d) { return rc ? rc : (rc = upload_vec(c, v, d)); };
     rc = 0;
     std::vector<double> udx3m_v = farr_range(udx3m, 1, nxm), udx3c_v = farr_range(udx3c, 1, nx), g3rc_v = farr_range(g3rc, 1, nx);
+    std::vector<double> xm_v = farr_range(xm, 1, nxm);
+    up(xm_v, &c->d_xm);
     up(udx3m_v, &c->d_udx3m); up(udx3c_v, &c->d_udx3c); up(rudx3m, &c->d_rudx3m); up(g3rc_v, &c->d_g3rc);
     up(c->h_ap3ck, &c->d_ap3ck); up(c->h_ac3ck, &c->d_ac3ck); up(c->h_am3ck, &c->d_am3ck);
     up(c->h_ap3sk, &c->d_ap3sk); up(c->h_ac3sk, &c->d_ac3sk); up(c->h_am3sk, &c->d_am3sk);
@@ -587,6 +591,8 @@ static int setup(afid_ctx *c) {
     if ((rc = upload_vec(c, by.w, &pd))) return rc; c->d_yb_w = pd;
     if ((rc = upload_vec(c, bz.w, &pd))) return rc; c->d_zb_w = pd;
     std::vector<double> udx3mr_v = farr_range(udx3mr, 1, nxmr);
+    std::vector<double> xmr_v = farr_range(xmr, 1, nxmr);
+    up(xmr_v, &c->d_xmr);
     up(udx3mr_v, &c->d_udx3mr); up(c->h_ap3sskr, &c->d_ap3sskr); up(c->h_ac3sskr, &c->d_ac3sskr); up(c->h_am3sskr, &c->d_am3sskr);
     up(lxa, &c->d_lxa);
     if (rc) return rc;
@@ -887,14 +893,14 @@ int afid_time_marcher(afid_ctx *c, double dt) {
             if ((rc = check_launch(c, "k_explicit_rhs_sal"))) return rc;
         }
         tstart(c, "tridiag_base");
-        if ((rc = run_tri(c, MAT_X, c->rhs4[0], c->f[AFID_VX]))) return rc;
-        if ((rc = run_tri(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]))) return rc;
-        if ((rc = run_tri(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]))) return rc;
-        if ((rc = run_tri(c, MAT_T, c->rhs4[3], c->f[AFID_TEMP]))) return rc;
+        if ((rc = run_tri(c, MAT_X, c->rhs4[0], c->f[AFID_VX], 0))) return rc;
+        if ((rc = run_tri(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY], 0))) return rc;
+        if ((rc = run_tri(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ], 0))) return rc;
+        if ((rc = run_tri(c, MAT_T, c->rhs4[3], c->f[AFID_TEMP], 0))) return rc;
         tstop(c);
         if (c->sal) {
             tstart(c, "tridiag_sal");
-            if ((rc = run_tri(c, MAT_S, c->f[AFID_RHSR], c->f[AFID_SAL]))) return rc;
+            if ((rc = run_tri(c, MAT_S, c->f[AFID_RHSR], c->f[AFID_SAL], 0))) return rc;
             tstop(c);
         }
         if ((rc = poisson(c, 0))) return rc;
@@ -907,6 +913,27 @@ int afid_time_marcher(afid_ctx *c, double dt) {
         }
     }
     return AFID_OK;
+}
+
+// CreateInitialConditions.F90:150-201 (RBC branch) with u = 0 and seeded
+// noise; also the scalar of CreateInitialSalinity on the refined grid.
+int afid_create_initial_conditions(afid_ctx *c, uint64_t seed, double eps) {
+    if (!c) return AFID_E_ARG;
+    for (int f : {AFID_VX, AFID_VY, AFID_VZ, AFID_PR, AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP})
+        HIPCHK(c, hipMemsetAsync(c->f[f], 0, sizeof(double) * size_t(c->nz) * c->ny * c->ldx, c->stream));
+    hipLaunchKernelGGL(k_init_scalar, point_grid(c->nx, c->ny, c->nz), kPointBlock, 0, c->stream, c->f[AFID_TEMP],
+                       c->f[AFID_TEMPBP], c->f[AFID_TEMPTP], c->d_xm, c->nxm, c->nx, c->ldx, c->ny, c->nz, c->c.alx3, eps,
+                       (unsigned long long)seed);
+    int rc = check_launch(c, "k_init_scalar");
+    if (rc || !c->sal) return rc;
+    for (int f : {AFID_RUSAL, AFID_VXR, AFID_VYR, AFID_VZR})
+        HIPCHK(c, hipMemsetAsync(c->f[f], 0, sizeof(double) * size_t(c->nzr) * c->nyr * c->ldxr, c->stream));
+    hipLaunchKernelGGL(k_init_scalar, point_grid(c->nxr, c->nyr, c->nzr), kPointBlock, 0, c->stream, c->f[AFID_SAL],
+                       c->f[AFID_SALBP], c->f[AFID_SALTP], c->d_xmr, c->nxmr, c->nxr, c->ldxr, c->nyr, c->nzr, c->c.alx3, eps,
+                       (unsigned long long)(seed + 1));
+    if ((rc = check_launch(c, "k_init_scalar"))) return rc;
+    // salc from the initial S (the reference interpolates at start-up, main.F90:206-213)
+    return afid_interp_sal_multigrid(c);
 }
 
 int afid_synchronize(afid_ctx *c) {

@@ -185,11 +185,12 @@ __device__ __forceinline__ void wave_tri_solve(double (&v)[CH], const TriFactors
 // One column per wavefront: q[r] += solve(rhs)[r] for r in [u0,u1); rows
 // listed in zero0/zero1 (if >= 0) are set to zero afterwards (the wall rows
 // of vx, ImplicitAndUpdateVX.F90:86-87).  rhs holds the raw RHS on entry and
-// the increment on exit (as the reference's rhs array does).
+// the increment on exit when keep_rhs is set (as the reference's rhs array
+// does); the fused step skips that write.
 template <int CH>
 __global__ void __launch_bounds__(256) k_tri_update(TriFactors f, double *rhs, double *q, int ldx,
                                                     long long plane, int ny, long long ncols, int u0, int u1,
-                                                    int zero0, int zero1) {
+                                                    int zero0, int zero1, int keep_rhs) {
     const long long col = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (col >= ncols) return;
     const int lane = threadIdx.x & 63;
@@ -206,7 +207,7 @@ __global__ void __launch_bounds__(256) k_tri_update(TriFactors f, double *rhs, d
     for (int m = 0; m < CH; ++m) {
         const int r = lane * CH + m;
         if (r < f.n) {
-            rhs[base + r] = v[m];
+            if (keep_rhs) rhs[base + r] = v[m];
             if (r >= u0 && r < u1) q[base + r] += v[m];
             if (r == zero0 || r == zero1) q[base + r] = 0.0;
         }
@@ -494,6 +495,35 @@ __global__ void __launch_bounds__(256) k_interp_sal(KP p) {
         res = a == 0 ? q1 * wx[0] : res + q1 * wx[a];
     }
     p.salc[bidx(p, k, j, i)] = res;
+}
+
+// ---------------------------------------------------------------------------
+// seeded initial condition (CreateInitialConditions.F90:175-201 RBC branch)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double splitmix_uniform(unsigned long long seed, unsigned long long idx) {
+    unsigned long long z = (idx ^ seed) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return double(z >> 11) * (1.0 / 9007199254740992.0);
+}
+// scalar = bottom + (top - bottom) x/alx3 + noise, noise amplitude clamped near
+// the walls as the reference does (CreateInitialConditions.F90:187-201); the
+// noise of point (k,j,i) is U[0,1) of splitmix64(seed ^ linear index) over the
+// (nxf, ny, nz) array, x fastest.
+__global__ void __launch_bounds__(256) k_init_scalar(double *f, const double *bot, const double *top, const double *xm,
+                                                     int nxm, int nxf, int ldx, int ny, int nz, double alx3,
+                                                     double eps, unsigned long long seed) {
+    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    if (k >= nxf || j >= ny) return;
+    const long long o = ((long long)i * ny + j) * ldx + k;
+    if (k >= nxm) { f[o] = 0.0; return; }
+    const long long p2 = (long long)i * ny + j;
+    const double x = xm[k];
+    double v = bot[p2] + (top[p2] - bot[p2]) * x / alx3;
+    const double r = splitmix_uniform(seed, (unsigned long long)(((long long)i * ny + j) * nxf + k));
+    const double amp = (fabs(x - 0.5) + eps > 0.5) ? 0.5 - fabs(x - 0.5) : eps;
+    f[o] = v + amp * (2.0 * r - 1.0);
 }
 
 // ---------------------------------------------------------------------------

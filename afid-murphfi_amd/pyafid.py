@@ -33,6 +33,7 @@ EXPORTS = [
     "afid_calc_local_divergence", "afid_solve_pressure_correction", "afid_correct_velocity", "afid_correct_pressure",
     "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher",
     "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_synchronize",
+    "afid_create_initial_conditions",
     "afid_set_timing", "afid_get_timing", "afid_grid_array",
 ]
 
@@ -109,6 +110,7 @@ def lib():
         L.afid_check_divergence.argtypes = [vp, dp, dp]
         L.afid_global_quantities.argtypes = [vp, dp]
         L.afid_synchronize.argtypes = [vp]
+        L.afid_create_initial_conditions.argtypes = [vp, C.c_uint64, C.c_double]
         L.afid_set_timing.argtypes = [vp, C.c_int]
         L.afid_get_timing.argtypes = [vp, C.POINTER(C.c_char_p), dp, C.c_int, C.POINTER(C.c_int)]
         L.afid_grid_array.argtypes = [vp, C.c_char_p, dp, C.c_int64, i64p]
@@ -190,6 +192,9 @@ class Solver:
 
     def time_marcher(self, dt):
         self._check(self.L.afid_time_marcher(self.h, dt), "afid_time_marcher")
+
+    def create_initial_conditions(self, seed, eps=1e-3):
+        self._check(self.L.afid_create_initial_conditions(self.h, seed, eps), "afid_create_initial_conditions")
 
     def synchronize(self):
         self._check(self.L.afid_synchronize(self.h), "afid_synchronize")

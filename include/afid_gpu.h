@@ -120,6 +120,15 @@ int afid_check_divergence(afid_ctx *ctx, double *qmax, double *qmaxr);  /* Check
 /* GlobalQuantities.F90:11-102: out = {Nu_vol, mean kinetic energy, rms velocity, mean T}. */
 int afid_global_quantities(afid_ctx *ctx, double out[4]);
 
+/* Seeded initial state (CreateInitialConditions.F90:150-201, RBC branch, with
+ * u = 0): T = tempbp + (temptp - tempbp) x + noise, the wall-clamped noise of
+ * amplitude eps being U[0,1) of splitmix64(seed ^ linear index) mapped to
+ * [-eps, eps]; with salinity, S likewise on the refined grid from salbp/saltp
+ * with seed+1, then salc.  The wall planes must be uploaded first.  The
+ * reference seeds from the system clock (CreateInitialConditions.F90:24);
+ * a counter-based generator makes CPU and GPU states identical. */
+int afid_create_initial_conditions(afid_ctx *ctx, uint64_t seed, double eps);
+
 /* Wait for all work enqueued on the context. */
 int afid_synchronize(afid_ctx *ctx);
 
