@@ -59,6 +59,8 @@ struct afid_ctx {
     int64_t bytes = 0;
     double *f[AFID_NFIELDS] = {nullptr};
     double *rhs4[4] = {nullptr};       // rhs for vx, vy, vz, temp
+    double *alt[4] = {nullptr};        // second buffers of vx, vy, vz, temp (fused column step)
+    double *sal_alt = nullptr;
     double *tpdv = nullptr;
     double2 *spec = nullptr;
     // metrics
@@ -83,6 +85,11 @@ struct afid_ctx {
     int Ly = 1, Lz = 1, lds_y = 0, lds_z = 0;
     PoissonFactors pf{};
     int pchunk = 1;
+    // LDS plan of the transfer kernels
+    int vxr_xc = 0, vxr_cj = 0, vxr_ci = 0, vy_cs = 0, vy_co = 0, vz_cs = 0, vz_co = 0;
+    size_t lds_vxr = 0, lds_vy = 0, lds_vz = 0;
+    int salc_xr = 0, salc_yr = 0;
+    size_t lds_salc = 0;
     // reductions
     unsigned long long *redmax = nullptr;
     double *redsum = nullptr, *redout = nullptr;
@@ -185,6 +192,8 @@ static KP make_kp(const afid_ctx *c) {
 
 static dim3 point_grid(int nxl, int ny, int nz) { return dim3((nxl + 63) / 64, (ny + 3) / 4, nz); }
 static const dim3 kPointBlock(64, 4, 1);
+// max-reductions loop over z inside the kernel: few blocks, few atomics
+static dim3 red_grid(int nxl, int ny, int nz) { return dim3((nxl + 63) / 64, (ny + 3) / 4, std::min(nz, 8)); }
 
 static void tstart(afid_ctx *c, const char *name) {
     if (!c->timing) return;
@@ -445,6 +454,8 @@ static int setup(afid_ctx *c) {
     for (int q = 0; q < 4; ++q)
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->rhs4[q]), sizeof(double) * nb))) return rc;
     c->f[AFID_RHS] = c->rhs4[0];
+    for (int q = 0; q < 4; ++q)
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->alt[q]), sizeof(double) * nb))) return rc;
     for (int f : {AFID_TEMPBP, AFID_TEMPTP})
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * size_t(c->ny) * c->nz))) return rc;
     if ((rc = dalloc(c, reinterpret_cast<void **>(&c->spec), sizeof(double2) * size_t(c->nz) * c->nymh * c->nxm))) return rc;
@@ -600,11 +611,39 @@ static int setup(afid_ctx *c) {
     const size_t nr = size_t(nzr) * nyr * c->ldxr;
     for (int f = AFID_SAL; f <= AFID_VZR; ++f)
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * nr))) return rc;
+    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->sal_alt), sizeof(double) * nr))) return rc;
     for (int f : {AFID_SALBP, AFID_SALTP})
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * size_t(nyr) * nzr))) return rc;
-    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->tpdv), sizeof(double) * size_t(c->nz) * c->ny * c->ldt))) return rc;
     if (nxmr > 1024) return set_err(c, AFID_E_ARG, "refined wall-normal size above 1024 is not supported");
-    if (4 * (nx + 3) * 8 > 65536) return set_err(c, AFID_E_ARG, "base wall-normal size too large for the vxr transfer");
+    // LDS plan of the tiled transfer kernels: largest coarse block of any tile
+    auto span = [](const std::vector<int> &src, int n, int tile) {
+        int m = 0;
+        for (int a = 0; a < n; a += tile) {
+            const int b = std::min(a + tile, n) - 1;
+            m = std::max(m, src[size_t(b)] - src[size_t(a)] + 4);
+        }
+        return m;
+    };
+    c->vxr_xc = span(sx.src, nxmr, 64);
+    c->vxr_cj = span(sy.src, nyr, VXR_TJ);
+    c->vxr_ci = span(sz.src, nzr, VXR_TI);
+    c->lds_vxr = sizeof(double) * size_t(c->vxr_xc * c->vxr_cj * c->vxr_ci + c->vxr_xc * c->vxr_cj * VXR_TI +
+                                         c->vxr_xc * VXR_TJ * VXR_TI + VXR_TJ * VXR_TI);
+    c->vy_cs = span(sy.src, nyr - 1, VYZ_CHUNK); c->vy_co = span(sz.src, nzr, VYZ_LINES);
+    c->vz_cs = span(sz.src, nzr - 1, VYZ_CHUNK); c->vz_co = span(sy.src, nyr, VYZ_LINES);
+    auto lds_yz = [&](int cs, int co) {
+        return sizeof(double) * size_t(std::max(c->vxr_xc * cs * co, cs * VYZ_LINES * 64) + c->vxr_xc * cs * VYZ_LINES);
+    };
+    c->lds_vy = lds_yz(c->vy_cs, c->vy_co);
+    c->lds_vz = lds_yz(c->vz_cs, c->vz_co);
+    {
+        std::vector<int> xb0(xb_src.begin(), xb_src.begin() + nxm);
+        c->salc_xr = span(xb0, nxm, 64);
+        c->salc_yr = span(yb_src, c->ny, SALC_TJ);
+        c->lds_salc = sizeof(double) * size_t(2 * c->salc_yr * c->salc_xr);
+    }
+    if (c->lds_vxr > 65536 || c->lds_vy > 65536 || c->lds_vz > 65536 || c->lds_salc > 65536)
+        return set_err(c, AFID_E_ARG, "refinement ratio too small for the transfer tiles (LDS plan above 64 KiB)");
     return AFID_OK;
 }
 
@@ -819,48 +858,22 @@ int afid_correct_pressure(afid_ctx *c) { NEED_SUBSTEP(c) return project(c, 0, 1)
 
 static int interp_vel(afid_ctx *c) {
     KP p = make_kp(c);
-    const dim3 gg((c->nx + 3 + 63) / 64, (c->ny + 3) / 4, c->nz);
     int rc;
-    // vx
-    tstart(c, "mgrd_gradient");
-    hipLaunchKernelGGL(k_mgrd_gradient, gg, kPointBlock, 0, c->stream, p, 0);
+    tstart(c, "mgrd_vxr");
+    hipLaunchKernelGGL(k_vxr_tile, dim3(unsigned((c->nyr + VXR_TJ - 1) / VXR_TJ), unsigned((c->nzr + VXR_TI - 1) / VXR_TI)),
+                       dim3(256), c->lds_vxr, c->stream, p, c->vxr_xc, c->vxr_cj, c->vxr_ci);
     tstop(c);
-    if ((rc = check_launch(c, "k_mgrd_gradient"))) return rc;
-    {
-        const int ch = (c->nxmr + 63) / 64;
-        const long long ncol = (long long)c->nyr * c->nzr;
-        const dim3 gr(unsigned((ncol + 3) / 4));
-        const size_t lds = sizeof(double) * 4 * size_t(c->nx + 3);
-        tstart(c, "mgrd_vxr");
-        switch (ch) {
-#define AFID_VXR_CASE(CH) case CH: hipLaunchKernelGGL(k_vxr<CH>, gr, dim3(256), lds, c->stream, p); break;
-            AFID_VXR_CASE(1) AFID_VXR_CASE(2) AFID_VXR_CASE(3) AFID_VXR_CASE(4) AFID_VXR_CASE(5) AFID_VXR_CASE(6)
-            AFID_VXR_CASE(7) AFID_VXR_CASE(8) AFID_VXR_CASE(9) AFID_VXR_CASE(10) AFID_VXR_CASE(11) AFID_VXR_CASE(12)
-            AFID_VXR_CASE(13) AFID_VXR_CASE(14) AFID_VXR_CASE(15) AFID_VXR_CASE(16)
-#undef AFID_VXR_CASE
-        default: return set_err(c, AFID_E_ARG, "nxmr too large");
-        }
-        tstop(c);
-        if ((rc = check_launch(c, "k_vxr"))) return rc;
-    }
-    // vy
-    tstart(c, "mgrd_gradient");
-    hipLaunchKernelGGL(k_mgrd_gradient, gg, kPointBlock, 0, c->stream, p, 1);
-    tstop(c);
-    if ((rc = check_launch(c, "k_mgrd_gradient"))) return rc;
+    if ((rc = check_launch(c, "k_vxr_tile"))) return rc;
     tstart(c, "mgrd_vyr");
-    hipLaunchKernelGGL(k_vyr, dim3(unsigned((c->nxr + 63) / 64), unsigned(c->nzr)), dim3(64), 0, c->stream, p);
+    hipLaunchKernelGGL(k_vyz_tile<1>, dim3(unsigned((c->nxmr + 63) / 64), unsigned((c->nzr + VYZ_LINES - 1) / VYZ_LINES)),
+                       dim3(64 * VYZ_LINES), c->lds_vy, c->stream, p, c->vxr_xc, c->vy_cs, c->vy_co);
     tstop(c);
-    if ((rc = check_launch(c, "k_vyr"))) return rc;
-    // vz
-    tstart(c, "mgrd_gradient");
-    hipLaunchKernelGGL(k_mgrd_gradient, gg, kPointBlock, 0, c->stream, p, 2);
-    tstop(c);
-    if ((rc = check_launch(c, "k_mgrd_gradient"))) return rc;
+    if ((rc = check_launch(c, "k_vyz_tile<1>"))) return rc;
     tstart(c, "mgrd_vzr");
-    hipLaunchKernelGGL(k_vzr, dim3(unsigned((c->nxr + 63) / 64), unsigned(c->nyr)), dim3(64), 0, c->stream, p);
+    hipLaunchKernelGGL(k_vyz_tile<2>, dim3(unsigned((c->nxmr + 63) / 64), unsigned((c->nyr + VYZ_LINES - 1) / VYZ_LINES)),
+                       dim3(64 * VYZ_LINES), c->lds_vz, c->stream, p, c->vxr_xc, c->vz_cs, c->vz_co);
     tstop(c);
-    return check_launch(c, "k_vzr");
+    return check_launch(c, "k_vyz_tile<2>");
 }
 int afid_interp_vel_mgrd(afid_ctx *c) { NEED_SAL(c) return interp_vel(c); }
 
@@ -868,10 +881,65 @@ int afid_interp_sal_multigrid(afid_ctx *c) {
     NEED_SAL(c)
     KP p = make_kp(c);
     tstart(c, "mgrd_interp_sal");
-    hipLaunchKernelGGL(k_interp_sal, point_grid(c->nx, c->ny, c->nz), kPointBlock, 0, c->stream, p);
+    hipLaunchKernelGGL(k_interp_sal_tile, dim3(unsigned((c->nxm + 63) / 64), unsigned((c->ny + SALC_TJ - 1) / SALC_TJ),
+                                               unsigned((c->nz + SALC_TZ - 1) / SALC_TZ)),
+                       dim3(256), c->lds_salc, c->stream, p, c->salc_xr, c->salc_yr);
     tstop(c);
-    return check_launch(c, "k_interp_sal");
+    return check_launch(c, "k_interp_sal_tile");
 }
+
+}  // extern "C"
+
+// fused column kernels of the step (one sweep per grid, ping-pong buffers)
+template <int CH>
+static void launch_column_base(afid_ctx *c, const KP &p, long long nblk) {
+    BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
+    hipLaunchKernelGGL(k_column_base<CH>, dim3(unsigned((nblk + 7) / 8 * 8)), dim3(256), 0, c->stream, p,
+                       factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
+}
+static int column_base(afid_ctx *c) {
+    KP p = make_kp(c);
+    const long long nblk = ((long long)c->ny * c->nz + 3) / 4;
+    const int ch = (c->nx + 63) / 64;
+    switch (ch) {
+#define AFID_CB_CASE(CH) case CH: launch_column_base<CH>(c, p, nblk); break;
+        AFID_CB_CASE(1) AFID_CB_CASE(2) AFID_CB_CASE(3) AFID_CB_CASE(4) AFID_CB_CASE(5) AFID_CB_CASE(6)
+        AFID_CB_CASE(7) AFID_CB_CASE(8) AFID_CB_CASE(9) AFID_CB_CASE(10) AFID_CB_CASE(11) AFID_CB_CASE(12)
+        AFID_CB_CASE(13) AFID_CB_CASE(14) AFID_CB_CASE(15) AFID_CB_CASE(16)
+#undef AFID_CB_CASE
+    default: return set_err(c, AFID_E_ARG, "nx too large");
+    }
+    int rc = check_launch(c, "k_column_base");
+    if (rc) return rc;
+    std::swap(c->f[AFID_VX], c->alt[0]);
+    std::swap(c->f[AFID_VY], c->alt[1]);
+    std::swap(c->f[AFID_VZ], c->alt[2]);
+    std::swap(c->f[AFID_TEMP], c->alt[3]);
+    return AFID_OK;
+}
+template <int CH>
+static void launch_column_sal(afid_ctx *c, const KP &p, long long nblk) {
+    hipLaunchKernelGGL(k_column_sal<CH>, dim3(unsigned((nblk + 7) / 8 * 8)), dim3(256), 0, c->stream, p, factors(c, MAT_S), c->sal_alt);
+}
+static int column_sal(afid_ctx *c) {
+    KP p = make_kp(c);
+    const long long nblk = ((long long)c->nyr * c->nzr + 3) / 4;
+    const int ch = (c->nxmr + 63) / 64;
+    switch (ch) {
+#define AFID_CS_CASE(CH) case CH: launch_column_sal<CH>(c, p, nblk); break;
+        AFID_CS_CASE(1) AFID_CS_CASE(2) AFID_CS_CASE(3) AFID_CS_CASE(4) AFID_CS_CASE(5) AFID_CS_CASE(6)
+        AFID_CS_CASE(7) AFID_CS_CASE(8) AFID_CS_CASE(9) AFID_CS_CASE(10) AFID_CS_CASE(11) AFID_CS_CASE(12)
+        AFID_CS_CASE(13) AFID_CS_CASE(14) AFID_CS_CASE(15) AFID_CS_CASE(16)
+#undef AFID_CS_CASE
+    default: return set_err(c, AFID_E_ARG, "nxmr too large");
+    }
+    int rc = check_launch(c, "k_column_sal");
+    if (rc) return rc;
+    std::swap(c->f[AFID_SAL], c->sal_alt);
+    return AFID_OK;
+}
+
+extern "C" {
 
 // TimeMarcher.F90:13-165
 int afid_time_marcher(afid_ctx *c, double dt) {
@@ -881,26 +949,12 @@ int afid_time_marcher(afid_ctx *c, double dt) {
     int rc;
     for (int ns = 1; ns <= c->c.nsst; ++ns) {
         if ((rc = afid_set_substep(c, dt, ns))) return rc;
-        KP p = make_kp(c);
-        tstart(c, "explicit_rhs");
-        hipLaunchKernelGGL(k_explicit_rhs_all, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p);
-        tstop(c);
-        if ((rc = check_launch(c, "k_explicit_rhs_all"))) return rc;
-        if (c->sal) {
-            tstart(c, "explicit_rhs_sal");
-            hipLaunchKernelGGL(k_explicit_rhs_sal, point_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p);
-            tstop(c);
-            if ((rc = check_launch(c, "k_explicit_rhs_sal"))) return rc;
-        }
-        tstart(c, "tridiag_base");
-        if ((rc = run_tri(c, MAT_X, c->rhs4[0], c->f[AFID_VX], 0))) return rc;
-        if ((rc = run_tri(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY], 0))) return rc;
-        if ((rc = run_tri(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ], 0))) return rc;
-        if ((rc = run_tri(c, MAT_T, c->rhs4[3], c->f[AFID_TEMP], 0))) return rc;
+        tstart(c, "column_base");
+        if ((rc = column_base(c))) return rc;
         tstop(c);
         if (c->sal) {
-            tstart(c, "tridiag_sal");
-            if ((rc = run_tri(c, MAT_S, c->f[AFID_RHSR], c->f[AFID_SAL], 0))) return rc;
+            tstart(c, "column_sal");
+            if ((rc = column_sal(c))) return rc;
             tstop(c);
         }
         if ((rc = poisson(c, 0))) return rc;
@@ -956,11 +1010,11 @@ int afid_calc_max_cfl(afid_ctx *c, double *cflm, double *cflmr) {
     if (!c || !cflm || !cflmr) return AFID_E_ARG;
     HIPCHK(c, hipMemsetAsync(c->redmax, 0, sizeof(unsigned long long) * 4, c->stream));
     KP p = make_kp(c);
-    hipLaunchKernelGGL(k_cfl_div, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, 0);
+    hipLaunchKernelGGL(k_cfl_div, red_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, 0);
     int rc = check_launch(c, "k_cfl_div");
     if (rc) return rc;
     if (c->sal) {
-        hipLaunchKernelGGL(k_cfl_div_r, point_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 0);
+        hipLaunchKernelGGL(k_cfl_div_r, red_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 0);
         if ((rc = check_launch(c, "k_cfl_div_r"))) return rc;
     }
     if ((rc = read_max(c, cflm, cflmr))) return rc;
@@ -973,11 +1027,11 @@ int afid_check_divergence(afid_ctx *c, double *qmax, double *qmaxr) {
     if (!c || !qmax || !qmaxr) return AFID_E_ARG;
     HIPCHK(c, hipMemsetAsync(c->redmax, 0, sizeof(unsigned long long) * 4, c->stream));
     KP p = make_kp(c);
-    hipLaunchKernelGGL(k_cfl_div, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, 1);
+    hipLaunchKernelGGL(k_cfl_div, red_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, 1);
     int rc = check_launch(c, "k_cfl_div");
     if (rc) return rc;
     if (c->sal) {
-        hipLaunchKernelGGL(k_cfl_div_r, point_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 1);
+        hipLaunchKernelGGL(k_cfl_div_r, red_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 1);
         if ((rc = check_launch(c, "k_cfl_div_r"))) return rc;
     }
     unsigned long long h[4];

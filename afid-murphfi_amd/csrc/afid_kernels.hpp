@@ -27,23 +27,28 @@ namespace afid {
 __global__ void __launch_bounds__(256) k_explicit_vx(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
     if (k == 0) return;   // qcap(1) is not computed (ExplicitTermsVX.F90:50)
-    p.qcap[bidx(p, k, j, i)] = ex_vx(p, k, j, jm, jp, i, im, ip);
+    const Nb n = make_nb(p, j, i);
+    p.qcap[bidx(p, k, j, i)] = ex_vx(p, n, k);
 }
 __global__ void __launch_bounds__(256) k_explicit_vy(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
-    p.dph[bidx(p, k, j, i)] = ex_vy(p, k, j, jm, jp, i, im, ip);
+    const Nb n = make_nb(p, j, i);
+    p.dph[bidx(p, k, j, i)] = ex_vy(p, n, k);
 }
 __global__ void __launch_bounds__(256) k_explicit_vz(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
-    p.dq[bidx(p, k, j, i)] = ex_vz(p, k, j, jm, jp, i, im, ip);
+    const Nb n = make_nb(p, j, i);
+    p.dq[bidx(p, k, j, i)] = ex_vz(p, n, k);
 }
 __global__ void __launch_bounds__(256) k_explicit_temp(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
-    p.hro[bidx(p, k, j, i)] = ex_temp(p, k, j, jm, jp, i, im, ip);
+    const Nb n = make_nb(p, j, i);
+    p.hro[bidx(p, k, j, i)] = ex_temp(p, n, k);
 }
 __global__ void __launch_bounds__(256) k_explicit_sal(KP p) {
     AFID_POINT_PROLOGUE(p.nxmr, p.nyr, p.nzr)
-    p.hsal[ridx(p, k, j, i)] = ex_sal(p, k, j, jm, jp, i, im, ip);
+    const NbS n = make_nbs(p, j, i);
+    p.hsal[ridx(p, k, j, i)] = ex_sal(p, n, k);
 }
 // AddSalBuoyancy, salinity.F90:495-506
 __global__ void __launch_bounds__(256) k_add_sal_buoyancy(KP p, double *rkv) {
@@ -60,75 +65,40 @@ __global__ void __launch_bounds__(256) k_rhs_vx(KP p) {
     if (k == 0) return;
     const long long c = bidx(p, k, j, i);
     const double h = p.qcap[c];
-    p.rhsx[c] = rhs_vx(p, h, p.rux[c], k, j, i);
+    const Nb n = make_nb(p, j, i);
+    p.rhsx[c] = rhs_vx(p, n, h, p.rux[c], k);
     p.rux[c] = h;
 }
 __global__ void __launch_bounds__(256) k_rhs_vy(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
     const long long c = bidx(p, k, j, i);
     const double h = p.dph[c];
-    p.rhsy[c] = rhs_vy(p, h, p.ruy[c], k, j, jm, i);
+    const Nb n = make_nb(p, j, i);
+    p.rhsy[c] = rhs_vy(p, n, h, p.ruy[c], k);
     p.ruy[c] = h;
 }
 __global__ void __launch_bounds__(256) k_rhs_vz(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
     const long long c = bidx(p, k, j, i);
     const double h = p.dq[c];
-    p.rhsz[c] = rhs_vz(p, h, p.ruz[c], k, j, i, im);
+    const Nb n = make_nb(p, j, i);
+    p.rhsz[c] = rhs_vz(p, n, h, p.ruz[c], k);
     p.ruz[c] = h;
 }
 __global__ void __launch_bounds__(256) k_rhs_temp(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
     const long long c = bidx(p, k, j, i);
     const double h = p.hro[c];
-    p.rhst[c] = rhs_temp(p, h, p.rut[c], k, j, i);
+    const Nb n = make_nb(p, j, i);
+    p.rhst[c] = rhs_temp(p, n, h, p.rut[c], k);
     p.rut[c] = h;
 }
 __global__ void __launch_bounds__(256) k_rhs_sal(KP p) {
     AFID_POINT_PROLOGUE(p.nxmr, p.nyr, p.nzr)
     const long long c = ridx(p, k, j, i);
     const double h = p.hsal[c];
-    p.rhsr[c] = rhs_sal(p, h, p.rusal[c], k, j, i);
-    p.rusal[c] = h;
-}
-
-// Fused explicit terms + implicit RHS of the four base fields (the
-// ExplicitTerms* and ImplicitAndUpdate* loops of one substep read only
-// pre-substep fields, so they can run in one sweep); AddSalBuoyancy folded in.
-__global__ void __launch_bounds__(256) k_explicit_rhs_all(KP p) {
-    AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
-    const long long c = bidx(p, k, j, i);
-    const double sb = (p.active_s == 1) ? p.bycs * p.salc[c] : 0.0;
-    if (k > 0) {
-        double h = ex_vx(p, k, j, jm, jp, i, im, ip);
-        if (p.gaxis == 1) h -= sb;
-        p.rhsx[c] = rhs_vx(p, h, p.rux[c], k, j, i);
-        p.rux[c] = h;
-    }
-    {
-        double h = ex_vy(p, k, j, jm, jp, i, im, ip);
-        if (p.gaxis == 2) h -= sb;
-        p.rhsy[c] = rhs_vy(p, h, p.ruy[c], k, j, jm, i);
-        p.ruy[c] = h;
-    }
-    {
-        double h = ex_vz(p, k, j, jm, jp, i, im, ip);
-        if (p.gaxis == 3) h -= sb;
-        p.rhsz[c] = rhs_vz(p, h, p.ruz[c], k, j, i, im);
-        p.ruz[c] = h;
-    }
-    {
-        double h = ex_temp(p, k, j, jm, jp, i, im, ip);
-        p.rhst[c] = rhs_temp(p, h, p.rut[c], k, j, i);
-        p.rut[c] = h;
-    }
-}
-// Fused ExplicitSalinity + ImplicitSalinity RHS
-__global__ void __launch_bounds__(256) k_explicit_rhs_sal(KP p) {
-    AFID_POINT_PROLOGUE(p.nxmr, p.nyr, p.nzr)
-    const long long c = ridx(p, k, j, i);
-    double h = ex_sal(p, k, j, jm, jp, i, im, ip);
-    p.rhsr[c] = rhs_sal(p, h, p.rusal[c], k, j, i);
+    const NbS n = make_nbs(p, j, i);
+    p.rhsr[c] = rhs_sal(p, n, h, p.rusal[c], k);
     p.rusal[c] = h;
 }
 
@@ -154,18 +124,13 @@ __device__ __forceinline__ void wave_affine_scan(double &A, double &B) {
 template <int CH>
 __device__ __forceinline__ void wave_tri_solve(double (&v)[CH], const TriFactors &f) {
     const int lane = threadIdx.x & 63;
-    const int r0 = lane * CH;
-    double l[CH], e[CH], g[CH];
+    const double *l = f.l + lane * CH, *e = f.e + lane * CH, *g = f.g + lane * CH, *sc = f.s + lane * CH;
 #pragma unroll
-    for (int m = 0; m < CH; ++m) {
-        const int r = r0 + m;
-        l[m] = f.l[r]; e[m] = f.e[r]; g[m] = f.g[r];
-        v[m] *= f.s[r];
-    }
+    for (int m = 0; m < CH; ++m) v[m] *= sc[m];
     // forward: y_r = v_r - l_r y_{r-1}
     double A = 1.0, B = 0.0;
 #pragma unroll
-    for (int m = 0; m < CH; ++m) { B = -l[m] * B + v[m]; A = -l[m] * A; }
+    for (int m = 0; m < CH; ++m) { const double lm = l[m]; B = -lm * B + v[m]; A = -lm * A; }
     wave_affine_scan<true>(A, B);
     double y = __shfl_up(B, 1, 64);
     if (lane == 0) y = 0.0;
@@ -174,7 +139,7 @@ __device__ __forceinline__ void wave_tri_solve(double (&v)[CH], const TriFactors
     // backward: x_r = g_r y_r - e_r x_{r+1}
     A = 1.0; B = 0.0;
 #pragma unroll
-    for (int m = CH - 1; m >= 0; --m) { B = -e[m] * B + g[m] * v[m]; A = -e[m] * A; }
+    for (int m = CH - 1; m >= 0; --m) { const double em = e[m]; B = -em * B + g[m] * v[m]; A = -em * A; }
     wave_affine_scan<false>(A, B);
     double x = __shfl_down(B, 1, 64);
     if (lane == 63) x = 0.0;
@@ -211,6 +176,179 @@ __global__ void __launch_bounds__(256) k_tri_update(TriFactors f, double *rhs, d
             if (r >= u0 && r < u1) q[base + r] += v[m];
             if (r == zero0 || r == zero1) q[base + r] = 0.0;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused column kernels: explicit terms + implicit RHS + tridiagonal solve +
+// update of one wall-normal column per wavefront.  Stencil phase: lane l
+// handles rows l, l+64, ... (coalesced loads of the column and its y/z
+// neighbours); the RHS goes through LDS to the chunked layout of the scan
+// solver and back.  Fields are read from the pre-substep arrays and the
+// updated column is written to a second buffer (other wavefronts still read
+// the old values as neighbours).
+// ---------------------------------------------------------------------------
+// Consecutive workgroups are dealt round-robin to the 8 XCDs; give each XCD
+// a contiguous range of columns so y/z neighbours share its L2.
+// The grid has a multiple of 8 workgroups (padded by the host); the map is a
+// bijection of [0, gridDim.x) and indices >= the real count are idle.
+__device__ __forceinline__ long long xcd_block(long long) {
+    const long long b = blockIdx.x;
+    const long long per = gridDim.x / 8;
+    return (b % 8) * per + b / 8;
+}
+
+// Row r of a column lives at lds_row<CH>(r): chunks of CH rows padded to
+// CH+1 so the chunked reads of the solver hit distinct banks.
+template <int CH>
+__device__ __forceinline__ unsigned lds_row(unsigned r) { return (r / CH) * (CH + 1) + r % CH; }
+template <int CH>
+constexpr int lds_col_size() { return 64 * (CH + 1); }
+
+template <int CH>
+__device__ __forceinline__ void column_solve_lds(double *buf, const TriFactors &f) {
+    const int lane = threadIdx.x & 63;
+    double v[CH];
+#pragma unroll
+    for (int m = 0; m < CH; ++m) v[m] = buf[lane * (CH + 1) + m];
+    wave_tri_solve<CH>(v, f);
+#pragma unroll
+    for (int m = 0; m < CH; ++m) buf[lane * (CH + 1) + m] = v[m];
+}
+
+// ExplicitSalinity + ImplicitSalinity + SolveImpEqnUpdate_Sal (salinity.F90:268-454)
+template <int CH>
+__global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, double *sal_out) {
+    __shared__ double lds[4][lds_col_size<CH>()];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned lane = threadIdx.x & 63;
+    const long long ncol = (long long)p.nyr * p.nzr, nblk = (ncol + 3) / 4;
+    const long long blk = xcd_block(nblk);
+    const long long col = blk * 4 + wave;
+    const bool live = blk < nblk && col < ncol;
+    const int j = live ? int(col % p.nyr) : 0, i = live ? int(col / p.nyr) : 0;
+    double *buf = lds[wave];
+    const NbS n = make_nbs(p, j, i);
+    const long long c0 = (long long)i * p.planer + (long long)j * p.ldxr;
+    double *rus = p.rusal + c0;
+    double own[CH];
+#pragma unroll
+    for (int m = 0; m < CH; ++m) {
+        const unsigned r = lane + 64 * m;
+        double rhs = 0.0;
+        own[m] = 0.0;
+        if (live && r < unsigned(p.nxmr)) {
+            const double h = ex_sal(p, n, r);
+            rhs = rhs_sal(p, n, h, rus[r], r);
+            rus[r] = h;
+            own[m] = n.s[r];
+        }
+        buf[lds_row<CH>(r)] = rhs;
+    }
+    __syncthreads();
+    column_solve_lds<CH>(buf, f);
+    __syncthreads();
+    if (!live) return;
+    double *out = sal_out + c0;
+#pragma unroll
+    for (int m = 0; m < CH; ++m) {
+        const unsigned r = lane + 64 * m;
+        if (r < unsigned(p.nxmr)) out[r] = own[m] + buf[lds_row<CH>(r)];
+        else if (r == unsigned(p.nxmr)) out[r] = n.s[r];
+    }
+    if (64 * CH == p.nxmr && lane == 0) out[p.nxmr] = n.s[p.nxmr];
+}
+
+// Explicit terms of vx, vy, vz, T (+ AddSalBuoyancy) and ImplicitAndUpdate
+// VX/VY/VZ/Temp with their solves (TimeMarcher.F90:45-95) for one column.
+struct BaseOut { double *vx, *vy, *vz, *temp; };
+
+// stencil phase of field FLD: RHS of each row into buf, old value into own
+template <int FLD, int CH>
+__device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long long c0, double *buf, double *own,
+                                                 unsigned lane, unsigned nrow, bool live) {
+    const double *salc = p.salc + c0;
+    double *ru = (FLD == 0 ? p.rux : FLD == 1 ? p.ruy : FLD == 2 ? p.ruz : p.rut) + c0;
+    const unsigned nxm = p.nxm;
+#pragma unroll 1
+    for (unsigned r = lane; r < nrow; r += 64) {
+        double rhs = 0.0, old = 0.0;
+        if (live && r < unsigned(p.nx)) {
+            const double sb = (p.active_s == 1 && r < nxm) ? p.bycs * salc[r] : 0.0;
+            if (FLD == 0) {
+                old = n.vx[r];
+                if (r >= 1 && r < nxm) {
+                    double h = ex_vx(p, n, r);
+                    if (p.gaxis == 1) h -= sb;
+                    rhs = rhs_vx(p, n, h, ru[r], r);
+                    ru[r] = h;
+                }
+            } else if (FLD == 1) {
+                old = n.vy[r];
+                if (r < nxm) {
+                    double h = ex_vy(p, n, r);
+                    if (p.gaxis == 2) h -= sb;
+                    rhs = rhs_vy(p, n, h, ru[r], r);
+                    ru[r] = h;
+                }
+            } else if (FLD == 2) {
+                old = n.vz[r];
+                if (r < nxm) {
+                    double h = ex_vz(p, n, r);
+                    if (p.gaxis == 3) h -= sb;
+                    rhs = rhs_vz(p, n, h, ru[r], r);
+                    ru[r] = h;
+                }
+            } else {
+                old = n.t[r];
+                if (r < nxm) {
+                    double h = ex_temp(p, n, r);
+                    rhs = rhs_temp(p, n, h, ru[r], r);
+                    ru[r] = h;
+                }
+            }
+        }
+        buf[lds_row<CH>(r)] = rhs;
+        own[lds_row<CH>(r)] = old;
+    }
+}
+
+template <int CH>
+__global__ void __launch_bounds__(256) k_column_base(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
+    __shared__ double lds[2][4][lds_col_size<CH>()];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned lane = threadIdx.x & 63;
+    const long long ncol = (long long)p.ny * p.nz, nblk = (ncol + 3) / 4;
+    const long long blk = xcd_block(nblk);
+    const long long col = blk * 4 + wave;
+    const bool live = blk < nblk && col < ncol;
+    const int j = live ? int(col % p.ny) : 0, i = live ? int(col / p.ny) : 0;
+    double *buf = lds[0][wave], *own = lds[1][wave];
+    const Nb n = make_nb(p, j, i);
+    const long long c0 = (long long)i * p.plane + (long long)j * p.ldx;
+    const unsigned nrow = 64 * CH, nx = p.nx, nxm = p.nxm;
+#pragma unroll 1
+    for (int fld = 0; fld < 4; ++fld) {
+        switch (fld) {
+        case 0: column_base_rows<0, CH>(p, n, c0, buf, own, lane, nrow, live); break;
+        case 1: column_base_rows<1, CH>(p, n, c0, buf, own, lane, nrow, live); break;
+        case 2: column_base_rows<2, CH>(p, n, c0, buf, own, lane, nrow, live); break;
+        default: column_base_rows<3, CH>(p, n, c0, buf, own, lane, nrow, live); break;
+        }
+        __syncthreads();
+        column_solve_lds<CH>(buf, fld == 0 ? fx : (fld == 3 ? ft : fyz));
+        __syncthreads();
+        if (live) {
+            double *out = (fld == 0 ? o.vx : fld == 1 ? o.vy : fld == 2 ? o.vz : o.temp) + c0;
+            for (unsigned r = lane; r < nx; r += 64) {
+                const unsigned q = lds_row<CH>(r);
+                double v = own[q];
+                if (fld == 0) v = (r >= 1 && r < nxm) ? v + buf[q] : 0.0;   // walls, ImplicitAndUpdateVX.F90:86-87
+                else if (r < nxm) v += buf[q];
+                out[r] = v;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -252,249 +390,276 @@ __global__ void __launch_bounds__(256) k_project(KP p, int do_vel, int do_pr) {
 // ---------------------------------------------------------------------------
 // multiresolution transfers
 // ---------------------------------------------------------------------------
-// Gradient of one velocity component at coarse cell centres with the wall
-// ghosts of InterpVelMgrd.F90:37-51 (vx), :75-86 (vy), :161-172 (vz).
-// tpdv x index kc = -1..nx+1 is stored at kc+1; -1 and nx+1 are zero.
-__global__ void __launch_bounds__(256) k_mgrd_gradient(KP p, int comp) {
-    const int kk = blockIdx.x * 64 + threadIdx.x;   // storage index 0..nx+2
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int i = blockIdx.z;
-    if (kk > p.nx + 2 || j >= p.ny) return;
-    const int kc = kk - 1;                          // reference index
-    auto grad = [&](int kc1) -> double {            // kc1 in 1..nxm
-        const int k = kc1 - 1;
-        if (comp == 0) return (p.vx[bidx(p, k + 1, j, i)] - p.vx[bidx(p, k, j, i)]) * p.udx3m[k];
-        if (comp == 1) return (p.vy[bidx(p, k, wrapp(j, p.ny), i)] - p.vy[bidx(p, k, j, i)]) * p.dy;
-        return (p.vz[bidx(p, k, j, wrapp(i, p.nz))] - p.vz[bidx(p, k, j, i)]) * p.dz;
-    };
-    double v;
-    if (kc >= 1 && kc <= p.nxm) v = grad(kc);
-    else if (kc == 0) v = (1 - 2 * p.inslws) * grad(1);
-    else if (kc == p.nx) v = (1 - 2 * p.inslwn) * grad(p.nxm);
-    else v = 0.0;
-    p.tpdv[(long long)i * p.ny * p.ldt + (long long)j * p.ldt + kk] = v;
-}
-
-__device__ __forceinline__ double tpdv_at(const KP &p, int kc, int j, int i) {
-    return p.tpdv[(long long)i * p.ny * p.ldt + (long long)j * p.ldt + (kc + 1)];
+// tiled transfers of InterpVelMgrd: the coarse gradient is computed on the
+// fly from the velocity (InterpVelMgrd.F90:37-51, :75-86, :161-172, with the
+// wall ghosts), a coarse block is staged in LDS once per tile, contracted
+// axis by axis (separable Hermite stencil), and the transferred gradient is
+// integrated along its axis with the running sum carried across chunks.
+// ---------------------------------------------------------------------------
+// coarse gradient of component comp at reference x index xc (-1..nx+1) and
+// 0-based periodic (j, i)
+__device__ __forceinline__ double mgrd_grad(const KP &p, int comp, int xc, int j, int i) {
+    int kc = xc, sgn = 1;
+    if (xc == 0) { kc = 1; sgn = 1 - 2 * p.inslws; }
+    else if (xc == p.nx) { kc = p.nxm; sgn = 1 - 2 * p.inslwn; }
+    else if (xc < 0 || xc > p.nx) return 0.0;
+    const int k = kc - 1;
+    double d;
+    if (comp == 0) {
+        const double *c = p.vx + (long long)i * p.plane + (long long)j * p.ldx;
+        d = (c[k + 1] - c[k]) * p.udx3m[k];
+    } else if (comp == 1) {
+        const double *c = p.vy + (long long)i * p.plane;
+        d = (c[(long long)wrapp(j, p.ny) * p.ldx + k] - c[(long long)j * p.ldx + k]) * p.dy;
+    } else {
+        const double *c = p.vz + (long long)j * p.ldx;
+        d = (c[(long long)wrapp(i, p.nz) * p.plane + k] - c[(long long)i * p.plane + k]) * p.dz;
+    }
+    return sgn * d;
 }
 __device__ __forceinline__ int wrapn(int j, int n) { j %= n; return j < 0 ? j + n : j; }
 
-// vxr: Hermite transfer of dvx/dx (interpolate_xyz_to_refined,
-// HermiteInterpolations.F90:182-214) integrated along x (InterpVelMgrd.F90:58-66).
-// One wavefront per refined column (jr, ir); coarse-x partial sums staged in LDS.
-template <int CH>
-__global__ void __launch_bounds__(256) k_vxr(KP p) {
-    extern __shared__ double lds[];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long long col = (long long)blockIdx.x * 4 + wave;
-    const long long ncol = (long long)p.nyr * p.nzr;
-    const int nq = p.nx + 3;                       // coarse x = -1..nx+1
-    double *Q = lds + wave * nq;
-    const bool live = col < ncol;
-    const int jr = live ? int(col % p.nyr) : 0, ir = live ? int(col / p.nyr) : 0;
-    if (live) {
-        const int js = p.ys_src[jr], is = p.zs_src[ir];
-        const double *wy = p.ys_w + 4 * jr, *wz = p.zs_w + 4 * ir;
-        int jj[4], ii[4];
-        for (int b = 0; b < 4; ++b) { jj[b] = wrapn(js - 1 + b, p.ny); ii[b] = wrapn(is - 1 + b, p.nz); }
-        for (int q = lane; q < nq; q += 64) {
-            const int kc = q - 1;
-            double acc = 0.0;
-            for (int b = 0; b < 4; ++b) {
-                double s2 = tpdv_at(p, kc, jj[b], ii[0]) * wz[0] + tpdv_at(p, kc, jj[b], ii[1]) * wz[1]
-                          + tpdv_at(p, kc, jj[b], ii[2]) * wz[2] + tpdv_at(p, kc, jj[b], ii[3]) * wz[3];
-                acc = b == 0 ? s2 * wy[0] : acc + s2 * wy[b];
+// Tile sizes of the transfer kernels (host computes the LDS each needs).
+constexpr int VXR_TJ = 16, VXR_TI = 4;      // refined columns per vxr tile
+constexpr int VYZ_LINES = 4;                // refined lines (waves) per vyr/vzr tile
+constexpr int VYZ_CHUNK = 16;               // refined points per scan chunk
+
+// vxr: tile of VXR_TJ x VXR_TI refined columns, x processed in chunks of 64
+// refined points (one per lane); xcmax/cjmax/cimax bound the coarse block.
+__global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, int cimax) {
+    extern __shared__ double sm[];
+    const int J0 = blockIdx.x * VXR_TJ, I0 = blockIdx.y * VXR_TI;
+    const int nj = min(VXR_TJ, p.nyr - J0), ni = min(VXR_TI, p.nzr - I0);
+    const int cj0 = p.ys_src[J0] - 1, CJ = p.ys_src[J0 + nj - 1] - 1 + 4 - cj0;
+    const int ci0 = p.zs_src[I0] - 1, CI = p.zs_src[I0 + ni - 1] - 1 + 4 - ci0;
+    double *B = sm;                                   // [ci][cj][xc]
+    double *Z = B + xcmax * cjmax * cimax;            // [ti][cj][xc]
+    double *Q = Z + xcmax * cjmax * VXR_TI;           // [ti][tj][xc]
+    double *carry = Q + xcmax * VXR_TJ * VXR_TI;      // [tj*ti]
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int ncol = nj * ni;
+    for (int q = t; q < ncol; q += 256) {
+        carry[q] = 0.0;
+        p.vxr[ridx(p, 0, J0 + q % nj, I0 + q / nj)] = 0.0;      // vxr(1) = 0
+    }
+    for (int R0 = 0; R0 < p.nxmr; R0 += 64) {
+        const int R1 = min(R0 + 64, p.nxmr);
+        const int x0 = p.xs_src[R0], XC = p.xs_src[R1 - 1] + 4 - x0;
+        __syncthreads();
+        for (int q = t; q < XC * CJ * CI; q += 256) {
+            const int xc = q % XC, cj = (q / XC) % CJ, ci = q / (XC * CJ);
+            B[(ci * cjmax + cj) * xcmax + xc] = mgrd_grad(p, 0, x0 + xc, wrapn(cj0 + cj, p.ny), wrapn(ci0 + ci, p.nz));
+        }
+        __syncthreads();
+        for (int q = t; q < XC * CJ * ni; q += 256) {
+            const int xc = q % XC, cj = (q / XC) % CJ, ti = q / (XC * CJ);
+            const int c0 = p.zs_src[I0 + ti] - 1 - ci0;
+            const double *w = p.zs_w + 4 * (I0 + ti), *b = B + (c0 * cjmax + cj) * xcmax + xc;
+            const int st = cjmax * xcmax;
+            Z[(ti * cjmax + cj) * xcmax + xc] = b[0] * w[0] + b[st] * w[1] + b[2 * st] * w[2] + b[3 * st] * w[3];
+        }
+        __syncthreads();
+        for (int q = t; q < XC * nj * ni; q += 256) {
+            const int xc = q % XC, tj = (q / XC) % nj, ti = q / (XC * nj);
+            const int b0 = p.ys_src[J0 + tj] - 1 - cj0;
+            const double *w = p.ys_w + 4 * (J0 + tj), *z = Z + (ti * cjmax + b0) * xcmax + xc;
+            Q[(ti * VXR_TJ + tj) * xcmax + xc] = z[0] * w[0] + z[xcmax] * w[1] + z[2 * xcmax] * w[2] + z[3 * xcmax] * w[3];
+        }
+        __syncthreads();
+        const int kr = R0 + lane;
+        int xa = 0;
+        double wx0 = 0, wx1 = 0, wx2 = 0, wx3 = 0, lx = 0;
+        if (kr < R1) {
+            xa = p.xs_src[kr] - x0;
+            wx0 = p.xs_w[4 * kr]; wx1 = p.xs_w[4 * kr + 1]; wx2 = p.xs_w[4 * kr + 2]; wx3 = p.xs_w[4 * kr + 3];
+            lx = p.lxa[kr];
+        }
+        for (int col = wave; col < ncol; col += 4) {
+            const int tj = col % nj, ti = col / nj;
+            double v = 0.0;
+            if (kr < R1) {
+                const double *qq = Q + (ti * VXR_TJ + tj) * xcmax + xa;
+                v = (qq[0] * wx0 + qq[1] * wx1 + qq[2] * wx2 + qq[3] * wx3) * lx;
             }
-            Q[q] = acc;
+            // inclusive prefix over the 64 lanes
+            for (int off = 1; off < 64; off <<= 1) {
+                const double u = __shfl_up(v, off, 64);
+                if (lane >= off) v += u;
+            }
+            const double base = carry[col];
+            if (kr < R1) p.vxr[ridx(p, kr + 1, J0 + tj, I0 + ti)] = base + v;
+            const double tot = __shfl(v, 63, 64);
+            if (lane == 0) carry[col] = base + tot;
         }
-    }
-    __syncthreads();
-    if (!live) return;
-    // refined increments inc[r] = tpdvr(r) * (xcr(r+1) - xcr(r)), r = 0..nxmr-1
-    double inc[CH];
-    double run = 0.0;
-#pragma unroll
-    for (int m = 0; m < CH; ++m) {
-        const int r = lane * CH + m;
-        double v = 0.0;
-        if (r < p.nxmr) {
-            const int xs = p.xs_src[r];          // reference coarse index of the first point
-            const double *w = p.xs_w + 4 * r;
-            const double *q = Q + (xs + 1);
-            v = (q[0] * w[0] + q[1] * w[1] + q[2] * w[2] + q[3] * w[3]) * p.lxa[r];
-        }
-        run += v;
-        inc[m] = run;                            // chunk-local inclusive prefix
-    }
-    double A = 1.0, B = run;                     // scan of sums: maps y -> y + run
-    wave_affine_scan<true>(A, B);
-    double off = __shfl_up(B, 1, 64);
-    if (lane == 0) off = 0.0;
-    const long long base = ridx(p, 0, jr, ir);
-    if (lane == 0) p.vxr[base] = 0.0;            // vxr(1) = 0
-#pragma unroll
-    for (int m = 0; m < CH; ++m) {
-        const int r = lane * CH + m;
-        if (r < p.nxmr) p.vxr[base + r + 1] = off + inc[m];
     }
 }
 
-// vyr (InterpVelMgrd.F90:71-151) and vzr (:157-237): first-guess plane by 2-D
-// Hermite transfer of the coarse plane, then a running sum of the transferred
-// gradient along y (resp. z).  One thread per (refined x, refined z) (resp.
-// (refined x, refined y)); the thread walks the line and keeps the four
-// coarse values its Hermite stencil needs in registers.
-__device__ __forceinline__ double plane_ghost(const KP &p, const double *f, int kc, long long off2, int sS, int sN) {
-    // vyxzc / vzxyc of InterpVelMgrd.F90:99-106 and :185-192: kc = -1..nxm+2
-    if (kc >= 1 && kc <= p.nxm) return f[off2 + (kc - 1)];
-    if (kc == 0) return sS * f[off2 + 0];
-    if (kc == p.nx) return sN * f[off2 + (p.nxm - 1)];
-    return 0.0;
-}
-
-__global__ void __launch_bounds__(64) k_vyr(KP p) {
-    const int kr = blockIdx.x * 64 + threadIdx.x;   // 0..nxr-1
-    const int ir = blockIdx.y;
-    if (kr >= p.nxr) return;
-    double *out = p.vyr + ridx(p, kr, 0, ir);
-    const long long sy = p.ldxr;
-    if (kr == p.nxmr) {                              // vyr(nxr) is never written: stays zero
-        for (int jr = 0; jr < p.nyr; ++jr) out[jr * sy] = 0.0;
-        return;
-    }
-    const int xs = p.xs_src[kr], is = p.zs_src[ir];
-    double wx[4], wz[4];
-    int ii[4];
-    for (int a = 0; a < 4; ++a) { wx[a] = p.xs_w[4 * kr + a]; wz[a] = p.zs_w[4 * ir + a]; ii[a] = wrapn(is - 1 + a, p.nz); }
-    const int sS = 1 - 2 * p.inslws, sN = 1 - 2 * p.inslwn;
-    // first guess at jr = 0 from the coarse plane j = 0 (czvy = czrs, cxvy = cxrs)
+// vyr (AX = 1) / vzr (AX = 2): lanes = 64 refined x, waves = VYZ_LINES lines
+// of the other periodic axis; the scan axis is walked in chunks of VYZ_CHUNK.
+// Per chunk: B = coarse gradient block [xc][cs][co], O = contraction over the
+// other axis [xc][cs][line], X = contraction over x [cs][line][lane], then a
+// running sum along the scan axis in registers.
+template <int AX>
+__global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, int comax) {
+    extern __shared__ double sm[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x;
+    const int nS = AX == 1 ? p.nyr : p.nzr, nO = AX == 1 ? p.nzr : p.nyr;          // refined extents
+    const int cS = AX == 1 ? p.ny : p.nz, cO = AX == 1 ? p.nz : p.ny;              // coarse extents
+    const int *sS = AX == 1 ? p.ys_src : p.zs_src, *sO = AX == 1 ? p.zs_src : p.ys_src;
+    const double *wS = AX == 1 ? p.ys_w : p.zs_w, *wO = AX == 1 ? p.zs_w : p.ys_w;
+    const int R0 = blockIdx.x * 64, R1 = min(R0 + 64, p.nxmr);
+    const int O0 = blockIdx.y * VYZ_LINES, nl = min(VYZ_LINES, nO - O0);
+    const int kr = R0 + lane, line = O0 + wave;
+    const bool live = wave < nl;
+    double *outp = AX == 1 ? p.vyr : p.vzr;
+    const long long sstride = AX == 1 ? (long long)p.ldxr : p.planer;
+    const long long obase = AX == 1 ? (long long)line * p.planer : (long long)line * p.ldxr;
+    // the unused x index nxr of vyr / vzr stays zero (InterpVelMgrd.F90:29-31)
+    if (blockIdx.x == gridDim.x - 1 && lane == 0 && live)
+        for (int s = 0; s < nS; ++s) outp[obase + s * sstride + p.nxmr] = 0.0;
+    const int x0 = p.xs_src[R0], XC = p.xs_src[R1 - 1] + 4 - x0;
+    const int co0 = sO[O0] - 1, CO = sO[O0 + nl - 1] - 1 + 4 - co0;
+    double wx[4] = {0, 0, 0, 0}, wo[4] = {0, 0, 0, 0};
+    int xa = 0, oa = 0;
+    if (kr < R1) { xa = p.xs_src[kr] - x0; for (int a = 0; a < 4; ++a) wx[a] = p.xs_w[4 * kr + a]; }
+    if (live) { oa = sO[line] - 1 - co0; for (int a = 0; a < 4; ++a) wo[a] = wO[4 * line + a]; }
+    // first guess on the plane s = 0 (InterpVelMgrd.F90:93-123 / :179-209):
+    // velocity plane with wall ghosts, contracted over the other axis, then x
     double v = 0.0;
-    for (int a = 0; a < 4; ++a) {
-        const int kc = xs + a;
-        double q1 = 0.0;
-        for (int c = 0; c < 4; ++c) {
-            double f = plane_ghost(p, p.vy, kc, (long long)ii[c] * p.plane, sS, sN);
-            q1 = c == 0 ? f * wz[0] : q1 + f * wz[c];
-        }
-        v = a == 0 ? q1 * wx[0] : v + q1 * wx[a];
-    }
-    out[0] = v;
-    // coarse window: W[b] = sum_a wx[a] sum_c wz[c] tpdv(xs+a, jc, is-1+c)
-    auto coarse = [&](int jc) -> double {
-        const int jw = wrapn(jc, p.ny);
-        double s = 0.0;
+    if (live && kr < R1 && (AX == 1 || p.nz > 1)) {
+        const int sSg = 1 - 2 * p.inslws, sNg = 1 - 2 * p.inslwn;
         for (int a = 0; a < 4; ++a) {
-            const int kc = xs + a;
-            double q = tpdv_at(p, kc, jw, ii[0]) * wz[0] + tpdv_at(p, kc, jw, ii[1]) * wz[1]
-                     + tpdv_at(p, kc, jw, ii[2]) * wz[2] + tpdv_at(p, kc, jw, ii[3]) * wz[3];
-            s = a == 0 ? q * wx[0] : s + q * wx[a];
-        }
-        return s;
-    };
-    int cur = p.ys_src[0] - 1;                    // coarse j of W[0]
-    double W[4];
-    for (int b = 0; b < 4; ++b) W[b] = coarse(cur + b);
-    const double lya = 1.0 / p.dyr;
-    for (int jr = 0; jr < p.nyr - 1; ++jr) {
-        const int want = p.ys_src[jr] - 1;
-        while (cur < want) { W[0] = W[1]; W[1] = W[2]; W[2] = W[3]; W[3] = coarse(cur + 4); ++cur; }
-        const double *wy = p.ys_w + 4 * jr;
-        const double t = W[0] * wy[0] + W[1] * wy[1] + W[2] * wy[2] + W[3] * wy[3];
-        v = v + t * lya;
-        out[(jr + 1) * sy] = v;
-    }
-}
-
-__global__ void __launch_bounds__(64) k_vzr(KP p) {
-    const int kr = blockIdx.x * 64 + threadIdx.x;
-    const int jr = blockIdx.y;
-    if (kr >= p.nxr) return;
-    double *out = p.vzr + ridx(p, kr, jr, 0);
-    const long long sz = p.planer;
-    if (kr == p.nxmr) {
-        for (int ir = 0; ir < p.nzr; ++ir) out[ir * sz] = 0.0;
-        return;
-    }
-    const int xs = p.xs_src[kr], js = p.ys_src[jr];
-    double wx[4], wy[4];
-    int jj[4];
-    for (int a = 0; a < 4; ++a) { wx[a] = p.xs_w[4 * kr + a]; wy[a] = p.ys_w[4 * jr + a]; jj[a] = wrapn(js - 1 + a, p.ny); }
-    const int sS = 1 - 2 * p.inslws, sN = 1 - 2 * p.inslwn;
-    double v = 0.0;
-    if (p.nz > 1) {   // InterpVelMgrd.F90:184 guard ic0 < xend(3)
-        for (int a = 0; a < 4; ++a) {
-            const int kc = xs + a;
+            const int kc = x0 + xa + a;
             double q1 = 0.0;
-            for (int b = 0; b < 4; ++b) {
-                double f = plane_ghost(p, p.vz, kc, (long long)jj[b] * p.ldx, sS, sN);
-                q1 = b == 0 ? f * wy[0] : q1 + f * wy[b];
+            for (int c = 0; c < 4; ++c) {
+                const int o = wrapn(co0 + oa + c, cO);
+                const double *col = AX == 1 ? p.vy + (long long)o * p.plane : p.vz + (long long)o * p.ldx;
+                double f;
+                if (kc >= 1 && kc <= p.nxm) f = col[kc - 1];
+                else if (kc == 0) f = sSg * col[0];
+                else if (kc == p.nx) f = sNg * col[p.nxm - 1];
+                else f = 0.0;
+                q1 = c == 0 ? f * wo[0] : q1 + f * wo[c];
             }
             v = a == 0 ? q1 * wx[0] : v + q1 * wx[a];
         }
     }
-    out[0] = v;
-    auto coarse = [&](int ic) -> double {
-        const int iw = wrapn(ic, p.nz);
-        double s = 0.0;
-        for (int a = 0; a < 4; ++a) {
-            const int kc = xs + a;
-            double q = tpdv_at(p, kc, jj[0], iw) * wy[0] + tpdv_at(p, kc, jj[1], iw) * wy[1]
-                     + tpdv_at(p, kc, jj[2], iw) * wy[2] + tpdv_at(p, kc, jj[3], iw) * wy[3];
-            s = a == 0 ? q * wx[0] : s + q * wx[a];
+    if (live && kr < R1) outp[obase + kr] = v;
+    const double dl = AX == 1 ? 1.0 / p.dyr : 1.0 / p.dzr;
+    double *B = sm;                                           // [co][cs][xc]
+    double *X = B;                                            // [cs][line][lane], reuses B
+    double *O = B + max(xcmax * csmax * comax, csmax * VYZ_LINES * 64);   // [line][cs][xc]
+    for (int S0 = 0; S0 < nS - 1; S0 += VYZ_CHUNK) {
+        const int S1 = min(S0 + VYZ_CHUNK, nS - 1);
+        const int cs0 = sS[S0] - 1, CS = sS[S1 - 1] - 1 + 4 - cs0;
+        __syncthreads();
+        for (int q = t; q < XC * CS * CO; q += 256) {
+            const int xc = q % XC, cs = (q / XC) % CS, co = q / (XC * CS);
+            const int js = wrapn(cs0 + cs, cS), jo = wrapn(co0 + co, cO);
+            B[(co * csmax + cs) * xcmax + xc] = AX == 1 ? mgrd_grad(p, 1, x0 + xc, js, jo) : mgrd_grad(p, 2, x0 + xc, jo, js);
         }
-        return s;
-    };
-    int cur = p.zs_src[0] - 1;
-    double W[4];
-    for (int c = 0; c < 4; ++c) W[c] = coarse(cur + c);
-    const double lza = 1.0 / p.dzr;
-    for (int ir = 0; ir < p.nzr - 1; ++ir) {
-        const int want = p.zs_src[ir] - 1;
-        while (cur < want) { W[0] = W[1]; W[1] = W[2]; W[2] = W[3]; W[3] = coarse(cur + 4); ++cur; }
-        const double *wz = p.zs_w + 4 * ir;
-        const double t = W[0] * wz[0] + W[1] * wz[1] + W[2] * wz[2] + W[3] * wz[3];
-        v = v + t * lza;
-        out[(long long)(ir + 1) * sz] = v;
+        __syncthreads();
+        for (int q = t; q < XC * CS * nl; q += 256) {
+            const int xc = q % XC, cs = (q / XC) % CS, l = q / (XC * CS);
+            const int c0 = sO[O0 + l] - 1 - co0;
+            const double *w = wO + 4 * (O0 + l), *b = B + (c0 * csmax + cs) * xcmax + xc;
+            const int st = csmax * xcmax;
+            O[(l * csmax + cs) * xcmax + xc] = b[0] * w[0] + b[st] * w[1] + b[2 * st] * w[2] + b[3 * st] * w[3];
+        }
+        __syncthreads();
+        if (live && kr < R1)
+            for (int cs = 0; cs < CS; ++cs) {
+                const double *o = O + (wave * csmax + cs) * xcmax + xa;
+                X[(cs * VYZ_LINES + wave) * 64 + lane] = o[0] * wx[0] + o[1] * wx[1] + o[2] * wx[2] + o[3] * wx[3];
+            }
+        __syncthreads();
+        if (live && kr < R1)
+            for (int s = S0; s < S1; ++s) {
+                const int b0 = sS[s] - 1 - cs0;
+                const double *w = wS + 4 * s, *xx = X + (b0 * VYZ_LINES + wave) * 64 + lane;
+                const int st = VYZ_LINES * 64;
+                const double tv = xx[0] * w[0] + xx[st] * w[1] + xx[2 * st] * w[2] + xx[3 * st] * w[3];
+                v = v + tv * dl;
+                outp[obase + (s + 1) * sstride + kr] = v;
+            }
     }
 }
 
 // InterpSalMultigrid (salinity.F90:458-491) through the fast coarse-grid loop
 // of interpolate_xyz_to_coarse_fast (HermiteInterpolations.F90:291-310).
-__global__ void __launch_bounds__(256) k_interp_sal(KP p) {
-    const int k = blockIdx.x * 64 + threadIdx.x;   // 0..nx-1 (salc(nx) = 0)
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k >= p.nx || j >= p.ny) return;
-    if (k == p.nxm) { p.salc[bidx(p, k, j, i)] = 0.0; return; }
-    const int kr0 = p.xb_src[k], jr0 = p.yb_src[j], ir0 = p.zb_src[i];   // reference indices of first point
-    const double *wx = p.xb_w + 4 * k, *wy = p.yb_w + 4 * j, *wz = p.zb_w + 4 * i;
-    int jj[4], ii[4];
-    for (int b = 0; b < 4; ++b) { jj[b] = wrapn(jr0 - 1 + b, p.nyr); ii[b] = wrapn(ir0 - 1 + b, p.nzr); }
-    double res = 0.0;
-    for (int a = 0; a < 4; ++a) {
-        const int kr = kr0 + a;   // tpdvr index -1..nxr+1
-        double q1 = 0.0;
-        for (int b = 0; b < 4; ++b) {
-            double q2 = 0.0;
-            for (int c = 0; c < 4; ++c) {
-                const long long o2 = (long long)ii[c] * p.planer + (long long)jj[b] * p.ldxr;
-                double f;
-                if (kr >= 1 && kr <= p.nxmr) f = p.sal[o2 + kr - 1];
-                else if (kr == 0) {
-                    const double s1 = p.sal[o2];
-                    f = p.sfixs == 1 ? 2.0 * p.salbp[(long long)ii[c] * p.nyr + jj[b]] - s1 : s1;
-                } else if (kr == p.nxr) {
-                    const double sn = p.sal[o2 + p.nxmr - 1];
-                    f = p.sfixn == 1 ? 2.0 * p.saltp[(long long)ii[c] * p.nyr + jj[b]] - sn : sn;
-                } else f = 0.0;
-                q2 = c == 0 ? f * wz[0] : q2 + f * wz[c];
-            }
-            q1 = b == 0 ? q2 * wy[0] : q1 + q2 * wy[b];
-        }
-        res = a == 0 ? q1 * wx[0] : res + q1 * wx[a];
+// Tile: 64 coarse x (lanes) x SALC_TJ coarse y (two rows per wave), marching
+// through SALC_TZ coarse z planes.  Each refined z plane the tile needs is
+// staged once in LDS (with the wall ghosts of salinity.F90:466-482, double
+// buffered), contracted over x and y, and the last four contracted planes are
+// kept in registers for the z contraction.
+constexpr int SALC_TJ = 8, SALC_TZ = 16;
+__device__ __forceinline__ double sal_ghosted(const KP &p, int kr, int jj, int ii) {
+    const long long o2 = (long long)ii * p.planer + (long long)jj * p.ldxr;
+    if (kr >= 1 && kr <= p.nxmr) return p.sal[o2 + kr - 1];
+    if (kr == 0) { const double s1 = p.sal[o2]; return p.sfixs == 1 ? 2.0 * p.salbp[(long long)ii * p.nyr + jj] - s1 : s1; }
+    if (kr == p.nxr) { const double sn = p.sal[o2 + p.nxmr - 1]; return p.sfixn == 1 ? 2.0 * p.saltp[(long long)ii * p.nyr + jj] - sn : sn; }
+    return 0.0;
+}
+__global__ void __launch_bounds__(256) k_interp_sal_tile(KP p, int xrmax, int yrmax) {
+    extern __shared__ double sm[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x;
+    const int K0 = blockIdx.x * 64, K1 = min(K0 + 64, p.nxm);
+    const int J0 = blockIdx.y * SALC_TJ, nj = min(SALC_TJ, p.ny - J0);
+    const int I0 = blockIdx.z * SALC_TZ, I1 = min(I0 + SALC_TZ, p.nz);
+    const int x0 = p.xb_src[K0], XR = p.xb_src[K1 - 1] + 4 - x0;          // reference refined x index
+    const int y0 = p.yb_src[J0] - 1, YR = p.yb_src[J0 + nj - 1] - 1 + 4 - y0; // 0-based refined y
+    const int zfirst = p.zb_src[I0] - 1, zlast = p.zb_src[I1 - 1] - 1 + 3;    // 0-based refined z (unwrapped)
+    const int k = K0 + lane;
+    const bool kok = k < K1;
+    double wx[4] = {0, 0, 0, 0};
+    int xa = 0;
+    if (kok) { xa = p.xb_src[k] - x0; for (int a = 0; a < 4; ++a) wx[a] = p.xb_w[4 * k + a]; }
+    // two coarse rows per wave: tj = wave and wave + 4
+    int ya[2] = {0, 0};
+    double wy[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    bool rok[2];
+    for (int h = 0; h < 2; ++h) {
+        const int tj = wave + 4 * h;
+        rok[h] = tj < nj;
+        if (rok[h]) { ya[h] = p.yb_src[J0 + tj] - 1 - y0; for (int b = 0; b < 4; ++b) wy[h][b] = p.yb_w[4 * (J0 + tj) + b]; }
     }
-    p.salc[bidx(p, k, j, i)] = res;
+    double ring[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};   // contracted refined planes zr-3..zr
+    const int plane = XR * YR;
+    int ic = I0;
+    for (int zr = zfirst; zr <= zlast; ++zr) {
+        double *B = sm + ((zr - zfirst) & 1) * xrmax * yrmax;     // [yr][xr], double buffered
+        const int ii = wrapn(zr, p.nzr);
+        for (int q = t; q < plane; q += 256) {
+            const int xr = q % XR, yr = q / XR;
+            B[yr * xrmax + xr] = sal_ghosted(p, x0 + xr, wrapn(y0 + yr, p.nyr), ii);
+        }
+        __syncthreads();
+        for (int h = 0; h < 2; ++h) {
+            double v = 0.0;
+            if (kok && rok[h]) {
+                for (int b = 0; b < 4; ++b) {
+                    const double *row = B + (ya[h] + b) * xrmax + xa;
+                    const double q1 = row[0] * wx[0] + row[1] * wx[1] + row[2] * wx[2] + row[3] * wx[3];
+                    v = b == 0 ? q1 * wy[h][0] : v + q1 * wy[h][b];
+                }
+            }
+            ring[h][0] = ring[h][1]; ring[h][1] = ring[h][2]; ring[h][2] = ring[h][3]; ring[h][3] = v;
+        }
+        // every coarse plane whose 4-plane stencil ends at zr is complete
+        while (ic < I1 && p.zb_src[ic] - 1 + 3 == zr) {
+            const double *wz = p.zb_w + 4 * ic;
+            for (int h = 0; h < 2; ++h)
+                if (kok && rok[h])
+                    p.salc[bidx(p, k, J0 + wave + 4 * h, ic)] =
+                        ring[h][0] * wz[0] + ring[h][1] * wz[1] + ring[h][2] * wz[2] + ring[h][3] * wz[3];
+            ++ic;
+        }
+    }
+    // salc(nx) stays zero (salinity.F90:463): written by the last x tile
+    if (blockIdx.x == gridDim.x - 1 && lane == 0)
+        for (int h = 0; h < 2; ++h)
+            if (rok[h])
+                for (int iz = I0; iz < I1; ++iz) p.salc[bidx(p, p.nxm, J0 + wave + 4 * h, iz)] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -554,35 +719,43 @@ __device__ __forceinline__ double block_sum(double v) {
 // CalcMaxCFL.F90:24-44 (what=0) / CheckDivergence.F90:24-43 (what=1) on the
 // base grid, slot 0; refined grid versions write slot 1.
 __global__ void __launch_bounds__(256) k_cfl_div(KP p, int what) {
-    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
     double v = 0.0;
+    for (int i = blockIdx.z; i < p.nz; i += gridDim.z) {
     if (k < p.nxm && j < p.ny) {
         const int jp = wrapp(j, p.ny), ip = wrapp(i, p.nz);
+        double w;
         if (what == 0)
-            v = fabs((p.vz[bidx(p, k, j, i)] + p.vz[bidx(p, k, j, ip)]) * 0.5 * p.dz)
+            w = fabs((p.vz[bidx(p, k, j, i)] + p.vz[bidx(p, k, j, ip)]) * 0.5 * p.dz)
               + fabs((p.vy[bidx(p, k, j, i)] + p.vy[bidx(p, k, jp, i)]) * 0.5 * p.dy)
               + fabs((p.vx[bidx(p, k, j, i)] + p.vx[bidx(p, k + 1, j, i)]) * 0.5 * p.udx3m[k]);
         else
-            v = fabs((p.vz[bidx(p, k, j, ip)] - p.vz[bidx(p, k, j, i)]) * p.dz
+            w = fabs((p.vz[bidx(p, k, j, ip)] - p.vz[bidx(p, k, j, i)]) * p.dz
                    + (p.vy[bidx(p, k, jp, i)] - p.vy[bidx(p, k, j, i)]) * p.dy
                    + (p.vx[bidx(p, k + 1, j, i)] - p.vx[bidx(p, k, j, i)]) * p.udx3m[k]);
+        v = fmax(v, w);
+    }
     }
     v = block_max(v);
     if (threadIdx.x == 0 && threadIdx.y == 0) atomicMax(&p.redmax[2 * what], (unsigned long long)__double_as_longlong(v));
 }
 __global__ void __launch_bounds__(256) k_cfl_div_r(KP p, int what) {
-    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
     double v = 0.0;
+    for (int i = blockIdx.z; i < p.nzr; i += gridDim.z) {
     if (k < p.nxmr && j < p.nyr) {
         const int jp = wrapp(j, p.nyr), ip = wrapp(i, p.nzr);
+        double w;
         if (what == 0)
-            v = fabs((p.vzr[ridx(p, k, j, i)] + p.vzr[ridx(p, k, j, ip)]) * 0.5 * p.dzr)
+            w = fabs((p.vzr[ridx(p, k, j, i)] + p.vzr[ridx(p, k, j, ip)]) * 0.5 * p.dzr)
               + fabs((p.vyr[ridx(p, k, j, i)] + p.vyr[ridx(p, k, jp, i)]) * 0.5 * p.dyr)
               + fabs((p.vxr[ridx(p, k, j, i)] + p.vxr[ridx(p, k + 1, j, i)]) * 0.5 * p.udx3mr[k]);
         else
-            v = fabs((p.vzr[ridx(p, k, j, ip)] - p.vzr[ridx(p, k, j, i)]) * p.dzr
+            w = fabs((p.vzr[ridx(p, k, j, ip)] - p.vzr[ridx(p, k, j, i)]) * p.dzr
                    + (p.vyr[ridx(p, k, jp, i)] - p.vyr[ridx(p, k, j, i)]) * p.dyr
                    + (p.vxr[ridx(p, k + 1, j, i)] - p.vxr[ridx(p, k, j, i)]) * p.udx3mr[k]);
+        v = fmax(v, w);
+    }
     }
     v = block_max(v);
     if (threadIdx.x == 0 && threadIdx.y == 0) atomicMax(&p.redmax[2 * what + 1], (unsigned long long)__double_as_longlong(v));

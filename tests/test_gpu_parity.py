@@ -169,8 +169,13 @@ def test_mgrd_setup_tables():
     assert relerr(a[:len(b)], b) < 1e-15
 
 
-def test_interp_vel_mgrd():
-    o, g, _ = make_mr_pair()
+MR_ODD = dict(MR, nxm=12, nym=8, nzm=6, nxmr=30, nymr=20, nzmr=18, istr3=4, str3=1.4, istr3r=6)
+MR_BIG = dict(MR, nxm=40, nym=24, nzm=36, nxmr=80, nymr=72, nzmr=72)
+
+
+@pytest.mark.parametrize("cfg", [MR, MR_ODD, MR_BIG])
+def test_interp_vel_mgrd(cfg):
+    o, g, _ = make_mr_pair(cfg)
     o.call("interp_vel_mgrd"); g.call("interp_vel_mgrd")
     for f in ("vxr", "vyr", "vzr"):
         cmp(o, g, f, tol=1e-12)
@@ -189,8 +194,9 @@ def test_interp_vel_mgrd_refined_divergence():
     assert d < 1e-12 * scale and dr < 1e-11 * scale
 
 
-def test_interp_sal_multigrid():
-    o, g, _ = make_mr_pair()
+@pytest.mark.parametrize("cfg", [MR, MR_ODD, MR_BIG])
+def test_interp_sal_multigrid(cfg):
+    o, g, _ = make_mr_pair(cfg)
     o.call("interp_sal_multigrid"); g.call("interp_sal_multigrid")
     cmp(o, g, "salc", tol=1e-13)
 
@@ -215,9 +221,9 @@ def test_add_sal_buoyancy():
     cmp(o, g, "qcap", np.s_[..., 1:16])
 
 
-@pytest.mark.parametrize("active_s", [0, 1])
-def test_time_marcher_multires(active_s):
-    cfg = dict(MR, active_s=active_s)
+@pytest.mark.parametrize("active_s,base", [(0, "MR"), (1, "MR"), (0, "MR_ODD"), (1, "MR_BIG")])
+def test_time_marcher_multires(active_s, base):
+    cfg = dict({"MR": MR, "MR_ODD": MR_ODD, "MR_BIG": MR_BIG}[base], active_s=active_s)
     o, g, _ = make_mr_pair(cfg)
     for _ in range(2):
         o.time_marcher(1e-3)
@@ -228,3 +234,29 @@ def test_time_marcher_multires(active_s):
         cmp(o, g, f, tol=1e-10)
     a = np.array(g.calc_max_cfl()); b = np.array(o.calc_max_cfl())
     assert relerr(a, b) < 1e-12
+    a = np.array(g.check_divergence()); b = np.array(o.check_divergence())
+    assert np.all(a < 1e-9) and np.all(b < 1e-9)
+
+
+@pytest.mark.parametrize("istr3,str3", [(0, 0.0), (1, 2.0), (2, 0.0), (3, 0.0), (4, 1.4), (6, 3.0), (7, 2.0), (8, 0.0)])
+def test_setup_arrays(istr3, str3):
+    cfg = dict(BASE, nxm=20, istr3=istr3, str3=str3, tfixs=1, tfixn=0, inslws=0, inslwn=1)
+    o, g, _ = make_pair(cfg)
+    for name in ("xc", "xm", "g3rc", "udx3c", "udx3m", "ap3ck", "ac3ck", "am3ck", "ap3sk", "ac3sk", "am3sk",
+                 "ap3ssk", "ac3ssk", "am3ssk", "ak1", "ak2", "amphk", "acphk", "apphk"):
+        a, b = g.grid_array(name), o.grid(name)
+        n = min(len(a), len(b))
+        assert relerr(a[:n], b[:n]) < 1e-14, name
+
+
+@pytest.mark.parametrize("cfg", [dict(BASE, nxm=12, nym=8, nzm=6, istr3=4, str3=1.4),
+                                 dict(BASE, nxm=12, nym=8, nzm=6),
+                                 dict(BASE, nym=8, nzm=6, istr3=4, str3=1.4)])
+def test_time_marcher_shapes(cfg):
+    o, g, _ = make_pair(cfg)
+    for _ in range(2):
+        o.time_marcher(2e-3)
+        g.time_marcher(2e-3)
+    nxm = cfg["nxm"]
+    for f in ("vx", "vy", "vz", "temp", "pr"):
+        cmp(o, g, f, np.s_[..., :nxm] if f != "vx" else np.s_[...], tol=1e-10)
