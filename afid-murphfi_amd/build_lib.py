@@ -22,6 +22,28 @@ def hipcc_cmd(out=LIB, extra=()):
             *extra, "-o", out, SRC]
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+HOST_EXE = os.path.join(HERE, "afid_host")
+
+
+def build_fortran_host(verbose=True):
+    """Fortran host (driver loop + TimeMarcher call tree) over the C ABI."""
+    import tempfile
+    fc = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+    with tempfile.TemporaryDirectory() as tmp:
+        cmd = [fc, "-O2", "-module-dir", tmp, "-o", HOST_EXE + ".tmp",
+               os.path.join(FORTRAN_DIR, "afid_gpu_mod.F90"), os.path.join(FORTRAN_DIR, "afid_host.F90"),
+               "-L" + HERE, "-lafid_gpu", "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("amdflang failed building the Fortran host")
+    os.replace(HOST_EXE + ".tmp", HOST_EXE)
+    return HOST_EXE
+
+
 def build(verbose=True):
     cmd = hipcc_cmd(out=LIB + ".tmp")
     if verbose:
@@ -36,6 +58,8 @@ def build(verbose=True):
         if lines:
             sys.stderr.write("\n".join(lines) + "\n")
     os.replace(LIB + ".tmp", LIB)
+    if os.path.exists("/opt/rocm/bin/amdflang") or os.environ.get("FC"):
+        build_fortran_host(verbose)
     return LIB
 
 

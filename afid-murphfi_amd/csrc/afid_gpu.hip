@@ -899,7 +899,7 @@ static void launch_column_base(afid_ctx *c, const KP &p, long long nblk) {
 }
 static int column_base(afid_ctx *c) {
     KP p = make_kp(c);
-    const long long nblk = ((long long)c->ny * c->nz + 3) / 4;
+    const long long nblk = strip_blocks(c->ny, c->nz);
     const int ch = (c->nx + 63) / 64;
     switch (ch) {
 #define AFID_CB_CASE(CH) case CH: launch_column_base<CH>(c, p, nblk); break;
@@ -923,7 +923,7 @@ static void launch_column_sal(afid_ctx *c, const KP &p, long long nblk) {
 }
 static int column_sal(afid_ctx *c) {
     KP p = make_kp(c);
-    const long long nblk = ((long long)c->nyr * c->nzr + 3) / 4;
+    const long long nblk = strip_blocks(c->nyr, c->nzr);
     const int ch = (c->nxmr + 63) / 64;
     switch (ch) {
 #define AFID_CS_CASE(CH) case CH: launch_column_sal<CH>(c, p, nblk); break;

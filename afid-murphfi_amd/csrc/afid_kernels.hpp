@@ -197,6 +197,23 @@ __device__ __forceinline__ long long xcd_block(long long) {
     const long long per = gridDim.x / 8;
     return (b % 8) * per + b / 8;
 }
+// Column walk: strips of STRIP_BLOCKS*4 columns in y, walked along z, so the
+// y and z neighbour columns a workgroup reads were touched by workgroups that
+// ran just before it on the same XCD (still in its L2).
+constexpr int STRIP_BLOCKS = 4;
+__host__ __device__ __forceinline__ long long strip_blocks(int ny, int nz) {
+    return (long long)((ny + 4 * STRIP_BLOCKS - 1) / (4 * STRIP_BLOCKS)) * STRIP_BLOCKS * nz;
+}
+// (j, i) of this wavefront's column; false if the slot is idle
+__device__ __forceinline__ bool strip_column(long long blk, int wave, int ny, int nz, int &j, int &i) {
+    const long long per_strip = (long long)STRIP_BLOCKS * nz;
+    const long long strip = blk / per_strip, rem = blk % per_strip;
+    i = int(rem / STRIP_BLOCKS);
+    j = int(strip * 4 * STRIP_BLOCKS + (rem % STRIP_BLOCKS) * 4 + wave);
+    const bool ok = blk < strip_blocks(ny, nz) && j < ny;
+    if (!ok) { j = 0; i = 0; }
+    return ok;
+}
 
 // Row r of a column lives at lds_row<CH>(r): chunks of CH rows padded to
 // CH+1 so the chunked reads of the solver hit distinct banks.
@@ -222,11 +239,8 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, double *
     __shared__ double lds[4][lds_col_size<CH>()];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
-    const long long ncol = (long long)p.nyr * p.nzr, nblk = (ncol + 3) / 4;
-    const long long blk = xcd_block(nblk);
-    const long long col = blk * 4 + wave;
-    const bool live = blk < nblk && col < ncol;
-    const int j = live ? int(col % p.nyr) : 0, i = live ? int(col / p.nyr) : 0;
+    int j, i;
+    const bool live = strip_column(xcd_block(0), wave, p.nyr, p.nzr, j, i);
     double *buf = lds[wave];
     const NbS n = make_nbs(p, j, i);
     const long long c0 = (long long)i * p.planer + (long long)j * p.ldxr;
@@ -318,11 +332,8 @@ __global__ void __launch_bounds__(256) k_column_base(KP p, TriFactors fx, TriFac
     __shared__ double lds[2][4][lds_col_size<CH>()];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
-    const long long ncol = (long long)p.ny * p.nz, nblk = (ncol + 3) / 4;
-    const long long blk = xcd_block(nblk);
-    const long long col = blk * 4 + wave;
-    const bool live = blk < nblk && col < ncol;
-    const int j = live ? int(col % p.ny) : 0, i = live ? int(col / p.ny) : 0;
+    int j, i;
+    const bool live = strip_column(xcd_block(0), wave, p.ny, p.nz, j, i);
     double *buf = lds[0][wave], *own = lds[1][wave];
     const Nb n = make_nb(p, j, i);
     const long long c0 = (long long)i * p.plane + (long long)j * p.ldx;

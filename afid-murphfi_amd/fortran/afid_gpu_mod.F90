@@ -1,0 +1,142 @@
+!> iso_c_binding interface of libafid_gpu (include/afid_gpu.h).
+!! Each binding carries the name of the reference routine it stands for.
+module afid_gpu
+    use iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: AFID_FP64 = 0, AFID_MIXED = 1
+    integer(c_int), parameter :: AFID_VX = 0, AFID_VY = 1, AFID_VZ = 2, AFID_TEMP = 3, AFID_PR = 4
+    integer(c_int), parameter :: AFID_SAL = 14, AFID_VXR = 18, AFID_VYR = 19, AFID_VZR = 20, AFID_SALC = 21
+    integer(c_int), parameter :: AFID_TEMPBP = 22, AFID_TEMPTP = 23, AFID_SALBP = 24, AFID_SALTP = 25
+
+    !> the bou.in value lines (ReadInputFile.F90:20-76) + arithmetic form
+    type, bind(C) :: afid_config
+        integer(c_int32_t) :: nxm, nym, nzm, nsst
+        integer(c_int32_t) :: multires, nxmr, nymr, nzmr
+        integer(c_int32_t) :: salinity, phasefield
+        real(c_double)     :: alx3, ylen, zlen
+        integer(c_int32_t) :: istr3
+        real(c_double)     :: str3
+        integer(c_int32_t) :: istr3r
+        real(c_double)     :: rayt, prat, rays, pras
+        integer(c_int32_t) :: ffscales
+        integer(c_int32_t) :: inslws, inslwn, tfixs, tfixn, sfixs, sfixn
+        integer(c_int32_t) :: active_t, active_s, gaxis
+        real(c_double)     :: xplusu, xminusu, dpdy, dpdz
+        integer(c_int32_t) :: precision
+    end type afid_config
+
+    interface
+        integer(c_int) function afid_gpu_init(cfg, ctx) bind(C, name='afid_gpu_init')
+            import; type(afid_config), intent(in) :: cfg; type(c_ptr), intent(out) :: ctx
+        end function
+        integer(c_int) function afid_gpu_finalize(ctx) bind(C, name='afid_gpu_finalize')
+            import; type(c_ptr), value :: ctx
+        end function
+        type(c_ptr) function afid_last_error(ctx) bind(C, name='afid_last_error')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_upload(ctx, field, host, lb, ext) bind(C, name='afid_upload')
+            import; type(c_ptr), value :: ctx; integer(c_int), value :: field
+            real(c_double), intent(in) :: host(*); integer(c_int64_t), intent(in) :: lb(3), ext(3)
+        end function
+        integer(c_int) function afid_download(ctx, field, host, lb, ext) bind(C, name='afid_download')
+            import; type(c_ptr), value :: ctx; integer(c_int), value :: field
+            real(c_double), intent(out) :: host(*); integer(c_int64_t), intent(in) :: lb(3), ext(3)
+        end function
+        !> TimeMarcher.F90:28,38-40
+        integer(c_int) function afid_set_substep(ctx, dt, ns) bind(C, name='afid_set_substep')
+            import; type(c_ptr), value :: ctx; real(c_double), value :: dt; integer(c_int), value :: ns
+        end function
+        integer(c_int) function afid_explicit_terms_vx(ctx) bind(C, name='afid_explicit_terms_vx')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_explicit_terms_vy(ctx) bind(C, name='afid_explicit_terms_vy')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_explicit_terms_vz(ctx) bind(C, name='afid_explicit_terms_vz')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_explicit_terms_temp(ctx) bind(C, name='afid_explicit_terms_temp')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_explicit_salinity(ctx) bind(C, name='afid_explicit_salinity')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_add_sal_buoyancy(ctx) bind(C, name='afid_add_sal_buoyancy')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_implicit_and_update_vx(ctx) bind(C, name='afid_implicit_and_update_vx')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_implicit_and_update_vy(ctx) bind(C, name='afid_implicit_and_update_vy')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_implicit_and_update_vz(ctx) bind(C, name='afid_implicit_and_update_vz')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_implicit_and_update_temp(ctx) bind(C, name='afid_implicit_and_update_temp')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_implicit_salinity(ctx) bind(C, name='afid_implicit_salinity')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_calc_local_divergence(ctx) bind(C, name='afid_calc_local_divergence')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_solve_pressure_correction(ctx) bind(C, name='afid_solve_pressure_correction')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_correct_velocity(ctx) bind(C, name='afid_correct_velocity')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_correct_pressure(ctx) bind(C, name='afid_correct_pressure')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_interp_vel_mgrd(ctx) bind(C, name='afid_interp_vel_mgrd')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_interp_sal_multigrid(ctx) bind(C, name='afid_interp_sal_multigrid')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_time_marcher(ctx, dt) bind(C, name='afid_time_marcher')
+            import; type(c_ptr), value :: ctx; real(c_double), value :: dt
+        end function
+        integer(c_int) function afid_calc_max_cfl(ctx, cflm, cflmr) bind(C, name='afid_calc_max_cfl')
+            import; type(c_ptr), value :: ctx; real(c_double), intent(out) :: cflm, cflmr
+        end function
+        integer(c_int) function afid_check_divergence(ctx, qmax, qmaxr) bind(C, name='afid_check_divergence')
+            import; type(c_ptr), value :: ctx; real(c_double), intent(out) :: qmax, qmaxr
+        end function
+        integer(c_int) function afid_global_quantities(ctx, out) bind(C, name='afid_global_quantities')
+            import; type(c_ptr), value :: ctx; real(c_double), intent(out) :: out(4)
+        end function
+        integer(c_int) function afid_create_initial_conditions(ctx, seed, eps) bind(C, name='afid_create_initial_conditions')
+            import; type(c_ptr), value :: ctx; integer(c_int64_t), value :: seed; real(c_double), value :: eps
+        end function
+        integer(c_int) function afid_synchronize(ctx) bind(C, name='afid_synchronize')
+            import; type(c_ptr), value :: ctx
+        end function
+    end interface
+
+contains
+
+    !> stop with the library's message if a call failed
+    subroutine afid_check(ctx, rc, what)
+        type(c_ptr), intent(in) :: ctx
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: what
+        character(kind=c_char), pointer :: msg(:)
+        integer :: n
+        if (rc == 0) return
+        call c_f_pointer(afid_last_error(ctx), msg, [1024])
+        n = 0
+        do while (n < 1024)
+            if (msg(n + 1) == c_null_char) exit
+            n = n + 1
+        end do
+        write(*, '(a,a,i0,a,1024a)') what, ' failed (', rc, '): ', msg(1:n)
+        error stop 1
+    end subroutine afid_check
+
+end module afid_gpu

@@ -1,0 +1,71 @@
+"""The Fortran host (driver loop of main.F90:281-422 and the TimeMarcher call
+tree, bound with iso_c_binding) against the FP64 CPU oracle driven the same
+way: same bou.in, same seeded initial state, same number of steps."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle
+from helpers import uniform
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "afid-murphfi_amd", "afid_host")
+EXAMPLE = os.path.join(ROOT, "afid-murphfi_amd", "fortran", "examples", "rbc_64", "bou.in")
+
+
+def write_bou(path, nxm, nym, nzm, ntst, dt):
+    txt = open(EXAMPLE).read()
+    txt = txt.replace("64     64     64     3", f"{nxm}     {nym}     {nzm}     3", 1)
+    txt = txt.replace("199     7000", f"{ntst}     7000", 1)
+    txt = txt.replace("0       1e-3    1.e-3", f"0       {dt}    1.e-3", 1)
+    open(path, "w").write(txt)
+
+
+def run_host(tmp_path, mode):
+    out = subprocess.run([EXE] + (["fused"] if mode == "fused" else []) + ["seed=7"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    vals = dict(re.findall(r"(\w+)=\s*([-+0-9.eE]+)", out.stdout))
+    return {k: float(v) for k, v in vals.items()}
+
+
+def oracle_run(nxm, nym, nzm, nsteps, dt, seed=7):
+    o = pyoracle.Oracle(dict(nxm=nxm, nym=nym, nzm=nzm, istr3=8, rayt=1e5, prat=1.0))
+    # SetTempBCs.F90:27-33 (RBC, inslwN = 1): T = +0.5 bottom, -0.5 top
+    o.field("tempbp")[...] = 0.5
+    o.field("temptp")[...] = -0.5
+    xm = o.grid("xm")[:nxm]
+    eps = 1e-3
+    amp = np.where(np.abs(xm - 0.5) + eps > 0.5, 0.5 - np.abs(xm - 0.5), eps)
+    T = o.field("temp")
+    T[..., :nxm] = 0.5 - xm + amp * (2 * uniform(seed, T.shape)[..., :nxm] - 1)
+    for _ in range(nsteps):
+        o.time_marcher(dt)
+    d, _ = o.check_divergence()
+    return o.global_quantities()["nu"], d
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+@pytest.mark.parametrize("mode", ["calltree", "fused"])
+def test_fortran_host_matches_oracle(tmp_path, mode):
+    nxm, nym, nzm, ntst, dt = 32, 24, 16, 19, 2e-3
+    write_bou(os.path.join(tmp_path, "bou.in"), nxm, nym, nzm, ntst, dt)
+    r = run_host(tmp_path, mode)
+    assert int(r["steps"]) == ntst + 1 and int(r["errorcode"]) == 555      # normal end: ntst reached
+    nu, d = oracle_run(nxm, nym, nzm, ntst + 1, dt)
+    assert abs(r["nu"] - nu) <= 1e-10 * abs(nu)
+    assert r["maxdiv"] < 1e-12 and d < 1e-12
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+def test_fortran_host_time_limit(tmp_path):
+    """A step that carries time past TMAX ends the run with errorcode 333
+    (main.F90:361), after one step, as the reference's loop does."""
+    write_bou(os.path.join(tmp_path, "bou.in"), 16, 8, 8, 5, 1e6)
+    r = run_host(tmp_path, "fused")
+    assert int(r["errorcode"]) == 333 and int(r["steps"]) == 1
