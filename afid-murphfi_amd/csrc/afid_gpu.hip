@@ -58,11 +58,18 @@ struct afid_ctx {
     std::vector<DevBuf> bufs;
     int64_t bytes = 0;
     double *f[AFID_NFIELDS] = {nullptr};
+    float *g[AFID_NFIELDS] = {nullptr};   // FP32 storage of a field (mixed form)
     double *rhs4[4] = {nullptr};       // rhs for vx, vy, vz, temp
     double *alt[4] = {nullptr};        // second buffers of vx, vy, vz, temp (fused column step)
     double *sal_alt = nullptr;
+    float *sal_alt_f = nullptr;
     double *tpdv = nullptr;
-    double2 *spec = nullptr;
+    void *spec = nullptr;              // cpx<double> (FP64 form) or cpx<float> (mixed form)
+    // arithmetic form
+    bool mixed = false;
+    double div_tol = 1e-10;
+    int max_sweeps = 3;
+    unsigned long long *slots = nullptr;   // refinement residuals of the last substep (bit patterns)
     // metrics
     std::map<std::string, std::vector<double>> host1d;   // for afid_grid_array
     const double *d_xm = nullptr, *d_xmr = nullptr;
@@ -71,6 +78,8 @@ struct afid_ctx {
     const double *d_udx3mr = nullptr, *d_ap3sskr = nullptr, *d_ac3sskr = nullptr, *d_am3sskr = nullptr, *d_lxa = nullptr;
     const int *d_xs_src = nullptr, *d_ys_src = nullptr, *d_zs_src = nullptr, *d_xb_src = nullptr, *d_yb_src = nullptr, *d_zb_src = nullptr;
     const double *d_xs_w = nullptr, *d_ys_w = nullptr, *d_zs_w = nullptr, *d_xb_w = nullptr, *d_yb_w = nullptr, *d_zb_w = nullptr;
+    const float *f_xs_w = nullptr, *f_ys_w = nullptr, *f_zs_w = nullptr, *f_xb_w = nullptr, *f_yb_w = nullptr, *f_zb_w = nullptr;
+    const float *f_lxa = nullptr;
     // host copies of the 1-D coefficients for the tridiagonal factors
     std::vector<double> h_ap3ck, h_ac3ck, h_am3ck, h_ap3sk, h_ac3sk, h_am3sk, h_ap3ssk, h_ac3ssk, h_am3ssk;
     std::vector<double> h_ap3sskr, h_ac3sskr, h_am3sskr;
@@ -81,9 +90,11 @@ struct afid_ctx {
     double fact_dt = -1.0;
     std::vector<double> tri_host;
     // Poisson
-    FftPlan plan_y, plan_z;
+    FftPlan<double> plan_y, plan_z;
+    FftPlan<float> plan_yf, plan_zf;
     int Ly = 1, Lz = 1, lds_y = 0, lds_z = 0;
-    PoissonFactors pf{};
+    PoissonFactors<double> pf{};
+    PoissonFactors<float> pff{};
     int pchunk = 1;
     // LDS plan of the transfer kernels
     int vxr_xc = 0, vxr_cj = 0, vxr_ci = 0, vy_cs = 0, vy_co = 0, vz_cs = 0, vz_co = 0;
@@ -165,7 +176,7 @@ static KP make_kp(const afid_ctx *c) {
     p.dx = c->dx; p.dy = c->dy; p.dz = c->dz; p.dxq = c->dx * c->dx; p.dyq = c->dy * c->dy; p.dzq = c->dz * c->dz;
     p.dxr = c->dxr; p.dyr = c->dyr; p.dzr = c->dzr; p.dxqr = c->dxr * c->dxr; p.dyqr = c->dyr * c->dyr; p.dzqr = c->dzr * c->dzr;
     p.inslws = c->c.inslws; p.inslwn = c->c.inslwn; p.tfixs = c->c.tfixs; p.tfixn = c->c.tfixn;
-    p.sfixs = c->c.sfixs; p.sfixn = c->c.sfixn; p.active_t = c->c.active_t; p.active_s = c->c.active_s; p.gaxis = c->c.gaxis;
+    p.sfixs = c->c.sfixs; p.sfixn = c->c.sfixn; p.active_t = c->c.active_t; p.active_s = c->sal ? c->c.active_s : 0; p.gaxis = c->c.gaxis;
     p.xplusu = c->c.xplusu; p.xminusu = c->c.xminusu;
     p.vx = c->f[AFID_VX]; p.vy = c->f[AFID_VY]; p.vz = c->f[AFID_VZ]; p.temp = c->f[AFID_TEMP]; p.pr = c->f[AFID_PR];
     p.rux = c->f[AFID_RUX]; p.ruy = c->f[AFID_RUY]; p.ruz = c->f[AFID_RUZ]; p.rut = c->f[AFID_RUTEMP];
@@ -187,6 +198,10 @@ static KP make_kp(const afid_ctx *c) {
     p.xb_src = c->d_xb_src; p.yb_src = c->d_yb_src; p.zb_src = c->d_zb_src;
     p.xb_w = c->d_xb_w; p.yb_w = c->d_yb_w; p.zb_w = c->d_zb_w;
     p.redmax = c->redmax; p.redsum = c->redsum;
+    p.salf = c->g[AFID_SAL]; p.rusalf = c->g[AFID_RUSAL]; p.vxrf = c->g[AFID_VXR]; p.vyrf = c->g[AFID_VYR]; p.vzrf = c->g[AFID_VZR];
+    p.ruxf = c->g[AFID_RUX]; p.ruyf = c->g[AFID_RUY]; p.ruzf = c->g[AFID_RUZ]; p.rutf = c->g[AFID_RUTEMP]; p.salcf = c->g[AFID_SALC];
+    p.xs_wf = c->f_xs_w; p.ys_wf = c->f_ys_w; p.zs_wf = c->f_zs_w; p.xb_wf = c->f_xb_w; p.yb_wf = c->f_yb_w; p.zb_wf = c->f_zb_w;
+    p.lxaf = c->f_lxa;
     return p;
 }
 
@@ -312,8 +327,9 @@ static int run_tri(afid_ctx *c, int kind, double *rhs, double *q, int keep = 1) 
 // ---------------------------------------------------------------------------
 // set-up
 // ---------------------------------------------------------------------------
-static int build_plan(afid_ctx *c, int n, FftPlan &pl) {
-    pl = FftPlan{};
+template <typename R>
+static int build_plan(afid_ctx *c, int n, FftPlan<R> &pl) {
+    pl = FftPlan<R>{};
     pl.n = n;
     int m = n;
     while (m % 8 == 0 && m > 1) { pl.rad[pl.nst++] = 8; m /= 8; }
@@ -321,17 +337,22 @@ static int build_plan(afid_ctx *c, int n, FftPlan &pl) {
     while (m % 2 == 0 && m > 1) { pl.rad[pl.nst++] = 2; m /= 2; }
     while (m % 3 == 0 && m > 1) { pl.rad[pl.nst++] = 3; m /= 3; }
     if (m != 1 || pl.nst > 12) return set_err(c, AFID_E_ARG, "periodic size " + std::to_string(n) + " must factor into 2 and 3");
-    std::vector<double2> tw((size_t)(n));
-    for (int t = 0; t < n; ++t) tw[size_t(t)] = make_double2(std::cos(2.0 * kPi * t / n), -std::sin(2.0 * kPi * t / n));
-    const double2 *d = nullptr;
+    std::vector<cpx<R>> tw((size_t)(n));
+    for (int t = 0; t < n; ++t) {
+        tw[size_t(t)].x = R(std::cos(2.0 * kPi * t / n));
+        tw[size_t(t)].y = R(-std::sin(2.0 * kPi * t / n));
+    }
+    const cpx<R> *d = nullptr;
     int rc = upload_vec(c, tw, &d);
     pl.tw = d;
     return rc;
 }
 
-static int pick_lines(int N, int maxlds, int cap) {
+// lines per transform workgroup: LDS of L lines of N complex values of
+// csize bytes, at most 32 values per thread
+static int pick_lines(int N, int csize, int maxlds, int cap) {
     int L = 1;
-    while (2 * L * N * 16 <= maxlds && 2 * L * N <= 32 * 256 && 2 * L <= cap) L *= 2;
+    while (2 * L * N * csize <= maxlds && 2 * L * N <= 32 * 256 && 2 * L <= cap) L *= 2;
     return L;
 }
 
@@ -340,7 +361,11 @@ static int setup(afid_ctx *c) {
     if (g.nxm < 2 || g.nym < 1 || g.nzm < 1) return set_err(c, AFID_E_ARG, "grid sizes must be nxm >= 2, nym >= 1, nzm >= 1");
     if (g.phasefield) return set_err(c, AFID_E_ARG, "phase-field configurations are not available in this build");
     if (g.nsst != 1 && g.nsst != 3) return set_err(c, AFID_E_ARG, "nsst must be 1 (Adams-Bashforth) or 3 (RK3)");
-    if (g.precision != AFID_FP64) return set_err(c, AFID_E_ARG, "only the FP64 form is available in this build");
+    if (g.precision != AFID_FP64 && g.precision != AFID_MIXED) return set_err(c, AFID_E_ARG, "precision must be AFID_FP64 or AFID_MIXED");
+    c->mixed = g.precision == AFID_MIXED;
+    c->div_tol = g.div_tol > 0.0 ? g.div_tol : 1e-10;
+    c->max_sweeps = g.max_sweeps > 0 ? g.max_sweeps : 3;
+    if (c->max_sweeps > 4) return set_err(c, AFID_E_ARG, "max_sweeps must be 1..4");
     c->sal = g.salinity != 0;
     if (c->sal && !g.multires) return set_err(c, AFID_E_ARG, "salinity requires the refined grid (MULTIRES=1)");
     c->nxm = g.nxm; c->nx = g.nxm + 1; c->ny = g.nym; c->nz = g.nzm;
@@ -445,38 +470,63 @@ static int setup(afid_ctx *c) {
     up(amphk, &c->d_amphk); up(acphk, &c->d_acphk); up(apphk, &c->d_apphk);
     if (rc) return rc;
 
-    // fields
+    // fields.  FP64 form: every field in FP64.  Mixed form: the explicit-term
+    // history and salc in FP32; qcap/dq/hro and the rhs arrays of the
+    // per-routine calls are not held (the mixed form runs through the fused
+    // step); dph holds the FP64 pressure correction.
     const size_t nb = size_t(c->nz) * c->ny * c->ldx;
     for (int f = 0; f < AFID_NFIELDS; ++f) {
         if (is_refined(f) || f == AFID_RHS || is_plane(f)) continue;
+        if (c->mixed && (f == AFID_QCAP || f == AFID_DQ || f == AFID_HRO)) continue;
+        if (c->mixed && ((f >= AFID_RUX && f <= AFID_RUTEMP) || f == AFID_SALC)) {
+            if (f == AFID_SALC && !c->sal) continue;
+            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->g[f]), sizeof(float) * nb))) return rc;
+            continue;
+        }
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * nb))) return rc;
     }
-    for (int q = 0; q < 4; ++q)
-        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->rhs4[q]), sizeof(double) * nb))) return rc;
-    c->f[AFID_RHS] = c->rhs4[0];
+    if (!c->mixed) {
+        for (int q = 0; q < 4; ++q)
+            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->rhs4[q]), sizeof(double) * nb))) return rc;
+        c->f[AFID_RHS] = c->rhs4[0];
+    }
     for (int q = 0; q < 4; ++q)
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->alt[q]), sizeof(double) * nb))) return rc;
     for (int f : {AFID_TEMPBP, AFID_TEMPTP})
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * size_t(c->ny) * c->nz))) return rc;
-    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->spec), sizeof(double2) * size_t(c->nz) * c->nymh * c->nxm))) return rc;
+    const int csize = c->mixed ? int(sizeof(cpx<float>)) : int(sizeof(cpx<double>));
+    if ((rc = dalloc(c, &c->spec, size_t(csize) * size_t(c->nz) * c->nymh * c->nxm))) return rc;
+    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->slots), sizeof(unsigned long long) * 4))) return rc;
 
     // Poisson transforms and factors
-    if ((rc = build_plan(c, c->ny, c->plan_y))) return rc;
-    if ((rc = build_plan(c, c->nz, c->plan_z))) return rc;
+    if (c->mixed) {
+        if ((rc = build_plan(c, c->ny, c->plan_yf))) return rc;
+        if ((rc = build_plan(c, c->nz, c->plan_zf))) return rc;
+    } else {
+        if ((rc = build_plan(c, c->ny, c->plan_y))) return rc;
+        if ((rc = build_plan(c, c->nz, c->plan_z))) return rc;
+    }
     int maxlds = 65536;
     {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && v > 0)
             maxlds = std::min(v, 160 * 1024);
     }
-    c->Ly = pick_lines(c->ny, maxlds, 16);      // complex lines; 2*Ly real x-values per block
-    c->Lz = pick_lines(c->nz, maxlds, 16);
-    c->lds_y = c->Ly * c->ny * 16;
-    c->lds_z = c->Lz * c->nz * 16;
-    if (c->lds_y > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_div_fft_y), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_y);
-    if (c->lds_y > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_ifft_y), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_y);
-    if (c->lds_z > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_fft_z), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_z);
-    if (c->ny * 16 > maxlds || c->nz * 16 > maxlds || c->ny > 8192 || c->nz > 8192)
+    c->Ly = pick_lines(c->ny, csize, maxlds, 16);      // complex lines; 2*Ly real x-values per block
+    c->Lz = pick_lines(c->nz, csize, maxlds, 16);
+    c->lds_y = c->Ly * c->ny * csize;
+    c->lds_z = c->Lz * c->nz * csize;
+    if (c->lds_y > 65536) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_div_fft_y<double>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_y);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_div_fft_y<float>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_y);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_ifft_y<double>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_y);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_ifft_y<float>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_y);
+    }
+    if (c->lds_z > 65536) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_fft_z<double>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_z);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_fft_z<float>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_z);
+    }
+    if (c->ny * csize > maxlds || c->nz * csize > maxlds || c->ny > 8192 || c->nz > 8192)
         return set_err(c, AFID_E_ARG, "periodic size too large for one workgroup transform");
     {
         c->pchunk = (nxm + 63) / 64;
@@ -494,6 +544,16 @@ static int setup(afid_ctx *c) {
         KP p = make_kp(c);
         hipLaunchKernelGGL(k_poisson_factors, dim3(unsigned((nmodes + 127) / 128)), dim3(128), 0, c->stream, p, dak1, dak2, fm, fg, fe, npad);
         if ((rc = check_launch(c, "k_poisson_factors"))) return rc;
+        if (c->mixed) {
+            float *tf[3];
+            const double *sf[3] = {fm, fg, fe};
+            for (int q = 0; q < 3; ++q) {
+                if ((rc = dalloc(c, reinterpret_cast<void **>(&tf[q]), sizeof(float) * size_t(nmodes) * npad))) return rc;
+                hipLaunchKernelGGL(k_to_float, dim3(1024), dim3(256), 0, c->stream, sf[q], tf[q], (long long)nmodes * npad);
+                if ((rc = check_launch(c, "k_to_float"))) return rc;
+            }
+            c->pff.m = tf[0]; c->pff.g = tf[1]; c->pff.e = tf[2]; c->pff.npad = npad;
+        }
     }
 
     // tridiagonal factor storage
@@ -609,9 +669,24 @@ static int setup(afid_ctx *c) {
     if (rc) return rc;
 
     const size_t nr = size_t(nzr) * nyr * c->ldxr;
-    for (int f = AFID_SAL; f <= AFID_VZR; ++f)
-        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * nr))) return rc;
-    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->sal_alt), sizeof(double) * nr))) return rc;
+    if (c->mixed) {
+        // FP32 refined fields; hsal/rhsr belong to the per-routine calls only
+        for (int f : {AFID_SAL, AFID_RUSAL, AFID_VXR, AFID_VYR, AFID_VZR})
+            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->g[f]), sizeof(float) * nr))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->sal_alt_f), sizeof(float) * nr))) return rc;
+        auto upf = [&](const std::vector<double> &v, const float **d) {
+            std::vector<float> w(v.begin(), v.end());
+            return rc ? rc : (rc = upload_vec(c, w, d));
+        };
+        upf(sx.w, &c->f_xs_w); upf(sy.w, &c->f_ys_w); upf(sz.w, &c->f_zs_w);
+        upf(bx.w, &c->f_xb_w); upf(by.w, &c->f_yb_w); upf(bz.w, &c->f_zb_w);
+        upf(lxa, &c->f_lxa);
+        if (rc) return rc;
+    } else {
+        for (int f = AFID_SAL; f <= AFID_VZR; ++f)
+            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * nr))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->sal_alt), sizeof(double) * nr))) return rc;
+    }
     for (int f : {AFID_SALBP, AFID_SALTP})
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * size_t(nyr) * nzr))) return rc;
     if (nxmr > 1024) return set_err(c, AFID_E_ARG, "refined wall-normal size above 1024 is not supported");
@@ -627,12 +702,13 @@ static int setup(afid_ctx *c) {
     c->vxr_xc = span(sx.src, nxmr, 64);
     c->vxr_cj = span(sy.src, nyr, VXR_TJ);
     c->vxr_ci = span(sz.src, nzr, VXR_TI);
-    c->lds_vxr = sizeof(double) * size_t(c->vxr_xc * c->vxr_cj * c->vxr_ci + c->vxr_xc * c->vxr_cj * VXR_TI +
-                                         c->vxr_xc * VXR_TJ * VXR_TI + VXR_TJ * VXR_TI);
+    const size_t es = c->mixed ? sizeof(float) : sizeof(double);      // element size of the transfer LDS
+    c->lds_vxr = es * size_t(c->vxr_xc * c->vxr_cj * c->vxr_ci + c->vxr_xc * c->vxr_cj * VXR_TI +
+                             c->vxr_xc * VXR_TJ * VXR_TI + VXR_TJ * VXR_TI);
     c->vy_cs = span(sy.src, nyr - 1, VYZ_CHUNK); c->vy_co = span(sz.src, nzr, VYZ_LINES);
     c->vz_cs = span(sz.src, nzr - 1, VYZ_CHUNK); c->vz_co = span(sy.src, nyr, VYZ_LINES);
     auto lds_yz = [&](int cs, int co) {
-        return sizeof(double) * size_t(std::max(c->vxr_xc * cs * co, cs * VYZ_LINES * 64) + c->vxr_xc * cs * VYZ_LINES);
+        return es * size_t(std::max(c->vxr_xc * cs * co, cs * VYZ_LINES * 64) + c->vxr_xc * cs * VYZ_LINES);
     };
     c->lds_vy = lds_yz(c->vy_cs, c->vy_co);
     c->lds_vz = lds_yz(c->vz_cs, c->vz_co);
@@ -640,7 +716,7 @@ static int setup(afid_ctx *c) {
         std::vector<int> xb0(xb_src.begin(), xb_src.begin() + nxm);
         c->salc_xr = span(xb0, nxm, 64);
         c->salc_yr = span(yb_src, c->ny, SALC_TJ);
-        c->lds_salc = sizeof(double) * size_t(2 * c->salc_yr * c->salc_xr);
+        c->lds_salc = es * size_t(2 * c->salc_yr * c->salc_xr);
     }
     if (c->lds_vxr > 65536 || c->lds_vy > 65536 || c->lds_vz > 65536 || c->lds_salc > 65536)
         return set_err(c, AFID_E_ARG, "refinement ratio too small for the transfer tiles (LDS plan above 64 KiB)");
@@ -693,7 +769,7 @@ const char *afid_last_error(const afid_ctx *c) { return c ? c->err.c_str() : "nu
 int64_t afid_device_bytes(const afid_ctx *c) { return c ? c->bytes : 0; }
 
 static int field_dims(const afid_ctx *c, int f, int &nxf, int &ny, int &nz, int &ld) {
-    if (f < 0 || f >= AFID_NFIELDS || !c->f[f]) return AFID_E_ARG;
+    if (f < 0 || f >= AFID_NFIELDS || (!c->f[f] && !c->g[f])) return AFID_E_ARG;
     const bool r = is_refined(f);
     ny = r ? c->nyr : c->ny;
     nz = r ? c->nzr : c->nz;
@@ -709,12 +785,18 @@ int afid_upload(afid_ctx *c, int f, const double *host, const int64_t lb[3], con
         return set_err(c, AFID_E_ARG, "afid_upload: host array does not cover the interior");
     const int kmax = int(std::min<int64_t>(nxf, lb[0] + ext[0] - 1));
     std::vector<double> slab(size_t(ny) * ld, 0.0);
+    std::vector<float> slabf(c->g[f] ? slab.size() : 0);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int i = 1; i <= nz; ++i) {
         for (int j = 1; j <= ny; ++j)
             for (int k = 1; k <= kmax; ++k)
                 slab[size_t(j - 1) * ld + (k - 1)] = host[(k - lb[0]) + ext[0] * ((j - lb[1]) + ext[1] * (int64_t)(i - lb[2]))];
-        HIPCHK(c, hipMemcpy(c->f[f] + size_t(i - 1) * ny * ld, slab.data(), sizeof(double) * slab.size(), hipMemcpyHostToDevice));
+        if (c->g[f]) {       // FP32 field of the mixed form: rounded to nearest
+            for (size_t q = 0; q < slab.size(); ++q) slabf[q] = float(slab[q]);
+            HIPCHK(c, hipMemcpy(c->g[f] + size_t(i - 1) * ny * ld, slabf.data(), sizeof(float) * slabf.size(), hipMemcpyHostToDevice));
+        } else {
+            HIPCHK(c, hipMemcpy(c->f[f] + size_t(i - 1) * ny * ld, slab.data(), sizeof(double) * slab.size(), hipMemcpyHostToDevice));
+        }
     }
     return AFID_OK;
 }
@@ -726,10 +808,16 @@ int afid_download(afid_ctx *c, int f, double *host, const int64_t lb[3], const i
         return set_err(c, AFID_E_ARG, "afid_download: host array does not cover the interior");
     const int kmax = int(std::min<int64_t>(nxf, lb[0] + ext[0] - 1));
     std::vector<double> slab(size_t(ny) * ld);
+    std::vector<float> slabf(c->g[f] ? slab.size() : 0);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int64_t ih = lb[2]; ih < lb[2] + ext[2]; ++ih) {
         const int i = int(((ih - 1) % nz + nz) % nz);    // periodic halo (halo_common.F90:66-71 at one rank)
-        HIPCHK(c, hipMemcpy(slab.data(), c->f[f] + size_t(i) * ny * ld, sizeof(double) * slab.size(), hipMemcpyDeviceToHost));
+        if (c->g[f]) {
+            HIPCHK(c, hipMemcpy(slabf.data(), c->g[f] + size_t(i) * ny * ld, sizeof(float) * slabf.size(), hipMemcpyDeviceToHost));
+            for (size_t q = 0; q < slab.size(); ++q) slab[q] = double(slabf[q]);
+        } else {
+            HIPCHK(c, hipMemcpy(slab.data(), c->f[f] + size_t(i) * ny * ld, sizeof(double) * slab.size(), hipMemcpyDeviceToHost));
+        }
         for (int64_t jh = lb[1]; jh < lb[1] + ext[1]; ++jh) {
             const int j = int(((jh - 1) % ny + ny) % ny);
             for (int k = 1; k <= kmax; ++k)
@@ -750,6 +838,7 @@ int afid_set_substep(afid_ctx *c, double dt, int ns) {
 
 #define NEED_SUBSTEP(c) if (!(c) || (c)->ns == 0) return set_err((c), AFID_E_STATE, "no substep set (afid_set_substep)");
 #define NEED_SAL(c) if (!(c)->sal) return set_err((c), AFID_E_STATE, "salinity is not enabled");
+#define NEED_FP64(c) if ((c)->mixed) return set_err((c), AFID_E_STATE, "the per-routine calls belong to the FP64 form; the mixed form runs through afid_time_marcher");
 
 static int launch_point(afid_ctx *c, void (*kern)(KP), bool refined, const char *name) {
     KP p = make_kp(c);
@@ -758,14 +847,14 @@ static int launch_point(afid_ctx *c, void (*kern)(KP), bool refined, const char 
     return check_launch(c, name);
 }
 
-int afid_explicit_terms_vx(afid_ctx *c) { NEED_SUBSTEP(c) return launch_point(c, k_explicit_vx, false, "k_explicit_vx"); }
-int afid_explicit_terms_vy(afid_ctx *c) { NEED_SUBSTEP(c) return launch_point(c, k_explicit_vy, false, "k_explicit_vy"); }
-int afid_explicit_terms_vz(afid_ctx *c) { NEED_SUBSTEP(c) return launch_point(c, k_explicit_vz, false, "k_explicit_vz"); }
-int afid_explicit_terms_temp(afid_ctx *c) { NEED_SUBSTEP(c) return launch_point(c, k_explicit_temp, false, "k_explicit_temp"); }
-int afid_explicit_salinity(afid_ctx *c) { NEED_SUBSTEP(c) NEED_SAL(c) return launch_point(c, k_explicit_sal, true, "k_explicit_sal"); }
+int afid_explicit_terms_vx(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return launch_point(c, k_explicit_vx, false, "k_explicit_vx"); }
+int afid_explicit_terms_vy(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return launch_point(c, k_explicit_vy, false, "k_explicit_vy"); }
+int afid_explicit_terms_vz(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return launch_point(c, k_explicit_vz, false, "k_explicit_vz"); }
+int afid_explicit_terms_temp(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return launch_point(c, k_explicit_temp, false, "k_explicit_temp"); }
+int afid_explicit_salinity(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) NEED_SAL(c) return launch_point(c, k_explicit_sal, true, "k_explicit_sal"); }
 
 int afid_add_sal_buoyancy(afid_ctx *c) {
-    NEED_SUBSTEP(c) NEED_SAL(c)
+    NEED_SUBSTEP(c) NEED_FP64(c) NEED_SAL(c)
     double *rkv = c->c.gaxis == 1 ? c->f[AFID_QCAP] : c->c.gaxis == 2 ? c->f[AFID_DPH] : c->f[AFID_DQ];
     KP p = make_kp(c);
     hipLaunchKernelGGL(k_add_sal_buoyancy, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, rkv);
@@ -773,60 +862,67 @@ int afid_add_sal_buoyancy(afid_ctx *c) {
 }
 
 int afid_implicit_and_update_vx(afid_ctx *c) {
-    NEED_SUBSTEP(c)
+    NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_vx, false, "k_rhs_vx");
     if (!rc) rc = run_tri(c, MAT_X, c->rhs4[0], c->f[AFID_VX]);
     c->f[AFID_RHS] = c->rhs4[0];
     return rc;
 }
 int afid_implicit_and_update_vy(afid_ctx *c) {
-    NEED_SUBSTEP(c)
+    NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_vy, false, "k_rhs_vy");
     if (!rc) rc = run_tri(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]);
     c->f[AFID_RHS] = c->rhs4[1];
     return rc;
 }
 int afid_implicit_and_update_vz(afid_ctx *c) {
-    NEED_SUBSTEP(c)
+    NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_vz, false, "k_rhs_vz");
     if (!rc) rc = run_tri(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]);
     c->f[AFID_RHS] = c->rhs4[2];
     return rc;
 }
 int afid_implicit_and_update_temp(afid_ctx *c) {
-    NEED_SUBSTEP(c)
+    NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_temp, false, "k_rhs_temp");
     if (!rc) rc = run_tri(c, MAT_T, c->rhs4[3], c->f[AFID_TEMP]);
     c->f[AFID_RHS] = c->rhs4[3];
     return rc;
 }
 int afid_implicit_salinity(afid_ctx *c) {
-    NEED_SUBSTEP(c) NEED_SAL(c)
+    NEED_SUBSTEP(c) NEED_FP64(c) NEED_SAL(c)
     int rc = launch_point(c, k_rhs_sal, true, "k_rhs_sal");
     if (!rc) rc = run_tri(c, MAT_S, c->f[AFID_RHSR], c->f[AFID_SAL]);
     return rc;
 }
 
-int afid_calc_local_divergence(afid_ctx *c) { NEED_SUBSTEP(c) return launch_point(c, k_local_divergence, false, "k_local_divergence"); }
+int afid_calc_local_divergence(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return launch_point(c, k_local_divergence, false, "k_local_divergence"); }
 
-// forward transforms from dph (from_dph = 1) or fused with the divergence (0)
-static int poisson(afid_ctx *c, int from_dph) {
+}  // extern "C"
+
+// One Poisson sweep: forward transforms of the source, per-mode tridiagonals,
+// inverse transforms into dph (or added to dph).  fwd_gate gates the forward
+// y transform, gate the rest; slot_out receives max|div| of a residual source.
+template <typename R>
+static int poisson_sweep(afid_ctx *c, const FftPlan<R> &py, const FftPlan<R> &pz, const PoissonFactors<R> &pf, int src,
+                         Gate fwd_gate, Gate gate, unsigned long long *slot_out, unsigned long long *clear, int accumulate) {
     KP p = make_kp(c);
+    cpx<R> *spec = static_cast<cpx<R> *>(c->spec);
     const int TXy = 2 * c->Ly;
     tstart(c, "poisson_fwd_y");
-    hipLaunchKernelGGL(k_div_fft_y, dim3(unsigned((c->nxm + TXy - 1) / TXy), unsigned(c->nz)), dim3(256), c->lds_y, c->stream,
-                       p, c->plan_y, c->spec, c->Ly, from_dph);
+    hipLaunchKernelGGL(k_div_fft_y<R>, dim3(unsigned((c->nxm + TXy - 1) / TXy), unsigned(c->nz)), dim3(256), c->lds_y, c->stream,
+                       p, py, spec, c->Ly, src, fwd_gate, slot_out, clear);
     tstop(c);
     int rc = check_launch(c, "k_div_fft_y");
     if (rc) return rc;
     tstart(c, "poisson_fft_z");
-    hipLaunchKernelGGL(k_fft_z, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(256), c->lds_z, c->stream,
-                       p, c->plan_z, c->spec, c->Lz, -1);
+    hipLaunchKernelGGL(k_fft_z<R>, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(256), c->lds_z, c->stream,
+                       p, pz, spec, c->Lz, -1, gate);
     tstop(c);
     if ((rc = check_launch(c, "k_fft_z"))) return rc;
     tstart(c, "poisson_tridiag");
     switch (c->pchunk) {
-#define AFID_P_CASE(CH) case CH: hipLaunchKernelGGL(k_poisson_tri<CH>, dim3(unsigned(c->nymh), unsigned(c->nz / 2 + 1)), dim3(128), 0, c->stream, p, c->pf, c->spec); break;
+#define AFID_P_CASE(CH) case CH: hipLaunchKernelGGL((k_poisson_tri<CH, R>), dim3(unsigned(c->nymh), unsigned(c->nz / 2 + 1)), dim3(128), 0, c->stream, p, pf, spec, gate); break;
         AFID_P_CASE(1) AFID_P_CASE(2) AFID_P_CASE(3) AFID_P_CASE(4) AFID_P_CASE(5) AFID_P_CASE(6) AFID_P_CASE(7) AFID_P_CASE(8)
         AFID_P_CASE(9) AFID_P_CASE(10) AFID_P_CASE(11) AFID_P_CASE(12) AFID_P_CASE(13) AFID_P_CASE(14) AFID_P_CASE(15) AFID_P_CASE(16)
 #undef AFID_P_CASE
@@ -835,43 +931,71 @@ static int poisson(afid_ctx *c, int from_dph) {
     tstop(c);
     if ((rc = check_launch(c, "k_poisson_tri"))) return rc;
     tstart(c, "poisson_fft_z");
-    hipLaunchKernelGGL(k_fft_z, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(256), c->lds_z, c->stream,
-                       p, c->plan_z, c->spec, c->Lz, +1);
+    hipLaunchKernelGGL(k_fft_z<R>, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(256), c->lds_z, c->stream,
+                       p, pz, spec, c->Lz, +1, gate);
     tstop(c);
     if ((rc = check_launch(c, "k_fft_z"))) return rc;
     tstart(c, "poisson_inv_y");
-    hipLaunchKernelGGL(k_ifft_y, dim3(unsigned((c->nxm + TXy - 1) / TXy), unsigned(c->nz)), dim3(256), c->lds_y, c->stream,
-                       p, c->plan_y, c->spec, c->Ly);
+    hipLaunchKernelGGL(k_ifft_y<R>, dim3(unsigned((c->nxm + TXy - 1) / TXy), unsigned(c->nz)), dim3(256), c->lds_y, c->stream,
+                       p, py, spec, c->Ly, accumulate, gate);
     tstop(c);
     return check_launch(c, "k_ifft_y");
 }
 
-int afid_solve_pressure_correction(afid_ctx *c) { NEED_SUBSTEP(c) return poisson(c, 1); }
+// FP64 form: one sweep, from dph (from_dph = 1) or fused with the divergence (0)
+static int poisson(afid_ctx *c, int from_dph) {
+    const Gate open{nullptr, 0.0};
+    return poisson_sweep<double>(c, c->plan_y, c->plan_z, c->pf, from_dph ? SRC_DPH : SRC_VELOCITY, open, open, nullptr, nullptr, 0);
+}
+
+// Mixed form: FP32 sweeps inside FP64-residual iterative refinement.  Sweep 1
+// solves for the divergence of the velocity; sweep s > 1 transforms the FP64
+// divergence that the projection with the accumulated dph would leave
+// (div_after_projection), records its maximum in slots[s-1], and solves and
+// adds its correction only while that maximum is above div_tol.  The gates
+// are evaluated on the device, so the step never waits for the host.
+static int poisson_mixed(afid_ctx *c) {
+    const Gate open{nullptr, 0.0};
+    int rc = poisson_sweep<float>(c, c->plan_yf, c->plan_zf, c->pff, SRC_VELOCITY, open, open, nullptr, c->slots, 0);
+    for (int s = 2; s <= c->max_sweeps && !rc; ++s) {
+        const Gate fwd{s == 2 ? nullptr : c->slots + (s - 2), c->div_tol};
+        const Gate rest{c->slots + (s - 1), c->div_tol};
+        rc = poisson_sweep<float>(c, c->plan_yf, c->plan_zf, c->pff, SRC_RESIDUAL, fwd, rest, c->slots + (s - 1), nullptr, 1);
+    }
+    return rc;
+}
+
+extern "C" {
+
+int afid_solve_pressure_correction(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return poisson(c, 1); }
 
 static int project(afid_ctx *c, int vel, int pr) {
     KP p = make_kp(c);
     hipLaunchKernelGGL(k_project, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, vel, pr);
     return check_launch(c, "k_project");
 }
-int afid_correct_velocity(afid_ctx *c) { NEED_SUBSTEP(c) return project(c, 1, 0); }
-int afid_correct_pressure(afid_ctx *c) { NEED_SUBSTEP(c) return project(c, 0, 1); }
+int afid_correct_velocity(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return project(c, 1, 0); }
+int afid_correct_pressure(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return project(c, 0, 1); }
 
 static int interp_vel(afid_ctx *c) {
     KP p = make_kp(c);
     int rc;
+    const dim3 gx(unsigned((c->nyr + VXR_TJ - 1) / VXR_TJ), unsigned((c->nzr + VXR_TI - 1) / VXR_TI));
+    const dim3 gy(unsigned((c->nxmr + 63) / 64), unsigned((c->nzr + VYZ_LINES - 1) / VYZ_LINES));
+    const dim3 gz(unsigned((c->nxmr + 63) / 64), unsigned((c->nyr + VYZ_LINES - 1) / VYZ_LINES));
     tstart(c, "mgrd_vxr");
-    hipLaunchKernelGGL(k_vxr_tile, dim3(unsigned((c->nyr + VXR_TJ - 1) / VXR_TJ), unsigned((c->nzr + VXR_TI - 1) / VXR_TI)),
-                       dim3(256), c->lds_vxr, c->stream, p, c->vxr_xc, c->vxr_cj, c->vxr_ci);
+    if (c->mixed) hipLaunchKernelGGL(k_vxr_tile<float>, gx, dim3(256), c->lds_vxr, c->stream, p, c->vxr_xc, c->vxr_cj, c->vxr_ci);
+    else hipLaunchKernelGGL(k_vxr_tile<double>, gx, dim3(256), c->lds_vxr, c->stream, p, c->vxr_xc, c->vxr_cj, c->vxr_ci);
     tstop(c);
     if ((rc = check_launch(c, "k_vxr_tile"))) return rc;
     tstart(c, "mgrd_vyr");
-    hipLaunchKernelGGL(k_vyz_tile<1>, dim3(unsigned((c->nxmr + 63) / 64), unsigned((c->nzr + VYZ_LINES - 1) / VYZ_LINES)),
-                       dim3(64 * VYZ_LINES), c->lds_vy, c->stream, p, c->vxr_xc, c->vy_cs, c->vy_co);
+    if (c->mixed) hipLaunchKernelGGL((k_vyz_tile<1, float>), gy, dim3(64 * VYZ_LINES), c->lds_vy, c->stream, p, c->vxr_xc, c->vy_cs, c->vy_co);
+    else hipLaunchKernelGGL((k_vyz_tile<1, double>), gy, dim3(64 * VYZ_LINES), c->lds_vy, c->stream, p, c->vxr_xc, c->vy_cs, c->vy_co);
     tstop(c);
     if ((rc = check_launch(c, "k_vyz_tile<1>"))) return rc;
     tstart(c, "mgrd_vzr");
-    hipLaunchKernelGGL(k_vyz_tile<2>, dim3(unsigned((c->nxmr + 63) / 64), unsigned((c->nyr + VYZ_LINES - 1) / VYZ_LINES)),
-                       dim3(64 * VYZ_LINES), c->lds_vz, c->stream, p, c->vxr_xc, c->vz_cs, c->vz_co);
+    if (c->mixed) hipLaunchKernelGGL((k_vyz_tile<2, float>), gz, dim3(64 * VYZ_LINES), c->lds_vz, c->stream, p, c->vxr_xc, c->vz_cs, c->vz_co);
+    else hipLaunchKernelGGL((k_vyz_tile<2, double>), gz, dim3(64 * VYZ_LINES), c->lds_vz, c->stream, p, c->vxr_xc, c->vz_cs, c->vz_co);
     tstop(c);
     return check_launch(c, "k_vyz_tile<2>");
 }
@@ -880,10 +1004,10 @@ int afid_interp_vel_mgrd(afid_ctx *c) { NEED_SAL(c) return interp_vel(c); }
 int afid_interp_sal_multigrid(afid_ctx *c) {
     NEED_SAL(c)
     KP p = make_kp(c);
+    const dim3 gr(unsigned((c->nxm + 63) / 64), unsigned((c->ny + SALC_TJ - 1) / SALC_TJ), unsigned((c->nz + SALC_TZ - 1) / SALC_TZ));
     tstart(c, "mgrd_interp_sal");
-    hipLaunchKernelGGL(k_interp_sal_tile, dim3(unsigned((c->nxm + 63) / 64), unsigned((c->ny + SALC_TJ - 1) / SALC_TJ),
-                                               unsigned((c->nz + SALC_TZ - 1) / SALC_TZ)),
-                       dim3(256), c->lds_salc, c->stream, p, c->salc_xr, c->salc_yr);
+    if (c->mixed) hipLaunchKernelGGL(k_interp_sal_tile<float>, gr, dim3(256), c->lds_salc, c->stream, p, c->salc_xr, c->salc_yr);
+    else hipLaunchKernelGGL(k_interp_sal_tile<double>, gr, dim3(256), c->lds_salc, c->stream, p, c->salc_xr, c->salc_yr);
     tstop(c);
     return check_launch(c, "k_interp_sal_tile");
 }
@@ -894,8 +1018,11 @@ int afid_interp_sal_multigrid(afid_ctx *c) {
 template <int CH>
 static void launch_column_base(afid_ctx *c, const KP &p, long long nblk) {
     BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
-    hipLaunchKernelGGL(k_column_base<CH>, dim3(unsigned((nblk + 7) / 8 * 8)), dim3(256), 0, c->stream, p,
-                       factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
+    const dim3 gr(unsigned((nblk + 7) / 8 * 8));
+    if (c->mixed)
+        hipLaunchKernelGGL((k_column_base<CH, float>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
+    else
+        hipLaunchKernelGGL((k_column_base<CH, double>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
 }
 static int column_base(afid_ctx *c) {
     KP p = make_kp(c);
@@ -919,7 +1046,9 @@ static int column_base(afid_ctx *c) {
 }
 template <int CH>
 static void launch_column_sal(afid_ctx *c, const KP &p, long long nblk) {
-    hipLaunchKernelGGL(k_column_sal<CH>, dim3(unsigned((nblk + 7) / 8 * 8)), dim3(256), 0, c->stream, p, factors(c, MAT_S), c->sal_alt);
+    const dim3 gr(unsigned((nblk + 7) / 8 * 8));
+    if (c->mixed) hipLaunchKernelGGL((k_column_sal<CH, float>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_S), c->sal_alt_f);
+    else hipLaunchKernelGGL((k_column_sal<CH, double>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_S), c->sal_alt);
 }
 static int column_sal(afid_ctx *c) {
     KP p = make_kp(c);
@@ -935,7 +1064,8 @@ static int column_sal(afid_ctx *c) {
     }
     int rc = check_launch(c, "k_column_sal");
     if (rc) return rc;
-    std::swap(c->f[AFID_SAL], c->sal_alt);
+    if (c->mixed) std::swap(c->g[AFID_SAL], c->sal_alt_f);
+    else std::swap(c->f[AFID_SAL], c->sal_alt);
     return AFID_OK;
 }
 
@@ -957,7 +1087,7 @@ int afid_time_marcher(afid_ctx *c, double dt) {
             if ((rc = column_sal(c))) return rc;
             tstop(c);
         }
-        if ((rc = poisson(c, 0))) return rc;
+        if ((rc = c->mixed ? poisson_mixed(c) : poisson(c, 0))) return rc;
         tstart(c, "project");
         if ((rc = project(c, 1, 1))) return rc;
         tstop(c);
@@ -973,18 +1103,29 @@ int afid_time_marcher(afid_ctx *c, double dt) {
 // noise; also the scalar of CreateInitialSalinity on the refined grid.
 int afid_create_initial_conditions(afid_ctx *c, uint64_t seed, double eps) {
     if (!c) return AFID_E_ARG;
-    for (int f : {AFID_VX, AFID_VY, AFID_VZ, AFID_PR, AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP})
-        HIPCHK(c, hipMemsetAsync(c->f[f], 0, sizeof(double) * size_t(c->nz) * c->ny * c->ldx, c->stream));
-    hipLaunchKernelGGL(k_init_scalar, point_grid(c->nx, c->ny, c->nz), kPointBlock, 0, c->stream, c->f[AFID_TEMP],
+    const size_t nb = size_t(c->nz) * c->ny * c->ldx;
+    for (int f : {AFID_VX, AFID_VY, AFID_VZ, AFID_PR, AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP}) {
+        if (c->g[f]) HIPCHK(c, hipMemsetAsync(c->g[f], 0, sizeof(float) * nb, c->stream));
+        else HIPCHK(c, hipMemsetAsync(c->f[f], 0, sizeof(double) * nb, c->stream));
+    }
+    hipLaunchKernelGGL(k_init_scalar<double>, point_grid(c->nx, c->ny, c->nz), kPointBlock, 0, c->stream, c->f[AFID_TEMP],
                        c->f[AFID_TEMPBP], c->f[AFID_TEMPTP], c->d_xm, c->nxm, c->nx, c->ldx, c->ny, c->nz, c->c.alx3, eps,
                        (unsigned long long)seed);
     int rc = check_launch(c, "k_init_scalar");
     if (rc || !c->sal) return rc;
-    for (int f : {AFID_RUSAL, AFID_VXR, AFID_VYR, AFID_VZR})
-        HIPCHK(c, hipMemsetAsync(c->f[f], 0, sizeof(double) * size_t(c->nzr) * c->nyr * c->ldxr, c->stream));
-    hipLaunchKernelGGL(k_init_scalar, point_grid(c->nxr, c->nyr, c->nzr), kPointBlock, 0, c->stream, c->f[AFID_SAL],
-                       c->f[AFID_SALBP], c->f[AFID_SALTP], c->d_xmr, c->nxmr, c->nxr, c->ldxr, c->nyr, c->nzr, c->c.alx3, eps,
-                       (unsigned long long)(seed + 1));
+    const size_t nr = size_t(c->nzr) * c->nyr * c->ldxr;
+    for (int f : {AFID_RUSAL, AFID_VXR, AFID_VYR, AFID_VZR}) {
+        if (c->g[f]) HIPCHK(c, hipMemsetAsync(c->g[f], 0, sizeof(float) * nr, c->stream));
+        else HIPCHK(c, hipMemsetAsync(c->f[f], 0, sizeof(double) * nr, c->stream));
+    }
+    if (c->mixed)
+        hipLaunchKernelGGL(k_init_scalar<float>, point_grid(c->nxr, c->nyr, c->nzr), kPointBlock, 0, c->stream, c->g[AFID_SAL],
+                           c->f[AFID_SALBP], c->f[AFID_SALTP], c->d_xmr, c->nxmr, c->nxr, c->ldxr, c->nyr, c->nzr, c->c.alx3, eps,
+                           (unsigned long long)(seed + 1));
+    else
+        hipLaunchKernelGGL(k_init_scalar<double>, point_grid(c->nxr, c->nyr, c->nzr), kPointBlock, 0, c->stream, c->f[AFID_SAL],
+                           c->f[AFID_SALBP], c->f[AFID_SALTP], c->d_xmr, c->nxmr, c->nxr, c->ldxr, c->nyr, c->nzr, c->c.alx3, eps,
+                           (unsigned long long)(seed + 1));
     if ((rc = check_launch(c, "k_init_scalar"))) return rc;
     // salc from the initial S (the reference interpolates at start-up, main.F90:206-213)
     return afid_interp_sal_multigrid(c);
@@ -1014,7 +1155,8 @@ int afid_calc_max_cfl(afid_ctx *c, double *cflm, double *cflmr) {
     int rc = check_launch(c, "k_cfl_div");
     if (rc) return rc;
     if (c->sal) {
-        hipLaunchKernelGGL(k_cfl_div_r, red_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 0);
+        if (c->mixed) hipLaunchKernelGGL(k_cfl_div_r<float>, red_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 0);
+        else hipLaunchKernelGGL(k_cfl_div_r<double>, red_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 0);
         if ((rc = check_launch(c, "k_cfl_div_r"))) return rc;
     }
     if ((rc = read_max(c, cflm, cflmr))) return rc;
@@ -1031,7 +1173,8 @@ int afid_check_divergence(afid_ctx *c, double *qmax, double *qmaxr) {
     int rc = check_launch(c, "k_cfl_div");
     if (rc) return rc;
     if (c->sal) {
-        hipLaunchKernelGGL(k_cfl_div_r, red_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 1);
+        if (c->mixed) hipLaunchKernelGGL(k_cfl_div_r<float>, red_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 1);
+        else hipLaunchKernelGGL(k_cfl_div_r<double>, red_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, 1);
         if ((rc = check_launch(c, "k_cfl_div_r"))) return rc;
     }
     unsigned long long h[4];
@@ -1060,6 +1203,16 @@ int afid_global_quantities(afid_ctx *c, double out[4]) {
     out[1] = 0.5 * s[2] * vol;
     out[2] = std::sqrt(s[2] * vol) * std::sqrt(std::fabs(c->c.rayt) / c->c.prat);  // :91
     out[3] = s[1] * vol;                                                            // :89
+    return AFID_OK;
+}
+
+int afid_refinement_residuals(afid_ctx *c, double *out, int max, int *count) {
+    if (!c || !count) return AFID_E_ARG;
+    unsigned long long h[4] = {0, 0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(h, c->slots, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *count = c->mixed ? c->max_sweeps - 1 : 0;
+    for (int q = 0; q < *count && q < max && out; ++q) std::memcpy(&out[q], &h[q + 1], sizeof(double));
     return AFID_OK;
 }
 

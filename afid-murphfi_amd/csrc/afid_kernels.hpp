@@ -28,27 +28,27 @@ __global__ void __launch_bounds__(256) k_explicit_vx(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
     if (k == 0) return;   // qcap(1) is not computed (ExplicitTermsVX.F90:50)
     const Nb n = make_nb(p, j, i);
-    p.qcap[bidx(p, k, j, i)] = ex_vx(p, n, k);
+    p.qcap[bidx(p, k, j, i)] = ex_vx<double>(p, n, k);
 }
 __global__ void __launch_bounds__(256) k_explicit_vy(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
     const Nb n = make_nb(p, j, i);
-    p.dph[bidx(p, k, j, i)] = ex_vy(p, n, k);
+    p.dph[bidx(p, k, j, i)] = ex_vy<double>(p, n, k);
 }
 __global__ void __launch_bounds__(256) k_explicit_vz(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
     const Nb n = make_nb(p, j, i);
-    p.dq[bidx(p, k, j, i)] = ex_vz(p, n, k);
+    p.dq[bidx(p, k, j, i)] = ex_vz<double>(p, n, k);
 }
 __global__ void __launch_bounds__(256) k_explicit_temp(KP p) {
     AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
     const Nb n = make_nb(p, j, i);
-    p.hro[bidx(p, k, j, i)] = ex_temp(p, n, k);
+    p.hro[bidx(p, k, j, i)] = ex_temp<double>(p, n, k);
 }
 __global__ void __launch_bounds__(256) k_explicit_sal(KP p) {
     AFID_POINT_PROLOGUE(p.nxmr, p.nyr, p.nzr)
-    const NbS n = make_nbs(p, j, i);
-    p.hsal[ridx(p, k, j, i)] = ex_sal(p, n, k);
+    const NbS<double> n = make_nbs<double>(p, j, i);
+    p.hsal[ridx(p, k, j, i)] = ex_sal<double, double>(p, n, k);
 }
 // AddSalBuoyancy, salinity.F90:495-506
 __global__ void __launch_bounds__(256) k_add_sal_buoyancy(KP p, double *rkv) {
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) k_rhs_sal(KP p) {
     AFID_POINT_PROLOGUE(p.nxmr, p.nyr, p.nzr)
     const long long c = ridx(p, k, j, i);
     const double h = p.hsal[c];
-    const NbS n = make_nbs(p, j, i);
+    const NbS<double> n = make_nbs<double>(p, j, i);
     p.rhsr[c] = rhs_sal(p, n, h, p.rusal[c], k);
     p.rusal[c] = h;
 }
@@ -233,18 +233,20 @@ __device__ __forceinline__ void column_solve_lds(double *buf, const TriFactors &
     for (int m = 0; m < CH; ++m) buf[lane * (CH + 1) + m] = v[m];
 }
 
-// ExplicitSalinity + ImplicitSalinity + SolveImpEqnUpdate_Sal (salinity.F90:268-454)
-template <int CH>
-__global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, double *sal_out) {
+// ExplicitSalinity + ImplicitSalinity + SolveImpEqnUpdate_Sal (salinity.F90:268-454).
+// RT is the storage (and explicit-stencil arithmetic) type of the refined
+// fields; the implicit RHS, the solve and the update run in FP64.
+template <int CH, typename RT>
+__global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_out) {
     __shared__ double lds[4][lds_col_size<CH>()];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
     int j, i;
     const bool live = strip_column(xcd_block(0), wave, p.nyr, p.nzr, j, i);
     double *buf = lds[wave];
-    const NbS n = make_nbs(p, j, i);
+    const NbS<RT> n = make_nbs<RT>(p, j, i);
     const long long c0 = (long long)i * p.planer + (long long)j * p.ldxr;
-    double *rus = p.rusal + c0;
+    RT *rus = sel<RT>(p.rusal, p.rusalf) + c0;
     double own[CH];
 #pragma unroll
     for (int m = 0; m < CH; ++m) {
@@ -252,10 +254,10 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, double *
         double rhs = 0.0;
         own[m] = 0.0;
         if (live && r < unsigned(p.nxmr)) {
-            const double h = ex_sal(p, n, r);
-            rhs = rhs_sal(p, n, h, rus[r], r);
+            const RT h = ex_sal<RT, RT>(p, n, r);
+            rhs = rhs_sal<RT>(p, n, double(h), double(rus[r]), r);
             rus[r] = h;
-            own[m] = n.s[r];
+            own[m] = double(n.s[r]);
         }
         buf[lds_row<CH>(r)] = rhs;
     }
@@ -263,11 +265,11 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, double *
     column_solve_lds<CH>(buf, f);
     __syncthreads();
     if (!live) return;
-    double *out = sal_out + c0;
+    RT *out = sal_out + c0;
 #pragma unroll
     for (int m = 0; m < CH; ++m) {
         const unsigned r = lane + 64 * m;
-        if (r < unsigned(p.nxmr)) out[r] = own[m] + buf[lds_row<CH>(r)];
+        if (r < unsigned(p.nxmr)) out[r] = RT(own[m] + buf[lds_row<CH>(r)]);
         else if (r == unsigned(p.nxmr)) out[r] = n.s[r];
     }
     if (64 * CH == p.nxmr && lane == 0) out[p.nxmr] = n.s[p.nxmr];
@@ -277,47 +279,50 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, double *
 // VX/VY/VZ/Temp with their solves (TimeMarcher.F90:45-95) for one column.
 struct BaseOut { double *vx, *vy, *vz, *temp; };
 
-// stencil phase of field FLD: RHS of each row into buf, old value into own
-template <int FLD, int CH>
+// stencil phase of field FLD: RHS of each row into buf, old value into own.
+// HT is the storage and arithmetic type of the explicit terms (and of salc);
+// the RHS and the solve run in FP64.
+template <int FLD, int CH, typename HT>
 __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long long c0, double *buf, double *own,
                                                  unsigned lane, unsigned nrow, bool live) {
-    const double *salc = p.salc + c0;
-    double *ru = (FLD == 0 ? p.rux : FLD == 1 ? p.ruy : FLD == 2 ? p.ruz : p.rut) + c0;
+    const HT *salc = sel<HT>(p.salc, p.salcf) + c0;
+    HT *ru = (FLD == 0 ? sel<HT>(p.rux, p.ruxf) : FLD == 1 ? sel<HT>(p.ruy, p.ruyf)
+              : FLD == 2 ? sel<HT>(p.ruz, p.ruzf) : sel<HT>(p.rut, p.rutf)) + c0;
     const unsigned nxm = p.nxm;
 #pragma unroll 1
     for (unsigned r = lane; r < nrow; r += 64) {
         double rhs = 0.0, old = 0.0;
         if (live && r < unsigned(p.nx)) {
-            const double sb = (p.active_s == 1 && r < nxm) ? p.bycs * salc[r] : 0.0;
+            const HT sb = (p.active_s == 1 && r < nxm) ? HT(p.bycs) * salc[r] : HT(0);
             if (FLD == 0) {
                 old = n.vx[r];
                 if (r >= 1 && r < nxm) {
-                    double h = ex_vx(p, n, r);
+                    HT h = ex_vx<HT>(p, n, r);
                     if (p.gaxis == 1) h -= sb;
-                    rhs = rhs_vx(p, n, h, ru[r], r);
+                    rhs = rhs_vx(p, n, double(h), double(ru[r]), r);
                     ru[r] = h;
                 }
             } else if (FLD == 1) {
                 old = n.vy[r];
                 if (r < nxm) {
-                    double h = ex_vy(p, n, r);
+                    HT h = ex_vy<HT>(p, n, r);
                     if (p.gaxis == 2) h -= sb;
-                    rhs = rhs_vy(p, n, h, ru[r], r);
+                    rhs = rhs_vy(p, n, double(h), double(ru[r]), r);
                     ru[r] = h;
                 }
             } else if (FLD == 2) {
                 old = n.vz[r];
                 if (r < nxm) {
-                    double h = ex_vz(p, n, r);
+                    HT h = ex_vz<HT>(p, n, r);
                     if (p.gaxis == 3) h -= sb;
-                    rhs = rhs_vz(p, n, h, ru[r], r);
+                    rhs = rhs_vz(p, n, double(h), double(ru[r]), r);
                     ru[r] = h;
                 }
             } else {
                 old = n.t[r];
                 if (r < nxm) {
-                    double h = ex_temp(p, n, r);
-                    rhs = rhs_temp(p, n, h, ru[r], r);
+                    HT h = ex_temp<HT>(p, n, r);
+                    rhs = rhs_temp(p, n, double(h), double(ru[r]), r);
                     ru[r] = h;
                 }
             }
@@ -327,7 +332,7 @@ __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long 
     }
 }
 
-template <int CH>
+template <int CH, typename HT>
 __global__ void __launch_bounds__(256) k_column_base(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
     __shared__ double lds[2][4][lds_col_size<CH>()];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -341,10 +346,10 @@ __global__ void __launch_bounds__(256) k_column_base(KP p, TriFactors fx, TriFac
 #pragma unroll 1
     for (int fld = 0; fld < 4; ++fld) {
         switch (fld) {
-        case 0: column_base_rows<0, CH>(p, n, c0, buf, own, lane, nrow, live); break;
-        case 1: column_base_rows<1, CH>(p, n, c0, buf, own, lane, nrow, live); break;
-        case 2: column_base_rows<2, CH>(p, n, c0, buf, own, lane, nrow, live); break;
-        default: column_base_rows<3, CH>(p, n, c0, buf, own, lane, nrow, live); break;
+        case 0: column_base_rows<0, CH, HT>(p, n, c0, buf, own, lane, nrow, live); break;
+        case 1: column_base_rows<1, CH, HT>(p, n, c0, buf, own, lane, nrow, live); break;
+        case 2: column_base_rows<2, CH, HT>(p, n, c0, buf, own, lane, nrow, live); break;
+        default: column_base_rows<3, CH, HT>(p, n, c0, buf, own, lane, nrow, live); break;
         }
         __syncthreads();
         column_solve_lds<CH>(buf, fld == 0 ? fx : (fld == 3 ? ft : fyz));
@@ -408,12 +413,14 @@ __global__ void __launch_bounds__(256) k_project(KP p, int do_vel, int do_pr) {
 // integrated along its axis with the running sum carried across chunks.
 // ---------------------------------------------------------------------------
 // coarse gradient of component comp at reference x index xc (-1..nx+1) and
-// 0-based periodic (j, i)
-__device__ __forceinline__ double mgrd_grad(const KP &p, int comp, int xc, int j, int i) {
+// 0-based periodic (j, i); the difference of the FP64 velocities is taken in
+// FP64 and handed on in the transfer type RT
+template <typename RT>
+__device__ __forceinline__ RT mgrd_grad(const KP &p, int comp, int xc, int j, int i) {
     int kc = xc, sgn = 1;
     if (xc == 0) { kc = 1; sgn = 1 - 2 * p.inslws; }
     else if (xc == p.nx) { kc = p.nxm; sgn = 1 - 2 * p.inslwn; }
-    else if (xc < 0 || xc > p.nx) return 0.0;
+    else if (xc < 0 || xc > p.nx) return RT(0);
     const int k = kc - 1;
     double d;
     if (comp == 0) {
@@ -426,7 +433,7 @@ __device__ __forceinline__ double mgrd_grad(const KP &p, int comp, int xc, int j
         const double *c = p.vz + (long long)j * p.ldx;
         d = (c[(long long)wrapp(i, p.nz) * p.plane + k] - c[(long long)i * p.plane + k]) * p.dz;
     }
-    return sgn * d;
+    return RT(sgn * d);
 }
 __device__ __forceinline__ int wrapn(int j, int n) { j %= n; return j < 0 ? j + n : j; }
 
@@ -437,21 +444,27 @@ constexpr int VYZ_CHUNK = 16;               // refined points per scan chunk
 
 // vxr: tile of VXR_TJ x VXR_TI refined columns, x processed in chunks of 64
 // refined points (one per lane); xcmax/cjmax/cimax bound the coarse block.
+// RT: storage and arithmetic type of the transfer (double or float).
+template <typename RT>
 __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, int cimax) {
-    extern __shared__ double sm[];
+    extern __shared__ __align__(16) unsigned char smraw[];
+    RT *sm = reinterpret_cast<RT *>(smraw);
+    RT *vxr = sel<RT>(p.vxr, p.vxrf);
+    const RT *xs_w = sel<RT>(p.xs_w, p.xs_wf), *ys_w = sel<RT>(p.ys_w, p.ys_wf), *zs_w = sel<RT>(p.zs_w, p.zs_wf);
+    const RT *lxa = sel<RT>(p.lxa, p.lxaf);
     const int J0 = blockIdx.x * VXR_TJ, I0 = blockIdx.y * VXR_TI;
     const int nj = min(VXR_TJ, p.nyr - J0), ni = min(VXR_TI, p.nzr - I0);
     const int cj0 = p.ys_src[J0] - 1, CJ = p.ys_src[J0 + nj - 1] - 1 + 4 - cj0;
     const int ci0 = p.zs_src[I0] - 1, CI = p.zs_src[I0 + ni - 1] - 1 + 4 - ci0;
-    double *B = sm;                                   // [ci][cj][xc]
-    double *Z = B + xcmax * cjmax * cimax;            // [ti][cj][xc]
-    double *Q = Z + xcmax * cjmax * VXR_TI;           // [ti][tj][xc]
-    double *carry = Q + xcmax * VXR_TJ * VXR_TI;      // [tj*ti]
+    RT *B = sm;                                       // [ci][cj][xc]
+    RT *Z = B + xcmax * cjmax * cimax;                // [ti][cj][xc]
+    RT *Q = Z + xcmax * cjmax * VXR_TI;               // [ti][tj][xc]
+    RT *carry = Q + xcmax * VXR_TJ * VXR_TI;          // [tj*ti]
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int ncol = nj * ni;
     for (int q = t; q < ncol; q += 256) {
-        carry[q] = 0.0;
-        p.vxr[ridx(p, 0, J0 + q % nj, I0 + q / nj)] = 0.0;      // vxr(1) = 0
+        carry[q] = RT(0);
+        vxr[ridx(p, 0, J0 + q % nj, I0 + q / nj)] = RT(0);      // vxr(1) = 0
     }
     for (int R0 = 0; R0 < p.nxmr; R0 += 64) {
         const int R1 = min(R0 + 64, p.nxmr);
@@ -459,13 +472,13 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
         __syncthreads();
         for (int q = t; q < XC * CJ * CI; q += 256) {
             const int xc = q % XC, cj = (q / XC) % CJ, ci = q / (XC * CJ);
-            B[(ci * cjmax + cj) * xcmax + xc] = mgrd_grad(p, 0, x0 + xc, wrapn(cj0 + cj, p.ny), wrapn(ci0 + ci, p.nz));
+            B[(ci * cjmax + cj) * xcmax + xc] = mgrd_grad<RT>(p, 0, x0 + xc, wrapn(cj0 + cj, p.ny), wrapn(ci0 + ci, p.nz));
         }
         __syncthreads();
         for (int q = t; q < XC * CJ * ni; q += 256) {
             const int xc = q % XC, cj = (q / XC) % CJ, ti = q / (XC * CJ);
             const int c0 = p.zs_src[I0 + ti] - 1 - ci0;
-            const double *w = p.zs_w + 4 * (I0 + ti), *b = B + (c0 * cjmax + cj) * xcmax + xc;
+            const RT *w = zs_w + 4 * (I0 + ti), *b = B + (c0 * cjmax + cj) * xcmax + xc;
             const int st = cjmax * xcmax;
             Z[(ti * cjmax + cj) * xcmax + xc] = b[0] * w[0] + b[st] * w[1] + b[2 * st] * w[2] + b[3 * st] * w[3];
         }
@@ -473,33 +486,33 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
         for (int q = t; q < XC * nj * ni; q += 256) {
             const int xc = q % XC, tj = (q / XC) % nj, ti = q / (XC * nj);
             const int b0 = p.ys_src[J0 + tj] - 1 - cj0;
-            const double *w = p.ys_w + 4 * (J0 + tj), *z = Z + (ti * cjmax + b0) * xcmax + xc;
+            const RT *w = ys_w + 4 * (J0 + tj), *z = Z + (ti * cjmax + b0) * xcmax + xc;
             Q[(ti * VXR_TJ + tj) * xcmax + xc] = z[0] * w[0] + z[xcmax] * w[1] + z[2 * xcmax] * w[2] + z[3 * xcmax] * w[3];
         }
         __syncthreads();
         const int kr = R0 + lane;
         int xa = 0;
-        double wx0 = 0, wx1 = 0, wx2 = 0, wx3 = 0, lx = 0;
+        RT wx0 = 0, wx1 = 0, wx2 = 0, wx3 = 0, lx = 0;
         if (kr < R1) {
             xa = p.xs_src[kr] - x0;
-            wx0 = p.xs_w[4 * kr]; wx1 = p.xs_w[4 * kr + 1]; wx2 = p.xs_w[4 * kr + 2]; wx3 = p.xs_w[4 * kr + 3];
-            lx = p.lxa[kr];
+            wx0 = xs_w[4 * kr]; wx1 = xs_w[4 * kr + 1]; wx2 = xs_w[4 * kr + 2]; wx3 = xs_w[4 * kr + 3];
+            lx = lxa[kr];
         }
         for (int col = wave; col < ncol; col += 4) {
             const int tj = col % nj, ti = col / nj;
-            double v = 0.0;
+            RT v = RT(0);
             if (kr < R1) {
-                const double *qq = Q + (ti * VXR_TJ + tj) * xcmax + xa;
+                const RT *qq = Q + (ti * VXR_TJ + tj) * xcmax + xa;
                 v = (qq[0] * wx0 + qq[1] * wx1 + qq[2] * wx2 + qq[3] * wx3) * lx;
             }
             // inclusive prefix over the 64 lanes
             for (int off = 1; off < 64; off <<= 1) {
-                const double u = __shfl_up(v, off, 64);
+                const RT u = __shfl_up(v, off, 64);
                 if (lane >= off) v += u;
             }
-            const double base = carry[col];
-            if (kr < R1) p.vxr[ridx(p, kr + 1, J0 + tj, I0 + ti)] = base + v;
-            const double tot = __shfl(v, 63, 64);
+            const RT base = carry[col];
+            if (kr < R1) vxr[ridx(p, kr + 1, J0 + tj, I0 + ti)] = base + v;
+            const RT tot = __shfl(v, 63, 64);
             if (lane == 0) carry[col] = base + tot;
         }
     }
@@ -510,56 +523,58 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
 // Per chunk: B = coarse gradient block [xc][cs][co], O = contraction over the
 // other axis [xc][cs][line], X = contraction over x [cs][line][lane], then a
 // running sum along the scan axis in registers.
-template <int AX>
+template <int AX, typename RT>
 __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, int comax) {
-    extern __shared__ double sm[];
+    extern __shared__ __align__(16) unsigned char smraw[];
+    RT *sm = reinterpret_cast<RT *>(smraw);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x;
     const int nS = AX == 1 ? p.nyr : p.nzr, nO = AX == 1 ? p.nzr : p.nyr;          // refined extents
     const int cS = AX == 1 ? p.ny : p.nz, cO = AX == 1 ? p.nz : p.ny;              // coarse extents
     const int *sS = AX == 1 ? p.ys_src : p.zs_src, *sO = AX == 1 ? p.zs_src : p.ys_src;
-    const double *wS = AX == 1 ? p.ys_w : p.zs_w, *wO = AX == 1 ? p.zs_w : p.ys_w;
+    const RT *ysw = sel<RT>(p.ys_w, p.ys_wf), *zsw = sel<RT>(p.zs_w, p.zs_wf), *xs_w = sel<RT>(p.xs_w, p.xs_wf);
+    const RT *wS = AX == 1 ? ysw : zsw, *wO = AX == 1 ? zsw : ysw;
     const int R0 = blockIdx.x * 64, R1 = min(R0 + 64, p.nxmr);
     const int O0 = blockIdx.y * VYZ_LINES, nl = min(VYZ_LINES, nO - O0);
     const int kr = R0 + lane, line = O0 + wave;
     const bool live = wave < nl;
-    double *outp = AX == 1 ? p.vyr : p.vzr;
+    RT *outp = AX == 1 ? sel<RT>(p.vyr, p.vyrf) : sel<RT>(p.vzr, p.vzrf);
     const long long sstride = AX == 1 ? (long long)p.ldxr : p.planer;
     const long long obase = AX == 1 ? (long long)line * p.planer : (long long)line * p.ldxr;
     // the unused x index nxr of vyr / vzr stays zero (InterpVelMgrd.F90:29-31)
     if (blockIdx.x == gridDim.x - 1 && lane == 0 && live)
-        for (int s = 0; s < nS; ++s) outp[obase + s * sstride + p.nxmr] = 0.0;
+        for (int s = 0; s < nS; ++s) outp[obase + s * sstride + p.nxmr] = RT(0);
     const int x0 = p.xs_src[R0], XC = p.xs_src[R1 - 1] + 4 - x0;
     const int co0 = sO[O0] - 1, CO = sO[O0 + nl - 1] - 1 + 4 - co0;
-    double wx[4] = {0, 0, 0, 0}, wo[4] = {0, 0, 0, 0};
+    RT wx[4] = {0, 0, 0, 0}, wo[4] = {0, 0, 0, 0};
     int xa = 0, oa = 0;
-    if (kr < R1) { xa = p.xs_src[kr] - x0; for (int a = 0; a < 4; ++a) wx[a] = p.xs_w[4 * kr + a]; }
+    if (kr < R1) { xa = p.xs_src[kr] - x0; for (int a = 0; a < 4; ++a) wx[a] = xs_w[4 * kr + a]; }
     if (live) { oa = sO[line] - 1 - co0; for (int a = 0; a < 4; ++a) wo[a] = wO[4 * line + a]; }
     // first guess on the plane s = 0 (InterpVelMgrd.F90:93-123 / :179-209):
     // velocity plane with wall ghosts, contracted over the other axis, then x
-    double v = 0.0;
+    RT v = RT(0);
     if (live && kr < R1 && (AX == 1 || p.nz > 1)) {
         const int sSg = 1 - 2 * p.inslws, sNg = 1 - 2 * p.inslwn;
         for (int a = 0; a < 4; ++a) {
             const int kc = x0 + xa + a;
-            double q1 = 0.0;
+            RT q1 = RT(0);
             for (int c = 0; c < 4; ++c) {
                 const int o = wrapn(co0 + oa + c, cO);
                 const double *col = AX == 1 ? p.vy + (long long)o * p.plane : p.vz + (long long)o * p.ldx;
-                double f;
-                if (kc >= 1 && kc <= p.nxm) f = col[kc - 1];
-                else if (kc == 0) f = sSg * col[0];
-                else if (kc == p.nx) f = sNg * col[p.nxm - 1];
-                else f = 0.0;
+                RT f;
+                if (kc >= 1 && kc <= p.nxm) f = RT(col[kc - 1]);
+                else if (kc == 0) f = RT(sSg * col[0]);
+                else if (kc == p.nx) f = RT(sNg * col[p.nxm - 1]);
+                else f = RT(0);
                 q1 = c == 0 ? f * wo[0] : q1 + f * wo[c];
             }
             v = a == 0 ? q1 * wx[0] : v + q1 * wx[a];
         }
     }
     if (live && kr < R1) outp[obase + kr] = v;
-    const double dl = AX == 1 ? 1.0 / p.dyr : 1.0 / p.dzr;
-    double *B = sm;                                           // [co][cs][xc]
-    double *X = B;                                            // [cs][line][lane], reuses B
-    double *O = B + max(xcmax * csmax * comax, csmax * VYZ_LINES * 64);   // [line][cs][xc]
+    const RT dl = AX == 1 ? RT(1.0 / p.dyr) : RT(1.0 / p.dzr);
+    RT *B = sm;                                               // [co][cs][xc]
+    RT *X = B;                                                // [cs][line][lane], reuses B
+    RT *O = B + max(xcmax * csmax * comax, csmax * VYZ_LINES * 64);   // [line][cs][xc]
     for (int S0 = 0; S0 < nS - 1; S0 += VYZ_CHUNK) {
         const int S1 = min(S0 + VYZ_CHUNK, nS - 1);
         const int cs0 = sS[S0] - 1, CS = sS[S1 - 1] - 1 + 4 - cs0;
@@ -567,29 +582,29 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
         for (int q = t; q < XC * CS * CO; q += 256) {
             const int xc = q % XC, cs = (q / XC) % CS, co = q / (XC * CS);
             const int js = wrapn(cs0 + cs, cS), jo = wrapn(co0 + co, cO);
-            B[(co * csmax + cs) * xcmax + xc] = AX == 1 ? mgrd_grad(p, 1, x0 + xc, js, jo) : mgrd_grad(p, 2, x0 + xc, jo, js);
+            B[(co * csmax + cs) * xcmax + xc] = AX == 1 ? mgrd_grad<RT>(p, 1, x0 + xc, js, jo) : mgrd_grad<RT>(p, 2, x0 + xc, jo, js);
         }
         __syncthreads();
         for (int q = t; q < XC * CS * nl; q += 256) {
             const int xc = q % XC, cs = (q / XC) % CS, l = q / (XC * CS);
             const int c0 = sO[O0 + l] - 1 - co0;
-            const double *w = wO + 4 * (O0 + l), *b = B + (c0 * csmax + cs) * xcmax + xc;
+            const RT *w = wO + 4 * (O0 + l), *b = B + (c0 * csmax + cs) * xcmax + xc;
             const int st = csmax * xcmax;
             O[(l * csmax + cs) * xcmax + xc] = b[0] * w[0] + b[st] * w[1] + b[2 * st] * w[2] + b[3 * st] * w[3];
         }
         __syncthreads();
         if (live && kr < R1)
             for (int cs = 0; cs < CS; ++cs) {
-                const double *o = O + (wave * csmax + cs) * xcmax + xa;
+                const RT *o = O + (wave * csmax + cs) * xcmax + xa;
                 X[(cs * VYZ_LINES + wave) * 64 + lane] = o[0] * wx[0] + o[1] * wx[1] + o[2] * wx[2] + o[3] * wx[3];
             }
         __syncthreads();
         if (live && kr < R1)
             for (int s = S0; s < S1; ++s) {
                 const int b0 = sS[s] - 1 - cs0;
-                const double *w = wS + 4 * s, *xx = X + (b0 * VYZ_LINES + wave) * 64 + lane;
+                const RT *w = wS + 4 * s, *xx = X + (b0 * VYZ_LINES + wave) * 64 + lane;
                 const int st = VYZ_LINES * 64;
-                const double tv = xx[0] * w[0] + xx[st] * w[1] + xx[2 * st] * w[2] + xx[3 * st] * w[3];
+                const RT tv = xx[0] * w[0] + xx[st] * w[1] + xx[2 * st] * w[2] + xx[3 * st] * w[3];
                 v = v + tv * dl;
                 outp[obase + (s + 1) * sstride + kr] = v;
             }
@@ -604,15 +619,21 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
 // buffered), contracted over x and y, and the last four contracted planes are
 // kept in registers for the z contraction.
 constexpr int SALC_TJ = 8, SALC_TZ = 16;
-__device__ __forceinline__ double sal_ghosted(const KP &p, int kr, int jj, int ii) {
+template <typename RT>
+__device__ __forceinline__ RT sal_ghosted(const KP &p, const RT *sal, int kr, int jj, int ii) {
     const long long o2 = (long long)ii * p.planer + (long long)jj * p.ldxr;
-    if (kr >= 1 && kr <= p.nxmr) return p.sal[o2 + kr - 1];
-    if (kr == 0) { const double s1 = p.sal[o2]; return p.sfixs == 1 ? 2.0 * p.salbp[(long long)ii * p.nyr + jj] - s1 : s1; }
-    if (kr == p.nxr) { const double sn = p.sal[o2 + p.nxmr - 1]; return p.sfixn == 1 ? 2.0 * p.saltp[(long long)ii * p.nyr + jj] - sn : sn; }
-    return 0.0;
+    if (kr >= 1 && kr <= p.nxmr) return sal[o2 + kr - 1];
+    if (kr == 0) { const RT s1 = sal[o2]; return p.sfixs == 1 ? RT(2.0 * p.salbp[(long long)ii * p.nyr + jj]) - s1 : s1; }
+    if (kr == p.nxr) { const RT sn = sal[o2 + p.nxmr - 1]; return p.sfixn == 1 ? RT(2.0 * p.saltp[(long long)ii * p.nyr + jj]) - sn : sn; }
+    return RT(0);
 }
+template <typename RT>
 __global__ void __launch_bounds__(256) k_interp_sal_tile(KP p, int xrmax, int yrmax) {
-    extern __shared__ double sm[];
+    extern __shared__ __align__(16) unsigned char smraw[];
+    RT *sm = reinterpret_cast<RT *>(smraw);
+    const RT *sal = sel<RT>(p.sal, p.salf);
+    RT *salc = sel<RT>(p.salc, p.salcf);
+    const RT *xb_w = sel<RT>(p.xb_w, p.xb_wf), *yb_w = sel<RT>(p.yb_w, p.yb_wf), *zb_w = sel<RT>(p.zb_w, p.zb_wf);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x;
     const int K0 = blockIdx.x * 64, K1 = min(K0 + 64, p.nxm);
     const int J0 = blockIdx.y * SALC_TJ, nj = min(SALC_TJ, p.ny - J0);
@@ -622,35 +643,35 @@ __global__ void __launch_bounds__(256) k_interp_sal_tile(KP p, int xrmax, int yr
     const int zfirst = p.zb_src[I0] - 1, zlast = p.zb_src[I1 - 1] - 1 + 3;    // 0-based refined z (unwrapped)
     const int k = K0 + lane;
     const bool kok = k < K1;
-    double wx[4] = {0, 0, 0, 0};
+    RT wx[4] = {0, 0, 0, 0};
     int xa = 0;
-    if (kok) { xa = p.xb_src[k] - x0; for (int a = 0; a < 4; ++a) wx[a] = p.xb_w[4 * k + a]; }
+    if (kok) { xa = p.xb_src[k] - x0; for (int a = 0; a < 4; ++a) wx[a] = xb_w[4 * k + a]; }
     // two coarse rows per wave: tj = wave and wave + 4
     int ya[2] = {0, 0};
-    double wy[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    RT wy[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     bool rok[2];
     for (int h = 0; h < 2; ++h) {
         const int tj = wave + 4 * h;
         rok[h] = tj < nj;
-        if (rok[h]) { ya[h] = p.yb_src[J0 + tj] - 1 - y0; for (int b = 0; b < 4; ++b) wy[h][b] = p.yb_w[4 * (J0 + tj) + b]; }
+        if (rok[h]) { ya[h] = p.yb_src[J0 + tj] - 1 - y0; for (int b = 0; b < 4; ++b) wy[h][b] = yb_w[4 * (J0 + tj) + b]; }
     }
-    double ring[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};   // contracted refined planes zr-3..zr
+    RT ring[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};   // contracted refined planes zr-3..zr
     const int plane = XR * YR;
     int ic = I0;
     for (int zr = zfirst; zr <= zlast; ++zr) {
-        double *B = sm + ((zr - zfirst) & 1) * xrmax * yrmax;     // [yr][xr], double buffered
+        RT *B = sm + ((zr - zfirst) & 1) * xrmax * yrmax;     // [yr][xr], double buffered
         const int ii = wrapn(zr, p.nzr);
         for (int q = t; q < plane; q += 256) {
             const int xr = q % XR, yr = q / XR;
-            B[yr * xrmax + xr] = sal_ghosted(p, x0 + xr, wrapn(y0 + yr, p.nyr), ii);
+            B[yr * xrmax + xr] = sal_ghosted<RT>(p, sal, x0 + xr, wrapn(y0 + yr, p.nyr), ii);
         }
         __syncthreads();
         for (int h = 0; h < 2; ++h) {
-            double v = 0.0;
+            RT v = RT(0);
             if (kok && rok[h]) {
                 for (int b = 0; b < 4; ++b) {
-                    const double *row = B + (ya[h] + b) * xrmax + xa;
-                    const double q1 = row[0] * wx[0] + row[1] * wx[1] + row[2] * wx[2] + row[3] * wx[3];
+                    const RT *row = B + (ya[h] + b) * xrmax + xa;
+                    const RT q1 = row[0] * wx[0] + row[1] * wx[1] + row[2] * wx[2] + row[3] * wx[3];
                     v = b == 0 ? q1 * wy[h][0] : v + q1 * wy[h][b];
                 }
             }
@@ -658,10 +679,10 @@ __global__ void __launch_bounds__(256) k_interp_sal_tile(KP p, int xrmax, int yr
         }
         // every coarse plane whose 4-plane stencil ends at zr is complete
         while (ic < I1 && p.zb_src[ic] - 1 + 3 == zr) {
-            const double *wz = p.zb_w + 4 * ic;
+            const RT *wz = zb_w + 4 * ic;
             for (int h = 0; h < 2; ++h)
                 if (kok && rok[h])
-                    p.salc[bidx(p, k, J0 + wave + 4 * h, ic)] =
+                    salc[bidx(p, k, J0 + wave + 4 * h, ic)] =
                         ring[h][0] * wz[0] + ring[h][1] * wz[1] + ring[h][2] * wz[2] + ring[h][3] * wz[3];
             ++ic;
         }
@@ -670,7 +691,7 @@ __global__ void __launch_bounds__(256) k_interp_sal_tile(KP p, int xrmax, int yr
     if (blockIdx.x == gridDim.x - 1 && lane == 0)
         for (int h = 0; h < 2; ++h)
             if (rok[h])
-                for (int iz = I0; iz < I1; ++iz) p.salc[bidx(p, p.nxm, J0 + wave + 4 * h, iz)] = 0.0;
+                for (int iz = I0; iz < I1; ++iz) salc[bidx(p, p.nxm, J0 + wave + 4 * h, iz)] = RT(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -687,19 +708,20 @@ __device__ __forceinline__ double splitmix_uniform(unsigned long long seed, unsi
 // the walls as the reference does (CreateInitialConditions.F90:187-201); the
 // noise of point (k,j,i) is U[0,1) of splitmix64(seed ^ linear index) over the
 // (nxf, ny, nz) array, x fastest.
-__global__ void __launch_bounds__(256) k_init_scalar(double *f, const double *bot, const double *top, const double *xm,
+template <typename FT>
+__global__ void __launch_bounds__(256) k_init_scalar(FT *f, const double *bot, const double *top, const double *xm,
                                                      int nxm, int nxf, int ldx, int ny, int nz, double alx3,
                                                      double eps, unsigned long long seed) {
     const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
     if (k >= nxf || j >= ny) return;
     const long long o = ((long long)i * ny + j) * ldx + k;
-    if (k >= nxm) { f[o] = 0.0; return; }
+    if (k >= nxm) { f[o] = FT(0); return; }
     const long long p2 = (long long)i * ny + j;
     const double x = xm[k];
     double v = bot[p2] + (top[p2] - bot[p2]) * x / alx3;
     const double r = splitmix_uniform(seed, (unsigned long long)(((long long)i * ny + j) * nxf + k));
     const double amp = (fabs(x - 0.5) + eps > 0.5) ? 0.5 - fabs(x - 0.5) : eps;
-    f[o] = v + amp * (2.0 * r - 1.0);
+    f[o] = FT(v + amp * (2.0 * r - 1.0));
 }
 
 // ---------------------------------------------------------------------------
@@ -750,21 +772,23 @@ __global__ void __launch_bounds__(256) k_cfl_div(KP p, int what) {
     v = block_max(v);
     if (threadIdx.x == 0 && threadIdx.y == 0) atomicMax(&p.redmax[2 * what], (unsigned long long)__double_as_longlong(v));
 }
+template <typename RT>
 __global__ void __launch_bounds__(256) k_cfl_div_r(KP p, int what) {
     const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const RT *vxr = sel<RT>(p.vxr, p.vxrf), *vyr = sel<RT>(p.vyr, p.vyrf), *vzr = sel<RT>(p.vzr, p.vzrf);
     double v = 0.0;
     for (int i = blockIdx.z; i < p.nzr; i += gridDim.z) {
     if (k < p.nxmr && j < p.nyr) {
         const int jp = wrapp(j, p.nyr), ip = wrapp(i, p.nzr);
         double w;
         if (what == 0)
-            w = fabs((p.vzr[ridx(p, k, j, i)] + p.vzr[ridx(p, k, j, ip)]) * 0.5 * p.dzr)
-              + fabs((p.vyr[ridx(p, k, j, i)] + p.vyr[ridx(p, k, jp, i)]) * 0.5 * p.dyr)
-              + fabs((p.vxr[ridx(p, k, j, i)] + p.vxr[ridx(p, k + 1, j, i)]) * 0.5 * p.udx3mr[k]);
+            w = fabs((double(vzr[ridx(p, k, j, i)]) + double(vzr[ridx(p, k, j, ip)])) * 0.5 * p.dzr)
+              + fabs((double(vyr[ridx(p, k, j, i)]) + double(vyr[ridx(p, k, jp, i)])) * 0.5 * p.dyr)
+              + fabs((double(vxr[ridx(p, k, j, i)]) + double(vxr[ridx(p, k + 1, j, i)])) * 0.5 * p.udx3mr[k]);
         else
-            w = fabs((p.vzr[ridx(p, k, j, ip)] - p.vzr[ridx(p, k, j, i)]) * p.dzr
-                   + (p.vyr[ridx(p, k, jp, i)] - p.vyr[ridx(p, k, j, i)]) * p.dyr
-                   + (p.vxr[ridx(p, k + 1, j, i)] - p.vxr[ridx(p, k, j, i)]) * p.udx3mr[k]);
+            w = fabs((double(vzr[ridx(p, k, j, ip)]) - double(vzr[ridx(p, k, j, i)])) * p.dzr
+                   + (double(vyr[ridx(p, k, jp, i)]) - double(vyr[ridx(p, k, j, i)])) * p.dyr
+                   + (double(vxr[ridx(p, k + 1, j, i)]) - double(vxr[ridx(p, k, j, i)])) * p.udx3mr[k]);
         v = fmax(v, w);
     }
     }

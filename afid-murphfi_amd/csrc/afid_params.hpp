@@ -49,6 +49,10 @@ struct KP {
     // refined fields
     double *sal, *rusal, *hsal, *rhsr, *vxr, *vyr, *vzr;
     const double *salbp, *saltp;        // [nzr][nyr]
+    // mixed form: FP32 storage of the refined fields and of the base
+    // explicit-term history (null in the FP64 form)
+    float *salf, *rusalf, *vxrf, *vyrf, *vzrf;
+    float *ruxf, *ruyf, *ruzf, *rutf, *salcf;
     // base metrics (0-based by kc-1)
     const double *udx3m, *udx3c, *rudx3m;
     const double *ap3ck, *ac3ck, *am3ck, *ap3sk, *ac3sk, *am3sk, *ap3ssk, *ac3ssk, *am3ssk;
@@ -61,6 +65,8 @@ struct KP {
     // refined -> coarse (InterpSalMultigrid fast path), per coarse index
     const int *xb_src, *yb_src, *zb_src;
     const double *xb_w, *yb_w, *zb_w;
+    // FP32 copies of the Hermite weights and of lxa (mixed form)
+    const float *xs_wf, *ys_wf, *zs_wf, *xb_wf, *yb_wf, *zb_wf, *lxaf;
     // reductions
     unsigned long long *redmax;         // [4] max slots (bit patterns of non-negative doubles)
     double *redsum;                     // per-block partial sums

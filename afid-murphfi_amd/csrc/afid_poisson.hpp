@@ -19,166 +19,233 @@
 
 namespace afid {
 
+// complex value of real type R (double in the FP64 form, float in the mixed form)
+template <typename R>
+struct __align__(2 * sizeof(R)) cpx { R x, y; };
+template <typename R>
+__device__ __forceinline__ cpx<R> mk(R a, R b) { cpx<R> z; z.x = a; z.y = b; return z; }
+
+template <typename R>
 struct FftPlan {
     int n = 1;
     int nst = 0;
     int rad[12] = {0};
-    const double2 *tw = nullptr;   // exp(-2 pi i t/n), t = 0..n-1
+    const cpx<R> *tw = nullptr;   // exp(-2 pi i t/n), t = 0..n-1
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+template <typename R>
+__device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) { return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename R>
+__device__ __forceinline__ cpx<R> cadd(cpx<R> a, cpx<R> b) { return mk<R>(a.x + b.x, a.y + b.y); }
+template <typename R>
+__device__ __forceinline__ cpx<R> csub(cpx<R> a, cpx<R> b) { return mk<R>(a.x - b.x, a.y - b.y); }
 // multiply by i*sign
-__device__ __forceinline__ double2 cmulis(double2 a, int sign) {
-    return sign < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
-}
+template <typename R>
+__device__ __forceinline__ cpx<R> cmulis(cpx<R> a, int sign) { return sign < 0 ? mk<R>(a.y, -a.x) : mk<R>(-a.y, a.x); }
 
-template <int R>
-__device__ __forceinline__ void dft_small(double2 (&v)[R], int sign);
+template <int N, typename R>
+struct DftSmall;
+template <typename R>
+struct DftSmall<2, R> {
+    static __device__ __forceinline__ void run(cpx<R> (&v)[2], int) {
+        cpx<R> a = v[0], b = v[1];
+        v[0] = cadd(a, b); v[1] = csub(a, b);
+    }
+};
+template <typename R>
+struct DftSmall<3, R> {
+    static __device__ __forceinline__ void run(cpx<R> (&v)[3], int sign) {
+        const R s3 = R(0.86602540378443864676);   // sin(2 pi/3)
+        cpx<R> t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        cpx<R> m = mk<R>(v[0].x - R(0.5) * t1.x, v[0].y - R(0.5) * t1.y);
+        cpx<R> r = cmulis(mk<R>(s3 * t2.x, s3 * t2.y), sign);
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(m, r);
+        v[2] = csub(m, r);
+    }
+};
+template <typename R>
+struct DftSmall<4, R> {
+    static __device__ __forceinline__ void run(cpx<R> (&v)[4], int sign) {
+        cpx<R> a = cadd(v[0], v[2]), b = csub(v[0], v[2]), c = cadd(v[1], v[3]), d = cmulis(csub(v[1], v[3]), sign);
+        v[0] = cadd(a, c); v[2] = csub(a, c);
+        v[1] = cadd(b, d); v[3] = csub(b, d);
+    }
+};
+template <typename R>
+struct DftSmall<8, R> {
+    static __device__ __forceinline__ void run(cpx<R> (&v)[8], int sign) {
+        const R h = R(0.70710678118654752440);    // 1/sqrt(2)
+        cpx<R> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        DftSmall<4, R>::run(e, sign);
+        DftSmall<4, R>::run(o, sign);
+        // w8^k = exp(sign 2 pi i k/8)
+        cpx<R> w1 = mk<R>(h, R(sign) * h), w3 = mk<R>(-h, R(sign) * h);
+        cpx<R> t0 = o[0], t1 = cmul(o[1], w1), t2 = cmulis(o[2], sign), t3 = cmul(o[3], w3);
+        v[0] = cadd(e[0], t0); v[4] = csub(e[0], t0);
+        v[1] = cadd(e[1], t1); v[5] = csub(e[1], t1);
+        v[2] = cadd(e[2], t2); v[6] = csub(e[2], t2);
+        v[3] = cadd(e[3], t3); v[7] = csub(e[3], t3);
+    }
+};
 
-template <>
-__device__ __forceinline__ void dft_small<2>(double2 (&v)[2], int) {
-    double2 a = v[0], b = v[1];
-    v[0] = cadd(a, b); v[1] = csub(a, b);
-}
-template <>
-__device__ __forceinline__ void dft_small<3>(double2 (&v)[3], int sign) {
-    const double s3 = 0.86602540378443864676;   // sin(2 pi/3)
-    double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-    double2 m = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
-    double2 r = cmulis(make_double2(s3 * t2.x, s3 * t2.y), sign);
-    v[0] = cadd(v[0], t1);
-    v[1] = cadd(m, r);
-    v[2] = csub(m, r);
-}
-template <>
-__device__ __forceinline__ void dft_small<4>(double2 (&v)[4], int sign) {
-    double2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]), c = cadd(v[1], v[3]), d = cmulis(csub(v[1], v[3]), sign);
-    v[0] = cadd(a, c); v[2] = csub(a, c);
-    v[1] = cadd(b, d); v[3] = csub(b, d);
-}
-template <>
-__device__ __forceinline__ void dft_small<8>(double2 (&v)[8], int sign) {
-    const double h = 0.70710678118654752440;    // 1/sqrt(2)
-    double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
-    dft_small<4>(e, sign);
-    dft_small<4>(o, sign);
-    // w8^k = exp(sign 2 pi i k/8)
-    double2 w1 = make_double2(h, sign * h), w3 = make_double2(-h, sign * h);
-    double2 t0 = o[0], t1 = cmul(o[1], w1), t2 = cmulis(o[2], sign), t3 = cmul(o[3], w3);
-    v[0] = cadd(e[0], t0); v[4] = csub(e[0], t0);
-    v[1] = cadd(e[1], t1); v[5] = csub(e[1], t1);
-    v[2] = cadd(e[2], t2); v[6] = csub(e[2], t2);
-    v[3] = cadd(e[3], t3); v[7] = csub(e[3], t3);
-}
-
-// One Stockham stage of radix R over L lines of length N held in LDS.
+// One Stockham stage of radix RD over L lines of length N held in LDS.
 // Each thread keeps at most MAXB butterflies of the stage in registers
 // between the read and the write phase, so the transform runs in place.
-template <int R, int MAXB>
-__device__ __forceinline__ void fft_stage(double2 *buf, int L, int N, int Ns, const double2 *tw, int sign) {
-    const int nbl = N / R, nb = L * nbl, T = blockDim.x;
-    double2 v[MAXB][R];
+template <int RD, int MAXB, typename R>
+__device__ __forceinline__ void fft_stage(cpx<R> *buf, int L, int N, int Ns, const cpx<R> *tw, int sign) {
+    const int nbl = N / RD, nb = L * nbl, T = blockDim.x;
+    cpx<R> v[MAXB][RD];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
         const int b = threadIdx.x + q * T;
         if (b < nb) {
             const int line = b / nbl, jj = b - line * nbl;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = buf[line * N + jj + r * nbl];
+            for (int r = 0; r < RD; ++r) v[q][r] = buf[line * N + jj + r * nbl];
         }
     }
     __syncthreads();
-    const int tstride = N / (Ns * R);
+    const int tstride = N / (Ns * RD);
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
         const int b = threadIdx.x + q * T;
         if (b < nb) {
             const int line = b / nbl, jj = b - line * nbl, kk = jj % Ns;
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                double2 w = tw[kk * r * tstride];
+            for (int r = 1; r < RD; ++r) {
+                cpx<R> w = tw[kk * r * tstride];
                 if (sign > 0) w.y = -w.y;
                 v[q][r] = cmul(v[q][r], w);
             }
-            dft_small<R>(v[q], sign);
-            const int dst = (jj / Ns) * Ns * R + kk;
+            DftSmall<RD, R>::run(v[q], sign);
+            const int dst = (jj / Ns) * Ns * RD + kk;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[line * N + dst + r * Ns] = v[q][r];
+            for (int r = 0; r < RD; ++r) buf[line * N + dst + r * Ns] = v[q][r];
         }
     }
     __syncthreads();
 }
 
 // Full transform of L lines; at most 32 complex values per thread (L*N <= 32*blockDim).
-__device__ __forceinline__ void lds_fft(double2 *buf, int L, const FftPlan &pl, int sign) {
+template <typename R>
+__device__ __forceinline__ void lds_fft(cpx<R> *buf, int L, const FftPlan<R> &pl, int sign) {
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
-        const int R = pl.rad[s];
-        switch (R) {
-        case 8: fft_stage<8, 4>(buf, L, pl.n, Ns, pl.tw, sign); break;
-        case 4: fft_stage<4, 8>(buf, L, pl.n, Ns, pl.tw, sign); break;
-        case 3: fft_stage<3, 11>(buf, L, pl.n, Ns, pl.tw, sign); break;
-        default: fft_stage<2, 16>(buf, L, pl.n, Ns, pl.tw, sign); break;
+        const int RD = pl.rad[s];
+        switch (RD) {
+        case 8: fft_stage<8, 4, R>(buf, L, pl.n, Ns, pl.tw, sign); break;
+        case 4: fft_stage<4, 8, R>(buf, L, pl.n, Ns, pl.tw, sign); break;
+        case 3: fft_stage<3, 11, R>(buf, L, pl.n, Ns, pl.tw, sign); break;
+        default: fft_stage<2, 16, R>(buf, L, pl.n, Ns, pl.tw, sign); break;
         }
-        Ns *= R;
+        Ns *= RD;
     }
 }
 
+// Refinement gate of the mixed form: a launch of a refinement sweep whose
+// gate slot holds a divergence at or below tol returns at once.  Slots hold
+// the bit pattern of a non-negative double (ordered like the value).
+struct Gate {
+    const unsigned long long *slot;   // null: always run
+    double tol;
+};
+__device__ __forceinline__ bool gate_closed(const Gate &g) {
+    return g.slot && __longlong_as_double((long long)*g.slot) <= g.tol;
+}
+
+// Divergence (CheckDivergence units, CheckDivergence.F90:29-41) of the
+// velocity that the projection with phi (RemoveDivergence, pressure.F90:
+// 245-266, as k_project applies it) would leave at cell (k, j, i): the FP64
+// residual of the mixed form's iterative refinement.
+__device__ __forceinline__ double div_after_projection(const KP &p, const double *phi, int k, int j, int i) {
+    const int jm = wrapm(j, p.ny), jp = wrapp(j, p.ny), im = wrapm(i, p.nz), ip = wrapp(i, p.nz);
+    const double adt = p.al * p.dt;
+    const long long c = bidx(p, k, j, i);
+    const double pc = phi[c];
+    double vx0 = p.vx[c], vx1 = p.vx[c + 1];
+    if (k >= 1) vx0 -= (pc - phi[c - 1]) * (adt * p.udx3c[k]);
+    if (k + 1 <= p.nxm - 1) vx1 -= (phi[c + 1] - pc) * (adt * p.udx3c[k + 1]);
+    const double vy0 = p.vy[c] - (pc - phi[bidx(p, k, jm, i)]) * (adt * p.dy);
+    const double vy1 = p.vy[bidx(p, k, jp, i)] - (phi[bidx(p, k, jp, i)] - pc) * (adt * p.dy);
+    const double vz0 = p.vz[c] - (pc - phi[bidx(p, k, j, im)]) * (adt * p.dz);
+    const double vz1 = p.vz[bidx(p, k, j, ip)] - (phi[bidx(p, k, j, ip)] - pc) * (adt * p.dz);
+    return (vz1 - vz0) * p.dz + (vy1 - vy0) * p.dy + (vx1 - vx0) * p.udx3m[k];
+}
+
+// Source of the forward transform
+enum PoissonSrc { SRC_VELOCITY = 0, SRC_DPH = 1, SRC_RESIDUAL = 2 };
+
 // Divergence (CalcLocalDivergence, pressure.F90:147-161) fused with the real
 // forward y transform (pressure.F90:171-174) and the 1/(ny nz) scaling.
+// SRC_RESIDUAL takes the divergence left by a projection with the current
+// dph (div_after_projection) and max-reduces its magnitude into *slot_out.
 // Block: 2L x-adjacent real lines at one z.  grid (ceil(nxm/2L), nz).
-__global__ void __launch_bounds__(256) k_div_fft_y(KP p, FftPlan pl, double2 *spec, int L, int from_dph) {
-    extern __shared__ double2 lb[];
+template <typename R>
+__global__ void __launch_bounds__(256) k_div_fft_y(KP p, FftPlan<R> pl, cpx<R> *spec, int L, int src, Gate gate,
+                                                   unsigned long long *slot_out, unsigned long long *slots_clear) {
+    if (gate_closed(gate)) return;
+    extern __shared__ __align__(16) unsigned char smraw[];
+    cpx<R> *lb = reinterpret_cast<cpx<R> *>(smraw);
     const int N = p.ny, TX = 2 * L, i = blockIdx.y, k0 = blockIdx.x * TX;
     const int ip = wrapp(i, p.nz);
     const double usdtal = 1.0 / (p.dt * p.al);
-    double *lr = reinterpret_cast<double *>(lb);
+    if (slots_clear && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) slots_clear[threadIdx.x] = 0ull;
+    R *lr = reinterpret_cast<R *>(lb);
+    double dmax = 0.0;
     for (int t = threadIdx.x; t < TX * N; t += blockDim.x) {
         const int xo = t % TX, j = t / TX, k = k0 + xo;
         double d = 0.0;
         if (k < p.nxm) {
-            if (from_dph) d = p.dph[bidx(p, k, j, i)];
-            else {
+            if (src == SRC_DPH) d = p.dph[bidx(p, k, j, i)];
+            else if (src == SRC_RESIDUAL) {
+                const double dv = div_after_projection(p, p.dph, k, j, i);
+                dmax = fmax(dmax, fabs(dv));
+                d = dv * usdtal;
+            } else {
                 const int jp = wrapp(j, N);
                 d = ((p.vz[bidx(p, k, j, ip)] - p.vz[bidx(p, k, j, i)]) * p.dz
                    + (p.vy[bidx(p, k, jp, i)] - p.vy[bidx(p, k, j, i)]) * p.dy
                    + (p.vx[bidx(p, k + 1, j, i)] - p.vx[bidx(p, k, j, i)]) * p.udx3m[k]) * usdtal;
             }
         }
-        lr[2 * ((xo >> 1) * N + j) + (xo & 1)] = d;
+        lr[2 * ((xo >> 1) * N + j) + (xo & 1)] = R(d);
+    }
+    if (src == SRC_RESIDUAL) {
+        dmax = block_max(dmax);
+        if (threadIdx.x == 0) atomicMax(slot_out, (unsigned long long)__double_as_longlong(dmax));
     }
     __syncthreads();
-    lds_fft(lb, L, pl, -1);
-    const double scale = 1.0 / (double(p.ny) * double(p.nz));
+    lds_fft<R>(lb, L, pl, -1);
+    const R scale = R(1.0 / (double(p.ny) * double(p.nz)));
     for (int t = threadIdx.x; t < TX * p.nymh; t += blockDim.x) {
         const int xo = t % TX, m = t / TX, k = k0 + xo;
         if (k >= p.nxm) continue;
         const int line = xo >> 1;
-        const double2 zm = lb[line * N + m], zn = lb[line * N + (m == 0 ? 0 : N - m)];
-        double2 out;
-        if ((xo & 1) == 0) out = make_double2(0.5 * (zm.x + zn.x), 0.5 * (zm.y - zn.y));     // (Z_m + conj Z_{N-m})/2
-        else out = make_double2(0.5 * (zm.y + zn.y), -0.5 * (zm.x - zn.x));                   // (Z_m - conj Z_{N-m})/(2i)
-        spec[((long long)i * p.nymh + m) * p.nxm + k] = make_double2(out.x * scale, out.y * scale);
+        const cpx<R> zm = lb[line * N + m], zn = lb[line * N + (m == 0 ? 0 : N - m)];
+        cpx<R> out;
+        if ((xo & 1) == 0) out = mk<R>(R(0.5) * (zm.x + zn.x), R(0.5) * (zm.y - zn.y));     // (Z_m + conj Z_{N-m})/2
+        else out = mk<R>(R(0.5) * (zm.y + zn.y), R(-0.5) * (zm.x - zn.x));                  // (Z_m - conj Z_{N-m})/(2i)
+        spec[((long long)i * p.nymh + m) * p.nxm + k] = mk<R>(out.x * scale, out.y * scale);
     }
 }
 
 // Complex transform along z (pressure.F90:176 forward, :189 backward), in
 // place on the spectral array.  Block: L x-adjacent lines at one y-mode m.
-__global__ void __launch_bounds__(256) k_fft_z(KP p, FftPlan pl, double2 *spec, int L, int sign) {
-    extern __shared__ double2 lb[];
+template <typename R>
+__global__ void __launch_bounds__(256) k_fft_z(KP p, FftPlan<R> pl, cpx<R> *spec, int L, int sign, Gate gate) {
+    if (gate_closed(gate)) return;
+    extern __shared__ __align__(16) unsigned char smraw[];
+    cpx<R> *lb = reinterpret_cast<cpx<R> *>(smraw);
     const int N = p.nz, m = blockIdx.y, k0 = blockIdx.x * L;
     const long long sz = (long long)p.nymh * p.nxm;
-    double2 *g = spec + (long long)m * p.nxm;
+    cpx<R> *g = spec + (long long)m * p.nxm;
     for (int t = threadIdx.x; t < L * N; t += blockDim.x) {
         const int xo = t % L, i = t / L, k = k0 + xo;
-        lb[xo * N + i] = k < p.nxm ? g[i * sz + k] : make_double2(0.0, 0.0);
+        lb[xo * N + i] = k < p.nxm ? g[i * sz + k] : mk<R>(R(0), R(0));
     }
     __syncthreads();
-    lds_fft(lb, L, pl, sign);
+    lds_fft<R>(lb, L, pl, sign);
     for (int t = threadIdx.x; t < L * N; t += blockDim.x) {
         const int xo = t % L, i = t / L, k = k0 + xo;
         if (k < p.nxm) g[i * sz + k] = lb[xo * N + i];
@@ -187,27 +254,33 @@ __global__ void __launch_bounds__(256) k_fft_z(KP p, FftPlan pl, double2 *spec, 
 
 // Inverse real y transform (pressure.F90:193-195): two x-adjacent spectra
 // A, B (Hermitian, imaginary parts of the m=0 and m=N/2 terms ignored as a
-// c2r transform does) become Z = A + iB, whose inverse is a + ib.
-__global__ void __launch_bounds__(256) k_ifft_y(KP p, FftPlan pl, const double2 *spec, int L) {
-    extern __shared__ double2 lb[];
+// c2r transform does) become Z = A + iB, whose inverse is a + ib.  The result
+// goes to dph (accumulate = 0) or is added to it (refinement sweeps).
+template <typename R>
+__global__ void __launch_bounds__(256) k_ifft_y(KP p, FftPlan<R> pl, const cpx<R> *spec, int L, int accumulate, Gate gate) {
+    if (gate_closed(gate)) return;
+    extern __shared__ __align__(16) unsigned char smraw[];
+    cpx<R> *lb = reinterpret_cast<cpx<R> *>(smraw);
     const int N = p.ny, TX = 2 * L, i = blockIdx.y, k0 = blockIdx.x * TX;
     for (int t = threadIdx.x; t < L * p.nymh; t += blockDim.x) {
         const int line = t % L, m = t / L;
         const int ka = k0 + 2 * line, kb = ka + 1;
-        const double2 *row = spec + ((long long)i * p.nymh + m) * p.nxm;
-        double2 A = ka < p.nxm ? row[ka] : make_double2(0.0, 0.0);
-        double2 B = kb < p.nxm ? row[kb] : make_double2(0.0, 0.0);
-        if (m == 0 || 2 * m == N) { A.y = 0.0; B.y = 0.0; }
-        lb[line * N + m] = make_double2(A.x - B.y, A.y + B.x);
-        if (m > 0 && N - m > m) lb[line * N + (N - m)] = make_double2(A.x + B.y, B.x - A.y);
+        const cpx<R> *row = spec + ((long long)i * p.nymh + m) * p.nxm;
+        cpx<R> A = ka < p.nxm ? row[ka] : mk<R>(R(0), R(0));
+        cpx<R> B = kb < p.nxm ? row[kb] : mk<R>(R(0), R(0));
+        if (m == 0 || 2 * m == N) { A.y = R(0); B.y = R(0); }
+        lb[line * N + m] = mk<R>(A.x - B.y, A.y + B.x);
+        if (m > 0 && N - m > m) lb[line * N + (N - m)] = mk<R>(A.x + B.y, B.x - A.y);
     }
     __syncthreads();
-    lds_fft(lb, L, pl, +1);
+    lds_fft<R>(lb, L, pl, +1);
     for (int t = threadIdx.x; t < TX * N; t += blockDim.x) {
         const int xo = t % TX, j = t / TX, k = k0 + xo;
         if (k >= p.nxm) continue;
-        const double2 z = lb[(xo >> 1) * N + j];
-        p.dph[bidx(p, k, j, i)] = (xo & 1) ? z.y : z.x;
+        const cpx<R> z = lb[(xo >> 1) * N + j];
+        const double v = double((xo & 1) ? z.y : z.x);
+        double *d = p.dph + bidx(p, k, j, i);
+        *d = accumulate ? *d + v : v;
     }
 }
 
@@ -215,56 +288,58 @@ __global__ void __launch_bounds__(256) k_ifft_y(KP p, FftPlan pl, const double2 
 // pressure.F90:201-236) with factors precomputed at set-up.  Modes i and
 // nz-i share a matrix (ak1 is even), so one block of two wavefronts solves
 // both against one copy of the factors.  grid (nymh, nz/2+1).
+template <typename R>
 struct PoissonFactors {
-    const double *m, *g, *e;   // [mode'][row padded to 64*CH], mode' = m*(nz/2+1) + i'
+    const R *m, *g, *e;   // [mode'][row padded to 64*CH], mode' = m*(nz/2+1) + i'
     int npad;
 };
-template <int CH>
-__global__ void __launch_bounds__(128) k_poisson_tri(KP p, PoissonFactors pf, double2 *spec) {
+template <int CH, typename R>
+__global__ void __launch_bounds__(128) k_poisson_tri(KP p, PoissonFactors<R> pf, cpx<R> *spec, Gate gate) {
+    if (gate_closed(gate)) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = blockIdx.x, ih = blockIdx.y, nzh = p.nz / 2 + 1;
     const int i = wave == 0 ? ih : p.nz - ih;
     if (wave == 1 && (ih == 0 || i == ih || i >= p.nz)) return;
     const long long fo = ((long long)m * nzh + ih) * pf.npad;
-    double2 *col = spec + ((long long)i * p.nymh + m) * p.nxm;
-    double fm[CH], fg[CH], fe[CH];
-    double2 v[CH];
+    cpx<R> *col = spec + ((long long)i * p.nymh + m) * p.nxm;
+    R fm[CH], fg[CH], fe[CH];
+    cpx<R> v[CH];
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
         const int r = lane * CH + q;
         fm[q] = pf.m[fo + r]; fg[q] = pf.g[fo + r]; fe[q] = pf.e[fo + r];
-        v[q] = r < p.nxm ? col[r] : make_double2(0.0, 0.0);
+        v[q] = r < p.nxm ? col[r] : mk<R>(R(0), R(0));
     }
     // forward y_r = d_r - m_r y_{r-1} (complex B, real A)
-    double A = 1.0;
-    double2 Bc = make_double2(0.0, 0.0);
+    R A = R(1);
+    cpx<R> Bc = mk<R>(R(0), R(0));
 #pragma unroll
-    for (int q = 0; q < CH; ++q) { Bc = make_double2(-fm[q] * Bc.x + v[q].x, -fm[q] * Bc.y + v[q].y); A = -fm[q] * A; }
+    for (int q = 0; q < CH; ++q) { Bc = mk<R>(-fm[q] * Bc.x + v[q].x, -fm[q] * Bc.y + v[q].y); A = -fm[q] * A; }
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        double Ap = __shfl_up(A, off, 64), Bx = __shfl_up(Bc.x, off, 64), By = __shfl_up(Bc.y, off, 64);
-        if (lane >= off) { Bc = make_double2(A * Bx + Bc.x, A * By + Bc.y); A = A * Ap; }
+        R Ap = __shfl_up(A, off, 64), Bx = __shfl_up(Bc.x, off, 64), By = __shfl_up(Bc.y, off, 64);
+        if (lane >= off) { Bc = mk<R>(A * Bx + Bc.x, A * By + Bc.y); A = A * Ap; }
     }
-    double2 y = make_double2(__shfl_up(Bc.x, 1, 64), __shfl_up(Bc.y, 1, 64));
-    if (lane == 0) y = make_double2(0.0, 0.0);
+    cpx<R> y = mk<R>(__shfl_up(Bc.x, 1, 64), __shfl_up(Bc.y, 1, 64));
+    if (lane == 0) y = mk<R>(R(0), R(0));
 #pragma unroll
-    for (int q = 0; q < CH; ++q) { y = make_double2(-fm[q] * y.x + v[q].x, -fm[q] * y.y + v[q].y); v[q] = y; }
+    for (int q = 0; q < CH; ++q) { y = mk<R>(-fm[q] * y.x + v[q].x, -fm[q] * y.y + v[q].y); v[q] = y; }
     // backward x_r = g_r y_r - e_r x_{r+1}
-    A = 1.0; Bc = make_double2(0.0, 0.0);
+    A = R(1); Bc = mk<R>(R(0), R(0));
 #pragma unroll
     for (int q = CH - 1; q >= 0; --q) {
-        Bc = make_double2(-fe[q] * Bc.x + fg[q] * v[q].x, -fe[q] * Bc.y + fg[q] * v[q].y); A = -fe[q] * A;
+        Bc = mk<R>(-fe[q] * Bc.x + fg[q] * v[q].x, -fe[q] * Bc.y + fg[q] * v[q].y); A = -fe[q] * A;
     }
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        double Ap = __shfl_down(A, off, 64), Bx = __shfl_down(Bc.x, off, 64), By = __shfl_down(Bc.y, off, 64);
-        if (lane + off < 64) { Bc = make_double2(A * Bx + Bc.x, A * By + Bc.y); A = A * Ap; }
+        R Ap = __shfl_down(A, off, 64), Bx = __shfl_down(Bc.x, off, 64), By = __shfl_down(Bc.y, off, 64);
+        if (lane + off < 64) { Bc = mk<R>(A * Bx + Bc.x, A * By + Bc.y); A = A * Ap; }
     }
-    double2 x = make_double2(__shfl_down(Bc.x, 1, 64), __shfl_down(Bc.y, 1, 64));
-    if (lane == 63) x = make_double2(0.0, 0.0);
+    cpx<R> x = mk<R>(__shfl_down(Bc.x, 1, 64), __shfl_down(Bc.y, 1, 64));
+    if (lane == 63) x = mk<R>(R(0), R(0));
 #pragma unroll
     for (int q = CH - 1; q >= 0; --q) {
-        x = make_double2(-fe[q] * x.x + fg[q] * v[q].x, -fe[q] * x.y + fg[q] * v[q].y); v[q] = x;
+        x = mk<R>(-fe[q] * x.x + fg[q] * v[q].x, -fe[q] * x.y + fg[q] * v[q].y); v[q] = x;
     }
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
@@ -302,6 +377,12 @@ __global__ void k_poisson_factors(KP p, const double *ak1, const double *ak2, do
         fe[fo + k] = c / u;
         u_prev = u; c_prev = c; s_prev = s;
     }
+}
+
+// FP64 -> FP32 copy (factor tables and weights of the mixed form)
+__global__ void k_to_float(const double *src, float *dst, long long n) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+        dst[q] = float(src[q]);
 }
 
 }  // namespace afid

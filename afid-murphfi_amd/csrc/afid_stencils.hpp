@@ -49,76 +49,80 @@ __device__ __forceinline__ Nb make_nb(const KP &p, int j, int i) {
 }
 
 // ExplicitTermsVX.F90:42-127, valid for k = 1..nxm-1 (kc = 2..nxm)
-__device__ __forceinline__ double ex_vx(const KP &p, const Nb &n, unsigned k) {
-    const double udy = p.dy * 0.25, udz = p.dz * 0.25, udyq = p.dyq / p.ren, udzq = p.dzq / p.ren;
-    const double rkm = p.rudx3m[k - 1], rkc = p.rudx3m[k], u3 = p.udx3c[k];
-    const double vxc = n.vx[k];
-    const double vxzp = n.vx_ip[k], vxzm = n.vx_im[k], vxyp = n.vx_jp[k], vxym = n.vx_jm[k];
-    double hxz = u3 * udz * ((rkm * n.vz_ip[k - 1] + rkc * n.vz_ip[k]) * (vxzp + vxc)
-                           - (rkm * n.vz[k - 1] + rkc * n.vz[k]) * (vxc + vxzm));
-    double hxy = u3 * udy * ((rkm * n.vy_jp[k - 1] + rkc * n.vy_jp[k]) * (vxyp + vxc)
-                           - (rkm * n.vy[k - 1] + rkc * n.vy[k]) * (vxc + vxym));
-    const double a = n.vx[k + 1] + vxc, b = vxc + n.vx[k - 1];
-    double hxx = (a * a - b * b) * u3 * 0.25;
-    double dzzvx = (vxzm - 2.0 * vxc + vxzp) * udzq;
-    double dyyvx = (vxym - 2.0 * vxc + vxyp) * udyq;
-    double q = -(hxx + hxy + hxz) + dyyvx + dzzvx;
-    if (p.gaxis == 1) q += p.byct * (p.active_t * 0.5 * (n.t[k] + n.t[k - 1]));
+template <typename T>
+__device__ __forceinline__ T ex_vx(const KP &p, const Nb &n, unsigned k) {
+    const T udy = T(p.dy) * T(0.25), udz = T(p.dz) * T(0.25), udyq = T(p.dyq) / T(p.ren), udzq = T(p.dzq) / T(p.ren);
+    const T rkm = T(p.rudx3m[k - 1]), rkc = T(p.rudx3m[k]), u3 = T(p.udx3c[k]);
+    const T vxc = T(n.vx[k]);
+    const T vxzp = T(n.vx_ip[k]), vxzm = T(n.vx_im[k]), vxyp = T(n.vx_jp[k]), vxym = T(n.vx_jm[k]);
+    T hxz = u3 * udz * ((rkm * T(n.vz_ip[k - 1]) + rkc * T(n.vz_ip[k])) * (vxzp + vxc)
+                           - (rkm * T(n.vz[k - 1]) + rkc * T(n.vz[k])) * (vxc + vxzm));
+    T hxy = u3 * udy * ((rkm * T(n.vy_jp[k - 1]) + rkc * T(n.vy_jp[k])) * (vxyp + vxc)
+                           - (rkm * T(n.vy[k - 1]) + rkc * T(n.vy[k])) * (vxc + vxym));
+    const T a = T(n.vx[k + 1]) + vxc, b = vxc + T(n.vx[k - 1]);
+    T hxx = (a * a - b * b) * u3 * T(0.25);
+    T dzzvx = (vxzm - T(2.0) * vxc + vxzp) * udzq;
+    T dyyvx = (vxym - T(2.0) * vxc + vxyp) * udyq;
+    T q = -(hxx + hxy + hxz) + dyyvx + dzzvx;
+    if (p.gaxis == 1) q += T(p.byct) * (T(p.active_t) * T(0.5) * (T(n.t[k]) + T(n.t[k - 1])));
     return q;
 }
 
 // ExplicitTermsVY.F90:50-118, k = 0..nxm-1
-__device__ __forceinline__ double ex_vy(const KP &p, const Nb &n, unsigned k) {
-    const double udyq = p.dyq / p.ren, udzq = p.dzq / p.ren, udy = p.dy * 0.25, udz = p.dz * 0.25;
+template <typename T>
+__device__ __forceinline__ T ex_vy(const KP &p, const Nb &n, unsigned k) {
+    const T udyq = T(p.dyq) / T(p.ren), udzq = T(p.dzq) / T(p.ren), udy = T(p.dy) * T(0.25), udz = T(p.dz) * T(0.25);
     const unsigned kmm = k == 0 ? 0 : k - 1, kpp = k == unsigned(p.nxm - 1) ? k : k + 1;
-    const double vyc = n.vy[k];
-    const double vyyp = n.vy_jp[k], vyym = n.vy_jm[k], vyzp = n.vy_ip[k], vyzm = n.vy_im[k];
-    double hyx = ((n.vx[k + 1] + n.vx_jm[k + 1]) * (n.vy[kpp] + vyc)
-                - (n.vx[k] + n.vx_jm[k]) * (vyc + n.vy[kmm])) * p.udx3m[k] * 0.25;
-    const double a = vyyp + vyc, b = vyym + vyc;
-    double hyy = (a * a - b * b) * udy;
-    double hyz = ((vyzp + vyc) * (n.vz_ip[k] + n.vz_jm_ip[k]) - (vyc + vyzm) * (n.vz[k] + n.vz_jm[k])) * udz;
-    double dyyvy = (vyyp - 2.0 * vyc + vyym) * udyq;
-    double dzzvy = (vyzp - 2.0 * vyc + vyzm) * udzq;
-    double q = -(hyx + hyy + hyz) + dyyvy + dzzvy + p.Gy;
-    if (p.gaxis == 2) q += p.byct * (p.active_t * 0.5 * (n.t[k] + n.t_jm[k]));
+    const T vyc = T(n.vy[k]);
+    const T vyyp = T(n.vy_jp[k]), vyym = T(n.vy_jm[k]), vyzp = T(n.vy_ip[k]), vyzm = T(n.vy_im[k]);
+    T hyx = ((T(n.vx[k + 1]) + T(n.vx_jm[k + 1])) * (T(n.vy[kpp]) + vyc)
+                - (T(n.vx[k]) + T(n.vx_jm[k])) * (vyc + T(n.vy[kmm]))) * T(p.udx3m[k]) * T(0.25);
+    const T a = vyyp + vyc, b = vyym + vyc;
+    T hyy = (a * a - b * b) * udy;
+    T hyz = ((vyzp + vyc) * (T(n.vz_ip[k]) + T(n.vz_jm_ip[k])) - (vyc + vyzm) * (T(n.vz[k]) + T(n.vz_jm[k]))) * udz;
+    T dyyvy = (vyyp - T(2.0) * vyc + vyym) * udyq;
+    T dzzvy = (vyzp - T(2.0) * vyc + vyzm) * udzq;
+    T q = -(hyx + hyy + hyz) + dyyvy + dzzvy + T(p.Gy);
+    if (p.gaxis == 2) q += T(p.byct) * (T(p.active_t) * T(0.5) * (T(n.t[k]) + T(n.t_jm[k])));
     return q;
 }
 
 // ExplicitTermsVZ.F90:45-114, k = 0..nxm-1
-__device__ __forceinline__ double ex_vz(const KP &p, const Nb &n, unsigned k) {
-    const double udyq = p.dyq / p.ren, udzq = p.dzq / p.ren, udy = p.dy * 0.25, udz = p.dz * 0.25;
+template <typename T>
+__device__ __forceinline__ T ex_vz(const KP &p, const Nb &n, unsigned k) {
+    const T udyq = T(p.dyq) / T(p.ren), udzq = T(p.dzq) / T(p.ren), udy = T(p.dy) * T(0.25), udz = T(p.dz) * T(0.25);
     const unsigned kmm = k == 0 ? 0 : k - 1, kpp = k == unsigned(p.nxm - 1) ? k : k + 1;
-    const double vzc = n.vz[k];
-    const double vzzp = n.vz_ip[k], vzzm = n.vz_im[k], vzyp = n.vz_jp[k], vzym = n.vz_jm[k];
-    const double a = vzzp + vzc, b = vzzm + vzc;
-    double hzz = (a * a - b * b) * udz;
-    double hzy = ((n.vy_jp[k] + n.vy_jp_im[k]) * (vzyp + vzc) - (n.vy[k] + n.vy_im[k]) * (vzc + vzym)) * udy;
-    double hzx = ((n.vx[k + 1] + n.vx_im[k + 1]) * (n.vz[kpp] + vzc)
-                - (n.vx[k] + n.vx_im[k]) * (vzc + n.vz[kmm])) * p.udx3m[k] * 0.25;
-    double dzzvz = (vzzp - 2.0 * vzc + vzzm) * udzq;
-    double dyyvz = (vzyp - 2.0 * vzc + vzym) * udyq;
-    double q = -(hzx + hzy + hzz) + dyyvz + dzzvz;
-    if (p.gaxis == 3) q += p.byct * (p.active_t * 0.5 * (n.t[k] + n.t_im[k]));
+    const T vzc = T(n.vz[k]);
+    const T vzzp = T(n.vz_ip[k]), vzzm = T(n.vz_im[k]), vzyp = T(n.vz_jp[k]), vzym = T(n.vz_jm[k]);
+    const T a = vzzp + vzc, b = vzzm + vzc;
+    T hzz = (a * a - b * b) * udz;
+    T hzy = ((T(n.vy_jp[k]) + T(n.vy_jp_im[k])) * (vzyp + vzc) - (T(n.vy[k]) + T(n.vy_im[k])) * (vzc + vzym)) * udy;
+    T hzx = ((T(n.vx[k + 1]) + T(n.vx_im[k + 1])) * (T(n.vz[kpp]) + vzc)
+                - (T(n.vx[k]) + T(n.vx_im[k])) * (vzc + T(n.vz[kmm]))) * T(p.udx3m[k]) * T(0.25);
+    T dzzvz = (vzzp - T(2.0) * vzc + vzzm) * udzq;
+    T dyyvz = (vzyp - T(2.0) * vzc + vzym) * udyq;
+    T q = -(hzx + hzy + hzz) + dyyvz + dzzvz;
+    if (p.gaxis == 3) q += T(p.byct) * (T(p.active_t) * T(0.5) * (T(n.t[k]) + T(n.t_im[k])));
     return q;
 }
 
 // ExplicitTermsTemp.F90:39-77, k = 0..nxm-1
-__device__ __forceinline__ double ex_temp(const KP &p, const Nb &n, unsigned k) {
-    const double udz = p.dz * 0.5, udy = p.dy * 0.5, udzq = p.dzq / p.pect, udyq = p.dyq / p.pect;
-    const double tc = n.t[k], tzp = n.t_ip[k], tzm = n.t_im[k], typ = n.t_jp[k], tym = n.t_jm[k];
-    const double vxp = n.vx[k + 1], vxc = n.vx[k];
-    double htx;
+template <typename T>
+__device__ __forceinline__ T ex_temp(const KP &p, const Nb &n, unsigned k) {
+    const T udz = T(p.dz) * T(0.5), udy = T(p.dy) * T(0.5), udzq = T(p.dzq) / T(p.pect), udyq = T(p.dyq) / T(p.pect);
+    const T tc = T(n.t[k]), tzp = T(n.t_ip[k]), tzm = T(n.t_im[k]), typ = T(n.t_jp[k]), tym = T(n.t_jm[k]);
+    const T vxp = T(n.vx[k + 1]), vxc = T(n.vx[k]);
+    T htx;
     if (k == 0)
-        htx = (vxp * (n.t[k + 1] + tc) - vxc * 2.0 * n.tbp) * 0.5 * p.udx3m[k];
+        htx = (vxp * (T(n.t[k + 1]) + tc) - vxc * T(2.0) * T(n.tbp)) * T(0.5) * T(p.udx3m[k]);
     else if (k == unsigned(p.nxm - 1))
-        htx = (vxp * 2.0 * n.ttp - vxc * (tc + n.t[k - 1])) * 0.5 * p.udx3m[k];
+        htx = (vxp * T(2.0) * T(n.ttp) - vxc * (tc + T(n.t[k - 1]))) * T(0.5) * T(p.udx3m[k]);
     else
-        htx = (vxp * (n.t[k + 1] + tc) - vxc * (tc + n.t[k - 1])) * 0.5 * p.udx3m[k];
-    double htz = (n.vz_ip[k] * (tzp + tc) - n.vz[k] * (tc + tzm)) * udz;
-    double hty = (n.vy_jp[k] * (typ + tc) - n.vy[k] * (tc + tym)) * udy;
-    double dzzt = (tzp - 2.0 * tc + tzm) * udzq;
-    double dyyt = (typ - 2.0 * tc + tym) * udyq;
+        htx = (vxp * (T(n.t[k + 1]) + tc) - vxc * (tc + T(n.t[k - 1]))) * T(0.5) * T(p.udx3m[k]);
+    T htz = (T(n.vz_ip[k]) * (tzp + tc) - T(n.vz[k]) * (tc + tzm)) * udz;
+    T hty = (T(n.vy_jp[k]) * (typ + tc) - T(n.vy[k]) * (tc + tym)) * udy;
+    T dzzt = (tzp - T(2.0) * tc + tzm) * udzq;
+    T dyyt = (typ - T(2.0) * tc + tym) * udyq;
     return -(htx + hty + htz) + dyyt + dzzt;
 }
 
@@ -162,51 +166,66 @@ __device__ __forceinline__ double rhs_temp(const KP &p, const Nb &n, double h, d
     return (p.ga * h + p.ro * ru + (p.al / p.pect) * dxxt) * p.dt;
 }
 
+// storage-type selection of a field that exists in FP64 and FP32 forms
+template <typename RT> __device__ __forceinline__ RT *sel(double *d, float *f);
+template <> __device__ __forceinline__ double *sel<double>(double *d, float *) { return d; }
+template <> __device__ __forceinline__ float *sel<float>(double *, float *f) { return f; }
+template <typename RT> __device__ __forceinline__ const RT *sel(const double *d, const float *f);
+template <> __device__ __forceinline__ const double *sel<double>(const double *d, const float *) { return d; }
+template <> __device__ __forceinline__ const float *sel<float>(const double *, const float *f) { return f; }
+
 // refined column (j,i) and neighbours for the salinity stencils
+template <typename RT>
 struct NbS {
-    const double *s, *s_jm, *s_jp, *s_im, *s_ip;
-    const double *vxr, *vyr, *vyr_jp, *vzr, *vzr_ip;
+    const RT *s, *s_jm, *s_jp, *s_im, *s_ip;
+    const RT *vxr, *vyr, *vyr_jp, *vzr, *vzr_ip;
     double sbp, stp;
 };
-__device__ __forceinline__ NbS make_nbs(const KP &p, int j, int i) {
+template <typename RT>
+__device__ __forceinline__ NbS<RT> make_nbs(const KP &p, int j, int i) {
     const int jm = wrapm(j, p.nyr), jp = wrapp(j, p.nyr), im = wrapm(i, p.nzr), ip = wrapp(i, p.nzr);
     auto o = [&](int jj, int ii) { return (long long)ii * p.planer + (long long)jj * p.ldxr; };
     const long long c = o(j, i);
-    NbS n;
-    n.s = p.sal + c; n.s_jm = p.sal + o(jm, i); n.s_jp = p.sal + o(jp, i); n.s_im = p.sal + o(j, im); n.s_ip = p.sal + o(j, ip);
-    n.vxr = p.vxr + c; n.vyr = p.vyr + c; n.vyr_jp = p.vyr + o(jp, i); n.vzr = p.vzr + c; n.vzr_ip = p.vzr + o(j, ip);
+    const RT *sal = sel<RT>(p.sal, p.salf), *vxr = sel<RT>(p.vxr, p.vxrf), *vyr = sel<RT>(p.vyr, p.vyrf), *vzr = sel<RT>(p.vzr, p.vzrf);
+    NbS<RT> n;
+    n.s = sal + c; n.s_jm = sal + o(jm, i); n.s_jp = sal + o(jp, i); n.s_im = sal + o(j, im); n.s_ip = sal + o(j, ip);
+    n.vxr = vxr + c; n.vyr = vyr + c; n.vyr_jp = vyr + o(jp, i); n.vzr = vzr + c; n.vzr_ip = vzr + o(j, ip);
     n.sbp = p.salbp[(long long)i * p.nyr + j];
     n.stp = p.saltp[(long long)i * p.nyr + j];
     return n;
 }
 
-// ExplicitSalinity, salinity.F90:298-363 (non-IBM), refined k = 0..nxmr-1
-__device__ __forceinline__ double ex_sal(const KP &p, const NbS &n, unsigned k) {
-    const double udyr = 0.5 * p.dyr, udzr = 0.5 * p.dzr, udyrq = p.dyqr / p.pecs, udzrq = p.dzqr / p.pecs;
-    const double sc = n.s[k], syp = n.s_jp[k], sym = n.s_jm[k], szp = n.s_ip[k], szm = n.s_im[k];
-    const double vp = n.vxr[k + 1], vc = n.vxr[k];
-    double hsx;
+// ExplicitSalinity, salinity.F90:298-363 (non-IBM), refined k = 0..nxmr-1,
+// arithmetic in T, storage RT
+template <typename T, typename RT>
+__device__ __forceinline__ T ex_sal(const KP &p, const NbS<RT> &n, unsigned k) {
+    const T udyr = T(0.5) * T(p.dyr), udzr = T(0.5) * T(p.dzr), udyrq = T(p.dyqr) / T(p.pecs), udzrq = T(p.dzqr) / T(p.pecs);
+    const T sc = T(n.s[k]), syp = T(n.s_jp[k]), sym = T(n.s_jm[k]), szp = T(n.s_ip[k]), szm = T(n.s_im[k]);
+    const T vp = T(n.vxr[k + 1]), vc = T(n.vxr[k]), u3 = T(p.udx3mr[k]);
+    T hsx;
     if (k == 0)
-        hsx = (vp * (n.s[k + 1] + sc) - vc * 2.0 * n.sbp) * p.udx3mr[k] * 0.5;
+        hsx = (vp * (T(n.s[k + 1]) + sc) - vc * T(2.0) * T(n.sbp)) * u3 * T(0.5);
     else if (k == unsigned(p.nxmr - 1))
-        hsx = (vp * 2.0 * n.stp - vc * (sc + n.s[k - 1])) * p.udx3mr[k] * 0.5;
+        hsx = (vp * T(2.0) * T(n.stp) - vc * (sc + T(n.s[k - 1]))) * u3 * T(0.5);
     else
-        hsx = (vp * (n.s[k + 1] + sc) - vc * (sc + n.s[k - 1])) * p.udx3mr[k] * 0.5;
-    double hsy = (n.vyr_jp[k] * (syp + sc) - n.vyr[k] * (sc + sym)) * udyr;
-    double hsz = (n.vzr_ip[k] * (szp + sc) - n.vzr[k] * (sc + szm)) * udzr;
-    double dyys = (syp - 2.0 * sc + sym) * udyrq;
-    double dzzs = (szp - 2.0 * sc + szm) * udzrq;
+        hsx = (vp * (T(n.s[k + 1]) + sc) - vc * (sc + T(n.s[k - 1]))) * u3 * T(0.5);
+    T hsy = (T(n.vyr_jp[k]) * (syp + sc) - T(n.vyr[k]) * (sc + sym)) * udyr;
+    T hsz = (T(n.vzr_ip[k]) * (szp + sc) - T(n.vzr[k]) * (sc + szm)) * udzr;
+    T dyys = (syp - T(2.0) * sc + sym) * udyrq;
+    T dzzs = (szp - T(2.0) * sc + szm) * udzrq;
     return -(hsx + hsy + hsz) + dyys + dzzs;
 }
-// ImplicitSalinity, salinity.F90:378-400
-__device__ __forceinline__ double rhs_sal(const KP &p, const NbS &n, double h, double ru, unsigned k) {
+// ImplicitSalinity, salinity.F90:378-400 (FP64 arithmetic in both forms)
+template <typename RT>
+__device__ __forceinline__ double rhs_sal(const KP &p, const NbS<RT> &n, double h, double ru, unsigned k) {
     double dxxs;
+    const double sc = double(n.s[k]);
     if (k == 0)
-        dxxs = n.s[k + 1] * p.ap3sskr[k] + n.s[k] * p.ac3sskr[k] - (p.ap3sskr[k] + p.ac3sskr[k]) * n.sbp * p.sfixs;
+        dxxs = double(n.s[k + 1]) * p.ap3sskr[k] + sc * p.ac3sskr[k] - (p.ap3sskr[k] + p.ac3sskr[k]) * n.sbp * p.sfixs;
     else if (k == unsigned(p.nxmr - 1))
-        dxxs = n.s[k] * p.ac3sskr[k] + n.s[k - 1] * p.am3sskr[k] - (p.am3sskr[k] + p.ac3sskr[k]) * n.stp * p.sfixn;
+        dxxs = sc * p.ac3sskr[k] + double(n.s[k - 1]) * p.am3sskr[k] - (p.am3sskr[k] + p.ac3sskr[k]) * n.stp * p.sfixn;
     else
-        dxxs = n.s[k + 1] * p.ap3sskr[k] + n.s[k] * p.ac3sskr[k] + n.s[k - 1] * p.am3sskr[k];
+        dxxs = double(n.s[k + 1]) * p.ap3sskr[k] + sc * p.ac3sskr[k] + double(n.s[k - 1]) * p.am3sskr[k];
     return (p.ga * h + p.ro * ru + (p.al / p.pecs) * dxxs) * p.dt;
 }
 

@@ -24,6 +24,8 @@ module afid_gpu
         integer(c_int32_t) :: active_t, active_s, gaxis
         real(c_double)     :: xplusu, xminusu, dpdy, dpdz
         integer(c_int32_t) :: precision
+        real(c_double)     :: div_tol
+        integer(c_int32_t) :: max_sweeps
     end type afid_config
 
     interface

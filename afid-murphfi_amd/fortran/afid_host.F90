@@ -3,7 +3,9 @@
 !! (src/flow_solver/TimeMarcher.F90:13-165), with the arithmetic in
 !! libafid_gpu through the iso_c_binding module afid_gpu.
 !!
-!! usage: afid_host [fused] [seed=N]   (reads bou.in in the working directory)
+!! usage: afid_host [fused] [mixed] [seed=N]   (reads bou.in in the working directory)
+!!   fused: one afid_time_marcher call per step instead of the call tree
+!!   mixed: the mixed FP32/FP64 form (runs through afid_time_marcher)
 module afid_host_state
     use iso_c_binding
     use afid_gpu
@@ -13,7 +15,7 @@ module afid_host_state
     integer :: nsst, ntst, idtv, nread, ireset, flagMR, flagsal, flagPF, flagmelt, solidtype, pf_IC
     real(c_double) :: walltimemax, tmax, tout, tframe, save_3D, dt, resid, limitCFL, dtmin, dtmax
     real(c_double) :: pf_D, pf_A, pf_S, pf_Tm
-    logical :: salinity = .false., fused = .false.
+    logical :: salinity = .false., fused = .false., mixed = .false.
 end module afid_host_state
 
 !> ReadInputFile.F90:20-76 (same line layout)
@@ -59,7 +61,8 @@ subroutine ReadInputFile
     cfg = afid_config(nxm, nym, nzm, nsst, flagMR, nxmr, nymr, nzmr, flagsal, flagPF, &
                       alx3, ylen, zlen, istr3, str3, istr3r, rayt, prat, rays, pras, FFscaleS, &
                       inslws, inslwn, TfixS, TfixN, SfixS, SfixN, active_T, active_S, gAxis, &
-                      xplusU, xminusU, dPdy, dPdz, AFID_FP64)
+                      xplusU, xminusU, dPdy, dPdz, AFID_FP64, 0.d0, 0)
+    if (mixed) cfg%precision = AFID_MIXED
 end subroutine ReadInputFile
 
 !> SetTempBCs.F90:18-40 and SetSalBCs (salinity.F90:105-122) for the RBC case
@@ -152,6 +155,9 @@ program afid_host
     do a = 1, narg
         call get_command_argument(a, arg)
         if (trim(arg) == 'fused') fused = .true.
+        if (trim(arg) == 'mixed') then
+            mixed = .true.; fused = .true.
+        end if
         if (arg(1:5) == 'seed=') read(arg(6:), *) seed
     end do
 

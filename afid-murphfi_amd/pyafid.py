@@ -31,7 +31,7 @@ EXPORTS = [
     "afid_implicit_and_update_vx", "afid_implicit_and_update_vy", "afid_implicit_and_update_vz",
     "afid_implicit_and_update_temp", "afid_implicit_salinity",
     "afid_calc_local_divergence", "afid_solve_pressure_correction", "afid_correct_velocity", "afid_correct_pressure",
-    "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher",
+    "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher", "afid_refinement_residuals",
     "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_synchronize",
     "afid_create_initial_conditions",
     "afid_set_timing", "afid_get_timing", "afid_grid_array",
@@ -62,6 +62,7 @@ class AfidConfig(C.Structure):
         ("active_t", C.c_int32), ("active_s", C.c_int32), ("gaxis", C.c_int32),
         ("xplusu", C.c_double), ("xminusu", C.c_double), ("dpdy", C.c_double), ("dpdz", C.c_double),
         ("precision", C.c_int32),
+        ("div_tol", C.c_double), ("max_sweeps", C.c_int32),
     ]
 
 
@@ -70,7 +71,7 @@ DEFAULTS = dict(nxm=16, nym=16, nzm=16, nsst=3, multires=0, nxmr=0, nymr=0, nzmr
                 istr3=8, str3=0.0, istr3r=8, rayt=1e5, prat=1.0, rays=-1e5, pras=1.0, ffscales=0,
                 inslws=1, inslwn=1, tfixs=1, tfixn=1, sfixs=1, sfixn=1,
                 active_t=1, active_s=0, gaxis=1, xplusu=0.0, xminusu=0.0, dpdy=0.0, dpdz=0.0,
-                precision=FP64)
+                precision=FP64, div_tol=0.0, max_sweeps=0)
 
 
 def make_config(d):
@@ -114,6 +115,7 @@ def lib():
         L.afid_set_timing.argtypes = [vp, C.c_int]
         L.afid_get_timing.argtypes = [vp, C.POINTER(C.c_char_p), dp, C.c_int, C.POINTER(C.c_int)]
         L.afid_grid_array.argtypes = [vp, C.c_char_p, dp, C.c_int64, i64p]
+        L.afid_refinement_residuals.argtypes = [vp, dp, C.c_int, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -214,6 +216,13 @@ class Solver:
         out = (C.c_double * 4)()
         self._check(self.L.afid_global_quantities(self.h, out), "afid_global_quantities")
         return dict(zip(["nu", "ke", "vrms", "tmean"], list(out)))
+
+    def refinement_residuals(self):
+        """Mixed form: max|div u| before each refinement sweep of the last substep."""
+        out = (C.c_double * 4)()
+        n = C.c_int()
+        self._check(self.L.afid_refinement_residuals(self.h, out, 4, C.byref(n)), "afid_refinement_residuals")
+        return list(out)[:n.value]
 
     # -- diagnostics
     def device_bytes(self):

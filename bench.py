@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Benchmark of the RK3 step (TimeMarcher.F90:13-165) on one MI355X.
+"""Benchmark of the RK3 step (TimeMarcher.F90:13-165) on one MI355X, in the
+mixed FP32/FP64 form against the plain FP64 form.
 
 Workload (the largest one-GPU configuration): multi-resolution Rayleigh-Benard
 convection with a passive scalar, base grid 384 (x, cosine-clustered) x 768 x
@@ -7,9 +8,14 @@ convection with a passive scalar, base grid 384 (x, cosine-clustered) x 768 x
 dt = 5e-5, u = 0 and T = 1-x, S = 1-x plus seeded 1e-3 noise.  Synthetic data
 generated on the device.
 
-Prints ONE JSON line: Mcell-steps/s = (base + refined cells) x steps / s,
-with the HBM roofline of the dominant kernel group (HIP-event timing on the
-library's stream inside the timed region) and a CPU baseline (the FP64 CPU
+Both forms start from the same seeded state and run the same steps; the
+mixed form's error is then reported against the FP64 form: max|div u|,
+Nu_vol and KE relative errors and relative L2 errors of the fields.
+
+Prints ONE JSON line: value = Mcell-steps/s of the mixed form =
+(base + refined cells) x steps / s, with the HBM roofline of its dominant
+kernel group (HIP-event timing on the library's stream inside the timed
+region), the FP64 form's numbers beside it, and a CPU baseline (the FP64 CPU
 oracle, OpenMP, on a bounded sample of the same workload).
 """
 import argparse
@@ -39,6 +45,10 @@ WORKLOADS = {
 DEFAULT = "rbc_multires_384x768x768_768x1536x1536"
 
 
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def cells(cfg):
     n = cfg["nxm"] * cfg["nym"] * cfg["nzm"]
     if cfg.get("salinity"):
@@ -46,33 +56,40 @@ def cells(cfg):
     return n
 
 
-def kernel_bytes(cfg):
-    """Algorithmic HBM bytes per launch of each kernel group (FP64, per
-    substep): every array the operation must read or write once."""
+def kernel_bytes(cfg, form, sweeps=1, checks=0):
+    """Algorithmic HBM bytes per substep of each kernel group: every array the
+    operation must read or write once, at the form's element sizes.  In the
+    mixed form `sweeps` Poisson sweeps ran, after `checks` residual checks."""
     nb = cfg["nxm"] * cfg["nym"] * cfg["nzm"]
     nr = cfg["nxmr"] * cfg["nymr"] * cfg["nzmr"] if cfg.get("salinity") else 0
     ns = cfg["nxm"] * (cfg["nym"] // 2 + 1) * cfg["nzm"]       # complex spectral values
-    d = 8
-    return {
+    nf = cfg["nxm"] * (cfg["nym"] // 2 + 1) * (cfg["nzm"] // 2 + 1)   # factor rows (modes i, nz-i shared)
+    d = 8                                                     # FP64 state
+    h = 8 if form == "fp64" else 4                            # explicit terms, refined fields
+    c = 16 if form == "fp64" else 8                           # complex spectrum
+    r = 8 if form == "fp64" else 4                            # real factor
+    b = {
         # explicit terms + RHS + x-solve of vx, vy, vz, T in one sweep:
         # read u, v, w, T, p and the 4 old explicit terms, write u, v, w, T and the 4 new terms
-        "column_base": 17 * d * nb,
+        "column_base": 9 * d * nb + 8 * h * nb,
         # salinity: read S, vxr, vyr, vzr, old H_S; write S, H_S
-        "column_sal": 7 * d * nr,
-        # read u,v,w; write complex half spectrum
-        "poisson_fwd_y": 3 * d * nb + 16 * ns,
-        "poisson_fft_z": 2 * 2 * 16 * ns,
-        "poisson_tridiag": 2 * 16 * ns,
-        "poisson_inv_y": 16 * ns + d * nb,
+        "column_sal": 7 * h * nr,
         # read phi, u,v,w,p; write u,v,w,p
         "project": 9 * d * nb,
-        "mgrd_gradient": 3 * 2 * d * nb,
-        "mgrd_vxr": d * nr + d * nb,
-        "mgrd_vyr": d * nr + d * nb,
-        "mgrd_vzr": d * nr + d * nb,
+        "mgrd_vxr": h * nr + d * nb,
+        "mgrd_vyr": h * nr + d * nb,
+        "mgrd_vzr": h * nr + d * nb,
         # read S, write salc
-        "mgrd_interp_sal": d * nr + d * nb,
+        "mgrd_interp_sal": h * nr + h * nb,
     }
+    # Poisson: forward y (velocity -> spectrum; a residual check also reads
+    # dph), two z transforms, the per-mode tridiagonals (+ their factors),
+    # inverse y (-> dph; refinement sweeps read and write dph)
+    b["poisson_fwd_y"] = (3 * d * nb + c * ns) + checks * (4 * d * nb + c * ns)
+    b["poisson_fft_z"] = sweeps * 2 * 2 * c * ns
+    b["poisson_tridiag"] = sweeps * (2 * c * ns + 3 * r * nf)
+    b["poisson_inv_y"] = (c * ns + d * nb) + (sweeps - 1) * (c * ns + 2 * d * nb)
+    return b
 
 
 def cpu_baseline(args):
@@ -129,26 +146,23 @@ def cpu_baseline(args):
                        f" cells, same spacing); minimum step time {best:.3f} s")}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-shrink", type=int, default=4)
-    ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--cpu-budget", type=float, default=25.0)
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
-        sys.exit(2)
+FIELDS_CMP = ["vx", "vy", "vz", "temp", "pr"]
 
+
+def interior(wl, name, a):
+    if name == "sal":
+        return a[..., :wl["nxmr"]]
+    if name == "vx":
+        return a[..., 1:wl["nxm"]]
+    a = a[..., :wl["nxm"]]
+    return a - a.mean() if name == "pr" else a
+
+
+def run_form(wl, dt, form, args):
+    """Initialise, warm up and time one form; returns its measurements and
+    its final fields (for the error report)."""
     import pyafid
-    wl = dict(WORKLOADS[args.workload])
-    dt = wl.pop("dt")
-    g = pyafid.Solver(wl)
+    g = pyafid.Solver(dict(wl, precision=pyafid.MIXED if form == "mixed" else pyafid.FP64))
     nz, ny = wl["nzm"], wl["nym"]
     g.upload("tempbp", np.ones((nz, ny, 1)))
     g.upload("temptp", np.zeros((nz, ny, 1)))
@@ -157,8 +171,7 @@ def main():
         g.upload("saltp", np.zeros((wl["nzmr"], wl["nymr"], 1)))
     g.create_initial_conditions(args.seed, 1e-3)
     g.synchronize()
-    print(f"bench: {args.workload}, device memory {g.device_bytes() / 2**30:.1f} GiB", file=sys.stderr, flush=True)
-
+    log(f"{form}: {args.workload}, device memory {g.device_bytes() / 2**30:.1f} GiB")
     for _ in range(args.warmup):
         g.time_marcher(dt)
     g.synchronize()
@@ -176,29 +189,101 @@ def main():
     g.synchronize()
     total_s = time.perf_counter() - t_total0
     g.set_timing(False)
+    log(f"{form}: {args.steps} steps in {total_s:.3f} s")
 
-    div, divr = g.check_divergence()
-    gq = g.global_quantities()
+    res = {"form": form, "total_s": total_s, "step_ms": step_ms,
+           "groups": {k: v / args.steps for k, v in groups.items()}}
+    res["div"], res["divr"] = g.check_divergence()
+    res["gq"] = g.global_quantities()
+    res["residuals"] = g.refinement_residuals()
+    if not args.no_errors:
+        names = FIELDS_CMP + (["sal"] if wl.get("salinity") else [])
+        res["fields"] = {n: interior(wl, n, g.download(n)) for n in names}
+    g.close()
+    return res
+
+
+def roofline(wl, res, form):
+    """Roofline of the dominant kernel group: algorithmic bytes per launch
+    over the average launch time (HIP events on the library's stream)."""
+    nsst = wl["nsst"]
+    r = res["residuals"]
+    sweeps = 1 + sum(1 for q in r if q > 0.0 and q > 1e-10) if form == "mixed" else 1
+    checks = sum(1 for q in r if q > 0.0) if form == "mixed" else 0
+    kb = kernel_bytes(wl, form, sweeps, checks)
+    launches = {"poisson_fft_z": 2 * sweeps, "poisson_tridiag": sweeps, "poisson_inv_y": sweeps,
+                "poisson_fwd_y": 1 + checks}
+    grp = res["groups"]
+    dom = max((k for k in grp if k in kb), key=grp.get)
+    nl = launches.get(dom, 1)
+    avg_ms = grp[dom] / nsst / nl
+    per_launch = kb[dom] / nl
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    step_bytes = sum(kb.values()) * nsst
+    return ({"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+             "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(per_launch)},
+            round(step_bytes / (min(res["step_ms"]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--form", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-errors", action="store_true", help="skip the field downloads of the error report")
+    ap.add_argument("--cpu-shrink", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=25.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    wl = dict(WORKLOADS[args.workload])
+    dt = wl.pop("dt")
     ncell = cells(wl)
-    ms_per_step = total_s * 1e3 / args.steps
-    value = ncell * args.steps / total_s / 1e6
+    forms = ["fp64", "mixed"] if args.form == "both" else [args.form]
+    runs = {f: run_form(wl, dt, f, args) for f in forms}
 
-    # roofline of the dominant kernel group
-    nsub = wl["nsst"] * args.steps
-    launches = {k: nsub * (3 if k == "mgrd_gradient" else (2 if k == "poisson_fft_z" else 1)) for k in groups}
-    kb = kernel_bytes(dict(wl))
-    dom = max((k for k in groups if k in kb), key=groups.get)
-    per_launch_bytes = kb[dom] / (3 if dom == "mgrd_gradient" else (2 if dom == "poisson_fft_z" else 1))
-    avg_ms = groups[dom] / launches[dom]
-    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(per_launch_bytes)}
-    total_alg = sum(kb.values()) * wl["nsst"]
-    step_frac = total_alg / (min(step_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS
+    head = runs["mixed"] if "mixed" in runs else runs["fp64"]
+    hform = head["form"]
+    value = ncell * args.steps / head["total_s"] / 1e6
+    rl, step_frac = roofline(wl, head, hform)
+
+    summary = {}
+    for f, r in runs.items():
+        rlf, sf = roofline(wl, r, f)
+        summary[f] = {"value": round(ncell * args.steps / r["total_s"] / 1e6, 2),
+                      "ms_per_step": round(r["total_s"] * 1e3 / args.steps, 3),
+                      "min_ms_per_step": round(min(r["step_ms"]), 3),
+                      "median_ms_per_step": round(float(np.median(r["step_ms"])), 3),
+                      "max_div": r["div"], "max_div_refined": r["divr"],
+                      "nu_vol": r["gq"]["nu"], "ke": r["gq"]["ke"],
+                      "roofline": rlf, "step_hbm_frac": sf,
+                      "kernel_ms_per_step": {k: round(v, 3) for k, v in sorted(r["groups"].items(), key=lambda t: -t[1])}}
+        if f == "mixed":
+            summary[f]["refinement_residuals_last_substep"] = r["residuals"]
+    errors = None
+    if len(runs) == 2:
+        a, b = runs["fp64"], runs["mixed"]
+        errors = {"steps": args.warmup + args.steps,
+                  "max_div_mixed": b["div"], "max_div_fp64": a["div"],
+                  "nu_rel": abs(b["gq"]["nu"] - a["gq"]["nu"]) / abs(a["gq"]["nu"]),
+                  "ke_rel": abs(b["gq"]["ke"] - a["gq"]["ke"]) / max(abs(a["gq"]["ke"]), 1e-300)}
+        if not args.no_errors:
+            errors["l2_rel"] = {n: float(np.linalg.norm((b["fields"][n] - a["fields"][n]).ravel()) /
+                                         max(np.linalg.norm(a["fields"][n].ravel()), 1e-300))
+                                for n in a["fields"]}
+        summary["speedup_mixed_over_fp64"] = round(a["total_s"] / b["total_s"], 3)
 
     cpu = None
     if not args.no_cpu_baseline:
+        log("CPU baseline")
         try:
             cpu = cpu_baseline(args)
         except Exception as e:   # reported, never replaces the GPU number
@@ -206,21 +291,21 @@ def main():
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "min_ms_per_step": round(min(step_ms), 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": args.workload, "form": "fp64",
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["total_s"] * 1e3 / args.steps, 3),
+        "min_ms_per_step": round(min(head["step_ms"]), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64 (mixed form)" if hform == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "form": hform,
                    "base": [wl["nxm"], wl["nym"], wl["nzm"]],
                    "refined": [wl["nxmr"], wl["nymr"], wl["nzmr"]] if wl.get("salinity") else None,
                    "Ra": wl["rayt"], "Pr": wl["prat"], "Sc": wl.get("pras") if wl.get("salinity") else None,
                    "dt": dt, "cells": ncell},
-        "roofline": roofline,
-        "step_hbm_frac": round(step_frac, 4),
-        "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in sorted(groups.items(), key=lambda t: -t[1])},
-        "max_div": div, "max_div_refined": divr, "nu_vol": gq["nu"], "ke": gq["ke"],
+        "roofline": rl,
+        "step_hbm_frac": step_frac,
+        "forms": summary,
+        "mixed_vs_fp64": errors,
         "cpu_baseline": cpu,
     }
-    g.close()
     print(json.dumps(line))
 
 

@@ -37,7 +37,10 @@ extern "C" {
 
 /* Arithmetic form of the context. */
 #define AFID_FP64 0       /* every field and every operation in FP64 */
-#define AFID_MIXED 1      /* FP32 refined-grid transfer fields, FP64 state */
+#define AFID_MIXED 1      /* FP64 state (u, T, p) and x-tridiagonals; FP32 storage and
+                             arithmetic of the explicit terms, the refined scalar
+                             and velocities and the Hermite transfers; FP32 Poisson
+                             transforms inside FP64-residual iterative refinement */
 
 /* The bou.in value lines (ReadInputFile.F90:20-76) plus the arithmetic form. */
 typedef struct afid_config {
@@ -51,6 +54,8 @@ typedef struct afid_config {
     int32_t active_t, active_s, gaxis;
     double xplusu, xminusu, dpdy, dpdz;
     int32_t precision;                             /* AFID_FP64 or AFID_MIXED */
+    double div_tol;        /* AFID_MIXED: refinement stops once max|div u| <= div_tol (0: 1e-10) */
+    int32_t max_sweeps;    /* AFID_MIXED: Poisson sweeps per substep, 1..4 (0: 3) */
 } afid_config;
 
 typedef struct afid_ctx afid_ctx;
@@ -111,8 +116,18 @@ int afid_correct_pressure(afid_ctx *ctx);          /* pressure.F90:270-301 */
 int afid_interp_vel_mgrd(afid_ctx *ctx);       /* InterpVelMgrd.F90:1-243 */
 int afid_interp_sal_multigrid(afid_ctx *ctx);  /* salinity.F90:458-491 (fast path, HermiteInterpolations.F90:256-312) */
 
-/* One full RK3 step, the whole call tree of TimeMarcher.F90:13-165. */
+/* One full RK3 step, the whole call tree of TimeMarcher.F90:13-165.  The
+ * only stepping call of an AFID_MIXED context (its per-routine calls return
+ * AFID_E_STATE): the mixed form replaces SolvePressureCorrection's single
+ * FP64 solve (pressure.F90:165-197) by FP32 sweeps inside FP64-residual
+ * iterative refinement. */
 int afid_time_marcher(afid_ctx *ctx, double dt);
+
+/* AFID_MIXED: max|div u| (CheckDivergence units) measured before each
+ * refinement sweep 2..max_sweeps of the last substep; 0 where the sweep was
+ * not needed.  count = max_sweeps - 1 (0 for an AFID_FP64 context).  No
+ * counterpart in the reference, which has no mixed form. */
+int afid_refinement_residuals(afid_ctx *ctx, double *out, int max, int *count);
 
 /* Reductions read back to the host. */
 int afid_calc_max_cfl(afid_ctx *ctx, double *cflm, double *cflmr);      /* CalcMaxCFL.F90:11-72 */
