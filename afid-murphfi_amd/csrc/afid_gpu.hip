@@ -348,11 +348,14 @@ static int build_plan(afid_ctx *c, int n, FftPlan<R> &pl) {
     return rc;
 }
 
-// lines per transform workgroup: LDS of L lines of N complex values of
-// csize bytes, at most 32 values per thread
-static int pick_lines(int N, int csize, int maxlds, int cap) {
+// lines per transform workgroup (a power of two): L lines of N complex
+// values within the LDS budget and the per-thread register budget
+// (L*N <= FFT_MAXV*FFT_THREADS), at most cap
+template <typename R>
+static int pick_lines(int N, int maxlds, int cap) {
     int L = 1;
-    while (2 * L * N * csize <= maxlds && 2 * L * N <= 32 * 256 && 2 * L <= cap) L *= 2;
+    while (2 * L * line_stride<R>(N) * int(sizeof(cpx<R>)) <= maxlds && 2 * L * N <= FFT_MAXV * FFT_THREADS && 2 * L <= cap)
+        L *= 2;
     return L;
 }
 
@@ -512,10 +515,19 @@ static int setup(afid_ctx *c) {
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) == hipSuccess && v > 0)
             maxlds = std::min(v, 160 * 1024);
     }
-    c->Ly = pick_lines(c->ny, csize, maxlds, 16);      // complex lines; 2*Ly real x-values per block
-    c->Lz = pick_lines(c->nz, csize, maxlds, 16);
-    c->lds_y = c->Ly * c->ny * csize;
-    c->lds_z = c->Lz * c->nz * csize;
+    // y: Ly complex lines = 2*Ly x-adjacent real lines (16 FP64 velocity
+    // values = 128 bytes per row); z: Lz x-adjacent lines (128-byte rows)
+    if (c->mixed) {
+        c->Ly = pick_lines<float>(c->ny, maxlds, 8);
+        c->Lz = pick_lines<float>(c->nz, maxlds, 16);
+        c->lds_y = c->Ly * line_stride<float>(c->ny) * csize;
+        c->lds_z = c->Lz * line_stride<float>(c->nz) * csize;
+    } else {
+        c->Ly = pick_lines<double>(c->ny, maxlds, 8);
+        c->Lz = pick_lines<double>(c->nz, maxlds, 8);
+        c->lds_y = c->Ly * line_stride<double>(c->ny) * csize;
+        c->lds_z = c->Lz * line_stride<double>(c->nz) * csize;
+    }
     if (c->lds_y > 65536) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_div_fft_y<double>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_y);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_div_fft_y<float>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_y);
@@ -526,7 +538,7 @@ static int setup(afid_ctx *c) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_fft_z<double>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_z);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_fft_z<float>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_z);
     }
-    if (c->ny * csize > maxlds || c->nz * csize > maxlds || c->ny > 8192 || c->nz > 8192)
+    if (c->lds_y > maxlds || c->lds_z > maxlds || c->ny > FFT_MAXV * FFT_THREADS || c->nz > FFT_MAXV * FFT_THREADS)
         return set_err(c, AFID_E_ARG, "periodic size too large for one workgroup transform");
     {
         c->pchunk = (nxm + 63) / 64;
@@ -901,23 +913,23 @@ int afid_calc_local_divergence(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) retur
 }  // extern "C"
 
 // One Poisson sweep: forward transforms of the source, per-mode tridiagonals,
-// inverse transforms into dph (or added to dph).  fwd_gate gates the forward
-// y transform, gate the rest; slot_out receives max|div| of a residual source.
+// inverse transforms into dph.  fwd_gate gates the forward y transform, gate
+// the rest; slot_out receives max|div u| of a velocity source.
 template <typename R>
 static int poisson_sweep(afid_ctx *c, const FftPlan<R> &py, const FftPlan<R> &pz, const PoissonFactors<R> &pf, int src,
-                         Gate fwd_gate, Gate gate, unsigned long long *slot_out, unsigned long long *clear, int accumulate) {
+                         Gate fwd_gate, Gate gate, unsigned long long *slot_out, unsigned long long *clear) {
     KP p = make_kp(c);
     cpx<R> *spec = static_cast<cpx<R> *>(c->spec);
     const int TXy = 2 * c->Ly;
     tstart(c, "poisson_fwd_y");
-    hipLaunchKernelGGL(k_div_fft_y<R>, dim3(unsigned((c->nxm + TXy - 1) / TXy), unsigned(c->nz)), dim3(256), c->lds_y, c->stream,
-                       p, py, spec, c->Ly, src, fwd_gate, slot_out, clear);
+    hipLaunchKernelGGL(k_div_fft_y<R>, dim3(unsigned((c->nxm + TXy - 1) / TXy), unsigned(c->nz)), dim3(FFT_THREADS), c->lds_y,
+                       c->stream, p, py, spec, c->Ly, src, fwd_gate, slot_out, clear);
     tstop(c);
     int rc = check_launch(c, "k_div_fft_y");
     if (rc) return rc;
     tstart(c, "poisson_fft_z");
-    hipLaunchKernelGGL(k_fft_z<R>, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(256), c->lds_z, c->stream,
-                       p, pz, spec, c->Lz, -1, gate);
+    hipLaunchKernelGGL(k_fft_z<R>, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(FFT_THREADS), c->lds_z,
+                       c->stream, p, pz, spec, c->Lz, -1, gate);
     tstop(c);
     if ((rc = check_launch(c, "k_fft_z"))) return rc;
     tstart(c, "poisson_tridiag");
@@ -931,13 +943,13 @@ static int poisson_sweep(afid_ctx *c, const FftPlan<R> &py, const FftPlan<R> &pz
     tstop(c);
     if ((rc = check_launch(c, "k_poisson_tri"))) return rc;
     tstart(c, "poisson_fft_z");
-    hipLaunchKernelGGL(k_fft_z<R>, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(256), c->lds_z, c->stream,
-                       p, pz, spec, c->Lz, +1, gate);
+    hipLaunchKernelGGL(k_fft_z<R>, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(FFT_THREADS), c->lds_z,
+                       c->stream, p, pz, spec, c->Lz, +1, gate);
     tstop(c);
     if ((rc = check_launch(c, "k_fft_z"))) return rc;
     tstart(c, "poisson_inv_y");
-    hipLaunchKernelGGL(k_ifft_y<R>, dim3(unsigned((c->nxm + TXy - 1) / TXy), unsigned(c->nz)), dim3(256), c->lds_y, c->stream,
-                       p, py, spec, c->Ly, accumulate, gate);
+    hipLaunchKernelGGL(k_ifft_y<R>, dim3(unsigned((c->nxm + TXy - 1) / TXy), unsigned(c->nz)), dim3(FFT_THREADS), c->lds_y,
+                       c->stream, p, py, spec, c->Ly, gate);
     tstop(c);
     return check_launch(c, "k_ifft_y");
 }
@@ -945,35 +957,63 @@ static int poisson_sweep(afid_ctx *c, const FftPlan<R> &py, const FftPlan<R> &pz
 // FP64 form: one sweep, from dph (from_dph = 1) or fused with the divergence (0)
 static int poisson(afid_ctx *c, int from_dph) {
     const Gate open{nullptr, 0.0};
-    return poisson_sweep<double>(c, c->plan_y, c->plan_z, c->pf, from_dph ? SRC_DPH : SRC_VELOCITY, open, open, nullptr, nullptr, 0);
+    return poisson_sweep<double>(c, c->plan_y, c->plan_z, c->pf, from_dph ? SRC_DPH : SRC_VELOCITY, open, open, nullptr, nullptr);
 }
 
-// Mixed form: FP32 sweeps inside FP64-residual iterative refinement.  Sweep 1
-// solves for the divergence of the velocity; sweep s > 1 transforms the FP64
-// divergence that the projection with the accumulated dph would leave
-// (div_after_projection), records its maximum in slots[s-1], and solves and
-// adds its correction only while that maximum is above div_tol.  The gates
-// are evaluated on the device, so the step never waits for the host.
+static int project(afid_ctx *c, int vel, int pr, Gate gate = Gate{nullptr, 0.0}) {
+    KP p = make_kp(c);
+    dim3 gr = point_grid(c->nxm, c->ny, c->nz);
+    if (gate.slot) gr.z = unsigned(std::min(c->nz, 32));
+    hipLaunchKernelGGL(k_project, gr, kPointBlock, 0, c->stream, p, vel, pr, gate);
+    return check_launch(c, "k_project");
+}
+
+// Mixed form: pressure solve and projection of one substep as FP32 sweeps
+// inside FP64-residual iterative refinement.  Sweep 1 solves for the
+// divergence of the velocity and is applied by k_project_res, which also
+// forms the FP64 divergence of the corrected field and max-reduces it into
+// slots[1].  Sweep s > 1 transforms the divergence of the current velocity
+// (the FP64 residual left by sweep s-1), solves, and projects in place,
+// adding its correction to the pressure; it runs only while slots[s-1] >
+// div_tol.  The gates are read on the device, so the step never waits for
+// the host.
 static int poisson_mixed(afid_ctx *c) {
     const Gate open{nullptr, 0.0};
-    int rc = poisson_sweep<float>(c, c->plan_yf, c->plan_zf, c->pff, SRC_VELOCITY, open, open, nullptr, c->slots, 0);
-    for (int s = 2; s <= c->max_sweeps && !rc; ++s) {
-        const Gate fwd{s == 2 ? nullptr : c->slots + (s - 2), c->div_tol};
-        const Gate rest{c->slots + (s - 1), c->div_tol};
-        rc = poisson_sweep<float>(c, c->plan_yf, c->plan_zf, c->pff, SRC_RESIDUAL, fwd, rest, c->slots + (s - 1), nullptr, 1);
+    int rc = poisson_sweep<float>(c, c->plan_yf, c->plan_zf, c->pff, SRC_VELOCITY, open, open, nullptr, c->slots);
+    if (rc) return rc;
+    {
+        KP p = make_kp(c);
+        tstart(c, "project");
+        hipLaunchKernelGGL(k_project_res, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p,
+                           c->alt[0], c->alt[1], c->alt[2], c->redsum);
+        hipLaunchKernelGGL(k_max_partials, dim3(256), dim3(256), 0, c->stream, c->redsum, c->nred, c->slots + 1);
+        tstop(c);
+        if ((rc = check_launch(c, "k_project_res"))) return rc;
+        std::swap(c->f[AFID_VX], c->alt[0]);
+        std::swap(c->f[AFID_VY], c->alt[1]);
+        std::swap(c->f[AFID_VZ], c->alt[2]);
     }
-    return rc;
+    for (int s = 2; s <= c->max_sweeps; ++s) {
+        // sweep s runs while the divergence left by sweep s-1 (slots[s-1]) is
+        // above div_tol; for s > 2 its forward transform is what measures
+        // that divergence, so it is gated on the sweep before (slots[s-2])
+        const Gate fwd{c->slots + (s == 2 ? 1 : s - 2), c->div_tol};
+        const Gate rest{c->slots + (s - 1), c->div_tol};
+        rc = poisson_sweep<float>(c, c->plan_yf, c->plan_zf, c->pff, SRC_VELOCITY, fwd, rest,
+                                  s > 2 ? c->slots + (s - 1) : nullptr, nullptr);
+        if (rc) return rc;
+        tstart(c, "project");
+        rc = project(c, 1, 1, rest);
+        tstop(c);
+        if (rc) return rc;
+    }
+    return AFID_OK;
 }
 
 extern "C" {
 
 int afid_solve_pressure_correction(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return poisson(c, 1); }
 
-static int project(afid_ctx *c, int vel, int pr) {
-    KP p = make_kp(c);
-    hipLaunchKernelGGL(k_project, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, vel, pr);
-    return check_launch(c, "k_project");
-}
 int afid_correct_velocity(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return project(c, 1, 0); }
 int afid_correct_pressure(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return project(c, 0, 1); }
 
@@ -1087,10 +1127,14 @@ int afid_time_marcher(afid_ctx *c, double dt) {
             if ((rc = column_sal(c))) return rc;
             tstop(c);
         }
-        if ((rc = c->mixed ? poisson_mixed(c) : poisson(c, 0))) return rc;
-        tstart(c, "project");
-        if ((rc = project(c, 1, 1))) return rc;
-        tstop(c);
+        if (c->mixed) {
+            if ((rc = poisson_mixed(c))) return rc;
+        } else {
+            if ((rc = poisson(c, 0))) return rc;
+            tstart(c, "project");
+            if ((rc = project(c, 1, 1))) return rc;
+            tstop(c);
+        }
         if (c->sal) {
             if ((rc = interp_vel(c))) return rc;
             if ((rc = afid_interp_sal_multigrid(c))) return rc;

@@ -21,6 +21,51 @@ namespace afid {
     const int jm = wrapm(j, NY), jp = wrapp(j, NY), im = wrapm(i, NZ), ip = wrapp(i, NZ); \
     (void)jm; (void)jp; (void)im; (void)ip;
 
+// Refinement gate of the mixed form: a launch of a refinement sweep whose
+// gate slot holds a divergence at or below tol returns at once.  Slots hold
+// the bit pattern of a non-negative double (ordered like the value).
+struct Gate {
+    const unsigned long long *slot;   // null: always run
+    double tol;
+};
+__device__ __forceinline__ bool gate_closed(const Gate &g) {
+    return g.slot && __longlong_as_double((long long)*g.slot) <= g.tol;
+}
+
+// block-wide reductions (result valid in thread 0)
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double sm[16];
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    const int t = threadIdx.x + threadIdx.y * blockDim.x, w = t >> 6;
+    if ((t & 63) == 0) sm[w] = v;
+    __syncthreads();
+    const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+    if (t == 0) { double m = sm[0]; for (int q = 1; q < nw; ++q) m = fmax(m, sm[q]); v = m; }
+    return v;
+}
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double ss[16];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int t = threadIdx.x + threadIdx.y * blockDim.x, w = t >> 6;
+    if ((t & 63) == 0) ss[w] = v;
+    __syncthreads();
+    const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+    if (t == 0) { double m = ss[0]; for (int q = 1; q < nw; ++q) m += ss[q]; v = m; }
+    __syncthreads();
+    return v;
+}
+
+// thread 0 of a workgroup folds its maximum into a slot; the atomic is
+// skipped when the slot already holds at least that value (the plain read
+// may be stale, which only costs an atomic), so a grid of many workgroups
+// does not serialise on one address
+__device__ __forceinline__ void max_to_slot(unsigned long long *slot, double v) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+        if (bits > *slot) atomicMax(slot, bits);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // per-routine explicit kernels
 // ---------------------------------------------------------------------------
@@ -215,22 +260,25 @@ __device__ __forceinline__ bool strip_column(long long blk, int wave, int ny, in
     return ok;
 }
 
-// Row r of a column lives at lds_row<CH>(r): chunks of CH rows padded to
-// CH+1 so the chunked reads of the solver hit distinct banks.
+// Row r of a column lives at lds_row<CH>(r): chunks of CH rows at an odd
+// stride of CHP >= CH+1 doubles, so the solver's lane-strided 64-bit reads
+// (lane * CHP) hit distinct bank pairs.
 template <int CH>
-__device__ __forceinline__ unsigned lds_row(unsigned r) { return (r / CH) * (CH + 1) + r % CH; }
+constexpr int lds_chunk() { return (CH + 1) % 2 ? CH + 1 : CH + 2; }
 template <int CH>
-constexpr int lds_col_size() { return 64 * (CH + 1); }
+__device__ __forceinline__ unsigned lds_row(unsigned r) { return (r / CH) * lds_chunk<CH>() + r % CH; }
+template <int CH>
+constexpr int lds_col_size() { return 64 * lds_chunk<CH>(); }
 
 template <int CH>
 __device__ __forceinline__ void column_solve_lds(double *buf, const TriFactors &f) {
     const int lane = threadIdx.x & 63;
     double v[CH];
 #pragma unroll
-    for (int m = 0; m < CH; ++m) v[m] = buf[lane * (CH + 1) + m];
+    for (int m = 0; m < CH; ++m) v[m] = buf[lane * lds_chunk<CH>() + m];
     wave_tri_solve<CH>(v, f);
 #pragma unroll
-    for (int m = 0; m < CH; ++m) buf[lane * (CH + 1) + m] = v[m];
+    for (int m = 0; m < CH; ++m) buf[lane * lds_chunk<CH>() + m] = v[m];
 }
 
 // ExplicitSalinity + ImplicitSalinity + SolveImpEqnUpdate_Sal (salinity.F90:268-454).
@@ -383,24 +431,79 @@ __global__ void __launch_bounds__(256) k_local_divergence(KP p) {
 
 // RemoveDivergence (pressure.F90:245-266) and/or CorrectPressure
 // (pressure.F90:276-300); both read only dph, so one sweep does both.
-__global__ void __launch_bounds__(256) k_project(KP p, int do_vel, int do_pr) {
-    AFID_POINT_PROLOGUE(p.nxm, p.ny, p.nz)
+__global__ void __launch_bounds__(256) k_project(KP p, int do_vel, int do_pr, Gate gate) {
+    if (gate_closed(gate)) return;
+    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    if (k >= p.nxm || j >= p.ny) return;
+    const int jm = wrapm(j, p.ny), jp = wrapp(j, p.ny);
     const double *d = p.dph;
     const int km = k == 0 ? 0 : k - 1, kp = k == p.nxm - 1 ? k : k + 1;
-    const long long c = bidx(p, k, j, i);
-    const double dc = d[c];
-    if (do_vel) {
-        const double adt = p.al * p.dt;
-        p.vx[c] -= (dc - d[bidx(p, km, j, i)]) * (adt * p.udx3c[k]);
-        p.vy[c] -= (dc - d[bidx(p, k, jm, i)]) * (adt * p.dy);
-        p.vz[c] -= (dc - d[bidx(p, k, j, im)]) * (adt * p.dz);
+    // z walked with the grid's z stride: a full grid (one plane each) or, for
+    // a gated launch, a short grid that costs little when the gate is closed
+    for (int i = blockIdx.z; i < p.nz; i += gridDim.z) {
+        const int im = wrapm(i, p.nz), ip = wrapp(i, p.nz);
+        const long long c = bidx(p, k, j, i);
+        const double dc = d[c];
+        if (do_vel) {
+            const double adt = p.al * p.dt;
+            p.vx[c] -= (dc - d[bidx(p, km, j, i)]) * (adt * p.udx3c[k]);
+            p.vy[c] -= (dc - d[bidx(p, k, jm, i)]) * (adt * p.dy);
+            p.vz[c] -= (dc - d[bidx(p, k, j, im)]) * (adt * p.dz);
+        }
+        if (do_pr) {
+            const double be = p.al * p.beta;
+            p.pr[c] += dc - be * ((d[bidx(p, k, j, ip)] - 2.0 * dc + d[bidx(p, k, j, im)]) * p.dzq +
+                                  (d[bidx(p, k, jp, i)] - 2.0 * dc + d[bidx(p, k, jm, i)]) * p.dyq +
+                                  (d[bidx(p, kp, j, i)] * p.apphk[k] + dc * p.acphk[k] + d[bidx(p, km, j, i)] * p.amphk[k]));
+        }
     }
-    if (do_pr) {
+}
+
+// The same projection writing the corrected velocity to a second buffer
+// (u_out), so that every cell can also form the divergence the corrected
+// field will have (its own and its +1 neighbours' faces, from the old
+// velocity and dph): the FP64 residual of the mixed form's iterative
+// refinement, max-reduced into *slot (CheckDivergence units).
+__global__ void __launch_bounds__(256) k_project_res(KP p, double *vx_out, double *vy_out, double *vz_out,
+                                                     double *partial) {
+    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    double dvm = 0.0;
+    if (k < p.nxm && j < p.ny) {
+        const int jm = wrapm(j, p.ny), jp = wrapp(j, p.ny), im = wrapm(i, p.nz), ip = wrapp(i, p.nz);
+        const int km = k == 0 ? 0 : k - 1, kp = k == p.nxm - 1 ? k : k + 1;
+        const double *d = p.dph;
+        const long long c = bidx(p, k, j, i);
+        const long long cjm = bidx(p, k, jm, i), cjp = bidx(p, k, jp, i), cim = bidx(p, k, j, im), cip = bidx(p, k, j, ip);
+        const double dc = d[c];
+        const double adt = p.al * p.dt;
+        // corrected faces of this cell (as k_project)
+        const double ux = p.vx[c] - (dc - d[bidx(p, km, j, i)]) * (adt * p.udx3c[k]);
+        const double uy = p.vy[c] - (dc - d[cjm]) * (adt * p.dy);
+        const double uz = p.vz[c] - (dc - d[cim]) * (adt * p.dz);
+        vx_out[c] = ux; vy_out[c] = uy; vz_out[c] = uz;
+        if (k == p.nxm - 1) {      // index nx: wall face of vx (not corrected), unused for vy, vz
+            vx_out[c + 1] = p.vx[c + 1]; vy_out[c + 1] = p.vy[c + 1]; vz_out[c + 1] = p.vz[c + 1];
+        }
+        // corrected +1 faces (owned by the neighbouring cells)
+        const double ux1 = k + 1 <= p.nxm - 1 ? p.vx[c + 1] - (d[c + 1] - dc) * (adt * p.udx3c[k + 1]) : p.vx[c + 1];
+        const double uy1 = p.vy[cjp] - (d[cjp] - dc) * (adt * p.dy);
+        const double uz1 = p.vz[cip] - (d[cip] - dc) * (adt * p.dz);
+        dvm = fmax(dvm, fabs((uz1 - uz) * p.dz + (uy1 - uy) * p.dy + (ux1 - ux) * p.udx3m[k]));
         const double be = p.al * p.beta;
-        p.pr[c] += dc - be * ((d[bidx(p, k, j, ip)] - 2.0 * dc + d[bidx(p, k, j, im)]) * p.dzq +
-                              (d[bidx(p, k, jp, i)] - 2.0 * dc + d[bidx(p, k, jm, i)]) * p.dyq +
+        p.pr[c] += dc - be * ((d[cip] - 2.0 * dc + d[cim]) * p.dzq + (d[cjp] - 2.0 * dc + d[cjm]) * p.dyq +
                               (d[bidx(p, kp, j, i)] * p.apphk[k] + dc * p.acphk[k] + d[bidx(p, km, j, i)] * p.amphk[k]));
     }
+    dvm = block_max(dvm);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        partial[((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = dvm;
+}
+// max of n per-workgroup partial maxima into *slot (grid-stride, one atomic
+// per workgroup of a small grid)
+__global__ void __launch_bounds__(256) k_max_partials(const double *partial, long long n, unsigned long long *slot) {
+    double v = 0.0;
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256) v = fmax(v, partial[q]);
+    v = block_max(v);
+    if (threadIdx.x == 0) atomicMax(slot, (unsigned long long)__double_as_longlong(v));
 }
 
 // ---------------------------------------------------------------------------
@@ -436,6 +539,27 @@ __device__ __forceinline__ RT mgrd_grad(const KP &p, int comp, int xc, int j, in
     return RT(sgn * d);
 }
 __device__ __forceinline__ int wrapn(int j, int n) { j %= n; return j < 0 ? j + n : j; }
+// periodic wrap of an index a few periods away at most, without a division
+__device__ __forceinline__ int wrapf(int j, int n) {
+    while (j < 0) j += n;
+    while (j >= n) j -= n;
+    return j;
+}
+
+// Components (a, b, c) of a linear index q = a + A (b + B c) that advances by a
+// fixed step: the divisions happen once, each step costs a few adds.
+struct Walk3 {
+    int a, b, c, da, db, dc, A, B;
+    __device__ __forceinline__ Walk3(int q, int step, int A_, int B_) : A(A_), B(B_) {
+        a = q % A; b = (q / A) % B; c = q / (A * B);
+        da = step % A; db = (step / A) % B; dc = step / (A * B);
+    }
+    __device__ __forceinline__ void next() {
+        a += da; b += db; c += dc;
+        if (a >= A) { a -= A; ++b; }
+        if (b >= B) { b -= B; ++c; }
+    }
+};
 
 // Tile sizes of the transfer kernels (host computes the LDS each needs).
 constexpr int VXR_TJ = 16, VXR_TI = 4;      // refined columns per vxr tile
@@ -470,21 +594,24 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
         const int R1 = min(R0 + 64, p.nxmr);
         const int x0 = p.xs_src[R0], XC = p.xs_src[R1 - 1] + 4 - x0;
         __syncthreads();
-        for (int q = t; q < XC * CJ * CI; q += 256) {
-            const int xc = q % XC, cj = (q / XC) % CJ, ci = q / (XC * CJ);
-            B[(ci * cjmax + cj) * xcmax + xc] = mgrd_grad<RT>(p, 0, x0 + xc, wrapn(cj0 + cj, p.ny), wrapn(ci0 + ci, p.nz));
+        {
+            Walk3 w(t, 256, XC, CJ);
+            for (int q = t; q < XC * CJ * CI; q += 256, w.next())
+                B[(w.c * cjmax + w.b) * xcmax + w.a] = mgrd_grad<RT>(p, 0, x0 + w.a, wrapf(cj0 + w.b, p.ny), wrapf(ci0 + w.c, p.nz));
         }
         __syncthreads();
-        for (int q = t; q < XC * CJ * ni; q += 256) {
-            const int xc = q % XC, cj = (q / XC) % CJ, ti = q / (XC * CJ);
+        Walk3 wz(t, 256, XC, CJ);
+        for (int q = t; q < XC * CJ * ni; q += 256, wz.next()) {
+            const int xc = wz.a, cj = wz.b, ti = wz.c;
             const int c0 = p.zs_src[I0 + ti] - 1 - ci0;
             const RT *w = zs_w + 4 * (I0 + ti), *b = B + (c0 * cjmax + cj) * xcmax + xc;
             const int st = cjmax * xcmax;
             Z[(ti * cjmax + cj) * xcmax + xc] = b[0] * w[0] + b[st] * w[1] + b[2 * st] * w[2] + b[3 * st] * w[3];
         }
         __syncthreads();
-        for (int q = t; q < XC * nj * ni; q += 256) {
-            const int xc = q % XC, tj = (q / XC) % nj, ti = q / (XC * nj);
+        Walk3 wy(t, 256, XC, nj);
+        for (int q = t; q < XC * nj * ni; q += 256, wy.next()) {
+            const int xc = wy.a, tj = wy.b, ti = wy.c;
             const int b0 = p.ys_src[J0 + tj] - 1 - cj0;
             const RT *w = ys_w + 4 * (J0 + tj), *z = Z + (ti * cjmax + b0) * xcmax + xc;
             Q[(ti * VXR_TJ + tj) * xcmax + xc] = z[0] * w[0] + z[xcmax] * w[1] + z[2 * xcmax] * w[2] + z[3 * xcmax] * w[3];
@@ -579,14 +706,18 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
         const int S1 = min(S0 + VYZ_CHUNK, nS - 1);
         const int cs0 = sS[S0] - 1, CS = sS[S1 - 1] - 1 + 4 - cs0;
         __syncthreads();
-        for (int q = t; q < XC * CS * CO; q += 256) {
-            const int xc = q % XC, cs = (q / XC) % CS, co = q / (XC * CS);
-            const int js = wrapn(cs0 + cs, cS), jo = wrapn(co0 + co, cO);
-            B[(co * csmax + cs) * xcmax + xc] = AX == 1 ? mgrd_grad<RT>(p, 1, x0 + xc, js, jo) : mgrd_grad<RT>(p, 2, x0 + xc, jo, js);
+        {
+            Walk3 w(t, 256, XC, CS);
+            for (int q = t; q < XC * CS * CO; q += 256, w.next()) {
+                const int js = wrapf(cs0 + w.b, cS), jo = wrapf(co0 + w.c, cO);
+                B[(w.c * csmax + w.b) * xcmax + w.a] =
+                    AX == 1 ? mgrd_grad<RT>(p, 1, x0 + w.a, js, jo) : mgrd_grad<RT>(p, 2, x0 + w.a, jo, js);
+            }
         }
         __syncthreads();
-        for (int q = t; q < XC * CS * nl; q += 256) {
-            const int xc = q % XC, cs = (q / XC) % CS, l = q / (XC * CS);
+        Walk3 wo(t, 256, XC, CS);
+        for (int q = t; q < XC * CS * nl; q += 256, wo.next()) {
+            const int xc = wo.a, cs = wo.b, l = wo.c;
             const int c0 = sO[O0 + l] - 1 - co0;
             const RT *w = wO + 4 * (O0 + l), *b = B + (c0 * csmax + cs) * xcmax + xc;
             const int st = csmax * xcmax;
@@ -661,10 +792,9 @@ __global__ void __launch_bounds__(256) k_interp_sal_tile(KP p, int xrmax, int yr
     for (int zr = zfirst; zr <= zlast; ++zr) {
         RT *B = sm + ((zr - zfirst) & 1) * xrmax * yrmax;     // [yr][xr], double buffered
         const int ii = wrapn(zr, p.nzr);
-        for (int q = t; q < plane; q += 256) {
-            const int xr = q % XR, yr = q / XR;
-            B[yr * xrmax + xr] = sal_ghosted<RT>(p, sal, x0 + xr, wrapn(y0 + yr, p.nyr), ii);
-        }
+        Walk3 w(t, 256, XR, YR);
+        for (int q = t; q < plane; q += 256, w.next())
+            B[w.b * xrmax + w.a] = sal_ghosted<RT>(p, sal, x0 + w.a, wrapf(y0 + w.b, p.nyr), ii);
         __syncthreads();
         for (int h = 0; h < 2; ++h) {
             RT v = RT(0);
@@ -727,28 +857,6 @@ __global__ void __launch_bounds__(256) k_init_scalar(FT *f, const double *bot, c
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double block_max(double v) {
-    __shared__ double sm[8];
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
-    const int t = threadIdx.x + threadIdx.y * blockDim.x, w = t >> 6;
-    if ((t & 63) == 0) sm[w] = v;
-    __syncthreads();
-    const int nw = (blockDim.x * blockDim.y + 63) >> 6;
-    if (t == 0) { double m = sm[0]; for (int q = 1; q < nw; ++q) m = fmax(m, sm[q]); v = m; }
-    return v;
-}
-__device__ __forceinline__ double block_sum(double v) {
-    __shared__ double ss[8];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    const int t = threadIdx.x + threadIdx.y * blockDim.x, w = t >> 6;
-    if ((t & 63) == 0) ss[w] = v;
-    __syncthreads();
-    const int nw = (blockDim.x * blockDim.y + 63) >> 6;
-    if (t == 0) { double m = ss[0]; for (int q = 1; q < nw; ++q) m += ss[q]; v = m; }
-    __syncthreads();
-    return v;
-}
-
 // CalcMaxCFL.F90:24-44 (what=0) / CheckDivergence.F90:24-43 (what=1) on the
 // base grid, slot 0; refined grid versions write slot 1.
 __global__ void __launch_bounds__(256) k_cfl_div(KP p, int what) {

@@ -89,27 +89,42 @@ struct DftSmall<8, R> {
     }
 };
 
+// Workgroup size of the transform kernels and the most complex values a
+// thread holds in a stage (L*N <= FFT_MAXV*FFT_THREADS): 16 wavefronts whose
+// butterflies of a stage stay in registers between its read and write phase.
+constexpr int FFT_THREADS = 1024, FFT_MAXV = 8;
+
+// LDS layout of L lines of N complex values: element j of line l sits at
+// l*S + lpad(j), with one pad element per 32 banks of data (16 FP32 or 8
+// FP64 complex values) so the stride-R writes of a Stockham stage spread
+// over the banks, and a line stride S = lpad(N)+1 that is not a multiple of
+// the 32 banks, so x-adjacent lines (the global-load order) do too.
+template <typename R>
+__host__ __device__ __forceinline__ int lpad(int i) { return i + (i >> (sizeof(R) == 4 ? 4 : 3)); }
+template <typename R>
+__host__ __device__ __forceinline__ int line_stride(int n) { return lpad<R>(n) + 1; }
+
 // One Stockham stage of radix RD over L lines of length N held in LDS.
 // Each thread keeps at most MAXB butterflies of the stage in registers
 // between the read and the write phase, so the transform runs in place.
 template <int RD, int MAXB, typename R>
-__device__ __forceinline__ void fft_stage(cpx<R> *buf, int L, int N, int Ns, const cpx<R> *tw, int sign) {
-    const int nbl = N / RD, nb = L * nbl, T = blockDim.x;
+__device__ __forceinline__ void fft_stage(cpx<R> *buf, int L, int N, int S, int Ns, const cpx<R> *tw, int sign) {
+    const int nbl = N / RD, nb = L * nbl;
     cpx<R> v[MAXB][RD];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-        const int b = threadIdx.x + q * T;
+        const int b = threadIdx.x + q * FFT_THREADS;
         if (b < nb) {
             const int line = b / nbl, jj = b - line * nbl;
 #pragma unroll
-            for (int r = 0; r < RD; ++r) v[q][r] = buf[line * N + jj + r * nbl];
+            for (int r = 0; r < RD; ++r) v[q][r] = buf[line * S + lpad<R>(jj + r * nbl)];
         }
     }
     __syncthreads();
     const int tstride = N / (Ns * RD);
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-        const int b = threadIdx.x + q * T;
+        const int b = threadIdx.x + q * FFT_THREADS;
         if (b < nb) {
             const int line = b / nbl, jj = b - line * nbl, kk = jj % Ns;
 #pragma unroll
@@ -121,108 +136,76 @@ __device__ __forceinline__ void fft_stage(cpx<R> *buf, int L, int N, int Ns, con
             DftSmall<RD, R>::run(v[q], sign);
             const int dst = (jj / Ns) * Ns * RD + kk;
 #pragma unroll
-            for (int r = 0; r < RD; ++r) buf[line * N + dst + r * Ns] = v[q][r];
+            for (int r = 0; r < RD; ++r) buf[line * S + lpad<R>(dst + r * Ns)] = v[q][r];
         }
     }
     __syncthreads();
 }
 
-// Full transform of L lines; at most 32 complex values per thread (L*N <= 32*blockDim).
+// Full transform of L lines (L*N <= FFT_MAXV*FFT_THREADS), line stride S.
 template <typename R>
-__device__ __forceinline__ void lds_fft(cpx<R> *buf, int L, const FftPlan<R> &pl, int sign) {
+__device__ __forceinline__ void lds_fft(cpx<R> *buf, int L, int S, const FftPlan<R> &pl, int sign) {
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int RD = pl.rad[s];
         switch (RD) {
-        case 8: fft_stage<8, 4, R>(buf, L, pl.n, Ns, pl.tw, sign); break;
-        case 4: fft_stage<4, 8, R>(buf, L, pl.n, Ns, pl.tw, sign); break;
-        case 3: fft_stage<3, 11, R>(buf, L, pl.n, Ns, pl.tw, sign); break;
-        default: fft_stage<2, 16, R>(buf, L, pl.n, Ns, pl.tw, sign); break;
+        case 8: fft_stage<8, (FFT_MAXV + 7) / 8, R>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
+        case 4: fft_stage<4, (FFT_MAXV + 3) / 4, R>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
+        case 3: fft_stage<3, (FFT_MAXV + 2) / 3, R>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
+        default: fft_stage<2, (FFT_MAXV + 1) / 2, R>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
         }
         Ns *= RD;
     }
 }
 
-// Refinement gate of the mixed form: a launch of a refinement sweep whose
-// gate slot holds a divergence at or below tol returns at once.  Slots hold
-// the bit pattern of a non-negative double (ordered like the value).
-struct Gate {
-    const unsigned long long *slot;   // null: always run
-    double tol;
-};
-__device__ __forceinline__ bool gate_closed(const Gate &g) {
-    return g.slot && __longlong_as_double((long long)*g.slot) <= g.tol;
-}
-
-// Divergence (CheckDivergence units, CheckDivergence.F90:29-41) of the
-// velocity that the projection with phi (RemoveDivergence, pressure.F90:
-// 245-266, as k_project applies it) would leave at cell (k, j, i): the FP64
-// residual of the mixed form's iterative refinement.
-__device__ __forceinline__ double div_after_projection(const KP &p, const double *phi, int k, int j, int i) {
-    const int jm = wrapm(j, p.ny), jp = wrapp(j, p.ny), im = wrapm(i, p.nz), ip = wrapp(i, p.nz);
-    const double adt = p.al * p.dt;
-    const long long c = bidx(p, k, j, i);
-    const double pc = phi[c];
-    double vx0 = p.vx[c], vx1 = p.vx[c + 1];
-    if (k >= 1) vx0 -= (pc - phi[c - 1]) * (adt * p.udx3c[k]);
-    if (k + 1 <= p.nxm - 1) vx1 -= (phi[c + 1] - pc) * (adt * p.udx3c[k + 1]);
-    const double vy0 = p.vy[c] - (pc - phi[bidx(p, k, jm, i)]) * (adt * p.dy);
-    const double vy1 = p.vy[bidx(p, k, jp, i)] - (phi[bidx(p, k, jp, i)] - pc) * (adt * p.dy);
-    const double vz0 = p.vz[c] - (pc - phi[bidx(p, k, j, im)]) * (adt * p.dz);
-    const double vz1 = p.vz[bidx(p, k, j, ip)] - (phi[bidx(p, k, j, ip)] - pc) * (adt * p.dz);
-    return (vz1 - vz0) * p.dz + (vy1 - vy0) * p.dy + (vx1 - vx0) * p.udx3m[k];
-}
-
 // Source of the forward transform
-enum PoissonSrc { SRC_VELOCITY = 0, SRC_DPH = 1, SRC_RESIDUAL = 2 };
+enum PoissonSrc { SRC_VELOCITY = 0, SRC_DPH = 1 };
 
 // Divergence (CalcLocalDivergence, pressure.F90:147-161) fused with the real
 // forward y transform (pressure.F90:171-174) and the 1/(ny nz) scaling.
-// SRC_RESIDUAL takes the divergence left by a projection with the current
-// dph (div_after_projection) and max-reduces its magnitude into *slot_out.
+// With slot_out, the maximum of |div u| (CheckDivergence units) is also
+// reduced into *slot_out; with slots_clear, workgroup 0 zeroes 4 slots.
 // Block: 2L x-adjacent real lines at one z.  grid (ceil(nxm/2L), nz).
 template <typename R>
-__global__ void __launch_bounds__(256) k_div_fft_y(KP p, FftPlan<R> pl, cpx<R> *spec, int L, int src, Gate gate,
-                                                   unsigned long long *slot_out, unsigned long long *slots_clear) {
+__global__ void __launch_bounds__(FFT_THREADS) k_div_fft_y(KP p, FftPlan<R> pl, cpx<R> *spec, int L, int src, Gate gate,
+                                                           unsigned long long *slot_out, unsigned long long *slots_clear) {
     if (gate_closed(gate)) return;
     extern __shared__ __align__(16) unsigned char smraw[];
     cpx<R> *lb = reinterpret_cast<cpx<R> *>(smraw);
-    const int N = p.ny, TX = 2 * L, i = blockIdx.y, k0 = blockIdx.x * TX;
+    const int N = p.ny, S = line_stride<R>(N), TX = 2 * L, i = blockIdx.y, k0 = blockIdx.x * TX;
     const int ip = wrapp(i, p.nz);
     const double usdtal = 1.0 / (p.dt * p.al);
     if (slots_clear && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) slots_clear[threadIdx.x] = 0ull;
-    R *lr = reinterpret_cast<R *>(lb);
     double dmax = 0.0;
-    for (int t = threadIdx.x; t < TX * N; t += blockDim.x) {
+    for (int t = threadIdx.x; t < TX * N; t += FFT_THREADS) {
         const int xo = t % TX, j = t / TX, k = k0 + xo;
         double d = 0.0;
         if (k < p.nxm) {
             if (src == SRC_DPH) d = p.dph[bidx(p, k, j, i)];
-            else if (src == SRC_RESIDUAL) {
-                const double dv = div_after_projection(p, p.dph, k, j, i);
+            else {
+                const int jp = wrapp(j, N);
+                const double dv = (p.vz[bidx(p, k, j, ip)] - p.vz[bidx(p, k, j, i)]) * p.dz
+                                + (p.vy[bidx(p, k, jp, i)] - p.vy[bidx(p, k, j, i)]) * p.dy
+                                + (p.vx[bidx(p, k + 1, j, i)] - p.vx[bidx(p, k, j, i)]) * p.udx3m[k];
                 dmax = fmax(dmax, fabs(dv));
                 d = dv * usdtal;
-            } else {
-                const int jp = wrapp(j, N);
-                d = ((p.vz[bidx(p, k, j, ip)] - p.vz[bidx(p, k, j, i)]) * p.dz
-                   + (p.vy[bidx(p, k, jp, i)] - p.vy[bidx(p, k, j, i)]) * p.dy
-                   + (p.vx[bidx(p, k + 1, j, i)] - p.vx[bidx(p, k, j, i)]) * p.udx3m[k]) * usdtal;
             }
         }
-        lr[2 * ((xo >> 1) * N + j) + (xo & 1)] = R(d);
+        R *slot = reinterpret_cast<R *>(&lb[(xo >> 1) * S + lpad<R>(j)]);
+        slot[xo & 1] = R(d);
     }
-    if (src == SRC_RESIDUAL) {
+    if (slot_out) {
         dmax = block_max(dmax);
-        if (threadIdx.x == 0) atomicMax(slot_out, (unsigned long long)__double_as_longlong(dmax));
+        max_to_slot(slot_out, dmax);
     }
     __syncthreads();
-    lds_fft<R>(lb, L, pl, -1);
+    lds_fft<R>(lb, L, S, pl, -1);
     const R scale = R(1.0 / (double(p.ny) * double(p.nz)));
-    for (int t = threadIdx.x; t < TX * p.nymh; t += blockDim.x) {
+    for (int t = threadIdx.x; t < TX * p.nymh; t += FFT_THREADS) {
         const int xo = t % TX, m = t / TX, k = k0 + xo;
         if (k >= p.nxm) continue;
         const int line = xo >> 1;
-        const cpx<R> zm = lb[line * N + m], zn = lb[line * N + (m == 0 ? 0 : N - m)];
+        const cpx<R> zm = lb[line * S + lpad<R>(m)], zn = lb[line * S + lpad<R>(m == 0 ? 0 : N - m)];
         cpx<R> out;
         if ((xo & 1) == 0) out = mk<R>(R(0.5) * (zm.x + zn.x), R(0.5) * (zm.y - zn.y));     // (Z_m + conj Z_{N-m})/2
         else out = mk<R>(R(0.5) * (zm.y + zn.y), R(-0.5) * (zm.x - zn.x));                  // (Z_m - conj Z_{N-m})/(2i)
@@ -233,54 +216,52 @@ __global__ void __launch_bounds__(256) k_div_fft_y(KP p, FftPlan<R> pl, cpx<R> *
 // Complex transform along z (pressure.F90:176 forward, :189 backward), in
 // place on the spectral array.  Block: L x-adjacent lines at one y-mode m.
 template <typename R>
-__global__ void __launch_bounds__(256) k_fft_z(KP p, FftPlan<R> pl, cpx<R> *spec, int L, int sign, Gate gate) {
+__global__ void __launch_bounds__(FFT_THREADS) k_fft_z(KP p, FftPlan<R> pl, cpx<R> *spec, int L, int sign, Gate gate) {
     if (gate_closed(gate)) return;
     extern __shared__ __align__(16) unsigned char smraw[];
     cpx<R> *lb = reinterpret_cast<cpx<R> *>(smraw);
-    const int N = p.nz, m = blockIdx.y, k0 = blockIdx.x * L;
+    const int N = p.nz, S = line_stride<R>(N), m = blockIdx.y, k0 = blockIdx.x * L;
     const long long sz = (long long)p.nymh * p.nxm;
     cpx<R> *g = spec + (long long)m * p.nxm;
-    for (int t = threadIdx.x; t < L * N; t += blockDim.x) {
+    for (int t = threadIdx.x; t < L * N; t += FFT_THREADS) {
         const int xo = t % L, i = t / L, k = k0 + xo;
-        lb[xo * N + i] = k < p.nxm ? g[i * sz + k] : mk<R>(R(0), R(0));
+        lb[xo * S + lpad<R>(i)] = k < p.nxm ? g[i * sz + k] : mk<R>(R(0), R(0));
     }
     __syncthreads();
-    lds_fft<R>(lb, L, pl, sign);
-    for (int t = threadIdx.x; t < L * N; t += blockDim.x) {
+    lds_fft<R>(lb, L, S, pl, sign);
+    for (int t = threadIdx.x; t < L * N; t += FFT_THREADS) {
         const int xo = t % L, i = t / L, k = k0 + xo;
-        if (k < p.nxm) g[i * sz + k] = lb[xo * N + i];
+        if (k < p.nxm) g[i * sz + k] = lb[xo * S + lpad<R>(i)];
     }
 }
 
 // Inverse real y transform (pressure.F90:193-195): two x-adjacent spectra
 // A, B (Hermitian, imaginary parts of the m=0 and m=N/2 terms ignored as a
-// c2r transform does) become Z = A + iB, whose inverse is a + ib.  The result
-// goes to dph (accumulate = 0) or is added to it (refinement sweeps).
+// c2r transform does) become Z = A + iB, whose inverse is a + ib, written
+// to dph.
 template <typename R>
-__global__ void __launch_bounds__(256) k_ifft_y(KP p, FftPlan<R> pl, const cpx<R> *spec, int L, int accumulate, Gate gate) {
+__global__ void __launch_bounds__(FFT_THREADS) k_ifft_y(KP p, FftPlan<R> pl, const cpx<R> *spec, int L, Gate gate) {
     if (gate_closed(gate)) return;
     extern __shared__ __align__(16) unsigned char smraw[];
     cpx<R> *lb = reinterpret_cast<cpx<R> *>(smraw);
-    const int N = p.ny, TX = 2 * L, i = blockIdx.y, k0 = blockIdx.x * TX;
-    for (int t = threadIdx.x; t < L * p.nymh; t += blockDim.x) {
+    const int N = p.ny, S = line_stride<R>(N), TX = 2 * L, i = blockIdx.y, k0 = blockIdx.x * TX;
+    for (int t = threadIdx.x; t < L * p.nymh; t += FFT_THREADS) {
         const int line = t % L, m = t / L;
         const int ka = k0 + 2 * line, kb = ka + 1;
         const cpx<R> *row = spec + ((long long)i * p.nymh + m) * p.nxm;
         cpx<R> A = ka < p.nxm ? row[ka] : mk<R>(R(0), R(0));
         cpx<R> B = kb < p.nxm ? row[kb] : mk<R>(R(0), R(0));
         if (m == 0 || 2 * m == N) { A.y = R(0); B.y = R(0); }
-        lb[line * N + m] = mk<R>(A.x - B.y, A.y + B.x);
-        if (m > 0 && N - m > m) lb[line * N + (N - m)] = mk<R>(A.x + B.y, B.x - A.y);
+        lb[line * S + lpad<R>(m)] = mk<R>(A.x - B.y, A.y + B.x);
+        if (m > 0 && N - m > m) lb[line * S + lpad<R>(N - m)] = mk<R>(A.x + B.y, B.x - A.y);
     }
     __syncthreads();
-    lds_fft<R>(lb, L, pl, +1);
-    for (int t = threadIdx.x; t < TX * N; t += blockDim.x) {
+    lds_fft<R>(lb, L, S, pl, +1);
+    for (int t = threadIdx.x; t < TX * N; t += FFT_THREADS) {
         const int xo = t % TX, j = t / TX, k = k0 + xo;
         if (k >= p.nxm) continue;
-        const cpx<R> z = lb[(xo >> 1) * N + j];
-        const double v = double((xo & 1) ? z.y : z.x);
-        double *d = p.dph + bidx(p, k, j, i);
-        *d = accumulate ? *d + v : v;
+        const cpx<R> z = lb[(xo >> 1) * S + lpad<R>(j)];
+        p.dph[bidx(p, k, j, i)] = double((xo & 1) ? z.y : z.x);
     }
 }
 
