@@ -281,11 +281,22 @@ __device__ __forceinline__ void column_solve_lds(double *buf, const TriFactors &
     for (int m = 0; m < CH; ++m) buf[lane * lds_chunk<CH>() + m] = v[m];
 }
 
+// two-element vector of the storage type (64-bit or 128-bit lane accesses)
+template <typename RT> struct Pair;
+template <> struct Pair<float> { using T = float2; };
+template <> struct Pair<double> { using T = double2; };
+
 // ExplicitSalinity + ImplicitSalinity + SolveImpEqnUpdate_Sal (salinity.F90:268-454).
 // RT is the storage (and explicit-stencil arithmetic) type of the refined
 // fields; the implicit RHS, the solve and the update run in FP64.
+// Stencil phase: each lane takes two adjacent rows (r0 = 2*lane + 128*m and
+// r0+1) with one vector access per array, so a wavefront access moves 128
+// values; the x-neighbours outside the pair come from two scalar loads that
+// hit the same cache lines.
 template <int CH, typename RT>
 __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_out) {
+    using V = typename Pair<RT>::T;
+    constexpr int M2 = (64 * CH + 127) / 128;          // pair iterations covering the solver's 64*CH rows
     __shared__ double lds[4][lds_col_size<CH>()];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
@@ -295,19 +306,39 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_
     const NbS<RT> n = make_nbs<RT>(p, j, i);
     const long long c0 = (long long)i * p.planer + (long long)j * p.ldxr;
     RT *rus = sel<RT>(p.rusal, p.rusalf) + c0;
-    double own[CH];
+    const unsigned nxmr = p.nxmr;
+    V own[M2];
 #pragma unroll
-    for (int m = 0; m < CH; ++m) {
-        const unsigned r = lane + 64 * m;
-        double rhs = 0.0;
-        own[m] = 0.0;
-        if (live && r < unsigned(p.nxmr)) {
-            const RT h = ex_sal<RT, RT>(p, n, r);
-            rhs = rhs_sal<RT>(p, n, double(h), double(rus[r]), r);
-            rus[r] = h;
-            own[m] = double(n.s[r]);
+    for (int m = 0; m < M2; ++m) {
+        const unsigned r0 = 2 * lane + 128 * m, r1 = r0 + 1;
+        double rhs0 = 0.0, rhs1 = 0.0;
+        own[m] = V{RT(0), RT(0)};
+        if (live && r0 < nxmr) {
+            // r0 even and every row starts 8-aligned: the pair is aligned, and
+            // r0+1 <= nxmr < ldxr stays inside the row
+            const V sc = *reinterpret_cast<const V *>(n.s + r0);
+            const V sjm = *reinterpret_cast<const V *>(n.s_jm + r0), sjp = *reinterpret_cast<const V *>(n.s_jp + r0);
+            const V sim = *reinterpret_cast<const V *>(n.s_im + r0), sip = *reinterpret_cast<const V *>(n.s_ip + r0);
+            const V vx = *reinterpret_cast<const V *>(n.vxr + r0);
+            const V vy = *reinterpret_cast<const V *>(n.vyr + r0), vyp = *reinterpret_cast<const V *>(n.vyr_jp + r0);
+            const V vz = *reinterpret_cast<const V *>(n.vzr + r0), vzp = *reinterpret_cast<const V *>(n.vzr_ip + r0);
+            const V ru = *reinterpret_cast<const V *>(rus + r0);
+            const RT sm = r0 > 0 ? n.s[r0 - 1] : RT(0);
+            const RT sp2 = r1 + 1 < nxmr ? n.s[r1 + 1] : RT(0);
+            const RT vx2 = r1 < nxmr ? n.vxr[r1 + 1] : RT(0);     // r1 + 1 <= nxmr < ldxr
+            const RT h0 = ex_sal_val<RT>(p, r0, sm, sc.x, sc.y, sjm.x, sjp.x, sim.x, sip.x, vx.x, vx.y, vy.x, vyp.x, vz.x, vzp.x, n.sbp, n.stp);
+            rhs0 = rhs_sal_val(p, r0, double(h0), double(ru.x), double(sm), double(sc.x), double(sc.y), n.sbp, n.stp);
+            own[m] = sc;
+            if (r1 < nxmr) {
+                const RT h1 = ex_sal_val<RT>(p, r1, sc.x, sc.y, sp2, sjm.y, sjp.y, sim.y, sip.y, vx.y, vx2, vy.y, vyp.y, vz.y, vzp.y, n.sbp, n.stp);
+                rhs1 = rhs_sal_val(p, r1, double(h1), double(ru.y), double(sc.x), double(sc.y), double(sp2), n.sbp, n.stp);
+                *reinterpret_cast<V *>(rus + r0) = V{h0, h1};
+            } else {
+                rus[r0] = h0;
+            }
         }
-        buf[lds_row<CH>(r)] = rhs;
+        if (r0 < 64u * CH) buf[lds_row<CH>(r0)] = rhs0;
+        if (r1 < 64u * CH) buf[lds_row<CH>(r1)] = rhs1;
     }
     __syncthreads();
     column_solve_lds<CH>(buf, f);
@@ -315,12 +346,14 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_
     if (!live) return;
     RT *out = sal_out + c0;
 #pragma unroll
-    for (int m = 0; m < CH; ++m) {
-        const unsigned r = lane + 64 * m;
-        if (r < unsigned(p.nxmr)) out[r] = RT(own[m] + buf[lds_row<CH>(r)]);
-        else if (r == unsigned(p.nxmr)) out[r] = n.s[r];
+    for (int m = 0; m < M2; ++m) {
+        const unsigned r0 = 2 * lane + 128 * m, r1 = r0 + 1;
+        if (r0 >= nxmr) continue;
+        const RT o0 = RT(double(own[m].x) + buf[lds_row<CH>(r0)]);
+        if (r1 < nxmr) *reinterpret_cast<V *>(out + r0) = V{o0, RT(double(own[m].y) + buf[lds_row<CH>(r1)])};
+        else out[r0] = o0;
     }
-    if (64 * CH == p.nxmr && lane == 0) out[p.nxmr] = n.s[p.nxmr];
+    if (lane == 0) out[nxmr] = n.s[nxmr];       // the unused index nxr is carried over
 }
 
 // Explicit terms of vx, vy, vz, T (+ AddSalBuoyancy) and ImplicitAndUpdate

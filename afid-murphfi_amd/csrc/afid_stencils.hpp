@@ -196,37 +196,49 @@ __device__ __forceinline__ NbS<RT> make_nbs(const KP &p, int j, int i) {
 }
 
 // ExplicitSalinity, salinity.F90:298-363 (non-IBM), refined k = 0..nxmr-1,
-// arithmetic in T, storage RT
-template <typename T, typename RT>
-__device__ __forceinline__ T ex_sal(const KP &p, const NbS<RT> &n, unsigned k) {
+// arithmetic in T, on the values of the row (sm, sc, sp: S at k-1, k, k+1;
+// sym/syp/szm/szp: S at the y/z neighbours; vc/vp: vxr at k and k+1; vy/vyp,
+// vz/vzp: vyr, vzr of the cell and of its +y/+z neighbour)
+template <typename T>
+__device__ __forceinline__ T ex_sal_val(const KP &p, unsigned k, T sm, T sc, T sp, T sym, T syp, T szm, T szp,
+                                        T vc, T vp, T vy, T vyp, T vz, T vzp, double sbp, double stp) {
     const T udyr = T(0.5) * T(p.dyr), udzr = T(0.5) * T(p.dzr), udyrq = T(p.dyqr) / T(p.pecs), udzrq = T(p.dzqr) / T(p.pecs);
-    const T sc = T(n.s[k]), syp = T(n.s_jp[k]), sym = T(n.s_jm[k]), szp = T(n.s_ip[k]), szm = T(n.s_im[k]);
-    const T vp = T(n.vxr[k + 1]), vc = T(n.vxr[k]), u3 = T(p.udx3mr[k]);
+    const T u3 = T(p.udx3mr[k]);
     T hsx;
     if (k == 0)
-        hsx = (vp * (T(n.s[k + 1]) + sc) - vc * T(2.0) * T(n.sbp)) * u3 * T(0.5);
+        hsx = (vp * (sp + sc) - vc * T(2.0) * T(sbp)) * u3 * T(0.5);
     else if (k == unsigned(p.nxmr - 1))
-        hsx = (vp * T(2.0) * T(n.stp) - vc * (sc + T(n.s[k - 1]))) * u3 * T(0.5);
+        hsx = (vp * T(2.0) * T(stp) - vc * (sc + sm)) * u3 * T(0.5);
     else
-        hsx = (vp * (T(n.s[k + 1]) + sc) - vc * (sc + T(n.s[k - 1]))) * u3 * T(0.5);
-    T hsy = (T(n.vyr_jp[k]) * (syp + sc) - T(n.vyr[k]) * (sc + sym)) * udyr;
-    T hsz = (T(n.vzr_ip[k]) * (szp + sc) - T(n.vzr[k]) * (sc + szm)) * udzr;
+        hsx = (vp * (sp + sc) - vc * (sc + sm)) * u3 * T(0.5);
+    T hsy = (vyp * (syp + sc) - vy * (sc + sym)) * udyr;
+    T hsz = (vzp * (szp + sc) - vz * (sc + szm)) * udzr;
     T dyys = (syp - T(2.0) * sc + sym) * udyrq;
     T dzzs = (szp - T(2.0) * sc + szm) * udzrq;
     return -(hsx + hsy + hsz) + dyys + dzzs;
 }
+template <typename T, typename RT>
+__device__ __forceinline__ T ex_sal(const KP &p, const NbS<RT> &n, unsigned k) {
+    const T sm = k > 0 ? T(n.s[k - 1]) : T(0), sp = k < unsigned(p.nxmr - 1) ? T(n.s[k + 1]) : T(0);
+    return ex_sal_val<T>(p, k, sm, T(n.s[k]), sp, T(n.s_jm[k]), T(n.s_jp[k]), T(n.s_im[k]), T(n.s_ip[k]),
+                         T(n.vxr[k]), T(n.vxr[k + 1]), T(n.vyr[k]), T(n.vyr_jp[k]), T(n.vzr[k]), T(n.vzr_ip[k]), n.sbp, n.stp);
+}
 // ImplicitSalinity, salinity.F90:378-400 (FP64 arithmetic in both forms)
+__device__ __forceinline__ double rhs_sal_val(const KP &p, unsigned k, double h, double ru, double sm, double sc,
+                                              double sp, double sbp, double stp) {
+    double dxxs;
+    if (k == 0)
+        dxxs = sp * p.ap3sskr[k] + sc * p.ac3sskr[k] - (p.ap3sskr[k] + p.ac3sskr[k]) * sbp * p.sfixs;
+    else if (k == unsigned(p.nxmr - 1))
+        dxxs = sc * p.ac3sskr[k] + sm * p.am3sskr[k] - (p.am3sskr[k] + p.ac3sskr[k]) * stp * p.sfixn;
+    else
+        dxxs = sp * p.ap3sskr[k] + sc * p.ac3sskr[k] + sm * p.am3sskr[k];
+    return (p.ga * h + p.ro * ru + (p.al / p.pecs) * dxxs) * p.dt;
+}
 template <typename RT>
 __device__ __forceinline__ double rhs_sal(const KP &p, const NbS<RT> &n, double h, double ru, unsigned k) {
-    double dxxs;
-    const double sc = double(n.s[k]);
-    if (k == 0)
-        dxxs = double(n.s[k + 1]) * p.ap3sskr[k] + sc * p.ac3sskr[k] - (p.ap3sskr[k] + p.ac3sskr[k]) * n.sbp * p.sfixs;
-    else if (k == unsigned(p.nxmr - 1))
-        dxxs = sc * p.ac3sskr[k] + double(n.s[k - 1]) * p.am3sskr[k] - (p.am3sskr[k] + p.ac3sskr[k]) * n.stp * p.sfixn;
-    else
-        dxxs = double(n.s[k + 1]) * p.ap3sskr[k] + sc * p.ac3sskr[k] + double(n.s[k - 1]) * p.am3sskr[k];
-    return (p.ga * h + p.ro * ru + (p.al / p.pecs) * dxxs) * p.dt;
+    const double sm = k > 0 ? double(n.s[k - 1]) : 0.0, sp = k < unsigned(p.nxmr - 1) ? double(n.s[k + 1]) : 0.0;
+    return rhs_sal_val(p, k, h, ru, sm, double(n.s[k]), sp, n.sbp, n.stp);
 }
 
 }  // namespace afid
