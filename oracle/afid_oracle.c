@@ -47,6 +47,10 @@ struct orc_state {
     double *ap3sskr, *ac3sskr, *am3sskr;
     int *irangs, *jrangs, *krangs, *irangb;
     double *cxrs, *cyrs, *czrs, *cxsalc, *cysalc, *czsalc;   /* (4, n) column major */
+    /* phase field (phasefield.F90:17-29, ReadInputFile.F90:125-126) */
+    double pf_a, pf_s, pf_tm, pf_d, pf_eps;
+    double *ap3spkr, *ac3spkr, *am3spkr;                     /* (1:nxr), Neumann (CreateMgrdGrid.F90:163) */
+    double *cxphic, *cyphic, *czphic;                        /* refined -> coarse centres, (4, n) */
     /* 3-D fields (x fastest, no halo) */
     double *f[ORC_NFIELDS];
     double *tpdv, *tpdvr;   /* x index -1..nx+1 / -1..nxr+1 */
@@ -305,7 +309,7 @@ orc_state *orc_create(const orc_config *cfg) {
     int mr = cfg->multires || cfg->salinity || cfg->phasefield;
     s->nxmr = mr ? cfg->nxmr : s->nxm; s->nymr = mr ? cfg->nymr : s->nym; s->nzmr = mr ? cfg->nzmr : s->nzm;
     s->nxr = s->nxmr + 1; s->nyr = s->nymr + 1; s->nzr = s->nzmr + 1;
-    if (cfg->phasefield) { fprintf(stderr, "oracle: phase-field not restated\n"); free(s); return NULL; }
+    if (cfg->phasefield && cfg->salinity) { fprintf(stderr, "oracle: phase field with salinity not restated\n"); free(s); return NULL; }
 
     /* derived constants, ReadInputFile.F90:101-122 */
     s->rrho = fabs(cfg->rayt) * cfg->pras / (fabs(cfg->rays) * cfg->prat);
@@ -313,6 +317,12 @@ orc_state *orc_create(const orc_config *cfg) {
     else { s->ren = sqrt(fabs(cfg->rayt) / cfg->prat); s->byct = 1.0; s->bycs = 1.0 / s->rrho; }
     s->pect = s->ren * cfg->prat;
     s->pecs = s->ren * cfg->pras;
+    /* ReadInputFile.F90:125-126: pf_D follows from the Stefan number and A */
+    s->pf_a = cfg->pf_a; s->pf_s = cfg->pf_s; s->pf_tm = cfg->pf_tm;
+    if (cfg->phasefield) {
+        s->pf_d = 1.2 / s->pect / cfg->pf_s / cfg->pf_a;
+        s->pf_eps = cfg->pf_eps > 0.0 ? cfg->pf_eps : 1.0 / s->nxmr;
+    }
 
     /* InitTimeMarchScheme.F90:17-49 */
     if (cfg->nsst > 1) {
@@ -401,11 +411,19 @@ orc_state *orc_create(const orc_config *cfg) {
     s->f[ORC_TEMPTP] = track(s, sizeof(double) * (size_t)nym * nzm);
     s->tpdv = track(s, sizeof(double) * (size_t)(nx + 3) * nym * nzm);
 
-    if (cfg->salinity) {
+    if (cfg->salinity || cfg->phasefield) {
         int nxr = s->nxr, nyr = s->nyr, nzr = s->nzr, nxmr = s->nxmr, nymr = s->nymr, nzmr = s->nzmr;
-        for (int id = ORC_SAL; id <= ORC_VZR; id++) s->f[id] = track(s, sizeof(double) * nref(s));
-        s->f[ORC_SALBP] = track(s, sizeof(double) * (size_t)nymr * nzmr);
-        s->f[ORC_SALTP] = track(s, sizeof(double) * (size_t)nymr * nzmr);
+        if (cfg->salinity) {
+            for (int id = ORC_SAL; id <= ORC_VZR; id++) s->f[id] = track(s, sizeof(double) * nref(s));
+            s->f[ORC_SALBP] = track(s, sizeof(double) * (size_t)nymr * nzmr);
+            s->f[ORC_SALTP] = track(s, sizeof(double) * (size_t)nymr * nzmr);
+        } else {
+            /* phase field: refined phi, its RK arrays, tempr and rhsr; coarse phic */
+            for (int id = ORC_PHI; id <= ORC_HPHI; id++) s->f[id] = track(s, sizeof(double) * nref(s));
+            s->f[ORC_TEMPR] = track(s, sizeof(double) * nref(s));
+            s->f[ORC_RHSR] = track(s, sizeof(double) * nref(s));
+            s->f[ORC_PHIC] = track(s, sizeof(double) * nbase(s));
+        }
         s->tpdvr = track(s, sizeof(double) * (size_t)(nxr + 3) * nymr * nzmr);
         s->xcr = r1(s, 1, nxr); s->xmr = r1(s, 0, nxr); s->ycr = r1(s, 1, nyr); s->ymr = r1(s, 0, nyr);
         s->zcr = r1(s, 1, nzr); s->zmr = r1(s, 0, nzr);
@@ -442,6 +460,8 @@ orc_state *orc_create(const orc_config *cfg) {
         for (int kc = 1; kc <= nxmr; kc++) { s->udx3mr[kc] = s->dxr / s->d3xcr[kc]; s->udx3cr[kc] = s->dxr / s->g3rcr[kc]; }
         s->udx3cr[nxr] = s->dxr / s->g3rcr[nxr];
         second_derivative_coeff(s->ap3sskr, s->ac3sskr, s->am3sskr, s->xmr, nxmr, cfg->alx3, cfg->sfixn, cfg->sfixs);
+        s->ap3spkr = r1(s, 1, nxr); s->ac3spkr = r1(s, 1, nxr); s->am3spkr = r1(s, 1, nxr);
+        second_derivative_coeff(s->ap3spkr, s->ac3spkr, s->am3spkr, s->xmr, nxmr, cfg->alx3, 0, 0);
 
         /* CreateMgrdStencil.F90:17-78 (the stencils the substep uses) */
         s->irangs = i1(s, 0, nx); s->jrangs = i1(s, 0, ny); s->krangs = i1(s, 0, nz); s->irangb = i1(s, 0, nx + 1);
@@ -457,7 +477,8 @@ orc_state *orc_create(const orc_config *cfg) {
         construct_stencil(s->czrs, s->zm, nzm, s->zmr, nzmr, cfg->zlen, s->krangs, 'z');
         {
             int *irangr = malloc(sizeof(int) * (nxr + 1)), *jrangr = malloc(sizeof(int) * (nyr + 1)), *krangr = malloc(sizeof(int) * (nzr + 1));
-            double *cxphic = malloc(sizeof(double) * 4 * nxm);
+            double *cxphic = track(s, sizeof(double) * 4 * nxm);
+            s->cxphic = cxphic; s->cyphic = s->cysalc; s->czphic = s->czsalc;
             interpolation_indices(irangr, s->xmr, nxmr, s->xm, nxm, cfg->alx3);
             construct_stencil(cxphic, s->xmr, nxmr, s->xm, nxm, cfg->alx3, irangr, 'x');
             interpolation_indices(jrangr, s->ymr, nymr, s->ym, nym, cfg->ylen);
@@ -474,7 +495,7 @@ orc_state *orc_create(const orc_config *cfg) {
                 for (int k = 0; k <= nx; k++) s->irangb[k] = s->irangs[k];
                 memcpy(s->cxsalc, cxphic, sizeof(double) * 4 * nxm);
             }
-            free(irangr); free(jrangr); free(krangr); free(cxphic);
+            free(irangr); free(jrangr); free(krangr);
         }
     }
     return s;
@@ -488,7 +509,8 @@ void orc_destroy(orc_state *s) {
 
 double *orc_field(orc_state *s, int id, int64_t dims[3]) {
     if (id < 0 || id >= ORC_NFIELDS || !s->f[id]) return NULL;
-    int refined = (id >= ORC_SAL && id <= ORC_VZR) || id == ORC_SALBP || id == ORC_SALTP;
+    int refined = (id >= ORC_SAL && id <= ORC_VZR) || id == ORC_SALBP || id == ORC_SALTP ||
+                  id == ORC_PHI || id == ORC_RUPHI || id == ORC_HPHI || id == ORC_TEMPR;
     int plane = id == ORC_TEMPBP || id == ORC_TEMPTP || id == ORC_SALBP || id == ORC_SALTP;
     dims[0] = refined ? s->nzmr : s->nzm;
     dims[1] = refined ? s->nymr : s->nym;
@@ -528,8 +550,14 @@ double *orc_grid(orc_state *s, int id, int64_t *n) {
     case ORC_G_APPHK: *n = nxm; return s->apphk + 1;
     default: break;
     }
-    if (!s->c.salinity) { *n = 0; return NULL; }
+    if (!s->c.salinity && !s->c.phasefield) { *n = 0; return NULL; }
     switch (id) {
+    case ORC_G_AP3SPKR: *n = s->nxmr; return s->ap3spkr + 1;
+    case ORC_G_AC3SPKR: *n = s->nxmr; return s->ac3spkr + 1;
+    case ORC_G_AM3SPKR: *n = s->nxmr; return s->am3spkr + 1;
+    case ORC_G_CXPHIC: *n = 4 * s->nxm; return s->cxphic;
+    case ORC_G_CYPHIC: *n = 4 * s->nym; return s->cyphic;
+    case ORC_G_CZPHIC: *n = 4 * s->nzm; return s->czphic;
     case ORC_G_XCR: *n = s->nxr; return s->xcr + 1;
     case ORC_G_XMR: *n = s->nxr; return s->xmr + 1;
     case ORC_G_UDX3MR: *n = s->nxmr; return s->udx3mr + 1;
@@ -548,7 +576,7 @@ double *orc_grid(orc_state *s, int id, int64_t *n) {
 }
 
 int orc_int_grid(orc_state *s, int id, int64_t *n, int32_t **out) {
-    if (!s->c.salinity) return -1;
+    if (!s->c.salinity && !s->c.phasefield) return -1;
     switch (id) {
     case 0: *n = s->nx + 1; *out = s->irangs; return 0;
     case 1: *n = s->ny + 1; *out = s->jrangs; return 0;
@@ -833,6 +861,58 @@ static void solve_columns(orc_state *s, double *q, double *rhs, int lx, int ny, 
     free(a); free(b); free(c); free(sc);
 }
 
+/* SolveImpEqnUpdate_X_pf (phasefield.F90:821-872) and SolveImpEqnUpdate_YZ_pf
+ * (:875-930): one tridiagonal per column, in which a row whose interpolated
+ * phase field exceeds pf_direct_force = 0.9 (phasefield.F90:29) becomes the
+ * identity with right-hand side -q, so that q is forced to zero there.
+ * axis 'x': rows 1 and nx are identity rows with rhs -vx, rows 2..nxm test
+ * 0.5 (phic(kc) + phic(kc+1)); axis 'y'/'z': rows 1..nxm test the average
+ * of phic with its y / z neighbour below.  q(upd0..upd1) += solution; rhs is
+ * left as assembled. */
+static void solve_columns_pf(orc_state *s, double *q, const double *rhs, char axis,
+                             const double *am, const double *ac, const double *ap, double betadx) {
+    BASE_DECL
+    const double *phic = s->f[ORC_PHIC];
+    const int n = axis == 'x' ? nx : nxm;
+    apply_threads();
+#pragma omp parallel
+    {
+        double *a = malloc(sizeof(double) * (n + 1)), *b = malloc(sizeof(double) * (n + 1));
+        double *c = malloc(sizeof(double) * (n + 1)), *f = malloc(sizeof(double) * (n + 1));
+#pragma omp for
+        for (int ic = 1; ic <= nzm; ic++) {
+            int i0 = W(ic, nzm), imm = W(ic - 1, nzm);
+            for (int jc = 1; jc <= nym; jc++) {
+                int j0 = W(jc, nym), jmm = W(jc - 1, nym);
+                for (int k = 0; k < n; k++) { a[k] = 0.0; b[k] = 1.0; c[k] = 0.0; }
+                int k0 = 1, k1 = n;
+                if (axis == 'x') {
+                    f[0] = -q[B(1, j0, i0)];
+                    f[nx - 1] = -q[B(nx, j0, i0)];
+                    k0 = 2; k1 = nxm;
+                }
+                for (int kc = k0; kc <= k1; kc++) {
+                    double philoc;
+                    if (axis == 'x') philoc = 0.5 * (phic[B(kc, j0, i0)] + phic[B(kc + 1, j0, i0)]);
+                    else if (axis == 'y') philoc = 0.5 * (phic[B(kc, j0, i0)] + phic[B(kc, jmm, i0)]);
+                    else philoc = 0.5 * (phic[B(kc, j0, i0)] + phic[B(kc, j0, imm)]);
+                    if (philoc > 0.9) {
+                        f[kc - 1] = -q[B(kc, j0, i0)];
+                    } else {
+                        double ackl_b = 1.0 / (1.0 - ac[kc] * betadx);
+                        if (kc > 1) a[kc - 1] = -am[kc] * betadx * ackl_b;
+                        if (kc < n) c[kc - 1] = -ap[kc] * betadx * ackl_b;
+                        f[kc - 1] = rhs[B(kc, j0, i0)] * ackl_b;
+                    }
+                }
+                orc_tridiag_solve(n, a, b, c, f);
+                for (int kc = k0; kc <= k1; kc++) q[B(kc, j0, i0)] += f[kc - 1];
+            }
+        }
+        free(a); free(b); free(c); free(f);
+    }
+}
+
 /* ImplicitAndUpdateVX.F90:14-89 + SolveImpEqnUpdate_X.F90:12-69 */
 void orc_implicit_and_update_vx(orc_state *s) {
     BASE_DECL
@@ -855,7 +935,8 @@ void orc_implicit_and_update_vx(orc_state *s) {
             }
         }
     }
-    solve_columns(s, vx, rhs, nx, nym, nzm, nx, s->am3ck, s->ac3ck, s->ap3ck, s->beta * s->al, 2, nxm, 2, nxm);
+    if (s->c.phasefield) solve_columns_pf(s, vx, rhs, 'x', s->am3ck, s->ac3ck, s->ap3ck, s->beta * s->al);
+    else solve_columns(s, vx, rhs, nx, nym, nzm, nx, s->am3ck, s->ac3ck, s->ap3ck, s->beta * s->al, 2, nxm, 2, nxm);
     for (size_t c = 0; c < (size_t)nym * nzm; c++) { vx[c * nx + 0] = 0.0; vx[c * nx + nx - 1] = 0.0; }
 }
 
@@ -886,7 +967,8 @@ void orc_implicit_and_update_vy(orc_state *s) {
             }
         }
     }
-    solve_columns(s, vy, rhs, nx, nym, nzm, nxm, s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al, 1, nxm, 1, nxm);
+    if (s->c.phasefield) solve_columns_pf(s, vy, rhs, 'y', s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al);
+    else solve_columns(s, vy, rhs, nx, nym, nzm, nxm, s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al, 1, nxm, 1, nxm);
 }
 
 /* ImplicitAndUpdateVZ.F90:14-80 + SolveImpEqnUpdate_YZ.F90:12-60 */
@@ -910,7 +992,8 @@ void orc_implicit_and_update_vz(orc_state *s) {
             }
         }
     }
-    solve_columns(s, vz, rhs, nx, nym, nzm, nxm, s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al, 1, nxm, 1, nxm);
+    if (s->c.phasefield) solve_columns_pf(s, vz, rhs, 'z', s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al);
+    else solve_columns(s, vz, rhs, nx, nym, nzm, nxm, s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al, 1, nxm, 1, nxm);
 }
 
 /* ImplicitAndUpdateTemp.F90:13-77 + SolveImpEqnUpdate_Temp.F90:11-63 */
@@ -1400,7 +1483,176 @@ void orc_interp_sal_multigrid(orc_state *s) {
     }
 }
 
-/* TimeMarcher.F90:13-165 (no phase field, IBM or moisture) */
+/* ------------------------------------------------------------------------ */
+/* phase field (src/phasefield.F90)                                          */
+/* ------------------------------------------------------------------------ */
+void orc_pf_scalars(orc_state *s, double out[4]) {
+    out[0] = s->pf_d; out[1] = s->pf_eps;
+    out[2] = s->ren * pow(1.51044285 * s->pf_eps, 2);     /* pf_eta, phasefield.F90:531 */
+    out[3] = s->pf_d / (s->pf_eps * s->pf_eps);            /* pf_B, phasefield.F90:357 */
+}
+
+/* ExplicitPhase, phasefield.F90:345-383 */
+void orc_explicit_phase(orc_state *s) {
+    REF_DECL
+    const double *phi = s->f[ORC_PHI], *tempr = s->f[ORC_TEMPR];
+    double *hphi = s->f[ORC_HPHI];
+    double pf_b = s->pf_d / (s->pf_eps * s->pf_eps);
+    double udzrq = s->pf_d * s->dzqr, udyrq = s->pf_d * s->dyqr;
+    apply_threads();
+#pragma omp parallel for
+    for (int ic = 1; ic <= nzmr; ic++) {
+        int i0 = W(ic, nzmr), im = W(ic - 1, nzmr), ip = W(ic + 1, nzmr);
+        for (int jc = 1; jc <= nymr; jc++) {
+            int j0 = W(jc, nymr), jm = W(jc - 1, nymr), jp = W(jc + 1, nymr);
+            for (int kc = 1; kc <= nxmr; kc++) {
+                double pc = phi[R(kc, j0, i0)];
+                double dyyp = (phi[R(kc, jp, i0)] - 2.0 * pc + phi[R(kc, jm, i0)]) * udyrq;
+                double dzzp = (phi[R(kc, j0, ip)] - 2.0 * pc + phi[R(kc, j0, im)]) * udzrq;
+                double nlphi = pf_b * pc * (1.0 - pc) * (1.0 - 2.0 * pc + s->pf_a * (tempr[R(kc, j0, i0)] - s->pf_tm));
+                hphi[R(kc, j0, i0)] = dyyp + dzzp - nlphi;
+            }
+        }
+    }
+}
+
+/* AddVolumePenalty, phasefield.F90:525-566 */
+void orc_add_volume_penalty(orc_state *s) {
+    BASE_DECL
+    const double *phic = s->f[ORC_PHIC], *vx = s->f[ORC_VX], *vy = s->f[ORC_VY], *vz = s->f[ORC_VZ];
+    double *qcap = s->f[ORC_QCAP], *dph = s->f[ORC_DPH], *dq = s->f[ORC_DQ];
+    double pf_eta = s->ren * pow(1.51044285 * s->pf_eps, 2), heta = 0.5 / pf_eta;
+    for (int ic = 1; ic <= nzm; ic++) {
+        int i0 = W(ic, nzm), imm = W(ic - 1, nzm);
+        for (int jc = 1; jc <= nym; jc++) {
+            int j0 = W(jc, nym), jmm = W(jc - 1, nym);
+            for (int kc = 2; kc <= nxm; kc++)
+                qcap[B(kc, j0, i0)] -= (phic[B(kc, j0, i0)] + phic[B(kc - 1, j0, i0)]) * vx[B(kc, j0, i0)] * heta;
+            for (int kc = 1; kc <= nxm; kc++)
+                dph[B(kc, j0, i0)] -= (phic[B(kc, j0, i0)] + phic[B(kc, jmm, i0)]) * vy[B(kc, j0, i0)] * heta;
+            for (int kc = 1; kc <= nxm; kc++)
+                dq[B(kc, j0, i0)] -= (phic[B(kc, j0, i0)] + phic[B(kc, j0, imm)]) * vz[B(kc, j0, i0)] * heta;
+        }
+    }
+}
+
+/* ImplicitPhase + SolveImpEqnUpdate_Phi, phasefield.F90:386-466; rhsr is
+ * left holding phi(l+1) - phi(l) for AddLatentHeat */
+void orc_implicit_phase(orc_state *s) {
+    REF_DECL
+    double *phi = s->f[ORC_PHI], *ruphi = s->f[ORC_RUPHI], *rhsr = s->f[ORC_RHSR];
+    const double *hphi = s->f[ORC_HPHI];
+    double alpec = s->al * s->pf_d;
+    apply_threads();
+#pragma omp parallel for
+    for (int ic = 1; ic <= nzmr; ic++) {
+        int i0 = W(ic, nzmr);
+        for (int jc = 1; jc <= nymr; jc++) {
+            int j0 = W(jc, nymr);
+            for (int kc = 1; kc <= nxmr; kc++) {
+                double dxxp;
+                if (kc == 1)
+                    dxxp = phi[R(kc + 1, j0, i0)] * s->ap3spkr[kc] + phi[R(kc, j0, i0)] * s->ac3spkr[kc];
+                else if (kc == nxmr)
+                    dxxp = phi[R(kc, j0, i0)] * s->ac3spkr[kc] + phi[R(kc - 1, j0, i0)] * s->am3spkr[kc];
+                else
+                    dxxp = phi[R(kc + 1, j0, i0)] * s->ap3spkr[kc] + phi[R(kc, j0, i0)] * s->ac3spkr[kc]
+                         + phi[R(kc - 1, j0, i0)] * s->am3spkr[kc];
+                rhsr[R(kc, j0, i0)] = (s->ga * hphi[R(kc, j0, i0)] + s->ro * ruphi[R(kc, j0, i0)] + alpec * dxxp) * s->dt;
+                ruphi[R(kc, j0, i0)] = hphi[R(kc, j0, i0)];
+            }
+        }
+    }
+    solve_columns(s, phi, rhsr, nxr, nymr, nzmr, nxmr, s->am3spkr, s->ac3spkr, s->ap3spkr,
+                  0.5 * s->al * s->dt * s->pf_d, 1, nxmr, 1, nxmr);
+}
+
+/* Refined field with x ghosts (tpdvr) to the coarse cell centres through the
+ * fast loop of interpolate_xyz_to_coarse_fast, "phi" stencils
+ * (HermiteInterpolations.F90:256-312, cxphic/irangs): out = v (acc = 0) or
+ * out += scale * v (acc = 1) for kc = 1..nxm. */
+static void pf_to_coarse(orc_state *s, const double *tpdvr, double *out, double scale, int acc) {
+    BASE_DECL REF_DECL
+    if (!(s->xmr[1] < s->xm[1] && s->xmr[nxmr] > s->xm[nxm])) {
+        fprintf(stderr, "oracle: refined-to-coarse general path not restated\n");
+        abort();
+    }
+    apply_threads();
+#pragma omp parallel for
+    for (int ic = 1; ic <= nzm; ic++) {
+        int icr = s->krangs[ic];
+        for (int jc = 1; jc <= nym; jc++) {
+            int jcr = s->jrangs[jc];
+            for (int kc = 1; kc <= nxm; kc++) {
+                int kcr = s->irangs[kc];
+                double qv3[4][4][4];
+                for (int c = 0; c < 4; c++)
+                    for (int b = 0; b < 4; b++)
+                        for (int a = 0; a < 4; a++)
+                            qv3[c][b][a] = tpdvr[TR(kcr - 2 + a, W(jcr - 2 + b, nymr), W(icr - 2 + c, nzmr))];
+                const double *cz = s->czphic + (size_t)(ic - 1) * 4, *cy = s->cyphic + (size_t)(jc - 1) * 4, *cx = s->cxphic + (size_t)(kc - 1) * 4;
+                double qv2[4][4], qv1[4];
+                for (int b = 0; b < 4; b++)
+                    for (int a = 0; a < 4; a++)
+                        qv2[b][a] = qv3[0][b][a] * cz[0] + qv3[1][b][a] * cz[1] + qv3[2][b][a] * cz[2] + qv3[3][b][a] * cz[3];
+                for (int a = 0; a < 4; a++) qv1[a] = qv2[0][a] * cy[0] + qv2[1][a] * cy[1] + qv2[2][a] * cy[2] + qv2[3][a] * cy[3];
+                double v = qv1[0] * cx[0] + qv1[1] * cx[1] + qv1[2] * cx[2] + qv1[3] * cx[3];
+                size_t o = B(kc, W(jc, nym), W(ic, nzm));
+                if (acc) out[o] += scale * v; else out[o] = v;
+            }
+        }
+    }
+}
+
+/* copy a refined field into tpdvr with the zero-gradient x ghosts
+ * tpdvr(0) = f(1), tpdvr(nxr) = f(nxmr) (phasefield.F90:479-487, :674-683) */
+static void fill_neumann_ghosts(orc_state *s, const double *f) {
+    REF_DECL
+    double *tpdvr = s->tpdvr;
+    memset(tpdvr, 0, sizeof(double) * (size_t)(nxr + 3) * nymr * nzmr);
+    for (int i = 0; i < nzmr; i++)
+        for (int j = 0; j < nymr; j++) {
+            for (int k = 1; k <= nxmr; k++) tpdvr[TR(k, j, i)] = f[R(k, j, i)];
+            tpdvr[TR(0, j, i)] = f[R(1, j, i)];
+            tpdvr[TR(nxr, j, i)] = f[R(nxmr, j, i)];
+        }
+}
+
+/* AddLatentHeat, phasefield.F90:651-725 (fast path): hro += pf_S d(phi)/dt
+ * on the coarse grid, d(phi) from rhsr after ImplicitPhase */
+void orc_add_latent_heat(orc_state *s) {
+    fill_neumann_ghosts(s, s->f[ORC_RHSR]);
+    pf_to_coarse(s, s->tpdvr, s->f[ORC_HRO], s->pf_s / (s->al * s->dt), 1);
+}
+
+/* InterpPhiMultigrid, phasefield.F90:470-494 (fast path) */
+void orc_interp_phi_multigrid(orc_state *s) {
+    memset(s->f[ORC_PHIC], 0, sizeof(double) * nbase(s));
+    fill_neumann_ghosts(s, s->f[ORC_PHI]);
+    pf_to_coarse(s, s->tpdvr, s->f[ORC_PHIC], 1.0, 0);
+}
+
+/* InterpTempMultigrid, phasefield.F90:498-520 */
+void orc_interp_temp_multigrid(orc_state *s) {
+    BASE_DECL REF_DECL
+    const double *temp = s->f[ORC_TEMP], *tbp = s->f[ORC_TEMPBP], *ttp = s->f[ORC_TEMPTP];
+    double *tpdv = s->tpdv, *tpdvr = s->tpdvr, *tempr = s->f[ORC_TEMPR];
+    memset(tpdv, 0, sizeof(double) * (size_t)(nx + 3) * nym * nzm);
+    memset(tpdvr, 0, sizeof(double) * (size_t)(nxr + 3) * nymr * nzmr);
+    memset(tempr, 0, sizeof(double) * nref(s));
+    for (int i = 0; i < nzm; i++)
+        for (int j = 0; j < nym; j++) {
+            tpdv[TP(0, j, i)] = (1.0 - 2.0 * s->c.tfixs) * temp[B(1, j, i)] + 2.0 * s->c.tfixs * tbp[P2(j, i)];
+            tpdv[TP(nx, j, i)] = (1.0 - 2.0 * s->c.tfixn) * temp[B(nxm, j, i)] + 2.0 * s->c.tfixn * ttp[P2(j, i)];
+            for (int k = 1; k <= nxm; k++) tpdv[TP(k, j, i)] = temp[B(k, j, i)];
+        }
+    interp_to_refined(s, tpdv, tpdvr);
+    for (int i = 0; i < nzmr; i++)
+        for (int j = 0; j < nymr; j++)
+            for (int k = 1; k <= nxmr; k++) tempr[R(k, j, i)] = tpdvr[TR(k, j, i)];
+}
+
+/* TimeMarcher.F90:13-165 (no IBM or moisture) */
 void orc_time_marcher(orc_state *s, double dt) {
     for (int ns = 1; ns <= s->c.nsst; ns++) {
         orc_set_substep(s, dt, ns);
@@ -1411,6 +1663,12 @@ void orc_time_marcher(orc_state *s, double dt) {
         if (s->c.salinity) {
             orc_explicit_salinity(s);
             if (s->c.active_s == 1) orc_add_sal_buoyancy(s);
+        }
+        if (s->c.phasefield) {
+            orc_explicit_phase(s);
+            orc_add_volume_penalty(s);
+            orc_implicit_phase(s);
+            orc_add_latent_heat(s);
         }
         orc_implicit_and_update_vx(s);
         orc_implicit_and_update_vy(s);
@@ -1424,6 +1682,10 @@ void orc_time_marcher(orc_state *s, double dt) {
         if (s->c.salinity) {
             orc_interp_vel_mgrd(s);
             orc_interp_sal_multigrid(s);
+        }
+        if (s->c.phasefield) {
+            orc_interp_temp_multigrid(s);
+            orc_interp_phi_multigrid(s);
         }
     }
 }

@@ -31,6 +31,9 @@ typedef struct orc_config {
     int32_t inslws, inslwn, tfixs, tfixn, sfixs, sfixn;
     int32_t active_t, active_s, gaxis;
     double xplusu, xminusu, dpdy, dpdz;
+    /* phase field: pf_A, pf_S, pf_Tm of the bou.in PF line; interface width
+       pf_eps (0 selects the reference's 1/nxmr, ReadInputFile.F90:126) */
+    double pf_a, pf_s, pf_tm, pf_eps;
 } orc_config;
 
 typedef struct orc_state orc_state;
@@ -42,6 +45,7 @@ enum {
     ORC_QCAP, ORC_DPH, ORC_DQ, ORC_HRO, ORC_RHS,
     ORC_SAL, ORC_RUSAL, ORC_HSAL, ORC_RHSR, ORC_VXR, ORC_VYR, ORC_VZR, ORC_SALC,
     ORC_TEMPBP, ORC_TEMPTP, ORC_SALBP, ORC_SALTP,
+    ORC_PHI, ORC_RUPHI, ORC_HPHI, ORC_PHIC, ORC_TEMPR,
     ORC_NFIELDS
 };
 
@@ -56,6 +60,7 @@ enum {
     ORC_G_XCR, ORC_G_XMR, ORC_G_UDX3MR, ORC_G_G3RMR,
     ORC_G_AP3SSKR, ORC_G_AC3SSKR, ORC_G_AM3SSKR,
     ORC_G_CXRS, ORC_G_CYRS, ORC_G_CZRS, ORC_G_CXSALC, ORC_G_CYSALC, ORC_G_CZSALC,
+    ORC_G_AP3SPKR, ORC_G_AC3SPKR, ORC_G_AM3SPKR, ORC_G_CXPHIC, ORC_G_CYPHIC, ORC_G_CZPHIC,
     ORC_G_NGRID
 };
 
@@ -84,6 +89,14 @@ void orc_add_sal_buoyancy(orc_state *s);
 void orc_implicit_salinity(orc_state *s);
 void orc_interp_vel_mgrd(orc_state *s);
 void orc_interp_sal_multigrid(orc_state *s);
+/* phase field (phasefield.F90) */
+void orc_explicit_phase(orc_state *s);
+void orc_add_volume_penalty(orc_state *s);
+void orc_implicit_phase(orc_state *s);
+void orc_add_latent_heat(orc_state *s);
+void orc_interp_temp_multigrid(orc_state *s);
+void orc_interp_phi_multigrid(orc_state *s);
+void orc_pf_scalars(orc_state *s, double out[4]);   /* pf_D, pf_eps, pf_eta, pf_B */
 void orc_time_marcher(orc_state *s, double dt);
 
 void orc_calc_max_cfl(orc_state *s, double *cflm, double *cflmr);

@@ -14,14 +14,16 @@ LIB_PATH = os.path.join(HERE, "liborc.so")
 FIELDS = ["vx", "vy", "vz", "temp", "pr", "rux", "ruy", "ruz", "rutemp",
           "qcap", "dph", "dq", "hro", "rhs",
           "sal", "rusal", "hsal", "rhsr", "vxr", "vyr", "vzr", "salc",
-          "tempbp", "temptp", "salbp", "saltp"]
+          "tempbp", "temptp", "salbp", "saltp",
+          "phi", "ruphi", "hphi", "phic", "tempr"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 
 GRID = ["xc", "xm", "g3rc", "g3rm", "d3xc", "d3xm", "udx3c", "udx3m",
         "ap3ck", "ac3ck", "am3ck", "ap3sk", "ac3sk", "am3sk", "ap3ssk", "ac3ssk", "am3ssk",
         "ak1", "ak2", "amphk", "acphk", "apphk",
         "xcr", "xmr", "udx3mr", "g3rmr", "ap3sskr", "ac3sskr", "am3sskr",
-        "cxrs", "cyrs", "czrs", "cxsalc", "cysalc", "czsalc"]
+        "cxrs", "cyrs", "czrs", "cxsalc", "cysalc", "czsalc",
+        "ap3spkr", "ac3spkr", "am3spkr", "cxphic", "cyphic", "czphic"]
 GRID_ID = {n: i for i, n in enumerate(GRID)}
 
 
@@ -39,6 +41,7 @@ class OrcConfig(C.Structure):
         ("sfixs", C.c_int32), ("sfixn", C.c_int32),
         ("active_t", C.c_int32), ("active_s", C.c_int32), ("gaxis", C.c_int32),
         ("xplusu", C.c_double), ("xminusu", C.c_double), ("dpdy", C.c_double), ("dpdz", C.c_double),
+        ("pf_a", C.c_double), ("pf_s", C.c_double), ("pf_tm", C.c_double), ("pf_eps", C.c_double),
     ]
 
 
@@ -68,7 +71,9 @@ def lib():
                      "implicit_and_update_vx", "implicit_and_update_vy", "implicit_and_update_vz",
                      "implicit_and_update_temp", "calc_local_divergence", "solve_pressure_correction",
                      "correct_velocity", "correct_pressure", "explicit_salinity", "add_sal_buoyancy",
-                     "implicit_salinity", "interp_vel_mgrd", "interp_sal_multigrid"]:
+                     "implicit_salinity", "interp_vel_mgrd", "interp_sal_multigrid",
+                     "explicit_phase", "add_volume_penalty", "implicit_phase", "add_latent_heat",
+                     "interp_temp_multigrid", "interp_phi_multigrid"]:
             getattr(L, "orc_" + name).argtypes = [vp]
         L.orc_calc_max_cfl.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_check_divergence.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -78,6 +83,7 @@ def lib():
         L.orc_dft_naive.argtypes = [dp, dp, dp, dp, C.c_int, C.c_int]
         L.orc_tridiag_solve.argtypes = [C.c_int, dp, dp, dp, dp]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_pf_scalars.argtypes = [vp, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -135,6 +141,11 @@ class Oracle:
         self.L.orc_scalars(self.h, out)
         return dict(zip(["ren", "pect", "pecs", "byct", "bycs", "dx", "dy", "dz"], list(out)))
 
+    def pf_scalars(self):
+        out = (C.c_double * 4)()
+        self.L.orc_pf_scalars(self.h, out)
+        return dict(zip(["pf_d", "pf_eps", "pf_eta", "pf_b"], list(out)))
+
     def set_substep(self, dt, ns):
         self.L.orc_set_substep(self.h, dt, ns)
 
@@ -164,7 +175,8 @@ DEFAULTS = dict(nxm=16, nym=16, nzm=16, nsst=3, multires=0, nxmr=0, nymr=0, nzmr
                 salinity=0, phasefield=0, alx3=1.0, ylen=1.0, zlen=1.0,
                 istr3=8, str3=0.0, istr3r=8, rayt=1e5, prat=1.0, rays=-1e5, pras=1.0, ffscales=0,
                 inslws=1, inslwn=1, tfixs=1, tfixn=1, sfixs=1, sfixn=1,
-                active_t=1, active_s=0, gaxis=1, xplusu=0.0, xminusu=0.0, dpdy=0.0, dpdz=0.0)
+                active_t=1, active_s=0, gaxis=1, xplusu=0.0, xminusu=0.0, dpdy=0.0, dpdz=0.0,
+                pf_a=1.0, pf_s=1.0, pf_tm=0.0, pf_eps=0.0)
 
 
 def make_config(d):
