@@ -21,6 +21,7 @@
 #include "afid_stencils.hpp"
 #include "afid_kernels.hpp"
 #include "afid_poisson.hpp"
+#include "afid_phase.hpp"
 
 using namespace afid;
 
@@ -65,6 +66,16 @@ struct afid_ctx {
     float *sal_alt_f = nullptr;
     double *tpdv = nullptr;
     void *spec = nullptr;              // cpx<double> (FP64 form) or cpx<float> (mixed form)
+    // phase field
+    bool pf = false;
+    double pf_d = 0, pf_eps = 0, pf_b = 0, heta = 0, pf_r0 = 0.1;
+    std::vector<double> h_ap3spkr, h_ac3spkr, h_am3spkr;
+    const double *d_ap3spkr = nullptr, *d_ac3spkr = nullptr, *d_am3spkr = nullptr;
+    const int *d_xp_src = nullptr;
+    const double *d_xp_w = nullptr;
+    double *phi_alt = nullptr;
+    int phic_xr = 0;
+    size_t lds_phic = 0;
     // arithmetic form
     bool mixed = false;
     double div_tol = 1e-10;
@@ -93,7 +104,7 @@ struct afid_ctx {
     FftPlan<double> plan_y, plan_z;
     FftPlan<float> plan_yf, plan_zf;
     int Ly = 1, Lz = 1, lds_y = 0, lds_z = 0;
-    PoissonFactors<double> pf{};
+    PoissonFactors<double> pois{};
     PoissonFactors<float> pff{};
     int pchunk = 1;
     // LDS plan of the transfer kernels
@@ -159,7 +170,8 @@ static std::vector<double> farr_range(const FArr &a, int lo, int hi) {
 }
 
 static bool is_refined(int f) {
-    return (f >= AFID_SAL && f <= AFID_VZR) || f == AFID_SALBP || f == AFID_SALTP;
+    return (f >= AFID_SAL && f <= AFID_VZR) || f == AFID_SALBP || f == AFID_SALTP ||
+           f == AFID_PHI || f == AFID_RUPHI || f == AFID_HPHI || f == AFID_TEMPR;
 }
 static bool is_plane(int f) {
     return f == AFID_TEMPBP || f == AFID_TEMPTP || f == AFID_SALBP || f == AFID_SALTP;
@@ -202,6 +214,12 @@ static KP make_kp(const afid_ctx *c) {
     p.ruxf = c->g[AFID_RUX]; p.ruyf = c->g[AFID_RUY]; p.ruzf = c->g[AFID_RUZ]; p.rutf = c->g[AFID_RUTEMP]; p.salcf = c->g[AFID_SALC];
     p.xs_wf = c->f_xs_w; p.ys_wf = c->f_ys_w; p.zs_wf = c->f_zs_w; p.xb_wf = c->f_xb_w; p.yb_wf = c->f_yb_w; p.zb_wf = c->f_zb_w;
     p.lxaf = c->f_lxa;
+    p.phi = c->f[AFID_PHI]; p.ruphi = c->f[AFID_RUPHI]; p.hphi = c->f[AFID_HPHI]; p.tempr = c->f[AFID_TEMPR];
+    p.phic = c->f[AFID_PHIC]; p.qlat = c->f[AFID_HRO];
+    p.pf_d = c->pf_d; p.pf_b = c->pf_b; p.pf_a = c->c.pf_a; p.pf_tm = c->c.pf_tm; p.pf_s = c->c.pf_s; p.heta = c->heta;
+    p.pf = c->pf ? 1 : 0;
+    p.ap3spkr = c->d_ap3spkr; p.ac3spkr = c->d_ac3spkr; p.am3spkr = c->d_am3spkr;
+    p.xp_src = c->d_xp_src; p.xp_w = c->d_xp_w;
     return p;
 }
 
@@ -233,12 +251,17 @@ static int check_launch(afid_ctx *c, const char *what) {
     return AFID_OK;
 }
 
+// arrays per tridiagonal kind and substep: L, E, G, S (factors) and A, C
+// (normalised off-diagonals, for masked solves)
+constexpr int kTriArrays = 6;
+
 // ---------------------------------------------------------------------------
-// tridiagonal factors (SolveImpEqnUpdate_X/YZ/Temp/Sal matrices, computed
+// tridiagonal factors (SolveImpEqnUpdate_X/YZ/Temp/Sal/Phi matrices, computed
 // on the host for each (dt, substep) and cached while dt is unchanged)
 // ---------------------------------------------------------------------------
 static void lu_rows(const std::vector<double> &am, const std::vector<double> &ac, const std::vector<double> &ap,
-                    double bd, int n, int r0, int r1, int npad, double *L, double *E, double *G, double *S) {
+                    double bd, int n, int r0, int r1, int npad, double *L, double *E, double *G, double *S,
+                    double *A, double *C) {
     // rows r (0-based) in [r0, r1) carry coefficients; others are identity rows with zero RHS
     std::vector<double> a(size_t(npad), 0.0), cc(size_t(npad), 0.0), s(size_t(npad), 0.0);
     for (int r = r0; r < r1; ++r) {
@@ -249,10 +272,11 @@ static void lu_rows(const std::vector<double> &am, const std::vector<double> &ac
     }
     double u_prev = 1.0;
     for (int r = 0; r < npad; ++r) {
-        if (r >= n) { L[r] = 0; E[r] = 0; G[r] = 0; S[r] = 0; continue; }
+        if (r >= n) { L[r] = 0; E[r] = 0; G[r] = 0; S[r] = 0; A[r] = 0; C[r] = 0; continue; }
         const double l = r > 0 ? a[size_t(r)] / u_prev : 0.0;
         const double u = 1.0 - l * (r > 0 ? cc[size_t(r - 1)] : 0.0);
         L[r] = l; E[r] = cc[size_t(r)] / u; G[r] = 1.0 / u; S[r] = s[size_t(r)];
+        A[r] = a[size_t(r)]; C[r] = cc[size_t(r)];
         u_prev = u;
     }
 }
@@ -260,25 +284,29 @@ static void lu_rows(const std::vector<double> &am, const std::vector<double> &ac
 static int ensure_factors(afid_ctx *c) {
     if (c->fact_dt == c->dt) return AFID_OK;
     const int np = c->npadmax;
-    c->tri_host.assign(size_t(3 * MAT_NKIND * 4 * np), 0.0);
+    c->tri_host.assign(size_t(3 * MAT_NKIND * kTriArrays * np), 0.0);
     for (int s = 0; s < c->c.nsst && s < 3; ++s) {
         const double al = c->alm[s], beta = c->dt / c->ren * 0.5;
         for (int kind = 0; kind < MAT_NKIND; ++kind) {
             if (kind == MAT_S && !c->sal) continue;
-            double *base = c->tri_host.data() + size_t((s * MAT_NKIND + kind) * 4 * np);
-            double *L = base, *E = base + np, *G = base + 2 * np, *S = base + 3 * np;
+            if (kind == MAT_P && !c->pf) continue;
+            double *base = c->tri_host.data() + size_t((s * MAT_NKIND + kind) * kTriArrays * np);
+            double *L = base, *E = base + np, *G = base + 2 * np, *S = base + 3 * np, *A = base + 4 * np, *C = base + 5 * np;
             switch (kind) {
             case MAT_X:   // SolveImpEqnUpdate_X.F90:22-54: identity rows 1 and nx
-                lu_rows(c->h_am3ck, c->h_ac3ck, c->h_ap3ck, beta * al, c->nx, 1, c->nxm, np, L, E, G, S);
+                lu_rows(c->h_am3ck, c->h_ac3ck, c->h_ap3ck, beta * al, c->nx, 1, c->nxm, np, L, E, G, S, A, C);
                 break;
             case MAT_YZ:  // SolveImpEqnUpdate_YZ.F90:25-44
-                lu_rows(c->h_am3sk, c->h_ac3sk, c->h_ap3sk, beta * al, c->nxm, 0, c->nxm, np, L, E, G, S);
+                lu_rows(c->h_am3sk, c->h_ac3sk, c->h_ap3sk, beta * al, c->nxm, 0, c->nxm, np, L, E, G, S, A, C);
                 break;
             case MAT_T:   // SolveImpEqnUpdate_Temp.F90:25-46
-                lu_rows(c->h_am3ssk, c->h_ac3ssk, c->h_ap3ssk, 0.5 * al * c->dt / c->pect, c->nxm, 0, c->nxm, np, L, E, G, S);
+                lu_rows(c->h_am3ssk, c->h_ac3ssk, c->h_ap3ssk, 0.5 * al * c->dt / c->pect, c->nxm, 0, c->nxm, np, L, E, G, S, A, C);
                 break;
             case MAT_S:   // salinity.F90:418-440
-                lu_rows(c->h_am3sskr, c->h_ac3sskr, c->h_ap3sskr, 0.5 * al * c->dt / c->pecs, c->nxmr, 0, c->nxmr, np, L, E, G, S);
+                lu_rows(c->h_am3sskr, c->h_ac3sskr, c->h_ap3sskr, 0.5 * al * c->dt / c->pecs, c->nxmr, 0, c->nxmr, np, L, E, G, S, A, C);
+                break;
+            case MAT_P:   // SolveImpEqnUpdate_Phi, phasefield.F90:427-443
+                lu_rows(c->h_am3spkr, c->h_ac3spkr, c->h_ap3spkr, 0.5 * al * c->dt * c->pf_d, c->nxmr, 0, c->nxmr, np, L, E, G, S, A, C);
                 break;
             }
         }
@@ -291,9 +319,9 @@ static int ensure_factors(afid_ctx *c) {
 
 static TriFactors factors(const afid_ctx *c, int kind) {
     const int np = c->npadmax;
-    const double *base = c->tri + size_t(((c->ns - 1) * MAT_NKIND + kind) * 4 * np);
+    const double *base = c->tri + size_t(((c->ns - 1) * MAT_NKIND + kind) * kTriArrays * np);
     TriFactors f;
-    f.l = base; f.e = base + np; f.g = base + 2 * np; f.s = base + 3 * np;
+    f.l = base; f.e = base + np; f.g = base + 2 * np; f.s = base + 3 * np; f.a = base + 4 * np; f.c = base + 5 * np;
     f.n = c->nrows[kind]; f.chunk = c->chunk[kind];
     return f;
 }
@@ -308,7 +336,7 @@ static void launch_tri(afid_ctx *c, const TriFactors &f, double *rhs, double *q,
 // keep = 1: leave the increment in rhs, as the reference's rhs array holds it
 static int run_tri(afid_ctx *c, int kind, double *rhs, double *q, int keep = 1) {
     TriFactors f = factors(c, kind);
-    const bool ref = kind == MAT_S;
+    const bool ref = kind == MAT_S || kind == MAT_P;
     const int ldx = ref ? c->ldxr : c->ldx, ny = ref ? c->nyr : c->ny, nz = ref ? c->nzr : c->nz;
     const long long plane = (long long)ny * ldx, ncols = (long long)ny * nz;
     int u0 = 0, u1 = f.n, z0 = -1, z1 = -1;
@@ -362,7 +390,6 @@ static int pick_lines(int N, int maxlds, int cap) {
 static int setup(afid_ctx *c) {
     const afid_config &g = c->c;
     if (g.nxm < 2 || g.nym < 1 || g.nzm < 1) return set_err(c, AFID_E_ARG, "grid sizes must be nxm >= 2, nym >= 1, nzm >= 1");
-    if (g.phasefield) return set_err(c, AFID_E_ARG, "phase-field configurations are not available in this build");
     if (g.nsst != 1 && g.nsst != 3) return set_err(c, AFID_E_ARG, "nsst must be 1 (Adams-Bashforth) or 3 (RK3)");
     if (g.precision != AFID_FP64 && g.precision != AFID_MIXED) return set_err(c, AFID_E_ARG, "precision must be AFID_FP64 or AFID_MIXED");
     c->mixed = g.precision == AFID_MIXED;
@@ -371,11 +398,16 @@ static int setup(afid_ctx *c) {
     if (c->max_sweeps > 4) return set_err(c, AFID_E_ARG, "max_sweeps must be 1..4");
     c->sal = g.salinity != 0;
     if (c->sal && !g.multires) return set_err(c, AFID_E_ARG, "salinity requires the refined grid (MULTIRES=1)");
+    c->pf = g.phasefield != 0;
+    if (c->pf && !g.multires) return set_err(c, AFID_E_ARG, "the phase field requires the refined grid (MULTIRES=1)");
+    if (c->pf && c->sal) return set_err(c, AFID_E_ARG, "phase field together with salinity is not supported");
+    if (c->pf && c->mixed) return set_err(c, AFID_E_ARG, "the phase field runs in the FP64 form only");
+    if (c->pf && !(g.pf_a != 0.0 && g.pf_s != 0.0)) return set_err(c, AFID_E_ARG, "phase field needs pf_A and pf_S nonzero");
     c->nxm = g.nxm; c->nx = g.nxm + 1; c->ny = g.nym; c->nz = g.nzm;
     c->ldx = (c->nx + 7) / 8 * 8;
     c->ldt = (c->nx + 3 + 7) / 8 * 8;
     c->nymh = c->ny / 2 + 1;
-    if (c->sal) {
+    if (c->sal || c->pf) {
         c->nxmr = g.nxmr; c->nxr = g.nxmr + 1; c->nyr = g.nymr; c->nzr = g.nzmr;
         c->ldxr = (c->nxr + 7) / 8 * 8;
         if (c->nxmr < 2 || c->nyr < 2 || c->nzr < 2) return set_err(c, AFID_E_ARG, "refined grid too small");
@@ -386,6 +418,13 @@ static int setup(afid_ctx *c) {
     else { c->ren = std::sqrt(std::fabs(g.rayt) / g.prat); c->byct = 1.0; c->bycs = 1.0 / c->rrho; }
     c->pect = c->ren * g.prat;
     c->pecs = c->ren * g.pras;
+    if (c->pf) {       // ReadInputFile.F90:125-126, phasefield.F90:357,531
+        c->pf_d = 1.2 / c->pect / g.pf_s / g.pf_a;
+        c->pf_eps = g.pf_eps > 0.0 ? g.pf_eps : 1.0 / g.nxmr;
+        c->pf_b = c->pf_d / (c->pf_eps * c->pf_eps);
+        c->heta = 0.5 / (c->ren * std::pow(1.51044285 * c->pf_eps, 2));
+        c->pf_r0 = g.pf_r0 > 0.0 ? g.pf_r0 : 0.1;
+    }
     // ExplicitTermsVY.F90:30-34
     c->Gy = (g.inslwn == 0 || g.inslws == 0) ? g.dpdy * g.dpdy / (c->ren * c->ren) : 8.0 * g.dpdy * g.dpdy / (c->ren * c->ren);
     // InitTimeMarchScheme.F90:17-49
@@ -479,7 +518,7 @@ static int setup(afid_ctx *c) {
     // step); dph holds the FP64 pressure correction.
     const size_t nb = size_t(c->nz) * c->ny * c->ldx;
     for (int f = 0; f < AFID_NFIELDS; ++f) {
-        if (is_refined(f) || f == AFID_RHS || is_plane(f)) continue;
+        if (is_refined(f) || f == AFID_RHS || is_plane(f) || f == AFID_PHIC) continue;
         if (c->mixed && (f == AFID_QCAP || f == AFID_DQ || f == AFID_HRO)) continue;
         if (c->mixed && ((f >= AFID_RUX && f <= AFID_RUTEMP) || f == AFID_SALC)) {
             if (f == AFID_SALC && !c->sal) continue;
@@ -552,7 +591,7 @@ static int setup(afid_ctx *c) {
         if ((rc = dalloc(c, reinterpret_cast<void **>(&fe), sizeof(double) * size_t(nmodes) * npad))) return rc;
         if ((rc = upload_vec(c, ak1, &dak1))) return rc;
         if ((rc = upload_vec(c, ak2, &dak2))) return rc;
-        c->pf.m = fm; c->pf.g = fg; c->pf.e = fe; c->pf.npad = npad;
+        c->pois.m = fm; c->pois.g = fg; c->pois.e = fe; c->pois.npad = npad;
         KP p = make_kp(c);
         hipLaunchKernelGGL(k_poisson_factors, dim3(unsigned((nmodes + 127) / 128)), dim3(128), 0, c->stream, p, dak1, dak2, fm, fg, fe, npad);
         if ((rc = check_launch(c, "k_poisson_factors"))) return rc;
@@ -570,11 +609,12 @@ static int setup(afid_ctx *c) {
 
     // tridiagonal factor storage
     c->nrows[MAT_X] = nx; c->nrows[MAT_YZ] = nxm; c->nrows[MAT_T] = nxm; c->nrows[MAT_S] = c->sal ? c->nxmr : 0;
+    c->nrows[MAT_P] = c->pf ? c->nxmr : 0;
     int chmax = 1;
     for (int k = 0; k < MAT_NKIND; ++k) { c->chunk[k] = std::max(1, (c->nrows[k] + 63) / 64); chmax = std::max(chmax, c->chunk[k]); }
     if (chmax > 16) return set_err(c, AFID_E_ARG, "wall-normal size above 1024 points is not supported");
     c->npadmax = 64 * chmax;
-    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->tri), sizeof(double) * size_t(3 * MAT_NKIND * 4) * c->npadmax))) return rc;
+    if ((rc = dalloc(c, reinterpret_cast<void **>(&c->tri), sizeof(double) * size_t(3 * MAT_NKIND * kTriArrays) * c->npadmax))) return rc;
 
     // reductions
     if ((rc = dalloc(c, reinterpret_cast<void **>(&c->redmax), sizeof(unsigned long long) * 4))) return rc;
@@ -585,7 +625,7 @@ static int setup(afid_ctx *c) {
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->redout), sizeof(double) * 4))) return rc;
     }
 
-    if (!c->sal) return AFID_OK;
+    if (!c->sal && !c->pf) return AFID_OK;
 
     // ---- refined grid: CreateMgrdGrid.F90:38-161 ----
     const int nxr = c->nxr, nxmr = c->nxmr, nyr = c->nyr, nzr = c->nzr;
@@ -605,6 +645,12 @@ static int setup(afid_ctx *c) {
     FArr ap3sskr, ac3sskr, am3sskr;
     second_derivative_coeff(ap3sskr, ac3sskr, am3sskr, xmrr, nxmr, g.alx3, g.sfixn, g.sfixs);
     c->h_ap3sskr = farr_range(ap3sskr, 1, nxmr); c->h_ac3sskr = farr_range(ac3sskr, 1, nxmr); c->h_am3sskr = farr_range(am3sskr, 1, nxmr);
+    if (c->pf) {       // zero-gradient operator of phi, CreateMgrdGrid.F90:163
+        FArr ap3spkr, ac3spkr, am3spkr;
+        second_derivative_coeff(ap3spkr, ac3spkr, am3spkr, xmrr, nxmr, g.alx3, 0, 0);
+        c->h_ap3spkr = farr_range(ap3spkr, 1, nxmr); c->h_ac3spkr = farr_range(ac3spkr, 1, nxmr); c->h_am3spkr = farr_range(am3spkr, 1, nxmr);
+        H["ap3spkr"] = c->h_ap3spkr; H["ac3spkr"] = c->h_ac3spkr; H["am3spkr"] = c->h_am3spkr;
+    }
     std::vector<double> lxa;
     for (int k = 1; k <= nxmr; ++k) lxa.push_back(xcr(k + 1) - xcr(k));   // InterpVelMgrd.F90:62
     H["xcr"] = farr_range(xcr, 1, nxr); H["xmr"] = farr_range(xmr, 1, nxr);
@@ -657,6 +703,17 @@ static int setup(afid_ctx *c) {
         for (int i = 1; i <= c->nz; ++i) zb_src[size_t(i - 1)] = krangs(i) - 2;
         bx.w.resize(size_t(4 * nx), 0.0);
     }
+    // phase field: refined -> coarse cell centres with cxphic and irangs
+    // (interpolate_xyz_to_coarse_fast, "phi", HermiteInterpolations.F90:283-290)
+    std::vector<int> xp_src((size_t)(nx), 0);
+    Stencil bp;
+    if (c->pf) {
+        IArr irangr = interpolation_indices(xmrr, nxmr, xmc, nxm, g.alx3);
+        bp = construct_stencil(xmrr, nxmr, xmc, nxm, g.alx3, irangr, 'x');
+        for (int k = 1; k <= nxm; ++k) xp_src[size_t(k - 1)] = irangs(k) - 2;
+        bp.w.resize(size_t(4 * nx), 0.0);
+        H["cxphic"] = bp.w;
+    }
     H["cxrs"] = sx.w; H["cyrs"] = sy.w; H["czrs"] = sz.w; H["cxsalc"] = bx.w; H["cysalc"] = by.w; H["czsalc"] = bz.w;
 
     const int *pi;
@@ -678,6 +735,11 @@ static int setup(afid_ctx *c) {
     up(xmr_v, &c->d_xmr);
     up(udx3mr_v, &c->d_udx3mr); up(c->h_ap3sskr, &c->d_ap3sskr); up(c->h_ac3sskr, &c->d_ac3sskr); up(c->h_am3sskr, &c->d_am3sskr);
     up(lxa, &c->d_lxa);
+    if (c->pf) {
+        up(c->h_ap3spkr, &c->d_ap3spkr); up(c->h_ac3spkr, &c->d_ac3spkr); up(c->h_am3spkr, &c->d_am3spkr);
+        up(bp.w, &c->d_xp_w);
+        if (!rc) { rc = upload_vec(c, xp_src, &pi); c->d_xp_src = pi; }
+    }
     if (rc) return rc;
 
     const size_t nr = size_t(nzr) * nyr * c->ldxr;
@@ -694,13 +756,20 @@ static int setup(afid_ctx *c) {
         upf(bx.w, &c->f_xb_w); upf(by.w, &c->f_yb_w); upf(bz.w, &c->f_zb_w);
         upf(lxa, &c->f_lxa);
         if (rc) return rc;
-    } else {
+    } else if (c->sal) {
         for (int f = AFID_SAL; f <= AFID_VZR; ++f)
             if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * nr))) return rc;
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->sal_alt), sizeof(double) * nr))) return rc;
+    } else {
+        // phase field: phi (+ second buffer), its RK arrays, tempr, rhsr; coarse phic
+        for (int f : {AFID_PHI, AFID_RUPHI, AFID_HPHI, AFID_TEMPR, AFID_RHSR})
+            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * nr))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->phi_alt), sizeof(double) * nr))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[AFID_PHIC]), sizeof(double) * size_t(c->nz) * c->ny * c->ldx))) return rc;
     }
-    for (int f : {AFID_SALBP, AFID_SALTP})
-        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * size_t(nyr) * nzr))) return rc;
+    if (c->sal)
+        for (int f : {AFID_SALBP, AFID_SALTP})
+            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * size_t(nyr) * nzr))) return rc;
     if (nxmr > 1024) return set_err(c, AFID_E_ARG, "refined wall-normal size above 1024 is not supported");
     // LDS plan of the tiled transfer kernels: largest coarse block of any tile
     auto span = [](const std::vector<int> &src, int n, int tile) {
@@ -729,6 +798,12 @@ static int setup(afid_ctx *c) {
         c->salc_xr = span(xb0, nxm, 64);
         c->salc_yr = span(yb_src, c->ny, SALC_TJ);
         c->lds_salc = es * size_t(2 * c->salc_yr * c->salc_xr);
+        if (c->pf) {
+            std::vector<int> xp0(xp_src.begin(), xp_src.begin() + nxm);
+            c->phic_xr = span(xp0, nxm, 64);
+            c->lds_phic = sizeof(double) * size_t(2 * c->salc_yr * c->phic_xr);
+            if (c->lds_phic > 65536) return set_err(c, AFID_E_ARG, "refinement ratio too small for the phic tiles");
+        }
     }
     if (c->lds_vxr > 65536 || c->lds_vy > 65536 || c->lds_vz > 65536 || c->lds_salc > 65536)
         return set_err(c, AFID_E_ARG, "refinement ratio too small for the transfer tiles (LDS plan above 64 KiB)");
@@ -873,24 +948,49 @@ int afid_add_sal_buoyancy(afid_ctx *c) {
     return check_launch(c, "k_add_sal_buoyancy");
 }
 
+}  // extern "C"
+
+// masked velocity solve of the phase field (SolveImpEqnUpdate_X_pf / _YZ_pf)
+template <int CH, int AX>
+static void launch_tri_pf(afid_ctx *c, const TriFactors &f, const double *rhs, double *q) {
+    KP p = make_kp(c);
+    const unsigned nb = unsigned(((long long)c->ny * c->nz + 3) / 4);
+    hipLaunchKernelGGL((k_tri_update_pf<CH, AX>), dim3(nb), dim3(256), 0, c->stream, p, f, rhs, q);
+}
+template <int AX>
+static int run_tri_pf(afid_ctx *c, int kind, const double *rhs, double *q) {
+    TriFactors f = factors(c, kind);
+    switch (f.chunk) {
+#define AFID_TP_CASE(CH) case CH: launch_tri_pf<CH, AX>(c, f, rhs, q); break;
+        AFID_TP_CASE(1) AFID_TP_CASE(2) AFID_TP_CASE(3) AFID_TP_CASE(4) AFID_TP_CASE(5) AFID_TP_CASE(6)
+        AFID_TP_CASE(7) AFID_TP_CASE(8) AFID_TP_CASE(9) AFID_TP_CASE(10) AFID_TP_CASE(11) AFID_TP_CASE(12)
+        AFID_TP_CASE(13) AFID_TP_CASE(14) AFID_TP_CASE(15) AFID_TP_CASE(16)
+#undef AFID_TP_CASE
+    default: return set_err(c, AFID_E_ARG, "wall-normal size above 1024 points is not supported");
+    }
+    return check_launch(c, "k_tri_update_pf");
+}
+
+extern "C" {
+
 int afid_implicit_and_update_vx(afid_ctx *c) {
     NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_vx, false, "k_rhs_vx");
-    if (!rc) rc = run_tri(c, MAT_X, c->rhs4[0], c->f[AFID_VX]);
+    if (!rc) rc = c->pf ? run_tri_pf<0>(c, MAT_X, c->rhs4[0], c->f[AFID_VX]) : run_tri(c, MAT_X, c->rhs4[0], c->f[AFID_VX]);
     c->f[AFID_RHS] = c->rhs4[0];
     return rc;
 }
 int afid_implicit_and_update_vy(afid_ctx *c) {
     NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_vy, false, "k_rhs_vy");
-    if (!rc) rc = run_tri(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]);
+    if (!rc) rc = c->pf ? run_tri_pf<1>(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]) : run_tri(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]);
     c->f[AFID_RHS] = c->rhs4[1];
     return rc;
 }
 int afid_implicit_and_update_vz(afid_ctx *c) {
     NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_vz, false, "k_rhs_vz");
-    if (!rc) rc = run_tri(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]);
+    if (!rc) rc = c->pf ? run_tri_pf<2>(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]) : run_tri(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]);
     c->f[AFID_RHS] = c->rhs4[2];
     return rc;
 }
@@ -957,7 +1057,7 @@ static int poisson_sweep(afid_ctx *c, const FftPlan<R> &py, const FftPlan<R> &pz
 // FP64 form: one sweep, from dph (from_dph = 1) or fused with the divergence (0)
 static int poisson(afid_ctx *c, int from_dph) {
     const Gate open{nullptr, 0.0};
-    return poisson_sweep<double>(c, c->plan_y, c->plan_z, c->pf, from_dph ? SRC_DPH : SRC_VELOCITY, open, open, nullptr, nullptr);
+    return poisson_sweep<double>(c, c->plan_y, c->plan_z, c->pois, from_dph ? SRC_DPH : SRC_VELOCITY, open, open, nullptr, nullptr);
 }
 
 static int project(afid_ctx *c, int vel, int pr, Gate gate = Gate{nullptr, 0.0}) {
@@ -1041,15 +1141,61 @@ static int interp_vel(afid_ctx *c) {
 }
 int afid_interp_vel_mgrd(afid_ctx *c) { NEED_SAL(c) return interp_vel(c); }
 
-int afid_interp_sal_multigrid(afid_ctx *c) {
-    NEED_SAL(c)
+// refined -> coarse cell centres (k_interp_coarse_tile)
+static int to_coarse(afid_ctx *c, const CoarseJob &jb, int xr, size_t lds, const char *timer) {
     KP p = make_kp(c);
     const dim3 gr(unsigned((c->nxm + 63) / 64), unsigned((c->ny + SALC_TJ - 1) / SALC_TJ), unsigned((c->nz + SALC_TZ - 1) / SALC_TZ));
-    tstart(c, "mgrd_interp_sal");
-    if (c->mixed) hipLaunchKernelGGL(k_interp_sal_tile<float>, gr, dim3(256), c->lds_salc, c->stream, p, c->salc_xr, c->salc_yr);
-    else hipLaunchKernelGGL(k_interp_sal_tile<double>, gr, dim3(256), c->lds_salc, c->stream, p, c->salc_xr, c->salc_yr);
+    tstart(c, timer);
+    if (c->mixed) hipLaunchKernelGGL(k_interp_coarse_tile<float>, gr, dim3(256), lds, c->stream, p, jb, xr, c->salc_yr);
+    else hipLaunchKernelGGL(k_interp_coarse_tile<double>, gr, dim3(256), lds, c->stream, p, jb, xr, c->salc_yr);
     tstop(c);
-    return check_launch(c, "k_interp_sal_tile");
+    return check_launch(c, "k_interp_coarse_tile");
+}
+
+int afid_interp_sal_multigrid(afid_ctx *c) {
+    NEED_SAL(c)
+    CoarseJob jb{};
+    jb.mode = CM_SAL;
+    if (c->mixed) { jb.src = c->g[AFID_SAL]; jb.out = c->g[AFID_SALC]; jb.x_w = c->f_xb_w; }
+    else { jb.src = c->f[AFID_SAL]; jb.out = c->f[AFID_SALC]; jb.x_w = c->d_xb_w; }
+    jb.x_src = c->d_xb_src;
+    jb.scale = 1.0;
+    return to_coarse(c, jb, c->salc_xr, c->lds_salc, "mgrd_interp_sal");
+}
+
+#define NEED_PF(c) if (!(c)->pf) return set_err((c), AFID_E_STATE, "the phase field is not enabled");
+
+int afid_explicit_phase(afid_ctx *c) { NEED_SUBSTEP(c) NEED_PF(c) return launch_point(c, k_explicit_phase, true, "k_explicit_phase"); }
+int afid_add_volume_penalty(afid_ctx *c) { NEED_SUBSTEP(c) NEED_PF(c) return launch_point(c, k_volume_penalty, false, "k_volume_penalty"); }
+int afid_implicit_phase(afid_ctx *c) {
+    NEED_SUBSTEP(c) NEED_PF(c)
+    int rc = launch_point(c, k_rhs_phase, true, "k_rhs_phase");
+    if (!rc) rc = run_tri(c, MAT_P, c->f[AFID_RHSR], c->f[AFID_PHI]);   // rhsr keeps the increment
+    return rc;
+}
+static int latent_heat(afid_ctx *c, double *out, int accumulate) {
+    CoarseJob jb{};
+    jb.mode = CM_LAT; jb.src = c->f[AFID_RHSR]; jb.out = out; jb.x_src = c->d_xp_src; jb.x_w = c->d_xp_w;
+    jb.scale = c->c.pf_s / (c->al * c->dt);          // pf_S d(phi)/(al dt), phasefield.F90:666,701
+    jb.accumulate = accumulate;
+    return to_coarse(c, jb, c->phic_xr, c->lds_phic, "pf_latent_heat");
+}
+int afid_add_latent_heat(afid_ctx *c) { NEED_SUBSTEP(c) NEED_PF(c) return latent_heat(c, c->f[AFID_HRO], 1); }
+int afid_interp_phi_multigrid(afid_ctx *c) {
+    NEED_PF(c)
+    CoarseJob jb{};
+    jb.mode = CM_PHI; jb.src = c->f[AFID_PHI]; jb.out = c->f[AFID_PHIC]; jb.x_src = c->d_xp_src; jb.x_w = c->d_xp_w;
+    jb.scale = 1.0;
+    return to_coarse(c, jb, c->phic_xr, c->lds_phic, "pf_interp_phi");
+}
+int afid_interp_temp_multigrid(afid_ctx *c) {
+    NEED_PF(c)
+    KP p = make_kp(c);
+    tstart(c, "pf_interp_temp");
+    hipLaunchKernelGGL(k_interp_temp_tile, dim3(unsigned((c->nyr + VXR_TJ - 1) / VXR_TJ), unsigned((c->nzr + VXR_TI - 1) / VXR_TI)),
+                       dim3(256), c->lds_vxr, c->stream, p, c->vxr_xc, c->vxr_cj, c->vxr_ci);
+    tstop(c);
+    return check_launch(c, "k_interp_temp_tile");
 }
 
 }  // extern "C"
@@ -1060,9 +1206,11 @@ static void launch_column_base(afid_ctx *c, const KP &p, long long nblk) {
     BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
     const dim3 gr(unsigned((nblk + 7) / 8 * 8));
     if (c->mixed)
-        hipLaunchKernelGGL((k_column_base<CH, float>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
+        hipLaunchKernelGGL((k_column_base<CH, float, false>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
+    else if (c->pf)
+        hipLaunchKernelGGL((k_column_base<CH, double, true>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
     else
-        hipLaunchKernelGGL((k_column_base<CH, double>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
+        hipLaunchKernelGGL((k_column_base<CH, double, false>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
 }
 static int column_base(afid_ctx *c) {
     KP p = make_kp(c);
@@ -1109,6 +1257,28 @@ static int column_sal(afid_ctx *c) {
     return AFID_OK;
 }
 
+template <int CH>
+static void launch_column_phi(afid_ctx *c, const KP &p, long long nblk) {
+    hipLaunchKernelGGL(k_column_phi<CH>, dim3(unsigned((nblk + 7) / 8 * 8)), dim3(256), 0, c->stream, p, factors(c, MAT_P), c->phi_alt);
+}
+static int column_phi(afid_ctx *c) {
+    KP p = make_kp(c);
+    const long long nblk = strip_blocks(c->nyr, c->nzr);
+    const int ch = (c->nxmr + 63) / 64;
+    switch (ch) {
+#define AFID_CP_CASE(CH) case CH: launch_column_phi<CH>(c, p, nblk); break;
+        AFID_CP_CASE(1) AFID_CP_CASE(2) AFID_CP_CASE(3) AFID_CP_CASE(4) AFID_CP_CASE(5) AFID_CP_CASE(6)
+        AFID_CP_CASE(7) AFID_CP_CASE(8) AFID_CP_CASE(9) AFID_CP_CASE(10) AFID_CP_CASE(11) AFID_CP_CASE(12)
+        AFID_CP_CASE(13) AFID_CP_CASE(14) AFID_CP_CASE(15) AFID_CP_CASE(16)
+#undef AFID_CP_CASE
+    default: return set_err(c, AFID_E_ARG, "nxmr too large");
+    }
+    int rc = check_launch(c, "k_column_phi");
+    if (rc) return rc;
+    std::swap(c->f[AFID_PHI], c->phi_alt);
+    return AFID_OK;
+}
+
 extern "C" {
 
 // TimeMarcher.F90:13-165
@@ -1119,6 +1289,14 @@ int afid_time_marcher(afid_ctx *c, double dt) {
     int rc;
     for (int ns = 1; ns <= c->c.nsst; ++ns) {
         if ((rc = afid_set_substep(c, dt, ns))) return rc;
+        if (c->pf) {
+            // ExplicitPhase + ImplicitPhase (phi, ruphi, rhsr), then the
+            // latent heat into qlat, read by the base column as hro
+            tstart(c, "pf_column_phi");
+            if ((rc = column_phi(c))) return rc;
+            tstop(c);
+            if ((rc = latent_heat(c, c->f[AFID_HRO], 0))) return rc;
+        }
         tstart(c, "column_base");
         if ((rc = column_base(c))) return rc;
         tstop(c);
@@ -1139,6 +1317,10 @@ int afid_time_marcher(afid_ctx *c, double dt) {
             if ((rc = interp_vel(c))) return rc;
             if ((rc = afid_interp_sal_multigrid(c))) return rc;
         }
+        if (c->pf) {
+            if ((rc = afid_interp_temp_multigrid(c))) return rc;
+            if ((rc = afid_interp_phi_multigrid(c))) return rc;
+        }
     }
     return AFID_OK;
 }
@@ -1156,7 +1338,22 @@ int afid_create_initial_conditions(afid_ctx *c, uint64_t seed, double eps) {
                        c->f[AFID_TEMPBP], c->f[AFID_TEMPTP], c->d_xm, c->nxm, c->nx, c->ldx, c->ny, c->nz, c->c.alx3, eps,
                        (unsigned long long)seed);
     int rc = check_launch(c, "k_init_scalar");
-    if (rc || !c->sal) return rc;
+    if (rc) return rc;
+    if (c->pf) {
+        // set_ice_sphere (phasefield.F90:311-341) of radius pf_r0, then
+        // tempr and phic as at start-up (main.F90:210-212)
+        KP p = make_kp(c);
+        const size_t nr = size_t(c->nzr) * c->nyr * c->ldxr;
+        for (int f : {AFID_RUPHI, AFID_HPHI, AFID_RHSR}) HIPCHK(c, hipMemsetAsync(c->f[f], 0, sizeof(double) * nr, c->stream));
+        hipLaunchKernelGGL(k_init_temp_sphere, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, c->d_xm, c->pf_r0,
+                           c->c.alx3, c->c.ylen, c->c.zlen);
+        hipLaunchKernelGGL(k_init_phi_sphere, point_grid(c->nxr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, c->d_xmr, c->pf_r0,
+                           c->pf_eps, c->c.alx3, c->c.ylen, c->c.zlen);
+        if ((rc = check_launch(c, "k_init_phi_sphere"))) return rc;
+        if ((rc = afid_interp_temp_multigrid(c))) return rc;
+        return afid_interp_phi_multigrid(c);
+    }
+    if (!c->sal) return rc;
     const size_t nr = size_t(c->nzr) * c->nyr * c->ldxr;
     for (int f : {AFID_RUSAL, AFID_VXR, AFID_VYR, AFID_VZR}) {
         if (c->g[f]) HIPCHK(c, hipMemsetAsync(c->g[f], 0, sizeof(float) * nr, c->stream));

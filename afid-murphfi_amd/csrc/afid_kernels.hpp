@@ -192,6 +192,59 @@ __device__ __forceinline__ void wave_tri_solve(double (&v)[CH], const TriFactors
     for (int m = CH - 1; m >= 0; --m) { x = -e[m] * x + g[m] * v[m]; v[m] = x; }
 }
 
+// 2x2 projective matrix (normalised so its largest entry is 1)
+struct Mob { double m00, m01, m10, m11; };
+__device__ __forceinline__ Mob mob_mul(const Mob &x, const Mob &y) {   // x * y
+    Mob r{x.m00 * y.m00 + x.m01 * y.m10, x.m00 * y.m01 + x.m01 * y.m11,
+          x.m10 * y.m00 + x.m11 * y.m10, x.m10 * y.m01 + x.m11 * y.m11};
+    const double mx = fmax(fmax(fabs(r.m00), fabs(r.m01)), fmax(fabs(r.m10), fabs(r.m11)));
+    if (mx > 0.0) { const double q = 1.0 / mx; r.m00 *= q; r.m01 *= q; r.m10 *= q; r.m11 *= q; }
+    return r;
+}
+
+// Solve a tridiagonal system whose rows differ from column to column (row
+// r: a_r x_{r-1} + x_r + c_r x_{r+1} = v_r), CH rows per lane.  The Thomas
+// recurrence c'_r = c_r / (1 - a_r c'_{r-1}) is a Moebius map of c'_{r-1};
+// its prefix over the wavefront is a scan of 2x2 matrix products.  With c'
+// known, forward elimination and back substitution are affine scans as in
+// wave_tri_solve.
+template <int CH>
+__device__ __forceinline__ void wave_tri_solve_general(double (&v)[CH], const double (&a)[CH], const double (&c)[CH]) {
+    const int lane = threadIdx.x & 63;
+    Mob L{1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+    for (int m = 0; m < CH; ++m) L = mob_mul(Mob{0.0, c[m], -a[m], 1.0}, L);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        Mob P{__shfl_up(L.m00, off, 64), __shfl_up(L.m01, off, 64), __shfl_up(L.m10, off, 64), __shfl_up(L.m11, off, 64)};
+        if (lane >= off) L = mob_mul(L, P);
+    }
+    // c' before this lane's first row: the exclusive prefix applied to (0, 1)
+    const double p0 = __shfl_up(L.m01, 1, 64), q0 = __shfl_up(L.m11, 1, 64);
+    double t = lane == 0 ? 0.0 : p0 / q0;
+    double cp[CH], g[CH];
+#pragma unroll
+    for (int m = 0; m < CH; ++m) { g[m] = 1.0 / (1.0 - a[m] * t); cp[m] = c[m] * g[m]; t = cp[m]; }
+    // forward: y_r = g_r v_r - g_r a_r y_{r-1}
+    double A = 1.0, B = 0.0;
+#pragma unroll
+    for (int m = 0; m < CH; ++m) { const double am = -g[m] * a[m]; B = am * B + g[m] * v[m]; A = am * A; }
+    wave_affine_scan<true>(A, B);
+    double y = __shfl_up(B, 1, 64);
+    if (lane == 0) y = 0.0;
+#pragma unroll
+    for (int m = 0; m < CH; ++m) { y = -g[m] * a[m] * y + g[m] * v[m]; v[m] = y; }
+    // backward: x_r = y_r - c'_r x_{r+1}
+    A = 1.0; B = 0.0;
+#pragma unroll
+    for (int m = CH - 1; m >= 0; --m) { B = -cp[m] * B + v[m]; A = -cp[m] * A; }
+    wave_affine_scan<false>(A, B);
+    double x = __shfl_down(B, 1, 64);
+    if (lane == 63) x = 0.0;
+#pragma unroll
+    for (int m = CH - 1; m >= 0; --m) { x = -cp[m] * x + v[m]; v[m] = x; }
+}
+
 // One column per wavefront: q[r] += solve(rhs)[r] for r in [u0,u1); rows
 // listed in zero0/zero1 (if >= 0) are set to zero afterwards (the wall rows
 // of vx, ImplicitAndUpdateVX.F90:86-87).  rhs holds the raw RHS on entry and
@@ -282,6 +335,28 @@ __device__ __forceinline__ void column_solve_lds(double *buf, const TriFactors &
 }
 
 // two-element vector of the storage type (64-bit or 128-bit lane accesses)
+// Masked solve of one column held in LDS (SolveImpEqnUpdate_X_pf /
+// _YZ_pf, phasefield.F90:821-930): rows flagged in msk are the identity with
+// the right-hand side already in buf; the others are the rows of f.
+template <int CH>
+__device__ __forceinline__ void column_solve_masked_lds(double *buf, const double *msk, const TriFactors &f) {
+    const int lane = threadIdx.x & 63;
+    double v[CH], a[CH], c[CH];
+#pragma unroll
+    for (int m = 0; m < CH; ++m) {
+        const int r = lane * CH + m;
+        const bool solid = msk[lane * lds_chunk<CH>() + m] != 0.0;
+        const double sv = f.s[r];
+        v[m] = buf[lane * lds_chunk<CH>() + m];
+        if (!solid) v[m] *= sv;
+        a[m] = solid ? 0.0 : f.a[r];
+        c[m] = solid ? 0.0 : f.c[r];
+    }
+    wave_tri_solve_general<CH>(v, a, c);
+#pragma unroll
+    for (int m = 0; m < CH; ++m) buf[lane * lds_chunk<CH>() + m] = v[m];
+}
+
 template <typename RT> struct Pair;
 template <> struct Pair<float> { using T = float2; };
 template <> struct Pair<double> { using T = double2; };
@@ -362,10 +437,15 @@ struct BaseOut { double *vx, *vy, *vz, *temp; };
 
 // stencil phase of field FLD: RHS of each row into buf, old value into own.
 // HT is the storage and arithmetic type of the explicit terms (and of salc);
-// the RHS and the solve run in FP64.
-template <int FLD, int CH, typename HT>
+// the RHS and the solve run in FP64.  With PF (phase field) the velocity
+// terms lose the volume penalty (AddVolumePenalty, phasefield.F90:525-566),
+// the temperature term gains the latent heat qlat (AddLatentHeat, :651-725),
+// and a velocity row whose averaged phic exceeds pf_direct_force = 0.9
+// (phasefield.F90:29) is flagged in msk with right-hand side -u, the
+// identity row of SolveImpEqnUpdate_X_pf / _YZ_pf (:821-930).
+template <int FLD, int CH, typename HT, bool PF>
 __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long long c0, double *buf, double *own,
-                                                 unsigned lane, unsigned nrow, bool live) {
+                                                 double *msk, unsigned lane, unsigned nrow, bool live) {
     const HT *salc = sel<HT>(p.salc, p.salcf) + c0;
     HT *ru = (FLD == 0 ? sel<HT>(p.rux, p.ruxf) : FLD == 1 ? sel<HT>(p.ruy, p.ruyf)
               : FLD == 2 ? sel<HT>(p.ruz, p.ruzf) : sel<HT>(p.rut, p.rutf)) + c0;
@@ -373,6 +453,7 @@ __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long 
 #pragma unroll 1
     for (unsigned r = lane; r < nrow; r += 64) {
         double rhs = 0.0, old = 0.0;
+        bool solid = false;
         if (live && r < unsigned(p.nx)) {
             const HT sb = (p.active_s == 1 && r < nxm) ? HT(p.bycs) * salc[r] : HT(0);
             if (FLD == 0) {
@@ -380,14 +461,25 @@ __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long 
                 if (r >= 1 && r < nxm) {
                     HT h = ex_vx<HT>(p, n, r);
                     if (p.gaxis == 1) h -= sb;
+                    if (PF) {
+                        h -= HT((n.pc[r] + n.pc[r - 1]) * n.vx[r] * p.heta);
+                        solid = 0.5 * (n.pc[r] + n.pc[r + 1]) > 0.9;
+                    }
                     rhs = rhs_vx(p, n, double(h), double(ru[r]), r);
                     ru[r] = h;
+                } else if (PF) {
+                    solid = true;                    // rows 1 and nx: identity with rhs -vx
                 }
             } else if (FLD == 1) {
                 old = n.vy[r];
                 if (r < nxm) {
                     HT h = ex_vy<HT>(p, n, r);
                     if (p.gaxis == 2) h -= sb;
+                    if (PF) {
+                        const double pa = n.pc[r] + n.pc_jm[r];
+                        h -= HT(pa * n.vy[r] * p.heta);
+                        solid = 0.5 * pa > 0.9;
+                    }
                     rhs = rhs_vy(p, n, double(h), double(ru[r]), r);
                     ru[r] = h;
                 }
@@ -396,6 +488,11 @@ __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long 
                 if (r < nxm) {
                     HT h = ex_vz<HT>(p, n, r);
                     if (p.gaxis == 3) h -= sb;
+                    if (PF) {
+                        const double pa = n.pc[r] + n.pc_im[r];
+                        h -= HT(pa * n.vz[r] * p.heta);
+                        solid = 0.5 * pa > 0.9;
+                    }
                     rhs = rhs_vz(p, n, double(h), double(ru[r]), r);
                     ru[r] = h;
                 }
@@ -403,37 +500,44 @@ __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long 
                 old = n.t[r];
                 if (r < nxm) {
                     HT h = ex_temp<HT>(p, n, r);
+                    if (PF) h += HT(p.qlat[c0 + r]);
                     rhs = rhs_temp(p, n, double(h), double(ru[r]), r);
                     ru[r] = h;
                 }
             }
+        }
+        if (PF && FLD < 3) {
+            if (solid) rhs = -old;
+            msk[lds_row<CH>(r)] = solid ? 1.0 : 0.0;
         }
         buf[lds_row<CH>(r)] = rhs;
         own[lds_row<CH>(r)] = old;
     }
 }
 
-template <int CH, typename HT>
+template <int CH, typename HT, bool PF>
 __global__ void __launch_bounds__(256) k_column_base(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
-    __shared__ double lds[2][4][lds_col_size<CH>()];
+    __shared__ double lds[PF ? 3 : 2][4][lds_col_size<CH>()];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
     int j, i;
     const bool live = strip_column(xcd_block(0), wave, p.ny, p.nz, j, i);
-    double *buf = lds[0][wave], *own = lds[1][wave];
+    double *buf = lds[0][wave], *own = lds[1][wave], *msk = lds[PF ? 2 : 1][wave];
     const Nb n = make_nb(p, j, i);
     const long long c0 = (long long)i * p.plane + (long long)j * p.ldx;
     const unsigned nrow = 64 * CH, nx = p.nx, nxm = p.nxm;
 #pragma unroll 1
     for (int fld = 0; fld < 4; ++fld) {
         switch (fld) {
-        case 0: column_base_rows<0, CH, HT>(p, n, c0, buf, own, lane, nrow, live); break;
-        case 1: column_base_rows<1, CH, HT>(p, n, c0, buf, own, lane, nrow, live); break;
-        case 2: column_base_rows<2, CH, HT>(p, n, c0, buf, own, lane, nrow, live); break;
-        default: column_base_rows<3, CH, HT>(p, n, c0, buf, own, lane, nrow, live); break;
+        case 0: column_base_rows<0, CH, HT, PF>(p, n, c0, buf, own, msk, lane, nrow, live); break;
+        case 1: column_base_rows<1, CH, HT, PF>(p, n, c0, buf, own, msk, lane, nrow, live); break;
+        case 2: column_base_rows<2, CH, HT, PF>(p, n, c0, buf, own, msk, lane, nrow, live); break;
+        default: column_base_rows<3, CH, HT, PF>(p, n, c0, buf, own, msk, lane, nrow, live); break;
         }
         __syncthreads();
-        column_solve_lds<CH>(buf, fld == 0 ? fx : (fld == 3 ? ft : fyz));
+        const TriFactors &f = fld == 0 ? fx : (fld == 3 ? ft : fyz);
+        if (PF && fld < 3) column_solve_masked_lds<CH>(buf, msk, f);
+        else column_solve_lds<CH>(buf, f);
         __syncthreads();
         if (live) {
             double *out = (fld == 0 ? o.vx : fld == 1 ? o.vy : fld == 2 ? o.vz : o.temp) + c0;
@@ -775,41 +879,67 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
     }
 }
 
-// InterpSalMultigrid (salinity.F90:458-491) through the fast coarse-grid loop
-// of interpolate_xyz_to_coarse_fast (HermiteInterpolations.F90:291-310).
+// Refined -> coarse cell centres through the fast coarse-grid loop of
+// interpolate_xyz_to_coarse_fast (HermiteInterpolations.F90:291-310):
+// InterpSalMultigrid (salinity.F90:458-491), InterpPhiMultigrid and the
+// transfer of AddLatentHeat (phasefield.F90:470-494, :651-725).
 // Tile: 64 coarse x (lanes) x SALC_TJ coarse y (two rows per wave), marching
 // through SALC_TZ coarse z planes.  Each refined z plane the tile needs is
 // staged once in LDS (with the wall ghosts of salinity.F90:466-482, double
 // buffered), contracted over x and y, and the last four contracted planes are
 // kept in registers for the z contraction.
 constexpr int SALC_TJ = 8, SALC_TZ = 16;
+// What a refined-to-coarse tile interpolates and where the result goes.
+enum CoarseMode {
+    CM_SAL = 0,     // S with the wall ghosts of salinity.F90:466-482 -> salc (salc(nx) = 0)
+    CM_PHI = 1,     // phi with zero-gradient ghosts (phasefield.F90:479-487) -> phic (phic(nx) = 0)
+    CM_LAT = 2      // d phi with zero-gradient ghosts (:674-683) -> out (+)= scale * value
+};
+struct CoarseJob {
+    int mode;
+    const void *src;            // refined field (RT)
+    void *out;                  // coarse field (RT for CM_SAL, double otherwise)
+    const int *x_src;           // first refined x source of each coarse x (reference index)
+    const void *x_w;            // its 4 weights (RT)
+    double scale;
+    int accumulate;
+};
 template <typename RT>
-__device__ __forceinline__ RT sal_ghosted(const KP &p, const RT *sal, int kr, int jj, int ii) {
+__device__ __forceinline__ RT coarse_src(const KP &p, const CoarseJob &jb, const RT *f, int kr, int jj, int ii) {
     const long long o2 = (long long)ii * p.planer + (long long)jj * p.ldxr;
-    if (kr >= 1 && kr <= p.nxmr) return sal[o2 + kr - 1];
-    if (kr == 0) { const RT s1 = sal[o2]; return p.sfixs == 1 ? RT(2.0 * p.salbp[(long long)ii * p.nyr + jj]) - s1 : s1; }
-    if (kr == p.nxr) { const RT sn = sal[o2 + p.nxmr - 1]; return p.sfixn == 1 ? RT(2.0 * p.saltp[(long long)ii * p.nyr + jj]) - sn : sn; }
+    if (kr >= 1 && kr <= p.nxmr) return f[o2 + kr - 1];
+    if (kr == 0) {
+        const RT s1 = f[o2];
+        if (jb.mode != CM_SAL) return s1;
+        return p.sfixs == 1 ? RT(2.0 * p.salbp[(long long)ii * p.nyr + jj]) - s1 : s1;
+    }
+    if (kr == p.nxr) {
+        const RT sn = f[o2 + p.nxmr - 1];
+        if (jb.mode != CM_SAL) return sn;
+        return p.sfixn == 1 ? RT(2.0 * p.saltp[(long long)ii * p.nyr + jj]) - sn : sn;
+    }
     return RT(0);
 }
 template <typename RT>
-__global__ void __launch_bounds__(256) k_interp_sal_tile(KP p, int xrmax, int yrmax) {
+__global__ void __launch_bounds__(256) k_interp_coarse_tile(KP p, CoarseJob jb, int xrmax, int yrmax) {
     extern __shared__ __align__(16) unsigned char smraw[];
     RT *sm = reinterpret_cast<RT *>(smraw);
-    const RT *sal = sel<RT>(p.sal, p.salf);
-    RT *salc = sel<RT>(p.salc, p.salcf);
-    const RT *xb_w = sel<RT>(p.xb_w, p.xb_wf), *yb_w = sel<RT>(p.yb_w, p.yb_wf), *zb_w = sel<RT>(p.zb_w, p.zb_wf);
+    const RT *src = static_cast<const RT *>(jb.src);
+    const int *x_src = jb.x_src;
+    const RT *xb_w = static_cast<const RT *>(jb.x_w);
+    const RT *yb_w = sel<RT>(p.yb_w, p.yb_wf), *zb_w = sel<RT>(p.zb_w, p.zb_wf);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x;
     const int K0 = blockIdx.x * 64, K1 = min(K0 + 64, p.nxm);
     const int J0 = blockIdx.y * SALC_TJ, nj = min(SALC_TJ, p.ny - J0);
     const int I0 = blockIdx.z * SALC_TZ, I1 = min(I0 + SALC_TZ, p.nz);
-    const int x0 = p.xb_src[K0], XR = p.xb_src[K1 - 1] + 4 - x0;          // reference refined x index
+    const int x0 = x_src[K0], XR = x_src[K1 - 1] + 4 - x0;                  // reference refined x index
     const int y0 = p.yb_src[J0] - 1, YR = p.yb_src[J0 + nj - 1] - 1 + 4 - y0; // 0-based refined y
     const int zfirst = p.zb_src[I0] - 1, zlast = p.zb_src[I1 - 1] - 1 + 3;    // 0-based refined z (unwrapped)
     const int k = K0 + lane;
     const bool kok = k < K1;
     RT wx[4] = {0, 0, 0, 0};
     int xa = 0;
-    if (kok) { xa = p.xb_src[k] - x0; for (int a = 0; a < 4; ++a) wx[a] = xb_w[4 * k + a]; }
+    if (kok) { xa = x_src[k] - x0; for (int a = 0; a < 4; ++a) wx[a] = xb_w[4 * k + a]; }
     // two coarse rows per wave: tj = wave and wave + 4
     int ya[2] = {0, 0};
     RT wy[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -827,7 +957,7 @@ __global__ void __launch_bounds__(256) k_interp_sal_tile(KP p, int xrmax, int yr
         const int ii = wrapn(zr, p.nzr);
         Walk3 w(t, 256, XR, YR);
         for (int q = t; q < plane; q += 256, w.next())
-            B[w.b * xrmax + w.a] = sal_ghosted<RT>(p, sal, x0 + w.a, wrapf(y0 + w.b, p.nyr), ii);
+            B[w.b * xrmax + w.a] = coarse_src<RT>(p, jb, src, x0 + w.a, wrapf(y0 + w.b, p.nyr), ii);
         __syncthreads();
         for (int h = 0; h < 2; ++h) {
             RT v = RT(0);
@@ -844,17 +974,28 @@ __global__ void __launch_bounds__(256) k_interp_sal_tile(KP p, int xrmax, int yr
         while (ic < I1 && p.zb_src[ic] - 1 + 3 == zr) {
             const RT *wz = zb_w + 4 * ic;
             for (int h = 0; h < 2; ++h)
-                if (kok && rok[h])
-                    salc[bidx(p, k, J0 + wave + 4 * h, ic)] =
-                        ring[h][0] * wz[0] + ring[h][1] * wz[1] + ring[h][2] * wz[2] + ring[h][3] * wz[3];
+                if (kok && rok[h]) {
+                    const RT v = ring[h][0] * wz[0] + ring[h][1] * wz[1] + ring[h][2] * wz[2] + ring[h][3] * wz[3];
+                    const long long o = bidx(p, k, J0 + wave + 4 * h, ic);
+                    if (jb.mode == CM_SAL) static_cast<RT *>(jb.out)[o] = v;
+                    else {
+                        double *out = static_cast<double *>(jb.out);
+                        const double val = jb.mode == CM_LAT ? jb.scale * double(v) : double(v);
+                        out[o] = jb.accumulate ? out[o] + val : val;
+                    }
+                }
             ++ic;
         }
     }
-    // salc(nx) stays zero (salinity.F90:463): written by the last x tile
-    if (blockIdx.x == gridDim.x - 1 && lane == 0)
+    // salc(nx), phic(nx) stay zero (salinity.F90:463, phasefield.F90:476): written by the last x tile
+    if (jb.mode != CM_LAT && blockIdx.x == gridDim.x - 1 && lane == 0)
         for (int h = 0; h < 2; ++h)
             if (rok[h])
-                for (int iz = I0; iz < I1; ++iz) salc[bidx(p, p.nxm, J0 + wave + 4 * h, iz)] = RT(0);
+                for (int iz = I0; iz < I1; ++iz) {
+                    const long long o = bidx(p, p.nxm, J0 + wave + 4 * h, iz);
+                    if (jb.mode == CM_SAL) static_cast<RT *>(jb.out)[o] = RT(0);
+                    else static_cast<double *>(jb.out)[o] = 0.0;
+                }
 }
 
 // ---------------------------------------------------------------------------

@@ -11,14 +11,16 @@
 
 namespace afid {
 
-enum MatKind { MAT_X = 0, MAT_YZ = 1, MAT_T = 2, MAT_S = 3, MAT_NKIND = 4 };
+enum MatKind { MAT_X = 0, MAT_YZ = 1, MAT_T = 2, MAT_S = 3, MAT_P = 4, MAT_NKIND = 5 };
 
 // LU factors of one constant wall-normal tridiagonal system, in the form the
 // wave solver consumes (row r, 0-based, padded with zeros to 64*chunk rows):
 //   forward   y_r = s_r d_r - l_r y_{r-1}
 //   backward  x_r = g_r y_r - e_r x_{r+1}
+// a, c: the normalised off-diagonals of each row (row r: a_r x_{r-1} + x_r +
+// c_r x_{r+1} = s_r d_r), for solves whose rows a mask may replace.
 struct TriFactors {
-    const double *l, *e, *g, *s;
+    const double *l, *e, *g, *s, *a, *c;
     int n, chunk;
 };
 
@@ -67,6 +69,15 @@ struct KP {
     const double *xb_w, *yb_w, *zb_w;
     // FP32 copies of the Hermite weights and of lxa (mixed form)
     const float *xs_wf, *ys_wf, *zs_wf, *xb_wf, *yb_wf, *zb_wf, *lxaf;
+    // phase field (phasefield.F90): refined phi, its RK arrays and the
+    // refined temperature; coarse phic and the latent-heat source qlat
+    double *phi, *ruphi, *hphi, *tempr, *phic, *qlat;
+    double pf_d, pf_b, pf_a, pf_tm, pf_s, heta;  // pf_D, pf_D/eps^2, A, T_m, S_t, 0.5/eta
+    int pf;
+    const double *ap3spkr, *ac3spkr, *am3spkr;   // zero-gradient x operator of phi (refined)
+    // refined -> coarse cell-centre stencil of phi (cxphic, irangs), per coarse x
+    const int *xp_src;
+    const double *xp_w;
     // reductions
     unsigned long long *redmax;         // [4] max slots (bit patterns of non-negative doubles)
     double *redsum;                     // per-block partial sums

@@ -29,6 +29,7 @@ struct Nb {
     const double *vz, *vz_jm, *vz_jp, *vz_im, *vz_ip, *vz_jm_ip;
     const double *t, *t_jm, *t_jp, *t_im, *t_ip;
     const double *pr, *pr_jm, *pr_im;
+    const double *pc, *pc_jm, *pc_im;     // phic of the column and its -y / -z neighbours (phase field)
     double tbp, ttp;
 };
 __device__ __forceinline__ Nb make_nb(const KP &p, int j, int i) {
@@ -43,6 +44,7 @@ __device__ __forceinline__ Nb make_nb(const KP &p, int j, int i) {
     n.vz_jm_ip = p.vz + o(jm, ip);
     n.t = p.temp + c; n.t_jm = p.temp + cjm; n.t_jp = p.temp + cjp; n.t_im = p.temp + cim; n.t_ip = p.temp + cip;
     n.pr = p.pr + c; n.pr_jm = p.pr + cjm; n.pr_im = p.pr + cim;
+    n.pc = p.phic + c; n.pc_jm = p.phic + cjm; n.pc_im = p.phic + cim;
     n.tbp = p.tempbp[(long long)i * p.ny + j];
     n.ttp = p.temptp[(long long)i * p.ny + j];
     return n;

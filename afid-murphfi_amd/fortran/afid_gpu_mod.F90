@@ -26,6 +26,7 @@ module afid_gpu
         integer(c_int32_t) :: precision
         real(c_double)     :: div_tol
         integer(c_int32_t) :: max_sweeps
+        real(c_double)     :: pf_a, pf_s, pf_tm, pf_eps, pf_r0
     end type afid_config
 
     interface

@@ -61,7 +61,8 @@ subroutine ReadInputFile
     cfg = afid_config(nxm, nym, nzm, nsst, flagMR, nxmr, nymr, nzmr, flagsal, flagPF, &
                       alx3, ylen, zlen, istr3, str3, istr3r, rayt, prat, rays, pras, FFscaleS, &
                       inslws, inslwn, TfixS, TfixN, SfixS, SfixN, active_T, active_S, gAxis, &
-                      xplusU, xminusU, dPdy, dPdz, AFID_FP64, 0.d0, 0)
+                      xplusU, xminusU, dPdy, dPdz, AFID_FP64, 0.d0, 0, &
+                      pf_A, pf_S, pf_Tm, 0.d0, 0.d0)
     if (mixed) cfg%precision = AFID_MIXED
 end subroutine ReadInputFile
 

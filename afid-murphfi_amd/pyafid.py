@@ -17,9 +17,10 @@ LIB_PATH = os.path.join(HERE, "libafid_gpu.so")
 FIELDS = ["vx", "vy", "vz", "temp", "pr", "rux", "ruy", "ruz", "rutemp",
           "qcap", "dph", "dq", "hro", "rhs",
           "sal", "rusal", "hsal", "rhsr", "vxr", "vyr", "vzr", "salc",
-          "tempbp", "temptp", "salbp", "saltp"]
+          "tempbp", "temptp", "salbp", "saltp",
+          "phi", "ruphi", "hphi", "phic", "tempr"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
-REFINED = {"sal", "rusal", "hsal", "rhsr", "vxr", "vyr", "vzr", "salbp", "saltp"}
+REFINED = {"sal", "rusal", "hsal", "rhsr", "vxr", "vyr", "vzr", "salbp", "saltp", "phi", "ruphi", "hphi", "tempr"}
 PLANES = {"tempbp", "temptp", "salbp", "saltp"}
 
 # every entry point declared in include/afid_gpu.h
@@ -31,6 +32,8 @@ EXPORTS = [
     "afid_implicit_and_update_vx", "afid_implicit_and_update_vy", "afid_implicit_and_update_vz",
     "afid_implicit_and_update_temp", "afid_implicit_salinity",
     "afid_calc_local_divergence", "afid_solve_pressure_correction", "afid_correct_velocity", "afid_correct_pressure",
+    "afid_explicit_phase", "afid_add_volume_penalty", "afid_implicit_phase", "afid_add_latent_heat",
+    "afid_interp_temp_multigrid", "afid_interp_phi_multigrid",
     "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher", "afid_refinement_residuals",
     "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_synchronize",
     "afid_create_initial_conditions",
@@ -42,7 +45,9 @@ ROUTINES = ["explicit_terms_vx", "explicit_terms_vy", "explicit_terms_vz", "expl
             "implicit_and_update_vx", "implicit_and_update_vy", "implicit_and_update_vz",
             "implicit_and_update_temp", "implicit_salinity",
             "calc_local_divergence", "solve_pressure_correction", "correct_velocity", "correct_pressure",
-            "interp_vel_mgrd", "interp_sal_multigrid"]
+            "interp_vel_mgrd", "interp_sal_multigrid",
+            "explicit_phase", "add_volume_penalty", "implicit_phase", "add_latent_heat",
+            "interp_temp_multigrid", "interp_phi_multigrid"]
 
 FP64, MIXED = 0, 1
 
@@ -63,6 +68,8 @@ class AfidConfig(C.Structure):
         ("xplusu", C.c_double), ("xminusu", C.c_double), ("dpdy", C.c_double), ("dpdz", C.c_double),
         ("precision", C.c_int32),
         ("div_tol", C.c_double), ("max_sweeps", C.c_int32),
+        ("pf_a", C.c_double), ("pf_s", C.c_double), ("pf_tm", C.c_double), ("pf_eps", C.c_double),
+        ("pf_r0", C.c_double),
     ]
 
 
@@ -71,7 +78,8 @@ DEFAULTS = dict(nxm=16, nym=16, nzm=16, nsst=3, multires=0, nxmr=0, nymr=0, nzmr
                 istr3=8, str3=0.0, istr3r=8, rayt=1e5, prat=1.0, rays=-1e5, pras=1.0, ffscales=0,
                 inslws=1, inslwn=1, tfixs=1, tfixn=1, sfixs=1, sfixn=1,
                 active_t=1, active_s=0, gaxis=1, xplusu=0.0, xminusu=0.0, dpdy=0.0, dpdz=0.0,
-                precision=FP64, div_tol=0.0, max_sweeps=0)
+                precision=FP64, div_tol=0.0, max_sweeps=0,
+                pf_a=1.0, pf_s=1.0, pf_tm=0.0, pf_eps=0.0, pf_r0=0.0)
 
 
 def make_config(d):

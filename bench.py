@@ -41,6 +41,13 @@ WORKLOADS = {
         rays=-1e9, pras=16.0, active_t=1, active_s=0, gaxis=1, dt=5e-5),
     "rbc_64x64x64": dict(nxm=64, nym=64, nzm=64, nsst=3, alx3=1.0, ylen=1.0, zlen=1.0, istr3=8,
                          rayt=1e5, prat=1.0, active_t=1, gaxis=1, dt=1e-3),
+    # melting of a solid sphere (radius 0.15, phase-field width 1.5 refined
+    # spacings) in RBC at Ra = 1e8: base 256 x 512 x 512, refined phi/T grid
+    # 512 x 1024 x 1024, Stefan number 1, A = 1, T_m = 0 (FP64 form)
+    "pf_sphere_256x512x512_512x1024x1024": dict(
+        nxm=256, nym=512, nzm=512, nsst=3, multires=1, nxmr=512, nymr=1024, nzmr=1024, phasefield=1,
+        alx3=1.0, ylen=2.0, zlen=2.0, istr3=8, istr3r=8, rayt=1e8, prat=1.0, active_t=1, gaxis=1,
+        pf_a=1.0, pf_s=1.0, pf_tm=0.0, pf_eps=1.5 / 512, pf_r0=0.15, dt=1e-4),
 }
 DEFAULT = "rbc_multires_384x768x768_768x1536x1536"
 
@@ -51,7 +58,7 @@ def log(msg):
 
 def cells(cfg):
     n = cfg["nxm"] * cfg["nym"] * cfg["nzm"]
-    if cfg.get("salinity"):
+    if cfg.get("salinity") or cfg.get("phasefield"):
         n += cfg["nxmr"] * cfg["nymr"] * cfg["nzmr"]
     return n
 
@@ -61,7 +68,8 @@ def kernel_bytes(cfg, form, sweeps=1, checks=0):
     operation must read or write once, at the form's element sizes.  In the
     mixed form `sweeps` Poisson sweeps ran, after `checks` residual checks."""
     nb = cfg["nxm"] * cfg["nym"] * cfg["nzm"]
-    nr = cfg["nxmr"] * cfg["nymr"] * cfg["nzmr"] if cfg.get("salinity") else 0
+    nr = cfg["nxmr"] * cfg["nymr"] * cfg["nzmr"] if (cfg.get("salinity") or cfg.get("phasefield")) else 0
+    pf = bool(cfg.get("phasefield"))
     ns = cfg["nxm"] * (cfg["nym"] // 2 + 1) * cfg["nzm"]       # complex spectral values
     nf = cfg["nxm"] * (cfg["nym"] // 2 + 1) * (cfg["nzm"] // 2 + 1)   # factor rows (modes i, nz-i shared)
     d = 8                                                     # FP64 state
@@ -71,7 +79,8 @@ def kernel_bytes(cfg, form, sweeps=1, checks=0):
     b = {
         # explicit terms + RHS + x-solve of vx, vy, vz, T in one sweep:
         # read u, v, w, T, p and the 4 old explicit terms, write u, v, w, T and the 4 new terms
-        "column_base": 9 * d * nb + 8 * h * nb,
+        # (+ phic and the latent heat with the phase field)
+        "column_base": 9 * d * nb + 8 * h * nb + (2 * d * nb if pf else 0),
         # salinity: read S, vxr, vyr, vzr, old H_S; write S, H_S
         "column_sal": 7 * h * nr,
         # read phi, u,v,w,p; write u,v,w,p
@@ -82,6 +91,13 @@ def kernel_bytes(cfg, form, sweeps=1, checks=0):
         # read S, write salc
         "mgrd_interp_sal": h * nr + h * nb,
     }
+    if pf:
+        b.pop("column_sal"); b.pop("mgrd_vxr"); b.pop("mgrd_vyr"); b.pop("mgrd_vzr"); b.pop("mgrd_interp_sal")
+        # phi column: read phi, tempr, old H; write phi, H, d phi
+        b["pf_column_phi"] = 6 * d * nr
+        b["pf_latent_heat"] = d * nr + d * nb       # read d phi, write the coarse source
+        b["pf_interp_temp"] = d * nb + d * nr       # read T, write tempr
+        b["pf_interp_phi"] = d * nr + d * nb        # read phi, write phic
     # Poisson: forward y (velocity -> spectrum; a residual check also reads
     # dph), two z transforms, the per-mode tridiagonals (+ their factors),
     # inverse y (-> dph; refinement sweeps read and write dph)
@@ -100,7 +116,7 @@ def cpu_baseline(args):
     wl = dict(WORKLOADS[args.workload])
     dt = wl.pop("dt")
     f = args.cpu_shrink
-    if wl.get("salinity"):
+    if wl.get("salinity") or wl.get("phasefield"):
         wl.update(nym=wl["nym"] // f, nzm=wl["nzm"] // f, nymr=wl["nymr"] // f, nzmr=wl["nzmr"] // f,
                   ylen=wl["ylen"] / f, zlen=wl["zlen"] / f)
     else:
@@ -127,6 +143,17 @@ def cpu_baseline(args):
         rr = uniform(args.seed + 1, S.shape)[..., :nxmr]
         S[..., :nxmr] = 1.0 - xmr + ampr * (2 * rr - 1)
         o.call("interp_sal_multigrid")
+    if wl.get("phasefield"):
+        # solid sphere of radius r0 at the centre of the sampled box
+        nxmr = wl["nxmr"]
+        xmr = o.grid("xmr")[:nxmr]
+        ymr = (np.arange(wl["nymr"]) + 0.5) * wl["ylen"] / wl["nymr"]
+        zmr = (np.arange(wl["nzmr"]) + 0.5) * wl["zlen"] / wl["nzmr"]
+        rr = np.sqrt((xmr[None, None, :] - 0.5) ** 2 + (ymr[None, :, None] - 0.5 * wl["ylen"]) ** 2 +
+                     (zmr[:, None, None] - 0.5 * wl["zlen"]) ** 2)
+        o.field("phi")[..., :nxmr] = 0.5 * (1 - np.tanh(0.5 * (rr - wl["pf_r0"]) / wl["pf_eps"]))
+        o.call("interp_temp_multigrid")
+        o.call("interp_phi_multigrid")
     times = []
     t_start = time.perf_counter()
     for _ in range(args.cpu_steps):
@@ -142,7 +169,7 @@ def cpu_baseline(args):
             "sample": (f"FP64 CPU oracle (C, OpenMP {threads} threads), {len(times)} RK3 step(s) of the same "
                        f"workload with the periodic extent cut by {f} in y and z "
                        f"({wl['nxm']}x{wl['nym']}x{wl['nzm']}" +
-                       (f" + {wl['nxmr']}x{wl['nymr']}x{wl['nzmr']}" if wl.get('salinity') else "") +
+                       (f" + {wl['nxmr']}x{wl['nymr']}x{wl['nzmr']}" if 'nxmr' in wl else "") +
                        f" cells, same spacing); minimum step time {best:.3f} s")}
 
 
@@ -150,7 +177,7 @@ FIELDS_CMP = ["vx", "vy", "vz", "temp", "pr"]
 
 
 def interior(wl, name, a):
-    if name == "sal":
+    if name in ("sal", "phi"):
         return a[..., :wl["nxmr"]]
     if name == "vx":
         return a[..., 1:wl["nxm"]]
@@ -197,7 +224,7 @@ def run_form(wl, dt, form, args):
     res["gq"] = g.global_quantities()
     res["residuals"] = g.refinement_residuals()
     if not args.no_errors:
-        names = FIELDS_CMP + (["sal"] if wl.get("salinity") else [])
+        names = FIELDS_CMP + (["sal"] if wl.get("salinity") else []) + (["phi"] if wl.get("phasefield") else [])
         res["fields"] = {n: interior(wl, n, g.download(n)) for n in names}
     g.close()
     return res
@@ -248,6 +275,11 @@ def main():
     dt = wl.pop("dt")
     ncell = cells(wl)
     forms = ["fp64", "mixed"] if args.form == "both" else [args.form]
+    if wl.get("phasefield"):
+        if args.form == "mixed":
+            print("bench.py: the phase-field workload runs in the FP64 form only", file=sys.stderr)
+            sys.exit(2)
+        forms = ["fp64"]
     runs = {f: run_form(wl, dt, f, args) for f in forms}
 
     head = runs["mixed"] if "mixed" in runs else runs["fp64"]
@@ -297,7 +329,7 @@ def main():
         "dtype": "f32+f64 (mixed form)" if hform == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": args.workload, "form": hform,
                    "base": [wl["nxm"], wl["nym"], wl["nzm"]],
-                   "refined": [wl["nxmr"], wl["nymr"], wl["nzmr"]] if wl.get("salinity") else None,
+                   "refined": [wl["nxmr"], wl["nymr"], wl["nzmr"]] if "nxmr" in wl else None,
                    "Ra": wl["rayt"], "Pr": wl["prat"], "Sc": wl.get("pras") if wl.get("salinity") else None,
                    "dt": dt, "cells": ncell},
         "roofline": rl,

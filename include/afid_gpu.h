@@ -56,6 +56,11 @@ typedef struct afid_config {
     int32_t precision;                             /* AFID_FP64 or AFID_MIXED */
     double div_tol;        /* AFID_MIXED: refinement stops once max|div u| <= div_tol (0: 1e-10) */
     int32_t max_sweeps;    /* AFID_MIXED: Poisson sweeps per substep, 1..4 (0: 3) */
+    /* phase field: pf_A, pf_S, pf_Tm of the bou.in PF line (pf_D follows,
+       ReadInputFile.F90:125); interface width pf_eps (0: 1/nxmr, :126);
+       radius pf_r0 of the solid sphere of the seeded initial state (0: 0.1,
+       phasefield.F90:109-110) */
+    double pf_a, pf_s, pf_tm, pf_eps, pf_r0;
 } afid_config;
 
 typedef struct afid_ctx afid_ctx;
@@ -67,6 +72,7 @@ enum afid_field {
     AFID_QCAP, AFID_DPH, AFID_DQ, AFID_HRO, AFID_RHS,
     AFID_SAL, AFID_RUSAL, AFID_HSAL, AFID_RHSR, AFID_VXR, AFID_VYR, AFID_VZR, AFID_SALC,
     AFID_TEMPBP, AFID_TEMPTP, AFID_SALBP, AFID_SALTP,
+    AFID_PHI, AFID_RUPHI, AFID_HPHI, AFID_PHIC, AFID_TEMPR,   /* phasefield.F90:12-16 */
     AFID_NFIELDS
 };
 
@@ -112,6 +118,16 @@ int afid_solve_pressure_correction(afid_ctx *ctx); /* pressure.F90:165-236 */
 int afid_correct_velocity(afid_ctx *ctx);          /* CorrectVelocity.F90:24 -> RemoveDivergence, pressure.F90:240-267 */
 int afid_correct_pressure(afid_ctx *ctx);          /* pressure.F90:270-301 */
 
+/* Phase field (phasefield.F90).  The velocity solves of
+ * afid_implicit_and_update_vx/vy/vz switch to the masked forms
+ * SolveImpEqnUpdate_X_pf / _YZ_pf (:821-930) when the phase field is on. */
+int afid_explicit_phase(afid_ctx *ctx);           /* phasefield.F90:345-383 -> hphi */
+int afid_add_volume_penalty(afid_ctx *ctx);       /* phasefield.F90:525-566 */
+int afid_implicit_phase(afid_ctx *ctx);           /* phasefield.F90:386-466 (rhsr keeps d phi) */
+int afid_add_latent_heat(afid_ctx *ctx);          /* phasefield.F90:651-725 */
+int afid_interp_temp_multigrid(afid_ctx *ctx);    /* phasefield.F90:498-520 */
+int afid_interp_phi_multigrid(afid_ctx *ctx);     /* phasefield.F90:470-494 */
+
 /* Multi-resolution transfers. */
 int afid_interp_vel_mgrd(afid_ctx *ctx);       /* InterpVelMgrd.F90:1-243 */
 int afid_interp_sal_multigrid(afid_ctx *ctx);  /* salinity.F90:458-491 (fast path, HermiteInterpolations.F90:256-312) */
@@ -139,9 +155,13 @@ int afid_global_quantities(afid_ctx *ctx, double out[4]);
  * u = 0): T = tempbp + (temptp - tempbp) x + noise, the wall-clamped noise of
  * amplitude eps being U[0,1) of splitmix64(seed ^ linear index) mapped to
  * [-eps, eps]; with salinity, S likewise on the refined grid from salbp/saltp
- * with seed+1, then salc.  The wall planes must be uploaded first.  The
- * reference seeds from the system clock (CreateInitialConditions.F90:24);
- * a counter-based generator makes CPU and GPU states identical. */
+ * with seed+1, then salc.  With the phase field, a solid sphere of radius
+ * pf_r0 at the centre (set_ice_sphere, phasefield.F90:311-341): phi =
+ * (1 - tanh((r - pf_r0)/(2 pf_eps)))/2 on the refined grid, T = 0 in the
+ * base cells with r <= pf_r0, then tempr and phic (main.F90:210-212).  The
+ * wall planes must be uploaded first.  The reference seeds from the system
+ * clock (CreateInitialConditions.F90:24); a counter-based generator makes
+ * CPU and GPU states identical. */
 int afid_create_initial_conditions(afid_ctx *ctx, uint64_t seed, double eps);
 
 /* Wait for all work enqueued on the context. */
