@@ -1,0 +1,377 @@
+// afid_ensemble.hpp -- an ensemble of independent 2-D runs (ny = 1, x-z
+// plane) advanced together: one launch per RK3 substep for every run.
+//
+// The reference advances one simulation per process (main.F90:281-422 around
+// TimeMarcher.F90:13-165).  Here a workgroup owns whole runs: it walks its
+// share of a static largest-first schedule and, for each run, does the full
+// substep inside the workgroup, with workgroup barriers between the phases
+// (runs never share data, so no grid-wide synchronisation exists):
+//
+//   factors   LU factors of the three wall-normal matrices for this run's dt
+//             (three wavefronts, Moebius prefix scan), into LDS;
+//   columns   explicit terms + implicit RHS + solve + update of vx, vz, T,
+//             one wavefront per column (the base column kernel's stencils;
+//             with ny = 1 every y difference is an exact zero and v stays 0);
+//   forward   divergence of two x-adjacent rows packed into one complex
+//             line, FFT along z in LDS, split into the two real spectra;
+//   modes     per-mode x-tridiagonal with the run's precomputed factors;
+//   inverse   the two spectra packed back, inverse FFT, into dph;
+//   project   RemoveDivergence + CorrectPressure;
+//   driver    (last substep) CFL and max|div| of the new state, time, the
+//             next dt and the run status (main.F90:285-302, 355-385).
+//
+// Per-run dt, time and status live in device memory, so a captured graph of
+// substep launches replays with no host round trip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "afid_params.hpp"
+#include "afid_stencils.hpp"
+#include "afid_kernels.hpp"
+#include "afid_poisson.hpp"
+
+namespace afid {
+
+// Read-only description of one run: its arrays and metrics (the KP of its
+// 2-D context, buffer set 0 in vx/vz/temp), both buffers of the fields the
+// column phase ping-pongs, the Poisson factors, spectrum and z transform,
+// the RK coefficients and the driver settings of its bou.in.
+struct EnsRun {
+    KP p;
+    double *vxb[2], *vzb[2], *tb[2];
+    PoissonFactors<double> pf;
+    cpx<double> *spec;
+    FftPlan<double> plan;
+    double gam[3], rom[3], alm[3];
+    double dtmin, dtmax, limit_cfl, tmax, resid;
+    int variable_dt, ntst, nsst, pad;
+};
+
+// Mutable state of one run (device), updated by the last substep of a step.
+// status: 0 running; otherwise the reference's exit code (165 CFL too high,
+// 166 dt below DTMIN, 169 divergence above RESID, 333 time above TMAX, 555
+// NTST steps done); a run with nonzero status is left untouched.
+struct EnsState {
+    double dt, time, cfl, dmax;
+    int ntime, status, cur, steps;
+};
+
+// order the LDS accesses of one wavefront (cross-lane exchange through LDS)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LU factors of one normalised constant tridiagonal matrix in the form
+// wave_tri_solve consumes, built by one wavefront into l/e/g/s (64*CH rows).
+// Rows [r0, r1) carry a_r = -am_r bd sb, c_r = -ap_r bd sb, s_r = sb =
+// 1/(1 - ac_r bd) (SolveImpEqnUpdate_X.F90:25-42, _YZ.F90:25-44,
+// _Temp.F90:25-46); the others are identity rows with zero right-hand side.
+// u_r = 1 - a_r c'_{r-1} comes from the Moebius prefix of c'.
+template <int CH>
+__device__ __forceinline__ void wave_build_factors(const double *am, const double *ac, const double *ap, double bd,
+                                                   int r0, int r1, double *l, double *e, double *g, double *s) {
+    const int lane = threadIdx.x & 63;
+    double a[CH], c[CH], gg[CH];
+#pragma unroll
+    for (int m = 0; m < CH; ++m) {
+        const int r = lane * CH + m;
+        if (r >= r0 && r < r1) {
+            const double sb = 1.0 / (1.0 - ac[r] * bd);
+            a[m] = -am[r] * bd * sb;
+            c[m] = -ap[r] * bd * sb;
+            s[r] = sb;
+        } else {
+            a[m] = 0.0; c[m] = 0.0; s[r] = 0.0;
+        }
+    }
+    double t = wave_mobius_prefix<CH>(a, c);
+#pragma unroll
+    for (int m = 0; m < CH; ++m) {
+        const int r = lane * CH + m;
+        gg[m] = 1.0 / (1.0 - a[m] * t);
+        t = c[m] * gg[m];
+        e[r] = t;
+        g[r] = gg[m];
+    }
+    double gprev = __shfl_up(gg[CH - 1], 1, 64);
+    if (lane == 0) gprev = 0.0;
+#pragma unroll
+    for (int m = 0; m < CH; ++m) l[lane * CH + m] = a[m] * (m == 0 ? gprev : gg[m - 1]);
+}
+
+// The reference's time-step control at the top of step s.ntime
+// (main.F90:285-302) from the CFL number of the state the step starts from.
+__device__ __forceinline__ void ens_top_of_step(const EnsRun &R, EnsState &s, double cfl) {
+    if (s.status) return;
+    if (R.variable_dt) {
+        if (s.ntime > 1) {
+            s.dt = cfl < 1.0e-8 ? R.dtmax : R.limit_cfl / cfl;
+            if (s.dt > R.dtmax) s.dt = R.dtmax;
+        }
+        if (s.dt < R.dtmin) s.status = 166;
+    } else if (cfl * s.dt > R.limit_cfl) {
+        s.status = 165;
+    }
+}
+
+// Workgroup max of two values (LDS scratch of 2*NT/64 doubles)
+template <int NT>
+__device__ __forceinline__ void block_max2(double &a, double &b, double *red) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmax(a, __shfl_down(a, off, 64));
+        b = fmax(b, __shfl_down(b, off, 64));
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) { red[wave] = a; red[NT / 64 + wave] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / 64; ++w) { a = fmax(a, red[w]); b = fmax(b, red[NT / 64 + w]); }
+    }
+}
+
+// CalcMaxCFL (CalcMaxCFL.F90:23-44) and CheckDivergence (CheckDivergence.F90:
+// 29-41) of one 2-D run, by the whole workgroup; the results are valid in
+// thread 0.  NaN propagates into dmax (fmax keeps it only if both are NaN, so
+// a NaN cell is turned into +inf first).
+template <int NT>
+__device__ __forceinline__ void ens_cfl_div(const KP &p, double &cfl, double &dmax, double *red) {
+    cfl = 1.0e-8;
+    dmax = 0.0;
+    const int nxm = p.nxm, nz = p.nz;
+    for (int t = threadIdx.x; t < nxm * nz; t += NT) {
+        const int k = t % nxm, i = t / nxm, ip = wrapp(i, nz);
+        const long long c = (long long)i * p.plane + k, cp = (long long)ip * p.plane + k;
+        const double q = fabs((p.vz[c] + p.vz[cp]) * 0.5 * p.dz) + fabs((p.vx[c] + p.vx[c + 1]) * 0.5 * p.udx3m[k]);
+        const double d = fabs((p.vz[cp] - p.vz[c]) * p.dz + (p.vx[c + 1] - p.vx[c]) * p.udx3m[k]);
+        cfl = fmax(cfl, q != q ? HUGE_VAL : q);
+        dmax = fmax(dmax, d != d ? HUGE_VAL : d);
+    }
+    block_max2<NT>(cfl, dmax, red);
+}
+
+// LDS of the substep kernel (bytes): the column phase (three factor sets and
+// two column buffers per wavefront) or the transform lines, whichever is
+// larger, plus the reduction scratch.
+template <int NT, int CH>
+__host__ __device__ constexpr int ens_col_lds() {
+    return int(sizeof(double)) * (12 * 64 * CH + (NT / 64) * 2 * lds_col_size<CH>());
+}
+// complex values per thread in a transform stage of the ensemble kernel
+constexpr int ENS_FFT_MAXV = 4;
+__host__ __device__ inline int ens_fft_lines(int n, int cap, int nt) {
+    int L = (ENS_FFT_MAXV * nt) / n;
+    if (L > cap) L = cap;
+    return L < 1 ? 1 : L;
+}
+
+// One RK3 substep ns (1-based) of every scheduled run: workgroup w owns runs
+// sched_run[sched_off[w] .. sched_off[w+1]).  Lines: transform lines per
+// LDS batch (the tuned tile size).
+template <int NT, int CH>
+__global__ void __launch_bounds__(NT) k_ens_substep(const EnsRun *__restrict__ runs, EnsState *st,
+                                                    const int *__restrict__ sched_off, const int *__restrict__ sched_run,
+                                                    int ns, int lines) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    constexpr int NW = NT / 64, NP = 64 * CH;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned lane = threadIdx.x & 63;
+    double *fac = reinterpret_cast<double *>(smraw);
+    double *cols = fac + 12 * NP;
+    cpx<double> *lb = reinterpret_cast<cpx<double> *>(smraw);
+    double *red = reinterpret_cast<double *>(smraw);       // reused after the last phase
+    const int q0 = sched_off[blockIdx.x], q1 = sched_off[blockIdx.x + 1];
+    for (int q = q0; q < q1; ++q) {
+        const int r = sched_run[q];
+        const EnsRun &R = runs[r];
+        if (__builtin_amdgcn_readfirstlane(st[r].status)) continue;     // uniform over the workgroup
+        const int cur = __builtin_amdgcn_readfirstlane(st[r].cur);
+        const double dt = st[r].dt;
+        KP p = R.p;
+        p.dt = dt;
+        p.al = R.alm[ns - 1]; p.ga = R.gam[ns - 1]; p.ro = R.rom[ns - 1];
+        p.beta = dt / p.ren * 0.5;
+        p.vx = R.vxb[cur]; p.vz = R.vzb[cur]; p.temp = R.tb[cur];
+        const int nx = p.nx, nxm = p.nxm, nz = p.nz;
+
+        // ---- factors of the vx, vz and T matrices for this dt and substep
+        if (wave < 3) {
+            double *f = fac + wave * 4 * NP;
+            if (wave == 0)        // identity rows 1 and nx (SolveImpEqnUpdate_X.F90:22-54)
+                wave_build_factors<CH>(p.am3ck, p.ac3ck, p.ap3ck, p.beta * p.al, 1, nxm, f, f + NP, f + 2 * NP, f + 3 * NP);
+            else if (wave == 1)
+                wave_build_factors<CH>(p.am3sk, p.ac3sk, p.ap3sk, p.beta * p.al, 0, nxm, f, f + NP, f + 2 * NP, f + 3 * NP);
+            else
+                wave_build_factors<CH>(p.am3ssk, p.ac3ssk, p.ap3ssk, 0.5 * p.al * p.dt / p.pect, 0, nxm, f, f + NP,
+                                       f + 2 * NP, f + 3 * NP);
+        }
+        __syncthreads();
+
+        // ---- columns: one wavefront per z index, vx, vz, T in turn
+        {
+            double *buf = cols + wave * 2 * lds_col_size<CH>(), *own = buf + lds_col_size<CH>();
+            for (int i = wave; i < nz; i += NW) {
+                const Nb n = make_nb(p, 0, i);
+                const long long c0 = (long long)i * p.plane;
+#pragma unroll 1
+                for (int fi = 0; fi < 3; ++fi) {
+                    switch (fi) {
+                    case 0: column_base_rows<0, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
+                    case 1: column_base_rows<2, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
+                    default: column_base_rows<3, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
+                    }
+                    wave_sync();
+                    const double *f = fac + fi * 4 * NP;
+                    TriFactors tf;
+                    tf.l = f; tf.e = f + NP; tf.g = f + 2 * NP; tf.s = f + 3 * NP; tf.a = nullptr; tf.c = nullptr;
+                    tf.n = fi == 0 ? nx : nxm; tf.chunk = CH;
+                    column_solve_lds<CH>(buf, tf);
+                    wave_sync();
+                    double *out = (fi == 0 ? R.vxb[cur ^ 1] : fi == 1 ? R.vzb[cur ^ 1] : R.tb[cur ^ 1]) + c0;
+                    for (unsigned rr = lane; rr < unsigned(nx); rr += 64) {
+                        const unsigned qq = lds_row<CH>(rr);
+                        double v = own[qq];
+                        if (fi == 0) v = (rr >= 1 && rr < unsigned(nxm)) ? v + buf[qq] : 0.0;   // walls, ImplicitAndUpdateVX.F90:86-87
+                        else if (rr < unsigned(nxm)) v += buf[qq];
+                        out[rr] = v;
+                    }
+                    wave_sync();
+                }
+            }
+        }
+        __syncthreads();
+        p.vx = R.vxb[cur ^ 1]; p.vz = R.vzb[cur ^ 1]; p.temp = R.tb[cur ^ 1];
+
+        // ---- divergence (pressure.F90:147-161) and forward transform along z
+        // (pressure.F90:171-179, normalised by 1/nz; ny = 1)
+        const int N = nz, S = line_stride<double>(N), nh = N / 2 + 1;
+        const int nlines = (nxm + 1) / 2;
+        const int L = ens_fft_lines(N, lines, NT);
+        const double usdtal = 1.0 / (p.dt * p.al), scale = 1.0 / double(N);
+        for (int l0 = 0; l0 < nlines; l0 += L) {
+            const int nl = min(L, nlines - l0), TX = 2 * nl, k0 = 2 * l0;
+            for (int t = threadIdx.x; t < TX * N; t += NT) {
+                const int xo = t % TX, i = t / TX, k = k0 + xo;
+                double d = 0.0;
+                if (k < nxm) {
+                    const long long c = (long long)i * p.plane + k, cp = (long long)wrapp(i, N) * p.plane + k;
+                    d = ((p.vz[cp] - p.vz[c]) * p.dz + (p.vx[c + 1] - p.vx[c]) * p.udx3m[k]) * usdtal;
+                }
+                double *slot = reinterpret_cast<double *>(&lb[(xo >> 1) * S + lpad<double>(i)]);
+                slot[xo & 1] = d;
+            }
+            __syncthreads();
+            lds_fft<double, NT, ENS_FFT_MAXV>(lb, nl, S, R.plan, -1);
+            for (int t = threadIdx.x; t < TX * nh; t += NT) {
+                const int xo = t % TX, m = t / TX, k = k0 + xo;
+                if (k >= nxm) continue;
+                const int line = xo >> 1;
+                const cpx<double> zm = lb[line * S + lpad<double>(m)], zn = lb[line * S + lpad<double>(m == 0 ? 0 : N - m)];
+                cpx<double> o;
+                if ((xo & 1) == 0) o = mk<double>(0.5 * (zm.x + zn.x), 0.5 * (zm.y - zn.y));
+                else o = mk<double>(0.5 * (zm.y + zn.y), -0.5 * (zm.x - zn.x));
+                R.spec[(long long)m * nxm + k] = mk<double>(o.x * scale, o.y * scale);
+            }
+            __syncthreads();
+        }
+
+        // ---- per-mode tridiagonals (pressure.F90:201-236); modes m and N-m
+        // share the matrix and the real data make them conjugate
+        for (int m = wave; m < nh; m += NW)
+            mode_tri_solve<CH, double>(R.pf.m, R.pf.g, R.pf.e, (long long)m * R.pf.npad, R.pf.npad,
+                                       R.spec + (long long)m * nxm, nxm);
+        __syncthreads();
+
+        // ---- inverse transform (pressure.F90:189-195) into dph
+        for (int l0 = 0; l0 < nlines; l0 += L) {
+            const int nl = min(L, nlines - l0), TX = 2 * nl, k0 = 2 * l0;
+            for (int t = threadIdx.x; t < nl * nh; t += NT) {
+                const int line = t % nl, m = t / nl;
+                const int ka = k0 + 2 * line, kb = ka + 1;
+                const cpx<double> *row = R.spec + (long long)m * nxm;
+                cpx<double> A = ka < nxm ? row[ka] : mk<double>(0.0, 0.0);
+                cpx<double> B = kb < nxm ? row[kb] : mk<double>(0.0, 0.0);
+                if (m == 0 || 2 * m == N) { A.y = 0.0; B.y = 0.0; }
+                lb[line * S + lpad<double>(m)] = mk<double>(A.x - B.y, A.y + B.x);
+                if (m > 0 && N - m > m) lb[line * S + lpad<double>(N - m)] = mk<double>(A.x + B.y, B.x - A.y);
+            }
+            __syncthreads();
+            lds_fft<double, NT, ENS_FFT_MAXV>(lb, nl, S, R.plan, +1);
+            for (int t = threadIdx.x; t < TX * N; t += NT) {
+                const int xo = t % TX, i = t / TX, k = k0 + xo;
+                if (k >= nxm) continue;
+                const cpx<double> z = lb[(xo >> 1) * S + lpad<double>(i)];
+                p.dph[(long long)i * p.plane + k] = (xo & 1) ? z.y : z.x;
+            }
+            __syncthreads();
+        }
+
+        // ---- RemoveDivergence + CorrectPressure (pressure.F90:245-300)
+        {
+            const double adt = p.al * p.dt, be = p.al * p.beta;
+            const double *d = p.dph;
+            for (int t = threadIdx.x; t < nxm * nz; t += NT) {
+                const int k = t % nxm, i = t / nxm, im = wrapm(i, nz), ip = wrapp(i, nz);
+                const int km = k == 0 ? 0 : k - 1, kp = k == nxm - 1 ? k : k + 1;
+                const long long c = (long long)i * p.plane + k;
+                const double dc = d[c];
+                p.vx[c] -= (dc - d[c - k + km]) * (adt * p.udx3c[k]);
+                p.vz[c] -= (dc - d[(long long)im * p.plane + k]) * (adt * p.dz);
+                p.pr[c] += dc - be * ((d[(long long)ip * p.plane + k] - 2.0 * dc + d[(long long)im * p.plane + k]) * p.dzq +
+                                      (d[c - k + kp] * p.apphk[k] + dc * p.acphk[k] + d[c - k + km] * p.amphk[k]));
+            }
+        }
+        __syncthreads();
+
+        // ---- driver: time, CFL / divergence checks and the next dt
+        if (ns == R.nsst) {
+            double cfl, dmax;
+            ens_cfl_div<NT>(p, cfl, dmax, red);
+            if (threadIdx.x == 0) {
+                EnsState s = st[r];
+                s.time += s.dt;
+                s.cfl = cfl; s.dmax = dmax;
+                if (!(dmax <= R.resid)) s.status = 169;
+                else if (s.time > R.tmax) s.status = 333;
+                else if (s.ntime == R.ntst) s.status = 555;
+                s.ntime += 1;
+                s.steps += 1;
+                ens_top_of_step(R, s, cfl);
+                s.cur = cur ^ 1;
+                st[r] = s;
+            }
+        } else if (threadIdx.x == 0) {
+            st[r].cur = cur ^ 1;
+        }
+        __syncthreads();
+    }
+}
+
+// CFL and divergence of the current state of every scheduled run; with
+// top != 0 the time-step control of the step about to start is applied
+// (after new initial conditions or uploads).
+template <int NT>
+__global__ void __launch_bounds__(NT) k_ens_prepare(const EnsRun *__restrict__ runs, EnsState *st,
+                                                    const int *__restrict__ sched_off, const int *__restrict__ sched_run,
+                                                    int top) {
+    __shared__ double red[2 * NT / 64];
+    const int q0 = sched_off[blockIdx.x], q1 = sched_off[blockIdx.x + 1];
+    for (int q = q0; q < q1; ++q) {
+        const int r = sched_run[q];
+        const EnsRun &R = runs[r];
+        EnsState s = st[r];
+        KP p = R.p;
+        p.vx = R.vxb[s.cur]; p.vz = R.vzb[s.cur];
+        double cfl, dmax;
+        ens_cfl_div<NT>(p, cfl, dmax, red);
+        if (threadIdx.x == 0) {
+            s.cfl = cfl; s.dmax = dmax;
+            if (top) ens_top_of_step(R, s, cfl);
+            st[r] = s;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace afid

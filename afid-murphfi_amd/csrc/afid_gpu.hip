@@ -43,6 +43,7 @@ struct afid_ctx {
     afid_config c{};
     std::string err;
     hipStream_t stream = nullptr;
+    bool own_stream = true;            // false: the stream belongs to an ensemble
     int device = 0;
     // sizes
     int nxm, nx, ny, nz, ldx, ldt, nymh;
@@ -815,7 +816,10 @@ static int setup(afid_ctx *c) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int afid_gpu_init(const afid_config *cfg, afid_ctx **out) {
+}  // extern "C"
+
+// A context on the caller's stream (shared) or on a stream of its own.
+static int ctx_create(const afid_config *cfg, hipStream_t shared, afid_ctx **out) {
     if (!cfg || !out) return AFID_E_ARG;
     *out = nullptr;
     afid_ctx *c = new afid_ctx();
@@ -827,7 +831,13 @@ int afid_gpu_init(const afid_config *cfg, afid_ctx **out) {
         return AFID_E_HIP;
     }
     c->device = dev;
-    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return AFID_E_HIP; }
+    if (shared) {
+        c->stream = shared;
+        c->own_stream = false;
+    } else if (hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return AFID_E_HIP;
+    }
     int rc;
     try {
         rc = setup(c);
@@ -842,12 +852,16 @@ int afid_gpu_init(const afid_config *cfg, afid_ctx **out) {
     return rc;
 }
 
+extern "C" {
+
+int afid_gpu_init(const afid_config *cfg, afid_ctx **out) { return ctx_create(cfg, nullptr, out); }
+
 int afid_gpu_finalize(afid_ctx *c) {
     if (!c) return AFID_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto &b : c->bufs) (void)hipFree(b.p);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return AFID_OK;
 }
@@ -1493,3 +1507,6 @@ int afid_grid_array(afid_ctx *c, const char *name, double *out, int64_t n, int64
 }
 
 }  // extern "C"
+
+// the batched 2-D ensemble (include/afid_ensemble.h)
+#include "afid_ensemble_api.hpp"

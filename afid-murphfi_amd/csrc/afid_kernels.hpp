@@ -208,8 +208,10 @@ __device__ __forceinline__ Mob mob_mul(const Mob &x, const Mob &y) {   // x * y
 // its prefix over the wavefront is a scan of 2x2 matrix products.  With c'
 // known, forward elimination and back substitution are affine scans as in
 // wave_tri_solve.
+// c' of the row before this lane's first row: the Moebius prefix of the
+// earlier lanes' rows applied to c' = 0
 template <int CH>
-__device__ __forceinline__ void wave_tri_solve_general(double (&v)[CH], const double (&a)[CH], const double (&c)[CH]) {
+__device__ __forceinline__ double wave_mobius_prefix(const double (&a)[CH], const double (&c)[CH]) {
     const int lane = threadIdx.x & 63;
     Mob L{1.0, 0.0, 0.0, 1.0};
 #pragma unroll
@@ -219,9 +221,15 @@ __device__ __forceinline__ void wave_tri_solve_general(double (&v)[CH], const do
         Mob P{__shfl_up(L.m00, off, 64), __shfl_up(L.m01, off, 64), __shfl_up(L.m10, off, 64), __shfl_up(L.m11, off, 64)};
         if (lane >= off) L = mob_mul(L, P);
     }
-    // c' before this lane's first row: the exclusive prefix applied to (0, 1)
+    // the exclusive prefix applied to (0, 1)
     const double p0 = __shfl_up(L.m01, 1, 64), q0 = __shfl_up(L.m11, 1, 64);
-    double t = lane == 0 ? 0.0 : p0 / q0;
+    return lane == 0 ? 0.0 : p0 / q0;
+}
+
+template <int CH>
+__device__ __forceinline__ void wave_tri_solve_general(double (&v)[CH], const double (&a)[CH], const double (&c)[CH]) {
+    const int lane = threadIdx.x & 63;
+    double t = wave_mobius_prefix<CH>(a, c);
     double cp[CH], g[CH];
 #pragma unroll
     for (int m = 0; m < CH; ++m) { g[m] = 1.0 / (1.0 - a[m] * t); cp[m] = c[m] * g[m]; t = cp[m]; }

@@ -104,16 +104,17 @@ __host__ __device__ __forceinline__ int lpad(int i) { return i + (i >> (sizeof(R
 template <typename R>
 __host__ __device__ __forceinline__ int line_stride(int n) { return lpad<R>(n) + 1; }
 
-// One Stockham stage of radix RD over L lines of length N held in LDS.
-// Each thread keeps at most MAXB butterflies of the stage in registers
-// between the read and the write phase, so the transform runs in place.
-template <int RD, int MAXB, typename R>
+// One Stockham stage of radix RD over L lines of length N held in LDS, by
+// the NT threads of the workgroup (L*N <= FFT_MAXV*NT).  Each thread keeps
+// at most MAXB butterflies of the stage in registers between the read and
+// the write phase, so the transform runs in place.
+template <int RD, int MAXB, typename R, int NT>
 __device__ __forceinline__ void fft_stage(cpx<R> *buf, int L, int N, int S, int Ns, const cpx<R> *tw, int sign) {
     const int nbl = N / RD, nb = L * nbl;
     cpx<R> v[MAXB][RD];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-        const int b = threadIdx.x + q * FFT_THREADS;
+        const int b = threadIdx.x + q * NT;
         if (b < nb) {
             const int line = b / nbl, jj = b - line * nbl;
 #pragma unroll
@@ -124,7 +125,7 @@ __device__ __forceinline__ void fft_stage(cpx<R> *buf, int L, int N, int S, int 
     const int tstride = N / (Ns * RD);
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-        const int b = threadIdx.x + q * FFT_THREADS;
+        const int b = threadIdx.x + q * NT;
         if (b < nb) {
             const int line = b / nbl, jj = b - line * nbl, kk = jj % Ns;
 #pragma unroll
@@ -142,17 +143,19 @@ __device__ __forceinline__ void fft_stage(cpx<R> *buf, int L, int N, int S, int 
     __syncthreads();
 }
 
-// Full transform of L lines (L*N <= FFT_MAXV*FFT_THREADS), line stride S.
-template <typename R>
+// Full transform of L lines (L*N <= FFT_MAXV*NT), line stride S, by a
+// workgroup of NT threads.
+// MAXV: the most complex values a thread holds in a stage (L*N <= MAXV*NT).
+template <typename R, int NT = FFT_THREADS, int MAXV = FFT_MAXV>
 __device__ __forceinline__ void lds_fft(cpx<R> *buf, int L, int S, const FftPlan<R> &pl, int sign) {
     int Ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int RD = pl.rad[s];
         switch (RD) {
-        case 8: fft_stage<8, (FFT_MAXV + 7) / 8, R>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
-        case 4: fft_stage<4, (FFT_MAXV + 3) / 4, R>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
-        case 3: fft_stage<3, (FFT_MAXV + 2) / 3, R>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
-        default: fft_stage<2, (FFT_MAXV + 1) / 2, R>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
+        case 8: fft_stage<8, (MAXV + 7) / 8, R, NT>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
+        case 4: fft_stage<4, (MAXV + 3) / 4, R, NT>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
+        case 3: fft_stage<3, (MAXV + 2) / 3, R, NT>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
+        default: fft_stage<2, (MAXV + 1) / 2, R, NT>(buf, L, pl.n, S, Ns, pl.tw, sign); break;
         }
         Ns *= RD;
     }
@@ -274,22 +277,21 @@ struct PoissonFactors {
     const R *m, *g, *e;   // [mode'][row padded to 64*CH], mode' = m*(nz/2+1) + i'
     int npad;
 };
+// Solve of one mode's column col (nxm complex values) by one wavefront,
+// CH rows per lane, with the factors at fm/fg/fe + fo (rows >= npad read as
+// zero, so any CH with 64*CH >= nxm serves).
 template <int CH, typename R>
-__global__ void __launch_bounds__(128) k_poisson_tri(KP p, PoissonFactors<R> pf, cpx<R> *spec, Gate gate) {
-    if (gate_closed(gate)) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int m = blockIdx.x, ih = blockIdx.y, nzh = p.nz / 2 + 1;
-    const int i = wave == 0 ? ih : p.nz - ih;
-    if (wave == 1 && (ih == 0 || i == ih || i >= p.nz)) return;
-    const long long fo = ((long long)m * nzh + ih) * pf.npad;
-    cpx<R> *col = spec + ((long long)i * p.nymh + m) * p.nxm;
+__device__ __forceinline__ void mode_tri_solve(const R *fmp, const R *fgp, const R *fep, long long fo, int npad,
+                                               cpx<R> *col, int nxm) {
+    const int lane = threadIdx.x & 63;
     R fm[CH], fg[CH], fe[CH];
     cpx<R> v[CH];
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
         const int r = lane * CH + q;
-        fm[q] = pf.m[fo + r]; fg[q] = pf.g[fo + r]; fe[q] = pf.e[fo + r];
-        v[q] = r < p.nxm ? col[r] : mk<R>(R(0), R(0));
+        const bool in = r < npad;
+        fm[q] = in ? fmp[fo + r] : R(0); fg[q] = in ? fgp[fo + r] : R(0); fe[q] = in ? fep[fo + r] : R(0);
+        v[q] = r < nxm ? col[r] : mk<R>(R(0), R(0));
     }
     // forward y_r = d_r - m_r y_{r-1} (complex B, real A)
     R A = R(1);
@@ -325,8 +327,19 @@ __global__ void __launch_bounds__(128) k_poisson_tri(KP p, PoissonFactors<R> pf,
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
         const int r = lane * CH + q;
-        if (r < p.nxm) col[r] = v[q];
+        if (r < nxm) col[r] = v[q];
     }
+}
+
+template <int CH, typename R>
+__global__ void __launch_bounds__(128) k_poisson_tri(KP p, PoissonFactors<R> pf, cpx<R> *spec, Gate gate) {
+    if (gate_closed(gate)) return;
+    const int wave = threadIdx.x >> 6;
+    const int m = blockIdx.x, ih = blockIdx.y, nzh = p.nz / 2 + 1;
+    const int i = wave == 0 ? ih : p.nz - ih;
+    if (wave == 1 && (ih == 0 || i == ih || i >= p.nz)) return;
+    const long long fo = ((long long)m * nzh + ih) * pf.npad;
+    mode_tri_solve<CH, R>(pf.m, pf.g, pf.e, fo, pf.npad, spec + ((long long)i * p.nymh + m) * p.nxm, p.nxm);
 }
 
 // Set-up of the per-mode factors: one thread per (m, i'), sequential over x.

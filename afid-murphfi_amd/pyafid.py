@@ -40,6 +40,14 @@ EXPORTS = [
     "afid_set_timing", "afid_get_timing", "afid_grid_array",
 ]
 
+# every entry point declared in include/afid_ensemble.h
+ENS_EXPORTS = [
+    "afid_ens_init", "afid_ens_finalize", "afid_ens_last_error", "afid_ens_device_bytes",
+    "afid_ens_upload", "afid_ens_download", "afid_ens_create_initial_conditions", "afid_ens_reset",
+    "afid_ens_step", "afid_ens_run_state", "afid_ens_global_quantities",
+    "afid_ens_autotune", "afid_ens_set_tuning", "afid_ens_get_tuning", "afid_ens_synchronize",
+]
+
 ROUTINES = ["explicit_terms_vx", "explicit_terms_vy", "explicit_terms_vz", "explicit_terms_temp",
             "explicit_salinity", "add_sal_buoyancy",
             "implicit_and_update_vx", "implicit_and_update_vy", "implicit_and_update_vz",
@@ -91,6 +99,40 @@ def make_config(d):
     return c
 
 
+class RunControl(C.Structure):
+    """Mirror of afid_run_control: IDTV, DT, RESID, CFLMAX, DTMIN, DTMAX of a
+    run's bou.in and its NTST / TMAX (main.F90:287-302, 355-385)."""
+    _fields_ = [("variable_dt", C.c_int32), ("dt", C.c_double), ("resid", C.c_double), ("limit_cfl", C.c_double),
+                ("dtmin", C.c_double), ("dtmax", C.c_double), ("ntst", C.c_int32), ("tmax", C.c_double)]
+
+
+class EnsTuning(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("workgroups", C.c_int32), ("lines", C.c_int32), ("ms_per_step", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RunState(C.Structure):
+    _fields_ = [("time", C.c_double), ("dt", C.c_double), ("cfl", C.c_double), ("max_div", C.c_double),
+                ("ntime", C.c_int32), ("status", C.c_int32), ("steps", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+CONTROL_DEFAULTS = dict(variable_dt=0, dt=1e-3, resid=1e-3, limit_cfl=1.0, dtmin=1e-8, dtmax=0.0, ntst=1 << 30, tmax=0.0)
+
+
+def make_control(d):
+    v = dict(CONTROL_DEFAULTS)
+    v.update(d)
+    c = RunControl()
+    for k, _ in RunControl._fields_:
+        setattr(c, k, v[k])
+    return c
+
+
 _lib = None
 
 
@@ -124,6 +166,24 @@ def lib():
         L.afid_get_timing.argtypes = [vp, C.POINTER(C.c_char_p), dp, C.c_int, C.POINTER(C.c_int)]
         L.afid_grid_array.argtypes = [vp, C.c_char_p, dp, C.c_int64, i64p]
         L.afid_refinement_residuals.argtypes = [vp, dp, C.c_int, C.POINTER(C.c_int)]
+        L.afid_ens_init.argtypes = [C.POINTER(AfidConfig), C.POINTER(RunControl), C.c_int32, C.POINTER(vp)]
+        L.afid_ens_finalize.argtypes = [vp]
+        L.afid_ens_last_error.argtypes = [vp]
+        L.afid_ens_last_error.restype = C.c_char_p
+        L.afid_ens_device_bytes.argtypes = [vp]
+        L.afid_ens_device_bytes.restype = C.c_int64
+        L.afid_ens_upload.argtypes = [vp, C.c_int32, C.c_int32, dp, i64p, i64p]
+        L.afid_ens_download.argtypes = [vp, C.c_int32, C.c_int32, dp, i64p, i64p]
+        L.afid_ens_create_initial_conditions.argtypes = [vp, C.c_uint64, C.c_double]
+        L.afid_ens_reset.argtypes = [vp]
+        L.afid_ens_step.argtypes = [vp, C.c_int32, C.c_int32]
+        L.afid_ens_run_state.argtypes = [vp, C.c_int32, C.POINTER(RunState)]
+        L.afid_ens_global_quantities.argtypes = [vp, C.c_int32, dp]
+        L.afid_ens_autotune.argtypes = [vp, C.c_int32, C.POINTER(EnsTuning), C.POINTER(EnsTuning), C.c_int32,
+                                        C.POINTER(C.c_int32)]
+        L.afid_ens_set_tuning.argtypes = [vp, C.POINTER(EnsTuning)]
+        L.afid_ens_get_tuning.argtypes = [vp, C.POINTER(EnsTuning)]
+        L.afid_ens_synchronize.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -253,3 +313,108 @@ class Solver:
         self._check(self.L.afid_grid_array(self.h, name.encode(), out.ctypes.data_as(C.POINTER(C.c_double)),
                                            n.value, C.byref(n)), "afid_grid_array")
         return out
+
+
+class Ensemble:
+    """Many independent 2-D runs (nym = 1) advanced together: each run is a
+    bou.in (config dict) plus its time-stepping control (RunControl fields).
+    Mirrors the reference's driver loop (main.F90:281-422) per run."""
+
+    def __init__(self, cfgs, controls):
+        assert len(cfgs) == len(controls) and cfgs
+        self.cfgds = []
+        for c in cfgs:
+            d = dict(DEFAULTS)
+            d.update(c)
+            self.cfgds.append(d)
+        self.n = len(cfgs)
+        arr = (AfidConfig * self.n)(*[make_config(d) for d in self.cfgds])
+        ctl = (RunControl * self.n)(*[make_control(k) for k in controls])
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.afid_ens_init(arr, ctl, self.n, C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.L.afid_ens_last_error(h).decode() if h else "no ensemble"
+            if h:
+                self.L.afid_ens_finalize(h)
+            self.h = None
+            raise AfidError(f"afid_ens_init failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.afid_ens_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise AfidError(f"{what} failed ({rc}): {self.L.afid_ens_last_error(self.h).decode()}")
+
+    def shape(self, run, name):
+        c = self.cfgds[run]
+        if name in PLANES:
+            return (c["nzm"], 1, 1)
+        return (c["nzm"], 1, c["nxm"] + 1)
+
+    def upload(self, run, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        ext = (C.c_int64 * 3)(a.shape[2], a.shape[1], a.shape[0])
+        lb = (C.c_int64 * 3)(1, 1, 1)
+        self._check(self.L.afid_ens_upload(self.h, run, FIELD_ID[name], a.ctypes.data_as(C.POINTER(C.c_double)), lb, ext),
+                    f"afid_ens_upload {name} run {run}")
+
+    def download(self, run, name):
+        shape = self.shape(run, name)
+        a = np.zeros(shape, dtype=np.float64)
+        ext = (C.c_int64 * 3)(shape[2], shape[1], shape[0])
+        lb = (C.c_int64 * 3)(1, 1, 1)
+        self._check(self.L.afid_ens_download(self.h, run, FIELD_ID[name], a.ctypes.data_as(C.POINTER(C.c_double)), lb, ext),
+                    f"afid_ens_download {name} run {run}")
+        return a
+
+    def create_initial_conditions(self, seed0, eps=1e-3):
+        self._check(self.L.afid_ens_create_initial_conditions(self.h, seed0, eps), "afid_ens_create_initial_conditions")
+
+    def reset(self):
+        self._check(self.L.afid_ens_reset(self.h), "afid_ens_reset")
+
+    def step(self, nsteps=1, graph=True):
+        self._check(self.L.afid_ens_step(self.h, nsteps, 1 if graph else 0), "afid_ens_step")
+
+    def synchronize(self):
+        self._check(self.L.afid_ens_synchronize(self.h), "afid_ens_synchronize")
+
+    def run_state(self, run):
+        s = RunState()
+        self._check(self.L.afid_ens_run_state(self.h, run, C.byref(s)), "afid_ens_run_state")
+        return s.as_dict()
+
+    def global_quantities(self, run):
+        out = (C.c_double * 4)()
+        self._check(self.L.afid_ens_global_quantities(self.h, run, out), "afid_ens_global_quantities")
+        return dict(zip(["nu", "ke", "vrms", "tmean"], list(out)))
+
+    def autotune(self, steps=2):
+        best = EnsTuning()
+        log = (EnsTuning * 64)()
+        n = C.c_int32()
+        self._check(self.L.afid_ens_autotune(self.h, steps, C.byref(best), log, 64, C.byref(n)), "afid_ens_autotune")
+        return best.as_dict(), [log[q].as_dict() for q in range(min(n.value, 64))]
+
+    def set_tuning(self, threads, workgroups=0, lines=8):
+        t = EnsTuning(threads, workgroups, lines, 0.0)
+        self._check(self.L.afid_ens_set_tuning(self.h, C.byref(t)), "afid_ens_set_tuning")
+
+    def get_tuning(self):
+        t = EnsTuning()
+        self._check(self.L.afid_ens_get_tuning(self.h, C.byref(t)), "afid_ens_get_tuning")
+        return t.as_dict()
+
+    def device_bytes(self):
+        return int(self.L.afid_ens_device_bytes(self.h))

@@ -1,5 +1,6 @@
 """CPU checks of the C ABI: the built library loads and exports every entry
-point declared in include/afid_gpu.h (no compute calls: no GPU here)."""
+point declared in include/afid_gpu.h and include/afid_ensemble.h (no
+compute calls: no GPU here)."""
 import os
 import re
 import subprocess
@@ -10,15 +11,18 @@ import pyafid
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "afid_gpu.h")
+ENS_HEADER = os.path.join(ROOT, "include", "afid_ensemble.h")
 
 
-def declared():
-    txt = open(HEADER).read()
+def declared(header=None):
+    files = [header] if header else [HEADER, ENS_HEADER]
+    txt = "".join(open(f).read() for f in files)
     return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*\*?\s*(afid_\w+)\s*\(", txt, re.M)))
 
 
 def test_header_and_binding_agree():
-    assert declared() == sorted(pyafid.EXPORTS)
+    assert declared(HEADER) == sorted(pyafid.EXPORTS)
+    assert declared(ENS_HEADER) == sorted(pyafid.ENS_EXPORTS)
 
 
 @pytest.mark.skipif(not os.path.exists(pyafid.LIB_PATH), reason="library not built")
@@ -39,3 +43,17 @@ def test_config_struct_layout():
     names = re.findall(r"\b(?:int32_t|double)\s+([^;]+);", body)
     flat = [n.strip() for group in names for n in group.split(",")]
     assert flat == [f for f, _ in pyafid.AfidConfig._fields_]
+
+
+def _struct_fields(header, name):
+    txt = open(header).read()
+    body = txt[txt.index("typedef struct " + name):txt.index("} " + name + ";")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"\b(?:int32_t|double)\s+([^;]+);", body)
+    return [n.strip() for group in names for n in group.split(",")]
+
+
+def test_ensemble_struct_layouts():
+    assert _struct_fields(ENS_HEADER, "afid_run_control") == [f for f, _ in pyafid.RunControl._fields_]
+    assert _struct_fields(ENS_HEADER, "afid_ens_tuning") == [f for f, _ in pyafid.EnsTuning._fields_]
+    assert _struct_fields(ENS_HEADER, "afid_run_state") == [f for f, _ in pyafid.RunState._fields_]
