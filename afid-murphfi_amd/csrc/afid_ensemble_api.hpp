@@ -394,6 +394,27 @@ int afid_ens_step(afid_ens *e, int32_t nsteps, int32_t graph) {
     return ens_steps_eager(e, nsteps);
 }
 
+int afid_ens_step_timed(afid_ens *e, int32_t nsteps, int32_t graph, double *ms) {
+    if (!e || !ms) return AFID_E_ARG;
+    int rc = AFID_OK;
+    if (graph && !e->graph && (rc = ens_capture(e))) return rc;    // capture outside the timed region
+    hipEvent_t a = nullptr, b = nullptr;
+    ENSCHK(e, hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return ens_err(e, AFID_E_HIP, "hipEventCreate failed"); }
+    hipError_t he = hipEventRecord(a, e->stream);
+    if (he == hipSuccess) rc = afid_ens_step(e, nsteps, graph);
+    if (he == hipSuccess && rc == AFID_OK) he = hipEventRecord(b, e->stream);
+    if (he == hipSuccess && rc == AFID_OK) he = hipEventSynchronize(b);
+    float t = 0.f;
+    if (he == hipSuccess && rc == AFID_OK) he = hipEventElapsedTime(&t, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (he != hipSuccess) return ens_err(e, AFID_E_HIP, std::string("timed step: ") + hipGetErrorString(he));
+    *ms = double(t);
+    return AFID_OK;
+}
+
 int afid_ens_run_state(afid_ens *e, int32_t run, afid_run_state *out) {
     if (!e || !out || run < 0 || run >= e->nruns) return ens_err(e, AFID_E_ARG, "ensemble: bad run index");
     EnsState s;

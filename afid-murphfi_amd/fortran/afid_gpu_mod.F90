@@ -29,6 +29,25 @@ module afid_gpu
         real(c_double)     :: pf_a, pf_s, pf_tm, pf_eps, pf_r0
     end type afid_config
 
+    !> batched 2-D ensemble (include/afid_ensemble.h): the time-stepping line
+    !! of each run's bou.in and its NTST / TMAX (main.F90:287-302, 355-385)
+    type, bind(C) :: afid_run_control
+        integer(c_int32_t) :: variable_dt
+        real(c_double)     :: dt, resid, limit_cfl, dtmin, dtmax
+        integer(c_int32_t) :: ntst
+        real(c_double)     :: tmax
+    end type afid_run_control
+
+    type, bind(C) :: afid_ens_tuning
+        integer(c_int32_t) :: threads, workgroups, lines
+        real(c_double)     :: ms_per_step
+    end type afid_ens_tuning
+
+    type, bind(C) :: afid_run_state
+        real(c_double)     :: time, dt, cfl, max_div
+        integer(c_int32_t) :: ntime, status, steps
+    end type afid_run_state
+
     interface
         integer(c_int) function afid_gpu_init(cfg, ctx) bind(C, name='afid_gpu_init')
             import; type(afid_config), intent(in) :: cfg; type(c_ptr), intent(out) :: ctx
@@ -119,6 +138,43 @@ module afid_gpu
         end function
         integer(c_int) function afid_synchronize(ctx) bind(C, name='afid_synchronize')
             import; type(c_ptr), value :: ctx
+        end function
+        !> ensemble: many 2-D runs advanced together (main.F90:281-422 per run)
+        integer(c_int) function afid_ens_init(cfgs, ctl, nruns, ens) bind(C, name='afid_ens_init')
+            import; type(afid_config), intent(in) :: cfgs(*); type(afid_run_control), intent(in) :: ctl(*)
+            integer(c_int32_t), value :: nruns; type(c_ptr), intent(out) :: ens
+        end function
+        integer(c_int) function afid_ens_finalize(ens) bind(C, name='afid_ens_finalize')
+            import; type(c_ptr), value :: ens
+        end function
+        type(c_ptr) function afid_ens_last_error(ens) bind(C, name='afid_ens_last_error')
+            import; type(c_ptr), value :: ens
+        end function
+        integer(c_int) function afid_ens_upload(ens, run, field, host, lb, ext) bind(C, name='afid_ens_upload')
+            import; type(c_ptr), value :: ens; integer(c_int32_t), value :: run, field
+            real(c_double), intent(in) :: host(*); integer(c_int64_t), intent(in) :: lb(3), ext(3)
+        end function
+        integer(c_int) function afid_ens_download(ens, run, field, host, lb, ext) bind(C, name='afid_ens_download')
+            import; type(c_ptr), value :: ens; integer(c_int32_t), value :: run, field
+            real(c_double), intent(out) :: host(*); integer(c_int64_t), intent(in) :: lb(3), ext(3)
+        end function
+        integer(c_int) function afid_ens_create_initial_conditions(ens, seed0, eps) &
+                bind(C, name='afid_ens_create_initial_conditions')
+            import; type(c_ptr), value :: ens; integer(c_int64_t), value :: seed0; real(c_double), value :: eps
+        end function
+        integer(c_int) function afid_ens_reset(ens) bind(C, name='afid_ens_reset')
+            import; type(c_ptr), value :: ens
+        end function
+        integer(c_int) function afid_ens_step(ens, nsteps, graph) bind(C, name='afid_ens_step')
+            import; type(c_ptr), value :: ens; integer(c_int32_t), value :: nsteps, graph
+        end function
+        integer(c_int) function afid_ens_run_state(ens, run, st) bind(C, name='afid_ens_run_state')
+            import; type(c_ptr), value :: ens; integer(c_int32_t), value :: run; type(afid_run_state), intent(out) :: st
+        end function
+        integer(c_int) function afid_ens_autotune(ens, steps, best, log, maxlog, nlog) bind(C, name='afid_ens_autotune')
+            import; type(c_ptr), value :: ens; integer(c_int32_t), value :: steps, maxlog
+            type(afid_ens_tuning), intent(out) :: best; type(afid_ens_tuning), intent(out) :: log(*)
+            integer(c_int32_t), intent(out) :: nlog
         end function
     end interface
 

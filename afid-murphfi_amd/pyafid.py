@@ -44,7 +44,7 @@ EXPORTS = [
 ENS_EXPORTS = [
     "afid_ens_init", "afid_ens_finalize", "afid_ens_last_error", "afid_ens_device_bytes",
     "afid_ens_upload", "afid_ens_download", "afid_ens_create_initial_conditions", "afid_ens_reset",
-    "afid_ens_step", "afid_ens_run_state", "afid_ens_global_quantities",
+    "afid_ens_step", "afid_ens_step_timed", "afid_ens_run_state", "afid_ens_global_quantities",
     "afid_ens_autotune", "afid_ens_set_tuning", "afid_ens_get_tuning", "afid_ens_synchronize",
 ]
 
@@ -177,6 +177,7 @@ def lib():
         L.afid_ens_create_initial_conditions.argtypes = [vp, C.c_uint64, C.c_double]
         L.afid_ens_reset.argtypes = [vp]
         L.afid_ens_step.argtypes = [vp, C.c_int32, C.c_int32]
+        L.afid_ens_step_timed.argtypes = [vp, C.c_int32, C.c_int32, dp]
         L.afid_ens_run_state.argtypes = [vp, C.c_int32, C.POINTER(RunState)]
         L.afid_ens_global_quantities.argtypes = [vp, C.c_int32, dp]
         L.afid_ens_autotune.argtypes = [vp, C.c_int32, C.POINTER(EnsTuning), C.POINTER(EnsTuning), C.c_int32,
@@ -386,6 +387,12 @@ class Ensemble:
 
     def step(self, nsteps=1, graph=True):
         self._check(self.L.afid_ens_step(self.h, nsteps, 1 if graph else 0), "afid_ens_step")
+
+    def step_timed(self, nsteps=1, graph=True):
+        """Steps bracketed by HIP events on the ensemble's stream; returns ms."""
+        ms = C.c_double()
+        self._check(self.L.afid_ens_step_timed(self.h, nsteps, 1 if graph else 0, C.byref(ms)), "afid_ens_step_timed")
+        return ms.value
 
     def synchronize(self):
         self._check(self.L.afid_ens_synchronize(self.h), "afid_ens_synchronize")

@@ -50,6 +50,153 @@ WORKLOADS = {
         pf_a=1.0, pf_s=1.0, pf_tm=0.0, pf_eps=1.5 / 512, pf_r0=0.15, dt=1e-4),
 }
 DEFAULT = "rbc_multires_384x768x768_768x1536x1536"
+ENSEMBLE = "ensemble_2d_256"
+
+
+def ensemble_runs(n=256, seed0=3000):
+    """The 2-D RBC ensemble: run r draws its grid and physics from seed
+    seed0 + r: nx in {64, 96, 128} wall-normal cells (cosine-clustered), nz in
+    {128, 256, 512} periodic cells, aspect ratio 2 (Lz = 2), Ra log-uniform in
+    [1e6, 1e9], Pr in {1, 7}; IDTV = 1, CFLMAX = 0.5, DT = 1e-4, DTMAX =
+    1e-2."""
+    cfgs, ctls = [], []
+    for r in range(n):
+        g = np.random.default_rng(seed0 + r)
+        nxm = int(g.choice([64, 96, 128]))
+        nzm = int(g.choice([128, 256, 512]))
+        ra = float(10.0 ** g.uniform(6.0, 9.0))
+        pr = float(g.choice([1.0, 7.0]))
+        cfgs.append(dict(nxm=nxm, nym=1, nzm=nzm, nsst=3, alx3=1.0, ylen=1.0, zlen=2.0, istr3=8,
+                         rayt=ra, prat=pr, active_t=1, gaxis=1))
+        ctls.append(dict(variable_dt=1, dt=1e-4, resid=1e-3, limit_cfl=0.5, dtmin=1e-8, dtmax=1e-2, ntst=1 << 30))
+    return cfgs, ctls
+
+
+def ensemble_bytes_per_cell():
+    """Algorithmic FP64 bytes per cell and substep of the ensemble kernel:
+    columns read u, w, T, p and the 3 explicit terms and write u, w, T and
+    the terms (13); divergence reads u, w and writes the half spectrum (3);
+    the mode solves read and write it plus 3 factors per mode row (3.5);
+    the inverse reads it and writes dph (2); the projection reads dph, u, w,
+    p and writes u, w, p (7); the step's CFL / divergence check reads u, w
+    once per step (2/3)."""
+    return 8.0 * (13 + 3 + 3.5 + 2 + 7 + 2.0 / 3.0)
+
+
+def _ensemble_cpu_worker(job):
+    """Advance a subset of the ensemble's runs on one core (the oracle with
+    one OpenMP thread) for a fixed number of steps; returns (cell-steps, s)."""
+    import pyoracle
+    from helpers import uniform
+    runs, cfgs, ctls, steps = job
+    pyoracle.set_threads(1)
+    orcs, dts = [], []
+    for r in runs:
+        c = cfgs[r]
+        o = pyoracle.Oracle(c)
+        nxm = c["nxm"]
+        xm = o.grid("xm")[:nxm]
+        o.field("tempbp")[...] = 1.0
+        o.field("temptp")[...] = 0.0
+        T = o.field("temp")
+        amp = np.where(np.abs(xm - 0.5) + 1e-3 > 0.5, 0.5 - np.abs(xm - 0.5), 1e-3)
+        T[..., :nxm] = 1.0 - xm + amp * (2 * uniform(3000 + r, T.shape)[..., :nxm] - 1)
+        orcs.append(o)
+        dts.append(ctls[r]["dt"])
+    t0 = time.perf_counter()
+    for s in range(steps):
+        for q, o in enumerate(orcs):
+            k = ctls[runs[q]]
+            if s > 1:          # main.F90:287-297
+                cfl = max(o.calc_max_cfl()[0], 1e-8)
+                dts[q] = min(k["limit_cfl"] / cfl, k["dtmax"])
+            o.time_marcher(dts[q])
+    el = time.perf_counter() - t0
+    n = sum(cfgs[r]["nxm"] * cfgs[r]["nzm"] for r in runs) * steps
+    for o in orcs:
+        o.close()
+    return n, el
+
+
+def ensemble_cpu_baseline(cfgs, ctls, args):
+    """The FP64 CPU oracle on every run of the ensemble, the runs dealt
+    largest-first over one worker process per core (the ensemble's own
+    parallelism on a CPU), each with the same CFL control; throughput over
+    the slowest worker.  Runs before the GPU is touched."""
+    import multiprocessing as mp
+    workers = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16))
+    order = sorted(range(len(cfgs)), key=lambda r: -cfgs[r]["nxm"] * cfgs[r]["nzm"])
+    load = [0] * workers
+    parts = [[] for _ in range(workers)]
+    for r in order:
+        w = load.index(min(load))
+        parts[w].append(r)
+        load[w] += cfgs[r]["nxm"] * cfgs[r]["nzm"]
+    steps = args.cpu_steps
+    with mp.get_context("fork").Pool(workers) as pool:
+        res = pool.map(_ensemble_cpu_worker, [(p, cfgs, ctls, steps) for p in parts if p])
+    n = sum(a for a, _ in res)
+    el = max(b for _, b in res)
+    ncell = sum(c["nxm"] * c["nzm"] for c in cfgs)
+    return {"value": round(n / el / 1e6, 3), "unit": "Mcell-steps/s", "cores": workers, "kind": "port",
+            "sample": (f"FP64 CPU oracle (C), all {len(cfgs)} runs ({ncell} cells) dealt over {workers} single-thread "
+                       f"worker processes, {steps} RK3 steps each with the same CFL control; slowest worker {el:.1f} s")}
+
+
+def run_ensemble(args):
+    import pyafid
+    cfgs, ctls = ensemble_runs()
+    ncell = sum(c["nxm"] * c["nzm"] for c in cfgs)
+    steps = args.steps if args.steps is not None else 1000
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (before the GPU is initialised)")
+        try:
+            cpu = ensemble_cpu_baseline(cfgs, ctls, args)
+        except Exception as e:    # reported, never replaces the GPU number
+            cpu = {"error": str(e)}
+    ens = pyafid.Ensemble(cfgs, ctls)
+    for r, c in enumerate(cfgs):
+        ens.upload(r, "tempbp", np.ones((c["nzm"], 1, 1)))
+        ens.upload(r, "temptp", np.zeros((c["nzm"], 1, 1)))
+    ens.create_initial_conditions(3000, 1e-3)
+    log(f"ensemble: {len(cfgs)} runs, {ncell} cells, device memory {ens.device_bytes() / 2**30:.2f} GiB")
+    best, cands = ens.autotune(steps=5)
+    log(f"ensemble: autotuner picked {best} from {len(cands)} launch shapes")
+    ens.create_initial_conditions(3000, 1e-3)     # the timed run starts from step 0
+    ens.step(args.warmup, graph=True)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ms = ens.step_timed(steps, graph=True)
+    wall = time.perf_counter() - t0
+    states = [ens.run_state(r) for r in range(len(cfgs))]
+    nus = [ens.global_quantities(r)["nu"] for r in range(len(cfgs))]
+    ens.close()
+    running = sum(1 for s in states if s["status"] == 0)
+    value = ncell * steps / (ms * 1e-3) / 1e6
+    nsub = 3 * steps
+    avg_launch_ms = ms / nsub
+    per_launch = ensemble_bytes_per_cell() * ncell
+    achieved = per_launch / (avg_launch_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(ms / steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": ENSEMBLE, "runs": len(cfgs), "cells": ncell,
+                   "nx": [64, 96, 128], "nz": [128, 256, 512], "aspect": 2, "Ra": [1e6, 1e9], "Pr": [1, 7],
+                   "cflmax": 0.5, "dt0": 1e-4, "dtmax": 1e-2, "graph": True, "tuning": best},
+        "roofline": {"bound": "hbm", "kernel": "k_ens_substep", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": int(per_launch)},
+        "host_wall_s": round(wall, 4),
+        "runs_running": running,
+        "exit_codes": sorted({s["status"] for s in states}),
+        "time_reached": [round(min(s["time"] for s in states), 5), round(max(s["time"] for s in states), 5)],
+        "nu_vol": [round(min(nus), 4), round(max(nus), 4)],
+        "autotune_candidates": cands,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
 
 
 def log(msg):
@@ -256,9 +403,9 @@ def roofline(wl, res, form):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None, help="timed RK3 steps (10; 1000 for the ensemble)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [ENSEMBLE])
     ap.add_argument("--form", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -271,6 +418,10 @@ def main():
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == ENSEMBLE:
+        return run_ensemble(args)
+    if args.steps is None:
+        args.steps = 10
     wl = dict(WORKLOADS[args.workload])
     dt = wl.pop("dt")
     ncell = cells(wl)

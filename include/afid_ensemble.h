@@ -87,6 +87,10 @@ int afid_ens_reset(afid_ens *e);
  * graph == 0 launches them one by one.  Asynchronous. */
 int afid_ens_step(afid_ens *e, int32_t nsteps, int32_t graph);
 
+/* afid_ens_step bracketed by HIP events on the ensemble's stream; waits for
+ * the steps and returns their elapsed time in *ms. */
+int afid_ens_step_timed(afid_ens *e, int32_t nsteps, int32_t graph, double *ms);
+
 int afid_ens_run_state(afid_ens *e, int32_t run, afid_run_state *out);
 
 /* GlobalQuantities of run r (afid_global_quantities layout: Nu_vol, KE, vrms, Tmean). */
