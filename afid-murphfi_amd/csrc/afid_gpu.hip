@@ -75,6 +75,8 @@ struct afid_ctx {
     const int *d_xp_src = nullptr;
     const double *d_xp_w = nullptr;
     double *phi_alt = nullptr;
+    float *phi_alt_f = nullptr;                 // second phi buffer of the mixed form
+    const float *f_xp_w = nullptr;              // FP32 cxphic weights (mixed form)
     int phic_xr = 0;
     size_t lds_phic = 0;
     // arithmetic form
@@ -217,6 +219,7 @@ static KP make_kp(const afid_ctx *c) {
     p.lxaf = c->f_lxa;
     p.phi = c->f[AFID_PHI]; p.ruphi = c->f[AFID_RUPHI]; p.hphi = c->f[AFID_HPHI]; p.tempr = c->f[AFID_TEMPR];
     p.phic = c->f[AFID_PHIC]; p.qlat = c->f[AFID_HRO];
+    p.phif = c->g[AFID_PHI]; p.ruphif = c->g[AFID_RUPHI]; p.temprf = c->g[AFID_TEMPR]; p.rhsrf = c->g[AFID_RHSR];
     p.pf_d = c->pf_d; p.pf_b = c->pf_b; p.pf_a = c->c.pf_a; p.pf_tm = c->c.pf_tm; p.pf_s = c->c.pf_s; p.heta = c->heta;
     p.pf = c->pf ? 1 : 0;
     p.ap3spkr = c->d_ap3spkr; p.ac3spkr = c->d_ac3spkr; p.am3spkr = c->d_am3spkr;
@@ -402,7 +405,6 @@ static int setup(afid_ctx *c) {
     c->pf = g.phasefield != 0;
     if (c->pf && !g.multires) return set_err(c, AFID_E_ARG, "the phase field requires the refined grid (MULTIRES=1)");
     if (c->pf && c->sal) return set_err(c, AFID_E_ARG, "phase field together with salinity is not supported");
-    if (c->pf && c->mixed) return set_err(c, AFID_E_ARG, "the phase field runs in the FP64 form only");
     if (c->pf && !(g.pf_a != 0.0 && g.pf_s != 0.0)) return set_err(c, AFID_E_ARG, "phase field needs pf_A and pf_S nonzero");
     c->nxm = g.nxm; c->nx = g.nxm + 1; c->ny = g.nym; c->nz = g.nzm;
     c->ldx = (c->nx + 7) / 8 * 8;
@@ -520,7 +522,8 @@ static int setup(afid_ctx *c) {
     const size_t nb = size_t(c->nz) * c->ny * c->ldx;
     for (int f = 0; f < AFID_NFIELDS; ++f) {
         if (is_refined(f) || f == AFID_RHS || is_plane(f) || f == AFID_PHIC) continue;
-        if (c->mixed && (f == AFID_QCAP || f == AFID_DQ || f == AFID_HRO)) continue;
+        // (hro carries the latent heat of the phase field in both forms)
+        if (c->mixed && (f == AFID_QCAP || f == AFID_DQ || (f == AFID_HRO && !c->pf))) continue;
         if (c->mixed && ((f >= AFID_RUX && f <= AFID_RUTEMP) || f == AFID_SALC)) {
             if (f == AFID_SALC && !c->sal) continue;
             if ((rc = dalloc(c, reinterpret_cast<void **>(&c->g[f]), sizeof(float) * nb))) return rc;
@@ -745,10 +748,18 @@ static int setup(afid_ctx *c) {
 
     const size_t nr = size_t(nzr) * nyr * c->ldxr;
     if (c->mixed) {
-        // FP32 refined fields; hsal/rhsr belong to the per-routine calls only
-        for (int f : {AFID_SAL, AFID_RUSAL, AFID_VXR, AFID_VYR, AFID_VZR})
-            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->g[f]), sizeof(float) * nr))) return rc;
-        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->sal_alt_f), sizeof(float) * nr))) return rc;
+        // FP32 refined fields; hsal/hphi and the salinity rhsr belong to the
+        // per-routine calls only (the phase field keeps d phi in rhsr)
+        if (c->sal) {
+            for (int f : {AFID_SAL, AFID_RUSAL, AFID_VXR, AFID_VYR, AFID_VZR})
+                if ((rc = dalloc(c, reinterpret_cast<void **>(&c->g[f]), sizeof(float) * nr))) return rc;
+            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->sal_alt_f), sizeof(float) * nr))) return rc;
+        } else {
+            for (int f : {AFID_PHI, AFID_RUPHI, AFID_TEMPR, AFID_RHSR})
+                if ((rc = dalloc(c, reinterpret_cast<void **>(&c->g[f]), sizeof(float) * nr))) return rc;
+            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->phi_alt_f), sizeof(float) * nr))) return rc;
+            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[AFID_PHIC]), sizeof(double) * size_t(c->nz) * c->ny * c->ldx))) return rc;
+        }
         auto upf = [&](const std::vector<double> &v, const float **d) {
             std::vector<float> w(v.begin(), v.end());
             return rc ? rc : (rc = upload_vec(c, w, d));
@@ -756,6 +767,7 @@ static int setup(afid_ctx *c) {
         upf(sx.w, &c->f_xs_w); upf(sy.w, &c->f_ys_w); upf(sz.w, &c->f_zs_w);
         upf(bx.w, &c->f_xb_w); upf(by.w, &c->f_yb_w); upf(bz.w, &c->f_zb_w);
         upf(lxa, &c->f_lxa);
+        if (c->pf) upf(bp.w, &c->f_xp_w);
         if (rc) return rc;
     } else if (c->sal) {
         for (int f = AFID_SAL; f <= AFID_VZR; ++f)
@@ -1179,26 +1191,30 @@ int afid_interp_sal_multigrid(afid_ctx *c) {
 
 #define NEED_PF(c) if (!(c)->pf) return set_err((c), AFID_E_STATE, "the phase field is not enabled");
 
-int afid_explicit_phase(afid_ctx *c) { NEED_SUBSTEP(c) NEED_PF(c) return launch_point(c, k_explicit_phase, true, "k_explicit_phase"); }
-int afid_add_volume_penalty(afid_ctx *c) { NEED_SUBSTEP(c) NEED_PF(c) return launch_point(c, k_volume_penalty, false, "k_volume_penalty"); }
+int afid_explicit_phase(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) NEED_PF(c) return launch_point(c, k_explicit_phase, true, "k_explicit_phase"); }
+int afid_add_volume_penalty(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) NEED_PF(c) return launch_point(c, k_volume_penalty, false, "k_volume_penalty"); }
 int afid_implicit_phase(afid_ctx *c) {
-    NEED_SUBSTEP(c) NEED_PF(c)
+    NEED_SUBSTEP(c) NEED_FP64(c) NEED_PF(c)
     int rc = launch_point(c, k_rhs_phase, true, "k_rhs_phase");
     if (!rc) rc = run_tri(c, MAT_P, c->f[AFID_RHSR], c->f[AFID_PHI]);   // rhsr keeps the increment
     return rc;
 }
 static int latent_heat(afid_ctx *c, double *out, int accumulate) {
     CoarseJob jb{};
-    jb.mode = CM_LAT; jb.src = c->f[AFID_RHSR]; jb.out = out; jb.x_src = c->d_xp_src; jb.x_w = c->d_xp_w;
+    jb.mode = CM_LAT; jb.out = out; jb.x_src = c->d_xp_src;
+    if (c->mixed) { jb.src = c->g[AFID_RHSR]; jb.x_w = c->f_xp_w; }
+    else { jb.src = c->f[AFID_RHSR]; jb.x_w = c->d_xp_w; }
     jb.scale = c->c.pf_s / (c->al * c->dt);          // pf_S d(phi)/(al dt), phasefield.F90:666,701
     jb.accumulate = accumulate;
     return to_coarse(c, jb, c->phic_xr, c->lds_phic, "pf_latent_heat");
 }
-int afid_add_latent_heat(afid_ctx *c) { NEED_SUBSTEP(c) NEED_PF(c) return latent_heat(c, c->f[AFID_HRO], 1); }
+int afid_add_latent_heat(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) NEED_PF(c) return latent_heat(c, c->f[AFID_HRO], 1); }
 int afid_interp_phi_multigrid(afid_ctx *c) {
     NEED_PF(c)
     CoarseJob jb{};
-    jb.mode = CM_PHI; jb.src = c->f[AFID_PHI]; jb.out = c->f[AFID_PHIC]; jb.x_src = c->d_xp_src; jb.x_w = c->d_xp_w;
+    jb.mode = CM_PHI; jb.out = c->f[AFID_PHIC]; jb.x_src = c->d_xp_src;
+    if (c->mixed) { jb.src = c->g[AFID_PHI]; jb.x_w = c->f_xp_w; }
+    else { jb.src = c->f[AFID_PHI]; jb.x_w = c->d_xp_w; }
     jb.scale = 1.0;
     return to_coarse(c, jb, c->phic_xr, c->lds_phic, "pf_interp_phi");
 }
@@ -1206,8 +1222,9 @@ int afid_interp_temp_multigrid(afid_ctx *c) {
     NEED_PF(c)
     KP p = make_kp(c);
     tstart(c, "pf_interp_temp");
-    hipLaunchKernelGGL(k_interp_temp_tile, dim3(unsigned((c->nyr + VXR_TJ - 1) / VXR_TJ), unsigned((c->nzr + VXR_TI - 1) / VXR_TI)),
-                       dim3(256), c->lds_vxr, c->stream, p, c->vxr_xc, c->vxr_cj, c->vxr_ci);
+    const dim3 gr(unsigned((c->nyr + VXR_TJ - 1) / VXR_TJ), unsigned((c->nzr + VXR_TI - 1) / VXR_TI));
+    if (c->mixed) hipLaunchKernelGGL(k_interp_temp_tile<float>, gr, dim3(256), c->lds_vxr, c->stream, p, c->vxr_xc, c->vxr_cj, c->vxr_ci);
+    else hipLaunchKernelGGL(k_interp_temp_tile<double>, gr, dim3(256), c->lds_vxr, c->stream, p, c->vxr_xc, c->vxr_cj, c->vxr_ci);
     tstop(c);
     return check_launch(c, "k_interp_temp_tile");
 }
@@ -1219,7 +1236,9 @@ template <int CH>
 static void launch_column_base(afid_ctx *c, const KP &p, long long nblk) {
     BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
     const dim3 gr(unsigned((nblk + 7) / 8 * 8));
-    if (c->mixed)
+    if (c->mixed && c->pf)
+        hipLaunchKernelGGL((k_column_base<CH, float, true>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
+    else if (c->mixed)
         hipLaunchKernelGGL((k_column_base<CH, float, false>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
     else if (c->pf)
         hipLaunchKernelGGL((k_column_base<CH, double, true>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
@@ -1273,7 +1292,9 @@ static int column_sal(afid_ctx *c) {
 
 template <int CH>
 static void launch_column_phi(afid_ctx *c, const KP &p, long long nblk) {
-    hipLaunchKernelGGL(k_column_phi<CH>, dim3(unsigned((nblk + 7) / 8 * 8)), dim3(256), 0, c->stream, p, factors(c, MAT_P), c->phi_alt);
+    const dim3 gr(unsigned((nblk + 7) / 8 * 8));
+    if (c->mixed) hipLaunchKernelGGL((k_column_phi<CH, float>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_P), c->phi_alt_f);
+    else hipLaunchKernelGGL((k_column_phi<CH, double>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_P), c->phi_alt);
 }
 static int column_phi(afid_ctx *c) {
     KP p = make_kp(c);
@@ -1289,7 +1310,8 @@ static int column_phi(afid_ctx *c) {
     }
     int rc = check_launch(c, "k_column_phi");
     if (rc) return rc;
-    std::swap(c->f[AFID_PHI], c->phi_alt);
+    if (c->mixed) std::swap(c->g[AFID_PHI], c->phi_alt_f);
+    else std::swap(c->f[AFID_PHI], c->phi_alt);
     return AFID_OK;
 }
 
@@ -1358,11 +1380,18 @@ int afid_create_initial_conditions(afid_ctx *c, uint64_t seed, double eps) {
         // tempr and phic as at start-up (main.F90:210-212)
         KP p = make_kp(c);
         const size_t nr = size_t(c->nzr) * c->nyr * c->ldxr;
-        for (int f : {AFID_RUPHI, AFID_HPHI, AFID_RHSR}) HIPCHK(c, hipMemsetAsync(c->f[f], 0, sizeof(double) * nr, c->stream));
+        for (int f : {AFID_RUPHI, AFID_HPHI, AFID_RHSR}) {
+            if (c->g[f]) HIPCHK(c, hipMemsetAsync(c->g[f], 0, sizeof(float) * nr, c->stream));
+            else if (c->f[f]) HIPCHK(c, hipMemsetAsync(c->f[f], 0, sizeof(double) * nr, c->stream));
+        }
         hipLaunchKernelGGL(k_init_temp_sphere, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, c->d_xm, c->pf_r0,
                            c->c.alx3, c->c.ylen, c->c.zlen);
-        hipLaunchKernelGGL(k_init_phi_sphere, point_grid(c->nxr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, c->d_xmr, c->pf_r0,
-                           c->pf_eps, c->c.alx3, c->c.ylen, c->c.zlen);
+        if (c->mixed)
+            hipLaunchKernelGGL(k_init_phi_sphere<float>, point_grid(c->nxr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, c->d_xmr,
+                               c->pf_r0, c->pf_eps, c->c.alx3, c->c.ylen, c->c.zlen);
+        else
+            hipLaunchKernelGGL(k_init_phi_sphere<double>, point_grid(c->nxr, c->nyr, c->nzr), kPointBlock, 0, c->stream, p, c->d_xmr,
+                               c->pf_r0, c->pf_eps, c->c.alx3, c->c.ylen, c->c.zlen);
         if ((rc = check_launch(c, "k_init_phi_sphere"))) return rc;
         if ((rc = afid_interp_temp_multigrid(c))) return rc;
         return afid_interp_phi_multigrid(c);

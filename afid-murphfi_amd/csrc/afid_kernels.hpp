@@ -585,21 +585,31 @@ __global__ void __launch_bounds__(256) k_project(KP p, int do_vel, int do_pr, Ga
     const int km = k == 0 ? 0 : k - 1, kp = k == p.nxm - 1 ? k : k + 1;
     // z walked with the grid's z stride: a full grid (one plane each) or, for
     // a gated launch, a short grid that costs little when the gate is closed
+    // every load of a cell ahead of its stores (the fields may alias as far
+    // as the compiler knows; interleaved stores would serialise the loads)
     for (int i = blockIdx.z; i < p.nz; i += gridDim.z) {
         const int im = wrapm(i, p.nz), ip = wrapp(i, p.nz);
         const long long c = bidx(p, k, j, i);
-        const double dc = d[c];
+        const double dc = d[c], dkm = d[bidx(p, km, j, i)], djm = d[bidx(p, k, jm, i)], dim = d[bidx(p, k, j, im)];
         if (do_vel) {
             const double adt = p.al * p.dt;
-            p.vx[c] -= (dc - d[bidx(p, km, j, i)]) * (adt * p.udx3c[k]);
-            p.vy[c] -= (dc - d[bidx(p, k, jm, i)]) * (adt * p.dy);
-            p.vz[c] -= (dc - d[bidx(p, k, j, im)]) * (adt * p.dz);
-        }
-        if (do_pr) {
+            const double ux = p.vx[c], uy = p.vy[c], uz = p.vz[c];
+            const double nux = ux - (dc - dkm) * (adt * p.udx3c[k]);
+            const double nuy = uy - (dc - djm) * (adt * p.dy);
+            const double nuz = uz - (dc - dim) * (adt * p.dz);
+            if (do_pr) {
+                const double be = p.al * p.beta, prc = p.pr[c];
+                const double npr = prc + (dc - be * ((d[bidx(p, k, j, ip)] - 2.0 * dc + dim) * p.dzq +
+                                                     (d[bidx(p, k, jp, i)] - 2.0 * dc + djm) * p.dyq +
+                                                     (d[bidx(p, kp, j, i)] * p.apphk[k] + dc * p.acphk[k] + dkm * p.amphk[k])));
+                p.pr[c] = npr;
+            }
+            p.vx[c] = nux; p.vy[c] = nuy; p.vz[c] = nuz;
+        } else if (do_pr) {
             const double be = p.al * p.beta;
-            p.pr[c] += dc - be * ((d[bidx(p, k, j, ip)] - 2.0 * dc + d[bidx(p, k, j, im)]) * p.dzq +
-                                  (d[bidx(p, k, jp, i)] - 2.0 * dc + d[bidx(p, k, jm, i)]) * p.dyq +
-                                  (d[bidx(p, kp, j, i)] * p.apphk[k] + dc * p.acphk[k] + d[bidx(p, km, j, i)] * p.amphk[k]));
+            p.pr[c] += dc - be * ((d[bidx(p, k, j, ip)] - 2.0 * dc + dim) * p.dzq +
+                                  (d[bidx(p, k, jp, i)] - 2.0 * dc + djm) * p.dyq +
+                                  (d[bidx(p, kp, j, i)] * p.apphk[k] + dc * p.acphk[k] + dkm * p.amphk[k]));
         }
     }
 }
@@ -619,24 +629,32 @@ __global__ void __launch_bounds__(256) k_project_res(KP p, double *vx_out, doubl
         const double *d = p.dph;
         const long long c = bidx(p, k, j, i);
         const long long cjm = bidx(p, k, jm, i), cjp = bidx(p, k, jp, i), cim = bidx(p, k, j, im), cip = bidx(p, k, j, ip);
-        const double dc = d[c];
+        // every load ahead of the stores (the outputs may alias the inputs
+        // as far as the compiler knows)
+        const double dc = d[c], dkm = d[bidx(p, km, j, i)], dkp = d[bidx(p, kp, j, i)];
+        const double djm = d[cjm], djp = d[cjp], dim = d[cim], dip = d[cip];
+        const double vxc = p.vx[c], vyc = p.vy[c], vzc = p.vz[c];
+        const double vx1 = p.vx[c + 1], vy1 = p.vy[cjp], vz1 = p.vz[cip];
+        const double d1 = d[c + 1], prc = p.pr[c];
         const double adt = p.al * p.dt;
         // corrected faces of this cell (as k_project)
-        const double ux = p.vx[c] - (dc - d[bidx(p, km, j, i)]) * (adt * p.udx3c[k]);
-        const double uy = p.vy[c] - (dc - d[cjm]) * (adt * p.dy);
-        const double uz = p.vz[c] - (dc - d[cim]) * (adt * p.dz);
-        vx_out[c] = ux; vy_out[c] = uy; vz_out[c] = uz;
-        if (k == p.nxm - 1) {      // index nx: wall face of vx (not corrected), unused for vy, vz
-            vx_out[c + 1] = p.vx[c + 1]; vy_out[c + 1] = p.vy[c + 1]; vz_out[c + 1] = p.vz[c + 1];
-        }
+        const double ux = vxc - (dc - dkm) * (adt * p.udx3c[k]);
+        const double uy = vyc - (dc - djm) * (adt * p.dy);
+        const double uz = vzc - (dc - dim) * (adt * p.dz);
         // corrected +1 faces (owned by the neighbouring cells)
-        const double ux1 = k + 1 <= p.nxm - 1 ? p.vx[c + 1] - (d[c + 1] - dc) * (adt * p.udx3c[k + 1]) : p.vx[c + 1];
-        const double uy1 = p.vy[cjp] - (d[cjp] - dc) * (adt * p.dy);
-        const double uz1 = p.vz[cip] - (d[cip] - dc) * (adt * p.dz);
+        const double ux1 = k + 1 <= p.nxm - 1 ? vx1 - (d1 - dc) * (adt * p.udx3c[k + 1]) : vx1;
+        const double uy1 = vy1 - (djp - dc) * (adt * p.dy);
+        const double uz1 = vz1 - (dip - dc) * (adt * p.dz);
         dvm = fmax(dvm, fabs((uz1 - uz) * p.dz + (uy1 - uy) * p.dy + (ux1 - ux) * p.udx3m[k]));
         const double be = p.al * p.beta;
-        p.pr[c] += dc - be * ((d[cip] - 2.0 * dc + d[cim]) * p.dzq + (d[cjp] - 2.0 * dc + d[cjm]) * p.dyq +
-                              (d[bidx(p, kp, j, i)] * p.apphk[k] + dc * p.acphk[k] + d[bidx(p, km, j, i)] * p.amphk[k]));
+        const double npr = prc + (dc - be * ((dip - 2.0 * dc + dim) * p.dzq + (djp - 2.0 * dc + djm) * p.dyq +
+                                            (dkp * p.apphk[k] + dc * p.acphk[k] + dkm * p.amphk[k])));
+        if (k == p.nxm - 1) {      // index nx: wall face of vx (not corrected), unused for vy, vz
+            const double wy = p.vy[c + 1], wz = p.vz[c + 1];
+            vx_out[c + 1] = vx1; vy_out[c + 1] = wy; vz_out[c + 1] = wz;
+        }
+        vx_out[c] = ux; vy_out[c] = uy; vz_out[c] = uz;
+        p.pr[c] = npr;
     }
     dvm = block_max(dvm);
     if (threadIdx.x == 0 && threadIdx.y == 0)

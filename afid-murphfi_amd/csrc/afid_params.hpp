@@ -72,6 +72,7 @@ struct KP {
     // phase field (phasefield.F90): refined phi, its RK arrays and the
     // refined temperature; coarse phic and the latent-heat source qlat
     double *phi, *ruphi, *hphi, *tempr, *phic, *qlat;
+    float *phif, *ruphif, *temprf, *rhsrf;      // FP32 storage of the refined phase-field arrays (mixed form)
     double pf_d, pf_b, pf_a, pf_tm, pf_s, heta;  // pf_D, pf_D/eps^2, A, T_m, S_t, 0.5/eta
     int pf;
     const double *ap3spkr, *ac3spkr, *am3spkr;   // zero-gradient x operator of phi (refined)

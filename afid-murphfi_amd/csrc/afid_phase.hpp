@@ -13,12 +13,12 @@ namespace afid {
 
 // ExplicitPhase, phasefield.F90:345-383, row k of a refined column: lateral
 // diffusion D (phi_yy + phi_zz) minus the Allen-Cahn source
-// pf_B phi (1 - phi) (1 - 2 phi + A (T_r - T_m))
-__device__ __forceinline__ double ex_phase_val(const KP &p, double pc, double pym, double pyp, double pzm, double pzp,
-                                               double tr) {
-    const double dyyp = (pyp - 2.0 * pc + pym) * (p.pf_d * p.dyqr);
-    const double dzzp = (pzp - 2.0 * pc + pzm) * (p.pf_d * p.dzqr);
-    const double nlphi = p.pf_b * pc * (1.0 - pc) * (1.0 - 2.0 * pc + p.pf_a * (tr - p.pf_tm));
+// pf_B phi (1 - phi) (1 - 2 phi + A (T_r - T_m)), in arithmetic T
+template <typename T>
+__device__ __forceinline__ T ex_phase_val(const KP &p, T pc, T pym, T pyp, T pzm, T pzp, T tr) {
+    const T dyyp = (pyp - T(2.0) * pc + pym) * T(p.pf_d * p.dyqr);
+    const T dzzp = (pzp - T(2.0) * pc + pzm) * T(p.pf_d * p.dzqr);
+    const T nlphi = T(p.pf_b) * pc * (T(1.0) - pc) * (T(1.0) - T(2.0) * pc + T(p.pf_a) * (tr - T(p.pf_tm)));
     return dyyp + dzzp - nlphi;
 }
 // ImplicitPhase, phasefield.F90:386-419: zero-gradient x second derivative
@@ -35,7 +35,7 @@ __device__ __forceinline__ double rhs_phase_val(const KP &p, unsigned k, double 
 __global__ void __launch_bounds__(256) k_explicit_phase(KP p) {
     AFID_POINT_PROLOGUE(p.nxmr, p.nyr, p.nzr)
     const double *f = p.phi;
-    p.hphi[ridx(p, k, j, i)] = ex_phase_val(p, f[ridx(p, k, j, i)], f[ridx(p, k, jm, i)], f[ridx(p, k, jp, i)],
+    p.hphi[ridx(p, k, j, i)] = ex_phase_val<double>(p, f[ridx(p, k, j, i)], f[ridx(p, k, jm, i)], f[ridx(p, k, jp, i)],
                                             f[ridx(p, k, j, im)], f[ridx(p, k, j, ip)], p.tempr[ridx(p, k, j, i)]);
 }
 __global__ void __launch_bounds__(256) k_rhs_phase(KP p) {
@@ -61,9 +61,11 @@ __global__ void __launch_bounds__(256) k_volume_penalty(KP p) {
 // 345-466) for one refined column per wavefront, rows in pairs as in
 // k_column_sal.  Writes the new phi to phi_out, the explicit term to ruphi
 // and the increment phi(l+1) - phi(l) to rhsr (read by AddLatentHeat).
-template <int CH>
-__global__ void __launch_bounds__(256) k_column_phi(KP p, TriFactors f, double *phi_out) {
-    using V = double2;
+// RT: storage and explicit-term arithmetic of the refined arrays; the
+// implicit RHS, the solve and the update run in FP64.
+template <int CH, typename RT>
+__global__ void __launch_bounds__(256) k_column_phi(KP p, TriFactors f, RT *phi_out) {
+    using V = typename Pair<RT>::T;
     constexpr int M2 = (64 * CH + 127) / 128;
     __shared__ double lds[4][lds_col_size<CH>()];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -74,30 +76,31 @@ __global__ void __launch_bounds__(256) k_column_phi(KP p, TriFactors f, double *
     const int jm = wrapm(j, p.nyr), jp = wrapp(j, p.nyr), im = wrapm(i, p.nzr), ip = wrapp(i, p.nzr);
     auto col = [&](int jj, int ii) { return (long long)ii * p.planer + (long long)jj * p.ldxr; };
     const long long c0 = col(j, i);
-    const double *ph = p.phi + c0, *pym = p.phi + col(jm, i), *pyp = p.phi + col(jp, i);
-    const double *pzm = p.phi + col(j, im), *pzp = p.phi + col(j, ip), *tr = p.tempr + c0;
-    double *ru = p.ruphi + c0;
+    const RT *phi = sel<RT>(p.phi, p.phif);
+    const RT *ph = phi + c0, *pym = phi + col(jm, i), *pyp = phi + col(jp, i);
+    const RT *pzm = phi + col(j, im), *pzp = phi + col(j, ip), *tr = sel<RT>(p.tempr, p.temprf) + c0;
+    RT *ru = sel<RT>(p.ruphi, p.ruphif) + c0;
     const unsigned nxmr = p.nxmr;
     V own[M2];
 #pragma unroll
     for (int m = 0; m < M2; ++m) {
         const unsigned r0 = 2 * lane + 128 * m, r1 = r0 + 1;
         double rhs0 = 0.0, rhs1 = 0.0;
-        own[m] = V{0.0, 0.0};
+        own[m] = V{RT(0), RT(0)};
         if (live && r0 < nxmr) {
             // r0 even, rows 8-aligned: aligned pairs; r0 + 1 <= nxmr < ldxr
             const V pc = *reinterpret_cast<const V *>(ph + r0);
             const V a = *reinterpret_cast<const V *>(pym + r0), b = *reinterpret_cast<const V *>(pyp + r0);
             const V cz = *reinterpret_cast<const V *>(pzm + r0), d = *reinterpret_cast<const V *>(pzp + r0);
             const V t = *reinterpret_cast<const V *>(tr + r0), u = *reinterpret_cast<const V *>(ru + r0);
-            const double pm = r0 > 0 ? ph[r0 - 1] : 0.0;
-            const double pp2 = r1 + 1 < nxmr ? ph[r1 + 1] : 0.0;
-            const double h0 = ex_phase_val(p, pc.x, a.x, b.x, cz.x, d.x, t.x);
-            rhs0 = rhs_phase_val(p, r0, h0, u.x, pm, pc.x, pc.y);
+            const RT pm = r0 > 0 ? ph[r0 - 1] : RT(0);
+            const RT pp2 = r1 + 1 < nxmr ? ph[r1 + 1] : RT(0);
+            const RT h0 = ex_phase_val<RT>(p, pc.x, a.x, b.x, cz.x, d.x, t.x);
+            rhs0 = rhs_phase_val(p, r0, double(h0), double(u.x), double(pm), double(pc.x), double(pc.y));
             own[m] = pc;
             if (r1 < nxmr) {
-                const double h1 = ex_phase_val(p, pc.y, a.y, b.y, cz.y, d.y, t.y);
-                rhs1 = rhs_phase_val(p, r1, h1, u.y, pc.x, pc.y, pp2);
+                const RT h1 = ex_phase_val<RT>(p, pc.y, a.y, b.y, cz.y, d.y, t.y);
+                rhs1 = rhs_phase_val(p, r1, double(h1), double(u.y), double(pc.x), double(pc.y), double(pp2));
                 *reinterpret_cast<V *>(ru + r0) = V{h0, h1};
             } else {
                 ru[r0] = h0;
@@ -110,7 +113,7 @@ __global__ void __launch_bounds__(256) k_column_phi(KP p, TriFactors f, double *
     column_solve_lds<CH>(buf, f);
     __syncthreads();
     if (!live) return;
-    double *out = phi_out + c0, *dphi = p.rhsr + c0;
+    RT *out = phi_out + c0, *dphi = sel<RT>(p.rhsr, p.rhsrf) + c0;
 #pragma unroll
     for (int m = 0; m < M2; ++m) {
         const unsigned r0 = 2 * lane + 128 * m, r1 = r0 + 1;
@@ -118,11 +121,11 @@ __global__ void __launch_bounds__(256) k_column_phi(KP p, TriFactors f, double *
         const double d0 = buf[lds_row<CH>(r0)];
         if (r1 < nxmr) {
             const double d1 = buf[lds_row<CH>(r1)];
-            *reinterpret_cast<V *>(out + r0) = V{own[m].x + d0, own[m].y + d1};
-            *reinterpret_cast<V *>(dphi + r0) = V{d0, d1};
+            *reinterpret_cast<V *>(out + r0) = V{RT(double(own[m].x) + d0), RT(double(own[m].y) + d1)};
+            *reinterpret_cast<V *>(dphi + r0) = V{RT(d0), RT(d1)};
         } else {
-            out[r0] = own[m].x + d0;
-            dphi[r0] = d0;
+            out[r0] = RT(double(own[m].x) + d0);
+            dphi[r0] = RT(d0);
         }
     }
     if (lane == 0) out[nxmr] = ph[nxmr];        // the unused index nxr is carried over
@@ -178,16 +181,19 @@ __device__ __forceinline__ double temp_ghosted(const KP &p, int xc, int j, int i
     if (xc == p.nx) return (1.0 - 2.0 * p.tfixn) * c[p.nxm - 1] + 2.0 * p.tfixn * p.temptp[pl];
     return 0.0;
 }
+template <typename RT>
 __global__ void __launch_bounds__(256) k_interp_temp_tile(KP p, int xcmax, int cjmax, int cimax) {
     extern __shared__ __align__(16) unsigned char smraw[];
-    double *sm = reinterpret_cast<double *>(smraw);
+    RT *sm = reinterpret_cast<RT *>(smraw);
+    RT *tempr = sel<RT>(p.tempr, p.temprf);
+    const RT *xs_w = sel<RT>(p.xs_w, p.xs_wf), *ys_w = sel<RT>(p.ys_w, p.ys_wf), *zs_w = sel<RT>(p.zs_w, p.zs_wf);
     const int J0 = blockIdx.x * VXR_TJ, I0 = blockIdx.y * VXR_TI;
     const int nj = min(VXR_TJ, p.nyr - J0), ni = min(VXR_TI, p.nzr - I0);
     const int cj0 = p.ys_src[J0] - 1, CJ = p.ys_src[J0 + nj - 1] - 1 + 4 - cj0;
     const int ci0 = p.zs_src[I0] - 1, CI = p.zs_src[I0 + ni - 1] - 1 + 4 - ci0;
-    double *B = sm;                                   // [ci][cj][xc]
-    double *Z = B + xcmax * cjmax * cimax;            // [ti][cj][xc]
-    double *Q = Z + xcmax * cjmax * VXR_TI;           // [ti][tj][xc]
+    RT *B = sm;                                       // [ci][cj][xc]
+    RT *Z = B + xcmax * cjmax * cimax;                // [ti][cj][xc]
+    RT *Q = Z + xcmax * cjmax * VXR_TI;               // [ti][tj][xc]
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int ncol = nj * ni;
     for (int R0 = 0; R0 < p.nxmr; R0 += 64) {
@@ -197,14 +203,14 @@ __global__ void __launch_bounds__(256) k_interp_temp_tile(KP p, int xcmax, int c
         {
             Walk3 w(t, 256, XC, CJ);
             for (int q = t; q < XC * CJ * CI; q += 256, w.next())
-                B[(w.c * cjmax + w.b) * xcmax + w.a] = temp_ghosted(p, x0 + w.a, wrapf(cj0 + w.b, p.ny), wrapf(ci0 + w.c, p.nz));
+                B[(w.c * cjmax + w.b) * xcmax + w.a] = RT(temp_ghosted(p, x0 + w.a, wrapf(cj0 + w.b, p.ny), wrapf(ci0 + w.c, p.nz)));
         }
         __syncthreads();
         Walk3 wz(t, 256, XC, CJ);
         for (int q = t; q < XC * CJ * ni; q += 256, wz.next()) {
             const int xc = wz.a, cj = wz.b, ti = wz.c;
             const int c0 = p.zs_src[I0 + ti] - 1 - ci0;
-            const double *w = p.zs_w + 4 * (I0 + ti), *b = B + (c0 * cjmax + cj) * xcmax + xc;
+            const RT *w = zs_w + 4 * (I0 + ti), *b = B + (c0 * cjmax + cj) * xcmax + xc;
             const int st = cjmax * xcmax;
             Z[(ti * cjmax + cj) * xcmax + xc] = b[0] * w[0] + b[st] * w[1] + b[2 * st] * w[2] + b[3 * st] * w[3];
         }
@@ -213,37 +219,39 @@ __global__ void __launch_bounds__(256) k_interp_temp_tile(KP p, int xcmax, int c
         for (int q = t; q < XC * nj * ni; q += 256, wy.next()) {
             const int xc = wy.a, tj = wy.b, ti = wy.c;
             const int b0 = p.ys_src[J0 + tj] - 1 - cj0;
-            const double *w = p.ys_w + 4 * (J0 + tj), *z = Z + (ti * cjmax + b0) * xcmax + xc;
+            const RT *w = ys_w + 4 * (J0 + tj), *z = Z + (ti * cjmax + b0) * xcmax + xc;
             Q[(ti * VXR_TJ + tj) * xcmax + xc] = z[0] * w[0] + z[xcmax] * w[1] + z[2 * xcmax] * w[2] + z[3 * xcmax] * w[3];
         }
         __syncthreads();
         const int kr = R0 + lane;
         if (kr < R1) {
             const int xa = p.xs_src[kr] - x0;
-            const double *wx = p.xs_w + 4 * kr;
+            const RT *wx = xs_w + 4 * kr;
             for (int cl = wave; cl < ncol; cl += 4) {
                 const int tj = cl % nj, ti = cl / nj;
-                const double *qq = Q + (ti * VXR_TJ + tj) * xcmax + xa;
-                p.tempr[ridx(p, kr, J0 + tj, I0 + ti)] = qq[0] * wx[0] + qq[1] * wx[1] + qq[2] * wx[2] + qq[3] * wx[3];
+                const RT *qq = Q + (ti * VXR_TJ + tj) * xcmax + xa;
+                tempr[ridx(p, kr, J0 + tj, I0 + ti)] = qq[0] * wx[0] + qq[1] * wx[1] + qq[2] * wx[2] + qq[3] * wx[3];
             }
         }
     }
     // the unused index nxr of tempr is zero (phasefield.F90:502)
-    for (int cl = t; cl < ncol; cl += 256) p.tempr[ridx(p, p.nxmr, J0 + cl % nj, I0 + cl / nj)] = 0.0;
+    for (int cl = t; cl < ncol; cl += 256) tempr[ridx(p, p.nxmr, J0 + cl % nj, I0 + cl / nj)] = RT(0);
 }
 
 // set_ice_sphere (phasefield.F90:311-341) with the radius r0 of the
 // seeded initial state: refined phi = (1 - tanh((r - r0)/(2 eps)))/2 ...
+template <typename RT>
 __global__ void __launch_bounds__(256) k_init_phi_sphere(KP p, const double *xmr, double r0, double eps, double alx3,
                                                           double ylen, double zlen) {
     const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
     if (k >= p.nxr || j >= p.nyr) return;
     const long long o = ridx(p, k, j, i);
-    if (k >= p.nxmr) { p.phi[o] = 0.0; return; }
+    RT *phi = sel<RT>(p.phi, p.phif);
+    if (k >= p.nxmr) { phi[o] = RT(0); return; }
     const double y = (j + 0.5) * ylen / p.nyr, z = (i + 0.5) * zlen / p.nzr;
     const double dx = xmr[k] - 0.5 * alx3, dy = y - 0.5 * ylen, dz = z - 0.5 * zlen;
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    p.phi[o] = 0.5 * (1.0 - tanh(0.5 * (r - r0) / eps));
+    phi[o] = RT(0.5 * (1.0 - tanh(0.5 * (r - r0) / eps)));
 }
 // ... and T = 0 in the base cells inside the sphere
 __global__ void __launch_bounds__(256) k_init_temp_sphere(KP p, const double *xm, double r0, double alx3, double ylen,

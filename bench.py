@@ -43,7 +43,7 @@ WORKLOADS = {
                          rayt=1e5, prat=1.0, active_t=1, gaxis=1, dt=1e-3),
     # melting of a solid sphere (radius 0.15, phase-field width 1.5 refined
     # spacings) in RBC at Ra = 1e8: base 256 x 512 x 512, refined phi/T grid
-    # 512 x 1024 x 1024, Stefan number 1, A = 1, T_m = 0 (FP64 form)
+    # 512 x 1024 x 1024, Stefan number 1, A = 1, T_m = 0
     "pf_sphere_256x512x512_512x1024x1024": dict(
         nxm=256, nym=512, nzm=512, nsst=3, multires=1, nxmr=512, nymr=1024, nzmr=1024, phasefield=1,
         alx3=1.0, ylen=2.0, zlen=2.0, istr3=8, istr3r=8, rayt=1e8, prat=1.0, active_t=1, gaxis=1,
@@ -241,10 +241,10 @@ def kernel_bytes(cfg, form, sweeps=1, checks=0):
     if pf:
         b.pop("column_sal"); b.pop("mgrd_vxr"); b.pop("mgrd_vyr"); b.pop("mgrd_vzr"); b.pop("mgrd_interp_sal")
         # phi column: read phi, tempr, old H; write phi, H, d phi
-        b["pf_column_phi"] = 6 * d * nr
-        b["pf_latent_heat"] = d * nr + d * nb       # read d phi, write the coarse source
-        b["pf_interp_temp"] = d * nb + d * nr       # read T, write tempr
-        b["pf_interp_phi"] = d * nr + d * nb        # read phi, write phic
+        b["pf_column_phi"] = 6 * h * nr
+        b["pf_latent_heat"] = h * nr + d * nb       # read d phi, write the coarse source
+        b["pf_interp_temp"] = d * nb + h * nr       # read T, write tempr
+        b["pf_interp_phi"] = h * nr + d * nb        # read phi, write phic
     # Poisson: forward y (velocity -> spectrum; a residual check also reads
     # dph), two z transforms, the per-mode tridiagonals (+ their factors),
     # inverse y (-> dph; refinement sweeps read and write dph)
@@ -409,6 +409,8 @@ def main():
     ap.add_argument("--form", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 2 GiB device array 3 times (the reference kernel for FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--no-errors", action="store_true", help="skip the field downloads of the error report")
     ap.add_argument("--cpu-shrink", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -417,6 +419,18 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        # a plain device-to-device copy of known size: the counter readings of
+        # its dispatches calibrate FETCH_SIZE / WRITE_SIZE against real bytes
+        import torch
+        n = 1 << 28
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "dispatches": 3}))
+        return
 
     if args.workload == ENSEMBLE:
         return run_ensemble(args)
@@ -426,11 +440,6 @@ def main():
     dt = wl.pop("dt")
     ncell = cells(wl)
     forms = ["fp64", "mixed"] if args.form == "both" else [args.form]
-    if wl.get("phasefield"):
-        if args.form == "mixed":
-            print("bench.py: the phase-field workload runs in the FP64 form only", file=sys.stderr)
-            sys.exit(2)
-        forms = ["fp64"]
     runs = {f: run_form(wl, dt, f, args) for f in forms}
 
     head = runs["mixed"] if "mixed" in runs else runs["fp64"]

@@ -178,3 +178,75 @@ def test_sphere_initial_condition():
     o.call("interp_phi_multigrid"); o.call("interp_temp_multigrid")
     cmp(o, g, "phic")
     cmp(o, g, "tempr")
+
+
+# ---- mixed FP32/FP64 form of the phase-field step ----
+PF_SPHERE = dict(PF, nxm=24, nym=16, nzm=16, nxmr=48, nymr=32, nzmr=32, pf_r0=0.2, pf_eps=0.03, rayt=1e6)
+
+
+def sphere_start(cfg, precision, seed=3):
+    g = pyafid.Solver(dict(cfg, precision=precision))
+    nz, ny = cfg["nzm"], cfg["nym"]
+    g.upload("tempbp", np.ones((nz, ny, 1)))
+    g.upload("temptp", np.zeros((nz, ny, 1)))
+    g.create_initial_conditions(seed, 1e-2)
+    return g
+
+
+def l2rel(a, b):
+    return float(np.linalg.norm((np.asarray(a) - np.asarray(b)).ravel()) / max(np.linalg.norm(np.asarray(b).ravel()), 1e-300))
+
+
+def test_mixed_phase_vs_fp64():
+    """Mixed form (FP32 phi, tempr, explicit terms and transfers; FP64
+    state and solves; refined Poisson) against the FP64 form from the same
+    seeded sphere state.  Bounds as for the mixed salinity step: max|div u|
+    <= 1e-10, Nu_vol and KE within 1e-6 relative, relative L2 of the fields
+    <= 1e-5 (FP32 rounding is 6e-8 per update)."""
+    cfg = PF_SPHERE
+    a = sphere_start(cfg, pyafid.FP64)
+    b = sphere_start(cfg, pyafid.MIXED)
+    for _ in range(8):
+        a.time_marcher(2e-3)
+        b.time_marcher(2e-3)
+    nxm, nxmr = cfg["nxm"], cfg["nxmr"]
+    assert a.check_divergence()[0] < 1e-12
+    assert b.check_divergence()[0] <= 1e-10
+    qa, qb = a.global_quantities(), b.global_quantities()
+    assert qa["ke"] > 1e-10
+    for k in ("nu", "ke"):
+        assert abs(qb[k] - qa[k]) <= 1e-6 * abs(qa[k]), (k, qa[k], qb[k])
+    for f, sl in (("vx", np.s_[..., 1:nxm]), ("vz", np.s_[..., :nxm]), ("vy", np.s_[..., :nxm]),
+                  ("temp", np.s_[..., :nxm]), ("phi", np.s_[..., :nxmr]), ("tempr", np.s_[..., :nxmr]),
+                  ("phic", np.s_[..., :nxm])):
+        e = l2rel(b.download(f)[sl], a.download(f)[sl])
+        assert e <= 1e-5, f"{f}: relative L2 {e:.3e}"
+    pa, pb = a.download("pr")[..., :nxm], b.download("pr")[..., :nxm]
+    assert l2rel(pb - pb.mean(), pa - pa.mean()) <= 1e-5
+    # the sphere has started to melt in both
+    assert abs(a.download("phi").sum() - b.download("phi").sum()) <= 1e-5 * a.download("phi").sum()
+
+
+def test_mixed_phase_vs_oracle():
+    cfg = PF_SPHERE
+    b = sphere_start(cfg, pyafid.MIXED)
+    o = pyoracle.Oracle(cfg)
+    for f in ("vx", "vy", "vz", "temp", "pr", "tempbp", "temptp", "phi", "phic", "tempr"):
+        o.field(f)[...] = b.download(f)
+    for _ in range(3):
+        b.time_marcher(2e-3)
+        o.time_marcher(2e-3)
+    nxm, nxmr = cfg["nxm"], cfg["nxmr"]
+    for f, sl in (("vz", np.s_[..., :nxm]), ("temp", np.s_[..., :nxm]), ("phi", np.s_[..., :nxmr])):
+        e = l2rel(b.download(f)[sl], o.field(f)[sl])
+        assert e <= 1e-5, f"{f}: relative L2 {e:.3e}"
+
+
+def test_mixed_phase_refuses_per_routine_calls():
+    g = pyafid.Solver(dict(PF, precision=pyafid.MIXED))
+    g.set_substep(1e-3, 1)
+    for r in ("explicit_phase", "implicit_phase", "add_volume_penalty", "add_latent_heat"):
+        with pytest.raises(pyafid.AfidError, match="FP64 form"):
+            g.call(r)
+    g.call("interp_phi_multigrid")          # the transfers run in both forms
+    g.call("interp_temp_multigrid")
