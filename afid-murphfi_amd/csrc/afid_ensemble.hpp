@@ -166,9 +166,177 @@ __host__ __device__ inline int ens_fft_lines(int n, int cap, int nt) {
     return L < 1 ? 1 : L;
 }
 
-// One RK3 substep ns (1-based) of every scheduled run: workgroup w owns runs
-// sched_run[sched_off[w] .. sched_off[w+1]).  Lines: transform lines per
-// LDS batch (the tuned tile size).
+// ---------------------------------------------------------------------------
+// phases of one run's substep, shared by the fused kernel (one workgroup per
+// run, all phases) and the phased kernels (one launch per phase over every
+// run's columns, lines, modes or cells)
+// ---------------------------------------------------------------------------
+// the run's KP for substep ns with dt, reading buffer set cur
+__device__ __forceinline__ KP ens_kp(const EnsRun &R, int ns, double dt, int cur) {
+    KP p = R.p;
+    p.dt = dt;
+    p.al = R.alm[ns - 1]; p.ga = R.gam[ns - 1]; p.ro = R.rom[ns - 1];
+    p.beta = dt / p.ren * 0.5;
+    p.vx = R.vxb[cur]; p.vz = R.vzb[cur]; p.temp = R.tb[cur];
+    return p;
+}
+
+// LU factors of matrix which (0 vx, 1 vz, 2 T) for p's dt and substep, by
+// one wavefront, into f (4 arrays of 64*CH rows)
+template <int CH>
+__device__ __forceinline__ void ens_factors(const KP &p, int which, double *f) {
+    constexpr int NP = 64 * CH;
+    if (which == 0)        // identity rows 1 and nx (SolveImpEqnUpdate_X.F90:22-54)
+        wave_build_factors<CH>(p.am3ck, p.ac3ck, p.ap3ck, p.beta * p.al, 1, p.nxm, f, f + NP, f + 2 * NP, f + 3 * NP);
+    else if (which == 1)
+        wave_build_factors<CH>(p.am3sk, p.ac3sk, p.ap3sk, p.beta * p.al, 0, p.nxm, f, f + NP, f + 2 * NP, f + 3 * NP);
+    else
+        wave_build_factors<CH>(p.am3ssk, p.ac3ssk, p.ap3ssk, 0.5 * p.al * p.dt / p.pect, 0, p.nxm, f, f + NP,
+                               f + 2 * NP, f + 3 * NP);
+}
+
+// column i of the run by one wavefront: vx, vz, T in turn, new values into
+// buffer set cur ^ 1; fac holds the three factor sets, buf/own this
+// wavefront's LDS columns
+template <int CH>
+__device__ __forceinline__ void ens_column(const EnsRun &R, const KP &p, int cur, int i, const double *fac, double *buf,
+                                           double *own, unsigned lane) {
+    constexpr int NP = 64 * CH;
+    const int nx = p.nx, nxm = p.nxm;
+    const Nb n = make_nb(p, 0, i);
+    const long long c0 = (long long)i * p.plane;
+#pragma unroll 1
+    for (int fi = 0; fi < 3; ++fi) {
+        switch (fi) {
+        case 0: column_base_rows<0, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
+        case 1: column_base_rows<2, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
+        default: column_base_rows<3, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
+        }
+        wave_sync();
+        const double *f = fac + fi * 4 * NP;
+        TriFactors tf;
+        tf.l = f; tf.e = f + NP; tf.g = f + 2 * NP; tf.s = f + 3 * NP; tf.a = nullptr; tf.c = nullptr;
+        tf.n = fi == 0 ? nx : nxm; tf.chunk = CH;
+        column_solve_lds<CH>(buf, tf);
+        wave_sync();
+        double *out = (fi == 0 ? R.vxb[cur ^ 1] : fi == 1 ? R.vzb[cur ^ 1] : R.tb[cur ^ 1]) + c0;
+        for (unsigned rr = lane; rr < unsigned(nx); rr += 64) {
+            const unsigned qq = lds_row<CH>(rr);
+            double v = own[qq];
+            if (fi == 0) v = (rr >= 1 && rr < unsigned(nxm)) ? v + buf[qq] : 0.0;   // walls, ImplicitAndUpdateVX.F90:86-87
+            else if (rr < unsigned(nxm)) v += buf[qq];
+            out[rr] = v;
+        }
+        wave_sync();
+    }
+}
+
+// divergence (pressure.F90:147-161) of the line pairs [l0, l0 + nl) and
+// their forward transform along z (pressure.F90:171-179, normalised by
+// 1/nz since ny = 1) into the half spectrum; p reads the new velocity.
+// Whole workgroup of NT threads; lb holds nl lines.
+template <int NT>
+__device__ __forceinline__ void ens_forward(const EnsRun &R, const KP &p, int l0, int nl, cpx<double> *lb) {
+    const int nxm = p.nxm, N = p.nz, S = line_stride<double>(N), nh = N / 2 + 1, TX = 2 * nl, k0 = 2 * l0;
+    const double usdtal = 1.0 / (p.dt * p.al), scale = 1.0 / double(N);
+    for (int t = threadIdx.x; t < TX * N; t += NT) {
+        const int xo = t % TX, i = t / TX, k = k0 + xo;
+        double d = 0.0;
+        if (k < nxm) {
+            const long long c = (long long)i * p.plane + k, cp = (long long)wrapp(i, N) * p.plane + k;
+            d = ((p.vz[cp] - p.vz[c]) * p.dz + (p.vx[c + 1] - p.vx[c]) * p.udx3m[k]) * usdtal;
+        }
+        double *slot = reinterpret_cast<double *>(&lb[(xo >> 1) * S + lpad<double>(i)]);
+        slot[xo & 1] = d;
+    }
+    __syncthreads();
+    lds_fft<double, NT, ENS_FFT_MAXV>(lb, nl, S, R.plan, -1);
+    for (int t = threadIdx.x; t < TX * nh; t += NT) {
+        const int xo = t % TX, m = t / TX, k = k0 + xo;
+        if (k >= nxm) continue;
+        const int line = xo >> 1;
+        const cpx<double> zm = lb[line * S + lpad<double>(m)], zn = lb[line * S + lpad<double>(m == 0 ? 0 : N - m)];
+        cpx<double> o;
+        if ((xo & 1) == 0) o = mk<double>(0.5 * (zm.x + zn.x), 0.5 * (zm.y - zn.y));
+        else o = mk<double>(0.5 * (zm.y + zn.y), -0.5 * (zm.x - zn.x));
+        R.spec[(long long)m * nxm + k] = mk<double>(o.x * scale, o.y * scale);
+    }
+    __syncthreads();
+}
+
+// per-mode tridiagonal (pressure.F90:201-236) of mode m by one wavefront;
+// modes m and N-m share the matrix and the real data make them conjugate
+template <int CH>
+__device__ __forceinline__ void ens_mode(const EnsRun &R, int nxm, int m) {
+    mode_tri_solve<CH, double>(R.pf.m, R.pf.g, R.pf.e, (long long)m * R.pf.npad, R.pf.npad, R.spec + (long long)m * nxm, nxm);
+}
+
+// inverse transform (pressure.F90:189-195) of the line pairs [l0, l0 + nl)
+// into dph.  Whole workgroup.
+template <int NT>
+__device__ __forceinline__ void ens_inverse(const EnsRun &R, const KP &p, int l0, int nl, cpx<double> *lb) {
+    const int nxm = p.nxm, N = p.nz, S = line_stride<double>(N), nh = N / 2 + 1, TX = 2 * nl, k0 = 2 * l0;
+    for (int t = threadIdx.x; t < nl * nh; t += NT) {
+        const int line = t % nl, m = t / nl;
+        const int ka = k0 + 2 * line, kb = ka + 1;
+        const cpx<double> *row = R.spec + (long long)m * nxm;
+        cpx<double> A = ka < nxm ? row[ka] : mk<double>(0.0, 0.0);
+        cpx<double> B = kb < nxm ? row[kb] : mk<double>(0.0, 0.0);
+        if (m == 0 || 2 * m == N) { A.y = 0.0; B.y = 0.0; }
+        lb[line * S + lpad<double>(m)] = mk<double>(A.x - B.y, A.y + B.x);
+        if (m > 0 && N - m > m) lb[line * S + lpad<double>(N - m)] = mk<double>(A.x + B.y, B.x - A.y);
+    }
+    __syncthreads();
+    lds_fft<double, NT, ENS_FFT_MAXV>(lb, nl, S, R.plan, +1);
+    for (int t = threadIdx.x; t < TX * N; t += NT) {
+        const int xo = t % TX, i = t / TX, k = k0 + xo;
+        if (k >= nxm) continue;
+        const cpx<double> z = lb[(xo >> 1) * S + lpad<double>(i)];
+        p.dph[(long long)i * p.plane + k] = (xo & 1) ? z.y : z.x;
+    }
+    __syncthreads();
+}
+
+// RemoveDivergence + CorrectPressure (pressure.F90:245-300) of the cells
+// t = k + nxm i in [t0, t1), t0 + threadIdx.x stepping by NT
+template <int NT>
+__device__ __forceinline__ void ens_project(const KP &p, int t0, int t1) {
+    const int nxm = p.nxm, nz = p.nz;
+    const double adt = p.al * p.dt, be = p.al * p.beta;
+    const double *d = p.dph;
+    for (int t = t0 + threadIdx.x; t < t1; t += NT) {
+        const int k = t % nxm, i = t / nxm, im = wrapm(i, nz), ip = wrapp(i, nz);
+        const int km = k == 0 ? 0 : k - 1, kp = k == nxm - 1 ? k : k + 1;
+        const long long c = (long long)i * p.plane + k, row = c - k;
+        const double dc = d[c], dkm = d[row + km], dkp = d[row + kp];
+        const double dim = d[(long long)im * p.plane + k], dip = d[(long long)ip * p.plane + k];
+        const double ux = p.vx[c], uz = p.vz[c], prc = p.pr[c];
+        const double nux = ux - (dc - dkm) * (adt * p.udx3c[k]);
+        const double nuz = uz - (dc - dim) * (adt * p.dz);
+        const double npr = prc + (dc - be * ((dip - 2.0 * dc + dim) * p.dzq +
+                                            (dkp * p.apphk[k] + dc * p.acphk[k] + dkm * p.amphk[k])));
+        p.vx[c] = nux; p.vz[c] = nuz; p.pr[c] = npr;
+    }
+}
+
+// the driver's update after step s.ntime (thread 0): time, exit codes
+// 169 / 333 / 555 (main.F90:355-385) and the next step's dt
+__device__ __forceinline__ void ens_end_of_step(const EnsRun &R, EnsState &s, double cfl, double dmax) {
+    s.time += s.dt;
+    s.cfl = cfl; s.dmax = dmax;
+    if (!(dmax <= R.resid)) s.status = 169;
+    else if (s.time > R.tmax) s.status = 333;
+    else if (s.ntime == R.ntst) s.status = 555;
+    s.ntime += 1;
+    s.steps += 1;
+    ens_top_of_step(R, s, cfl);
+}
+
+// ---------------------------------------------------------------------------
+// fused form: one RK3 substep ns (1-based) of every scheduled run in one
+// launch; workgroup w owns runs sched_run[sched_off[w] .. sched_off[w+1]).
+// Lines: transform lines per LDS batch (the tuned tile size).
+// ---------------------------------------------------------------------------
 template <int NT, int CH>
 __global__ void __launch_bounds__(NT) k_ens_substep(const EnsRun *__restrict__ runs, EnsState *st,
                                                     const int *__restrict__ sched_off, const int *__restrict__ sched_run,
@@ -187,157 +355,28 @@ __global__ void __launch_bounds__(NT) k_ens_substep(const EnsRun *__restrict__ r
         const EnsRun &R = runs[r];
         if (__builtin_amdgcn_readfirstlane(st[r].status)) continue;     // uniform over the workgroup
         const int cur = __builtin_amdgcn_readfirstlane(st[r].cur);
-        const double dt = st[r].dt;
-        KP p = R.p;
-        p.dt = dt;
-        p.al = R.alm[ns - 1]; p.ga = R.gam[ns - 1]; p.ro = R.rom[ns - 1];
-        p.beta = dt / p.ren * 0.5;
-        p.vx = R.vxb[cur]; p.vz = R.vzb[cur]; p.temp = R.tb[cur];
-        const int nx = p.nx, nxm = p.nxm, nz = p.nz;
-
-        // ---- factors of the vx, vz and T matrices for this dt and substep
-        if (wave < 3) {
-            double *f = fac + wave * 4 * NP;
-            if (wave == 0)        // identity rows 1 and nx (SolveImpEqnUpdate_X.F90:22-54)
-                wave_build_factors<CH>(p.am3ck, p.ac3ck, p.ap3ck, p.beta * p.al, 1, nxm, f, f + NP, f + 2 * NP, f + 3 * NP);
-            else if (wave == 1)
-                wave_build_factors<CH>(p.am3sk, p.ac3sk, p.ap3sk, p.beta * p.al, 0, nxm, f, f + NP, f + 2 * NP, f + 3 * NP);
-            else
-                wave_build_factors<CH>(p.am3ssk, p.ac3ssk, p.ap3ssk, 0.5 * p.al * p.dt / p.pect, 0, nxm, f, f + NP,
-                                       f + 2 * NP, f + 3 * NP);
-        }
+        KP p = ens_kp(R, ns, st[r].dt, cur);
+        if (wave < 3) ens_factors<CH>(p, wave, fac + wave * 4 * NP);
         __syncthreads();
-
-        // ---- columns: one wavefront per z index, vx, vz, T in turn
         {
             double *buf = cols + wave * 2 * lds_col_size<CH>(), *own = buf + lds_col_size<CH>();
-            for (int i = wave; i < nz; i += NW) {
-                const Nb n = make_nb(p, 0, i);
-                const long long c0 = (long long)i * p.plane;
-#pragma unroll 1
-                for (int fi = 0; fi < 3; ++fi) {
-                    switch (fi) {
-                    case 0: column_base_rows<0, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
-                    case 1: column_base_rows<2, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
-                    default: column_base_rows<3, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
-                    }
-                    wave_sync();
-                    const double *f = fac + fi * 4 * NP;
-                    TriFactors tf;
-                    tf.l = f; tf.e = f + NP; tf.g = f + 2 * NP; tf.s = f + 3 * NP; tf.a = nullptr; tf.c = nullptr;
-                    tf.n = fi == 0 ? nx : nxm; tf.chunk = CH;
-                    column_solve_lds<CH>(buf, tf);
-                    wave_sync();
-                    double *out = (fi == 0 ? R.vxb[cur ^ 1] : fi == 1 ? R.vzb[cur ^ 1] : R.tb[cur ^ 1]) + c0;
-                    for (unsigned rr = lane; rr < unsigned(nx); rr += 64) {
-                        const unsigned qq = lds_row<CH>(rr);
-                        double v = own[qq];
-                        if (fi == 0) v = (rr >= 1 && rr < unsigned(nxm)) ? v + buf[qq] : 0.0;   // walls, ImplicitAndUpdateVX.F90:86-87
-                        else if (rr < unsigned(nxm)) v += buf[qq];
-                        out[rr] = v;
-                    }
-                    wave_sync();
-                }
-            }
+            for (int i = wave; i < p.nz; i += NW) ens_column<CH>(R, p, cur, i, fac, buf, own, lane);
         }
         __syncthreads();
         p.vx = R.vxb[cur ^ 1]; p.vz = R.vzb[cur ^ 1]; p.temp = R.tb[cur ^ 1];
-
-        // ---- divergence (pressure.F90:147-161) and forward transform along z
-        // (pressure.F90:171-179, normalised by 1/nz; ny = 1)
-        const int N = nz, S = line_stride<double>(N), nh = N / 2 + 1;
-        const int nlines = (nxm + 1) / 2;
-        const int L = ens_fft_lines(N, lines, NT);
-        const double usdtal = 1.0 / (p.dt * p.al), scale = 1.0 / double(N);
-        for (int l0 = 0; l0 < nlines; l0 += L) {
-            const int nl = min(L, nlines - l0), TX = 2 * nl, k0 = 2 * l0;
-            for (int t = threadIdx.x; t < TX * N; t += NT) {
-                const int xo = t % TX, i = t / TX, k = k0 + xo;
-                double d = 0.0;
-                if (k < nxm) {
-                    const long long c = (long long)i * p.plane + k, cp = (long long)wrapp(i, N) * p.plane + k;
-                    d = ((p.vz[cp] - p.vz[c]) * p.dz + (p.vx[c + 1] - p.vx[c]) * p.udx3m[k]) * usdtal;
-                }
-                double *slot = reinterpret_cast<double *>(&lb[(xo >> 1) * S + lpad<double>(i)]);
-                slot[xo & 1] = d;
-            }
-            __syncthreads();
-            lds_fft<double, NT, ENS_FFT_MAXV>(lb, nl, S, R.plan, -1);
-            for (int t = threadIdx.x; t < TX * nh; t += NT) {
-                const int xo = t % TX, m = t / TX, k = k0 + xo;
-                if (k >= nxm) continue;
-                const int line = xo >> 1;
-                const cpx<double> zm = lb[line * S + lpad<double>(m)], zn = lb[line * S + lpad<double>(m == 0 ? 0 : N - m)];
-                cpx<double> o;
-                if ((xo & 1) == 0) o = mk<double>(0.5 * (zm.x + zn.x), 0.5 * (zm.y - zn.y));
-                else o = mk<double>(0.5 * (zm.y + zn.y), -0.5 * (zm.x - zn.x));
-                R.spec[(long long)m * nxm + k] = mk<double>(o.x * scale, o.y * scale);
-            }
-            __syncthreads();
-        }
-
-        // ---- per-mode tridiagonals (pressure.F90:201-236); modes m and N-m
-        // share the matrix and the real data make them conjugate
-        for (int m = wave; m < nh; m += NW)
-            mode_tri_solve<CH, double>(R.pf.m, R.pf.g, R.pf.e, (long long)m * R.pf.npad, R.pf.npad,
-                                       R.spec + (long long)m * nxm, nxm);
+        const int nlines = (p.nxm + 1) / 2, L = ens_fft_lines(p.nz, lines, NT), nh = p.nz / 2 + 1;
+        for (int l0 = 0; l0 < nlines; l0 += L) ens_forward<NT>(R, p, l0, min(L, nlines - l0), lb);
+        for (int m = wave; m < nh; m += NW) ens_mode<CH>(R, p.nxm, m);
         __syncthreads();
-
-        // ---- inverse transform (pressure.F90:189-195) into dph
-        for (int l0 = 0; l0 < nlines; l0 += L) {
-            const int nl = min(L, nlines - l0), TX = 2 * nl, k0 = 2 * l0;
-            for (int t = threadIdx.x; t < nl * nh; t += NT) {
-                const int line = t % nl, m = t / nl;
-                const int ka = k0 + 2 * line, kb = ka + 1;
-                const cpx<double> *row = R.spec + (long long)m * nxm;
-                cpx<double> A = ka < nxm ? row[ka] : mk<double>(0.0, 0.0);
-                cpx<double> B = kb < nxm ? row[kb] : mk<double>(0.0, 0.0);
-                if (m == 0 || 2 * m == N) { A.y = 0.0; B.y = 0.0; }
-                lb[line * S + lpad<double>(m)] = mk<double>(A.x - B.y, A.y + B.x);
-                if (m > 0 && N - m > m) lb[line * S + lpad<double>(N - m)] = mk<double>(A.x + B.y, B.x - A.y);
-            }
-            __syncthreads();
-            lds_fft<double, NT, ENS_FFT_MAXV>(lb, nl, S, R.plan, +1);
-            for (int t = threadIdx.x; t < TX * N; t += NT) {
-                const int xo = t % TX, i = t / TX, k = k0 + xo;
-                if (k >= nxm) continue;
-                const cpx<double> z = lb[(xo >> 1) * S + lpad<double>(i)];
-                p.dph[(long long)i * p.plane + k] = (xo & 1) ? z.y : z.x;
-            }
-            __syncthreads();
-        }
-
-        // ---- RemoveDivergence + CorrectPressure (pressure.F90:245-300)
-        {
-            const double adt = p.al * p.dt, be = p.al * p.beta;
-            const double *d = p.dph;
-            for (int t = threadIdx.x; t < nxm * nz; t += NT) {
-                const int k = t % nxm, i = t / nxm, im = wrapm(i, nz), ip = wrapp(i, nz);
-                const int km = k == 0 ? 0 : k - 1, kp = k == nxm - 1 ? k : k + 1;
-                const long long c = (long long)i * p.plane + k;
-                const double dc = d[c];
-                p.vx[c] -= (dc - d[c - k + km]) * (adt * p.udx3c[k]);
-                p.vz[c] -= (dc - d[(long long)im * p.plane + k]) * (adt * p.dz);
-                p.pr[c] += dc - be * ((d[(long long)ip * p.plane + k] - 2.0 * dc + d[(long long)im * p.plane + k]) * p.dzq +
-                                      (d[c - k + kp] * p.apphk[k] + dc * p.acphk[k] + d[c - k + km] * p.amphk[k]));
-            }
-        }
+        for (int l0 = 0; l0 < nlines; l0 += L) ens_inverse<NT>(R, p, l0, min(L, nlines - l0), lb);
+        ens_project<NT>(p, 0, p.nxm * p.nz);
         __syncthreads();
-
-        // ---- driver: time, CFL / divergence checks and the next dt
         if (ns == R.nsst) {
             double cfl, dmax;
             ens_cfl_div<NT>(p, cfl, dmax, red);
             if (threadIdx.x == 0) {
                 EnsState s = st[r];
-                s.time += s.dt;
-                s.cfl = cfl; s.dmax = dmax;
-                if (!(dmax <= R.resid)) s.status = 169;
-                else if (s.time > R.tmax) s.status = 333;
-                else if (s.ntime == R.ntst) s.status = 555;
-                s.ntime += 1;
-                s.steps += 1;
-                ens_top_of_step(R, s, cfl);
+                ens_end_of_step(R, s, cfl, dmax);
                 s.cur = cur ^ 1;
                 st[r] = s;
             }
@@ -345,6 +384,108 @@ __global__ void __launch_bounds__(NT) k_ens_substep(const EnsRun *__restrict__ r
             st[r].cur = cur ^ 1;
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// phased form: the same substep as a sequence of launches, each over the
+// work items of every running run (a run's items share nothing with other
+// runs; the launch boundaries order the phases):
+//   factors (one workgroup per run) -> columns (4 per workgroup) ->
+//   forward (line batches) -> modes (4 per workgroup) -> inverse ->
+//   project (cell chunks) -> finish (one workgroup per run: buffer parity,
+//   and after the last substep the driver's step update).
+// An item is (run, first index); frozen runs' items return at once.
+// ---------------------------------------------------------------------------
+struct EnsItem { int run, start; };
+constexpr int ENSP_NT = 256;                      // threads of the phased kernels
+constexpr int ENSP_CELLS = 4 * ENSP_NT;           // cells per projection item
+
+template <int CH>
+__global__ void __launch_bounds__(192) k_ensp_factors(const EnsRun *__restrict__ runs, const EnsState *st, int ns,
+                                                      double *fac) {
+    const int r = blockIdx.x;
+    if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
+    const EnsRun &R = runs[r];
+    const KP p = ens_kp(R, ns, st[r].dt, 0);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    ens_factors<CH>(p, wave, fac + ((long long)r * 3 + wave) * 4 * 64 * CH);
+}
+
+template <int CH>
+__global__ void __launch_bounds__(ENSP_NT) k_ensp_columns(const EnsRun *__restrict__ runs, const EnsState *st,
+                                                          const EnsItem *__restrict__ items, int ns, const double *fac) {
+    __shared__ double lds[ENSP_NT / 64][2][lds_col_size<CH>()];
+    const EnsItem it = items[blockIdx.x];
+    const int r = it.run;
+    if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
+    const EnsRun &R = runs[r];
+    const int cur = __builtin_amdgcn_readfirstlane(st[r].cur);
+    const KP p = ens_kp(R, ns, st[r].dt, cur);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = it.start + wave;
+    if (i >= p.nz) return;
+    ens_column<CH>(R, p, cur, i, fac + (long long)r * 12 * 64 * CH, lds[wave][0], lds[wave][1], threadIdx.x & 63);
+}
+
+// forward (inverse = 0) or inverse (inverse = 1) transform of one line batch
+__global__ void __launch_bounds__(ENSP_NT) k_ensp_transform(const EnsRun *__restrict__ runs, const EnsState *st,
+                                                            const EnsItem *__restrict__ items, int ns, int lines,
+                                                            int inverse) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    cpx<double> *lb = reinterpret_cast<cpx<double> *>(smraw);
+    const EnsItem it = items[blockIdx.x];
+    const int r = it.run;
+    if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
+    const EnsRun &R = runs[r];
+    const int cur = __builtin_amdgcn_readfirstlane(st[r].cur);
+    const KP p = ens_kp(R, ns, st[r].dt, cur ^ 1);          // the columns' new velocity
+    const int nlines = (p.nxm + 1) / 2, L = ens_fft_lines(p.nz, lines, ENSP_NT);
+    const int nl = min(L, nlines - it.start);
+    if (inverse) ens_inverse<ENSP_NT>(R, p, it.start, nl, lb);
+    else ens_forward<ENSP_NT>(R, p, it.start, nl, lb);
+}
+
+template <int CH>
+__global__ void __launch_bounds__(ENSP_NT) k_ensp_modes(const EnsRun *__restrict__ runs, const EnsState *st,
+                                                        const EnsItem *__restrict__ items) {
+    const EnsItem it = items[blockIdx.x];
+    const int r = it.run;
+    if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
+    const EnsRun &R = runs[r];
+    const int m = it.start + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (m <= R.p.nz / 2) ens_mode<CH>(R, R.p.nxm, m);
+}
+
+__global__ void __launch_bounds__(ENSP_NT) k_ensp_project(const EnsRun *__restrict__ runs, const EnsState *st,
+                                                          const EnsItem *__restrict__ items, int ns) {
+    const EnsItem it = items[blockIdx.x];
+    const int r = it.run;
+    if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
+    const EnsRun &R = runs[r];
+    const int cur = __builtin_amdgcn_readfirstlane(st[r].cur);
+    const KP p = ens_kp(R, ns, st[r].dt, cur ^ 1);
+    ens_project<ENSP_NT>(p, it.start, min(it.start + ENSP_CELLS, p.nxm * p.nz));
+}
+
+__global__ void __launch_bounds__(ENSP_NT) k_ensp_finish(const EnsRun *__restrict__ runs, EnsState *st, int ns) {
+    __shared__ double red[2 * ENSP_NT / 64];
+    const int r = blockIdx.x;
+    if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
+    const EnsRun &R = runs[r];
+    const int cur = __builtin_amdgcn_readfirstlane(st[r].cur);
+    if (ns == R.nsst) {
+        const KP p = ens_kp(R, ns, st[r].dt, cur ^ 1);
+        double cfl, dmax;
+        ens_cfl_div<ENSP_NT>(p, cfl, dmax, red);
+        if (threadIdx.x == 0) {
+            EnsState s = st[r];
+            ens_end_of_step(R, s, cfl, dmax);
+            s.cur = cur ^ 1;
+            st[r] = s;
+        }
+    } else if (threadIdx.x == 0) {
+        st[r].cur = cur ^ 1;
     }
 }
 

@@ -24,8 +24,13 @@ struct afid_ens {
     int *d_id_off = nullptr, *d_id_run = nullptr;      // one run per workgroup (prepare kernel)
     int *d_sched_off = nullptr, *d_sched_run = nullptr;
     int sched_groups = 0;
-    afid_ens_tuning tune{1024, 0, 8, 0.0};
+    afid_ens_tuning tune{1024, 0, 8, 0, 0.0};
     size_t lds = 0;
+    // phased form: work items per phase and the per-run factor sets
+    EnsItem *d_items_col = nullptr, *d_items_mode = nullptr, *d_items_line = nullptr, *d_items_cell = nullptr;
+    int n_col = 0, n_mode = 0, n_line = 0, n_cell = 0, max_line = 0;
+    double *d_fac = nullptr;
+    size_t lds_line = 0;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
 };
@@ -84,7 +89,44 @@ const void *ens_kernel_ptr(int ch) {
     }
 }
 
+int ens_launched(afid_ens *e, const char *what) {
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return ens_err(e, AFID_E_HIP, std::string("launch of ") + what + " failed: " + hipGetErrorString(err));
+    return AFID_OK;
+}
+
+// the phased form of substep ns: seven launches
+template <int CH>
+int ens_substep_phased_ch(afid_ens *e, int ns) {
+    int rc;
+    const dim3 blk(ENSP_NT);
+    hipLaunchKernelGGL((k_ensp_factors<CH>), dim3(unsigned(e->nruns)), dim3(192), 0, e->stream, e->d_runs, e->d_state, ns, e->d_fac);
+    if ((rc = ens_launched(e, "k_ensp_factors"))) return rc;
+    hipLaunchKernelGGL((k_ensp_columns<CH>), dim3(unsigned(e->n_col)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_col, ns, e->d_fac);
+    if ((rc = ens_launched(e, "k_ensp_columns"))) return rc;
+    hipLaunchKernelGGL(k_ensp_transform, dim3(unsigned(e->n_line)), blk, e->lds_line, e->stream, e->d_runs, e->d_state, e->d_items_line,
+                       ns, e->tune.lines, 0);
+    if ((rc = ens_launched(e, "k_ensp_transform"))) return rc;
+    hipLaunchKernelGGL((k_ensp_modes<CH>), dim3(unsigned(e->n_mode)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_mode);
+    if ((rc = ens_launched(e, "k_ensp_modes"))) return rc;
+    hipLaunchKernelGGL(k_ensp_transform, dim3(unsigned(e->n_line)), blk, e->lds_line, e->stream, e->d_runs, e->d_state, e->d_items_line,
+                       ns, e->tune.lines, 1);
+    if ((rc = ens_launched(e, "k_ensp_transform"))) return rc;
+    hipLaunchKernelGGL(k_ensp_project, dim3(unsigned(e->n_cell)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_cell, ns);
+    if ((rc = ens_launched(e, "k_ensp_project"))) return rc;
+    hipLaunchKernelGGL(k_ensp_finish, dim3(unsigned(e->nruns)), blk, 0, e->stream, e->d_runs, e->d_state, ns);
+    return ens_launched(e, "k_ensp_finish");
+}
+
 int ens_substep(afid_ens *e, int ns) {
+    if (e->tune.phased) {
+        switch (e->ch) {
+        case 1: return ens_substep_phased_ch<1>(e, ns);
+        case 2: return ens_substep_phased_ch<2>(e, ns);
+        case 3: return ens_substep_phased_ch<3>(e, ns);
+        default: return ens_substep_phased_ch<4>(e, ns);
+        }
+    }
     const dim3 grid(unsigned(e->sched_groups));
     switch (e->tune.threads) {
     case 256: return ens_launch_nt<256>(e, e->ch, grid, ns);
@@ -149,7 +191,41 @@ int ens_drop_graph(afid_ens *e) {
     return AFID_OK;
 }
 
+// line-batch work items of the phased transforms for a lines cap
+int ens_set_line_items(afid_ens *e, int lines) {
+    std::vector<EnsItem> it;
+    size_t lds = 0;
+    for (int r = 0; r < e->nruns; ++r) {
+        const afid_ctx *c = e->runs[size_t(r)];
+        if (c->nz > ENS_FFT_MAXV * ENSP_NT) return ens_err(e, AFID_E_ARG, "ensemble: nz too large for the phased transforms");
+        const int L = ens_fft_lines(c->nz, lines, ENSP_NT), nlines = (c->nxm + 1) / 2;
+        for (int l0 = 0; l0 < nlines; l0 += L) it.push_back(EnsItem{r, l0});
+        lds = std::max(lds, size_t(L) * size_t(line_stride<double>(c->nz)) * sizeof(cpx<double>));
+    }
+    if (int(it.size()) > e->max_line) return ens_err(e, AFID_E_ARG, "ensemble: line items exceed their table");
+    if (lds > size_t(e->max_lds)) return ens_err(e, AFID_E_ARG, "ensemble tuning: transform lines do not fit the LDS");
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    ENSCHK(e, hipMemcpy(e->d_items_line, it.data(), sizeof(EnsItem) * it.size(), hipMemcpyHostToDevice));
+    e->n_line = int(it.size());
+    e->lds_line = lds;
+    ENSCHK(e, hipFuncSetAttribute(reinterpret_cast<const void *>(k_ensp_transform), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(std::max<size_t>(lds, 256))));
+    return AFID_OK;
+}
+
 int ens_apply_tuning(afid_ens *e, const afid_ens_tuning &t) {
+    if (t.phased) {
+        if (t.lines < 1) return ens_err(e, AFID_E_ARG, "ensemble tuning: lines must be >= 1");
+        int rc = ens_set_line_items(e, t.lines);
+        if (rc) return rc;
+        e->tune.phased = 1;
+        e->tune.lines = t.lines;
+        e->tune.threads = ENSP_NT;
+        e->tune.workgroups = 0;
+        ens_drop_graph(e);
+        return AFID_OK;
+    }
+    e->tune.phased = 0;
     if (t.threads != 256 && t.threads != 512 && t.threads != 1024)
         return ens_err(e, AFID_E_ARG, "ensemble tuning: threads must be 256, 512 or 1024");
     if (t.lines < 1) return ens_err(e, AFID_E_ARG, "ensemble tuning: lines must be >= 1");
@@ -302,6 +378,29 @@ int afid_ens_init(const afid_config *cfgs, const afid_run_control *ctl, int32_t 
     if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_id_run), sizeof(int) * size_t(nruns)))) return rc;
     if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_sched_off), sizeof(int) * size_t(nruns + 1)))) return rc;
     if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_sched_run), sizeof(int) * size_t(nruns)))) return rc;
+    {
+        // phased form: fixed work-item tables (columns, modes, cells) and the
+        // line table at its largest (one line pair per item)
+        std::vector<EnsItem> col, mode, cell;
+        int lines = 0;
+        for (int r = 0; r < nruns; ++r) {
+            const afid_ctx *c = e->runs[size_t(r)];
+            for (int i = 0; i < c->nz; i += ENSP_NT / 64) col.push_back(EnsItem{r, i});
+            for (int m = 0; m <= c->nz / 2; m += ENSP_NT / 64) mode.push_back(EnsItem{r, m});
+            for (int q = 0; q < c->nxm * c->nz; q += ENSP_CELLS) cell.push_back(EnsItem{r, q});
+            lines += (c->nxm + 1) / 2;
+        }
+        e->n_col = int(col.size()); e->n_mode = int(mode.size()); e->n_cell = int(cell.size()); e->max_line = lines;
+        if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_items_col), sizeof(EnsItem) * col.size()))) return rc;
+        if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_items_mode), sizeof(EnsItem) * mode.size()))) return rc;
+        if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_items_cell), sizeof(EnsItem) * cell.size()))) return rc;
+        if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_items_line), sizeof(EnsItem) * size_t(lines)))) return rc;
+        if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_fac), sizeof(double) * size_t(nruns) * 12 * 64 * e->ch))) return rc;
+        ENSCHK(e, hipStreamSynchronize(e->stream));
+        ENSCHK(e, hipMemcpy(e->d_items_col, col.data(), sizeof(EnsItem) * col.size(), hipMemcpyHostToDevice));
+        ENSCHK(e, hipMemcpy(e->d_items_mode, mode.data(), sizeof(EnsItem) * mode.size(), hipMemcpyHostToDevice));
+        ENSCHK(e, hipMemcpy(e->d_items_cell, cell.data(), sizeof(EnsItem) * cell.size(), hipMemcpyHostToDevice));
+    }
     ENSCHK(e, hipStreamSynchronize(e->stream));
     ENSCHK(e, hipMemcpy(e->d_runs, tab.data(), sizeof(EnsRun) * tab.size(), hipMemcpyHostToDevice));
     std::vector<int> ioff(size_t(nruns) + 1), irun((size_t)nruns);
@@ -310,7 +409,7 @@ int afid_ens_init(const afid_config *cfgs, const afid_run_control *ctl, int32_t 
     ENSCHK(e, hipMemcpy(e->d_id_off, ioff.data(), sizeof(int) * ioff.size(), hipMemcpyHostToDevice));
     ENSCHK(e, hipMemcpy(e->d_id_run, irun.data(), sizeof(int) * irun.size(), hipMemcpyHostToDevice));
     // default launch shape: 1024 threads, one run per workgroup, 8 lines
-    afid_ens_tuning t{1024, nruns, 8, 0.0};
+    afid_ens_tuning t{1024, nruns, 8, 0, 0.0};
     if ((rc = ens_apply_tuning(e, t))) return rc;
     if ((rc = ens_reset_state(e))) return rc;
     ENSCHK(e, hipStreamSynchronize(e->stream));
@@ -487,17 +586,26 @@ int afid_ens_autotune(afid_ens *e, int32_t steps, afid_ens_tuning *best, afid_en
     std::vector<int> groups = {e->nruns};
     for (int g : {e->n_cu, 2 * e->n_cu, e->nruns / 2})
         if (g >= 1 && g < e->nruns && std::find(groups.begin(), groups.end(), g) == groups.end()) groups.push_back(g);
-    afid_ens_tuning win{0, 0, 0, HUGE_VAL};
+    afid_ens_tuning win{0, 0, 0, 0, HUGE_VAL};
     int n = 0, rc = AFID_OK;
     hipEvent_t a = nullptr, b = nullptr;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
-    for (int nt : {256, 512, 1024}) {
+    // candidate shapes: the phased form over its line caps, then the
+    // persistent form over threads x lines x grid
+    std::vector<afid_ens_tuning> cands;
+    for (int lines : {1, 2, 4, 8})
+        if (lines == 1 || ens_fft_lines(e->nzmax, lines, ENSP_NT) != ens_fft_lines(e->nzmax, lines / 2, ENSP_NT))
+            cands.push_back(afid_ens_tuning{ENSP_NT, 0, lines, 1, 0.0});
+    for (int nt : {256, 512, 1024})
         for (int lines : {2, 4, 8, 16}) {
             if (ens_lds_bytes(e, nt, lines) == 0) continue;
             if (lines > 2 && ens_fft_lines(e->nzmax, lines, nt) == ens_fft_lines(e->nzmax, lines / 2, nt)) continue;
-            for (int g : groups) {
-                afid_ens_tuning t{nt, g, lines, 0.0};
+            for (int g : groups) cands.push_back(afid_ens_tuning{nt, g, lines, 0, 0.0});
+        }
+    {
+        for (afid_ens_tuning t : cands) {
+            {
                 if ((rc = ens_apply_tuning(e, t))) break;
                 if ((rc = ens_steps_eager(e, 1))) break;
                 (void)hipEventRecord(a, e->stream);
@@ -508,6 +616,7 @@ int afid_ens_autotune(afid_ens *e, int32_t steps, afid_ens_tuning *best, afid_en
                 (void)hipEventElapsedTime(&ms, a, b);
                 t.ms_per_step = double(ms) / steps;
                 t.workgroups = e->tune.workgroups;
+                t.threads = e->tune.threads;
                 if (log && n < maxlog) log[n] = t;
                 ++n;
                 if (t.ms_per_step < win.ms_per_step) win = t;
@@ -518,9 +627,7 @@ int afid_ens_autotune(afid_ens *e, int32_t steps, afid_ens_tuning *best, afid_en
                 if (herr == hipSuccess) herr = hipStreamSynchronize(e->stream);
                 if (herr != hipSuccess) { rc = ens_err(e, AFID_E_HIP, std::string("autotune restore: ") + hipGetErrorString(herr)); break; }
             }
-            if (rc) break;
         }
-        if (rc) break;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

@@ -39,7 +39,7 @@ module afid_gpu
     end type afid_run_control
 
     type, bind(C) :: afid_ens_tuning
-        integer(c_int32_t) :: threads, workgroups, lines
+        integer(c_int32_t) :: threads, workgroups, lines, phased
         real(c_double)     :: ms_per_step
     end type afid_ens_tuning
 

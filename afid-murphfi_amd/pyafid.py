@@ -107,7 +107,8 @@ class RunControl(C.Structure):
 
 
 class EnsTuning(C.Structure):
-    _fields_ = [("threads", C.c_int32), ("workgroups", C.c_int32), ("lines", C.c_int32), ("ms_per_step", C.c_double)]
+    _fields_ = [("threads", C.c_int32), ("workgroups", C.c_int32), ("lines", C.c_int32), ("phased", C.c_int32),
+                ("ms_per_step", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -414,8 +415,8 @@ class Ensemble:
         self._check(self.L.afid_ens_autotune(self.h, steps, C.byref(best), log, 64, C.byref(n)), "afid_ens_autotune")
         return best.as_dict(), [log[q].as_dict() for q in range(min(n.value, 64))]
 
-    def set_tuning(self, threads, workgroups=0, lines=8):
-        t = EnsTuning(threads, workgroups, lines, 0.0)
+    def set_tuning(self, threads=1024, workgroups=0, lines=8, phased=0):
+        t = EnsTuning(threads, workgroups, lines, phased, 0.0)
         self._check(self.L.afid_ens_set_tuning(self.h, C.byref(t)), "afid_ens_set_tuning")
 
     def get_tuning(self):

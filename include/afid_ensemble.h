@@ -41,11 +41,17 @@ typedef struct afid_run_control {
     double tmax;          /* TMAX: exit 333 once the time exceeds it */
 } afid_run_control;
 
-/* Launch shape of the substep kernel, chosen by afid_ens_autotune or set. */
+/* Launch shape of a substep, chosen by afid_ens_autotune or set.
+ * phased = 0: one persistent launch per substep; each workgroup owns whole
+ *             runs (threads, workgroups, lines apply);
+ * phased = 1: one launch per phase (factors, columns, forward transform,
+ *             modes, inverse transform, projection, finish), each over the
+ *             work items of every run (lines applies; 256 threads). */
 typedef struct afid_ens_tuning {
-    int32_t threads;      /* workgroup size: 256, 512 or 1024 */
+    int32_t threads;      /* workgroup size of the persistent launch: 256, 512 or 1024 */
     int32_t workgroups;   /* persistent grid: runs are dealt largest-first to this many workgroups */
     int32_t lines;        /* transform lines per LDS batch (upper bound) */
+    int32_t phased;       /* 0 persistent launch per substep, 1 one launch per phase */
     double ms_per_step;   /* measured time of one RK3 step of the whole ensemble (autotune) */
 } afid_ens_tuning;
 

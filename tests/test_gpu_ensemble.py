@@ -176,6 +176,9 @@ def _fresh(cs, ctl, seed=11):
 
 
 def test_graph_replay_and_launch_shapes_are_bit_identical():
+    """The persistent form under three launch shapes and the phased form
+    under two line caps, replayed from a captured graph, against eager
+    launches of the default shape."""
     cs = cfgs()
     ctl = dict(variable_dt=1, dt=1e-3, limit_cfl=0.5, dtmax=1e-2)
     a = _fresh(cs, ctl)
@@ -183,7 +186,7 @@ def test_graph_replay_and_launch_shapes_are_bit_identical():
     ref = _fields(a)
     a.close()
     shapes = [dict(threads=1024, workgroups=0, lines=8), dict(threads=256, workgroups=2, lines=2),
-              dict(threads=512, workgroups=3, lines=4)]
+              dict(threads=512, workgroups=3, lines=4), dict(phased=1, lines=1), dict(phased=1, lines=4)]
     for sh in shapes:
         b = _fresh(cs, ctl)
         b.set_tuning(**sh)
@@ -226,6 +229,7 @@ def test_autotune_keeps_state_and_results():
     before = _fields(a)
     best, log = a.autotune(steps=1)
     assert log and best["threads"] in (256, 512, 1024) and best["ms_per_step"] > 0
+    assert {c["phased"] for c in log} == {0, 1}
     after = _fields(a)
     for r in range(len(cs)):
         for f in before[r]:
