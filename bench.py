@@ -185,7 +185,10 @@ def run_ensemble(args):
         "config": {"workload": ENSEMBLE, "runs": len(cfgs), "cells": ncell,
                    "nx": [64, 96, 128], "nz": [128, 256, 512], "aspect": 2, "Ra": [1e6, 1e9], "Pr": [1, 7],
                    "cflmax": 0.5, "dt0": 1e-4, "dtmax": 1e-2, "graph": True, "tuning": best},
-        "roofline": {"bound": "hbm", "kernel": "k_ens_substep", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm",
+                     "kernel": ("substep: k_ensp_factors, k_ensp_columns, k_ensp_transform x2, k_ensp_modes, "
+                                "k_ensp_project, k_ensp_finish" if best.get("phased") else "k_ens_substep"),
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": int(per_launch)},
         "host_wall_s": round(wall, 4),
