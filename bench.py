@@ -380,7 +380,17 @@ def run_form(wl, dt, form, args):
     return res
 
 
-def roofline(wl, res, form):
+# HBM bytes per launch of a dominant kernel group, measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE (one pass each, bench.py --form F --steps 1),
+# scaled by the same counters on a 2 GiB copy (fetch x2.0, write x1.0, the
+# --copy-calibration run).  Only valid for the kernels as measured: re-measure
+# after changing them.
+MEASURED_TRAFFIC = {
+    ("rbc_multires_384x768x768_768x1536x1536", "mixed", "column_sal"): 53.64e9,
+}
+
+
+def roofline(wl, res, form, workload=None):
     """Roofline of the dominant kernel group: algorithmic bytes per launch
     over the average launch time (HIP events on the library's stream)."""
     nsst = wl["nsst"]
@@ -397,8 +407,9 @@ def roofline(wl, res, form):
     per_launch = kb[dom] / nl
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     step_bytes = sum(kb.values()) * nsst
+    traffic = MEASURED_TRAFFIC.get((workload, form, dom))
     return ({"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
              "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(per_launch)},
             round(step_bytes / (min(res["step_ms"]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
 
@@ -448,11 +459,11 @@ def main():
     head = runs["mixed"] if "mixed" in runs else runs["fp64"]
     hform = head["form"]
     value = ncell * args.steps / head["total_s"] / 1e6
-    rl, step_frac = roofline(wl, head, hform)
+    rl, step_frac = roofline(wl, head, hform, args.workload)
 
     summary = {}
     for f, r in runs.items():
-        rlf, sf = roofline(wl, r, f)
+        rlf, sf = roofline(wl, r, f, args.workload)
         summary[f] = {"value": round(ncell * args.steps / r["total_s"] / 1e6, 2),
                       "ms_per_step": round(r["total_s"] * 1e3 / args.steps, 3),
                       "min_ms_per_step": round(min(r["step_ms"]), 3),
