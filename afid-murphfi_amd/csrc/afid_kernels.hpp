@@ -729,12 +729,41 @@ constexpr int VXR_TJ = 16, VXR_TI = 4;      // refined columns per vxr tile
 constexpr int VYZ_LINES = 4;                // refined lines (waves) per vyr/vzr tile
 constexpr int VYZ_CHUNK = 16;               // refined points per scan chunk
 
+// Inclusive prefix sum over the 64 lanes of a wavefront with DPP lane
+// moves: Kogge-Stone within each row of 16 lanes (row_shr 1, 2, 4, 8),
+// then the row totals carried forward (row_bcast 15 into rows 1 and 3,
+// row_bcast 31 into rows 2 and 3).  No LDS traffic.
+template <int CTRL, int ROWS, bool BC>
+__device__ __forceinline__ int dpp32(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, BC); }
+template <int CTRL, int ROWS, bool BC>
+__device__ __forceinline__ float dpp_move(float v) { return __builtin_bit_cast(float, dpp32<CTRL, ROWS, BC>(__builtin_bit_cast(int, v))); }
+template <int CTRL, int ROWS, bool BC>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = dpp32<CTRL, ROWS, BC>(int(b & 0xffffffffLL)), hi = dpp32<CTRL, ROWS, BC>(int(b >> 32));
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+    v += dpp_move<0x111, 0xf, true>(v);      // row_shr:1
+    v += dpp_move<0x112, 0xf, true>(v);      // row_shr:2
+    v += dpp_move<0x114, 0xf, true>(v);      // row_shr:4
+    v += dpp_move<0x118, 0xf, true>(v);      // row_shr:8
+    v += dpp_move<0x142, 0xa, false>(v);     // row_bcast:15
+    v += dpp_move<0x143, 0xc, false>(v);     // row_bcast:31
+    return v;
+}
+
 // vxr: tile of VXR_TJ x VXR_TI refined columns, x processed in chunks of 64
 // refined points (one per lane); xcmax/cjmax/cimax bound the coarse block.
-// RT: storage and arithmetic type of the transfer (double or float).
+// RT: storage and arithmetic type of the transfer (double or float).  The
+// tile's coarse column offsets and its y / z stencils are staged in LDS once.
 template <typename RT>
 __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, int cimax) {
     extern __shared__ __align__(16) unsigned char smraw[];
+    __shared__ long long coff[16 * 16];                 // coarse column offsets [ci][cj] (cimax, cjmax <= 16)
+    __shared__ RT yw[VXR_TJ][4], zw[VXR_TI][4];
+    __shared__ int yo[VXR_TJ], zo[VXR_TI];
     RT *sm = reinterpret_cast<RT *>(smraw);
     RT *vxr = sel<RT>(p.vxr, p.vxrf);
     const RT *xs_w = sel<RT>(p.xs_w, p.xs_wf), *ys_w = sel<RT>(p.ys_w, p.ys_wf), *zs_w = sel<RT>(p.zs_w, p.zs_wf);
@@ -746,64 +775,82 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
     RT *B = sm;                                       // [ci][cj][xc]
     RT *Z = B + xcmax * cjmax * cimax;                // [ti][cj][xc]
     RT *Q = Z + xcmax * cjmax * VXR_TI;               // [ti][tj][xc]
-    RT *carry = Q + xcmax * VXR_TJ * VXR_TI;          // [tj*ti]
+    RT *carry = Q + xcmax * VXR_TJ * VXR_TI;          // [ti][tj]
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int ncol = nj * ni;
-    for (int q = t; q < ncol; q += 256) {
-        carry[q] = RT(0);
-        vxr[ridx(p, 0, J0 + q % nj, I0 + q / nj)] = RT(0);      // vxr(1) = 0
+    for (int q = t; q < CI * CJ; q += 256) {
+        const int c = q / CJ, b = q - c * CJ;
+        coff[c * CJ + b] = (long long)wrapf(ci0 + c, p.nz) * p.plane + (long long)wrapf(cj0 + b, p.ny) * p.ldx;
     }
+    if (t < nj) {
+        yo[t] = p.ys_src[J0 + t] - 1 - cj0;
+        for (int a = 0; a < 4; ++a) yw[t][a] = ys_w[4 * (J0 + t) + a];
+    }
+    if (t < ni) {
+        zo[t] = p.zs_src[I0 + t] - 1 - ci0;
+        for (int a = 0; a < 4; ++a) zw[t][a] = zs_w[4 * (I0 + t) + a];
+    }
+    for (int q = t; q < nj * ni; q += 256) {
+        const int ti = q / nj, tj = q - ti * nj;
+        carry[ti * VXR_TJ + tj] = RT(0);
+        vxr[ridx(p, 0, J0 + tj, I0 + ti)] = RT(0);      // vxr(1) = 0
+    }
+    const int sS = 1 - 2 * p.inslws, sN = 1 - 2 * p.inslwn;
     for (int R0 = 0; R0 < p.nxmr; R0 += 64) {
         const int R1 = min(R0 + 64, p.nxmr);
         const int x0 = p.xs_src[R0], XC = p.xs_src[R1 - 1] + 4 - x0;
         __syncthreads();
         {
+            // coarse gradient dvx/dx at reference x index xc with the wall
+            // ghosts (InterpVelMgrd.F90:37-51); zero outside -1..nx+1
             Walk3 w(t, 256, XC, CJ);
-            for (int q = t; q < XC * CJ * CI; q += 256, w.next())
-                B[(w.c * cjmax + w.b) * xcmax + w.a] = mgrd_grad<RT>(p, 0, x0 + w.a, wrapf(cj0 + w.b, p.ny), wrapf(ci0 + w.c, p.nz));
+            for (int q = t; q < XC * CJ * CI; q += 256, w.next()) {
+                const int xc = x0 + w.a;
+                RT g = RT(0);
+                if (xc >= 0 && xc <= p.nx) {
+                    const int k = xc == 0 ? 0 : (xc == p.nx ? p.nxm - 1 : xc - 1);
+                    const double *col = p.vx + coff[w.c * CJ + w.b];
+                    const double d = (col[k + 1] - col[k]) * p.udx3m[k];
+                    g = RT(xc == 0 ? sS * d : (xc == p.nx ? sN * d : d));
+                }
+                B[(w.c * cjmax + w.b) * xcmax + w.a] = g;
+            }
         }
         __syncthreads();
         Walk3 wz(t, 256, XC, CJ);
         for (int q = t; q < XC * CJ * ni; q += 256, wz.next()) {
             const int xc = wz.a, cj = wz.b, ti = wz.c;
-            const int c0 = p.zs_src[I0 + ti] - 1 - ci0;
-            const RT *w = zs_w + 4 * (I0 + ti), *b = B + (c0 * cjmax + cj) * xcmax + xc;
+            const RT *b = B + (zo[ti] * cjmax + cj) * xcmax + xc;
             const int st = cjmax * xcmax;
-            Z[(ti * cjmax + cj) * xcmax + xc] = b[0] * w[0] + b[st] * w[1] + b[2 * st] * w[2] + b[3 * st] * w[3];
+            Z[(ti * cjmax + cj) * xcmax + xc] = b[0] * zw[ti][0] + b[st] * zw[ti][1] + b[2 * st] * zw[ti][2] + b[3 * st] * zw[ti][3];
         }
         __syncthreads();
         Walk3 wy(t, 256, XC, nj);
         for (int q = t; q < XC * nj * ni; q += 256, wy.next()) {
             const int xc = wy.a, tj = wy.b, ti = wy.c;
-            const int b0 = p.ys_src[J0 + tj] - 1 - cj0;
-            const RT *w = ys_w + 4 * (J0 + tj), *z = Z + (ti * cjmax + b0) * xcmax + xc;
-            Q[(ti * VXR_TJ + tj) * xcmax + xc] = z[0] * w[0] + z[xcmax] * w[1] + z[2 * xcmax] * w[2] + z[3 * xcmax] * w[3];
+            const RT *z = Z + (ti * cjmax + yo[tj]) * xcmax + xc;
+            Q[(ti * VXR_TJ + tj) * xcmax + xc] = z[0] * yw[tj][0] + z[xcmax] * yw[tj][1] + z[2 * xcmax] * yw[tj][2] + z[3 * xcmax] * yw[tj][3];
         }
         __syncthreads();
         const int kr = R0 + lane;
+        const bool in = kr < R1;
         int xa = 0;
         RT wx0 = 0, wx1 = 0, wx2 = 0, wx3 = 0, lx = 0;
-        if (kr < R1) {
+        if (in) {
             xa = p.xs_src[kr] - x0;
             wx0 = xs_w[4 * kr]; wx1 = xs_w[4 * kr + 1]; wx2 = xs_w[4 * kr + 2]; wx3 = xs_w[4 * kr + 3];
             lx = lxa[kr];
         }
-        for (int col = wave; col < ncol; col += 4) {
-            const int tj = col % nj, ti = col / nj;
-            RT v = RT(0);
-            if (kr < R1) {
+        for (int ti = 0; ti < ni; ++ti) {
+            RT *orow = vxr + (long long)(I0 + ti) * p.planer + (long long)J0 * p.ldxr + kr + 1;
+            for (int tj = wave; tj < nj; tj += 4) {
                 const RT *qq = Q + (ti * VXR_TJ + tj) * xcmax + xa;
-                v = (qq[0] * wx0 + qq[1] * wx1 + qq[2] * wx2 + qq[3] * wx3) * lx;
+                RT v = in ? (qq[0] * wx0 + qq[1] * wx1 + qq[2] * wx2 + qq[3] * wx3) * lx : RT(0);
+                v = wave_incl_scan(v);
+                const RT base = carry[ti * VXR_TJ + tj];
+                if (in) orow[(long long)tj * p.ldxr] = base + v;
+                const RT tot = __shfl(v, 63, 64);
+                if (lane == 0) carry[ti * VXR_TJ + tj] = base + tot;
             }
-            // inclusive prefix over the 64 lanes
-            for (int off = 1; off < 64; off <<= 1) {
-                const RT u = __shfl_up(v, off, 64);
-                if (lane >= off) v += u;
-            }
-            const RT base = carry[col];
-            if (kr < R1) vxr[ridx(p, kr + 1, J0 + tj, I0 + ti)] = base + v;
-            const RT tot = __shfl(v, 63, 64);
-            if (lane == 0) carry[col] = base + tot;
         }
     }
 }
@@ -865,26 +912,53 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
     RT *B = sm;                                               // [co][cs][xc]
     RT *X = B;                                                // [cs][line][lane], reuses B
     RT *O = B + max(xcmax * csmax * comax, csmax * VYZ_LINES * 64);   // [line][cs][xc]
+    // coarse offsets of the tile: the other axis (fixed) and, per chunk, the
+    // scan axis and its periodic successor; the lines' stencils
+    __shared__ long long ooff[32], soff[32], snext[32];
+    __shared__ RT lw[VYZ_LINES][4];
+    __shared__ int lo[VYZ_LINES];
+    const long long ostride = AX == 1 ? p.plane : (long long)p.ldx, sstr = AX == 1 ? (long long)p.ldx : p.plane;
+    for (int q = t; q < CO; q += 256) ooff[q] = (long long)wrapf(co0 + q, cO) * ostride;
+    if (t < nl) {
+        lo[t] = sO[O0 + t] - 1 - co0;
+        for (int a = 0; a < 4; ++a) lw[t][a] = wO[4 * (O0 + t) + a];
+    }
+    const double *vel = AX == 1 ? p.vy : p.vz;
+    const double dgrid = AX == 1 ? p.dy : p.dz;
+    const int sSg = 1 - 2 * p.inslws, sNg = 1 - 2 * p.inslwn;
     for (int S0 = 0; S0 < nS - 1; S0 += VYZ_CHUNK) {
         const int S1 = min(S0 + VYZ_CHUNK, nS - 1);
         const int cs0 = sS[S0] - 1, CS = sS[S1 - 1] - 1 + 4 - cs0;
         __syncthreads();
+        for (int q = t; q < CS; q += 256) {
+            const int js = wrapf(cs0 + q, cS);
+            soff[q] = (long long)js * sstr;
+            snext[q] = (long long)wrapp(js, cS) * sstr;
+        }
+        __syncthreads();
         {
+            // coarse gradient d(vy)/dy or d(vz)/dz at reference x index xc
+            // with the wall ghosts (InterpVelMgrd.F90:75-86, :161-172)
             Walk3 w(t, 256, XC, CS);
             for (int q = t; q < XC * CS * CO; q += 256, w.next()) {
-                const int js = wrapf(cs0 + w.b, cS), jo = wrapf(co0 + w.c, cO);
-                B[(w.c * csmax + w.b) * xcmax + w.a] =
-                    AX == 1 ? mgrd_grad<RT>(p, 1, x0 + w.a, js, jo) : mgrd_grad<RT>(p, 2, x0 + w.a, jo, js);
+                const int xc = x0 + w.a;
+                RT g = RT(0);
+                if (xc >= 0 && xc <= p.nx) {
+                    const int k = xc == 0 ? 0 : (xc == p.nx ? p.nxm - 1 : xc - 1);
+                    const double *cb = vel + ooff[w.c] + k;
+                    const double d = (cb[snext[w.b]] - cb[soff[w.b]]) * dgrid;
+                    g = RT(xc == 0 ? sSg * d : (xc == p.nx ? sNg * d : d));
+                }
+                B[(w.c * csmax + w.b) * xcmax + w.a] = g;
             }
         }
         __syncthreads();
         Walk3 wo(t, 256, XC, CS);
         for (int q = t; q < XC * CS * nl; q += 256, wo.next()) {
             const int xc = wo.a, cs = wo.b, l = wo.c;
-            const int c0 = sO[O0 + l] - 1 - co0;
-            const RT *w = wO + 4 * (O0 + l), *b = B + (c0 * csmax + cs) * xcmax + xc;
+            const RT *b = B + (lo[l] * csmax + cs) * xcmax + xc;
             const int st = csmax * xcmax;
-            O[(l * csmax + cs) * xcmax + xc] = b[0] * w[0] + b[st] * w[1] + b[2 * st] * w[2] + b[3 * st] * w[3];
+            O[(l * csmax + cs) * xcmax + xc] = b[0] * lw[l][0] + b[st] * lw[l][1] + b[2 * st] * lw[l][2] + b[3 * st] * lw[l][3];
         }
         __syncthreads();
         if (live && kr < R1)
