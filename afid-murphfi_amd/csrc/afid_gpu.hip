@@ -813,6 +813,7 @@ static int setup(afid_ctx *c) {
         std::vector<int> xb0(xb_src.begin(), xb_src.begin() + nxm);
         c->salc_xr = span(xb0, nxm, 64);
         c->salc_yr = span(yb_src, c->ny, SALC_TJ);
+        if (c->salc_yr > 64) return set_err(c, AFID_E_ARG, "refinement ratio too large for the refined -> coarse tiles");
         c->lds_salc = es * size_t(2 * c->salc_yr * c->salc_xr);
         if (c->pf) {
             std::vector<int> xp0(xp_src.begin(), xp_src.begin() + nxm);

@@ -1051,13 +1051,23 @@ __global__ void __launch_bounds__(256) k_interp_coarse_tile(KP p, CoarseJob jb, 
     }
     RT ring[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};   // contracted refined planes zr-3..zr
     const int plane = XR * YR;
+    // refined y rows of the tile (periodic): their indices and row offsets
+    __shared__ int rowj[64];
+    for (int q = t; q < YR; q += 256) rowj[q] = wrapf(y0 + q, p.nyr);
+    __syncthreads();
     int ic = I0;
     for (int zr = zfirst; zr <= zlast; ++zr) {
         RT *B = sm + ((zr - zfirst) & 1) * xrmax * yrmax;     // [yr][xr], double buffered
         const int ii = wrapn(zr, p.nzr);
+        const RT *fz = src + (long long)ii * p.planer;
         Walk3 w(t, 256, XR, YR);
-        for (int q = t; q < plane; q += 256, w.next())
-            B[w.b * xrmax + w.a] = coarse_src<RT>(p, jb, src, x0 + w.a, wrapf(y0 + w.b, p.nyr), ii);
+        for (int q = t; q < plane; q += 256, w.next()) {
+            const int kr = x0 + w.a;
+            RT v;
+            if (kr >= 1 && kr <= p.nxmr) v = fz[(long long)rowj[w.b] * p.ldxr + kr - 1];   // interior (most points)
+            else v = coarse_src<RT>(p, jb, src, kr, rowj[w.b], ii);                       // wall ghosts, beyond
+            B[w.b * xrmax + w.a] = v;
+        }
         __syncthreads();
         for (int h = 0; h < 2; ++h) {
             RT v = RT(0);

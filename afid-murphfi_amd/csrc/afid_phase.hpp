@@ -184,6 +184,9 @@ __device__ __forceinline__ double temp_ghosted(const KP &p, int xc, int j, int i
 template <typename RT>
 __global__ void __launch_bounds__(256) k_interp_temp_tile(KP p, int xcmax, int cjmax, int cimax) {
     extern __shared__ __align__(16) unsigned char smraw[];
+    __shared__ int cjs[16], cis[16];                  // the tile's coarse rows (periodic), cjmax, cimax <= 16
+    __shared__ RT yw[VXR_TJ][4], zw[VXR_TI][4];
+    __shared__ int yo[VXR_TJ], zo[VXR_TI];
     RT *sm = reinterpret_cast<RT *>(smraw);
     RT *tempr = sel<RT>(p.tempr, p.temprf);
     const RT *xs_w = sel<RT>(p.xs_w, p.xs_wf), *ys_w = sel<RT>(p.ys_w, p.ys_wf), *zs_w = sel<RT>(p.zs_w, p.zs_wf);
@@ -195,7 +198,16 @@ __global__ void __launch_bounds__(256) k_interp_temp_tile(KP p, int xcmax, int c
     RT *Z = B + xcmax * cjmax * cimax;                // [ti][cj][xc]
     RT *Q = Z + xcmax * cjmax * VXR_TI;               // [ti][tj][xc]
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int ncol = nj * ni;
+    if (t < CJ) cjs[t] = wrapf(cj0 + t, p.ny);
+    if (t >= 32 && t - 32 < CI) cis[t - 32] = wrapf(ci0 + t - 32, p.nz);
+    if (t < nj) {
+        yo[t] = p.ys_src[J0 + t] - 1 - cj0;
+        for (int a = 0; a < 4; ++a) yw[t][a] = ys_w[4 * (J0 + t) + a];
+    }
+    if (t >= 64 && t - 64 < ni) {
+        zo[t - 64] = p.zs_src[I0 + t - 64] - 1 - ci0;
+        for (int a = 0; a < 4; ++a) zw[t - 64][a] = zs_w[4 * (I0 + t - 64) + a];
+    }
     for (int R0 = 0; R0 < p.nxmr; R0 += 64) {
         const int R1 = min(R0 + 64, p.nxmr);
         const int x0 = p.xs_src[R0], XC = p.xs_src[R1 - 1] + 4 - x0;
@@ -203,37 +215,39 @@ __global__ void __launch_bounds__(256) k_interp_temp_tile(KP p, int xcmax, int c
         {
             Walk3 w(t, 256, XC, CJ);
             for (int q = t; q < XC * CJ * CI; q += 256, w.next())
-                B[(w.c * cjmax + w.b) * xcmax + w.a] = RT(temp_ghosted(p, x0 + w.a, wrapf(cj0 + w.b, p.ny), wrapf(ci0 + w.c, p.nz)));
+                B[(w.c * cjmax + w.b) * xcmax + w.a] = RT(temp_ghosted(p, x0 + w.a, cjs[w.b], cis[w.c]));
         }
         __syncthreads();
         Walk3 wz(t, 256, XC, CJ);
         for (int q = t; q < XC * CJ * ni; q += 256, wz.next()) {
             const int xc = wz.a, cj = wz.b, ti = wz.c;
-            const int c0 = p.zs_src[I0 + ti] - 1 - ci0;
-            const RT *w = zs_w + 4 * (I0 + ti), *b = B + (c0 * cjmax + cj) * xcmax + xc;
+            const RT *b = B + (zo[ti] * cjmax + cj) * xcmax + xc;
             const int st = cjmax * xcmax;
-            Z[(ti * cjmax + cj) * xcmax + xc] = b[0] * w[0] + b[st] * w[1] + b[2 * st] * w[2] + b[3 * st] * w[3];
+            Z[(ti * cjmax + cj) * xcmax + xc] = b[0] * zw[ti][0] + b[st] * zw[ti][1] + b[2 * st] * zw[ti][2] + b[3 * st] * zw[ti][3];
         }
         __syncthreads();
         Walk3 wy(t, 256, XC, nj);
         for (int q = t; q < XC * nj * ni; q += 256, wy.next()) {
             const int xc = wy.a, tj = wy.b, ti = wy.c;
-            const int b0 = p.ys_src[J0 + tj] - 1 - cj0;
-            const RT *w = ys_w + 4 * (J0 + tj), *z = Z + (ti * cjmax + b0) * xcmax + xc;
-            Q[(ti * VXR_TJ + tj) * xcmax + xc] = z[0] * w[0] + z[xcmax] * w[1] + z[2 * xcmax] * w[2] + z[3 * xcmax] * w[3];
+            const RT *z = Z + (ti * cjmax + yo[tj]) * xcmax + xc;
+            Q[(ti * VXR_TJ + tj) * xcmax + xc] = z[0] * yw[tj][0] + z[xcmax] * yw[tj][1] + z[2 * xcmax] * yw[tj][2] + z[3 * xcmax] * yw[tj][3];
         }
         __syncthreads();
         const int kr = R0 + lane;
         if (kr < R1) {
             const int xa = p.xs_src[kr] - x0;
             const RT *wx = xs_w + 4 * kr;
-            for (int cl = wave; cl < ncol; cl += 4) {
-                const int tj = cl % nj, ti = cl / nj;
-                const RT *qq = Q + (ti * VXR_TJ + tj) * xcmax + xa;
-                tempr[ridx(p, kr, J0 + tj, I0 + ti)] = qq[0] * wx[0] + qq[1] * wx[1] + qq[2] * wx[2] + qq[3] * wx[3];
+            const RT w0 = wx[0], w1 = wx[1], w2 = wx[2], w3 = wx[3];
+            for (int ti = 0; ti < ni; ++ti) {
+                RT *orow = tempr + (long long)(I0 + ti) * p.planer + (long long)J0 * p.ldxr + kr;
+                for (int tj = wave; tj < nj; tj += 4) {
+                    const RT *qq = Q + (ti * VXR_TJ + tj) * xcmax + xa;
+                    orow[(long long)tj * p.ldxr] = qq[0] * w0 + qq[1] * w1 + qq[2] * w2 + qq[3] * w3;
+                }
             }
         }
     }
+    const int ncol = nj * ni;
     // the unused index nxr of tempr is zero (phasefield.F90:502)
     for (int cl = t; cl < ncol; cl += 256) tempr[ridx(p, p.nxmr, J0 + cl % nj, I0 + cl / nj)] = RT(0);
 }
