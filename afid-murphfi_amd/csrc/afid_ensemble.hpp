@@ -195,6 +195,118 @@ __device__ __forceinline__ void ens_factors(const KP &p, int which, double *f) {
                                f + 2 * NP, f + 3 * NP);
 }
 
+// Explicit terms of a 2-D run (ny = 1, v = 0): the stencils of
+// ExplicitTermsVX.F90:42-127, ExplicitTermsVZ.F90:45-114 and
+// ExplicitTermsTemp.F90:39-77 with every y term dropped.  With one periodic
+// y cell each y difference of the 3-D stencils is an exact zero (and v is
+// identically zero), so the arithmetic that remains is the same.
+// Column (i) pointers: c = i, m = i - 1, p = i + 1 (periodic).
+struct Nb2 {
+    const double *vx, *vx_m, *vx_p, *vz, *vz_m, *vz_p, *t, *t_m, *t_p, *pr, *pr_m;
+    double tbp, ttp;
+};
+__device__ __forceinline__ Nb2 make_nb2(const KP &p, int i) {
+    const long long c = (long long)i * p.plane, m = (long long)wrapm(i, p.nz) * p.plane, q = (long long)wrapp(i, p.nz) * p.plane;
+    Nb2 n;
+    n.vx = p.vx + c; n.vx_m = p.vx + m; n.vx_p = p.vx + q;
+    n.vz = p.vz + c; n.vz_m = p.vz + m; n.vz_p = p.vz + q;
+    n.t = p.temp + c; n.t_m = p.temp + m; n.t_p = p.temp + q;
+    n.pr = p.pr + c; n.pr_m = p.pr + m;
+    n.tbp = p.tempbp[i];
+    n.ttp = p.temptp[i];
+    return n;
+}
+__device__ __forceinline__ double ex_vx_2d(const KP &p, const Nb2 &n, unsigned k) {
+    const double udz = p.dz * 0.25, udzq = p.dzq / p.ren;
+    const double rkm = p.rudx3m[k - 1], rkc = p.rudx3m[k], u3 = p.udx3c[k];
+    const double vxc = n.vx[k], vxzp = n.vx_p[k], vxzm = n.vx_m[k];
+    const double hxz = u3 * udz * ((rkm * n.vz_p[k - 1] + rkc * n.vz_p[k]) * (vxzp + vxc)
+                                   - (rkm * n.vz[k - 1] + rkc * n.vz[k]) * (vxc + vxzm));
+    const double a = n.vx[k + 1] + vxc, b = vxc + n.vx[k - 1];
+    const double hxx = (a * a - b * b) * u3 * 0.25;
+    const double dzzvx = (vxzm - 2.0 * vxc + vxzp) * udzq;
+    double q = -(hxx + hxz) + dzzvx;
+    if (p.gaxis == 1) q += p.byct * (double(p.active_t) * 0.5 * (n.t[k] + n.t[k - 1]));
+    return q;
+}
+__device__ __forceinline__ double ex_vz_2d(const KP &p, const Nb2 &n, unsigned k) {
+    const double udzq = p.dzq / p.ren, udz = p.dz * 0.25;
+    const unsigned kmm = k == 0 ? 0 : k - 1, kpp = k == unsigned(p.nxm - 1) ? k : k + 1;
+    const double vzc = n.vz[k], vzzp = n.vz_p[k], vzzm = n.vz_m[k];
+    const double a = vzzp + vzc, b = vzzm + vzc;
+    const double hzz = (a * a - b * b) * udz;
+    const double hzx = ((n.vx[k + 1] + n.vx_m[k + 1]) * (n.vz[kpp] + vzc)
+                        - (n.vx[k] + n.vx_m[k]) * (vzc + n.vz[kmm])) * p.udx3m[k] * 0.25;
+    const double dzzvz = (vzzp - 2.0 * vzc + vzzm) * udzq;
+    double q = -(hzx + hzz) + dzzvz;
+    if (p.gaxis == 3) q += p.byct * (double(p.active_t) * 0.5 * (n.t[k] + n.t_m[k]));
+    return q;
+}
+__device__ __forceinline__ double ex_temp_2d(const KP &p, const Nb2 &n, unsigned k) {
+    const double udz = p.dz * 0.5, udzq = p.dzq / p.pect;
+    const double tc = n.t[k], tzp = n.t_p[k], tzm = n.t_m[k];
+    const double vxp = n.vx[k + 1], vxc = n.vx[k];
+    double htx;
+    if (k == 0) htx = (vxp * (n.t[k + 1] + tc) - vxc * 2.0 * n.tbp) * 0.5 * p.udx3m[k];
+    else if (k == unsigned(p.nxm - 1)) htx = (vxp * 2.0 * n.ttp - vxc * (tc + n.t[k - 1])) * 0.5 * p.udx3m[k];
+    else htx = (vxp * (n.t[k + 1] + tc) - vxc * (tc + n.t[k - 1])) * 0.5 * p.udx3m[k];
+    const double htz = (n.vz_p[k] * (tzp + tc) - n.vz[k] * (tc + tzm)) * udz;
+    const double dzzt = (tzp - 2.0 * tc + tzm) * udzq;
+    return -(htx + htz) + dzzt;
+}
+
+// stencil phase of field FLD (0 vx, 2 vz, 3 T) of a 2-D column: RHS into buf
+// and the old value into own (rows in the solver's LDS layout), new explicit
+// terms into the history (ImplicitAndUpdateVX/VZ/Temp.F90 RHS)
+template <int FLD, int CH>
+__device__ __forceinline__ void column_2d_rows(const KP &p, const Nb2 &n, long long c0, double *buf, double *own,
+                                               unsigned lane) {
+    double *ru = (FLD == 0 ? p.rux : FLD == 2 ? p.ruz : p.rut) + c0;
+    const unsigned nx = p.nx, nxm = p.nxm;
+#pragma unroll 1
+    for (unsigned r = lane; r < 64u * CH; r += 64) {
+        double rhs = 0.0, old = 0.0;
+        if (r < nx) {
+            if (FLD == 0) {
+                old = n.vx[r];
+                if (r >= 1 && r < nxm) {
+                    const double h = ex_vx_2d(p, n, r);
+                    const double dxx = n.vx[r + 1] * p.ap3ck[r] + n.vx[r] * p.ac3ck[r] + n.vx[r - 1] * p.am3ck[r];
+                    const double dxp = (n.pr[r] - n.pr[r - 1]) * (p.al * p.udx3c[r]);
+                    rhs = (p.ga * h + p.ro * ru[r] + (p.al / p.ren) * dxx - dxp) * p.dt;
+                    ru[r] = h;
+                }
+            } else if (FLD == 2) {
+                old = n.vz[r];
+                if (r < nxm) {
+                    const double h = ex_vz_2d(p, n, r);
+                    const unsigned kmm = r == 0 ? 0 : r - 1, kpp = r == nxm - 1 ? r : r + 1;
+                    const double dxx = n.vz[kpp] * p.ap3sk[r] + n.vz[r] * p.ac3sk[r] + n.vz[kmm] * p.am3sk[r];
+                    const double dzp = (n.pr[r] - n.pr_m[r]) * p.dz * p.al;
+                    rhs = (p.ga * h + p.ro * ru[r] + (p.al / p.ren) * dxx - dzp) * p.dt;
+                    ru[r] = h;
+                }
+            } else {
+                old = n.t[r];
+                if (r < nxm) {
+                    const double h = ex_temp_2d(p, n, r);
+                    double dxx;
+                    if (r == 0)
+                        dxx = n.t[r + 1] * p.ap3ssk[r] + n.t[r] * p.ac3ssk[r] - (p.ap3ssk[r] + p.ac3ssk[r]) * n.tbp * p.tfixs;
+                    else if (r == nxm - 1)
+                        dxx = n.t[r] * p.ac3ssk[r] + n.t[r - 1] * p.am3ssk[r] - (p.am3ssk[r] + p.ac3ssk[r]) * n.ttp * p.tfixn;
+                    else
+                        dxx = n.t[r + 1] * p.ap3ssk[r] + n.t[r] * p.ac3ssk[r] + n.t[r - 1] * p.am3ssk[r];
+                    rhs = (p.ga * h + p.ro * ru[r] + (p.al / p.pect) * dxx) * p.dt;
+                    ru[r] = h;
+                }
+            }
+        }
+        buf[lds_row<CH>(r)] = rhs;
+        own[lds_row<CH>(r)] = old;
+    }
+}
+
 // column i of the run by one wavefront: vx, vz, T in turn, new values into
 // buffer set cur ^ 1; fac holds the three factor sets, buf/own this
 // wavefront's LDS columns
@@ -203,14 +315,14 @@ __device__ __forceinline__ void ens_column(const EnsRun &R, const KP &p, int cur
                                            double *own, unsigned lane) {
     constexpr int NP = 64 * CH;
     const int nx = p.nx, nxm = p.nxm;
-    const Nb n = make_nb(p, 0, i);
+    const Nb2 n = make_nb2(p, i);
     const long long c0 = (long long)i * p.plane;
 #pragma unroll 1
     for (int fi = 0; fi < 3; ++fi) {
         switch (fi) {
-        case 0: column_base_rows<0, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
-        case 1: column_base_rows<2, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
-        default: column_base_rows<3, CH, double, false>(p, n, c0, buf, own, buf, lane, NP, true); break;
+        case 0: column_2d_rows<0, CH>(p, n, c0, buf, own, lane); break;
+        case 1: column_2d_rows<2, CH>(p, n, c0, buf, own, lane); break;
+        default: column_2d_rows<3, CH>(p, n, c0, buf, own, lane); break;
         }
         wave_sync();
         const double *f = fac + fi * 4 * NP;
