@@ -1114,9 +1114,10 @@ static int poisson_mixed(afid_ctx *c) {
     {
         KP p = make_kp(c);
         tstart(c, "project");
-        hipLaunchKernelGGL(k_project_res, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p,
-                           c->alt[0], c->alt[1], c->alt[2], c->redsum);
-        hipLaunchKernelGGL(k_max_partials, dim3(256), dim3(256), 0, c->stream, c->redsum, c->nred, c->slots + 1);
+        const dim3 gr = point_grid(c->nxm, c->ny, (c->nz + PROJ_ZC - 1) / PROJ_ZC);
+        hipLaunchKernelGGL(k_project_res, gr, kPointBlock, 0, c->stream, p, c->alt[0], c->alt[1], c->alt[2], c->redsum);
+        hipLaunchKernelGGL(k_max_partials, dim3(256), dim3(256), 0, c->stream, c->redsum, (long long)gr.x * gr.y * gr.z,
+                           c->slots + 1);
         tstop(c);
         if ((rc = check_launch(c, "k_project_res"))) return rc;
         std::swap(c->f[AFID_VX], c->alt[0]);

@@ -381,8 +381,13 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_
     using V = typename Pair<RT>::T;
     constexpr int M2 = (64 * CH + 127) / 128;          // pair iterations covering the solver's 64*CH rows
     __shared__ double lds[4][lds_col_size<CH>()];
+    // the solver's factors, read by every column of the workgroup
+    __shared__ double fac[4][64 * CH];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
+    for (int q = threadIdx.x; q < 64 * CH; q += 256) { fac[0][q] = f.l[q]; fac[1][q] = f.e[q]; fac[2][q] = f.g[q]; fac[3][q] = f.s[q]; }
+    TriFactors fl = f;
+    fl.l = fac[0]; fl.e = fac[1]; fl.g = fac[2]; fl.s = fac[3];
     int j, i;
     const bool live = strip_column(xcd_block(0), wave, p.nyr, p.nzr, j, i);
     double *buf = lds[wave];
@@ -424,7 +429,7 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_
         if (r1 < 64u * CH) buf[lds_row<CH>(r1)] = rhs1;
     }
     __syncthreads();
-    column_solve_lds<CH>(buf, f);
+    column_solve_lds<CH>(buf, fl);
     __syncthreads();
     if (!live) return;
     RT *out = sal_out + c0;
@@ -618,43 +623,54 @@ __global__ void __launch_bounds__(256) k_project(KP p, int do_vel, int do_pr, Ga
 // (u_out), so that every cell can also form the divergence the corrected
 // field will have (its own and its +1 neighbours' faces, from the old
 // velocity and dph): the FP64 residual of the mixed form's iterative
-// refinement, max-reduced into *slot (CheckDivergence units).
+// refinement, max-reduced into *slot (CheckDivergence units).  Each thread
+// marches along z over PROJ_ZC planes and carries dph and vz of the planes
+// i-1, i, i+1 in registers, so no plane is read twice.
+constexpr int PROJ_ZC = 8;
 __global__ void __launch_bounds__(256) k_project_res(KP p, double *vx_out, double *vy_out, double *vz_out,
                                                      double *partial) {
-    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, i = blockIdx.z;
+    const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int i0 = blockIdx.z * PROJ_ZC, i1 = min(i0 + PROJ_ZC, p.nz);
     double dvm = 0.0;
     if (k < p.nxm && j < p.ny) {
-        const int jm = wrapm(j, p.ny), jp = wrapp(j, p.ny), im = wrapm(i, p.nz), ip = wrapp(i, p.nz);
+        const int jm = wrapm(j, p.ny), jp = wrapp(j, p.ny);
         const int km = k == 0 ? 0 : k - 1, kp = k == p.nxm - 1 ? k : k + 1;
         const double *d = p.dph;
-        const long long c = bidx(p, k, j, i);
-        const long long cjm = bidx(p, k, jm, i), cjp = bidx(p, k, jp, i), cim = bidx(p, k, j, im), cip = bidx(p, k, j, ip);
-        // every load ahead of the stores (the outputs may alias the inputs
-        // as far as the compiler knows)
-        const double dc = d[c], dkm = d[bidx(p, km, j, i)], dkp = d[bidx(p, kp, j, i)];
-        const double djm = d[cjm], djp = d[cjp], dim = d[cim], dip = d[cip];
-        const double vxc = p.vx[c], vyc = p.vy[c], vzc = p.vz[c];
-        const double vx1 = p.vx[c + 1], vy1 = p.vy[cjp], vz1 = p.vz[cip];
-        const double d1 = d[c + 1], prc = p.pr[c];
-        const double adt = p.al * p.dt;
-        // corrected faces of this cell (as k_project)
-        const double ux = vxc - (dc - dkm) * (adt * p.udx3c[k]);
-        const double uy = vyc - (dc - djm) * (adt * p.dy);
-        const double uz = vzc - (dc - dim) * (adt * p.dz);
-        // corrected +1 faces (owned by the neighbouring cells)
-        const double ux1 = k + 1 <= p.nxm - 1 ? vx1 - (d1 - dc) * (adt * p.udx3c[k + 1]) : vx1;
-        const double uy1 = vy1 - (djp - dc) * (adt * p.dy);
-        const double uz1 = vz1 - (dip - dc) * (adt * p.dz);
-        dvm = fmax(dvm, fabs((uz1 - uz) * p.dz + (uy1 - uy) * p.dy + (ux1 - ux) * p.udx3m[k]));
-        const double be = p.al * p.beta;
-        const double npr = prc + (dc - be * ((dip - 2.0 * dc + dim) * p.dzq + (djp - 2.0 * dc + djm) * p.dyq +
-                                            (dkp * p.apphk[k] + dc * p.acphk[k] + dkm * p.amphk[k])));
-        if (k == p.nxm - 1) {      // index nx: wall face of vx (not corrected), unused for vy, vz
-            const double wy = p.vy[c + 1], wz = p.vz[c + 1];
-            vx_out[c + 1] = vx1; vy_out[c + 1] = wy; vz_out[c + 1] = wz;
+        const double adt = p.al * p.dt, be = p.al * p.beta;
+        const double ucx = adt * p.udx3c[k], ucx1 = k + 1 <= p.nxm - 1 ? adt * p.udx3c[k + 1] : 0.0;
+        const long long row = (long long)j * p.ldx + k, rowjm = (long long)jm * p.ldx + k, rowjp = (long long)jp * p.ldx + k;
+        double d_m = d[(long long)wrapm(i0, p.nz) * p.plane + row];
+        double d_c = d[(long long)i0 * p.plane + row];
+        double w_c = p.vz[(long long)i0 * p.plane + row];
+        for (int i = i0; i < i1; ++i) {
+            const long long pl = (long long)i * p.plane, plp = (long long)wrapp(i, p.nz) * p.plane;
+            const long long c = pl + row;
+            // every load of the cell ahead of its stores (the outputs may
+            // alias the inputs as far as the compiler knows)
+            const double d_p = d[plp + row], w_p = p.vz[plp + row];
+            const double dkm = d[pl + row - k + km], dkp = d[pl + row - k + kp];
+            const double djm = d[pl + rowjm], djp = d[pl + rowjp];
+            const double vxc = p.vx[c], vyc = p.vy[c], vx1 = p.vx[c + 1], vy1 = p.vy[pl + rowjp];
+            const double d1 = d[c + 1], prc = p.pr[c];
+            // corrected faces of this cell (as k_project)
+            const double ux = vxc - (d_c - dkm) * ucx;
+            const double uy = vyc - (d_c - djm) * (adt * p.dy);
+            const double uz = w_c - (d_c - d_m) * (adt * p.dz);
+            // corrected +1 faces (owned by the neighbouring cells)
+            const double ux1 = k + 1 <= p.nxm - 1 ? vx1 - (d1 - d_c) * ucx1 : vx1;
+            const double uy1 = vy1 - (djp - d_c) * (adt * p.dy);
+            const double uz1 = w_p - (d_p - d_c) * (adt * p.dz);
+            dvm = fmax(dvm, fabs((uz1 - uz) * p.dz + (uy1 - uy) * p.dy + (ux1 - ux) * p.udx3m[k]));
+            const double npr = prc + (d_c - be * ((d_p - 2.0 * d_c + d_m) * p.dzq + (djp - 2.0 * d_c + djm) * p.dyq +
+                                                 (dkp * p.apphk[k] + d_c * p.acphk[k] + dkm * p.amphk[k])));
+            if (k == p.nxm - 1) {      // index nx: wall face of vx (not corrected), unused for vy, vz
+                const double wy = p.vy[c + 1], wz = p.vz[c + 1];
+                vx_out[c + 1] = vx1; vy_out[c + 1] = wy; vz_out[c + 1] = wz;
+            }
+            vx_out[c] = ux; vy_out[c] = uy; vz_out[c] = uz;
+            p.pr[c] = npr;
+            d_m = d_c; d_c = d_p; w_c = w_p;
         }
-        vx_out[c] = ux; vy_out[c] = uy; vz_out[c] = uz;
-        p.pr[c] = npr;
     }
     dvm = block_max(dvm);
     if (threadIdx.x == 0 && threadIdx.y == 0)

@@ -68,8 +68,13 @@ __global__ void __launch_bounds__(256) k_column_phi(KP p, TriFactors f, RT *phi_
     using V = typename Pair<RT>::T;
     constexpr int M2 = (64 * CH + 127) / 128;
     __shared__ double lds[4][lds_col_size<CH>()];
+    // the solver's factors, read by every column of the workgroup
+    __shared__ double fac[4][64 * CH];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
+    for (int q = threadIdx.x; q < 64 * CH; q += 256) { fac[0][q] = f.l[q]; fac[1][q] = f.e[q]; fac[2][q] = f.g[q]; fac[3][q] = f.s[q]; }
+    TriFactors fl = f;
+    fl.l = fac[0]; fl.e = fac[1]; fl.g = fac[2]; fl.s = fac[3];
     int j, i;
     const bool live = strip_column(xcd_block(0), wave, p.nyr, p.nzr, j, i);
     double *buf = lds[wave];
@@ -110,7 +115,7 @@ __global__ void __launch_bounds__(256) k_column_phi(KP p, TriFactors f, RT *phi_
         if (r1 < 64u * CH) buf[lds_row<CH>(r1)] = rhs1;
     }
     __syncthreads();
-    column_solve_lds<CH>(buf, f);
+    column_solve_lds<CH>(buf, fl);
     __syncthreads();
     if (!live) return;
     RT *out = phi_out + c0, *dphi = sel<RT>(p.rhsr, p.rhsrf) + c0;
