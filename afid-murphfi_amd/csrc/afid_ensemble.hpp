@@ -41,6 +41,14 @@ struct EnsRun {
     PoissonFactors<double> pf;
     cpx<double> *spec;
     FftPlan<double> plan;
+    // mixed form: FP32 spectrum, factors and transform; the residual of each
+    // Poisson sweep (max|div u| bit patterns, slots 1..3) gates the next
+    PoissonFactors<float> pff;
+    cpx<float> *specf;
+    FftPlan<float> planf;
+    unsigned long long *rslot;
+    double div_tol;
+    int max_sweeps;
     double gam[3], rom[3], alm[3];
     double dtmin, dtmax, limit_cfl, tmax, resid;
     int variable_dt, ntst, nsst, pad;
@@ -216,52 +224,57 @@ __device__ __forceinline__ Nb2 make_nb2(const KP &p, int i) {
     n.ttp = p.temptp[i];
     return n;
 }
-__device__ __forceinline__ double ex_vx_2d(const KP &p, const Nb2 &n, unsigned k) {
-    const double udz = p.dz * 0.25, udzq = p.dzq / p.ren;
-    const double rkm = p.rudx3m[k - 1], rkc = p.rudx3m[k], u3 = p.udx3c[k];
-    const double vxc = n.vx[k], vxzp = n.vx_p[k], vxzm = n.vx_m[k];
-    const double hxz = u3 * udz * ((rkm * n.vz_p[k - 1] + rkc * n.vz_p[k]) * (vxzp + vxc)
-                                   - (rkm * n.vz[k - 1] + rkc * n.vz[k]) * (vxc + vxzm));
-    const double a = n.vx[k + 1] + vxc, b = vxc + n.vx[k - 1];
-    const double hxx = (a * a - b * b) * u3 * 0.25;
-    const double dzzvx = (vxzm - 2.0 * vxc + vxzp) * udzq;
-    double q = -(hxx + hxz) + dzzvx;
-    if (p.gaxis == 1) q += p.byct * (double(p.active_t) * 0.5 * (n.t[k] + n.t[k - 1]));
+// (arithmetic in T: double in the FP64 form, float in the mixed form, the
+// FP64 state rounded on load)
+template <typename T>
+__device__ __forceinline__ T ex_vx_2d(const KP &p, const Nb2 &n, unsigned k) {
+    const T udz = T(p.dz) * T(0.25), udzq = T(p.dzq) / T(p.ren);
+    const T rkm = T(p.rudx3m[k - 1]), rkc = T(p.rudx3m[k]), u3 = T(p.udx3c[k]);
+    const T vxc = T(n.vx[k]), vxzp = T(n.vx_p[k]), vxzm = T(n.vx_m[k]);
+    const T hxz = u3 * udz * ((rkm * T(n.vz_p[k - 1]) + rkc * T(n.vz_p[k])) * (vxzp + vxc)
+                              - (rkm * T(n.vz[k - 1]) + rkc * T(n.vz[k])) * (vxc + vxzm));
+    const T a = T(n.vx[k + 1]) + vxc, b = vxc + T(n.vx[k - 1]);
+    const T hxx = (a * a - b * b) * u3 * T(0.25);
+    const T dzzvx = (vxzm - T(2.0) * vxc + vxzp) * udzq;
+    T q = -(hxx + hxz) + dzzvx;
+    if (p.gaxis == 1) q += T(p.byct) * (T(p.active_t) * T(0.5) * (T(n.t[k]) + T(n.t[k - 1])));
     return q;
 }
-__device__ __forceinline__ double ex_vz_2d(const KP &p, const Nb2 &n, unsigned k) {
-    const double udzq = p.dzq / p.ren, udz = p.dz * 0.25;
+template <typename T>
+__device__ __forceinline__ T ex_vz_2d(const KP &p, const Nb2 &n, unsigned k) {
+    const T udzq = T(p.dzq) / T(p.ren), udz = T(p.dz) * T(0.25);
     const unsigned kmm = k == 0 ? 0 : k - 1, kpp = k == unsigned(p.nxm - 1) ? k : k + 1;
-    const double vzc = n.vz[k], vzzp = n.vz_p[k], vzzm = n.vz_m[k];
-    const double a = vzzp + vzc, b = vzzm + vzc;
-    const double hzz = (a * a - b * b) * udz;
-    const double hzx = ((n.vx[k + 1] + n.vx_m[k + 1]) * (n.vz[kpp] + vzc)
-                        - (n.vx[k] + n.vx_m[k]) * (vzc + n.vz[kmm])) * p.udx3m[k] * 0.25;
-    const double dzzvz = (vzzp - 2.0 * vzc + vzzm) * udzq;
-    double q = -(hzx + hzz) + dzzvz;
-    if (p.gaxis == 3) q += p.byct * (double(p.active_t) * 0.5 * (n.t[k] + n.t_m[k]));
+    const T vzc = T(n.vz[k]), vzzp = T(n.vz_p[k]), vzzm = T(n.vz_m[k]);
+    const T a = vzzp + vzc, b = vzzm + vzc;
+    const T hzz = (a * a - b * b) * udz;
+    const T hzx = ((T(n.vx[k + 1]) + T(n.vx_m[k + 1])) * (T(n.vz[kpp]) + vzc)
+                   - (T(n.vx[k]) + T(n.vx_m[k])) * (vzc + T(n.vz[kmm]))) * T(p.udx3m[k]) * T(0.25);
+    const T dzzvz = (vzzp - T(2.0) * vzc + vzzm) * udzq;
+    T q = -(hzx + hzz) + dzzvz;
+    if (p.gaxis == 3) q += T(p.byct) * (T(p.active_t) * T(0.5) * (T(n.t[k]) + T(n.t_m[k])));
     return q;
 }
-__device__ __forceinline__ double ex_temp_2d(const KP &p, const Nb2 &n, unsigned k) {
-    const double udz = p.dz * 0.5, udzq = p.dzq / p.pect;
-    const double tc = n.t[k], tzp = n.t_p[k], tzm = n.t_m[k];
-    const double vxp = n.vx[k + 1], vxc = n.vx[k];
-    double htx;
-    if (k == 0) htx = (vxp * (n.t[k + 1] + tc) - vxc * 2.0 * n.tbp) * 0.5 * p.udx3m[k];
-    else if (k == unsigned(p.nxm - 1)) htx = (vxp * 2.0 * n.ttp - vxc * (tc + n.t[k - 1])) * 0.5 * p.udx3m[k];
-    else htx = (vxp * (n.t[k + 1] + tc) - vxc * (tc + n.t[k - 1])) * 0.5 * p.udx3m[k];
-    const double htz = (n.vz_p[k] * (tzp + tc) - n.vz[k] * (tc + tzm)) * udz;
-    const double dzzt = (tzp - 2.0 * tc + tzm) * udzq;
+template <typename T>
+__device__ __forceinline__ T ex_temp_2d(const KP &p, const Nb2 &n, unsigned k) {
+    const T udz = T(p.dz) * T(0.5), udzq = T(p.dzq) / T(p.pect);
+    const T tc = T(n.t[k]), tzp = T(n.t_p[k]), tzm = T(n.t_m[k]);
+    const T vxp = T(n.vx[k + 1]), vxc = T(n.vx[k]);
+    T htx;
+    if (k == 0) htx = (vxp * (T(n.t[k + 1]) + tc) - vxc * T(2.0) * T(n.tbp)) * T(0.5) * T(p.udx3m[k]);
+    else if (k == unsigned(p.nxm - 1)) htx = (vxp * T(2.0) * T(n.ttp) - vxc * (tc + T(n.t[k - 1]))) * T(0.5) * T(p.udx3m[k]);
+    else htx = (vxp * (T(n.t[k + 1]) + tc) - vxc * (tc + T(n.t[k - 1]))) * T(0.5) * T(p.udx3m[k]);
+    const T htz = (T(n.vz_p[k]) * (tzp + tc) - T(n.vz[k]) * (tc + tzm)) * udz;
+    const T dzzt = (tzp - T(2.0) * tc + tzm) * udzq;
     return -(htx + htz) + dzzt;
 }
 
 // stencil phase of field FLD (0 vx, 2 vz, 3 T) of a 2-D column: RHS into buf
 // and the old value into own (rows in the solver's LDS layout), new explicit
 // terms into the history (ImplicitAndUpdateVX/VZ/Temp.F90 RHS)
-template <int FLD, int CH>
+template <int FLD, int CH, typename HT>
 __device__ __forceinline__ void column_2d_rows(const KP &p, const Nb2 &n, long long c0, double *buf, double *own,
                                                unsigned lane) {
-    double *ru = (FLD == 0 ? p.rux : FLD == 2 ? p.ruz : p.rut) + c0;
+    HT *ru = (FLD == 0 ? sel<HT>(p.rux, p.ruxf) : FLD == 2 ? sel<HT>(p.ruz, p.ruzf) : sel<HT>(p.rut, p.rutf)) + c0;
     const unsigned nx = p.nx, nxm = p.nxm;
 #pragma unroll 1
     for (unsigned r = lane; r < 64u * CH; r += 64) {
@@ -270,26 +283,26 @@ __device__ __forceinline__ void column_2d_rows(const KP &p, const Nb2 &n, long l
             if (FLD == 0) {
                 old = n.vx[r];
                 if (r >= 1 && r < nxm) {
-                    const double h = ex_vx_2d(p, n, r);
+                    const HT h = ex_vx_2d<HT>(p, n, r);
                     const double dxx = n.vx[r + 1] * p.ap3ck[r] + n.vx[r] * p.ac3ck[r] + n.vx[r - 1] * p.am3ck[r];
                     const double dxp = (n.pr[r] - n.pr[r - 1]) * (p.al * p.udx3c[r]);
-                    rhs = (p.ga * h + p.ro * ru[r] + (p.al / p.ren) * dxx - dxp) * p.dt;
+                    rhs = (p.ga * double(h) + p.ro * double(ru[r]) + (p.al / p.ren) * dxx - dxp) * p.dt;
                     ru[r] = h;
                 }
             } else if (FLD == 2) {
                 old = n.vz[r];
                 if (r < nxm) {
-                    const double h = ex_vz_2d(p, n, r);
+                    const HT h = ex_vz_2d<HT>(p, n, r);
                     const unsigned kmm = r == 0 ? 0 : r - 1, kpp = r == nxm - 1 ? r : r + 1;
                     const double dxx = n.vz[kpp] * p.ap3sk[r] + n.vz[r] * p.ac3sk[r] + n.vz[kmm] * p.am3sk[r];
                     const double dzp = (n.pr[r] - n.pr_m[r]) * p.dz * p.al;
-                    rhs = (p.ga * h + p.ro * ru[r] + (p.al / p.ren) * dxx - dzp) * p.dt;
+                    rhs = (p.ga * double(h) + p.ro * double(ru[r]) + (p.al / p.ren) * dxx - dzp) * p.dt;
                     ru[r] = h;
                 }
             } else {
                 old = n.t[r];
                 if (r < nxm) {
-                    const double h = ex_temp_2d(p, n, r);
+                    const HT h = ex_temp_2d<HT>(p, n, r);
                     double dxx;
                     if (r == 0)
                         dxx = n.t[r + 1] * p.ap3ssk[r] + n.t[r] * p.ac3ssk[r] - (p.ap3ssk[r] + p.ac3ssk[r]) * n.tbp * p.tfixs;
@@ -297,7 +310,7 @@ __device__ __forceinline__ void column_2d_rows(const KP &p, const Nb2 &n, long l
                         dxx = n.t[r] * p.ac3ssk[r] + n.t[r - 1] * p.am3ssk[r] - (p.am3ssk[r] + p.ac3ssk[r]) * n.ttp * p.tfixn;
                     else
                         dxx = n.t[r + 1] * p.ap3ssk[r] + n.t[r] * p.ac3ssk[r] + n.t[r - 1] * p.am3ssk[r];
-                    rhs = (p.ga * h + p.ro * ru[r] + (p.al / p.pect) * dxx) * p.dt;
+                    rhs = (p.ga * double(h) + p.ro * double(ru[r]) + (p.al / p.pect) * dxx) * p.dt;
                     ru[r] = h;
                 }
             }
@@ -310,7 +323,7 @@ __device__ __forceinline__ void column_2d_rows(const KP &p, const Nb2 &n, long l
 // column i of the run by one wavefront: vx, vz, T in turn, new values into
 // buffer set cur ^ 1; fac holds the three factor sets, buf/own this
 // wavefront's LDS columns
-template <int CH>
+template <int CH, typename HT = double>
 __device__ __forceinline__ void ens_column(const EnsRun &R, const KP &p, int cur, int i, const double *fac, double *buf,
                                            double *own, unsigned lane) {
     constexpr int NP = 64 * CH;
@@ -320,9 +333,9 @@ __device__ __forceinline__ void ens_column(const EnsRun &R, const KP &p, int cur
 #pragma unroll 1
     for (int fi = 0; fi < 3; ++fi) {
         switch (fi) {
-        case 0: column_2d_rows<0, CH>(p, n, c0, buf, own, lane); break;
-        case 1: column_2d_rows<2, CH>(p, n, c0, buf, own, lane); break;
-        default: column_2d_rows<3, CH>(p, n, c0, buf, own, lane); break;
+        case 0: column_2d_rows<0, CH, HT>(p, n, c0, buf, own, lane); break;
+        case 1: column_2d_rows<2, CH, HT>(p, n, c0, buf, own, lane); break;
+        default: column_2d_rows<3, CH, HT>(p, n, c0, buf, own, lane); break;
         }
         wave_sync();
         const double *f = fac + fi * 4 * NP;
@@ -343,14 +356,30 @@ __device__ __forceinline__ void ens_column(const EnsRun &R, const KP &p, int cur
     }
 }
 
-// divergence (pressure.F90:147-161) of the line pairs [l0, l0 + nl) and
-// their forward transform along z (pressure.F90:171-179, normalised by
-// 1/nz since ny = 1) into the half spectrum; p reads the new velocity.
-// Whole workgroup of NT threads; lb holds nl lines.
-template <int NT>
-__device__ __forceinline__ void ens_forward(const EnsRun &R, const KP &p, int l0, int nl, cpx<double> *lb) {
-    const int nxm = p.nxm, N = p.nz, S = line_stride<double>(N), nh = N / 2 + 1, TX = 2 * nl, k0 = 2 * l0;
-    const double usdtal = 1.0 / (p.dt * p.al), scale = 1.0 / double(N);
+// the run's spectrum, transform and mode factors in arithmetic R
+template <typename R> struct EnsPoisson;
+template <> struct EnsPoisson<double> {
+    static __device__ __forceinline__ cpx<double> *spec(const EnsRun &E) { return E.spec; }
+    static __device__ __forceinline__ const FftPlan<double> &plan(const EnsRun &E) { return E.plan; }
+    static __device__ __forceinline__ const PoissonFactors<double> &pf(const EnsRun &E) { return E.pf; }
+};
+template <> struct EnsPoisson<float> {
+    static __device__ __forceinline__ cpx<float> *spec(const EnsRun &E) { return E.specf; }
+    static __device__ __forceinline__ const FftPlan<float> &plan(const EnsRun &E) { return E.planf; }
+    static __device__ __forceinline__ const PoissonFactors<float> &pf(const EnsRun &E) { return E.pff; }
+};
+
+// divergence (pressure.F90:147-161, formed in FP64) of the line pairs
+// [l0, l0 + nl) and their forward transform along z (pressure.F90:171-179,
+// normalised by 1/nz since ny = 1) in arithmetic R into the half spectrum;
+// p reads the current velocity.  Whole workgroup of NT threads; lb holds nl
+// lines.
+template <int NT, typename R = double>
+__device__ __forceinline__ void ens_forward(const EnsRun &E, const KP &p, int l0, int nl, cpx<R> *lb) {
+    const int nxm = p.nxm, N = p.nz, S = line_stride<R>(N), nh = N / 2 + 1, TX = 2 * nl, k0 = 2 * l0;
+    const double usdtal = 1.0 / (p.dt * p.al);
+    const R scale = R(1.0 / double(N));
+    cpx<R> *spec = EnsPoisson<R>::spec(E);
     for (int t = threadIdx.x; t < TX * N; t += NT) {
         const int xo = t % TX, i = t / TX, k = k0 + xo;
         double d = 0.0;
@@ -358,53 +387,55 @@ __device__ __forceinline__ void ens_forward(const EnsRun &R, const KP &p, int l0
             const long long c = (long long)i * p.plane + k, cp = (long long)wrapp(i, N) * p.plane + k;
             d = ((p.vz[cp] - p.vz[c]) * p.dz + (p.vx[c + 1] - p.vx[c]) * p.udx3m[k]) * usdtal;
         }
-        double *slot = reinterpret_cast<double *>(&lb[(xo >> 1) * S + lpad<double>(i)]);
-        slot[xo & 1] = d;
+        R *slot = reinterpret_cast<R *>(&lb[(xo >> 1) * S + lpad<R>(i)]);
+        slot[xo & 1] = R(d);
     }
     __syncthreads();
-    lds_fft<double, NT, ENS_FFT_MAXV>(lb, nl, S, R.plan, -1);
+    lds_fft<R, NT, ENS_FFT_MAXV>(lb, nl, S, EnsPoisson<R>::plan(E), -1);
     for (int t = threadIdx.x; t < TX * nh; t += NT) {
         const int xo = t % TX, m = t / TX, k = k0 + xo;
         if (k >= nxm) continue;
         const int line = xo >> 1;
-        const cpx<double> zm = lb[line * S + lpad<double>(m)], zn = lb[line * S + lpad<double>(m == 0 ? 0 : N - m)];
-        cpx<double> o;
-        if ((xo & 1) == 0) o = mk<double>(0.5 * (zm.x + zn.x), 0.5 * (zm.y - zn.y));
-        else o = mk<double>(0.5 * (zm.y + zn.y), -0.5 * (zm.x - zn.x));
-        R.spec[(long long)m * nxm + k] = mk<double>(o.x * scale, o.y * scale);
+        const cpx<R> zm = lb[line * S + lpad<R>(m)], zn = lb[line * S + lpad<R>(m == 0 ? 0 : N - m)];
+        cpx<R> o;
+        if ((xo & 1) == 0) o = mk<R>(R(0.5) * (zm.x + zn.x), R(0.5) * (zm.y - zn.y));
+        else o = mk<R>(R(0.5) * (zm.y + zn.y), R(-0.5) * (zm.x - zn.x));
+        spec[(long long)m * nxm + k] = mk<R>(o.x * scale, o.y * scale);
     }
     __syncthreads();
 }
 
 // per-mode tridiagonal (pressure.F90:201-236) of mode m by one wavefront;
 // modes m and N-m share the matrix and the real data make them conjugate
-template <int CH>
-__device__ __forceinline__ void ens_mode(const EnsRun &R, int nxm, int m) {
-    mode_tri_solve<CH, double>(R.pf.m, R.pf.g, R.pf.e, (long long)m * R.pf.npad, R.pf.npad, R.spec + (long long)m * nxm, nxm);
+template <int CH, typename R = double>
+__device__ __forceinline__ void ens_mode(const EnsRun &E, int nxm, int m) {
+    const PoissonFactors<R> &pf = EnsPoisson<R>::pf(E);
+    mode_tri_solve<CH, R>(pf.m, pf.g, pf.e, (long long)m * pf.npad, pf.npad, EnsPoisson<R>::spec(E) + (long long)m * nxm, nxm);
 }
 
 // inverse transform (pressure.F90:189-195) of the line pairs [l0, l0 + nl)
-// into dph.  Whole workgroup.
-template <int NT>
-__device__ __forceinline__ void ens_inverse(const EnsRun &R, const KP &p, int l0, int nl, cpx<double> *lb) {
-    const int nxm = p.nxm, N = p.nz, S = line_stride<double>(N), nh = N / 2 + 1, TX = 2 * nl, k0 = 2 * l0;
+// into dph (FP64).  Whole workgroup.
+template <int NT, typename R = double>
+__device__ __forceinline__ void ens_inverse(const EnsRun &E, const KP &p, int l0, int nl, cpx<R> *lb) {
+    const int nxm = p.nxm, N = p.nz, S = line_stride<R>(N), nh = N / 2 + 1, TX = 2 * nl, k0 = 2 * l0;
+    const cpx<R> *spec = EnsPoisson<R>::spec(E);
     for (int t = threadIdx.x; t < nl * nh; t += NT) {
         const int line = t % nl, m = t / nl;
         const int ka = k0 + 2 * line, kb = ka + 1;
-        const cpx<double> *row = R.spec + (long long)m * nxm;
-        cpx<double> A = ka < nxm ? row[ka] : mk<double>(0.0, 0.0);
-        cpx<double> B = kb < nxm ? row[kb] : mk<double>(0.0, 0.0);
-        if (m == 0 || 2 * m == N) { A.y = 0.0; B.y = 0.0; }
-        lb[line * S + lpad<double>(m)] = mk<double>(A.x - B.y, A.y + B.x);
-        if (m > 0 && N - m > m) lb[line * S + lpad<double>(N - m)] = mk<double>(A.x + B.y, B.x - A.y);
+        const cpx<R> *row = spec + (long long)m * nxm;
+        cpx<R> A = ka < nxm ? row[ka] : mk<R>(R(0), R(0));
+        cpx<R> B = kb < nxm ? row[kb] : mk<R>(R(0), R(0));
+        if (m == 0 || 2 * m == N) { A.y = R(0); B.y = R(0); }
+        lb[line * S + lpad<R>(m)] = mk<R>(A.x - B.y, A.y + B.x);
+        if (m > 0 && N - m > m) lb[line * S + lpad<R>(N - m)] = mk<R>(A.x + B.y, B.x - A.y);
     }
     __syncthreads();
-    lds_fft<double, NT, ENS_FFT_MAXV>(lb, nl, S, R.plan, +1);
+    lds_fft<R, NT, ENS_FFT_MAXV>(lb, nl, S, EnsPoisson<R>::plan(E), +1);
     for (int t = threadIdx.x; t < TX * N; t += NT) {
         const int xo = t % TX, i = t / TX, k = k0 + xo;
         if (k >= nxm) continue;
-        const cpx<double> z = lb[(xo >> 1) * S + lpad<double>(i)];
-        p.dph[(long long)i * p.plane + k] = (xo & 1) ? z.y : z.x;
+        const cpx<R> z = lb[(xo >> 1) * S + lpad<R>(i)];
+        p.dph[(long long)i * p.plane + k] = double((xo & 1) ? z.y : z.x);
     }
     __syncthreads();
 }
@@ -513,6 +544,15 @@ struct EnsItem { int run, start; };
 constexpr int ENSP_NT = 256;                      // threads of the phased kernels
 constexpr int ENSP_CELLS = 4 * ENSP_NT;           // cells per projection item
 
+// Mixed form: Poisson sweep s (1-based) of a run runs when s = 1, or while
+// the FP64 residual max|div u| left by sweep s-1 (rslot[s-1]) is above
+// div_tol, up to max_sweeps; the FP64 form has one sweep.
+__device__ __forceinline__ bool ens_sweep_open(const EnsRun &E, int s) {
+    if (s <= 1) return true;
+    if (s > E.max_sweeps) return false;
+    return __longlong_as_double((long long)E.rslot[s - 1]) > E.div_tol;
+}
+
 template <int CH>
 __global__ void __launch_bounds__(192) k_ensp_factors(const EnsRun *__restrict__ runs, const EnsState *st, int ns,
                                                       double *fac) {
@@ -521,10 +561,11 @@ __global__ void __launch_bounds__(192) k_ensp_factors(const EnsRun *__restrict__
     const EnsRun &R = runs[r];
     const KP p = ens_kp(R, ns, st[r].dt, 0);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (R.rslot && threadIdx.x < 4) R.rslot[threadIdx.x] = 0ull;      // residual slots of this substep
     ens_factors<CH>(p, wave, fac + ((long long)r * 3 + wave) * 4 * 64 * CH);
 }
 
-template <int CH>
+template <int CH, typename HT>
 __global__ void __launch_bounds__(ENSP_NT) k_ensp_columns(const EnsRun *__restrict__ runs, const EnsState *st,
                                                           const EnsItem *__restrict__ items, int ns, const double *fac) {
     __shared__ double lds[ENSP_NT / 64][2][lds_col_size<CH>()];
@@ -537,47 +578,75 @@ __global__ void __launch_bounds__(ENSP_NT) k_ensp_columns(const EnsRun *__restri
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = it.start + wave;
     if (i >= p.nz) return;
-    ens_column<CH>(R, p, cur, i, fac + (long long)r * 12 * 64 * CH, lds[wave][0], lds[wave][1], threadIdx.x & 63);
+    ens_column<CH, HT>(R, p, cur, i, fac + (long long)r * 12 * 64 * CH, lds[wave][0], lds[wave][1], threadIdx.x & 63);
 }
 
 // forward (inverse = 0) or inverse (inverse = 1) transform of one line batch
+// of Poisson sweep `sweep`, in arithmetic R
+template <typename R>
 __global__ void __launch_bounds__(ENSP_NT) k_ensp_transform(const EnsRun *__restrict__ runs, const EnsState *st,
                                                             const EnsItem *__restrict__ items, int ns, int lines,
-                                                            int inverse) {
+                                                            int inverse, int sweep) {
     extern __shared__ __align__(16) unsigned char smraw[];
-    cpx<double> *lb = reinterpret_cast<cpx<double> *>(smraw);
+    cpx<R> *lb = reinterpret_cast<cpx<R> *>(smraw);
     const EnsItem it = items[blockIdx.x];
     const int r = it.run;
     if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
-    const EnsRun &R = runs[r];
+    const EnsRun &R_ = runs[r];
+    if (!ens_sweep_open(R_, sweep)) return;
     const int cur = __builtin_amdgcn_readfirstlane(st[r].cur);
-    const KP p = ens_kp(R, ns, st[r].dt, cur ^ 1);          // the columns' new velocity
+    const KP p = ens_kp(R_, ns, st[r].dt, cur ^ 1);          // the columns' new velocity
     const int nlines = (p.nxm + 1) / 2, L = ens_fft_lines(p.nz, lines, ENSP_NT);
     const int nl = min(L, nlines - it.start);
-    if (inverse) ens_inverse<ENSP_NT>(R, p, it.start, nl, lb);
-    else ens_forward<ENSP_NT>(R, p, it.start, nl, lb);
+    if (inverse) ens_inverse<ENSP_NT, R>(R_, p, it.start, nl, lb);
+    else ens_forward<ENSP_NT, R>(R_, p, it.start, nl, lb);
 }
 
-template <int CH>
+template <int CH, typename R>
 __global__ void __launch_bounds__(ENSP_NT) k_ensp_modes(const EnsRun *__restrict__ runs, const EnsState *st,
-                                                        const EnsItem *__restrict__ items) {
+                                                        const EnsItem *__restrict__ items, int sweep) {
     const EnsItem it = items[blockIdx.x];
     const int r = it.run;
     if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
-    const EnsRun &R = runs[r];
+    const EnsRun &E = runs[r];
+    if (!ens_sweep_open(E, sweep)) return;
     const int m = it.start + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (m <= R.p.nz / 2) ens_mode<CH>(R, R.p.nxm, m);
+    if (m <= E.p.nz / 2) ens_mode<CH, R>(E, E.p.nxm, m);
 }
 
 __global__ void __launch_bounds__(ENSP_NT) k_ensp_project(const EnsRun *__restrict__ runs, const EnsState *st,
-                                                          const EnsItem *__restrict__ items, int ns) {
+                                                          const EnsItem *__restrict__ items, int ns, int sweep) {
     const EnsItem it = items[blockIdx.x];
     const int r = it.run;
     if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
     const EnsRun &R = runs[r];
+    if (!ens_sweep_open(R, sweep)) return;
     const int cur = __builtin_amdgcn_readfirstlane(st[r].cur);
     const KP p = ens_kp(R, ns, st[r].dt, cur ^ 1);
     ens_project<ENSP_NT>(p, it.start, min(it.start + ENSP_CELLS, p.nxm * p.nz));
+}
+
+// Mixed form: FP64 residual max|div u| (CheckDivergence units) after the
+// projection of sweep `sweep`, max-reduced per run into rslot[sweep]
+__global__ void __launch_bounds__(ENSP_NT) k_ensp_resid(const EnsRun *__restrict__ runs, const EnsState *st,
+                                                        const EnsItem *__restrict__ items, int ns, int sweep) {
+    const EnsItem it = items[blockIdx.x];
+    const int r = it.run;
+    if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
+    const EnsRun &R = runs[r];
+    if (!ens_sweep_open(R, sweep)) return;
+    const int cur = __builtin_amdgcn_readfirstlane(st[r].cur);
+    const KP p = ens_kp(R, ns, st[r].dt, cur ^ 1);
+    const int nxm = p.nxm, nz = p.nz, t1 = min(it.start + ENSP_CELLS, nxm * nz);
+    double dmax = 0.0;
+    for (int t = it.start + threadIdx.x; t < t1; t += ENSP_NT) {
+        const int k = t % nxm, i = t / nxm;
+        const long long c = (long long)i * p.plane + k, cp = (long long)wrapp(i, nz) * p.plane + k;
+        const double d = fabs((p.vz[cp] - p.vz[c]) * p.dz + (p.vx[c + 1] - p.vx[c]) * p.udx3m[k]);
+        dmax = fmax(dmax, d != d ? HUGE_VAL : d);
+    }
+    dmax = block_max(dmax);
+    if (threadIdx.x == 0) atomicMax(R.rslot + sweep, (unsigned long long)__double_as_longlong(dmax));
 }
 
 __global__ void __launch_bounds__(ENSP_NT) k_ensp_finish(const EnsRun *__restrict__ runs, EnsState *st, int ns) {

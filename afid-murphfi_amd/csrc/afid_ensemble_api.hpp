@@ -27,6 +27,9 @@ struct afid_ens {
     afid_ens_tuning tune{1024, 0, 8, 0, 0.0};
     size_t lds = 0;
     // phased form: work items per phase and the per-run factor sets
+    bool mixed = false;                          // AFID_MIXED runs (phased form only)
+    unsigned long long *d_rslot = nullptr;       // per run: residual slots of the mixed form's sweeps
+    int max_sweeps = 1;
     EnsItem *d_items_col = nullptr, *d_items_mode = nullptr, *d_items_line = nullptr, *d_items_cell = nullptr;
     int n_col = 0, n_mode = 0, n_line = 0, n_cell = 0, max_line = 0;
     double *d_fac = nullptr;
@@ -95,27 +98,41 @@ int ens_launched(afid_ens *e, const char *what) {
     return AFID_OK;
 }
 
-// the phased form of substep ns: seven launches
-template <int CH>
-int ens_substep_phased_ch(afid_ens *e, int ns) {
+// the phased form of substep ns: factors, columns, then per Poisson sweep
+// forward transform, modes, inverse transform, projection (and in the mixed
+// form the FP64 residual that gates the next sweep), then finish
+template <int CH, typename HT, typename R>
+int ens_substep_phased_t(afid_ens *e, int ns) {
     int rc;
     const dim3 blk(ENSP_NT);
     hipLaunchKernelGGL((k_ensp_factors<CH>), dim3(unsigned(e->nruns)), dim3(192), 0, e->stream, e->d_runs, e->d_state, ns, e->d_fac);
     if ((rc = ens_launched(e, "k_ensp_factors"))) return rc;
-    hipLaunchKernelGGL((k_ensp_columns<CH>), dim3(unsigned(e->n_col)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_col, ns, e->d_fac);
+    hipLaunchKernelGGL((k_ensp_columns<CH, HT>), dim3(unsigned(e->n_col)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_col,
+                       ns, e->d_fac);
     if ((rc = ens_launched(e, "k_ensp_columns"))) return rc;
-    hipLaunchKernelGGL(k_ensp_transform, dim3(unsigned(e->n_line)), blk, e->lds_line, e->stream, e->d_runs, e->d_state, e->d_items_line,
-                       ns, e->tune.lines, 0);
-    if ((rc = ens_launched(e, "k_ensp_transform"))) return rc;
-    hipLaunchKernelGGL((k_ensp_modes<CH>), dim3(unsigned(e->n_mode)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_mode);
-    if ((rc = ens_launched(e, "k_ensp_modes"))) return rc;
-    hipLaunchKernelGGL(k_ensp_transform, dim3(unsigned(e->n_line)), blk, e->lds_line, e->stream, e->d_runs, e->d_state, e->d_items_line,
-                       ns, e->tune.lines, 1);
-    if ((rc = ens_launched(e, "k_ensp_transform"))) return rc;
-    hipLaunchKernelGGL(k_ensp_project, dim3(unsigned(e->n_cell)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_cell, ns);
-    if ((rc = ens_launched(e, "k_ensp_project"))) return rc;
+    const int sweeps = e->mixed ? e->max_sweeps : 1;
+    for (int sw = 1; sw <= sweeps; ++sw) {
+        hipLaunchKernelGGL(k_ensp_transform<R>, dim3(unsigned(e->n_line)), blk, e->lds_line, e->stream, e->d_runs, e->d_state,
+                           e->d_items_line, ns, e->tune.lines, 0, sw);
+        if ((rc = ens_launched(e, "k_ensp_transform"))) return rc;
+        hipLaunchKernelGGL((k_ensp_modes<CH, R>), dim3(unsigned(e->n_mode)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_mode, sw);
+        if ((rc = ens_launched(e, "k_ensp_modes"))) return rc;
+        hipLaunchKernelGGL(k_ensp_transform<R>, dim3(unsigned(e->n_line)), blk, e->lds_line, e->stream, e->d_runs, e->d_state,
+                           e->d_items_line, ns, e->tune.lines, 1, sw);
+        if ((rc = ens_launched(e, "k_ensp_transform"))) return rc;
+        hipLaunchKernelGGL(k_ensp_project, dim3(unsigned(e->n_cell)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_cell, ns, sw);
+        if ((rc = ens_launched(e, "k_ensp_project"))) return rc;
+        if (e->mixed) {
+            hipLaunchKernelGGL(k_ensp_resid, dim3(unsigned(e->n_cell)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_cell, ns, sw);
+            if ((rc = ens_launched(e, "k_ensp_resid"))) return rc;
+        }
+    }
     hipLaunchKernelGGL(k_ensp_finish, dim3(unsigned(e->nruns)), blk, 0, e->stream, e->d_runs, e->d_state, ns);
     return ens_launched(e, "k_ensp_finish");
+}
+template <int CH>
+int ens_substep_phased_ch(afid_ens *e, int ns) {
+    return e->mixed ? ens_substep_phased_t<CH, float, float>(e, ns) : ens_substep_phased_t<CH, double, double>(e, ns);
 }
 
 int ens_substep(afid_ens *e, int ns) {
@@ -200,7 +217,9 @@ int ens_set_line_items(afid_ens *e, int lines) {
         if (c->nz > ENS_FFT_MAXV * ENSP_NT) return ens_err(e, AFID_E_ARG, "ensemble: nz too large for the phased transforms");
         const int L = ens_fft_lines(c->nz, lines, ENSP_NT), nlines = (c->nxm + 1) / 2;
         for (int l0 = 0; l0 < nlines; l0 += L) it.push_back(EnsItem{r, l0});
-        lds = std::max(lds, size_t(L) * size_t(line_stride<double>(c->nz)) * sizeof(cpx<double>));
+        const size_t line = e->mixed ? size_t(line_stride<float>(c->nz)) * sizeof(cpx<float>)
+                                     : size_t(line_stride<double>(c->nz)) * sizeof(cpx<double>);
+        lds = std::max(lds, size_t(L) * line);
     }
     if (int(it.size()) > e->max_line) return ens_err(e, AFID_E_ARG, "ensemble: line items exceed their table");
     if (lds > size_t(e->max_lds)) return ens_err(e, AFID_E_ARG, "ensemble tuning: transform lines do not fit the LDS");
@@ -208,8 +227,9 @@ int ens_set_line_items(afid_ens *e, int lines) {
     ENSCHK(e, hipMemcpy(e->d_items_line, it.data(), sizeof(EnsItem) * it.size(), hipMemcpyHostToDevice));
     e->n_line = int(it.size());
     e->lds_line = lds;
-    ENSCHK(e, hipFuncSetAttribute(reinterpret_cast<const void *>(k_ensp_transform), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  int(std::max<size_t>(lds, 256))));
+    const void *kt = e->mixed ? reinterpret_cast<const void *>(k_ensp_transform<float>)
+                              : reinterpret_cast<const void *>(k_ensp_transform<double>);
+    ENSCHK(e, hipFuncSetAttribute(kt, hipFuncAttributeMaxDynamicSharedMemorySize, int(std::max<size_t>(lds, 256))));
     return AFID_OK;
 }
 
@@ -225,6 +245,7 @@ int ens_apply_tuning(afid_ens *e, const afid_ens_tuning &t) {
         ens_drop_graph(e);
         return AFID_OK;
     }
+    if (e->mixed) return ens_err(e, AFID_E_ARG, "ensemble tuning: the mixed form runs in the phased form only");
     e->tune.phased = 0;
     if (t.threads != 256 && t.threads != 512 && t.threads != 1024)
         return ens_err(e, AFID_E_ARG, "ensemble tuning: threads must be 256, 512 or 1024");
@@ -325,13 +346,14 @@ int afid_ens_init(const afid_config *cfgs, const afid_run_control *ctl, int32_t 
     }
     e->nruns = nruns;
     e->nsst = cfgs[0].nsst;
+    e->mixed = cfgs[0].precision == AFID_MIXED;
     int maxnx = 1;
     for (int r = 0; r < nruns; ++r) {
         const afid_config &g = cfgs[r];
         const std::string who = "ensemble run " + std::to_string(r) + ": ";
         if (g.nym != 1) return ens_err(e, AFID_E_ARG, who + "the ensemble holds 2-D runs (nym = 1)");
         if (g.multires || g.salinity || g.phasefield) return ens_err(e, AFID_E_ARG, who + "no refined grid, salinity or phase field");
-        if (g.precision != AFID_FP64) return ens_err(e, AFID_E_ARG, who + "the ensemble runs in the FP64 form");
+        if (g.precision != cfgs[0].precision) return ens_err(e, AFID_E_ARG, who + "every run needs the same precision");
         if (g.nsst != e->nsst) return ens_err(e, AFID_E_ARG, who + "every run needs the same nsst");
         if (g.gaxis == 2) return ens_err(e, AFID_E_ARG, who + "gravity along y has no effect in a 2-D run (gaxis 1 or 3)");
         if (!(ctl[r].dt > 0.0) || !(ctl[r].limit_cfl > 0.0)) return ens_err(e, AFID_E_ARG, who + "DT and CFLMAX must be positive");
@@ -362,9 +384,19 @@ int afid_ens_init(const afid_config *cfgs, const afid_run_control *ctl, int32_t 
         R.vxb[0] = c->f[AFID_VX]; R.vxb[1] = c->alt[0];
         R.vzb[0] = c->f[AFID_VZ]; R.vzb[1] = c->alt[2];
         R.tb[0] = c->f[AFID_TEMP]; R.tb[1] = c->alt[3];
-        R.pf = c->pois;
-        R.spec = static_cast<cpx<double> *>(c->spec);
-        R.plan = c->plan_z;
+        if (e->mixed) {
+            R.pff = c->pff;
+            R.specf = static_cast<cpx<float> *>(c->spec);
+            R.planf = c->plan_zf;
+            R.div_tol = c->div_tol;
+            R.max_sweeps = c->max_sweeps;
+            e->max_sweeps = std::max(e->max_sweeps, c->max_sweeps);
+        } else {
+            R.pf = c->pois;
+            R.spec = static_cast<cpx<double> *>(c->spec);
+            R.plan = c->plan_z;
+            R.max_sweeps = 1;
+        }
         for (int s = 0; s < 3; ++s) { R.gam[s] = c->gam[s]; R.rom[s] = c->rom[s]; R.alm[s] = c->alm[s]; }
         const afid_run_control &k = ctl[r];
         R.dtmin = k.dtmin; R.dtmax = k.dtmax > 0.0 ? k.dtmax : k.dt; R.limit_cfl = k.limit_cfl;
@@ -372,6 +404,10 @@ int afid_ens_init(const afid_config *cfgs, const afid_run_control *ctl, int32_t 
         R.variable_dt = k.variable_dt ? 1 : 0; R.ntst = k.ntst; R.nsst = c->c.nsst;
     }
     int rc;
+    if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_rslot), sizeof(unsigned long long) * 4 * size_t(nruns)))) return rc;
+    ENSCHK(e, hipMemset(e->d_rslot, 0, sizeof(unsigned long long) * 4 * size_t(nruns)));
+    if (e->mixed)
+        for (int r = 0; r < nruns; ++r) tab[size_t(r)].rslot = e->d_rslot + 4 * r;
     if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_runs), sizeof(EnsRun) * tab.size()))) return rc;
     if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_state), sizeof(EnsState) * size_t(nruns)))) return rc;
     if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_id_off), sizeof(int) * size_t(nruns + 1)))) return rc;
@@ -409,7 +445,8 @@ int afid_ens_init(const afid_config *cfgs, const afid_run_control *ctl, int32_t 
     ENSCHK(e, hipMemcpy(e->d_id_off, ioff.data(), sizeof(int) * ioff.size(), hipMemcpyHostToDevice));
     ENSCHK(e, hipMemcpy(e->d_id_run, irun.data(), sizeof(int) * irun.size(), hipMemcpyHostToDevice));
     // default launch shape: 1024 threads, one run per workgroup, 8 lines
-    afid_ens_tuning t{1024, nruns, 8, 0, 0.0};
+    // (the mixed form: phased, 2 lines)
+    afid_ens_tuning t = e->mixed ? afid_ens_tuning{ENSP_NT, 0, 2, 1, 0.0} : afid_ens_tuning{1024, nruns, 8, 0, 0.0};
     if ((rc = ens_apply_tuning(e, t))) return rc;
     if ((rc = ens_reset_state(e))) return rc;
     ENSCHK(e, hipStreamSynchronize(e->stream));
@@ -554,23 +591,27 @@ int afid_ens_autotune(afid_ens *e, int32_t steps, afid_ens_tuning *best, afid_en
     if (!e || steps < 1) return AFID_E_ARG;
     ENSCHK(e, hipStreamSynchronize(e->stream));
     // snapshot
-    std::vector<std::pair<double *, size_t>> arrays;
+    std::vector<std::pair<void *, size_t>> arrays;      // (array, bytes)
     for (afid_ctx *c : e->runs) {
         const size_t nb = size_t(c->nz) * c->ny * c->ldx;
         for (double *a : {c->f[AFID_VX], c->alt[0], c->f[AFID_VZ], c->alt[2], c->f[AFID_TEMP], c->alt[3], c->f[AFID_PR],
-                          c->f[AFID_RUX], c->f[AFID_RUZ], c->f[AFID_RUTEMP], c->f[AFID_DPH]})
-            arrays.push_back({a, nb});
+                          c->f[AFID_DPH]})
+            arrays.push_back({a, sizeof(double) * nb});
+        for (int f : {AFID_RUX, AFID_RUZ, AFID_RUTEMP}) {
+            if (c->g[f]) arrays.push_back({c->g[f], sizeof(float) * nb});
+            else arrays.push_back({c->f[f], sizeof(double) * nb});
+        }
     }
     size_t total = 0;
     for (auto &a : arrays) total += a.second;
-    double *save = nullptr;
-    ENSCHK(e, hipMalloc(reinterpret_cast<void **>(&save), sizeof(double) * total));
+    unsigned char *save = nullptr;
+    ENSCHK(e, hipMalloc(reinterpret_cast<void **>(&save), total));
     std::vector<EnsState> st_save(size_t(e->nruns));
     auto copy_all = [&](bool to_save) -> hipError_t {
         size_t off = 0;
         for (auto &a : arrays) {
-            hipError_t r = to_save ? hipMemcpyAsync(save + off, a.first, sizeof(double) * a.second, hipMemcpyDeviceToDevice, e->stream)
-                                   : hipMemcpyAsync(a.first, save + off, sizeof(double) * a.second, hipMemcpyDeviceToDevice, e->stream);
+            hipError_t r = to_save ? hipMemcpyAsync(save + off, a.first, a.second, hipMemcpyDeviceToDevice, e->stream)
+                                   : hipMemcpyAsync(a.first, save + off, a.second, hipMemcpyDeviceToDevice, e->stream);
             if (r != hipSuccess) return r;
             off += a.second;
         }
@@ -599,6 +640,7 @@ int afid_ens_autotune(afid_ens *e, int32_t steps, afid_ens_tuning *best, afid_en
             cands.push_back(afid_ens_tuning{ENSP_NT, 0, lines, 1, 0.0});
     for (int nt : {256, 512, 1024})
         for (int lines : {2, 4, 8, 16}) {
+            if (e->mixed) break;                          // the mixed form runs phased only
             if (ens_lds_bytes(e, nt, lines) == 0) continue;
             if (lines > 2 && ens_fft_lines(e->nzmax, lines, nt) == ens_fft_lines(e->nzmax, lines / 2, nt)) continue;
             for (int g : groups) cands.push_back(afid_ens_tuning{nt, g, lines, 0, 0.0});

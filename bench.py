@@ -143,8 +143,31 @@ def ensemble_cpu_baseline(cfgs, ctls, args):
                        f"worker processes, {steps} RK3 steps each with the same CFL control; slowest worker {el:.1f} s")}
 
 
-def run_ensemble(args):
+def ensemble_form(cfgs, ctls, form, steps, args):
+    """Initialise, tune and time one form of the ensemble."""
     import pyafid
+    prec = pyafid.MIXED if form == "mixed" else pyafid.FP64
+    ens = pyafid.Ensemble([dict(c, precision=prec) for c in cfgs], ctls)
+    for r, c in enumerate(cfgs):
+        ens.upload(r, "tempbp", np.ones((c["nzm"], 1, 1)))
+        ens.upload(r, "temptp", np.zeros((c["nzm"], 1, 1)))
+    ens.create_initial_conditions(3000, 1e-3)
+    log(f"ensemble {form}: {len(cfgs)} runs, device memory {ens.device_bytes() / 2**30:.2f} GiB")
+    best, cands = ens.autotune(steps=5)
+    log(f"ensemble {form}: autotuner picked {best} from {len(cands)} launch shapes")
+    ens.create_initial_conditions(3000, 1e-3)     # the timed run starts from step 0
+    ens.step(args.warmup, graph=True)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ms = ens.step_timed(steps, graph=True)
+    wall = time.perf_counter() - t0
+    states = [ens.run_state(r) for r in range(len(cfgs))]
+    nus = [ens.global_quantities(r)["nu"] for r in range(len(cfgs))]
+    ens.close()
+    return dict(ms=ms, wall=wall, best=best, cands=cands, states=states, nus=nus)
+
+
+def run_ensemble(args):
     cfgs, ctls = ensemble_runs()
     ncell = sum(c["nxm"] * c["nzm"] for c in cfgs)
     steps = args.steps if args.steps is not None else 1000
@@ -155,25 +178,20 @@ def run_ensemble(args):
             cpu = ensemble_cpu_baseline(cfgs, ctls, args)
         except Exception as e:    # reported, never replaces the GPU number
             cpu = {"error": str(e)}
-    ens = pyafid.Ensemble(cfgs, ctls)
-    for r, c in enumerate(cfgs):
-        ens.upload(r, "tempbp", np.ones((c["nzm"], 1, 1)))
-        ens.upload(r, "temptp", np.zeros((c["nzm"], 1, 1)))
-    ens.create_initial_conditions(3000, 1e-3)
-    log(f"ensemble: {len(cfgs)} runs, {ncell} cells, device memory {ens.device_bytes() / 2**30:.2f} GiB")
-    best, cands = ens.autotune(steps=5)
-    log(f"ensemble: autotuner picked {best} from {len(cands)} launch shapes")
-    ens.create_initial_conditions(3000, 1e-3)     # the timed run starts from step 0
-    ens.step(args.warmup, graph=True)
-    ens.synchronize()
-    t0 = time.perf_counter()
-    ms = ens.step_timed(steps, graph=True)
-    wall = time.perf_counter() - t0
-    states = [ens.run_state(r) for r in range(len(cfgs))]
-    nus = [ens.global_quantities(r)["nu"] for r in range(len(cfgs))]
-    ens.close()
+    forms = ["fp64", "mixed"] if args.form == "both" else [args.form]
+    runs = {f: ensemble_form(cfgs, ctls, f, steps, args) for f in forms}
+    hform = "mixed" if "mixed" in runs else "fp64"
+    res = runs[hform]
+    ms, wall, best, cands, states, nus = res["ms"], res["wall"], res["best"], res["cands"], res["states"], res["nus"]
     running = sum(1 for s in states if s["status"] == 0)
     value = ncell * steps / (ms * 1e-3) / 1e6
+    summary = {f: {"value": round(ncell * steps / (r["ms"] * 1e-3) / 1e6, 2), "ms_per_step": round(r["ms"] / steps, 4),
+                   "tuning": r["best"], "max_div": max(s["max_div"] for s in r["states"]),
+                   "exit_codes": sorted({s["status"] for s in r["states"]}),
+                   "time_reached": [round(min(s["time"] for s in r["states"]), 5), round(max(s["time"] for s in r["states"]), 5)]}
+               for f, r in runs.items()}
+    if len(runs) == 2:
+        summary["speedup_mixed_over_fp64"] = round(runs["fp64"]["ms"] / runs["mixed"]["ms"], 3)
     nsub = 3 * steps
     avg_launch_ms = ms / nsub
     per_launch = ensemble_bytes_per_cell() * ncell
@@ -181,8 +199,9 @@ def run_ensemble(args):
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": steps, "warmup": args.warmup, "ms_per_step": round(ms / steps, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": ENSEMBLE, "runs": len(cfgs), "cells": ncell,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64 (mixed form)" if hform == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": ENSEMBLE, "form": hform, "runs": len(cfgs), "cells": ncell,
                    "nx": [64, 96, 128], "nz": [128, 256, 512], "aspect": 2, "Ra": [1e6, 1e9], "Pr": [1, 7],
                    "cflmax": 0.5, "dt0": 1e-4, "dtmax": 1e-2, "graph": True, "tuning": best},
         "roofline": {"bound": "hbm",
@@ -196,6 +215,7 @@ def run_ensemble(args):
         "exit_codes": sorted({s["status"] for s in states}),
         "time_reached": [round(min(s["time"] for s in states), 5), round(max(s["time"] for s in states), 5)],
         "nu_vol": [round(min(nus), 4), round(max(nus), 4)],
+        "forms": summary,
         "autotune_candidates": cands,
         "cpu_baseline": cpu,
     }

@@ -11,7 +11,11 @@
  * kernel launch; afid_ens_step replays a captured graph of such launches.
  *
  * A run is described by the afid_config of its bou.in (nym must be 1; no
- * multires, salinity or phase field; FP64) and an afid_run_control with the
+ * multires, salinity or phase field; every run of an ensemble has the same
+ * precision: AFID_FP64, or AFID_MIXED with FP32 explicit terms and Poisson
+ * sweeps, FP64 state and solves, and further sweeps while the FP64 max|div u|
+ * exceeds div_tol, which runs only the phased launch shape) and an
+ * afid_run_control with the
  * time-stepping line of its bou.in (IDTV, DT, RESID, CFLMAX, DTMIN, DTMAX)
  * and its NTST / TMAX.  v is identically zero in such a run (every y
  * difference vanishes when nym = 1), so the ensemble does not advance it.

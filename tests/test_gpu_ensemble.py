@@ -256,3 +256,58 @@ def test_rejects_unsupported_runs():
     with pytest.raises(pyafid.AfidError, match="identically zero"):
         ens.upload(0, "vy", np.ones((16, 1, 17)))
     ens.close()
+
+
+# ---- mixed FP32/FP64 form of the ensemble ----
+def test_mixed_ensemble_vs_fp64():
+    """Mixed form (FP32 explicit terms and Poisson sweeps, FP64 state,
+    solves and residual; further sweeps while max|div u| > 1e-10) against
+    the FP64 ensemble from the same states: max|div u| <= 1e-10 in every
+    run, Nu within 1e-6, relative L2 of the fields <= 1e-5."""
+    cs = cfgs()
+    ctl = dict(variable_dt=1, dt=1e-3, limit_cfl=0.5, dtmax=1e-2)
+    a = _fresh(cs, ctl, seed=41)
+    b = pyafid.Ensemble([dict(c, precision=pyafid.MIXED) for c in cs], [ctl] * len(cs))
+    for r, c in enumerate(cs):
+        for name, arr in random_run_state(c, 41 + r).items():
+            b.upload(r, name, arr)
+    b.reset()
+    a.step(6, graph=False)
+    b.step(6, graph=True)
+    for r, c in enumerate(cs):
+        sa, sb = a.run_state(r), b.run_state(r)
+        assert sb["status"] == 0 and sb["steps"] == 6
+        assert sb["max_div"] <= 1e-10, (r, sb["max_div"])
+        nxm = c["nxm"]
+        for f, sl in (("vx", np.s_[..., 1:nxm]), ("vz", np.s_[..., :nxm]), ("temp", np.s_[..., :nxm])):
+            x, y = b.download(r, f)[sl], a.download(r, f)[sl]
+            e = float(np.linalg.norm((x - y).ravel()) / np.linalg.norm(y.ravel()))
+            assert e <= 1e-5, f"run {r} {f}: relative L2 {e:.3e}"
+        na, nb = a.global_quantities(r)["nu"], b.global_quantities(r)["nu"]
+        assert abs(nb - na) <= 1e-6 * abs(na)
+    a.close()
+    b.close()
+
+
+def test_mixed_ensemble_shapes_bit_identical_and_phased_only():
+    cs = cfgs()
+    ctl = dict(variable_dt=1, dt=1e-3, limit_cfl=0.5, dtmax=1e-2)
+    mk = lambda: pyafid.Ensemble([dict(c, precision=pyafid.MIXED) for c in cs], [ctl] * len(cs))
+    outs = []
+    for lines, graph in ((2, False), (1, True), (4, True)):
+        e = mk()
+        for r, c in enumerate(cs):
+            for name, arr in random_run_state(c, 51 + r).items():
+                e.upload(r, name, arr)
+        e.reset()
+        e.set_tuning(phased=1, lines=lines)
+        e.step(12, graph=graph)
+        outs.append(_fields(e))
+        if lines == 2:
+            with pytest.raises(pyafid.AfidError, match="phased"):
+                e.set_tuning(threads=1024, phased=0)
+        e.close()
+    for o in outs[1:]:
+        for r in range(len(cs)):
+            for f in o[r]:
+                assert np.array_equal(o[r][f], outs[0][r][f])
