@@ -744,6 +744,7 @@ struct Walk3 {
 constexpr int VXR_TJ = 16, VXR_TI = 4;      // refined columns per vxr tile
 constexpr int VYZ_LINES = 4;                // refined lines (waves) per vyr/vzr tile
 constexpr int VYZ_CHUNK = 16;               // refined points per scan chunk
+constexpr int VXR_BU = 4, VYZ_BU = 4;       // coarse-block items loaded per batch and thread
 
 // Inclusive prefix sum over the 64 lanes of a wavefront with DPP lane
 // moves: Kogge-Stone within each row of 16 lanes (row_shr 1, 2, 4, 8),
@@ -817,18 +818,33 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
         __syncthreads();
         {
             // coarse gradient dvx/dx at reference x index xc with the wall
-            // ghosts (InterpVelMgrd.F90:37-51); zero outside -1..nx+1
+            // ghosts (InterpVelMgrd.F90:37-51); zero outside -1..nx+1.
+            // The loads of VXR_BU items are issued before any is used: an
+            // item outside the block or the grid reads a valid address.
+            const int nB = XC * CJ * CI;
             Walk3 w(t, 256, XC, CJ);
-            for (int q = t; q < XC * CJ * CI; q += 256, w.next()) {
-                const int xc = x0 + w.a;
-                RT g = RT(0);
-                if (xc >= 0 && xc <= p.nx) {
-                    const int k = xc == 0 ? 0 : (xc == p.nx ? p.nxm - 1 : xc - 1);
-                    const double *col = p.vx + coff[w.c * CJ + w.b];
-                    const double d = (col[k + 1] - col[k]) * p.udx3m[k];
-                    g = RT(xc == 0 ? sS * d : (xc == p.nx ? sN * d : d));
+            for (int q0 = t; q0 < nB; q0 += 256 * VXR_BU) {
+                double a0[VXR_BU], a1[VXR_BU], u[VXR_BU];
+                int dst[VXR_BU], xs[VXR_BU];
+#pragma unroll
+                for (int r = 0; r < VXR_BU; ++r) {
+                    const int xc = x0 + w.a;
+                    const bool live = q0 + 256 * r < nB, ok = live && xc >= 0 && xc <= p.nx;
+                    const int k = !ok ? 0 : (xc == 0 ? 0 : (xc == p.nx ? p.nxm - 1 : xc - 1));
+                    const long long off = coff[min(w.c, CI - 1) * CJ + min(w.b, CJ - 1)];
+                    const double *col = p.vx + (ok ? off : 0);
+                    a0[r] = col[k]; a1[r] = col[k + 1]; u[r] = p.udx3m[k];
+                    dst[r] = live ? (w.c * cjmax + w.b) * xcmax + w.a : -1;
+                    xs[r] = ok ? xc : -1;
+                    w.next();
                 }
-                B[(w.c * cjmax + w.b) * xcmax + w.a] = g;
+#pragma unroll
+                for (int r = 0; r < VXR_BU; ++r)
+                    if (dst[r] >= 0) {
+                        const int xc = xs[r];
+                        const double d = (a1[r] - a0[r]) * u[r];
+                        B[dst[r]] = xc < 0 ? RT(0) : RT(xc == 0 ? sS * d : (xc == p.nx ? sN * d : d));
+                    }
             }
         }
         __syncthreads();
@@ -955,17 +971,33 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
         {
             // coarse gradient d(vy)/dy or d(vz)/dz at reference x index xc
             // with the wall ghosts (InterpVelMgrd.F90:75-86, :161-172)
+            // (loads of VYZ_BU items issued before any is used; an item
+            // outside the block or the grid reads a valid address)
+            const int nB = XC * CS * CO;
             Walk3 w(t, 256, XC, CS);
-            for (int q = t; q < XC * CS * CO; q += 256, w.next()) {
-                const int xc = x0 + w.a;
-                RT g = RT(0);
-                if (xc >= 0 && xc <= p.nx) {
-                    const int k = xc == 0 ? 0 : (xc == p.nx ? p.nxm - 1 : xc - 1);
-                    const double *cb = vel + ooff[w.c] + k;
-                    const double d = (cb[snext[w.b]] - cb[soff[w.b]]) * dgrid;
-                    g = RT(xc == 0 ? sSg * d : (xc == p.nx ? sNg * d : d));
+            for (int q0 = t; q0 < nB; q0 += 256 * VYZ_BU) {
+                double a0[VYZ_BU], a1[VYZ_BU];
+                int dst[VYZ_BU], xs[VYZ_BU];
+#pragma unroll
+                for (int r = 0; r < VYZ_BU; ++r) {
+                    const int xc = x0 + w.a;
+                    const bool live = q0 + 256 * r < nB, ok = live && xc >= 0 && xc <= p.nx;
+                    const int k = !ok ? 0 : (xc == 0 ? 0 : (xc == p.nx ? p.nxm - 1 : xc - 1));
+                    const int c = min(w.c, CO - 1), b = min(w.b, CS - 1);
+                    const double *cb = vel + (ok ? ooff[c] : 0) + k;
+                    a1[r] = cb[ok ? snext[b] : 0];
+                    a0[r] = cb[ok ? soff[b] : 0];
+                    dst[r] = live ? (w.c * csmax + w.b) * xcmax + w.a : -1;
+                    xs[r] = ok ? xc : -1;
+                    w.next();
                 }
-                B[(w.c * csmax + w.b) * xcmax + w.a] = g;
+#pragma unroll
+                for (int r = 0; r < VYZ_BU; ++r)
+                    if (dst[r] >= 0) {
+                        const int xc = xs[r];
+                        const double d = (a1[r] - a0[r]) * dgrid;
+                        B[dst[r]] = xc < 0 ? RT(0) : RT(xc == 0 ? sSg * d : (xc == p.nx ? sNg * d : d));
+                    }
             }
         }
         __syncthreads();
@@ -1005,6 +1037,8 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
 // buffered), contracted over x and y, and the last four contracted planes are
 // kept in registers for the z contraction.
 constexpr int SALC_TJ = 8, SALC_TZ = 16;
+constexpr int SALC_BU = 4;      // plane items loaded per batch and thread
+constexpr int SALC_IN = -(1 << 30);   // marks an interior point of a staged plane
 // What a refined-to-coarse tile interpolates and where the result goes.
 enum CoarseMode {
     CM_SAL = 0,     // S with the wall ghosts of salinity.F90:466-482 -> salc (salc(nx) = 0)
@@ -1076,13 +1110,26 @@ __global__ void __launch_bounds__(256) k_interp_coarse_tile(KP p, CoarseJob jb, 
         RT *B = sm + ((zr - zfirst) & 1) * xrmax * yrmax;     // [yr][xr], double buffered
         const int ii = wrapn(zr, p.nzr);
         const RT *fz = src + (long long)ii * p.planer;
+        // SALC_BU loads per thread in flight: interior points read first
+        // (an item outside the plane reads a valid address), wall ghosts and
+        // points beyond them are filled in afterwards
         Walk3 w(t, 256, XR, YR);
-        for (int q = t; q < plane; q += 256, w.next()) {
-            const int kr = x0 + w.a;
-            RT v;
-            if (kr >= 1 && kr <= p.nxmr) v = fz[(long long)rowj[w.b] * p.ldxr + kr - 1];   // interior (most points)
-            else v = coarse_src<RT>(p, jb, src, kr, rowj[w.b], ii);                       // wall ghosts, beyond
-            B[w.b * xrmax + w.a] = v;
+        for (int q0 = t; q0 < plane; q0 += 256 * SALC_BU) {
+            RT v[SALC_BU];
+            int dst[SALC_BU], kx[SALC_BU], jy[SALC_BU];
+#pragma unroll
+            for (int r = 0; r < SALC_BU; ++r) {
+                const int kr = x0 + w.a, jj = rowj[min(w.b, YR - 1)];
+                const bool live = q0 + 256 * r < plane, in = live && kr >= 1 && kr <= p.nxmr;
+                v[r] = fz[(long long)jj * p.ldxr + (in ? kr - 1 : 0)];
+                dst[r] = live ? w.b * xrmax + w.a : -1;
+                kx[r] = in ? SALC_IN : kr;
+                jy[r] = jj;
+                w.next();
+            }
+#pragma unroll
+            for (int r = 0; r < SALC_BU; ++r)
+                if (dst[r] >= 0) B[dst[r]] = kx[r] == SALC_IN ? v[r] : coarse_src<RT>(p, jb, src, kx[r], jy[r], ii);
         }
         __syncthreads();
         for (int h = 0; h < 2; ++h) {
