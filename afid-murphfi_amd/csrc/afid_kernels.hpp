@@ -368,18 +368,26 @@ __device__ __forceinline__ void column_solve_masked_lds(double *buf, const doubl
 template <typename RT> struct Pair;
 template <> struct Pair<float> { using T = float2; };
 template <> struct Pair<double> { using T = double2; };
+// 16-byte vector of the storage type: 4 floats or 2 doubles per lane access
+template <typename RT> struct Vec16;
+template <> struct Vec16<float> { using T = float4; static constexpr int n = 4; };
+template <> struct Vec16<double> { using T = double2; static constexpr int n = 2; };
+template <typename RT>
+__device__ __forceinline__ RT vget(const typename Vec16<RT>::T &v, int m) { return reinterpret_cast<const RT *>(&v)[m]; }
 
 // ExplicitSalinity + ImplicitSalinity + SolveImpEqnUpdate_Sal (salinity.F90:268-454).
 // RT is the storage (and explicit-stencil arithmetic) type of the refined
 // fields; the implicit RHS, the solve and the update run in FP64.
-// Stencil phase: each lane takes two adjacent rows (r0 = 2*lane + 128*m and
-// r0+1) with one vector access per array, so a wavefront access moves 128
-// values; the x-neighbours outside the pair come from two scalar loads that
-// hit the same cache lines.
+// Stencil phase: each lane takes SUB adjacent rows (r0 = SUB*lane + 64*SUB*m
+// .. r0+SUB-1, 16 bytes) with one vector access per array, so a wavefront
+// access moves 1 KiB; the x-neighbours outside the group come from two
+// scalar loads that hit the same cache lines.  The RHS goes through LDS to
+// the solver's layout (CH rows per lane).
 template <int CH, typename RT>
 __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_out) {
-    using V = typename Pair<RT>::T;
-    constexpr int M2 = (64 * CH + 127) / 128;          // pair iterations covering the solver's 64*CH rows
+    using V = typename Vec16<RT>::T;
+    constexpr int SUB = Vec16<RT>::n;
+    constexpr int MG = (64 * CH + 64 * SUB - 1) / (64 * SUB);   // group iterations covering the solver's 64*CH rows
     __shared__ double lds[4][lds_col_size<CH>()];
     // the solver's factors, read by every column of the workgroup
     __shared__ double fac[4][64 * CH];
@@ -395,38 +403,53 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_
     const long long c0 = (long long)i * p.planer + (long long)j * p.ldxr;
     RT *rus = sel<RT>(p.rusal, p.rusalf) + c0;
     const unsigned nxmr = p.nxmr;
-    V own[M2];
+    V own[MG];
 #pragma unroll
-    for (int m = 0; m < M2; ++m) {
-        const unsigned r0 = 2 * lane + 128 * m, r1 = r0 + 1;
-        double rhs0 = 0.0, rhs1 = 0.0;
-        own[m] = V{RT(0), RT(0)};
-        if (live && r0 < nxmr) {
-            // r0 even and every row starts 8-aligned: the pair is aligned, and
-            // r0+1 <= nxmr < ldxr stays inside the row
-            const V sc = *reinterpret_cast<const V *>(n.s + r0);
-            const V sjm = *reinterpret_cast<const V *>(n.s_jm + r0), sjp = *reinterpret_cast<const V *>(n.s_jp + r0);
-            const V sim = *reinterpret_cast<const V *>(n.s_im + r0), sip = *reinterpret_cast<const V *>(n.s_ip + r0);
-            const V vx = *reinterpret_cast<const V *>(n.vxr + r0);
-            const V vy = *reinterpret_cast<const V *>(n.vyr + r0), vyp = *reinterpret_cast<const V *>(n.vyr_jp + r0);
-            const V vz = *reinterpret_cast<const V *>(n.vzr + r0), vzp = *reinterpret_cast<const V *>(n.vzr_ip + r0);
-            const V ru = *reinterpret_cast<const V *>(rus + r0);
+    for (int m = 0; m < MG; ++m) {
+        const unsigned r0 = SUB * lane + 64 * SUB * m;
+        double rhs[SUB];
+#pragma unroll
+        for (int q = 0; q < SUB; ++q) { rhs[q] = 0.0; reinterpret_cast<RT *>(&own[m])[q] = RT(0); }
+        if (live && r0 + SUB <= nxmr) {
+            // r0 a multiple of SUB and every row 8-aligned: the group is one
+            // aligned 16-byte access; r0 + SUB <= nxmr < ldxr stays in the row
+            auto ld = [&](const RT *a) { return *reinterpret_cast<const V *>(a + r0); };
+            const V sc = ld(n.s), sjm = ld(n.s_jm), sjp = ld(n.s_jp), sim = ld(n.s_im), sip = ld(n.s_ip);
+            const V vx = ld(n.vxr), vy = ld(n.vyr), vyp = ld(n.vyr_jp), vz = ld(n.vzr), vzp = ld(n.vzr_ip), ru = ld(rus);
             const RT sm = r0 > 0 ? n.s[r0 - 1] : RT(0);
-            const RT sp2 = r1 + 1 < nxmr ? n.s[r1 + 1] : RT(0);
-            const RT vx2 = r1 < nxmr ? n.vxr[r1 + 1] : RT(0);     // r1 + 1 <= nxmr < ldxr
-            const RT h0 = ex_sal_val<RT>(p, r0, sm, sc.x, sc.y, sjm.x, sjp.x, sim.x, sip.x, vx.x, vx.y, vy.x, vyp.x, vz.x, vzp.x, n.sbp, n.stp);
-            rhs0 = rhs_sal_val(p, r0, double(h0), double(ru.x), double(sm), double(sc.x), double(sc.y), n.sbp, n.stp);
+            const RT sn = r0 + SUB < nxmr ? n.s[r0 + SUB] : RT(0);
+            const RT vn = n.vxr[r0 + SUB];
+            V h;
+#pragma unroll
+            for (int q = 0; q < SUB; ++q) {
+                const unsigned k = r0 + q;
+                const RT a = q == 0 ? sm : vget<RT>(sc, q - 1), b = vget<RT>(sc, q);
+                const RT c = q == SUB - 1 ? sn : vget<RT>(sc, q + 1);
+                const RT x1 = q == SUB - 1 ? vn : vget<RT>(vx, q + 1);
+                const RT hq = ex_sal_val<RT>(p, k, a, b, c, vget<RT>(sjm, q), vget<RT>(sjp, q), vget<RT>(sim, q),
+                                             vget<RT>(sip, q), vget<RT>(vx, q), x1, vget<RT>(vy, q), vget<RT>(vyp, q),
+                                             vget<RT>(vz, q), vget<RT>(vzp, q), n.sbp, n.stp);
+                reinterpret_cast<RT *>(&h)[q] = hq;
+                rhs[q] = rhs_sal_val(p, k, double(hq), double(vget<RT>(ru, q)), double(a), double(b), double(c), n.sbp, n.stp);
+            }
             own[m] = sc;
-            if (r1 < nxmr) {
-                const RT h1 = ex_sal_val<RT>(p, r1, sc.x, sc.y, sp2, sjm.y, sjp.y, sim.y, sip.y, vx.y, vx2, vy.y, vyp.y, vz.y, vzp.y, n.sbp, n.stp);
-                rhs1 = rhs_sal_val(p, r1, double(h1), double(ru.y), double(sc.x), double(sc.y), double(sp2), n.sbp, n.stp);
-                *reinterpret_cast<V *>(rus + r0) = V{h0, h1};
-            } else {
-                rus[r0] = h0;
+            *reinterpret_cast<V *>(rus + r0) = h;
+        } else if (live) {
+            // the group holding row nxmr - 1 when nxmr is not a multiple of SUB
+#pragma unroll
+            for (int q = 0; q < SUB; ++q) {
+                const unsigned k = r0 + q;
+                if (k < nxmr) {
+                    const RT hq = ex_sal<RT, RT>(p, n, k);
+                    rhs[q] = rhs_sal<RT>(p, n, double(hq), double(rus[k]), k);
+                    reinterpret_cast<RT *>(&own[m])[q] = n.s[k];
+                    rus[k] = hq;
+                }
             }
         }
-        if (r0 < 64u * CH) buf[lds_row<CH>(r0)] = rhs0;
-        if (r1 < 64u * CH) buf[lds_row<CH>(r1)] = rhs1;
+#pragma unroll
+        for (int q = 0; q < SUB; ++q)
+            if (r0 + q < 64u * CH) buf[lds_row<CH>(r0 + q)] = rhs[q];
     }
     __syncthreads();
     column_solve_lds<CH>(buf, fl);
@@ -434,12 +457,18 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_
     if (!live) return;
     RT *out = sal_out + c0;
 #pragma unroll
-    for (int m = 0; m < M2; ++m) {
-        const unsigned r0 = 2 * lane + 128 * m, r1 = r0 + 1;
+    for (int m = 0; m < MG; ++m) {
+        const unsigned r0 = SUB * lane + 64 * SUB * m;
         if (r0 >= nxmr) continue;
-        const RT o0 = RT(double(own[m].x) + buf[lds_row<CH>(r0)]);
-        if (r1 < nxmr) *reinterpret_cast<V *>(out + r0) = V{o0, RT(double(own[m].y) + buf[lds_row<CH>(r1)])};
-        else out[r0] = o0;
+        V o;
+#pragma unroll
+        for (int q = 0; q < SUB; ++q)
+            reinterpret_cast<RT *>(&o)[q] = RT(double(vget<RT>(own[m], q)) + (r0 + q < 64u * CH ? buf[lds_row<CH>(r0 + q)] : 0.0));
+        if (r0 + SUB <= nxmr) *reinterpret_cast<V *>(out + r0) = o;
+        else
+#pragma unroll
+            for (int q = 0; q < SUB; ++q)
+                if (r0 + q < nxmr) out[r0 + q] = vget<RT>(o, q);
     }
     if (lane == 0) out[nxmr] = n.s[nxmr];       // the unused index nxr is carried over
 }
