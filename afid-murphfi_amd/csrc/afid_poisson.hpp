@@ -94,6 +94,16 @@ struct DftSmall<8, R> {
 // butterflies of a stage stay in registers between its read and write phase.
 constexpr int FFT_THREADS = 1024, FFT_MAXV = 8;
 
+// Division by a workgroup-uniform d through one multiply: for x, d < 2^16,
+// floor(x / d) = umulhi(x, ceil(2^32 / d)) (the error term x*e/2^32 < 2^-16
+// never reaches the next integer, since frac(x/d) <= 1 - 1/d).  d = 1 is
+// passed through.
+struct FastDiv {
+    unsigned d, m;
+    __device__ __forceinline__ explicit FastDiv(unsigned d_) : d(d_), m(d_ > 1 ? 0xFFFFFFFFu / d_ + 1u : 0u) {}
+    __device__ __forceinline__ unsigned div(unsigned x) const { return d == 1 ? x : __umulhi(x, m); }
+};
+
 // LDS layout of L lines of N complex values: element j of line l sits at
 // l*S + lpad(j), with one pad element per 32 banks of data (16 FP32 or 8
 // FP64 complex values) so the stride-R writes of a Stockham stage spread
@@ -111,12 +121,13 @@ __host__ __device__ __forceinline__ int line_stride(int n) { return lpad<R>(n) +
 template <int RD, int MAXB, typename R, int NT>
 __device__ __forceinline__ void fft_stage(cpx<R> *buf, int L, int N, int S, int Ns, const cpx<R> *tw, int sign) {
     const int nbl = N / RD, nb = L * nbl;
+    const FastDiv dnbl(nbl), dns(Ns);
     cpx<R> v[MAXB][RD];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
         const int b = threadIdx.x + q * NT;
         if (b < nb) {
-            const int line = b / nbl, jj = b - line * nbl;
+            const int line = dnbl.div(b), jj = b - line * nbl;
 #pragma unroll
             for (int r = 0; r < RD; ++r) v[q][r] = buf[line * S + lpad<R>(jj + r * nbl)];
         }
@@ -127,7 +138,7 @@ __device__ __forceinline__ void fft_stage(cpx<R> *buf, int L, int N, int S, int 
     for (int q = 0; q < MAXB; ++q) {
         const int b = threadIdx.x + q * NT;
         if (b < nb) {
-            const int line = b / nbl, jj = b - line * nbl, kk = jj % Ns;
+            const int line = dnbl.div(b), jj = b - line * nbl, jq = dns.div(jj), kk = jj - jq * Ns;
 #pragma unroll
             for (int r = 1; r < RD; ++r) {
                 cpx<R> w = tw[kk * r * tstride];
@@ -135,7 +146,7 @@ __device__ __forceinline__ void fft_stage(cpx<R> *buf, int L, int N, int S, int 
                 v[q][r] = cmul(v[q][r], w);
             }
             DftSmall<RD, R>::run(v[q], sign);
-            const int dst = (jj / Ns) * Ns * RD + kk;
+            const int dst = jq * Ns * RD + kk;
 #pragma unroll
             for (int r = 0; r < RD; ++r) buf[line * S + lpad<R>(dst + r * Ns)] = v[q][r];
         }
@@ -180,8 +191,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_div_fft_y(KP p, FftPlan<R> pl, 
     const double usdtal = 1.0 / (p.dt * p.al);
     if (slots_clear && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) slots_clear[threadIdx.x] = 0ull;
     double dmax = 0.0;
+    const FastDiv dtx(TX);
     for (int t = threadIdx.x; t < TX * N; t += FFT_THREADS) {
-        const int xo = t % TX, j = t / TX, k = k0 + xo;
+        const int j = dtx.div(t), xo = t - j * TX, k = k0 + xo;
         double d = 0.0;
         if (k < p.nxm) {
             if (src == SRC_DPH) d = p.dph[bidx(p, k, j, i)];
@@ -205,7 +217,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_div_fft_y(KP p, FftPlan<R> pl, 
     lds_fft<R>(lb, L, S, pl, -1);
     const R scale = R(1.0 / (double(p.ny) * double(p.nz)));
     for (int t = threadIdx.x; t < TX * p.nymh; t += FFT_THREADS) {
-        const int xo = t % TX, m = t / TX, k = k0 + xo;
+        const int m = dtx.div(t), xo = t - m * TX, k = k0 + xo;
         if (k >= p.nxm) continue;
         const int line = xo >> 1;
         const cpx<R> zm = lb[line * S + lpad<R>(m)], zn = lb[line * S + lpad<R>(m == 0 ? 0 : N - m)];
@@ -226,14 +238,15 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_z(KP p, FftPlan<R> pl, cpx<
     const int N = p.nz, S = line_stride<R>(N), m = blockIdx.y, k0 = blockIdx.x * L;
     const long long sz = (long long)p.nymh * p.nxm;
     cpx<R> *g = spec + (long long)m * p.nxm;
+    const FastDiv dl(L);
     for (int t = threadIdx.x; t < L * N; t += FFT_THREADS) {
-        const int xo = t % L, i = t / L, k = k0 + xo;
+        const int i = dl.div(t), xo = t - i * L, k = k0 + xo;
         lb[xo * S + lpad<R>(i)] = k < p.nxm ? g[i * sz + k] : mk<R>(R(0), R(0));
     }
     __syncthreads();
     lds_fft<R>(lb, L, S, pl, sign);
     for (int t = threadIdx.x; t < L * N; t += FFT_THREADS) {
-        const int xo = t % L, i = t / L, k = k0 + xo;
+        const int i = dl.div(t), xo = t - i * L, k = k0 + xo;
         if (k < p.nxm) g[i * sz + k] = lb[xo * S + lpad<R>(i)];
     }
 }
@@ -248,8 +261,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_ifft_y(KP p, FftPlan<R> pl, con
     extern __shared__ __align__(16) unsigned char smraw[];
     cpx<R> *lb = reinterpret_cast<cpx<R> *>(smraw);
     const int N = p.ny, S = line_stride<R>(N), TX = 2 * L, i = blockIdx.y, k0 = blockIdx.x * TX;
+    const FastDiv dl(L), dtx(TX);
     for (int t = threadIdx.x; t < L * p.nymh; t += FFT_THREADS) {
-        const int line = t % L, m = t / L;
+        const int m = dl.div(t), line = t - m * L;
         const int ka = k0 + 2 * line, kb = ka + 1;
         const cpx<R> *row = spec + ((long long)i * p.nymh + m) * p.nxm;
         cpx<R> A = ka < p.nxm ? row[ka] : mk<R>(R(0), R(0));
@@ -261,7 +275,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_ifft_y(KP p, FftPlan<R> pl, con
     __syncthreads();
     lds_fft<R>(lb, L, S, pl, +1);
     for (int t = threadIdx.x; t < TX * N; t += FFT_THREADS) {
-        const int xo = t % TX, j = t / TX, k = k0 + xo;
+        const int j = dtx.div(t), xo = t - j * TX, k = k0 + xo;
         if (k >= p.nxm) continue;
         const cpx<R> z = lb[(xo >> 1) * S + lpad<R>(j)];
         p.dph[bidx(p, k, j, i)] = double((xo & 1) ? z.y : z.x);
