@@ -799,7 +799,7 @@ static int setup(afid_ctx *c) {
     if (c->vxr_cj * c->vxr_ci > 256) return set_err(c, AFID_E_ARG, "refinement ratio too small for the vxr tiles");
     const size_t es = c->mixed ? sizeof(float) : sizeof(double);      // element size of the transfer LDS
     c->lds_vxr = es * size_t(c->vxr_xc * c->vxr_cj * c->vxr_ci + c->vxr_xc * c->vxr_cj * VXR_TI +
-                             c->vxr_xc * VXR_TJ * VXR_TI + VXR_TJ * VXR_TI);
+                             c->vxr_xc * VXR_TJ * VXR_TI);
     c->vy_cs = span(sy.src, nyr - 1, VYZ_CHUNK); c->vy_co = span(sz.src, nzr, VYZ_LINES);
     c->vz_cs = span(sz.src, nzr - 1, VYZ_CHUNK); c->vz_co = span(sy.src, nyr, VYZ_LINES);
     if (std::max(std::max(c->vy_cs, c->vz_cs), std::max(c->vy_co, c->vz_co)) > 32)

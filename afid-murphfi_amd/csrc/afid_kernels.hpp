@@ -754,6 +754,10 @@ __device__ __forceinline__ int wrapf(int j, int n) {
     return j;
 }
 
+// (c * B + b) * A + a for staging-block indices (all below 2^24): 24-bit
+// multiplies, which issue at full rate
+__device__ __forceinline__ int ix3(int c, int B, int b, int A, int a) { return __mul24(__mul24(c, B) + b, A) + a; }
+
 // Components (a, b, c) of a linear index q = a + A (b + B c) that advances by a
 // fixed step: the divisions happen once, each step costs a few adds.
 struct Walk3 {
@@ -789,6 +793,15 @@ __device__ __forceinline__ double dpp_move(double v) {
     const int lo = dpp32<CTRL, ROWS, BC>(int(b & 0xffffffffLL)), hi = dpp32<CTRL, ROWS, BC>(int(b >> 32));
     return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
+// value of lane 63 (the wavefront total of an inclusive scan), uniform
+__device__ __forceinline__ float wave_lane63(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+__device__ __forceinline__ double wave_lane63(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane(int(b & 0xffffffffLL), 63), hi = __builtin_amdgcn_readlane(int(b >> 32), 63);
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
     v += dpp_move<0x111, 0xf, true>(v);      // row_shr:1
@@ -805,7 +818,7 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
 // RT: storage and arithmetic type of the transfer (double or float).  The
 // tile's coarse column offsets and its y / z stencils are staged in LDS once.
 template <typename RT>
-__global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, int cimax) {
+__device__ __forceinline__ void vxr_tile(const KP &p, int xcmax, int cjmax, int cimax) {
     extern __shared__ __align__(16) unsigned char smraw[];
     __shared__ long long coff[16 * 16];                 // coarse column offsets [ci][cj] (cimax, cjmax <= 16)
     __shared__ RT yw[VXR_TJ][4], zw[VXR_TI][4];
@@ -821,7 +834,6 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
     RT *B = sm;                                       // [ci][cj][xc]
     RT *Z = B + xcmax * cjmax * cimax;                // [ti][cj][xc]
     RT *Q = Z + xcmax * cjmax * VXR_TI;               // [ti][tj][xc]
-    RT *carry = Q + xcmax * VXR_TJ * VXR_TI;          // [ti][tj]
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     for (int q = t; q < CI * CJ; q += 256) {
         const int c = q / CJ, b = q - c * CJ;
@@ -837,9 +849,15 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
     }
     for (int q = t; q < nj * ni; q += 256) {
         const int ti = q / nj, tj = q - ti * nj;
-        carry[ti * VXR_TJ + tj] = RT(0);
         vxr[ridx(p, 0, J0 + tj, I0 + ti)] = RT(0);      // vxr(1) = 0
     }
+    // running sums of the columns (ti, wave + 4u) this wavefront writes,
+    // carried from one x chunk to the next
+    RT run[VXR_TI][VXR_TJ / 4];
+#pragma unroll
+    for (int ti = 0; ti < VXR_TI; ++ti)
+#pragma unroll
+        for (int u = 0; u < VXR_TJ / 4; ++u) run[ti][u] = RT(0);
     const int sS = 1 - 2 * p.inslws, sN = 1 - 2 * p.inslwn;
     for (int R0 = 0; R0 < p.nxmr; R0 += 64) {
         const int R1 = min(R0 + 64, p.nxmr);
@@ -863,7 +881,7 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
                     const long long off = coff[min(w.c, CI - 1) * CJ + min(w.b, CJ - 1)];
                     const double *col = p.vx + (ok ? off : 0);
                     a0[r] = col[k]; a1[r] = col[k + 1]; u[r] = p.udx3m[k];
-                    dst[r] = live ? (w.c * cjmax + w.b) * xcmax + w.a : -1;
+                    dst[r] = live ? ix3(w.c, cjmax, w.b, xcmax, w.a) : -1;
                     xs[r] = ok ? xc : -1;
                     w.next();
                 }
@@ -880,16 +898,16 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
         Walk3 wz(t, 256, XC, CJ);
         for (int q = t; q < XC * CJ * ni; q += 256, wz.next()) {
             const int xc = wz.a, cj = wz.b, ti = wz.c;
-            const RT *b = B + (zo[ti] * cjmax + cj) * xcmax + xc;
-            const int st = cjmax * xcmax;
-            Z[(ti * cjmax + cj) * xcmax + xc] = b[0] * zw[ti][0] + b[st] * zw[ti][1] + b[2 * st] * zw[ti][2] + b[3 * st] * zw[ti][3];
+            const RT *b = B + ix3(zo[ti], cjmax, cj, xcmax, xc);
+            const int st = __mul24(cjmax, xcmax);
+            Z[ix3(ti, cjmax, cj, xcmax, xc)] = b[0] * zw[ti][0] + b[st] * zw[ti][1] + b[2 * st] * zw[ti][2] + b[3 * st] * zw[ti][3];
         }
         __syncthreads();
         Walk3 wy(t, 256, XC, nj);
         for (int q = t; q < XC * nj * ni; q += 256, wy.next()) {
             const int xc = wy.a, tj = wy.b, ti = wy.c;
-            const RT *z = Z + (ti * cjmax + yo[tj]) * xcmax + xc;
-            Q[(ti * VXR_TJ + tj) * xcmax + xc] = z[0] * yw[tj][0] + z[xcmax] * yw[tj][1] + z[2 * xcmax] * yw[tj][2] + z[3 * xcmax] * yw[tj][3];
+            const RT *z = Z + ix3(ti, cjmax, yo[tj], xcmax, xc);
+            Q[ix3(ti, VXR_TJ, tj, xcmax, xc)] = z[0] * yw[tj][0] + z[xcmax] * yw[tj][1] + z[2 * xcmax] * yw[tj][2] + z[3 * xcmax] * yw[tj][3];
         }
         __syncthreads();
         const int kr = R0 + lane;
@@ -901,19 +919,31 @@ __global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, in
             wx0 = xs_w[4 * kr]; wx1 = xs_w[4 * kr + 1]; wx2 = xs_w[4 * kr + 2]; wx3 = xs_w[4 * kr + 3];
             lx = lxa[kr];
         }
-        for (int ti = 0; ti < ni; ++ti) {
+#pragma unroll
+        for (int ti = 0; ti < VXR_TI; ++ti) {
+            if (ti >= ni) break;
             RT *orow = vxr + (long long)(I0 + ti) * p.planer + (long long)J0 * p.ldxr + kr + 1;
-            for (int tj = wave; tj < nj; tj += 4) {
-                const RT *qq = Q + (ti * VXR_TJ + tj) * xcmax + xa;
+#pragma unroll
+            for (int u = 0; u < VXR_TJ / 4; ++u) {
+                const int tj = wave + 4 * u;
+                if (tj >= nj) break;
+                const RT *qq = Q + ix3(ti, VXR_TJ, tj, xcmax, xa);
                 RT v = in ? (qq[0] * wx0 + qq[1] * wx1 + qq[2] * wx2 + qq[3] * wx3) * lx : RT(0);
                 v = wave_incl_scan(v);
-                const RT base = carry[ti * VXR_TJ + tj];
+                const RT base = run[ti][u];
                 if (in) orow[(long long)tj * p.ldxr] = base + v;
-                const RT tot = __shfl(v, 63, 64);
-                if (lane == 0) carry[ti * VXR_TJ + tj] = base + tot;
+                run[ti][u] = base + wave_lane63(v);
             }
         }
     }
+}
+
+template <typename RT>
+__global__ void __launch_bounds__(256) k_vxr_tile(KP p, int xcmax, int cjmax, int cimax) { vxr_tile<RT>(p, xcmax, cjmax, cimax); }
+// FP32: 96 registers leave room for the 5 workgroups per CU its LDS allows
+template <>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_vxr_tile<float>(KP p, int xcmax, int cjmax, int cimax) {
+    vxr_tile<float>(p, xcmax, cjmax, cimax);
 }
 
 // vyr (AX = 1) / vzr (AX = 2): lanes = 64 refined x, waves = VYZ_LINES lines
@@ -1016,7 +1046,7 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
                     const double *cb = vel + (ok ? ooff[c] : 0) + k;
                     a1[r] = cb[ok ? snext[b] : 0];
                     a0[r] = cb[ok ? soff[b] : 0];
-                    dst[r] = live ? (w.c * csmax + w.b) * xcmax + w.a : -1;
+                    dst[r] = live ? ix3(w.c, csmax, w.b, xcmax, w.a) : -1;
                     xs[r] = ok ? xc : -1;
                     w.next();
                 }
@@ -1030,17 +1060,24 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
             }
         }
         __syncthreads();
-        Walk3 wo(t, 256, XC, CS);
-        for (int q = t; q < XC * CS * nl; q += 256, wo.next()) {
-            const int xc = wo.a, cs = wo.b, l = wo.c;
-            const RT *b = B + (lo[l] * csmax + cs) * xcmax + xc;
-            const int st = csmax * xcmax;
-            O[(l * csmax + cs) * xcmax + xc] = b[0] * lw[l][0] + b[st] * lw[l][1] + b[2 * st] * lw[l][2] + b[3 * st] * lw[l][3];
+        // contraction over the other axis: one (cs, xc) per item, all lines
+        {
+            const int st = __mul24(csmax, xcmax);
+            Walk3 wo(t, 256, XC, CS);
+            for (int q = t; q < XC * CS; q += 256, wo.next()) {
+                const int base = __mul24(wo.b, xcmax) + wo.a;
+#pragma unroll
+                for (int l = 0; l < VYZ_LINES; ++l)
+                    if (l < nl) {
+                        const RT *b = B + __mul24(lo[l], st) + base;
+                        O[__mul24(l, st) + base] = b[0] * lw[l][0] + b[st] * lw[l][1] + b[2 * st] * lw[l][2] + b[3 * st] * lw[l][3];
+                    }
+            }
         }
         __syncthreads();
         if (live && kr < R1)
             for (int cs = 0; cs < CS; ++cs) {
-                const RT *o = O + (wave * csmax + cs) * xcmax + xa;
+                const RT *o = O + ix3(wave, csmax, cs, xcmax, xa);
                 X[(cs * VYZ_LINES + wave) * 64 + lane] = o[0] * wx[0] + o[1] * wx[1] + o[2] * wx[2] + o[3] * wx[3];
             }
         __syncthreads();
@@ -1151,7 +1188,7 @@ __global__ void __launch_bounds__(256) k_interp_coarse_tile(KP p, CoarseJob jb, 
                 const int kr = x0 + w.a, jj = rowj[min(w.b, YR - 1)];
                 const bool live = q0 + 256 * r < plane, in = live && kr >= 1 && kr <= p.nxmr;
                 v[r] = fz[(long long)jj * p.ldxr + (in ? kr - 1 : 0)];
-                dst[r] = live ? w.b * xrmax + w.a : -1;
+                dst[r] = live ? __mul24(w.b, xrmax) + w.a : -1;
                 kx[r] = in ? SALC_IN : kr;
                 jy[r] = jj;
                 w.next();
@@ -1165,7 +1202,7 @@ __global__ void __launch_bounds__(256) k_interp_coarse_tile(KP p, CoarseJob jb, 
             RT v = RT(0);
             if (kok && rok[h]) {
                 for (int b = 0; b < 4; ++b) {
-                    const RT *row = B + (ya[h] + b) * xrmax + xa;
+                    const RT *row = B + __mul24(ya[h] + b, xrmax) + xa;
                     const RT q1 = row[0] * wx[0] + row[1] * wx[1] + row[2] * wx[2] + row[3] * wx[3];
                     v = b == 0 ? q1 * wy[h][0] : v + q1 * wy[h][b];
                 }
