@@ -22,6 +22,7 @@
 #include "afid_kernels.hpp"
 #include "afid_poisson.hpp"
 #include "afid_phase.hpp"
+#include "afid_means.hpp"
 
 using namespace afid;
 
@@ -89,6 +90,10 @@ struct afid_ctx {
     const double *d_xm = nullptr, *d_xmr = nullptr;
     const double *d_udx3m, *d_udx3c, *d_rudx3m, *d_ap3ck, *d_ac3ck, *d_am3ck, *d_ap3sk, *d_ac3sk, *d_am3sk;
     const double *d_ap3ssk, *d_ac3ssk, *d_am3ssk, *d_amphk, *d_acphk, *d_apphk, *d_g3rc;
+    // CalcMeanProfiles: dx/g3rm per cell, the per-group partial sums and the profiles
+    const double *d_sdx = nullptr;
+    double *d_mean_part = nullptr, *d_mean_out = nullptr;
+    int mean_groups = 0;
     const double *d_udx3mr = nullptr, *d_ap3sskr = nullptr, *d_ac3sskr = nullptr, *d_am3sskr = nullptr, *d_lxa = nullptr;
     const int *d_xs_src = nullptr, *d_ys_src = nullptr, *d_zs_src = nullptr, *d_xb_src = nullptr, *d_yb_src = nullptr, *d_zb_src = nullptr;
     const double *d_xs_w = nullptr, *d_ys_w = nullptr, *d_zs_w = nullptr, *d_xb_w = nullptr, *d_yb_w = nullptr, *d_zb_w = nullptr;
@@ -508,6 +513,9 @@ static int setup(afid_ctx *c) {
     std::vector<double> udx3m_v = farr_range(udx3m, 1, nxm), udx3c_v = farr_range(udx3c, 1, nx), g3rc_v = farr_range(g3rc, 1, nx);
     std::vector<double> xm_v = farr_range(xm, 1, nxm);
     up(xm_v, &c->d_xm);
+    std::vector<double> sdx_v;
+    for (int k = 1; k <= nxm; ++k) sdx_v.push_back(c->dx / g3rm(k));     // CalcMeanProfiles.F90:116-123
+    up(sdx_v, &c->d_sdx);
     up(udx3m_v, &c->d_udx3m); up(udx3c_v, &c->d_udx3c); up(rudx3m, &c->d_rudx3m); up(g3rc_v, &c->d_g3rc);
     up(c->h_ap3ck, &c->d_ap3ck); up(c->h_ac3ck, &c->d_ac3ck); up(c->h_am3ck, &c->d_am3ck);
     up(c->h_ap3sk, &c->d_ap3sk); up(c->h_ac3sk, &c->d_ac3sk); up(c->h_am3sk, &c->d_am3sk);
@@ -1492,6 +1500,29 @@ int afid_global_quantities(afid_ctx *c, double out[4]) {
     out[1] = 0.5 * s[2] * vol;
     out[2] = std::sqrt(s[2] * vol) * std::sqrt(std::fabs(c->c.rayt) / c->c.prat);  // :91
     out[3] = s[1] * vol;                                                            // :89
+    return AFID_OK;
+}
+
+int afid_calc_mean_profiles(afid_ctx *c, double *out) {
+    if (!c || !out) return AFID_E_ARG;
+    if (c->nxm < 2) return set_err(c, AFID_E_ARG, "CalcMeanProfiles needs nxm >= 2");
+    int rc;
+    const int nxm = c->nxm;
+    if (!c->d_mean_part) {
+        const long long ncol = (long long)c->ny * c->nz;
+        c->mean_groups = int(std::min<long long>(MEAN_GROUPS, (ncol + 3) / 4));
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_mean_part), sizeof(double) * size_t(c->mean_groups) * MEAN_NQ * nxm))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_mean_out), sizeof(double) * size_t(MEAN_NQ) * nxm))) return rc;
+    }
+    KP p = make_kp(c);
+    hipLaunchKernelGGL(k_mean_partials, dim3(unsigned((nxm + 63) / 64), unsigned(c->mean_groups)), dim3(256), 0, c->stream, p,
+                       c->d_sdx, c->d_mean_part);
+    if ((rc = check_launch(c, "k_mean_partials"))) return rc;
+    hipLaunchKernelGGL(k_mean_finish, dim3(unsigned((MEAN_NQ * nxm + 255) / 256)), dim3(256), 0, c->stream, p, c->d_mean_part,
+                       c->mean_groups, c->d_mean_out);
+    if ((rc = check_launch(c, "k_mean_finish"))) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_mean_out, sizeof(double) * size_t(MEAN_NQ) * nxm, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return AFID_OK;
 }
 

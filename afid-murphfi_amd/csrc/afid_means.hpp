@@ -1,0 +1,140 @@
+// afid_means.hpp -- wall-parallel mean profiles (CalcMeanProfiles.F90:11-229)
+// on the base grid: the 15 profiles the reference writes to means.h5, each
+// of nxm values, in its write order (CalcMeanProfiles.F90:230-283).
+//
+// Two kernels.  k_mean_partials: a block of 64 x points (one per lane) and 4
+// wavefronts; each wavefront walks every (4 * groups)-th (j, i) column of
+// its group and accumulates the 15 per-x sums of CalcMeanProfiles.F90:44-73,
+// :104-127 and :165-215 in FP64 registers; the four wavefronts' sums are
+// combined in LDS and written to part[group][q][k].  k_mean_finish: one
+// thread per (q, k) sums the groups in a fixed order and applies the
+// normalisations of :87-101 and :219-223, so a call's result does not
+// depend on scheduling.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "afid_params.hpp"
+#include "afid_stencils.hpp"
+
+namespace afid {
+
+constexpr int MEAN_NQ = 15;         // Tbar vybar vzbar vxT vyT vzT Trms vxrms vyrms vzrms vxvy vxvz vyvz chiT epsilon
+constexpr int MEAN_GROUPS = 512;    // column groups (second grid dimension) of k_mean_partials
+
+// sdx[k] = dx / g3rm(k+1) (CreateGrid.F90:120-123), k = 0..nxm-1
+__global__ void __launch_bounds__(256) k_mean_partials(KP p, const double *__restrict__ sdx, double *__restrict__ part) {
+    __shared__ double red[4][MEAN_NQ][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane, nxm = p.nxm;
+    const bool kok = k < nxm;
+    double a[MEAN_NQ];
+#pragma unroll
+    for (int q = 0; q < MEAN_NQ; ++q) a[q] = 0.0;
+    const int ncol = p.ny * p.nz, step = 4 * gridDim.y;
+    const double dy = p.dy, dz = p.dz;
+    if (kok) {
+        const int kp = k + 1, km = k - 1;
+        const double s0 = sdx[k];
+        for (int col = blockIdx.y * 4 + wave; col < ncol; col += step) {
+            const int i = col / p.ny, j = col - i * p.ny;
+            const int ip = wrapp(i, p.nz), im = wrapm(i, p.nz), jp = wrapp(j, p.ny), jm = wrapm(j, p.ny);
+            auto at = [&](const double *f, int kk, int jj, int ii) { return f[bidx(p, kk, jj, ii)]; };
+            const double *vx = p.vx, *vy = p.vy, *vz = p.vz, *tp = p.temp;
+            // :44-73
+            const double t = at(tp, k, j, i);
+            const double vxk = at(vx, k, j, i), vxp = at(vx, kp, j, i);
+            const double vyk = at(vy, k, j, i), vyp = at(vy, k, jp, i);
+            const double vzk = at(vz, k, j, i), vzp = at(vz, k, j, ip);
+            a[0] += t;
+            a[1] += 0.5 * (vyk + vyp);
+            a[2] += 0.5 * (vzk + vzp);
+            a[3] += 0.5 * (vxk + vxp) * t;
+            a[4] += 0.5 * (vyk + vyp) * t;
+            a[5] += 0.5 * (vzk + vzp) * t;
+            a[6] += t * t;
+            a[7] += 0.5 * (vxk * vxk + vxp * vxp);
+            a[8] += 0.5 * (vyk * vyk + vyp * vyp);
+            a[9] += 0.5 * (vzk * vzk + vzp * vzp);
+            a[10] += 0.25 * (vxk + vxp) * (vyk + vyp);
+            a[11] += 0.25 * (vxk + vxp) * (vzk + vzp);
+            a[12] += 0.25 * (vyk + vyp) * (vzk + vzp);
+            // chiT, :104-127: y/z terms at every k, then the x term (wall
+            // forms at the first and last cell, tdx = 0.5 dx / g3rm)
+            {
+                const double ga = (at(tp, k, j, ip) - at(tp, k, j, im)) * 0.5 * dz;
+                const double gb = (at(tp, k, jp, i) - at(tp, k, jm, i)) * 0.5 * dy;
+                double ch = ga * ga + gb * gb;
+                const double tdx = 0.5 * s0;
+                if (k == 0) {
+                    const double w = (at(tp, 1, j, i) - t + 2.0 * p.tfixs * (t - p.tempbp[(long long)i * p.ny + j])) * tdx;
+                    ch += w * w;
+                }
+                if (k >= 1 && k <= nxm - 2) {
+                    const double d = (at(tp, kp, j, i) - at(tp, km, j, i)) * tdx;
+                    ch += d * d;
+                }
+                if (k == nxm - 1) {
+                    const double w = (t - at(tp, km, j, i) + 2.0 * p.tfixn * (p.temptp[(long long)i * p.ny + j] - t)) * tdx;
+                    ch += w * w;
+                }
+                a[13] += ch;
+            }
+            // epsilon, :165-215
+            {
+                const double e1 = (vzp - vzk) * dz;
+                const double b1 = at(vy, k, jp, ip) - at(vy, k, jp, im), b2 = at(vy, k, j, ip) - at(vy, k, j, im);
+                const double c1 = at(vx, kp, j, ip) - at(vx, kp, j, im), c2 = at(vx, k, j, ip) - at(vx, k, j, im);
+                const double d1 = at(vz, k, jp, ip) - at(vz, k, jm, ip), d2 = at(vz, k, jp, i) - at(vz, k, jm, i);
+                const double e2 = (vyp - vyk) * dy;
+                const double f1 = at(vx, kp, jp, i) - at(vx, kp, jm, i), f2 = at(vx, k, jp, i) - at(vx, k, jm, i);
+                const double g = (vxp - vxk) * p.udx3m[k];
+                double ep = e1 * e1 + 0.125 * dz * dz * (b1 * b1 + b2 * b2) + 0.125 * dz * dz * (c1 * c1 + c2 * c2)
+                          + 0.125 * dy * dy * (d1 * d1 + d2 * d2) + e2 * e2 + 0.125 * dy * dy * (f1 * f1 + f2 * f2) + g * g;
+                const double tdx = 0.125 * s0 * s0;
+                if (k == 0) {
+                    const double z1 = at(vz, 1, j, ip) - vzp + p.inslws * 2.0 * (vzp - 0.0);
+                    const double z2 = at(vz, 1, j, i) - vzk + p.inslws * 2.0 * (vzk - 0.0);
+                    const double y1 = at(vy, 1, jp, i) - vyp + p.inslws * 2.0 * (vyp - p.xminusu);
+                    const double y2 = at(vy, 1, j, i) - vyk + p.inslws * 2.0 * (vyk - p.xminusu);
+                    ep += tdx * (z1 * z1 + z2 * z2) + tdx * (y1 * y1 + y2 * y2);
+                }
+                if (k >= 1 && k <= nxm - 2) {
+                    const double p1 = at(vz, kp, j, ip) - at(vz, km, j, ip), p2 = at(vz, kp, j, i) - at(vz, km, j, i);
+                    const double q1 = at(vy, kp, jp, i) - at(vy, km, jp, i), q2 = at(vy, kp, j, i) - at(vy, km, j, i);
+                    ep += tdx * (p1 * p1 + p2 * p2) + tdx * (q1 * q1 + q2 * q2);
+                }
+                if (k == nxm - 1) {
+                    const double z1 = vzp - at(vz, km, j, ip) + p.inslwn * 2.0 * (0.0 - vzp);
+                    const double z2 = vzk - at(vz, km, j, i) + p.inslwn * 2.0 * (0.0 - vzk);
+                    const double y1 = vyp - at(vy, km, jp, i) + p.inslwn * 2.0 * (p.xplusu - vyp);
+                    const double y2 = vyk - at(vy, km, j, i) + p.inslwn * 2.0 * (p.xplusu - vyk);
+                    ep += tdx * (z1 * z1 + z2 * z2) + tdx * (y1 * y1 + y2 * y2);
+                }
+                a[14] += ep;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < MEAN_NQ; ++q) red[wave][q][lane] = a[q];
+    __syncthreads();
+    if (wave == 0 && kok) {
+#pragma unroll
+        for (int q = 0; q < MEAN_NQ; ++q)
+            part[((long long)blockIdx.y * MEAN_NQ + q) * nxm + k] = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+    }
+}
+
+// sum of the groups in order and the normalisations; out[q * nxm + k]
+__global__ void __launch_bounds__(256) k_mean_finish(KP p, const double *__restrict__ part, int groups, double *__restrict__ out) {
+    const int t = blockIdx.x * 256 + threadIdx.x, nxm = p.nxm;
+    if (t >= MEAN_NQ * nxm) return;
+    const int q = t / nxm, k = t - q * nxm;
+    double s = 0.0;
+    for (int g = 0; g < groups; ++g) s += part[((long long)g * MEAN_NQ + q) * nxm + k];
+    s = s * (1.0 / double(p.ny)) * (1.0 / double(p.nz));     // *inym*inzm, left to right
+    if (q >= 6 && q <= 9) s = sqrt(s);            // Trms, vxrms, vyrms, vzrms (:95-98)
+    else if (q == 13) s /= p.pect;                 // chiT (:220)
+    else if (q == 14) s /= p.ren;                  // epsilon (:222)
+    out[t] = s;
+}
+
+}  // namespace afid

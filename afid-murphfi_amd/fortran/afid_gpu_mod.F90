@@ -133,6 +133,10 @@ module afid_gpu
         integer(c_int) function afid_global_quantities(ctx, out) bind(C, name='afid_global_quantities')
             import; type(c_ptr), value :: ctx; real(c_double), intent(out) :: out(4)
         end function
+        ! CalcMeanProfiles: out(nxm, 15) in the order of means.h5 (Tbar, vybar, ..., chiT, epsilon)
+        integer(c_int) function afid_calc_mean_profiles(ctx, out) bind(C, name='afid_calc_mean_profiles')
+            import; type(c_ptr), value :: ctx; real(c_double), intent(out) :: out(*)
+        end function
         integer(c_int) function afid_create_initial_conditions(ctx, seed, eps) bind(C, name='afid_create_initial_conditions')
             import; type(c_ptr), value :: ctx; integer(c_int64_t), value :: seed; real(c_double), value :: eps
         end function

@@ -35,10 +35,14 @@ EXPORTS = [
     "afid_explicit_phase", "afid_add_volume_penalty", "afid_implicit_phase", "afid_add_latent_heat",
     "afid_interp_temp_multigrid", "afid_interp_phi_multigrid",
     "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher", "afid_refinement_residuals",
-    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_synchronize",
+    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_synchronize",
     "afid_create_initial_conditions",
     "afid_set_timing", "afid_get_timing", "afid_grid_array",
 ]
+
+# the profiles of afid_calc_mean_profiles, in the order of means.h5 (CalcMeanProfiles.F90:230-283)
+MEAN_PROFILES = ["Tbar", "vybar", "vzbar", "vxT", "vyT", "vzT", "Trms", "vxrms", "vyrms", "vzrms",
+                 "vxvy", "vxvz", "vyvz", "chiT", "epsilon"]
 
 # every entry point declared in include/afid_ensemble.h
 ENS_EXPORTS = [
@@ -161,6 +165,7 @@ def lib():
         L.afid_calc_max_cfl.argtypes = [vp, dp, dp]
         L.afid_check_divergence.argtypes = [vp, dp, dp]
         L.afid_global_quantities.argtypes = [vp, dp]
+        L.afid_calc_mean_profiles.argtypes = [vp, dp]
         L.afid_synchronize.argtypes = [vp]
         L.afid_create_initial_conditions.argtypes = [vp, C.c_uint64, C.c_double]
         L.afid_set_timing.argtypes = [vp, C.c_int]
@@ -286,6 +291,14 @@ class Solver:
         out = (C.c_double * 4)()
         self._check(self.L.afid_global_quantities(self.h, out), "afid_global_quantities")
         return dict(zip(["nu", "ke", "vrms", "tmean"], list(out)))
+
+    def mean_profiles(self):
+        """CalcMeanProfiles (CalcMeanProfiles.F90:11-229): dict of the 15 profiles
+        (each nxm values) in the reference's means.h5 order."""
+        nxm = int(self.cfg.nxm)
+        out = np.zeros(len(MEAN_PROFILES) * nxm)
+        self._check(self.L.afid_calc_mean_profiles(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "afid_calc_mean_profiles")
+        return {n: out[q * nxm:(q + 1) * nxm].copy() for q, n in enumerate(MEAN_PROFILES)}
 
     def refinement_residuals(self):
         """Mixed form: max|div u| before each refinement sweep of the last substep."""

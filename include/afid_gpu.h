@@ -150,6 +150,13 @@ int afid_calc_max_cfl(afid_ctx *ctx, double *cflm, double *cflmr);      /* CalcM
 int afid_check_divergence(afid_ctx *ctx, double *qmax, double *qmaxr);  /* CheckDivergence.F90:11-87 */
 /* GlobalQuantities.F90:11-102: out = {Nu_vol, mean kinetic energy, rms velocity, mean T}. */
 int afid_global_quantities(afid_ctx *ctx, double out[4]);
+/* CalcMeanProfiles.F90:11-229: the 15 wall-parallel mean profiles of the base
+ * grid, out[q * nxm + k] (k = 0..nxm-1) with q in the order the reference
+ * writes them to means.h5 (:230-283): Tbar, vybar, vzbar, vxT, vyT, vzT, Trms,
+ * vxrms, vyrms, vzrms, vxvy, vxvz, vyvz, chiT, epsilon.  out holds 15*nxm
+ * doubles; the host writes the file (HdfSerialWriteReal1D).  FP64 in both
+ * forms.  Needs nxm >= 2.  Waits for the result. */
+int afid_calc_mean_profiles(afid_ctx *ctx, double *out);
 
 /* Seeded initial state (CreateInitialConditions.F90:150-201, RBC branch, with
  * u = 0): T = tempbp + (temptp - tempbp) x + noise, the wall-clamped noise of

@@ -1764,6 +1764,134 @@ void orc_check_divergence(orc_state *s, double *qmax, double *qmaxr) {
     } else *qmaxr = m;
 }
 
+/* CalcMeanProfiles.F90:11-229 : the 15 wall-parallel mean profiles, each of
+ * nxm values, in the order the reference writes them to means.h5
+ * (:230-283): Tbar, vybar, vzbar, vxT, vyT, vzT, Trms, vxrms, vyrms, vzrms,
+ * vxvy, vxvz, vyvz, chiT, epsilon.  Sums in the reference's loop order (one
+ * rank), then the normalisations of :87-101 and :219-223. */
+void orc_mean_profiles(orc_state *s, double *out) {
+    BASE_DECL
+    const double *vx = s->f[ORC_VX], *vy = s->f[ORC_VY], *vz = s->f[ORC_VZ], *temp = s->f[ORC_TEMP];
+    const double *tbp = s->f[ORC_TEMPBP], *ttp = s->f[ORC_TEMPTP];
+    double *Tbar = out, *vybar = out + nxm, *vzbar = out + 2 * nxm, *vxT = out + 3 * nxm, *vyT = out + 4 * nxm;
+    double *vzT = out + 5 * nxm, *Trms = out + 6 * nxm, *vxrms = out + 7 * nxm, *vyrms = out + 8 * nxm;
+    double *vzrms = out + 9 * nxm, *vxvy = out + 10 * nxm, *vxvz = out + 11 * nxm, *vyvz = out + 12 * nxm;
+    double *chiT = out + 13 * nxm, *eps = out + 14 * nxm;
+    for (int q = 0; q < 15 * nxm; q++) out[q] = 0.0;
+    double *T = Tbar - 1, *VYB = vybar - 1, *VZB = vzbar - 1, *VXT = vxT - 1, *VYT = vyT - 1, *VZT = vzT - 1;
+    double *TR = Trms - 1, *VXR = vxrms - 1, *VYR = vyrms - 1, *VZR = vzrms - 1, *XY = vxvy - 1, *XZ = vxvz - 1;
+    double *YZ = vyvz - 1, *CH = chiT - 1, *EP = eps - 1;       /* 1-based views */
+    const double inym = 1.0 / nym, inzm = 1.0 / nzm, dy = s->dy, dz = s->dz, dx = s->dx;
+    /* :44-73 */
+    for (int ic = 1; ic <= nzm; ic++) {
+        int i0 = W(ic, nzm), ip = W(ic + 1, nzm);
+        for (int jc = 1; jc <= nym; jc++) {
+            int j0 = W(jc, nym), jp = W(jc + 1, nym);
+            for (int k = 1; k <= nxm; k++) {
+                int kp = k + 1;
+                double t = temp[B(k, j0, i0)];
+                double vxk = vx[B(k, j0, i0)], vxp = vx[B(kp, j0, i0)];
+                double vyk = vy[B(k, j0, i0)], vyp = vy[B(k, jp, i0)];
+                double vzk = vz[B(k, j0, i0)], vzp = vz[B(k, j0, ip)];
+                T[k] = T[k] + t;
+                VYB[k] = VYB[k] + 0.5 * (vyk + vyp);
+                VZB[k] = VZB[k] + 0.5 * (vzk + vzp);
+                VXT[k] = VXT[k] + 0.5 * (vxk + vxp) * t;
+                VYT[k] = VYT[k] + 0.5 * (vyk + vyp) * t;
+                VZT[k] = VZT[k] + 0.5 * (vzk + vzp) * t;
+                TR[k] = TR[k] + t * t;
+                VXR[k] = VXR[k] + 0.5 * (vxk * vxk + vxp * vxp);
+                VYR[k] = VYR[k] + 0.5 * (vyk * vyk + vyp * vyp);
+                VZR[k] = VZR[k] + 0.5 * (vzk * vzk + vzp * vzp);
+                XY[k] = XY[k] + 0.25 * (vxk + vxp) * (vyk + vyp);
+                XZ[k] = XZ[k] + 0.25 * (vxk + vxp) * (vzk + vzp);
+                YZ[k] = YZ[k] + 0.25 * (vyk + vyp) * (vzk + vzp);
+            }
+        }
+    }
+    /* :89-101 */
+    for (int k = 1; k <= nxm; k++) {
+        T[k] = T[k] * inym * inzm;
+        VYB[k] = VYB[k] * inym * inzm;
+        VZB[k] = VZB[k] * inym * inzm;
+        VXT[k] = VXT[k] * inym * inzm;
+        VYT[k] = VYT[k] * inym * inzm;
+        VZT[k] = VZT[k] * inym * inzm;
+        TR[k] = sqrt(TR[k] * inym * inzm);
+        VXR[k] = sqrt(VXR[k] * inym * inzm);
+        VYR[k] = sqrt(VYR[k] * inym * inzm);
+        VZR[k] = sqrt(VZR[k] * inym * inzm);
+        XY[k] = XY[k] * inym * inzm;
+        XZ[k] = XZ[k] * inym * inzm;
+        YZ[k] = YZ[k] * inym * inzm;
+    }
+    /* chiT, :104-127 (tdx = 0.5*dx/g3rm(k)) */
+    for (int ic = 1; ic <= nzm; ic++) {
+        int i0 = W(ic, nzm), ip = W(ic + 1, nzm), im = W(ic - 1, nzm);
+        for (int jc = 1; jc <= nym; jc++) {
+            int j0 = W(jc, nym), jp = W(jc + 1, nym), jm = W(jc - 1, nym);
+            for (int k = 1; k <= nxm; k++) {
+                double a = (temp[B(k, j0, ip)] - temp[B(k, j0, im)]) * 0.5 * dz;
+                double b = (temp[B(k, jp, i0)] - temp[B(k, jm, i0)]) * 0.5 * dy;
+                CH[k] = CH[k] + a * a + b * b;
+            }
+            double tdx = 0.5 * dx / s->g3rm[1];
+            double w = (temp[B(2, j0, i0)] - temp[B(1, j0, i0)] + 2.0 * s->c.tfixs * (temp[B(1, j0, i0)] - tbp[P2(j0, i0)])) * tdx;
+            CH[1] = CH[1] + w * w;
+            for (int k = 2; k <= nxm - 1; k++) {
+                tdx = 0.5 * dx / s->g3rm[k];
+                double d = (temp[B(k + 1, j0, i0)] - temp[B(k - 1, j0, i0)]) * tdx;
+                CH[k] = CH[k] + d * d;
+            }
+            tdx = 0.5 * dx / s->g3rm[nxm];
+            w = (temp[B(nxm, j0, i0)] - temp[B(nxm - 1, j0, i0)] + 2.0 * s->c.tfixn * (ttp[P2(j0, i0)] - temp[B(nxm, j0, i0)])) * tdx;
+            CH[nxm] = CH[nxm] + w * w;
+        }
+    }
+    /* epsilon, :165-215 */
+    for (int ic = 1; ic <= nzm; ic++) {
+        int i0 = W(ic, nzm), ip = W(ic + 1, nzm), im = W(ic - 1, nzm);
+        for (int jc = 1; jc <= nym; jc++) {
+            int j0 = W(jc, nym), jp = W(jc + 1, nym), jm = W(jc - 1, nym);
+            for (int k = 1; k <= nxm; k++) {
+                int kp = k + 1;
+                double a = (vz[B(k, j0, ip)] - vz[B(k, j0, i0)]) * dz;
+                double b1 = vy[B(k, jp, ip)] - vy[B(k, jp, im)], b2 = vy[B(k, j0, ip)] - vy[B(k, j0, im)];
+                double c1 = vx[B(kp, j0, ip)] - vx[B(kp, j0, im)], c2 = vx[B(k, j0, ip)] - vx[B(k, j0, im)];
+                double d1 = vz[B(k, jp, ip)] - vz[B(k, jm, ip)], d2 = vz[B(k, jp, i0)] - vz[B(k, jm, i0)];
+                double e = (vy[B(k, jp, i0)] - vy[B(k, j0, i0)]) * dy;
+                double f1 = vx[B(kp, jp, i0)] - vx[B(kp, jm, i0)], f2 = vx[B(k, jp, i0)] - vx[B(k, jm, i0)];
+                double g = (vx[B(kp, j0, i0)] - vx[B(k, j0, i0)]) * s->udx3m[k];
+                EP[k] = EP[k] + a * a + 0.125 * dz * dz * (b1 * b1 + b2 * b2) + 0.125 * dz * dz * (c1 * c1 + c2 * c2)
+                        + 0.125 * dy * dy * (d1 * d1 + d2 * d2) + e * e + 0.125 * dy * dy * (f1 * f1 + f2 * f2) + g * g;
+            }
+            double tdx = 0.125 * (dx / s->g3rm[1]) * (dx / s->g3rm[1]);
+            double z1 = vz[B(2, j0, ip)] - vz[B(1, j0, ip)] + s->c.inslws * 2.0 * (vz[B(1, j0, ip)] - 0.0);
+            double z2 = vz[B(2, j0, i0)] - vz[B(1, j0, i0)] + s->c.inslws * 2.0 * (vz[B(1, j0, i0)] - 0.0);
+            double y1 = vy[B(2, jp, i0)] - vy[B(1, jp, i0)] + s->c.inslws * 2.0 * (vy[B(1, jp, i0)] - s->c.xminusu);
+            double y2 = vy[B(2, j0, i0)] - vy[B(1, j0, i0)] + s->c.inslws * 2.0 * (vy[B(1, j0, i0)] - s->c.xminusu);
+            EP[1] = EP[1] + tdx * (z1 * z1 + z2 * z2) + tdx * (y1 * y1 + y2 * y2);
+            for (int k = 2; k <= nxm - 1; k++) {
+                tdx = 0.125 * (dx / s->g3rm[k]) * (dx / s->g3rm[k]);
+                double p1 = vz[B(k + 1, j0, ip)] - vz[B(k - 1, j0, ip)], p2 = vz[B(k + 1, j0, i0)] - vz[B(k - 1, j0, i0)];
+                double q1 = vy[B(k + 1, jp, i0)] - vy[B(k - 1, jp, i0)], q2 = vy[B(k + 1, j0, i0)] - vy[B(k - 1, j0, i0)];
+                EP[k] = EP[k] + tdx * (p1 * p1 + p2 * p2) + tdx * (q1 * q1 + q2 * q2);
+            }
+            tdx = 0.125 * (dx / s->g3rm[nxm]) * (dx / s->g3rm[nxm]);
+            z1 = vz[B(nxm, j0, ip)] - vz[B(nxm - 1, j0, ip)] + s->c.inslwn * 2.0 * (0.0 - vz[B(nxm, j0, ip)]);
+            z2 = vz[B(nxm, j0, i0)] - vz[B(nxm - 1, j0, i0)] + s->c.inslwn * 2.0 * (0.0 - vz[B(nxm, j0, i0)]);
+            y1 = vy[B(nxm, jp, i0)] - vy[B(nxm - 1, jp, i0)] + s->c.inslwn * 2.0 * (s->c.xplusu - vy[B(nxm, jp, i0)]);
+            y2 = vy[B(nxm, j0, i0)] - vy[B(nxm - 1, j0, i0)] + s->c.inslwn * 2.0 * (s->c.xplusu - vy[B(nxm, j0, i0)]);
+            EP[nxm] = EP[nxm] + tdx * (z1 * z1 + z2 * z2) + tdx * (y1 * y1 + y2 * y2);
+        }
+    }
+    /* :219-223 */
+    for (int k = 1; k <= nxm; k++) {
+        CH[k] = CH[k] * inym * inzm / s->pect;
+        EP[k] = EP[k] * inym * inzm / s->ren;
+    }
+}
+
 /* GlobalQuantities.F90:11-102 : Nu_vol (:79), mean kinetic energy with the same
  * g3rc weights (:55-58), rms velocity (:91) and mean temperature (:89). */
 void orc_global_quantities(orc_state *s, double out[4]) {

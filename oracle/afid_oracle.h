@@ -102,6 +102,9 @@ void orc_time_marcher(orc_state *s, double dt);
 void orc_calc_max_cfl(orc_state *s, double *cflm, double *cflmr);
 void orc_check_divergence(orc_state *s, double *qmax, double *qmaxr);
 void orc_global_quantities(orc_state *s, double out[4]); /* Nu_vol, KE, vrms, Tmean */
+/* CalcMeanProfiles: 15 profiles of nxm values (Tbar, vybar, vzbar, vxT, vyT, vzT, Trms,
+ * vxrms, vyrms, vzrms, vxvy, vxvz, vyvz, chiT, epsilon) */
+void orc_mean_profiles(orc_state *s, double *out);
 
 /* free-standing numerics used by the pins */
 void orc_fft(double *re, double *im, int n, int sign);      /* in place, unnormalised */

@@ -78,6 +78,7 @@ def lib():
         L.orc_calc_max_cfl.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_check_divergence.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_global_quantities.argtypes = [vp, C.POINTER(C.c_double)]
+        L.orc_mean_profiles.argtypes = [vp, C.POINTER(C.c_double)]
         dp = C.POINTER(C.c_double)
         L.orc_fft.argtypes = [dp, dp, C.c_int, C.c_int]
         L.orc_dft_naive.argtypes = [dp, dp, dp, dp, C.c_int, C.c_int]
@@ -169,6 +170,16 @@ class Oracle:
         out = (C.c_double * 4)()
         self.L.orc_global_quantities(self.h, out)
         return dict(zip(["nu", "ke", "vrms", "tmean"], list(out)))
+
+    MEAN_PROFILES = ["Tbar", "vybar", "vzbar", "vxT", "vyT", "vzT", "Trms", "vxrms", "vyrms", "vzrms",
+                     "vxvy", "vxvz", "vyvz", "chiT", "epsilon"]
+
+    def mean_profiles(self):
+        """CalcMeanProfiles.F90:11-229: dict of the 15 profiles (nxm values each)."""
+        nxm = int(self.cfg.nxm)
+        out = np.zeros(15 * nxm)
+        self.L.orc_mean_profiles(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return {n: out[q * nxm:(q + 1) * nxm].copy() for q, n in enumerate(self.MEAN_PROFILES)}
 
 
 DEFAULTS = dict(nxm=16, nym=16, nzm=16, nsst=3, multires=0, nxmr=0, nymr=0, nzmr=0,
