@@ -91,9 +91,9 @@ struct afid_ctx {
     const double *d_udx3m, *d_udx3c, *d_rudx3m, *d_ap3ck, *d_ac3ck, *d_am3ck, *d_ap3sk, *d_ac3sk, *d_am3sk;
     const double *d_ap3ssk, *d_ac3ssk, *d_am3ssk, *d_amphk, *d_acphk, *d_apphk, *d_g3rc;
     // CalcMeanProfiles: dx/g3rm per cell, the per-group partial sums and the profiles
-    const double *d_sdx = nullptr;
-    double *d_mean_part = nullptr, *d_mean_out = nullptr;
-    int mean_groups = 0;
+    const double *d_sdx = nullptr, *d_sdxr = nullptr;
+    double *d_mean_part = nullptr, *d_mean_out = nullptr, *d_sal_part = nullptr, *d_sal_out = nullptr;
+    int mean_groups = 0, sal_groups = 0;
     const double *d_udx3mr = nullptr, *d_ap3sskr = nullptr, *d_ac3sskr = nullptr, *d_am3sskr = nullptr, *d_lxa = nullptr;
     const int *d_xs_src = nullptr, *d_ys_src = nullptr, *d_zs_src = nullptr, *d_xb_src = nullptr, *d_yb_src = nullptr, *d_zb_src = nullptr;
     const double *d_xs_w = nullptr, *d_ys_w = nullptr, *d_zs_w = nullptr, *d_xb_w = nullptr, *d_yb_w = nullptr, *d_zb_w = nullptr;
@@ -745,6 +745,9 @@ static int setup(afid_ctx *c) {
     std::vector<double> udx3mr_v = farr_range(udx3mr, 1, nxmr);
     std::vector<double> xmr_v = farr_range(xmr, 1, nxmr);
     up(xmr_v, &c->d_xmr);
+    std::vector<double> sdxr_v;
+    for (int k = 1; k <= nxmr; ++k) sdxr_v.push_back(c->dxr / g3rmr(k));     // CalcDissipationSal, salinity.F90:620-622
+    up(sdxr_v, &c->d_sdxr);
     up(udx3mr_v, &c->d_udx3mr); up(c->h_ap3sskr, &c->d_ap3sskr); up(c->h_ac3sskr, &c->d_ac3sskr); up(c->h_am3sskr, &c->d_am3sskr);
     up(lxa, &c->d_lxa);
     if (c->pf) {
@@ -1522,6 +1525,31 @@ int afid_calc_mean_profiles(afid_ctx *c, double *out) {
                        c->mean_groups, c->d_mean_out);
     if ((rc = check_launch(c, "k_mean_finish"))) return rc;
     HIPCHK(c, hipMemcpyAsync(out, c->d_mean_out, sizeof(double) * size_t(MEAN_NQ) * nxm, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AFID_OK;
+}
+
+int afid_calc_sal_stats(afid_ctx *c, double *out) {
+    if (!c || !out) return AFID_E_ARG;
+    if (!c->sal) return set_err(c, AFID_E_ARG, "CalcSalStats needs the salinity field");
+    if (c->nxmr < 2) return set_err(c, AFID_E_ARG, "CalcSalStats needs nxmr >= 2");
+    int rc;
+    const int n = c->nxmr;
+    if (!c->d_sal_part) {
+        const long long ncol = (long long)c->nyr * c->nzr;
+        c->sal_groups = int(std::min<long long>(MEAN_GROUPS, (ncol + 3) / 4));
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_sal_part), sizeof(double) * size_t(c->sal_groups) * SAL_NQ * n))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_sal_out), sizeof(double) * size_t(SAL_NQ) * n))) return rc;
+    }
+    KP p = make_kp(c);
+    const dim3 gr(unsigned((n + 63) / 64), unsigned(c->sal_groups));
+    if (c->mixed) hipLaunchKernelGGL(k_sal_stats_partials<float>, gr, dim3(256), 0, c->stream, p, c->d_sdxr, c->d_sal_part);
+    else hipLaunchKernelGGL(k_sal_stats_partials<double>, gr, dim3(256), 0, c->stream, p, c->d_sdxr, c->d_sal_part);
+    if ((rc = check_launch(c, "k_sal_stats_partials"))) return rc;
+    hipLaunchKernelGGL(k_sal_stats_finish, dim3(unsigned((SAL_NQ * n + 255) / 256)), dim3(256), 0, c->stream, p, c->d_sal_part,
+                       c->sal_groups, c->d_sal_out);
+    if ((rc = check_launch(c, "k_sal_stats_finish"))) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_sal_out, sizeof(double) * size_t(SAL_NQ) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return AFID_OK;
 }

@@ -137,4 +137,76 @@ __global__ void __launch_bounds__(256) k_mean_finish(KP p, const double *__restr
     out[t] = s;
 }
 
+// CalcSalStats + CalcDissipationSal (salinity.F90:510-577, :611-690, non-IBM
+// branches) on the refined grid: Sbar, Srms, vxS, vyS, vzS, chiS (each nxmr
+// values, the write order of :583-597).  RT is the storage type of the
+// refined fields (FP32 in the mixed form); every sum is FP64.  sdxr[k] =
+// dxr / g3rmr(k+1) (CreateMgrdGrid).  Same two-kernel structure as the base
+// profiles.
+constexpr int SAL_NQ = 6;
+template <typename RT>
+__global__ void __launch_bounds__(256) k_sal_stats_partials(KP p, const double *__restrict__ sdxr, double *__restrict__ part) {
+    __shared__ double red[4][SAL_NQ][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane, n = p.nxmr;
+    const bool kok = k < n;
+    double a[SAL_NQ];
+#pragma unroll
+    for (int q = 0; q < SAL_NQ; ++q) a[q] = 0.0;
+    if (kok) {
+        const RT *sal = sel<RT>(p.sal, p.salf), *vxr = sel<RT>(p.vxr, p.vxrf), *vyr = sel<RT>(p.vyr, p.vyrf),
+                 *vzr = sel<RT>(p.vzr, p.vzrf);
+        const int ncol = p.nyr * p.nzr, step = 4 * gridDim.y;
+        const double tdx = 0.5 * sdxr[k], hdz = 0.5 * p.dzr, hdy = 0.5 * p.dyr;
+        for (int col = blockIdx.y * 4 + wave; col < ncol; col += step) {
+            const int i = col / p.nyr, j = col - i * p.nyr;
+            const int ip = wrapp(i, p.nzr), im = wrapm(i, p.nzr), jp = wrapp(j, p.nyr), jm = wrapm(j, p.nyr);
+            auto at = [&](const RT *f, int kk, int jj, int ii) { return double(f[ridx(p, kk, jj, ii)]); };
+            const double v = at(sal, k, j, i);
+            a[0] += v;                                                      // :544-552
+            a[1] += v * v;
+            a[2] += 0.5 * (at(vxr, k, j, i) + at(vxr, k + 1, j, i)) * v;    // :555-563
+            a[3] += 0.5 * (at(vyr, k, j, i) + at(vyr, k, jp, i)) * v;
+            a[4] += 0.5 * (at(vzr, k, j, i) + at(vzr, k, j, ip)) * v;
+            const double gz = (at(sal, k, j, ip) - at(sal, k, j, im)) * hdz;   // :677-678
+            a[5] += gz * gz;
+            const double gy = (at(sal, k, jp, i) - at(sal, k, jm, i)) * hdy;
+            a[5] += gy * gy;
+            if (k == 0) {                                                   // :680-682
+                const double w = (at(sal, 1, j, i) - v + 2.0 * p.sfixs * (v - p.salbp[(long long)i * p.nyr + j])) * tdx;
+                a[5] += w * w;
+            }
+            if (k >= 1 && k <= n - 2) {                                     // :683-685
+                const double d = (at(sal, k + 1, j, i) - at(sal, k - 1, j, i)) * tdx;
+                a[5] += d * d;
+            }
+            if (k == n - 1) {                                               // :686-688
+                const double w = (v - at(sal, k - 1, j, i) + 2.0 * p.sfixn * (p.saltp[(long long)i * p.nyr + j] - v)) * tdx;
+                a[5] += w * w;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < SAL_NQ; ++q) red[wave][q][lane] = a[q];
+    __syncthreads();
+    if (wave == 0 && kok) {
+#pragma unroll
+        for (int q = 0; q < SAL_NQ; ++q)
+            part[((long long)blockIdx.y * SAL_NQ + q) * n + k] = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sal_stats_finish(KP p, const double *__restrict__ part, int groups, double *__restrict__ out) {
+    const int t = blockIdx.x * 256 + threadIdx.x, n = p.nxmr;
+    if (t >= SAL_NQ * n) return;
+    const int q = t / n, k = t - q * n;
+    double s = 0.0;
+    for (int g = 0; g < groups; ++g) s += part[((long long)g * SAL_NQ + q) * n + k];
+    const double inyzmr = 1.0 / double(p.nyr) / double(p.nzr);     // :537
+    if (q == 1) s = sqrt(s * inyzmr);                               // :574-581
+    else if (q == 5) s = s / p.pecs * inyzmr;
+    else s = s * inyzmr;
+    out[t] = s;
+}
+
 }  // namespace afid

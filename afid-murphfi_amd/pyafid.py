@@ -35,7 +35,8 @@ EXPORTS = [
     "afid_explicit_phase", "afid_add_volume_penalty", "afid_implicit_phase", "afid_add_latent_heat",
     "afid_interp_temp_multigrid", "afid_interp_phi_multigrid",
     "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher", "afid_refinement_residuals",
-    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_synchronize",
+    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_calc_sal_stats",
+    "afid_synchronize",
     "afid_create_initial_conditions",
     "afid_set_timing", "afid_get_timing", "afid_grid_array",
 ]
@@ -43,6 +44,8 @@ EXPORTS = [
 # the profiles of afid_calc_mean_profiles, in the order of means.h5 (CalcMeanProfiles.F90:230-283)
 MEAN_PROFILES = ["Tbar", "vybar", "vzbar", "vxT", "vyT", "vzT", "Trms", "vxrms", "vyrms", "vzrms",
                  "vxvy", "vxvz", "vyvz", "chiT", "epsilon"]
+# the refined-grid profiles of afid_calc_sal_stats (salinity.F90:583-597)
+SAL_PROFILES = ["Sbar", "Srms", "vxS", "vyS", "vzS", "chiS"]
 
 # every entry point declared in include/afid_ensemble.h
 ENS_EXPORTS = [
@@ -166,6 +169,7 @@ def lib():
         L.afid_check_divergence.argtypes = [vp, dp, dp]
         L.afid_global_quantities.argtypes = [vp, dp]
         L.afid_calc_mean_profiles.argtypes = [vp, dp]
+        L.afid_calc_sal_stats.argtypes = [vp, dp]
         L.afid_synchronize.argtypes = [vp]
         L.afid_create_initial_conditions.argtypes = [vp, C.c_uint64, C.c_double]
         L.afid_set_timing.argtypes = [vp, C.c_int]
@@ -299,6 +303,14 @@ class Solver:
         out = np.zeros(len(MEAN_PROFILES) * nxm)
         self._check(self.L.afid_calc_mean_profiles(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "afid_calc_mean_profiles")
         return {n: out[q * nxm:(q + 1) * nxm].copy() for q, n in enumerate(MEAN_PROFILES)}
+
+    def sal_stats(self):
+        """CalcSalStats (salinity.F90:510-690): dict of the 6 refined profiles
+        (each nxmr values) in the reference's means.h5 order."""
+        n = int(self.cfg.nxmr)
+        out = np.zeros(len(SAL_PROFILES) * n)
+        self._check(self.L.afid_calc_sal_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "afid_calc_sal_stats")
+        return {m: out[q * n:(q + 1) * n].copy() for q, m in enumerate(SAL_PROFILES)}
 
     def refinement_residuals(self):
         """Mixed form: max|div u| before each refinement sweep of the last substep."""

@@ -157,6 +157,12 @@ int afid_global_quantities(afid_ctx *ctx, double out[4]);
  * doubles; the host writes the file (HdfSerialWriteReal1D).  FP64 in both
  * forms.  Needs nxm >= 2.  Waits for the result. */
 int afid_calc_mean_profiles(afid_ctx *ctx, double *out);
+/* CalcSalStats + CalcDissipationSal (salinity.F90:510-690, non-IBM): the 6
+ * refined-grid profiles Sbar, Srms, vxS, vyS, vzS, chiS, out[q * nxmr + k],
+ * the order of means.h5 (:583-597); out holds 6*nxmr doubles.  Sums in FP64
+ * in both forms (the mixed form reads its FP32 refined fields).  Needs the
+ * salinity field and nxmr >= 2.  Waits for the result. */
+int afid_calc_sal_stats(afid_ctx *ctx, double *out);
 
 /* Seeded initial state (CreateInitialConditions.F90:150-201, RBC branch, with
  * u = 0): T = tempbp + (temptp - tempbp) x + noise, the wall-clamped noise of

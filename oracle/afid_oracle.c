@@ -1361,6 +1361,74 @@ void orc_interp_vel_mgrd(orc_state *s) {
     }
 }
 
+/* CalcSalStats + CalcDissipationSal, salinity.F90:510-577 and :611-690 (non-IBM
+ * branches): the 6 refined-grid profiles of means.h5, each of nxmr values, in
+ * the reference's write order (:583-597): Sbar, Srms, vxS, vyS, vzS, chiS.
+ * Sums in the reference's loop order (one rank), then :574-581. */
+void orc_sal_stats(orc_state *s, double *out) {
+    REF_DECL
+    const double *sal = s->f[ORC_SAL], *vxr = s->f[ORC_VXR], *vyr = s->f[ORC_VYR], *vzr = s->f[ORC_VZR];
+    const double *sbp = s->f[ORC_SALBP], *stp = s->f[ORC_SALTP];
+    for (int q = 0; q < 6 * nxmr; q++) out[q] = 0.0;
+    double *Sb = out - 1, *Sr = out + nxmr - 1, *XS = out + 2 * nxmr - 1, *YS = out + 3 * nxmr - 1;
+    double *ZS = out + 4 * nxmr - 1, *CH = out + 5 * nxmr - 1;        /* 1-based views */
+    const double inyzmr = 1.0 / nymr / nzmr;
+    for (int ic = 1; ic <= nzmr; ic++) {          /* :544-552 */
+        int i0 = W(ic, nzmr);
+        for (int jc = 1; jc <= nymr; jc++) {
+            int j0 = W(jc, nymr);
+            for (int k = 1; k <= nxmr; k++) {
+                double v = sal[R(k, j0, i0)];
+                Sb[k] = Sb[k] + v;
+                Sr[k] = Sr[k] + v * v;
+            }
+        }
+    }
+    for (int ic = 1; ic <= nzmr; ic++) {          /* :555-563 */
+        int i0 = W(ic, nzmr), ip = W(ic + 1, nzmr);
+        for (int jc = 1; jc <= nymr; jc++) {
+            int j0 = W(jc, nymr), jp = W(jc + 1, nymr);
+            for (int k = 1; k <= nxmr; k++) {
+                double v = sal[R(k, j0, i0)];
+                XS[k] = XS[k] + 0.5 * (vxr[R(k, j0, i0)] + vxr[R(k + 1, j0, i0)]) * v;
+                YS[k] = YS[k] + 0.5 * (vyr[R(k, j0, i0)] + vyr[R(k, jp, i0)]) * v;
+                ZS[k] = ZS[k] + 0.5 * (vzr[R(k, j0, i0)] + vzr[R(k, j0, ip)]) * v;
+            }
+        }
+    }
+    for (int ic = 1; ic <= nzmr; ic++) {          /* CalcDissipationSal, :668-690 */
+        int i0 = W(ic, nzmr), ip = W(ic + 1, nzmr), im = W(ic - 1, nzmr);
+        for (int jc = 1; jc <= nymr; jc++) {
+            int j0 = W(jc, nymr), jp = W(jc + 1, nymr), jm = W(jc - 1, nymr);
+            for (int k = 1; k <= nxmr; k++) {
+                double a = (sal[R(k, j0, ip)] - sal[R(k, j0, im)]) * 0.5 * s->dzr;
+                CH[k] = CH[k] + a * a;
+                double b = (sal[R(k, jp, i0)] - sal[R(k, jm, i0)]) * 0.5 * s->dyr;
+                CH[k] = CH[k] + b * b;
+            }
+            double t1 = 0.5 * s->dxr / s->g3rmr[1];
+            double w = (sal[R(2, j0, i0)] - sal[R(1, j0, i0)] + 2.0 * s->c.sfixs * (sal[R(1, j0, i0)] - sbp[RP2(j0, i0)])) * t1;
+            CH[1] = CH[1] + w * w;
+            for (int k = 2; k <= nxmr - 1; k++) {
+                double tk = 0.5 * s->dxr / s->g3rmr[k];
+                double d = (sal[R(k + 1, j0, i0)] - sal[R(k - 1, j0, i0)]) * tk;
+                CH[k] = CH[k] + d * d;
+            }
+            double tn = 0.5 * s->dxr / s->g3rmr[nxmr];
+            w = (sal[R(nxmr, j0, i0)] - sal[R(nxmr - 1, j0, i0)] + 2.0 * s->c.sfixn * (stp[RP2(j0, i0)] - sal[R(nxmr, j0, i0)])) * tn;
+            CH[nxmr] = CH[nxmr] + w * w;
+        }
+    }
+    for (int k = 1; k <= nxmr; k++) {             /* :574-581 */
+        Sb[k] = Sb[k] * inyzmr;
+        Sr[k] = sqrt(Sr[k] * inyzmr);
+        XS[k] = XS[k] * inyzmr;
+        YS[k] = YS[k] * inyzmr;
+        ZS[k] = ZS[k] * inyzmr;
+        CH[k] = CH[k] / s->pecs * inyzmr;
+    }
+}
+
 /* ExplicitSalinity, salinity.F90:268-366 (non-IBM branch) */
 void orc_explicit_salinity(orc_state *s) {
     REF_DECL

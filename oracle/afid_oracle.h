@@ -105,6 +105,8 @@ void orc_global_quantities(orc_state *s, double out[4]); /* Nu_vol, KE, vrms, Tm
 /* CalcMeanProfiles: 15 profiles of nxm values (Tbar, vybar, vzbar, vxT, vyT, vzT, Trms,
  * vxrms, vyrms, vzrms, vxvy, vxvz, vyvz, chiT, epsilon) */
 void orc_mean_profiles(orc_state *s, double *out);
+/* CalcSalStats: 6 refined profiles of nxmr values (Sbar, Srms, vxS, vyS, vzS, chiS) */
+void orc_sal_stats(orc_state *s, double *out);
 
 /* free-standing numerics used by the pins */
 void orc_fft(double *re, double *im, int n, int sign);      /* in place, unnormalised */

@@ -79,6 +79,7 @@ def lib():
         L.orc_check_divergence.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_global_quantities.argtypes = [vp, C.POINTER(C.c_double)]
         L.orc_mean_profiles.argtypes = [vp, C.POINTER(C.c_double)]
+        L.orc_sal_stats.argtypes = [vp, C.POINTER(C.c_double)]
         dp = C.POINTER(C.c_double)
         L.orc_fft.argtypes = [dp, dp, C.c_int, C.c_int]
         L.orc_dft_naive.argtypes = [dp, dp, dp, dp, C.c_int, C.c_int]
@@ -180,6 +181,15 @@ class Oracle:
         out = np.zeros(15 * nxm)
         self.L.orc_mean_profiles(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return {n: out[q * nxm:(q + 1) * nxm].copy() for q, n in enumerate(self.MEAN_PROFILES)}
+
+    SAL_PROFILES = ["Sbar", "Srms", "vxS", "vyS", "vzS", "chiS"]
+
+    def sal_stats(self):
+        """CalcSalStats (salinity.F90:510-690): dict of the 6 refined profiles (nxmr values each)."""
+        n = int(self.cfg.nxmr)
+        out = np.zeros(6 * n)
+        self.L.orc_sal_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return {m: out[q * n:(q + 1) * n].copy() for q, m in enumerate(self.SAL_PROFILES)}
 
 
 DEFAULTS = dict(nxm=16, nym=16, nzm=16, nsst=3, multires=0, nxmr=0, nymr=0, nzmr=0,

@@ -79,3 +79,40 @@ def test_mean_profiles_after_steps_and_closed_form():
         e = np.abs(a[n] - b[n]).max() / den
         assert e <= 1e-11, f"{n}: {e:.2e}"
 
+
+
+MR = dict(nxm=16, nym=8, nzm=12, istr3=8, multires=1, salinity=1, nxmr=32, nymr=16, nzmr=24, istr3r=8,
+          rayt=1e6, prat=1.0, rays=-1e6, pras=4.0)
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+@pytest.mark.parametrize("cfg", [MR, dict(MR, nxmr=70, nymr=24, nzmr=12, sfixn=0, istr3r=6, str3=0.0)])
+def test_sal_stats_match_oracle(cfg, form):
+    """CalcSalStats on random refined fields.  The mixed form stores the
+    refined fields in FP32: the oracle gets the values the GPU holds
+    (downloaded), so both sides sum the same numbers in FP64."""
+    g = pyafid.Solver(dict(cfg, precision=pyafid.MIXED if form == "mixed" else pyafid.FP64))
+    o = pyoracle.Oracle(cfg)
+    shp = o.field("sal").shape
+    for q, n in enumerate(("sal", "vxr", "vyr", "vzr")):
+        g.upload(n, uniform(40 + q, shp) - 0.5)
+    g.upload("salbp", 1.0 + 0.1 * uniform(50, o.field("salbp").shape))
+    g.upload("saltp", 0.1 * uniform(51, o.field("saltp").shape))
+    for n in ("sal", "vxr", "vyr", "vzr", "salbp", "saltp"):
+        o.field(n)[...] = g.download(n)
+    a, b = g.sal_stats(), o.sal_stats()
+    for n in pyafid.SAL_PROFILES:
+        e = relerr(a[n], b[n])
+        assert e <= 1e-12, f"{n}: relative error {e:.2e}"
+    c = g.sal_stats()
+    for n in pyafid.SAL_PROFILES:
+        assert np.array_equal(a[n], c[n]), n
+    g.close()
+    o.close()
+
+
+def test_sal_stats_refused_without_salinity():
+    g = pyafid.Solver(dict(nxm=8, nym=4, nzm=4, istr3=8))
+    with pytest.raises(pyafid.AfidError, match="salinity"):
+        g.sal_stats()
+    g.close()

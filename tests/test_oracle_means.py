@@ -1,4 +1,5 @@
-"""Pins of the oracle's mean profiles (CalcMeanProfiles.F90:11-229), CPU only.
+"""Pins of the oracle's mean profiles (CalcMeanProfiles.F90:11-229) and
+refined salinity profiles (CalcSalStats, salinity.F90:510-690), CPU only.
 
 Each check compares the oracle with something it shares no code with:
 closed forms of the discrete operators on fields whose profiles are known
@@ -173,3 +174,105 @@ def test_periodic_shift_invariance():
     r = q.mean_profiles()
     for k in NAMES:
         assert np.allclose(r[k], m[k], rtol=1e-12, atol=1e-15), k
+
+
+# ------------------------------------------------- CalcSalStats (refined grid)
+SAL_NAMES = po.Oracle.SAL_PROFILES
+
+
+def mr_cfg(**kw):
+    c = dict(nxm=8, nym=4, nzm=6, istr3=8, multires=1, salinity=1, nxmr=16, nymr=8, nzmr=12, istr3r=8,
+             rayt=1e6, prat=1.0, rays=-1e6, pras=4.0)
+    c.update(kw)
+    return c
+
+
+def pecs_of(cfg):
+    ren, _ = consts(cfg)
+    return ren * cfg["pras"]
+
+
+def fresh_mr(cfg):
+    o = po.Oracle(cfg)
+    for n in ("sal", "vxr", "vyr", "vzr"):
+        o.field(n)[...] = 0.0
+    o.field("salbp")[...] = 1.0
+    o.field("saltp")[...] = 0.0
+    return o
+
+
+def test_sal_conductive_profile_dissipation_is_one_over_pecs():
+    """S = 1 - x on the refined cell centres with S = 1, 0 on the walls:
+    chiS = 1/pecs exactly (interior and both wall forms), Sbar = S, Srms = |S|."""
+    cfg = mr_cfg()
+    o = fresh_mr(cfg)
+    xmr = o.grid("xmr")[:16]
+    S = 1.0 - xmr
+    o.field("sal")[..., :16] = S
+    m = o.sal_stats()
+    assert np.allclose(m["chiS"], 1.0 / pecs_of(cfg), rtol=1e-13, atol=0)
+    assert np.allclose(m["Sbar"], S, rtol=1e-14, atol=0)
+    assert np.allclose(m["Srms"], np.abs(S), rtol=1e-14, atol=0)
+    for n in ("vxS", "vyS", "vzS"):
+        assert np.all(m[n] == 0.0), n
+
+
+def test_sal_fluxes_of_separable_fields():
+    """vxr = a, vyr = b + A cos(y mode), vzr = d + B sin(z mode), S = S(x): the
+    face averages of the modes vanish, so vxS = a S, vyS = b S, vzS = d S."""
+    cfg = mr_cfg()
+    o = fresh_mr(cfg)
+    a, b, d = 0.4, -0.3, 0.25
+    o.field("vxr")[...] = a
+    o.field("vyr")[..., :16] = (b + 0.2 * np.cos(2 * np.pi * 3 * np.arange(8) / 8))[None, :, None]
+    o.field("vzr")[..., :16] = (d + 0.5 * np.sin(2 * np.pi * 5 * np.arange(12) / 12))[:, None, None]
+    S = 0.2 + 0.05 * np.arange(16)
+    o.field("sal")[..., :16] = S
+    m = o.sal_stats()
+    assert np.allclose(m["vxS"], a * S, rtol=1e-14)
+    assert np.allclose(m["vyS"], b * S, rtol=1e-13)
+    assert np.allclose(m["vzS"], d * S, rtol=1e-13)
+
+
+def test_sal_dissipation_discrete_symbol():
+    """S = A sin(2 pi m j/N) in y with no wall flux terms (sfixs = sfixn = 0):
+    the centred y difference has mean square 2 A^2 sin^2(2 pi m/N), so chiS =
+    0.5 A^2 sin^2(2 pi m/N) dyr^2 / pecs."""
+    cfg = mr_cfg(sfixs=0, sfixn=0, ylen=1.7)
+    o = fresh_mr(cfg)
+    A, mode, N = 0.6, 3, 8
+    o.field("sal")[..., :16] = (A * np.sin(2 * np.pi * mode * np.arange(N) / N))[None, :, None]
+    m = o.sal_stats()
+    dyr = N / cfg["ylen"]
+    assert np.allclose(m["chiS"], 0.5 * A * A * np.sin(2 * np.pi * mode / N) ** 2 * dyr * dyr / pecs_of(cfg), rtol=1e-12, atol=0)
+
+
+def _random_mr(cfg, seed):
+    o = po.Oracle(cfg)
+    for q, n in enumerate(("sal", "vxr", "vyr", "vzr")):
+        o.field(n)[...] = uniform(seed * 10 + q, o.field(n).shape) - 0.5
+    o.field("salbp")[...] = 1.0 + 0.1 * uniform(seed + 7, o.field("salbp").shape)
+    o.field("saltp")[...] = 0.1 * uniform(seed + 8, o.field("saltp").shape)
+    return o
+
+
+def test_sal_y_z_exchange_and_shift():
+    """Exchanging y and z (vyr with vzr) swaps vyS and vzS and leaves the rest;
+    a periodic roll of every field leaves all six profiles unchanged."""
+    cfg = mr_cfg(nym=4, nzm=4, nymr=8, nzmr=8, ylen=1.0, zlen=1.0)
+    o = _random_mr(cfg, 4)
+    m = o.sal_stats()
+    q = po.Oracle(cfg)
+    sw = lambda a: np.ascontiguousarray(np.swapaxes(a, 0, 1))
+    for src, dst in (("sal", "sal"), ("vxr", "vxr"), ("vyr", "vzr"), ("vzr", "vyr"), ("salbp", "salbp"), ("saltp", "saltp")):
+        q.field(dst)[...] = sw(o.field(src))
+    n = q.sal_stats()
+    pairs = {"vyS": "vzS", "vzS": "vyS"}
+    for k in SAL_NAMES:
+        assert np.allclose(n[pairs.get(k, k)], m[k], rtol=1e-12, atol=1e-15), k
+    r = po.Oracle(cfg)
+    for f in ("sal", "vxr", "vyr", "vzr", "salbp", "saltp"):
+        r.field(f)[...] = np.roll(np.roll(o.field(f), 2, axis=0), -3, axis=1)
+    t = r.sal_stats()
+    for k in SAL_NAMES:
+        assert np.allclose(t[k], m[k], rtol=1e-12, atol=1e-15), k
