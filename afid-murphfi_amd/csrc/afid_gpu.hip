@@ -93,7 +93,8 @@ struct afid_ctx {
     // CalcMeanProfiles: dx/g3rm per cell, the per-group partial sums and the profiles
     const double *d_sdx = nullptr, *d_sdxr = nullptr;
     double *d_mean_part = nullptr, *d_mean_out = nullptr, *d_sal_part = nullptr, *d_sal_out = nullptr;
-    int mean_groups = 0, sal_groups = 0;
+    double *d_phi_part = nullptr, *d_phi_out = nullptr;
+    int mean_groups = 0, sal_groups = 0, phi_groups = 0;
     const double *d_udx3mr = nullptr, *d_ap3sskr = nullptr, *d_ac3sskr = nullptr, *d_am3sskr = nullptr, *d_lxa = nullptr;
     const int *d_xs_src = nullptr, *d_ys_src = nullptr, *d_zs_src = nullptr, *d_xb_src = nullptr, *d_yb_src = nullptr, *d_zb_src = nullptr;
     const double *d_xs_w = nullptr, *d_ys_w = nullptr, *d_zs_w = nullptr, *d_xb_w = nullptr, *d_yb_w = nullptr, *d_zb_w = nullptr;
@@ -1550,6 +1551,30 @@ int afid_calc_sal_stats(afid_ctx *c, double *out) {
                        c->sal_groups, c->d_sal_out);
     if ((rc = check_launch(c, "k_sal_stats_finish"))) return rc;
     HIPCHK(c, hipMemcpyAsync(out, c->d_sal_out, sizeof(double) * size_t(SAL_NQ) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AFID_OK;
+}
+
+int afid_calc_phi_stats(afid_ctx *c, double *out) {
+    if (!c || !out) return AFID_E_ARG;
+    if (!c->pf) return set_err(c, AFID_E_ARG, "CalcPhiStats needs the phase field");
+    int rc;
+    const int n = c->nxmr;
+    if (!c->d_phi_part) {
+        const long long ncol = (long long)c->nyr * c->nzr;
+        c->phi_groups = int(std::min<long long>(MEAN_GROUPS, (ncol + 3) / 4));
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_phi_part), sizeof(double) * size_t(c->phi_groups) * 2 * n))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_phi_out), sizeof(double) * 2 * size_t(n)))) return rc;
+    }
+    KP p = make_kp(c);
+    const dim3 gr(unsigned((n + 63) / 64), unsigned(c->phi_groups));
+    if (c->mixed) hipLaunchKernelGGL(k_phi_stats_partials<float>, gr, dim3(256), 0, c->stream, p, c->d_phi_part);
+    else hipLaunchKernelGGL(k_phi_stats_partials<double>, gr, dim3(256), 0, c->stream, p, c->d_phi_part);
+    if ((rc = check_launch(c, "k_phi_stats_partials"))) return rc;
+    hipLaunchKernelGGL(k_phi_stats_finish, dim3(unsigned((2 * n + 255) / 256)), dim3(256), 0, c->stream, p, c->d_phi_part,
+                       c->phi_groups, c->d_phi_out);
+    if ((rc = check_launch(c, "k_phi_stats_finish"))) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_phi_out, sizeof(double) * 2 * size_t(n), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return AFID_OK;
 }

@@ -209,4 +209,41 @@ __global__ void __launch_bounds__(256) k_sal_stats_finish(KP p, const double *__
     out[t] = s;
 }
 
+// CalcPhiStats (phasefield.F90:750-798): phibar and phirms of the refined
+// phase field (each nxmr values, the write order of :787-791), RT its storage
+// type (FP32 in the mixed form), FP64 sums.
+template <typename RT>
+__global__ void __launch_bounds__(256) k_phi_stats_partials(KP p, double *__restrict__ part) {
+    __shared__ double red[4][2][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane, n = p.nxmr;
+    const bool kok = k < n;
+    double s0 = 0.0, s1 = 0.0;
+    if (kok) {
+        const RT *phi = sel<RT>(p.phi, p.phif);
+        const int ncol = p.nyr * p.nzr, step = 4 * gridDim.y;
+        for (int col = blockIdx.y * 4 + wave; col < ncol; col += step) {
+            const int i = col / p.nyr, j = col - i * p.nyr;
+            const double v = double(phi[ridx(p, k, j, i)]);
+            s0 += v;                                                        // :770-776
+            s1 += v * v;
+        }
+    }
+    red[wave][0][lane] = s0;
+    red[wave][1][lane] = s1;
+    __syncthreads();
+    if (wave == 0 && kok)
+        for (int q = 0; q < 2; ++q)
+            part[((long long)blockIdx.y * 2 + q) * n + k] = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+}
+__global__ void __launch_bounds__(256) k_phi_stats_finish(KP p, const double *__restrict__ part, int groups, double *__restrict__ out) {
+    const int t = blockIdx.x * 256 + threadIdx.x, n = p.nxmr;
+    if (t >= 2 * n) return;
+    const int q = t / n, k = t - q * n;
+    double s = 0.0;
+    for (int g = 0; g < groups; ++g) s += part[((long long)g * 2 + q) * n + k];
+    const double inyzm = 1.0 / double(p.nyr) / double(p.nzr);      // :765
+    out[t] = q == 0 ? s * inyzm : sqrt(s * inyzm);                  // :781-784
+}
+
 }  // namespace afid

@@ -141,6 +141,10 @@ module afid_gpu
         integer(c_int) function afid_calc_sal_stats(ctx, out) bind(C, name='afid_calc_sal_stats')
             import; type(c_ptr), value :: ctx; real(c_double), intent(out) :: out(*)
         end function
+        ! CalcPhiStats: out(nxmr, 2) = phibar, phirms
+        integer(c_int) function afid_calc_phi_stats(ctx, out) bind(C, name='afid_calc_phi_stats')
+            import; type(c_ptr), value :: ctx; real(c_double), intent(out) :: out(*)
+        end function
         integer(c_int) function afid_create_initial_conditions(ctx, seed, eps) bind(C, name='afid_create_initial_conditions')
             import; type(c_ptr), value :: ctx; integer(c_int64_t), value :: seed; real(c_double), value :: eps
         end function

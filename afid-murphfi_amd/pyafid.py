@@ -35,7 +35,7 @@ EXPORTS = [
     "afid_explicit_phase", "afid_add_volume_penalty", "afid_implicit_phase", "afid_add_latent_heat",
     "afid_interp_temp_multigrid", "afid_interp_phi_multigrid",
     "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher", "afid_refinement_residuals",
-    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_calc_sal_stats",
+    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_calc_sal_stats", "afid_calc_phi_stats",
     "afid_synchronize",
     "afid_create_initial_conditions",
     "afid_set_timing", "afid_get_timing", "afid_grid_array",
@@ -170,6 +170,7 @@ def lib():
         L.afid_global_quantities.argtypes = [vp, dp]
         L.afid_calc_mean_profiles.argtypes = [vp, dp]
         L.afid_calc_sal_stats.argtypes = [vp, dp]
+        L.afid_calc_phi_stats.argtypes = [vp, dp]
         L.afid_synchronize.argtypes = [vp]
         L.afid_create_initial_conditions.argtypes = [vp, C.c_uint64, C.c_double]
         L.afid_set_timing.argtypes = [vp, C.c_int]
@@ -311,6 +312,13 @@ class Solver:
         out = np.zeros(len(SAL_PROFILES) * n)
         self._check(self.L.afid_calc_sal_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "afid_calc_sal_stats")
         return {m: out[q * n:(q + 1) * n].copy() for q, m in enumerate(SAL_PROFILES)}
+
+    def phi_stats(self):
+        """CalcPhiStats (phasefield.F90:750-798): dict of phibar, phirms (nxmr values each)."""
+        n = int(self.cfg.nxmr)
+        out = np.zeros(2 * n)
+        self._check(self.L.afid_calc_phi_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "afid_calc_phi_stats")
+        return {"phibar": out[:n].copy(), "phirms": out[n:].copy()}
 
     def refinement_residuals(self):
         """Mixed form: max|div u| before each refinement sweep of the last substep."""

@@ -163,6 +163,10 @@ int afid_calc_mean_profiles(afid_ctx *ctx, double *out);
  * in both forms (the mixed form reads its FP32 refined fields).  Needs the
  * salinity field and nxmr >= 2.  Waits for the result. */
 int afid_calc_sal_stats(afid_ctx *ctx, double *out);
+/* CalcPhiStats (phasefield.F90:750-798): phibar and phirms of the refined
+ * phase field, out[q * nxmr + k], q = 0 phibar, 1 phirms (2*nxmr doubles).
+ * FP64 sums in both forms.  Needs the phase field.  Waits for the result. */
+int afid_calc_phi_stats(afid_ctx *ctx, double *out);
 
 /* Seeded initial state (CreateInitialConditions.F90:150-201, RBC branch, with
  * u = 0): T = tempbp + (temptp - tempbp) x + noise, the wall-clamped noise of

@@ -1429,6 +1429,31 @@ void orc_sal_stats(orc_state *s, double *out) {
     }
 }
 
+/* CalcPhiStats, phasefield.F90:750-798: phibar and phirms of the refined phase
+ * field (nxmr values each), sums in the reference's loop order, then :781-784. */
+void orc_phi_stats(orc_state *s, double *out) {
+    REF_DECL
+    const double *phi = s->f[ORC_PHI];
+    double *pb = out - 1, *pr = out + nxmr - 1;
+    for (int q = 0; q < 2 * nxmr; q++) out[q] = 0.0;
+    const double inyzm = 1.0 / nymr / nzmr;
+    for (int ic = 1; ic <= nzmr; ic++) {
+        int i0 = W(ic, nzmr);
+        for (int jc = 1; jc <= nymr; jc++) {
+            int j0 = W(jc, nymr);
+            for (int k = 1; k <= nxmr; k++) {
+                double v = phi[R(k, j0, i0)];
+                pb[k] = pb[k] + v;
+                pr[k] = pr[k] + v * v;
+            }
+        }
+    }
+    for (int k = 1; k <= nxmr; k++) {
+        pb[k] = pb[k] * inyzm;
+        pr[k] = sqrt(pr[k] * inyzm);
+    }
+}
+
 /* ExplicitSalinity, salinity.F90:268-366 (non-IBM branch) */
 void orc_explicit_salinity(orc_state *s) {
     REF_DECL

@@ -107,6 +107,8 @@ void orc_global_quantities(orc_state *s, double out[4]); /* Nu_vol, KE, vrms, Tm
 void orc_mean_profiles(orc_state *s, double *out);
 /* CalcSalStats: 6 refined profiles of nxmr values (Sbar, Srms, vxS, vyS, vzS, chiS) */
 void orc_sal_stats(orc_state *s, double *out);
+/* CalcPhiStats: phibar, phirms of nxmr values each */
+void orc_phi_stats(orc_state *s, double *out);
 
 /* free-standing numerics used by the pins */
 void orc_fft(double *re, double *im, int n, int sign);      /* in place, unnormalised */

@@ -80,6 +80,7 @@ def lib():
         L.orc_global_quantities.argtypes = [vp, C.POINTER(C.c_double)]
         L.orc_mean_profiles.argtypes = [vp, C.POINTER(C.c_double)]
         L.orc_sal_stats.argtypes = [vp, C.POINTER(C.c_double)]
+        L.orc_phi_stats.argtypes = [vp, C.POINTER(C.c_double)]
         dp = C.POINTER(C.c_double)
         L.orc_fft.argtypes = [dp, dp, C.c_int, C.c_int]
         L.orc_dft_naive.argtypes = [dp, dp, dp, dp, C.c_int, C.c_int]
@@ -183,6 +184,13 @@ class Oracle:
         return {n: out[q * nxm:(q + 1) * nxm].copy() for q, n in enumerate(self.MEAN_PROFILES)}
 
     SAL_PROFILES = ["Sbar", "Srms", "vxS", "vyS", "vzS", "chiS"]
+
+    def phi_stats(self):
+        """CalcPhiStats (phasefield.F90:750-798): dict of phibar, phirms."""
+        n = int(self.cfg.nxmr)
+        out = np.zeros(2 * n)
+        self.L.orc_phi_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return {"phibar": out[:n].copy(), "phirms": out[n:].copy()}
 
     def sal_stats(self):
         """CalcSalStats (salinity.F90:510-690): dict of the 6 refined profiles (nxmr values each)."""

@@ -116,3 +116,31 @@ def test_sal_stats_refused_without_salinity():
     with pytest.raises(pyafid.AfidError, match="salinity"):
         g.sal_stats()
     g.close()
+
+
+PF = dict(nxm=16, nym=12, nzm=12, istr3=8, multires=1, phasefield=1, nxmr=32, nymr=24, nzmr=24, istr3r=8,
+          rayt=1e6, prat=1.0, pf_a=1.0, pf_s=1.0, pf_tm=0.0)
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+@pytest.mark.parametrize("cfg", [PF, dict(PF, nxm=12, nym=8, nzm=6, nxmr=30, nymr=20, nzmr=18, istr3=4, str3=1.4, istr3r=6)])
+def test_phi_stats_match_oracle(cfg, form):
+    """CalcPhiStats on a random refined phase field (the oracle sums the values
+    the GPU holds, FP32 in the mixed form)."""
+    g = pyafid.Solver(dict(cfg, precision=pyafid.MIXED if form == "mixed" else pyafid.FP64))
+    o = pyoracle.Oracle(cfg)
+    g.upload("phi", uniform(60, o.field("phi").shape))
+    o.field("phi")[...] = g.download("phi")
+    a, b = g.phi_stats(), o.phi_stats()
+    for n in ("phibar", "phirms"):
+        e = relerr(a[n], b[n])
+        assert e <= 1e-12, f"{n}: relative error {e:.2e}"
+    g.close()
+    o.close()
+
+
+def test_phi_stats_refused_without_phase_field():
+    g = pyafid.Solver(dict(nxm=8, nym=4, nzm=4, istr3=8))
+    with pytest.raises(pyafid.AfidError, match="phase field"):
+        g.phi_stats()
+    g.close()

@@ -276,3 +276,23 @@ def test_sal_y_z_exchange_and_shift():
     t = r.sal_stats()
     for k in SAL_NAMES:
         assert np.allclose(t[k], m[k], rtol=1e-12, atol=1e-15), k
+
+
+# ------------------------------------------------- CalcPhiStats (refined grid)
+def test_phi_stats_closed_forms():
+    """phi = f(x) + c + A sin(2 pi m j/N): the y mode averages out, so phibar =
+    f + c and phirms^2 = (f + c)^2 + A^2/2; a periodic roll changes nothing."""
+    cfg = dict(nxm=8, nym=4, nzm=6, istr3=8, multires=1, phasefield=1, nxmr=16, nymr=12, nzmr=10, istr3r=8)
+    o = po.Oracle(cfg)
+    f = 0.1 * np.arange(16) - 0.4
+    A, c = 0.3, 0.25
+    phi = np.zeros(o.field("phi").shape)
+    phi[..., :16] = f + c + (A * np.sin(2 * np.pi * 5 * np.arange(12) / 12))[None, :, None]
+    o.field("phi")[...] = phi
+    m = o.phi_stats()
+    assert np.allclose(m["phibar"], f + c, rtol=1e-13, atol=1e-15)
+    assert np.allclose(m["phirms"], np.sqrt((f + c) ** 2 + A * A / 2), rtol=1e-13, atol=0)
+    o.field("phi")[...] = np.roll(np.roll(phi, 4, axis=0), -5, axis=1)
+    r = o.phi_stats()
+    assert np.allclose(r["phibar"], m["phibar"], rtol=1e-13, atol=1e-15)
+    assert np.allclose(r["phirms"], m["phirms"], rtol=1e-13, atol=0)
