@@ -1513,8 +1513,7 @@ int afid_calc_mean_profiles(afid_ctx *c, double *out) {
     int rc;
     const int nxm = c->nxm;
     if (!c->d_mean_part) {
-        const long long ncol = (long long)c->ny * c->nz;
-        c->mean_groups = int(std::min<long long>(MEAN_GROUPS, (ncol + 3) / 4));
+        c->mean_groups = mean_ygroups(c->ny);
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_mean_part), sizeof(double) * size_t(c->mean_groups) * MEAN_NQ * nxm))) return rc;
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_mean_out), sizeof(double) * size_t(MEAN_NQ) * nxm))) return rc;
     }
@@ -1537,13 +1536,12 @@ int afid_calc_sal_stats(afid_ctx *c, double *out) {
     int rc;
     const int n = c->nxmr;
     if (!c->d_sal_part) {
-        const long long ncol = (long long)c->nyr * c->nzr;
-        c->sal_groups = int(std::min<long long>(MEAN_GROUPS, (ncol + 3) / 4));
+        c->sal_groups = mean_ygroups(c->nyr);
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_sal_part), sizeof(double) * size_t(c->sal_groups) * SAL_NQ * n))) return rc;
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_sal_out), sizeof(double) * size_t(SAL_NQ) * n))) return rc;
     }
     KP p = make_kp(c);
-    const dim3 gr(unsigned((n + 63) / 64), unsigned(c->sal_groups));
+    const dim3 gr(unsigned((n + 63) / 64), unsigned(mean_ygroups(c->nyr)));
     if (c->mixed) hipLaunchKernelGGL(k_sal_stats_partials<float>, gr, dim3(256), 0, c->stream, p, c->d_sdxr, c->d_sal_part);
     else hipLaunchKernelGGL(k_sal_stats_partials<double>, gr, dim3(256), 0, c->stream, p, c->d_sdxr, c->d_sal_part);
     if ((rc = check_launch(c, "k_sal_stats_partials"))) return rc;
@@ -1561,13 +1559,12 @@ int afid_calc_phi_stats(afid_ctx *c, double *out) {
     int rc;
     const int n = c->nxmr;
     if (!c->d_phi_part) {
-        const long long ncol = (long long)c->nyr * c->nzr;
-        c->phi_groups = int(std::min<long long>(MEAN_GROUPS, (ncol + 3) / 4));
+        c->phi_groups = mean_ygroups(c->nyr);
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_phi_part), sizeof(double) * size_t(c->phi_groups) * 2 * n))) return rc;
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_phi_out), sizeof(double) * 2 * size_t(n)))) return rc;
     }
     KP p = make_kp(c);
-    const dim3 gr(unsigned((n + 63) / 64), unsigned(c->phi_groups));
+    const dim3 gr(unsigned((n + 63) / 64), unsigned(mean_ygroups(c->nyr)));
     if (c->mixed) hipLaunchKernelGGL(k_phi_stats_partials<float>, gr, dim3(256), 0, c->stream, p, c->d_phi_part);
     else hipLaunchKernelGGL(k_phi_stats_partials<double>, gr, dim3(256), 0, c->stream, p, c->d_phi_part);
     if ((rc = check_launch(c, "k_phi_stats_partials"))) return rc;

@@ -3,10 +3,12 @@
 // of nxm values, in its write order (CalcMeanProfiles.F90:230-283).
 //
 // Two kernels.  k_mean_partials: a block of 64 x points (one per lane) and 4
-// wavefronts; each wavefront walks every (4 * groups)-th (j, i) column of
-// its group and accumulates the 15 per-x sums of CalcMeanProfiles.F90:44-73,
-// :104-127 and :165-215 in FP64 registers; the four wavefronts' sums are
-// combined in LDS and written to part[group][q][k].  k_mean_finish: one
+// wavefronts, one y index j each (group = j / 4); each wavefront marches
+// along z, so the z neighbours it reads are its own previous and next
+// columns and the y neighbours are its sibling wavefronts' (both in cache),
+// and accumulates the 15 per-x sums of CalcMeanProfiles.F90:44-73, :104-127
+// and :165-215 in FP64 registers; the four wavefronts' sums are combined in
+// LDS and written to part[group][q][k].  k_mean_finish: one
 // thread per (q, k) sums the groups in a fixed order and applies the
 // normalisations of :87-101 and :219-223, so a call's result does not
 // depend on scheduling.
@@ -18,7 +20,9 @@
 namespace afid {
 
 constexpr int MEAN_NQ = 15;         // Tbar vybar vzbar vxT vyT vzT Trms vxrms vyrms vzrms vxvy vxvz vyvz chiT epsilon
-constexpr int MEAN_GROUPS = 512;    // column groups (second grid dimension) of k_mean_partials
+// Grid of the partial-sum kernels: (x chunks of 64, ceil(ny / 4) y groups);
+// every y group writes one partial per profile.
+__host__ __device__ constexpr int mean_ygroups(int ny) { return (ny + 3) / 4; }
 
 // sdx[k] = dx / g3rm(k+1) (CreateGrid.F90:120-123), k = 0..nxm-1
 __global__ void __launch_bounds__(256) k_mean_partials(KP p, const double *__restrict__ sdx, double *__restrict__ part) {
@@ -29,14 +33,14 @@ __global__ void __launch_bounds__(256) k_mean_partials(KP p, const double *__res
     double a[MEAN_NQ];
 #pragma unroll
     for (int q = 0; q < MEAN_NQ; ++q) a[q] = 0.0;
-    const int ncol = p.ny * p.nz, step = 4 * gridDim.y;
     const double dy = p.dy, dz = p.dz;
-    if (kok) {
+    const int j = blockIdx.y * 4 + wave;
+    if (kok && j < p.ny) {
         const int kp = k + 1, km = k - 1;
         const double s0 = sdx[k];
-        for (int col = blockIdx.y * 4 + wave; col < ncol; col += step) {
-            const int i = col / p.ny, j = col - i * p.ny;
-            const int ip = wrapp(i, p.nz), im = wrapm(i, p.nz), jp = wrapp(j, p.ny), jm = wrapm(j, p.ny);
+        const int jp = wrapp(j, p.ny), jm = wrapm(j, p.ny);
+        for (int i = 0; i < p.nz; ++i) {
+            const int ip = wrapp(i, p.nz), im = wrapm(i, p.nz);
             auto at = [&](const double *f, int kk, int jj, int ii) { return f[bidx(p, kk, jj, ii)]; };
             const double *vx = p.vx, *vy = p.vy, *vz = p.vz, *tp = p.temp;
             // :44-73
@@ -119,7 +123,8 @@ __global__ void __launch_bounds__(256) k_mean_partials(KP p, const double *__res
     if (wave == 0 && kok) {
 #pragma unroll
         for (int q = 0; q < MEAN_NQ; ++q)
-            part[((long long)blockIdx.y * MEAN_NQ + q) * nxm + k] = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+            part[((long long)blockIdx.y * MEAN_NQ + q) * nxm + k] =
+                ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
     }
 }
 
@@ -156,11 +161,11 @@ __global__ void __launch_bounds__(256) k_sal_stats_partials(KP p, const double *
     if (kok) {
         const RT *sal = sel<RT>(p.sal, p.salf), *vxr = sel<RT>(p.vxr, p.vxrf), *vyr = sel<RT>(p.vyr, p.vyrf),
                  *vzr = sel<RT>(p.vzr, p.vzrf);
-        const int ncol = p.nyr * p.nzr, step = 4 * gridDim.y;
         const double tdx = 0.5 * sdxr[k], hdz = 0.5 * p.dzr, hdy = 0.5 * p.dyr;
-        for (int col = blockIdx.y * 4 + wave; col < ncol; col += step) {
-            const int i = col / p.nyr, j = col - i * p.nyr;
-            const int ip = wrapp(i, p.nzr), im = wrapm(i, p.nzr), jp = wrapp(j, p.nyr), jm = wrapm(j, p.nyr);
+        const int j = blockIdx.y * 4 + wave;
+        const int jp = wrapp(j, p.nyr), jm = wrapm(j, p.nyr);
+        for (int i = 0; i < p.nzr && j < p.nyr; ++i) {
+            const int ip = wrapp(i, p.nzr), im = wrapm(i, p.nzr);
             auto at = [&](const RT *f, int kk, int jj, int ii) { return double(f[ridx(p, kk, jj, ii)]); };
             const double v = at(sal, k, j, i);
             a[0] += v;                                                      // :544-552
@@ -192,7 +197,8 @@ __global__ void __launch_bounds__(256) k_sal_stats_partials(KP p, const double *
     if (wave == 0 && kok) {
 #pragma unroll
         for (int q = 0; q < SAL_NQ; ++q)
-            part[((long long)blockIdx.y * SAL_NQ + q) * n + k] = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+            part[((long long)blockIdx.y * SAL_NQ + q) * n + k] =
+                ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
     }
 }
 
@@ -221,9 +227,8 @@ __global__ void __launch_bounds__(256) k_phi_stats_partials(KP p, double *__rest
     double s0 = 0.0, s1 = 0.0;
     if (kok) {
         const RT *phi = sel<RT>(p.phi, p.phif);
-        const int ncol = p.nyr * p.nzr, step = 4 * gridDim.y;
-        for (int col = blockIdx.y * 4 + wave; col < ncol; col += step) {
-            const int i = col / p.nyr, j = col - i * p.nyr;
+        const int j = blockIdx.y * 4 + wave;
+        for (int i = 0; i < p.nzr && j < p.nyr; ++i) {
             const double v = double(phi[ridx(p, k, j, i)]);
             s0 += v;                                                        // :770-776
             s1 += v * v;
@@ -234,7 +239,8 @@ __global__ void __launch_bounds__(256) k_phi_stats_partials(KP p, double *__rest
     __syncthreads();
     if (wave == 0 && kok)
         for (int q = 0; q < 2; ++q)
-            part[((long long)blockIdx.y * 2 + q) * n + k] = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+            part[((long long)blockIdx.y * 2 + q) * n + k] =
+                ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
 }
 __global__ void __launch_bounds__(256) k_phi_stats_finish(KP p, const double *__restrict__ part, int groups, double *__restrict__ out) {
     const int t = blockIdx.x * 256 + threadIdx.x, n = p.nxmr;
