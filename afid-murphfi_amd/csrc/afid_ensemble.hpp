@@ -147,14 +147,17 @@ template <int NT>
 __device__ __forceinline__ void ens_cfl_div(const KP &p, double &cfl, double &dmax, double *red) {
     cfl = 1.0e-8;
     dmax = 0.0;
-    const int nxm = p.nxm, nz = p.nz;
-    for (int t = threadIdx.x; t < nxm * nz; t += NT) {
-        const int k = t % nxm, i = t / nxm, ip = wrapp(i, nz);
-        const long long c = (long long)i * p.plane + k, cp = (long long)ip * p.plane + k;
-        const double q = fabs((p.vz[c] + p.vz[cp]) * 0.5 * p.dz) + fabs((p.vx[c] + p.vx[c + 1]) * 0.5 * p.udx3m[k]);
-        const double d = fabs((p.vz[cp] - p.vz[c]) * p.dz + (p.vx[c + 1] - p.vx[c]) * p.udx3m[k]);
-        cfl = fmax(cfl, q != q ? HUGE_VAL : q);
-        dmax = fmax(dmax, d != d ? HUGE_VAL : d);
+    const int nxm = p.nxm, nz = p.nz, lane = threadIdx.x & 63;
+    // one z column per wavefront, x across the lanes (a max is exact in any order)
+    for (int i = threadIdx.x >> 6; i < nz; i += NT / 64) {
+        const int ip = wrapp(i, nz);
+        for (int k = lane; k < nxm; k += 64) {
+            const long long c = (long long)i * p.plane + k, cp = (long long)ip * p.plane + k;
+            const double q = fabs((p.vz[c] + p.vz[cp]) * 0.5 * p.dz) + fabs((p.vx[c] + p.vx[c + 1]) * 0.5 * p.udx3m[k]);
+            const double d = fabs((p.vz[cp] - p.vz[c]) * p.dz + (p.vx[c + 1] - p.vx[c]) * p.udx3m[k]);
+            cfl = fmax(cfl, q != q ? HUGE_VAL : q);
+            dmax = fmax(dmax, d != d ? HUGE_VAL : d);
+        }
     }
     block_max2<NT>(cfl, dmax, red);
 }
@@ -547,6 +550,12 @@ constexpr int ENSP_CELLS = 4 * ENSP_NT;           // cells per projection item
 // Mixed form: Poisson sweep s (1-based) of a run runs when s = 1, or while
 // the FP64 residual max|div u| left by sweep s-1 (rslot[s-1]) is above
 // div_tol, up to max_sweeps; the FP64 form has one sweep.
+// any[s] != 0: some run's residual after sweep s exceeded its div_tol (set by
+// k_ensp_resid, cleared by k_ensp_factors); a launch of sweep s >= 2 whose
+// flag is clear exits before reading its work item
+__device__ __forceinline__ bool ens_any_open(const unsigned long long *any, int s) {
+    return s <= 1 || __builtin_amdgcn_readfirstlane(int(any[s - 1] != 0ull));
+}
 __device__ __forceinline__ bool ens_sweep_open(const EnsRun &E, int s) {
     if (s <= 1) return true;
     if (s > E.max_sweeps) return false;
@@ -555,8 +564,9 @@ __device__ __forceinline__ bool ens_sweep_open(const EnsRun &E, int s) {
 
 template <int CH>
 __global__ void __launch_bounds__(192) k_ensp_factors(const EnsRun *__restrict__ runs, const EnsState *st, int ns,
-                                                      double *fac) {
+                                                      double *fac, unsigned long long *any) {
     const int r = blockIdx.x;
+    if (r == 0 && threadIdx.x < 4) any[threadIdx.x] = 0ull;         // sweep flags of this substep
     if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
     const EnsRun &R = runs[r];
     const KP p = ens_kp(R, ns, st[r].dt, 0);
@@ -586,7 +596,8 @@ __global__ void __launch_bounds__(ENSP_NT) k_ensp_columns(const EnsRun *__restri
 template <typename R>
 __global__ void __launch_bounds__(ENSP_NT) k_ensp_transform(const EnsRun *__restrict__ runs, const EnsState *st,
                                                             const EnsItem *__restrict__ items, int ns, int lines,
-                                                            int inverse, int sweep) {
+                                                            int inverse, int sweep, const unsigned long long *any) {
+    if (!ens_any_open(any, sweep)) return;
     extern __shared__ __align__(16) unsigned char smraw[];
     cpx<R> *lb = reinterpret_cast<cpx<R> *>(smraw);
     const EnsItem it = items[blockIdx.x];
@@ -604,7 +615,9 @@ __global__ void __launch_bounds__(ENSP_NT) k_ensp_transform(const EnsRun *__rest
 
 template <int CH, typename R>
 __global__ void __launch_bounds__(ENSP_NT) k_ensp_modes(const EnsRun *__restrict__ runs, const EnsState *st,
-                                                        const EnsItem *__restrict__ items, int sweep) {
+                                                        const EnsItem *__restrict__ items, int sweep,
+                                                        const unsigned long long *any) {
+    if (!ens_any_open(any, sweep)) return;
     const EnsItem it = items[blockIdx.x];
     const int r = it.run;
     if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
@@ -615,7 +628,9 @@ __global__ void __launch_bounds__(ENSP_NT) k_ensp_modes(const EnsRun *__restrict
 }
 
 __global__ void __launch_bounds__(ENSP_NT) k_ensp_project(const EnsRun *__restrict__ runs, const EnsState *st,
-                                                          const EnsItem *__restrict__ items, int ns, int sweep) {
+                                                          const EnsItem *__restrict__ items, int ns, int sweep,
+                                                          const unsigned long long *any) {
+    if (!ens_any_open(any, sweep)) return;
     const EnsItem it = items[blockIdx.x];
     const int r = it.run;
     if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
@@ -629,7 +644,9 @@ __global__ void __launch_bounds__(ENSP_NT) k_ensp_project(const EnsRun *__restri
 // Mixed form: FP64 residual max|div u| (CheckDivergence units) after the
 // projection of sweep `sweep`, max-reduced per run into rslot[sweep]
 __global__ void __launch_bounds__(ENSP_NT) k_ensp_resid(const EnsRun *__restrict__ runs, const EnsState *st,
-                                                        const EnsItem *__restrict__ items, int ns, int sweep) {
+                                                        const EnsItem *__restrict__ items, int ns, int sweep,
+                                                        unsigned long long *any) {
+    if (!ens_any_open(any, sweep)) return;
     const EnsItem it = items[blockIdx.x];
     const int r = it.run;
     if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
@@ -646,11 +663,20 @@ __global__ void __launch_bounds__(ENSP_NT) k_ensp_resid(const EnsRun *__restrict
         dmax = fmax(dmax, d != d ? HUGE_VAL : d);
     }
     dmax = block_max(dmax);
-    if (threadIdx.x == 0) atomicMax(R.rslot + sweep, (unsigned long long)__double_as_longlong(dmax));
+    if (threadIdx.x == 0) {
+        atomicMax(R.rslot + sweep, (unsigned long long)__double_as_longlong(dmax));
+        // the next sweep opens for this run; one atomic on the shared flag
+        // suffices, so it is skipped once the flag reads set
+        if (!(dmax <= R.div_tol) && __hip_atomic_load(any + sweep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull)
+            atomicOr(any + sweep, 1ull);
+    }
 }
 
-__global__ void __launch_bounds__(ENSP_NT) k_ensp_finish(const EnsRun *__restrict__ runs, EnsState *st, int ns) {
-    __shared__ double red[2 * ENSP_NT / 64];
+// one workgroup of ENSP_FIN_NT threads per run (the CFL / divergence sweep
+// of the last substep reads the whole run)
+constexpr int ENSP_FIN_NT = 1024;
+__global__ void __launch_bounds__(ENSP_FIN_NT) k_ensp_finish(const EnsRun *__restrict__ runs, EnsState *st, int ns) {
+    __shared__ double red[2 * ENSP_FIN_NT / 64];
     const int r = blockIdx.x;
     if (__builtin_amdgcn_readfirstlane(st[r].status)) return;
     const EnsRun &R = runs[r];
@@ -658,7 +684,7 @@ __global__ void __launch_bounds__(ENSP_NT) k_ensp_finish(const EnsRun *__restric
     if (ns == R.nsst) {
         const KP p = ens_kp(R, ns, st[r].dt, cur ^ 1);
         double cfl, dmax;
-        ens_cfl_div<ENSP_NT>(p, cfl, dmax, red);
+        ens_cfl_div<ENSP_FIN_NT>(p, cfl, dmax, red);
         if (threadIdx.x == 0) {
             EnsState s = st[r];
             ens_end_of_step(R, s, cfl, dmax);

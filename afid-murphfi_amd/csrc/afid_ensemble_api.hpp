@@ -32,6 +32,8 @@ struct afid_ens {
     int max_sweeps = 1;
     EnsItem *d_items_col = nullptr, *d_items_mode = nullptr, *d_items_line = nullptr, *d_items_cell = nullptr;
     int n_col = 0, n_mode = 0, n_line = 0, n_cell = 0, max_line = 0;
+    // per sweep: some run's residual still exceeds its tolerance (device flags)
+    unsigned long long *d_any = nullptr;
     double *d_fac = nullptr;
     size_t lds_line = 0;
     hipGraphExec_t graph = nullptr;
@@ -105,7 +107,8 @@ template <int CH, typename HT, typename R>
 int ens_substep_phased_t(afid_ens *e, int ns) {
     int rc;
     const dim3 blk(ENSP_NT);
-    hipLaunchKernelGGL((k_ensp_factors<CH>), dim3(unsigned(e->nruns)), dim3(192), 0, e->stream, e->d_runs, e->d_state, ns, e->d_fac);
+    hipLaunchKernelGGL((k_ensp_factors<CH>), dim3(unsigned(e->nruns)), dim3(192), 0, e->stream, e->d_runs, e->d_state, ns, e->d_fac,
+                       e->d_any);
     if ((rc = ens_launched(e, "k_ensp_factors"))) return rc;
     hipLaunchKernelGGL((k_ensp_columns<CH, HT>), dim3(unsigned(e->n_col)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_col,
                        ns, e->d_fac);
@@ -113,21 +116,24 @@ int ens_substep_phased_t(afid_ens *e, int ns) {
     const int sweeps = e->mixed ? e->max_sweeps : 1;
     for (int sw = 1; sw <= sweeps; ++sw) {
         hipLaunchKernelGGL(k_ensp_transform<R>, dim3(unsigned(e->n_line)), blk, e->lds_line, e->stream, e->d_runs, e->d_state,
-                           e->d_items_line, ns, e->tune.lines, 0, sw);
+                           e->d_items_line, ns, e->tune.lines, 0, sw, e->d_any);
         if ((rc = ens_launched(e, "k_ensp_transform"))) return rc;
-        hipLaunchKernelGGL((k_ensp_modes<CH, R>), dim3(unsigned(e->n_mode)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_mode, sw);
+        hipLaunchKernelGGL((k_ensp_modes<CH, R>), dim3(unsigned(e->n_mode)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_mode, sw,
+                           e->d_any);
         if ((rc = ens_launched(e, "k_ensp_modes"))) return rc;
         hipLaunchKernelGGL(k_ensp_transform<R>, dim3(unsigned(e->n_line)), blk, e->lds_line, e->stream, e->d_runs, e->d_state,
-                           e->d_items_line, ns, e->tune.lines, 1, sw);
+                           e->d_items_line, ns, e->tune.lines, 1, sw, e->d_any);
         if ((rc = ens_launched(e, "k_ensp_transform"))) return rc;
-        hipLaunchKernelGGL(k_ensp_project, dim3(unsigned(e->n_cell)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_cell, ns, sw);
+        hipLaunchKernelGGL(k_ensp_project, dim3(unsigned(e->n_cell)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_cell, ns, sw,
+                           e->d_any);
         if ((rc = ens_launched(e, "k_ensp_project"))) return rc;
         if (e->mixed) {
-            hipLaunchKernelGGL(k_ensp_resid, dim3(unsigned(e->n_cell)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_cell, ns, sw);
+            hipLaunchKernelGGL(k_ensp_resid, dim3(unsigned(e->n_cell)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_cell, ns, sw,
+                               e->d_any);
             if ((rc = ens_launched(e, "k_ensp_resid"))) return rc;
         }
     }
-    hipLaunchKernelGGL(k_ensp_finish, dim3(unsigned(e->nruns)), blk, 0, e->stream, e->d_runs, e->d_state, ns);
+    hipLaunchKernelGGL(k_ensp_finish, dim3(unsigned(e->nruns)), dim3(ENSP_FIN_NT), 0, e->stream, e->d_runs, e->d_state, ns);
     return ens_launched(e, "k_ensp_finish");
 }
 template <int CH>
@@ -406,6 +412,8 @@ int afid_ens_init(const afid_config *cfgs, const afid_run_control *ctl, int32_t 
     int rc;
     if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_rslot), sizeof(unsigned long long) * 4 * size_t(nruns)))) return rc;
     ENSCHK(e, hipMemset(e->d_rslot, 0, sizeof(unsigned long long) * 4 * size_t(nruns)));
+    if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_any), sizeof(unsigned long long) * 4))) return rc;
+    ENSCHK(e, hipMemset(e->d_any, 0, sizeof(unsigned long long) * 4));
     if (e->mixed)
         for (int r = 0; r < nruns; ++r) tab[size_t(r)].rslot = e->d_rslot + 4 * r;
     if ((rc = ens_alloc(e, reinterpret_cast<void **>(&e->d_runs), sizeof(EnsRun) * tab.size()))) return rc;
