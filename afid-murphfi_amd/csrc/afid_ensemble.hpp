@@ -383,8 +383,9 @@ __device__ __forceinline__ void ens_forward(const EnsRun &E, const KP &p, int l0
     const double usdtal = 1.0 / (p.dt * p.al);
     const R scale = R(1.0 / double(N));
     cpx<R> *spec = EnsPoisson<R>::spec(E);
+    const FastDiv dtx(TX);                  // t < 2 nl N: t * TX < 2^32
     for (int t = threadIdx.x; t < TX * N; t += NT) {
-        const int xo = t % TX, i = t / TX, k = k0 + xo;
+        const int i = dtx.div(t), xo = t - i * TX, k = k0 + xo;
         double d = 0.0;
         if (k < nxm) {
             const long long c = (long long)i * p.plane + k, cp = (long long)wrapp(i, N) * p.plane + k;
@@ -396,7 +397,7 @@ __device__ __forceinline__ void ens_forward(const EnsRun &E, const KP &p, int l0
     __syncthreads();
     lds_fft<R, NT, ENS_FFT_MAXV>(lb, nl, S, EnsPoisson<R>::plan(E), -1);
     for (int t = threadIdx.x; t < TX * nh; t += NT) {
-        const int xo = t % TX, m = t / TX, k = k0 + xo;
+        const int m = dtx.div(t), xo = t - m * TX, k = k0 + xo;
         if (k >= nxm) continue;
         const int line = xo >> 1;
         const cpx<R> zm = lb[line * S + lpad<R>(m)], zn = lb[line * S + lpad<R>(m == 0 ? 0 : N - m)];
@@ -422,8 +423,9 @@ template <int NT, typename R = double>
 __device__ __forceinline__ void ens_inverse(const EnsRun &E, const KP &p, int l0, int nl, cpx<R> *lb) {
     const int nxm = p.nxm, N = p.nz, S = line_stride<R>(N), nh = N / 2 + 1, TX = 2 * nl, k0 = 2 * l0;
     const cpx<R> *spec = EnsPoisson<R>::spec(E);
+    const FastDiv dnl(nl), dtx(TX);
     for (int t = threadIdx.x; t < nl * nh; t += NT) {
-        const int line = t % nl, m = t / nl;
+        const int m = dnl.div(t), line = t - m * nl;
         const int ka = k0 + 2 * line, kb = ka + 1;
         const cpx<R> *row = spec + (long long)m * nxm;
         cpx<R> A = ka < nxm ? row[ka] : mk<R>(R(0), R(0));
@@ -435,7 +437,7 @@ __device__ __forceinline__ void ens_inverse(const EnsRun &E, const KP &p, int l0
     __syncthreads();
     lds_fft<R, NT, ENS_FFT_MAXV>(lb, nl, S, EnsPoisson<R>::plan(E), +1);
     for (int t = threadIdx.x; t < TX * N; t += NT) {
-        const int xo = t % TX, i = t / TX, k = k0 + xo;
+        const int i = dtx.div(t), xo = t - i * TX, k = k0 + xo;
         if (k >= nxm) continue;
         const cpx<R> z = lb[(xo >> 1) * S + lpad<R>(i)];
         p.dph[(long long)i * p.plane + k] = double((xo & 1) ? z.y : z.x);
@@ -450,8 +452,9 @@ __device__ __forceinline__ void ens_project(const KP &p, int t0, int t1) {
     const int nxm = p.nxm, nz = p.nz;
     const double adt = p.al * p.dt, be = p.al * p.beta;
     const double *d = p.dph;
+    const FastDiv dn(nxm);                  // t < nxm nz: t * nxm < 2^32 for every supported run
     for (int t = t0 + threadIdx.x; t < t1; t += NT) {
-        const int k = t % nxm, i = t / nxm, im = wrapm(i, nz), ip = wrapp(i, nz);
+        const int i = dn.div(t), k = t - i * nxm, im = wrapm(i, nz), ip = wrapp(i, nz);
         const int km = k == 0 ? 0 : k - 1, kp = k == nxm - 1 ? k : k + 1;
         const long long c = (long long)i * p.plane + k, row = c - k;
         const double dc = d[c], dkm = d[row + km], dkp = d[row + kp];
@@ -656,8 +659,9 @@ __global__ void __launch_bounds__(ENSP_NT) k_ensp_resid(const EnsRun *__restrict
     const KP p = ens_kp(R, ns, st[r].dt, cur ^ 1);
     const int nxm = p.nxm, nz = p.nz, t1 = min(it.start + ENSP_CELLS, nxm * nz);
     double dmax = 0.0;
+    const FastDiv dn(nxm);
     for (int t = it.start + threadIdx.x; t < t1; t += ENSP_NT) {
-        const int k = t % nxm, i = t / nxm;
+        const int i = dn.div(t), k = t - i * nxm;
         const long long c = (long long)i * p.plane + k, cp = (long long)wrapp(i, nz) * p.plane + k;
         const double d = fabs((p.vz[cp] - p.vz[c]) * p.dz + (p.vx[c + 1] - p.vx[c]) * p.udx3m[k]);
         dmax = fmax(dmax, d != d ? HUGE_VAL : d);

@@ -221,6 +221,8 @@ int ens_set_line_items(afid_ens *e, int lines) {
     for (int r = 0; r < e->nruns; ++r) {
         const afid_ctx *c = e->runs[size_t(r)];
         if (c->nz > ENS_FFT_MAXV * ENSP_NT) return ens_err(e, AFID_E_ARG, "ensemble: nz too large for the phased transforms");
+        // the cell loops divide by nxm with one multiply, exact while cells * nxm <= 2^32
+        if ((long long)c->nxm * c->nz * c->nxm > (1LL << 32)) return ens_err(e, AFID_E_ARG, "ensemble: run too large");
         const int L = ens_fft_lines(c->nz, lines, ENSP_NT), nlines = (c->nxm + 1) / 2;
         for (int l0 = 0; l0 < nlines; l0 += L) it.push_back(EnsItem{r, l0});
         const size_t line = e->mixed ? size_t(line_stride<float>(c->nz)) * sizeof(cpx<float>)

@@ -94,10 +94,10 @@ struct DftSmall<8, R> {
 // butterflies of a stage stay in registers between its read and write phase.
 constexpr int FFT_THREADS = 1024, FFT_MAXV = 8;
 
-// Division by a workgroup-uniform d through one multiply: for x, d < 2^16,
-// floor(x / d) = umulhi(x, ceil(2^32 / d)) (the error term x*e/2^32 < 2^-16
-// never reaches the next integer, since frac(x/d) <= 1 - 1/d).  d = 1 is
-// passed through.
+// Division by a workgroup-uniform d through one multiply: whenever x * d <=
+// 2^32, floor(x / d) = umulhi(x, ceil(2^32 / d)) (the error term, below
+// x / 2^32 <= 1/d, never reaches the next integer, since frac(x/d) <= 1 - 1/d).
+// d = 1 is passed through.
 struct FastDiv {
     unsigned d, m;
     __device__ __forceinline__ explicit FastDiv(unsigned d_) : d(d_), m(d_ > 1 ? 0xFFFFFFFFu / d_ + 1u : 0u) {}
