@@ -440,18 +440,18 @@ __device__ __forceinline__ void ens_inverse(const EnsRun &E, const KP &p, int l0
         const int i = dtx.div(t), xo = t - i * TX, k = k0 + xo;
         if (k >= nxm) continue;
         const cpx<R> z = lb[(xo >> 1) * S + lpad<R>(i)];
-        p.dph[(long long)i * p.plane + k] = double((xo & 1) ? z.y : z.x);
+        sel<R>(p.dph, p.dphf)[(long long)i * p.plane + k] = (xo & 1) ? z.y : z.x;     // dph in arithmetic R
     }
     __syncthreads();
 }
 
 // RemoveDivergence + CorrectPressure (pressure.F90:245-300) of the cells
 // t = k + nxm i in [t0, t1), t0 + threadIdx.x stepping by NT
-template <int NT>
+template <int NT, typename DT = double>
 __device__ __forceinline__ void ens_project(const KP &p, int t0, int t1) {
     const int nxm = p.nxm, nz = p.nz;
     const double adt = p.al * p.dt, be = p.al * p.beta;
-    const double *d = p.dph;
+    const DT *d = sel<DT>(p.dph, p.dphf);         // FP32 in the mixed form
     const FastDiv dn(nxm);                  // t < nxm nz: t * nxm < 2^32 for every supported run
     for (int t = t0 + threadIdx.x; t < t1; t += NT) {
         const int i = dn.div(t), k = t - i * nxm, im = wrapm(i, nz), ip = wrapp(i, nz);
@@ -630,6 +630,7 @@ __global__ void __launch_bounds__(ENSP_NT) k_ensp_modes(const EnsRun *__restrict
     if (m <= E.p.nz / 2) ens_mode<CH, R>(E, E.p.nxm, m);
 }
 
+template <typename RT>
 __global__ void __launch_bounds__(ENSP_NT) k_ensp_project(const EnsRun *__restrict__ runs, const EnsState *st,
                                                           const EnsItem *__restrict__ items, int ns, int sweep,
                                                           const unsigned long long *any) {
@@ -641,7 +642,7 @@ __global__ void __launch_bounds__(ENSP_NT) k_ensp_project(const EnsRun *__restri
     if (!ens_sweep_open(R, sweep)) return;
     const int cur = __builtin_amdgcn_readfirstlane(st[r].cur);
     const KP p = ens_kp(R, ns, st[r].dt, cur ^ 1);
-    ens_project<ENSP_NT>(p, it.start, min(it.start + ENSP_CELLS, p.nxm * p.nz));
+    ens_project<ENSP_NT, RT>(p, it.start, min(it.start + ENSP_CELLS, p.nxm * p.nz));
 }
 
 // Mixed form: FP64 residual max|div u| (CheckDivergence units) after the

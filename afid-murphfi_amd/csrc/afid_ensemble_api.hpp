@@ -124,7 +124,7 @@ int ens_substep_phased_t(afid_ens *e, int ns) {
         hipLaunchKernelGGL(k_ensp_transform<R>, dim3(unsigned(e->n_line)), blk, e->lds_line, e->stream, e->d_runs, e->d_state,
                            e->d_items_line, ns, e->tune.lines, 1, sw, e->d_any);
         if ((rc = ens_launched(e, "k_ensp_transform"))) return rc;
-        hipLaunchKernelGGL(k_ensp_project, dim3(unsigned(e->n_cell)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_cell, ns, sw,
+        hipLaunchKernelGGL(k_ensp_project<R>, dim3(unsigned(e->n_cell)), blk, 0, e->stream, e->d_runs, e->d_state, e->d_items_cell, ns, sw,
                            e->d_any);
         if ((rc = ens_launched(e, "k_ensp_project"))) return rc;
         if (e->mixed) {
@@ -604,10 +604,9 @@ int afid_ens_autotune(afid_ens *e, int32_t steps, afid_ens_tuning *best, afid_en
     std::vector<std::pair<void *, size_t>> arrays;      // (array, bytes)
     for (afid_ctx *c : e->runs) {
         const size_t nb = size_t(c->nz) * c->ny * c->ldx;
-        for (double *a : {c->f[AFID_VX], c->alt[0], c->f[AFID_VZ], c->alt[2], c->f[AFID_TEMP], c->alt[3], c->f[AFID_PR],
-                          c->f[AFID_DPH]})
+        for (double *a : {c->f[AFID_VX], c->alt[0], c->f[AFID_VZ], c->alt[2], c->f[AFID_TEMP], c->alt[3], c->f[AFID_PR]})
             arrays.push_back({a, sizeof(double) * nb});
-        for (int f : {AFID_RUX, AFID_RUZ, AFID_RUTEMP}) {
+        for (int f : {AFID_DPH, AFID_RUX, AFID_RUZ, AFID_RUTEMP}) {
             if (c->g[f]) arrays.push_back({c->g[f], sizeof(float) * nb});
             else arrays.push_back({c->f[f], sizeof(double) * nb});
         }

@@ -226,6 +226,7 @@ static KP make_kp(const afid_ctx *c) {
     p.phi = c->f[AFID_PHI]; p.ruphi = c->f[AFID_RUPHI]; p.hphi = c->f[AFID_HPHI]; p.tempr = c->f[AFID_TEMPR];
     p.phic = c->f[AFID_PHIC]; p.qlat = c->f[AFID_HRO];
     p.phif = c->g[AFID_PHI]; p.ruphif = c->g[AFID_RUPHI]; p.temprf = c->g[AFID_TEMPR]; p.rhsrf = c->g[AFID_RHSR];
+    p.dphf = c->g[AFID_DPH];
     p.pf_d = c->pf_d; p.pf_b = c->pf_b; p.pf_a = c->c.pf_a; p.pf_tm = c->c.pf_tm; p.pf_s = c->c.pf_s; p.heta = c->heta;
     p.pf = c->pf ? 1 : 0;
     p.ap3spkr = c->d_ap3spkr; p.ac3spkr = c->d_ac3spkr; p.am3spkr = c->d_am3spkr;
@@ -527,13 +528,14 @@ static int setup(afid_ctx *c) {
     // fields.  FP64 form: every field in FP64.  Mixed form: the explicit-term
     // history and salc in FP32; qcap/dq/hro and the rhs arrays of the
     // per-routine calls are not held (the mixed form runs through the fused
-    // step); dph holds the FP64 pressure correction.
+    // step); dph holds the pressure correction of an FP32 sweep in FP32 (the
+    // inverse transform computes it in FP32, so nothing is lost).
     const size_t nb = size_t(c->nz) * c->ny * c->ldx;
     for (int f = 0; f < AFID_NFIELDS; ++f) {
         if (is_refined(f) || f == AFID_RHS || is_plane(f) || f == AFID_PHIC) continue;
         // (hro carries the latent heat of the phase field in both forms)
         if (c->mixed && (f == AFID_QCAP || f == AFID_DQ || (f == AFID_HRO && !c->pf))) continue;
-        if (c->mixed && ((f >= AFID_RUX && f <= AFID_RUTEMP) || f == AFID_SALC)) {
+        if (c->mixed && ((f >= AFID_RUX && f <= AFID_RUTEMP) || f == AFID_SALC || f == AFID_DPH)) {
             if (f == AFID_SALC && !c->sal) continue;
             if ((rc = dalloc(c, reinterpret_cast<void **>(&c->g[f]), sizeof(float) * nb))) return rc;
             continue;
@@ -1106,7 +1108,8 @@ static int project(afid_ctx *c, int vel, int pr, Gate gate = Gate{nullptr, 0.0})
     KP p = make_kp(c);
     dim3 gr = point_grid(c->nxm, c->ny, c->nz);
     if (gate.slot) gr.z = unsigned(std::min(c->nz, 32));
-    hipLaunchKernelGGL(k_project, gr, kPointBlock, 0, c->stream, p, vel, pr, gate);
+    if (c->mixed) hipLaunchKernelGGL(k_project<float>, gr, kPointBlock, 0, c->stream, p, vel, pr, gate);
+    else hipLaunchKernelGGL(k_project<double>, gr, kPointBlock, 0, c->stream, p, vel, pr, gate);
     return check_launch(c, "k_project");
 }
 

@@ -610,12 +610,14 @@ __global__ void __launch_bounds__(256) k_local_divergence(KP p) {
 
 // RemoveDivergence (pressure.F90:245-266) and/or CorrectPressure
 // (pressure.F90:276-300); both read only dph, so one sweep does both.
+// DT: storage type of dph (FP32 in the mixed form's sweeps)
+template <typename DT>
 __global__ void __launch_bounds__(256) k_project(KP p, int do_vel, int do_pr, Gate gate) {
     if (gate_closed(gate)) return;
     const int k = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
     if (k >= p.nxm || j >= p.ny) return;
     const int jm = wrapm(j, p.ny), jp = wrapp(j, p.ny);
-    const double *d = p.dph;
+    const DT *d = sel<DT>(p.dph, p.dphf);
     const int km = k == 0 ? 0 : k - 1, kp = k == p.nxm - 1 ? k : k + 1;
     // z walked with the grid's z stride: a full grid (one plane each) or, for
     // a gated launch, a short grid that costs little when the gate is closed
@@ -664,7 +666,7 @@ __global__ void __launch_bounds__(256) k_project_res(KP p, double *vx_out, doubl
     if (k < p.nxm && j < p.ny) {
         const int jm = wrapm(j, p.ny), jp = wrapp(j, p.ny);
         const int km = k == 0 ? 0 : k - 1, kp = k == p.nxm - 1 ? k : k + 1;
-        const double *d = p.dph;
+        const float *d = p.dphf;                  // the mixed form's FP32 sweep correction
         const double adt = p.al * p.dt, be = p.al * p.beta;
         const double ucx = adt * p.udx3c[k], ucx1 = k + 1 <= p.nxm - 1 ? adt * p.udx3c[k + 1] : 0.0;
         const long long row = (long long)j * p.ldx + k, rowjm = (long long)jm * p.ldx + k, rowjp = (long long)jp * p.ldx + k;

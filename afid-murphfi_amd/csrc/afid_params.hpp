@@ -73,6 +73,7 @@ struct KP {
     // refined temperature; coarse phic and the latent-heat source qlat
     double *phi, *ruphi, *hphi, *tempr, *phic, *qlat;
     float *phif, *ruphif, *temprf, *rhsrf;      // FP32 storage of the refined phase-field arrays (mixed form)
+    float *dphf;                                // FP32 pressure correction of the mixed form's Poisson sweeps
     double pf_d, pf_b, pf_a, pf_tm, pf_s, heta;  // pf_D, pf_D/eps^2, A, T_m, S_t, 0.5/eta
     int pf;
     const double *ap3spkr, *ac3spkr, *am3spkr;   // zero-gradient x operator of phi (refined)

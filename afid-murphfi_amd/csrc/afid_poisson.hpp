@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_ifft_y(KP p, FftPlan<R> pl, con
         const int j = dtx.div(t), xo = t - j * TX, k = k0 + xo;
         if (k >= p.nxm) continue;
         const cpx<R> z = lb[(xo >> 1) * S + lpad<R>(j)];
-        p.dph[bidx(p, k, j, i)] = double((xo & 1) ? z.y : z.x);
+        sel<R>(p.dph, p.dphf)[bidx(p, k, j, i)] = (xo & 1) ? z.y : z.x;      // dph in the sweep's arithmetic R
     }
 }
 
