@@ -1,4 +1,5 @@
-"""Build step for the HIP library libafid_gpu.so (gfx950 only).
+"""Build step for the HIP library libafid_gpu.so (gfx950 only), the host-only
+HDF5 library libafid_io.so and the Fortran host.
 
 Used by __graft_entry__.build(); can also be run directly:
     python afid-murphfi_amd/build_lib.py
@@ -20,6 +21,29 @@ def hipcc_cmd(out=LIB, extra=()):
             "-Wall", "-Wno-unused-function",
             "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
             *extra, "-o", out, SRC]
+
+
+IO_LIB = os.path.join(HERE, "libafid_io.so")
+IO_SRC = os.path.join(HERE, "csrc", "afid_io.c")
+# serial HDF5 C library of the image (the reference needs HDF5 for its files)
+HDF5_PREFIX = os.environ.get("AFID_HDF5_PREFIX", "/opt/conda")
+
+
+def build_io(verbose=True):
+    """Host-only HDF5 library libafid_io.so (checkpoints, means.h5)."""
+    cc = os.environ.get("CC", "gcc")
+    inc, libdir = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    cmd = [cc, "-O2", "-std=c11", "-Wall", "-Wextra", "-fPIC", "-shared",
+           "-I", os.path.join(ROOT, "include"), "-I", inc, "-o", IO_LIB + ".tmp", IO_SRC,
+           "-L" + libdir, "-lhdf5", "-Wl,-rpath," + libdir]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building libafid_io.so failed (HDF5 under %s)" % HDF5_PREFIX)
+    os.replace(IO_LIB + ".tmp", IO_LIB)
+    return IO_LIB
 
 
 FORTRAN_DIR = os.path.join(HERE, "fortran")
@@ -58,6 +82,7 @@ def build(verbose=True):
         if lines:
             sys.stderr.write("\n".join(lines) + "\n")
     os.replace(LIB + ".tmp", LIB)
+    build_io(verbose)
     if os.path.exists("/opt/rocm/bin/amdflang") or os.environ.get("FC"):
         build_fortran_host(verbose)
     return LIB

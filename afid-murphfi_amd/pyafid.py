@@ -66,6 +66,19 @@ ROUTINES = ["explicit_terms_vx", "explicit_terms_vy", "explicit_terms_vz", "expl
 
 FP64, MIXED = 0, 1
 
+# libafid_io.so: the solver's HDF5 files (include/afid_io.h)
+IO_LIB_PATH = os.path.join(HERE, "libafid_io.so")
+IO_EXPORTS = [
+    "afid_h5_last_error", "afid_h5_create_file", "afid_h5_create_group",
+    "afid_h5_write_field3d", "afid_h5_read_field3d", "afid_h5_dataset_dims",
+    "afid_h5_write_int_scalar", "afid_h5_write_real_scalar", "afid_h5_read_int_scalar", "afid_h5_read_real_scalar",
+    "afid_h5_write_real1d", "afid_h5_read_real1d",
+]
+# checkpoint file of each field (HdfReadContinua.F90:44-61, WriteFlowField.F90:26-50)
+CONTINUA = {"vx": "vx", "vy": "vy", "vz": "vz", "temp": "temp", "sal": "sal", "phi": "phi", "pr": "pr"}
+# the phase-field profiles of afid_calc_phi_stats (phasefield.F90:788-793)
+PHI_PROFILES = ["phibar", "phirms"]
+
 
 class AfidConfig(C.Structure):
     """Mirror of afid_config: the bou.in value lines (ReadInputFile.F90:20-76)."""
@@ -200,6 +213,126 @@ def lib():
     return _lib
 
 
+_io = None
+
+
+def io_lib():
+    """Load libafid_io.so (host-only HDF5 library); raises if it has not been built."""
+    global _io
+    if _io is None:
+        if not os.path.exists(IO_LIB_PATH):
+            raise RuntimeError("libafid_io.so is not built: run __graft_entry__.build() "
+                               "or python afid-murphfi_amd/build_lib.py")
+        L = C.CDLL(IO_LIB_PATH)
+        cp, dp, i64 = C.c_char_p, C.POINTER(C.c_double), C.c_int64
+        L.afid_h5_last_error.argtypes = []
+        L.afid_h5_last_error.restype = cp
+        L.afid_h5_create_file.argtypes = [cp]
+        L.afid_h5_create_group.argtypes = [cp, cp]
+        L.afid_h5_write_field3d.argtypes = [cp, dp, i64, i64, i64]
+        L.afid_h5_read_field3d.argtypes = [cp, dp, i64, i64, i64]
+        L.afid_h5_dataset_dims.argtypes = [cp, cp, C.POINTER(i64), C.c_int]
+        L.afid_h5_write_int_scalar.argtypes = [cp, cp, C.c_int32]
+        L.afid_h5_write_real_scalar.argtypes = [cp, cp, C.c_double]
+        L.afid_h5_read_int_scalar.argtypes = [cp, cp, C.POINTER(C.c_int32)]
+        L.afid_h5_read_real_scalar.argtypes = [cp, cp, dp]
+        L.afid_h5_write_real1d.argtypes = [cp, cp, dp, i64]
+        L.afid_h5_read_real1d.argtypes = [cp, cp, dp, i64]
+        for n in IO_EXPORTS:
+            if n != "afid_h5_last_error":
+                getattr(L, n).restype = C.c_int
+        _io = L
+    return _io
+
+
+class H5Error(RuntimeError):
+    pass
+
+
+def _h5(rc, what):
+    if rc != 0:
+        raise H5Error(f"{what} failed ({rc}): {io_lib().afid_h5_last_error().decode()}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def h5_create_file(path):
+    """HdfCreateBlankFile (HdfRoutines.F90:11-22)."""
+    _h5(io_lib().afid_h5_create_file(os.fsencode(path)), "afid_h5_create_file " + str(path))
+
+
+def h5_create_group(path, name):
+    _h5(io_lib().afid_h5_create_group(os.fsencode(path), name.encode()), f"afid_h5_create_group {name}")
+
+
+def h5_write_field(path, arr):
+    """HdfWriteRealHalo3D (HdfRoutines.F90:428-520): `arr` is [nz][ny][nx]
+    (x fastest), stored as dataset "var" of Fortran dims (nx, ny, nz)."""
+    a = np.ascontiguousarray(arr, dtype=np.float64)
+    nz, ny, nx = a.shape
+    _h5(io_lib().afid_h5_write_field3d(os.fsencode(path), _dptr(a), nx, ny, nz), "afid_h5_write_field3d " + str(path))
+
+
+def h5_read_field(path, shape):
+    """HdfReadContinua (HdfReadContinua.F90:11-107): the [nz][ny][nx] array of
+    `path`; raises if the file's dims differ from `shape`."""
+    a = np.empty(shape, dtype=np.float64)
+    nz, ny, nx = shape
+    _h5(io_lib().afid_h5_read_field3d(os.fsencode(path), _dptr(a), nx, ny, nz), "afid_h5_read_field3d " + str(path))
+    return a
+
+
+def h5_dims(path, name="var"):
+    """Fortran-order dims (fastest first) of dataset `name`."""
+    d = (C.c_int64 * 8)()
+    r = io_lib().afid_h5_dataset_dims(os.fsencode(path), name.encode(), d, 8)
+    if r < 0:
+        raise H5Error(f"afid_h5_dataset_dims {name} failed ({r}): {io_lib().afid_h5_last_error().decode()}")
+    return tuple(d[i] for i in range(r))
+
+
+def h5_write_scalar(path, name, value):
+    """HdfSerialWriteIntScalar / HdfSerialWriteRealScalar (HdfRoutines.F90:25-96):
+    Python ints are written as 32-bit integers, floats as doubles."""
+    L = io_lib()
+    if isinstance(value, (int, np.integer)):
+        rc = L.afid_h5_write_int_scalar(os.fsencode(path), name.encode(), int(value))
+    else:
+        rc = L.afid_h5_write_real_scalar(os.fsencode(path), name.encode(), float(value))
+    _h5(rc, f"writing scalar {name}")
+
+
+def h5_read_int(path, name):
+    v = C.c_int32()
+    _h5(io_lib().afid_h5_read_int_scalar(os.fsencode(path), name.encode(), C.byref(v)), f"reading {name}")
+    return int(v.value)
+
+
+def h5_read_real(path, name):
+    v = C.c_double()
+    _h5(io_lib().afid_h5_read_real_scalar(os.fsencode(path), name.encode(), C.byref(v)), f"reading {name}")
+    return float(v.value)
+
+
+def h5_write_1d(path, name, arr):
+    """HdfSerialWriteReal1D (HdfRoutines.F90:139-177)."""
+    a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+    _h5(io_lib().afid_h5_write_real1d(os.fsencode(path), name.encode(), _dptr(a), a.size), f"writing {name}")
+
+
+def h5_read_1d(path, name, n):
+    a = np.empty(n, dtype=np.float64)
+    _h5(io_lib().afid_h5_read_real1d(os.fsencode(path), name.encode(), _dptr(a), n), f"reading {name}")
+    return a
+
+
+def fortran_nint(x):
+    """Fortran NINT: nearest integer, halves away from zero."""
+    return int(np.sign(x) * np.floor(abs(x) + 0.5))
+
+
 class AfidError(RuntimeError):
     pass
 
@@ -319,6 +452,120 @@ class Solver:
         out = np.zeros(2 * n)
         self._check(self.L.afid_calc_phi_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "afid_calc_phi_stats")
         return {"phibar": out[:n].copy(), "phirms": out[n:].copy()}
+
+    # -- files (libafid_io.so)
+    def _grid_scalars(self):
+        c = self.cfgd
+        g = dict(nx=c["nxm"] + 1, ny=c["nym"] + 1, nz=c["nzm"] + 1, ylen=float(c["ylen"]), zlen=float(c["zlen"]),
+                 istr3=int(c["istr3"]), str3=float(c["str3"]))
+        if c["multires"]:
+            g.update(nxr=c["nxmr"] + 1, nyr=c["nymr"] + 1, nzr=c["nzmr"] + 1, istr3r=int(c["istr3r"]))
+        return g
+
+    def _continua_fields(self, final):
+        c = self.cfgd
+        names = ["temp", "vx", "vy", "vz"]
+        if c["salinity"]:
+            names.append("sal")
+        if c["phasefield"]:
+            names.append("phi")
+        if final:
+            names.append("pr")
+        return names
+
+    def write_flow_field(self, outdir, time, final=True, save_3D=None):
+        """WriteFlowField (WriteFlowField.F90:11-86): outdir/continua_<field>.h5
+        for temp, vx, vy, vz (+ sal, phi on the refined grid; + pr when final)
+        and, when final, outdir/continua_master.h5 with the grid and the time.
+        Not final: outdir/fields/<nint(time/save_3D) as 5 digits>_<field>.h5,
+        without pr and master.  Returns the list of files written."""
+        if final:
+            base = os.path.join(outdir, "continua")
+        else:
+            if not save_3D:
+                raise ValueError("write_flow_field(final=False) needs save_3D")
+            os.makedirs(os.path.join(outdir, "fields"), exist_ok=True)
+            base = os.path.join(outdir, "fields", "%05d" % fortran_nint(time / save_3D))
+        out = []
+        for n in self._continua_fields(final):
+            path = f"{base}_{CONTINUA[n]}.h5"
+            h5_write_field(path, self.download(n))
+            out.append(path)
+        if final:
+            path = os.path.join(outdir, "continua_master.h5")
+            h5_create_file(path)
+            g = self._grid_scalars()
+            for k in ["nx", "ny", "nz", "ylen", "zlen"]:
+                h5_write_scalar(path, k, g[k])
+            h5_write_scalar(path, "time", float(time))
+            h5_write_scalar(path, "istr3", g["istr3"])
+            h5_write_scalar(path, "str3", g["str3"])
+            if self.cfgd["multires"]:
+                for k in ["nxr", "nyr", "nzr", "istr3r"]:
+                    h5_write_scalar(path, k, g[k])
+            out.append(path)
+        return out
+
+    def read_flow_field(self, outdir, interpolate=True):
+        """ReadFlowInterp (ReadFlowInterp.F90:1-179) on the same grid: reads
+        continua_master.h5, checks the grid against the configured one (:84-89),
+        uploads vx, vy, vz, temp (+ sal, phi; + pr if continua_pr.h5 exists,
+        :143-171) and returns the checkpoint's time.  A checkpoint on another
+        grid raises: the interpolation onto a new grid (:90-141) is not built.
+        With `interpolate`, the start-up transfers that main.F90:205-213 runs
+        after either initial state follow: InterpVelMgrd and InterpSalMultigrid
+        (salinity), InterpTempMultigrid and InterpPhiMultigrid (phase field)."""
+        master = os.path.join(outdir, "continua_master.h5")
+        g = self._grid_scalars()
+        old = {k: h5_read_int(master, k) for k in ["nx", "ny", "nz", "istr3"]}
+        old["str3"] = h5_read_real(master, "str3")
+        if self.cfgd["multires"]:
+            old.update({k: h5_read_int(master, k) for k in ["nxr", "nyr", "nzr", "istr3r"]})
+        t = h5_read_real(master, "time")
+        same = all(old[k] == g[k] for k in old if k != "str3") and abs(g["str3"] - old["str3"]) <= 1e-8
+        if not same:
+            raise AfidError(f"checkpoint grid {old} differs from the configured grid {g}: "
+                            "restart onto a new grid (ReadFlowInterp.F90:90-141) is not built")
+        for n in self._continua_fields(final=False):
+            self.upload(n, h5_read_field(os.path.join(outdir, f"continua_{CONTINUA[n]}.h5"), self.shape(n)))
+        pr = os.path.join(outdir, "continua_pr.h5")
+        if os.path.exists(pr):
+            self.upload("pr", h5_read_field(pr, self.shape("pr")))
+        if interpolate:
+            if self.cfgd["salinity"]:
+                self.call("interp_vel_mgrd")
+                self.call("interp_sal_multigrid")
+            if self.cfgd["phasefield"]:
+                self.call("interp_temp_multigrid")
+                self.call("interp_phi_multigrid")
+        return t
+
+    def write_means(self, outdir, time, tout):
+        """The file part of CalcMeanProfiles (CalcMeanProfiles.F90:209-283),
+        CalcSalStats (salinity.F90:588-603) and CalcPhiStats
+        (phasefield.F90:786-793): outdir/means.h5 gets <profile>/<nint(time/tout)
+        as 5 digits> for every profile; the file and its groups are created on
+        the first call.  Returns the profiles written."""
+        path = os.path.join(outdir, "means.h5")
+        c = self.cfgd
+        groups = list(MEAN_PROFILES)
+        if c["salinity"]:
+            groups += SAL_PROFILES
+        if c["phasefield"]:
+            groups += PHI_PROFILES
+        if not os.path.exists(path):
+            h5_create_file(path)
+            for gname in groups:
+                h5_create_group(path, gname)
+        prof = self.mean_profiles()
+        if c["salinity"]:
+            prof.update(self.sal_stats())
+        if c["phasefield"]:
+            prof.update(self.phi_stats())
+        nstat = "%05d" % fortran_nint(time / tout)
+        for gname in groups:
+            h5_write_1d(path, f"{gname}/{nstat}", prof[gname])
+        return prof
 
     def refinement_residuals(self):
         """Mixed form: max|div u| before each refinement sweep of the last substep."""
