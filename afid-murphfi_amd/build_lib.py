@@ -56,8 +56,9 @@ def build_fortran_host(verbose=True):
     fc = os.environ.get("FC", "/opt/rocm/bin/amdflang")
     with tempfile.TemporaryDirectory() as tmp:
         cmd = [fc, "-O2", "-module-dir", tmp, "-o", HOST_EXE + ".tmp",
-               os.path.join(FORTRAN_DIR, "afid_gpu_mod.F90"), os.path.join(FORTRAN_DIR, "afid_host.F90"),
-               "-L" + HERE, "-lafid_gpu", "-Wl,-rpath,$ORIGIN"]
+               os.path.join(FORTRAN_DIR, "afid_gpu_mod.F90"), os.path.join(FORTRAN_DIR, "afid_io_mod.F90"),
+               os.path.join(FORTRAN_DIR, "afid_host.F90"),
+               "-L" + HERE, "-lafid_gpu", "-lafid_io", "-Wl,-rpath,$ORIGIN"]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)

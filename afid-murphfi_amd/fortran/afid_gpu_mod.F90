@@ -8,6 +8,7 @@ module afid_gpu
     integer(c_int), parameter :: AFID_VX = 0, AFID_VY = 1, AFID_VZ = 2, AFID_TEMP = 3, AFID_PR = 4
     integer(c_int), parameter :: AFID_SAL = 14, AFID_VXR = 18, AFID_VYR = 19, AFID_VZR = 20, AFID_SALC = 21
     integer(c_int), parameter :: AFID_TEMPBP = 22, AFID_TEMPTP = 23, AFID_SALBP = 24, AFID_SALTP = 25
+    integer(c_int), parameter :: AFID_PHI = 26
 
     !> the bou.in value lines (ReadInputFile.F90:20-76) + arithmetic form
     type, bind(C) :: afid_config
@@ -119,6 +120,13 @@ module afid_gpu
             import; type(c_ptr), value :: ctx
         end function
         integer(c_int) function afid_interp_sal_multigrid(ctx) bind(C, name='afid_interp_sal_multigrid')
+            import; type(c_ptr), value :: ctx
+        end function
+        !> InterpTempMultigrid / InterpPhiMultigrid (phasefield.F90)
+        integer(c_int) function afid_interp_temp_multigrid(ctx) bind(C, name='afid_interp_temp_multigrid')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_interp_phi_multigrid(ctx) bind(C, name='afid_interp_phi_multigrid')
             import; type(c_ptr), value :: ctx
         end function
         integer(c_int) function afid_time_marcher(ctx, dt) bind(C, name='afid_time_marcher')

@@ -4,6 +4,8 @@
 !! libafid_gpu through the iso_c_binding module afid_gpu.
 !!
 !! usage: afid_host [fused] [mixed] [seed=N]   (reads bou.in in the working directory)
+!!   files: outputdir/continua_*.h5 at a normal end, outputdir/fields/ every SAVE_3D,
+!!   outputdir/means.h5 every TOUT; NREAD = 1 restarts from outputdir/continua_*.h5
 !!   fused: one afid_time_marcher call per step instead of the call tree
 !!   mixed: the mixed FP32/FP64 form (runs through afid_time_marcher)
 module afid_host_state
@@ -141,6 +143,193 @@ subroutine TimeMarcher
     end do
 end subroutine TimeMarcher
 
+!> extent (x incl. the top point, y, z) of a base or refined module array
+function field_extent(refined) result(ext)
+    use afid_host_state
+    implicit none
+    logical, intent(in) :: refined
+    integer(c_int64_t) :: ext(3)
+    if (refined) then
+        ext = [int(cfg%nxmr + 1, c_int64_t), int(cfg%nymr, c_int64_t), int(cfg%nzmr, c_int64_t)]
+    else
+        ext = [int(cfg%nxm + 1, c_int64_t), int(cfg%nym, c_int64_t), int(cfg%nzm, c_int64_t)]
+    end if
+end function field_extent
+
+!> one module array to / from its HDF5 file (HdfWriteRealHalo3D,
+!! HdfRoutines.F90:428-520 / HdfReadContinua, HdfReadContinua.F90:11-107)
+subroutine field_file(field, refined, path, write_it)
+    use afid_host_state
+    use afid_io
+    implicit none
+    integer(c_int), intent(in) :: field
+    logical, intent(in) :: refined, write_it
+    character(len=*), intent(in) :: path
+    interface
+        function field_extent(refined) result(ext)
+            import :: c_int64_t
+            logical, intent(in) :: refined
+            integer(c_int64_t) :: ext(3)
+        end function
+    end interface
+    real(c_double), allocatable :: buf(:)
+    integer(c_int64_t) :: ext(3), lb(3)
+    lb = 1
+    ext = field_extent(refined)
+    allocate(buf(ext(1) * ext(2) * ext(3)))
+    if (write_it) then
+        call afid_check(ctx, afid_download(ctx, field, buf, lb, ext), 'download '//path)
+        call h5_check(afid_h5_write_field3d(cstr(path), buf, ext(1), ext(2), ext(3)), 'write '//path)
+    else
+        call h5_check(afid_h5_read_field3d(cstr(path), buf, ext(1), ext(2), ext(3)), 'read '//path)
+        call afid_check(ctx, afid_upload(ctx, field, buf, lb, ext), 'upload '//path)
+    end if
+end subroutine field_file
+
+!> WriteFlowField.F90:11-86
+subroutine WriteFlowField(final, time)
+    use afid_host_state
+    use afid_io
+    implicit none
+    logical, intent(in) :: final
+    real(c_double), intent(in) :: time
+    character(len=64) :: base, frame, master
+    base = 'outputdir/continua'
+    if (.not. final) then
+        write(frame, '(i5.5)') nint(time / save_3D)
+        base = 'outputdir/fields/' // trim(frame)
+    end if
+    call field_file(AFID_TEMP, .false., trim(base)//'_temp.h5', .true.)
+    call field_file(AFID_VX, .false., trim(base)//'_vx.h5', .true.)
+    call field_file(AFID_VY, .false., trim(base)//'_vy.h5', .true.)
+    call field_file(AFID_VZ, .false., trim(base)//'_vz.h5', .true.)
+    if (salinity) call field_file(AFID_SAL, .true., trim(base)//'_sal.h5', .true.)
+    if (cfg%phasefield /= 0) call field_file(AFID_PHI, .true., trim(base)//'_phi.h5', .true.)
+    if (.not. final) return
+    call field_file(AFID_PR, .false., trim(base)//'_pr.h5', .true.)
+    master = 'outputdir/continua_master.h5'
+    call h5_check(afid_h5_create_file(cstr(master)), 'create '//master)
+    call h5_check(afid_h5_write_int_scalar(cstr(master), cstr('nx'), cfg%nxm + 1), 'nx')
+    call h5_check(afid_h5_write_int_scalar(cstr(master), cstr('ny'), cfg%nym + 1), 'ny')
+    call h5_check(afid_h5_write_int_scalar(cstr(master), cstr('nz'), cfg%nzm + 1), 'nz')
+    call h5_check(afid_h5_write_real_scalar(cstr(master), cstr('ylen'), cfg%ylen), 'ylen')
+    call h5_check(afid_h5_write_real_scalar(cstr(master), cstr('zlen'), cfg%zlen), 'zlen')
+    call h5_check(afid_h5_write_real_scalar(cstr(master), cstr('time'), time), 'time')
+    call h5_check(afid_h5_write_int_scalar(cstr(master), cstr('istr3'), cfg%istr3), 'istr3')
+    call h5_check(afid_h5_write_real_scalar(cstr(master), cstr('str3'), cfg%str3), 'str3')
+    if (cfg%multires /= 0) then
+        call h5_check(afid_h5_write_int_scalar(cstr(master), cstr('nxr'), cfg%nxmr + 1), 'nxr')
+        call h5_check(afid_h5_write_int_scalar(cstr(master), cstr('nyr'), cfg%nymr + 1), 'nyr')
+        call h5_check(afid_h5_write_int_scalar(cstr(master), cstr('nzr'), cfg%nzmr + 1), 'nzr')
+        call h5_check(afid_h5_write_int_scalar(cstr(master), cstr('istr3r'), cfg%istr3r), 'istr3r')
+    end if
+end subroutine WriteFlowField
+
+!> ReadFlowInterp.F90:1-179 on the same grid; another grid stops the run
+!! (the interpolation onto a new grid, :90-141, is not built)
+subroutine ReadFlowInterp(time)
+    use afid_host_state
+    use afid_io
+    implicit none
+    real(c_double), intent(out) :: time
+    character(len=*), parameter :: master = 'outputdir/continua_master.h5'
+    integer(c_int32_t) :: nxo, nyo, nzo, istr3o, nxro, nyro, nzro, istr3ro
+    real(c_double) :: str3o
+    logical :: newgrid, ex
+    call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('nx'), nxo), 'read nx')
+    call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('ny'), nyo), 'read ny')
+    call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('nz'), nzo), 'read nz')
+    call h5_check(afid_h5_read_real_scalar(cstr(master), cstr('time'), time), 'read time')
+    call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('istr3'), istr3o), 'read istr3')
+    call h5_check(afid_h5_read_real_scalar(cstr(master), cstr('str3'), str3o), 'read str3')
+    newgrid = (cfg%nxm + 1 /= nxo) .or. (cfg%nym + 1 /= nyo) .or. (cfg%nzm + 1 /= nzo) .or. &
+              (cfg%istr3 /= istr3o) .or. (abs(cfg%str3 - str3o) > 1d-8)
+    if (cfg%multires /= 0) then
+        call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('nxr'), nxro), 'read nxr')
+        call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('nyr'), nyro), 'read nyr')
+        call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('nzr'), nzro), 'read nzr')
+        call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('istr3r'), istr3ro), 'read istr3r')
+        newgrid = newgrid .or. (cfg%nxmr + 1 /= nxro) .or. (cfg%nymr + 1 /= nyro) .or. &
+                  (cfg%nzmr + 1 /= nzro) .or. (cfg%istr3r /= istr3ro)
+    end if
+    if (newgrid) then
+        write(0, '(a)') 'ReadFlowInterp: the checkpoint is on another grid; restart onto a new grid is not built'
+        error stop 2
+    end if
+    call field_file(AFID_VX, .false., 'outputdir/continua_vx.h5', .false.)
+    call field_file(AFID_VY, .false., 'outputdir/continua_vy.h5', .false.)
+    call field_file(AFID_VZ, .false., 'outputdir/continua_vz.h5', .false.)
+    call field_file(AFID_TEMP, .false., 'outputdir/continua_temp.h5', .false.)
+    if (salinity) call field_file(AFID_SAL, .true., 'outputdir/continua_sal.h5', .false.)
+    if (cfg%phasefield /= 0) call field_file(AFID_PHI, .true., 'outputdir/continua_phi.h5', .false.)
+    inquire(file='outputdir/continua_pr.h5', exist=ex)
+    if (ex) call field_file(AFID_PR, .false., 'outputdir/continua_pr.h5', .false.)
+    if (ireset /= 0) time = 0.d0
+end subroutine ReadFlowInterp
+
+!> the file part of CalcMeanProfiles (CalcMeanProfiles.F90:209-283), CalcSalStats
+!! (salinity.F90:588-603) and CalcPhiStats (phasefield.F90:786-793)
+subroutine WriteMeans(time)
+    use afid_host_state
+    use afid_io
+    implicit none
+    real(c_double), intent(in) :: time
+    character(len=*), parameter :: path = 'outputdir/means.h5'
+    character(len=7), parameter :: mnames(15) = [character(len=7) :: 'Tbar', 'vybar', 'vzbar', 'vxT', 'vyT', &
+        'vzT', 'Trms', 'vxrms', 'vyrms', 'vzrms', 'vxvy', 'vxvz', 'vyvz', 'chiT', 'epsilon']
+    character(len=7), parameter :: snames(6) = [character(len=7) :: 'Sbar', 'Srms', 'vxS', 'vyS', 'vzS', 'chiS']
+    character(len=7), parameter :: pnames(2) = [character(len=7) :: 'phibar', 'phirms']
+    real(c_double), allocatable :: buf(:)
+    character(len=5) :: nstat
+    logical :: ex
+    integer :: q
+    integer(c_int64_t) :: n
+    inquire(file=path, exist=ex)
+    if (.not. ex) then
+        call h5_check(afid_h5_create_file(cstr(path)), 'create '//path)
+        do q = 1, 15
+            call h5_check(afid_h5_create_group(cstr(path), cstr(mnames(q))), 'group')
+        end do
+        if (salinity) then
+            do q = 1, 6
+                call h5_check(afid_h5_create_group(cstr(path), cstr(snames(q))), 'group')
+            end do
+        end if
+        if (cfg%phasefield /= 0) then
+            do q = 1, 2
+                call h5_check(afid_h5_create_group(cstr(path), cstr(pnames(q))), 'group')
+            end do
+        end if
+    end if
+    write(nstat, '(i5.5)') nint(time / tout)
+    n = cfg%nxm
+    allocate(buf(15 * n))
+    call afid_check(ctx, afid_calc_mean_profiles(ctx, buf), 'CalcMeanProfiles')
+    do q = 1, 15
+        call h5_check(afid_h5_write_real1d(cstr(path), cstr(trim(mnames(q))//'/'//nstat), buf((q - 1) * n + 1), n), &
+                      'write '//mnames(q))
+    end do
+    deallocate(buf)
+    n = cfg%nxmr
+    if (salinity) then
+        allocate(buf(6 * n))
+        call afid_check(ctx, afid_calc_sal_stats(ctx, buf), 'CalcSalStats')
+        do q = 1, 6
+            call h5_check(afid_h5_write_real1d(cstr(path), cstr(trim(snames(q))//'/'//nstat), buf((q - 1) * n + 1), n), &
+                          'write '//snames(q))
+        end do
+        deallocate(buf)
+    end if
+    if (cfg%phasefield /= 0) then
+        allocate(buf(2 * n))
+        call afid_check(ctx, afid_calc_phi_stats(ctx, buf), 'CalcPhiStats')
+        do q = 1, 2
+            call h5_check(afid_h5_write_real1d(cstr(path), cstr(trim(pnames(q))//'/'//nstat), buf((q - 1) * n + 1), n), &
+                          'write '//pnames(q))
+        end do
+    end if
+end subroutine WriteMeans
+
 program afid_host
     use afid_host_state
     implicit none
@@ -163,13 +352,31 @@ program afid_host
     end do
 
     call ReadInputFile
+    ! main.F90:77, before anything touches the GPU
+    call execute_command_line('mkdir -p outputdir/fields')
     rc = afid_gpu_init(cfg, ctx)
     call afid_check(ctx, rc, 'afid_gpu_init')
     call SetBCs
-    call afid_check(ctx, afid_create_initial_conditions(ctx, seed, 1.d-3), 'CreateInitialConditions')
+    if (nread /= 0) then
+        ! main.F90:156-161, then the start-up transfers of :205-213
+        call ReadFlowInterp(time)
+        if (salinity) then
+            call afid_check(ctx, afid_interp_vel_mgrd(ctx), 'InterpVelMgrd')
+            call afid_check(ctx, afid_interp_sal_multigrid(ctx), 'InterpSalMultigrid')
+        end if
+        if (cfg%phasefield /= 0) then
+            call afid_check(ctx, afid_interp_temp_multigrid(ctx), 'InterpTempMultigrid')
+            call afid_check(ctx, afid_interp_phi_multigrid(ctx), 'InterpPhiMultigrid')
+        end if
+    else
+        time = 0.d0
+        call afid_check(ctx, afid_create_initial_conditions(ctx, seed, 1.d-3), 'CreateInitialConditions')
+    end if
+    ! main.F90:227 and :241
+    call WriteMeans(time)
+    call WriteFlowField(.false., time)
 
     call cpu_time(tin)
-    time = 0.d0
     minwtdt = huge(0.d0)
     errorcode = 0
     dmax = 0.d0; dmaxr = 0.d0
@@ -194,12 +401,16 @@ program afid_host
 
         call TimeMarcher
         time = time + dt
+        ! main.F90:315-320
+        if (mod(time, tout) < dt) call WriteMeans(time)
 
         if (ntime == 1 .or. mod(time, tout) < dt) then
             call afid_check(ctx, afid_check_divergence(ctx, dmax, dmaxr), 'CheckDivergence')
             if (abs(dmax) > resid) errorcode = 169
         end if
         if (time > tmax) errorcode = 333
+        ! main.F90:376-380
+        if (mod(time, save_3D) < dt .and. floor(time / tframe) /= 0) call WriteFlowField(.false., time)
         call afid_check(ctx, afid_synchronize(ctx), 'synchronize')
         call cpu_time(t1)
         minwtdt = min(minwtdt, t1 - t0)
@@ -208,6 +419,8 @@ program afid_host
         if (errorcode /= 0) exit
     end do
 
+    ! QuitRoutine.F90:34-38: a normal end (333, 334, 555) writes the checkpoint
+    if (errorcode == 333 .or. errorcode == 334 .or. errorcode == 555) call WriteFlowField(.true., time)
     call afid_check(ctx, afid_check_divergence(ctx, dmax, dmaxr), 'CheckDivergence')
     call afid_check(ctx, afid_global_quantities(ctx, q), 'GlobalQuantities')
     write(*, '(a,i0,a,i0)') 'steps=', ntime + 1, ' errorcode=', errorcode

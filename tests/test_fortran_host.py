@@ -69,3 +69,39 @@ def test_fortran_host_time_limit(tmp_path):
     write_bou(os.path.join(tmp_path, "bou.in"), 16, 8, 8, 5, 1e6)
     r = run_host(tmp_path, "fused")
     assert int(r["errorcode"]) == 333 and int(r["steps"]) == 1
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+def test_fortran_host_checkpoint_and_restart(tmp_path):
+    """NREAD = 1 restarts from outputdir/continua_*.h5 (ReadFlowInterp.F90:143-171):
+    5 steps, the checkpoint a normal end writes (QuitRoutine.F90:34-38), then
+    5 more steps from it give the bits of 10 uninterrupted steps.  The run also
+    writes means.h5 every TOUT and the start-up snapshot of main.F90:241."""
+    import pyafid
+    nxm, nym, nzm, dt = 24, 16, 16, 2e-3
+    one, two = tmp_path / "one", tmp_path / "two"
+    os.makedirs(one)
+    os.makedirs(two)
+    write_bou(os.path.join(one, "bou.in"), nxm, nym, nzm, 9, dt)
+    r10 = run_host(one, "fused")
+    assert int(r10["steps"]) == 10 and int(r10["errorcode"]) == 555
+    write_bou(os.path.join(two, "bou.in"), nxm, nym, nzm, 4, dt)
+    r5 = run_host(two, "fused")
+    assert int(r5["steps"]) == 5
+    out = two / "outputdir"
+    assert pyafid.h5_dims(out / "continua_vx.h5") == (nxm + 1, nym, nzm)
+    assert pyafid.h5_read_int(out / "continua_master.h5", "nx") == nxm + 1
+    assert abs(pyafid.h5_read_real(out / "continua_master.h5", "time") - 5 * dt) < 1e-15
+    assert pyafid.h5_dims(out / "fields" / "00000_temp.h5") == (nxm + 1, nym, nzm)
+    assert pyafid.h5_dims(out / "means.h5", "Tbar/00000") == (nxm,)
+    txt = open(os.path.join(two, "bou.in")).read()
+    assert txt.count("NREAD       IRESET\n0           0") == 1
+    open(os.path.join(two, "bou.in"), "w").write(txt.replace("NREAD       IRESET\n0           0",
+                                                             "NREAD       IRESET\n1           0"))
+    r = run_host(two, "fused")
+    assert int(r["steps"]) == 5
+    assert abs(r["time"] - 10 * dt) < 1e-14
+    assert r["nu"] == r10["nu"] and r["ke"] == r10["ke"] and r["maxdiv"] == r10["maxdiv"]
+    a = pyafid.h5_read_field(one / "outputdir" / "continua_temp.h5", (nzm, nym, nxm + 1))
+    b = pyafid.h5_read_field(out / "continua_temp.h5", (nzm, nym, nxm + 1))
+    assert np.array_equal(a, b)

@@ -252,3 +252,9 @@ def test_write_means_host(tmp_path):
         assert np.array_equal(pyafid.h5_read_1d(f, f"{name}/00003", MR["nxm"]), p2[name])
     for name in pyafid.SAL_PROFILES + pyafid.PHI_PROFILES:
         assert pyafid.h5_dims(f, f"{name}/00003") == (MR["nxmr"],)
+
+
+def test_fortran_binding_covers_the_header():
+    src = open(os.path.join(ROOT, "afid-murphfi_amd", "fortran", "afid_io_mod.F90")).read()
+    bound = sorted(set(re.findall(r"bind\(C, name='(afid_h5_\w+)'\)", src)))
+    assert bound == header_functions()
