@@ -21,6 +21,7 @@ from helpers import uniform
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HDF5_SIGNATURE = b"\x89HDF\r\n\x1a\n"
+pytestmark = pytest.mark.skipif(not os.path.exists(pyafid.IO_LIB_PATH), reason="libafid_io.so not built")
 
 
 def header_functions():
