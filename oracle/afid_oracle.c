@@ -7,7 +7,7 @@
  * the arithmetic follow the reference's 1-based Fortran convention:
  * k = x (wall-normal, contiguous), j = y, i = z.
  *
- * Departures from the reference, all deliberate and documented in DESIGN.md:
+ * Departures from the reference, all deliberate:
  *  - periodic halos are index wraps (one rank: halo_common.F90:66-71 copies);
  *  - LAPACK dgttrf/dgttrs and zgttrf/zgttrs are replaced by Thomas
  *    elimination without pivoting (the matrices are diagonally dominant);
