@@ -5,7 +5,8 @@
  * arrays to and from files; afid_download / afid_upload of afid_gpu.h move
  * the same arrays to and from the device.  Every entry point returns 0 on
  * success and a non-zero code on failure, with the message in
- * afid_h5_last_error().
+ * afid_h5_last_error().  The serial HDF5 library underneath is not
+ * thread-safe: call these functions from one thread at a time.
  *
  * Layout of a 3-D field file (HdfWriteRealHalo3D, HdfRoutines.F90:428-520):
  * one dataset "var", 64-bit IEEE doubles, Fortran dims (nx, ny, nz) = C dims
