@@ -71,21 +71,39 @@ def test_fortran_host_time_limit(tmp_path):
     assert int(r["errorcode"]) == 333 and int(r["steps"]) == 1
 
 
+def with_salinity(path):
+    """Turn the bou.in at `path` into a multires case with salinity on a 2x refined grid."""
+    txt = open(path).read()
+    nxm, nym, nzm = (int(v) for v in txt.split("NSST(>1=RK,else AB)\n", 1)[1].split()[:3])
+    old = "MULTIRES    NXMR    NYMR    NZMR\n0           64     64     64"
+    assert txt.count(old) == 1 and txt.count("FLAGSAL     FLAGPF\n0           0") == 1
+    txt = txt.replace(old, f"MULTIRES    NXMR    NYMR    NZMR\n1           {2 * nxm}     {2 * nym}     {2 * nzm}")
+    txt = txt.replace("FLAGSAL     FLAGPF\n0           0", "FLAGSAL     FLAGPF\n1           0")
+    open(path, "w").write(txt)
+
+
 @pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
-def test_fortran_host_checkpoint_and_restart(tmp_path):
+@pytest.mark.parametrize("case", ["rbc", "multires"])
+def test_fortran_host_checkpoint_and_restart(tmp_path, case):
     """NREAD = 1 restarts from outputdir/continua_*.h5 (ReadFlowInterp.F90:143-171):
     5 steps, the checkpoint a normal end writes (QuitRoutine.F90:34-38), then
     5 more steps from it give the bits of 10 uninterrupted steps.  The run also
-    writes means.h5 every TOUT and the start-up snapshot of main.F90:241."""
+    writes means.h5 every TOUT and the start-up snapshot of main.F90:241.  The
+    multires case restarts S on the refined grid and reruns the start-up
+    transfers (main.F90:205-209)."""
     import pyafid
-    nxm, nym, nzm, dt = 24, 16, 16, 2e-3
+    nxm, nym, nzm, dt = (24, 16, 16, 2e-3) if case == "rbc" else (16, 12, 12, 2e-3)
     one, two = tmp_path / "one", tmp_path / "two"
     os.makedirs(one)
     os.makedirs(two)
     write_bou(os.path.join(one, "bou.in"), nxm, nym, nzm, 9, dt)
+    if case == "multires":
+        with_salinity(os.path.join(one, "bou.in"))
     r10 = run_host(one, "fused")
     assert int(r10["steps"]) == 10 and int(r10["errorcode"]) == 555
     write_bou(os.path.join(two, "bou.in"), nxm, nym, nzm, 4, dt)
+    if case == "multires":
+        with_salinity(os.path.join(two, "bou.in"))
     r5 = run_host(two, "fused")
     assert int(r5["steps"]) == 5
     out = two / "outputdir"
@@ -105,3 +123,10 @@ def test_fortran_host_checkpoint_and_restart(tmp_path):
     a = pyafid.h5_read_field(one / "outputdir" / "continua_temp.h5", (nzm, nym, nxm + 1))
     b = pyafid.h5_read_field(out / "continua_temp.h5", (nzm, nym, nxm + 1))
     assert np.array_equal(a, b)
+    if case == "multires":
+        shape = (2 * nzm, 2 * nym, 2 * nxm + 1)
+        a = pyafid.h5_read_field(one / "outputdir" / "continua_sal.h5", shape)
+        b = pyafid.h5_read_field(out / "continua_sal.h5", shape)
+        assert np.isfinite(a).all() and np.array_equal(a, b)
+        assert pyafid.h5_read_int(out / "continua_master.h5", "nxr") == 2 * nxm + 1
+        assert pyafid.h5_dims(out / "means.h5", "Sbar/00000") == (2 * nxm,)
