@@ -479,6 +479,7 @@ class Solver:
         and, when final, outdir/continua_master.h5 with the grid and the time.
         Not final: outdir/fields/<nint(time/save_3D) as 5 digits>_<field>.h5,
         without pr and master.  Returns the list of files written."""
+        os.makedirs(outdir, exist_ok=True)      # main.F90:77 creates outputdir at start-up
         if final:
             base = os.path.join(outdir, "continua")
         else:
@@ -546,6 +547,7 @@ class Solver:
         (phasefield.F90:786-793): outdir/means.h5 gets <profile>/<nint(time/tout)
         as 5 digits> for every profile; the file and its groups are created on
         the first call.  Returns the profiles written."""
+        os.makedirs(outdir, exist_ok=True)
         path = os.path.join(outdir, "means.h5")
         c = self.cfgd
         groups = list(MEAN_PROFILES)

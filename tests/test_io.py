@@ -234,6 +234,14 @@ def test_read_flow_field_str3_tolerance(tmp_path):
     assert np.array_equal(b.arrays["temp"], a.arrays["temp"])
 
 
+def test_write_flow_field_creates_the_output_folder(tmp_path):
+    a = filled(MR, 1)
+    a.write_flow_field(tmp_path / "outputdir", time=0.5)
+    a.write_means(tmp_path / "other", time=0.5, tout=0.5)
+    assert (tmp_path / "outputdir" / "continua_master.h5").exists()
+    assert (tmp_path / "other" / "means.h5").exists()
+
+
 def test_write_flow_field_snapshot_names(tmp_path):
     a = filled(dict(MR, salinity=0, phasefield=1), 1)
     files = a.write_flow_field(tmp_path, time=0.3, final=False, save_3D=0.1)
