@@ -246,6 +246,7 @@ def kernel_bytes(cfg, form, sweeps=1, checks=0):
     h = 8 if form == "fp64" else 4                            # explicit terms, refined fields
     c = 16 if form == "fp64" else 8                           # complex spectrum
     r = 8 if form == "fp64" else 4                            # real factor
+    q = 8 if form == "fp64" else 4                            # pressure correction dph of the Poisson sweeps
     b = {
         # explicit terms + RHS + x-solve of vx, vy, vz, T in one sweep:
         # read u, v, w, T, p and the 4 old explicit terms, write u, v, w, T and the 4 new terms
@@ -253,8 +254,8 @@ def kernel_bytes(cfg, form, sweeps=1, checks=0):
         "column_base": 9 * d * nb + 8 * h * nb + (2 * d * nb if pf else 0),
         # salinity: read S, vxr, vyr, vzr, old H_S; write S, H_S
         "column_sal": 7 * h * nr,
-        # read phi, u,v,w,p; write u,v,w,p
-        "project": 9 * d * nb,
+        # read dph, u,v,w,p; write u,v,w,p
+        "project": 8 * d * nb + q * nb,
         "mgrd_vxr": h * nr + d * nb,
         "mgrd_vyr": h * nr + d * nb,
         "mgrd_vzr": h * nr + d * nb,
@@ -274,7 +275,7 @@ def kernel_bytes(cfg, form, sweeps=1, checks=0):
     b["poisson_fwd_y"] = (3 * d * nb + c * ns) + checks * (4 * d * nb + c * ns)
     b["poisson_fft_z"] = sweeps * 2 * 2 * c * ns
     b["poisson_tridiag"] = sweeps * (2 * c * ns + 3 * r * nf)
-    b["poisson_inv_y"] = (c * ns + d * nb) + (sweeps - 1) * (c * ns + 2 * d * nb)
+    b["poisson_inv_y"] = (c * ns + q * nb) + (sweeps - 1) * (c * ns + 2 * q * nb)
     return b
 
 
