@@ -45,21 +45,38 @@ struct orc_state {
     double *ak1, *ak2, *amphk, *acphk, *apphk;
     double *xcr, *xmr, *ycr, *ymr, *zcr, *zmr, *g3rcr, *g3rmr, *d3xcr, *d3xmr, *udx3cr, *udx3mr;
     double *ap3sskr, *ac3sskr, *am3sskr;
-    int *irangs, *jrangs, *krangs, *irangb;
+    int *irangs, *jrangs, *krangs, *irangb, *jrangb, *krangb;
     double *cxrs, *cyrs, *czrs, *cxsalc, *cysalc, *czsalc;   /* (4, n) column major */
     /* phase field (phasefield.F90:17-29, ReadInputFile.F90:125-126) */
     double pf_a, pf_s, pf_tm, pf_d, pf_eps;
     double *ap3spkr, *ac3spkr, *am3spkr;                     /* (1:nxr), Neumann (CreateMgrdGrid.F90:163) */
     double *cxphic, *cyphic, *czphic;                        /* refined -> coarse centres, (4, n) */
+    int *irangr, *jrangr, *krangr;                           /* refined -> coarse centre indices (0:n_ref+1) */
     /* 3-D fields (x fastest, no halo) */
     double *f[ORC_NFIELDS];
     double *tpdv, *tpdvr;   /* x index -1..nx+1 / -1..nxr+1 */
+    /* immersed boundary (src/ibm/ibm_param.F90): per solve (vx, vy, vz, T,
+     * S) the mask (2 liquid, 0 solid, 1 / -1 upper / lower boundary point;
+     * x fastest) and the boundary distances in the reference's loop order */
+    int32_t *ibm_mask[5];
+    double *ibm_dist[5];
+    int64_t ibm_ndist[5];
     void *allocs[MAXALLOC];
     int nalloc;
 };
 
 static int g_threads = 0;
 void orc_set_threads(int n) { g_threads = n; }
+
+/* Planted mistakes, for the tests that show each pin catches a plausible
+ * error (0 = none, the faithful restatement): 1 flips the sign of the
+ * pressure gradient in the vx RHS; 2 drops the alpha*beta term of
+ * CorrectPressure; 3 drops the 1/2 of the vx buoyancy average; 4 leaves vy
+ * out of the kinetic energy; 5 takes vx at the lower face instead of the
+ * cell-centre average in the Nusselt sum; 6 drops alpha from the x diffusion
+ * of the vz RHS. */
+static int g_plant = 0;
+void orc_set_plant(int n) { g_plant = n; }
 static void apply_threads(void) {
 #ifdef _OPENMP
     if (g_threads > 0) omp_set_num_threads(g_threads);
@@ -465,6 +482,7 @@ orc_state *orc_create(const orc_config *cfg) {
 
         /* CreateMgrdStencil.F90:17-78 (the stencils the substep uses) */
         s->irangs = i1(s, 0, nx); s->jrangs = i1(s, 0, ny); s->krangs = i1(s, 0, nz); s->irangb = i1(s, 0, nx + 1);
+        s->jrangb = i1(s, 0, ny); s->krangb = i1(s, 0, nz);
         s->cxrs = track(s, sizeof(double) * 4 * nxmr); s->cyrs = track(s, sizeof(double) * 4 * nymr);
         s->czrs = track(s, sizeof(double) * 4 * nzmr);
         s->cxsalc = track(s, sizeof(double) * 4 * nx); s->cysalc = track(s, sizeof(double) * 4 * nym);
@@ -476,26 +494,43 @@ orc_state *orc_create(const orc_config *cfg) {
         interpolation_indices(s->krangs, s->zm, nzm, s->zmr, nzmr, cfg->zlen);
         construct_stencil(s->czrs, s->zm, nzm, s->zmr, nzmr, cfg->zlen, s->krangs, 'z');
         {
-            int *irangr = malloc(sizeof(int) * (nxr + 1)), *jrangr = malloc(sizeof(int) * (nyr + 1)), *krangr = malloc(sizeof(int) * (nzr + 1));
+            int *irangr = i1(s, 0, nxr), *jrangr = i1(s, 0, nyr), *krangr = i1(s, 0, nzr);
+            s->irangr = irangr; s->jrangr = jrangr; s->krangr = krangr;
             double *cxphic = track(s, sizeof(double) * 4 * nxm);
-            s->cxphic = cxphic; s->cyphic = s->cysalc; s->czphic = s->czsalc;
+            s->cxphic = cxphic;
+            s->cyphic = track(s, sizeof(double) * 4 * nym); s->czphic = track(s, sizeof(double) * 4 * nzm);
             interpolation_indices(irangr, s->xmr, nxmr, s->xm, nxm, cfg->alx3);
             construct_stencil(cxphic, s->xmr, nxmr, s->xm, nxm, cfg->alx3, irangr, 'x');
             interpolation_indices(jrangr, s->ymr, nymr, s->ym, nym, cfg->ylen);
-            construct_stencil(s->cysalc, s->ymr, nymr, s->ym, nym, cfg->ylen, jrangr, 'y');
+            construct_stencil(s->cyphic, s->ymr, nymr, s->ym, nym, cfg->ylen, jrangr, 'y');
             interpolation_indices(krangr, s->zmr, nzmr, s->zm, nzm, cfg->zlen);
-            construct_stencil(s->czsalc, s->zmr, nzmr, s->zm, nzm, cfg->zlen, krangr, 'z');
+            construct_stencil(s->czphic, s->zmr, nzmr, s->zm, nzm, cfg->zlen, krangr, 'z');
+            /* S to the coarse grid: faces along gAxis (CreateMgrdStencil.F90:25-32, :47-54, :69-76) */
+            for (int k = 0; k <= nx; k++) s->irangb[k] = s->irangs[k];
+            for (int j = 0; j <= ny; j++) s->jrangb[j] = s->jrangs[j];
+            for (int k = 0; k <= nz; k++) s->krangb[k] = s->krangs[k];
+            memcpy(s->cxsalc, cxphic, sizeof(double) * 4 * nxm);
+            memcpy(s->cysalc, s->cyphic, sizeof(double) * 4 * nym);
+            memcpy(s->czsalc, s->czphic, sizeof(double) * 4 * nzm);
             if (cfg->gaxis == 1) {
                 int *irange = malloc(sizeof(int) * (nxr + 1));
                 interpolation_indices(irange, s->xmr, nxmr, s->xc, nx, cfg->alx3);
                 construct_stencil(s->cxsalc, s->xmr, nxmr, s->xc, nx, cfg->alx3, irange, 'x');
                 interpolation_indices(s->irangb, s->xc, nx, s->xmr, nxmr, cfg->alx3);
                 free(irange);
-            } else {
-                for (int k = 0; k <= nx; k++) s->irangb[k] = s->irangs[k];
-                memcpy(s->cxsalc, cxphic, sizeof(double) * 4 * nxm);
+            } else if (cfg->gaxis == 2) {
+                int *jrange = malloc(sizeof(int) * (nyr + 1));
+                interpolation_indices(jrange, s->ymr, nymr, s->yc, nym, cfg->ylen);
+                construct_stencil(s->cysalc, s->ymr, nymr, s->yc, nym, cfg->ylen, jrange, 'y');
+                interpolation_indices(s->jrangb, s->yc, nym, s->ymr, nymr, cfg->ylen);
+                free(jrange);
+            } else if (cfg->gaxis == 3) {
+                int *krange = malloc(sizeof(int) * (nzr + 1));
+                interpolation_indices(krange, s->zmr, nzmr, s->zc, nzm, cfg->zlen);
+                construct_stencil(s->czsalc, s->zmr, nzmr, s->zc, nzm, cfg->zlen, krange, 'z');
+                interpolation_indices(s->krangb, s->zc, nzm, s->zmr, nzmr, cfg->zlen);
+                free(krange);
             }
-            free(irangr); free(jrangr); free(krangr);
         }
     }
     return s;
@@ -719,7 +754,7 @@ void orc_explicit_terms_vx(orc_state *s) {
             }
             if (s->c.gaxis == 1)
                 for (int kc = 2; kc <= nxm; kc++) {
-                    double tempit = s->c.active_t * 0.5 * (temp[B(kc, j0, i0)] + temp[B(kc - 1, j0, i0)]);
+                    double tempit = s->c.active_t * (g_plant == 3 ? 1.0 : 0.5) * (temp[B(kc, j0, i0)] + temp[B(kc - 1, j0, i0)]);
                     qcap[B(kc, j0, i0)] += s->byct * tempit;
                 }
         }
@@ -861,6 +896,76 @@ static void solve_columns(orc_state *s, double *q, double *rhs, int lx, int ny, 
     free(a); free(b); free(c); free(sc);
 }
 
+/* SolveImpEqnUpdate_X_ibm (src/ibm/SolveImpEqnUpdate_X_ibm.F90:12-86),
+ * _YZ_ibm (:12-75), _Temp_ibm (:12-100) and _Sal_ibm (:12-76): one
+ * tridiagonal per column whose rows follow the mask: liquid rows are the
+ * normalised rows of the plain solve; solid rows the identity with rhs -q
+ * (+ Tb for T, Tb = 1); upper / lower boundary points the rows (1, -d) /
+ * (-d, 1) with rhs d q(k+1) - q(k) / d q(k-1) - q(k) (+ (1-d) Tb for T),
+ * d taken from the list in loop order (ic, jc, kc).  S treats solid rows as
+ * liquid and takes d = 1.  vx: rows 1 and nx are the identity with rhs -vx,
+ * and the update covers 2..nxm.  slot: 0 vx, 1 vy, 2 vz, 3 T, 4 S. */
+static void solve_columns_ibm(orc_state *s, int slot, double *q, const double *rhs, int lx, int ny, int nzz, int n,
+                              const double *am, const double *ac, const double *ap, double betadx) {
+    const int32_t *mask = s->ibm_mask[slot];
+    const double *dist = s->ibm_dist[slot];
+    const double tb = 1.0;
+    double *a = malloc(sizeof(double) * (lx + 1)), *b = malloc(sizeof(double) * (lx + 1));
+    double *c = malloc(sizeof(double) * (lx + 1)), *f = malloc(sizeof(double) * (lx + 1));
+    int64_t nd = 0;
+    const int nsolve = slot == 0 ? lx : n;          /* vx: nx rows; others nxm (nxmr) */
+    for (int ic = 0; ic < nzz; ic++)
+        for (int jc = 0; jc < ny; jc++) {
+            size_t base = ((size_t)ic * ny + jc) * lx;
+            for (int k = 0; k < nsolve; k++) { a[k] = 0.0; b[k] = 1.0; c[k] = 0.0; f[k] = 0.0; }
+            for (int kc = 1; kc <= n; kc++) {
+                int km = kc - 1 < 1 ? 1 : kc - 1;
+                int kp = slot == 0 ? kc + 1 : (kc + 1 > n ? n : kc + 1);
+                int m = mask[base + kc - 1];
+                double qk = q[base + kc - 1];
+                if (slot == 4 ? (m % 2 == 0) : (m == 2)) {
+                    double ackl_b = 1.0 / (1.0 - ac[kc] * betadx);
+                    a[kc - 1] = -am[kc] * betadx * ackl_b;
+                    c[kc - 1] = -ap[kc] * betadx * ackl_b;
+                    f[kc - 1] = rhs[base + kc - 1] * ackl_b;
+                } else if (m == 0) {
+                    f[kc - 1] = -qk + (slot == 3 ? tb : 0.0);
+                } else if (m == 1 || m == -1) {
+                    double d = slot == 4 ? 1.0 : dist[nd];
+                    if (slot != 4 && nd >= s->ibm_ndist[slot]) d = 0.0;      /* list too short: caught by orc_set_ibm */
+                    double qn = q[base + (m == 1 ? kp : km) - 1];
+                    if (m == 1) c[kc - 1] = -d; else a[kc - 1] = -d;
+                    f[kc - 1] = d * qn - qk + (slot == 3 ? (1.0 - d) * tb : 0.0);
+                    nd++;
+                }
+            }
+            if (slot == 0) {          /* identity wall rows, SolveImpEqnUpdate_X_ibm.F90:66-70 */
+                a[0] = 0.0; b[0] = 1.0; c[0] = 0.0; f[0] = -q[base];
+                a[lx - 1] = 0.0; c[lx - 1] = 0.0; f[lx - 1] = -q[base + lx - 1];
+            }
+            a[0] = 0.0; c[nsolve - 1] = 0.0;     /* dgttrf takes the sub- / super-diagonals of rows 2.. / ..n-1 */
+            orc_tridiag_solve(nsolve, a, b, c, f);
+            int u0 = slot == 0 ? 2 : 1, u1 = n;
+            for (int kc = u0; kc <= u1; kc++) q[base + kc - 1] += f[kc - 1];
+        }
+    free(a); free(b); free(c); free(f);
+}
+
+int orc_set_ibm(orc_state *s, int field, const int32_t *mask, const double *dist, int64_t ndist) {
+    int slot = field == ORC_VX ? 0 : field == ORC_VY ? 1 : field == ORC_VZ ? 2 : field == ORC_TEMP ? 3 : field == ORC_SAL ? 4 : -1;
+    if (slot < 0 || (slot == 4 && !s->c.salinity)) return 1;
+    size_t n = slot == 4 ? nref(s) : nbase(s);
+    int64_t nb = 0;
+    for (size_t q = 0; q < n; q++) if (mask[q] == 1 || mask[q] == -1) nb++;
+    if (slot != 4 && ndist < nb) return 2;
+    s->ibm_mask[slot] = track(s, sizeof(int32_t) * n);
+    memcpy(s->ibm_mask[slot], mask, sizeof(int32_t) * n);
+    s->ibm_dist[slot] = track(s, sizeof(double) * (size_t)(ndist > 0 ? ndist : 1));
+    if (ndist > 0) memcpy(s->ibm_dist[slot], dist, sizeof(double) * (size_t)ndist);
+    s->ibm_ndist[slot] = ndist;
+    return 0;
+}
+
 /* SolveImpEqnUpdate_X_pf (phasefield.F90:821-872) and SolveImpEqnUpdate_YZ_pf
  * (:875-930): one tridiagonal per column, in which a row whose interpolated
  * phase field exceeds pf_direct_force = 0.9 (phasefield.F90:29) becomes the
@@ -930,12 +1035,14 @@ void orc_implicit_and_update_vx(orc_state *s) {
                 double udx3 = s->al * s->udx3c[kc];
                 double dxxvx = vx[B(kp, j0, i0)] * s->ap3ck[kc] + vx[B(kc, j0, i0)] * s->ac3ck[kc] + vx[B(km, j0, i0)] * s->am3ck[kc];
                 double dxp = (pr[B(kc, j0, i0)] - pr[B(km, j0, i0)]) * udx3;
+                if (g_plant == 1) dxp = -dxp;
                 rhs[B(kc, j0, i0)] = (s->ga * qcap[B(kc, j0, i0)] + s->ro * rux[B(kc, j0, i0)] + alre * dxxvx - dxp) * s->dt;
                 rux[B(kc, j0, i0)] = qcap[B(kc, j0, i0)];
             }
         }
     }
-    if (s->c.phasefield) solve_columns_pf(s, vx, rhs, 'x', s->am3ck, s->ac3ck, s->ap3ck, s->beta * s->al);
+    if (s->ibm_mask[0]) solve_columns_ibm(s, 0, vx, rhs, nx, nym, nzm, nxm, s->am3ck, s->ac3ck, s->ap3ck, s->beta * s->al);
+    else if (s->c.phasefield) solve_columns_pf(s, vx, rhs, 'x', s->am3ck, s->ac3ck, s->ap3ck, s->beta * s->al);
     else solve_columns(s, vx, rhs, nx, nym, nzm, nx, s->am3ck, s->ac3ck, s->ap3ck, s->beta * s->al, 2, nxm, 2, nxm);
     for (size_t c = 0; c < (size_t)nym * nzm; c++) { vx[c * nx + 0] = 0.0; vx[c * nx + nx - 1] = 0.0; }
 }
@@ -967,7 +1074,8 @@ void orc_implicit_and_update_vy(orc_state *s) {
             }
         }
     }
-    if (s->c.phasefield) solve_columns_pf(s, vy, rhs, 'y', s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al);
+    if (s->ibm_mask[1]) solve_columns_ibm(s, 1, vy, rhs, nx, nym, nzm, nxm, s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al);
+    else if (s->c.phasefield) solve_columns_pf(s, vy, rhs, 'y', s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al);
     else solve_columns(s, vy, rhs, nx, nym, nzm, nxm, s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al, 1, nxm, 1, nxm);
 }
 
@@ -976,7 +1084,7 @@ void orc_implicit_and_update_vz(orc_state *s) {
     BASE_DECL
     double *vz = s->f[ORC_VZ], *rhs = s->f[ORC_RHS], *ruz = s->f[ORC_RUZ];
     const double *dq = s->f[ORC_DQ], *pr = s->f[ORC_PR];
-    double alre = s->al / s->ren;
+    double alre = (g_plant == 6 ? 1.0 : s->al) / s->ren;
     apply_threads();
 #pragma omp parallel for
     for (int ic = 1; ic <= nzm; ic++) {
@@ -992,7 +1100,8 @@ void orc_implicit_and_update_vz(orc_state *s) {
             }
         }
     }
-    if (s->c.phasefield) solve_columns_pf(s, vz, rhs, 'z', s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al);
+    if (s->ibm_mask[2]) solve_columns_ibm(s, 2, vz, rhs, nx, nym, nzm, nxm, s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al);
+    else if (s->c.phasefield) solve_columns_pf(s, vz, rhs, 'z', s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al);
     else solve_columns(s, vz, rhs, nx, nym, nzm, nxm, s->am3sk, s->ac3sk, s->ap3sk, s->beta * s->al, 1, nxm, 1, nxm);
 }
 
@@ -1024,7 +1133,10 @@ void orc_implicit_and_update_temp(orc_state *s) {
             }
         }
     }
-    solve_columns(s, temp, rhs, nx, nym, nzm, nxm, s->am3ssk, s->ac3ssk, s->ap3ssk, 0.5 * s->al * s->dt / s->pect, 1, nxm, 1, nxm);
+    if (s->ibm_mask[3] && !s->c.phasefield)      /* ImplicitAndUpdateTemp.F90:69-70 */
+        solve_columns_ibm(s, 3, temp, rhs, nx, nym, nzm, nxm, s->am3ssk, s->ac3ssk, s->ap3ssk, 0.5 * s->al * s->dt / s->pect);
+    else
+        solve_columns(s, temp, rhs, nx, nym, nzm, nxm, s->am3ssk, s->ac3ssk, s->ap3ssk, 0.5 * s->al * s->dt / s->pect, 1, nxm, 1, nxm);
 }
 
 /* CalcLocalDivergence, pressure.F90:141-162 */
@@ -1162,7 +1274,7 @@ void orc_correct_pressure(orc_state *s) {
     BASE_DECL
     double *pr = s->f[ORC_PR];
     const double *d = s->f[ORC_DPH];
-    double be = s->al * s->beta;
+    double be = g_plant == 2 ? 0.0 : s->al * s->beta;
     apply_threads();
 #pragma omp parallel for
     for (int ic = 1; ic <= nzm; ic++) {
@@ -1529,34 +1641,88 @@ void orc_implicit_salinity(orc_state *s) {
             }
         }
     }
-    solve_columns(s, sal, rhsr, nxr, nymr, nzmr, nxmr, s->am3sskr, s->ac3sskr, s->ap3sskr,
+    if (s->ibm_mask[4])            /* salinity.F90:403-404 */
+        solve_columns_ibm(s, 4, sal, rhsr, nxr, nymr, nzmr, nxmr, s->am3sskr, s->ac3sskr, s->ap3sskr, 0.5 * s->al * s->dt / s->pecs);
+    else
+        solve_columns(s, sal, rhsr, nxr, nymr, nzmr, nxmr, s->am3sskr, s->ac3sskr, s->ap3sskr,
                   0.5 * s->al * s->dt / s->pecs, 1, nxmr, 1, nxmr);
 }
 
 /* InterpSalMultigrid (salinity.F90:458-491) via interpolate_xyz_to_coarse_fast
  * (HermiteInterpolations.F90:256-312).  Only the fast path is restated: it
  * is taken whenever the refined grid brackets the coarse one. */
+/* interpolate_xyz_to_coarse, HermiteInterpolations.F90:218-250: the general
+ * refined -> coarse loop, taken when the refined cell centres do not bracket
+ * the coarse ones (xmr(1) >= xm(1) or xmr(nxmr) <= xm(nxm)).  Every refined
+ * cell kcr = 0..nxmr (y, z: 0..n, the periodic halo cell 0 included) spreads
+ * its 4x4x4 block tpdvr(kcr-1:kcr+2, ...) to the coarse points it owns
+ * (irangr(kcr) <= kc < irangr(kcr+1)) with the cxphic / cyphic / czphic
+ * weights.  tpdvr holds the x ghosts at 0 and nxr, zero beyond them.
+ * out = v (acc = 0) or out += scale * v (acc = 1), kc = 1..nxm. */
+static void to_coarse_general(orc_state *s, const double *tpdvr, double *out, double scale, int acc) {
+    BASE_DECL REF_DECL
+    for (int icr = 0; icr <= nzmr; icr++) {
+        int ilo = s->krangr[icr] > 1 ? s->krangr[icr] : 1, ihi = s->krangr[icr + 1] - 1 < nzm ? s->krangr[icr + 1] - 1 : nzm;
+        for (int jcr = 0; jcr <= nymr; jcr++) {
+            int jlo = s->jrangr[jcr] > 1 ? s->jrangr[jcr] : 1, jhi = s->jrangr[jcr + 1] - 1 < nym ? s->jrangr[jcr + 1] - 1 : nym;
+            for (int kcr = 0; kcr <= nxmr; kcr++) {
+                int klo = s->irangr[kcr] > 1 ? s->irangr[kcr] : 1, khi = s->irangr[kcr + 1] - 1 < nxm ? s->irangr[kcr + 1] - 1 : nxm;
+                if (ilo > ihi || jlo > jhi || klo > khi) continue;
+                double qv3[4][4][4];
+                for (int c = 0; c < 4; c++)
+                    for (int b = 0; b < 4; b++)
+                        for (int a = 0; a < 4; a++) {
+                            int kk = kcr - 1 + a;
+                            qv3[c][b][a] = (kk < -1 || kk > nxr + 1) ? 0.0
+                                         : tpdvr[TR(kk, W(jcr - 1 + b, nymr), W(icr - 1 + c, nzmr))];
+                        }
+                for (int ic = ilo; ic <= ihi; ic++) {
+                    const double *cz = s->czphic + (size_t)(ic - 1) * 4;
+                    double qv2[4][4];
+                    for (int b = 0; b < 4; b++)
+                        for (int a = 0; a < 4; a++)
+                            qv2[b][a] = qv3[0][b][a] * cz[0] + qv3[1][b][a] * cz[1] + qv3[2][b][a] * cz[2] + qv3[3][b][a] * cz[3];
+                    for (int jc = jlo; jc <= jhi; jc++) {
+                        const double *cy = s->cyphic + (size_t)(jc - 1) * 4;
+                        double qv1[4];
+                        for (int a = 0; a < 4; a++) qv1[a] = qv2[0][a] * cy[0] + qv2[1][a] * cy[1] + qv2[2][a] * cy[2] + qv2[3][a] * cy[3];
+                        for (int kc = klo; kc <= khi; kc++) {
+                            const double *cx = s->cxphic + (size_t)(kc - 1) * 4;
+                            double v = qv1[0] * cx[0] + qv1[1] * cx[1] + qv1[2] * cx[2] + qv1[3] * cx[3];
+                            size_t o = B(kc, jc - 1, ic - 1);
+                            if (acc) out[o] += scale * v; else out[o] = v;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+static int brackets(orc_state *s) { return s->xmr[1] < s->xm[1] && s->xmr[s->nxmr] > s->xm[s->nxm]; }
+
 void orc_interp_sal_multigrid(orc_state *s) {
     BASE_DECL REF_DECL
     const double *sal = s->f[ORC_SAL], *sbp = s->f[ORC_SALBP], *stp = s->f[ORC_SALTP];
     double *salc = s->f[ORC_SALC], *tpdvr = s->tpdvr;
-    if (!(s->xmr[1] < s->xm[1] && s->xmr[nxmr] > s->xm[nxm])) {
-        fprintf(stderr, "oracle: InterpSalMultigrid general path not restated\n");
-        abort();
-    }
     memset(salc, 0, sizeof(double) * nbase(s));
+    memset(tpdvr, 0, sizeof(double) * (size_t)(nxr + 3) * nymr * nzmr);
     for (int i = 0; i < nzmr; i++)
         for (int j = 0; j < nymr; j++) {
             for (int k = 1; k <= nxmr; k++) tpdvr[TR(k, j, i)] = sal[R(k, j, i)];
             tpdvr[TR(0, j, i)] = s->c.sfixs == 1 ? 2.0 * sbp[RP2(j, i)] - sal[R(1, j, i)] : sal[R(1, j, i)];
             tpdvr[TR(nxr, j, i)] = s->c.sfixn == 1 ? 2.0 * stp[RP2(j, i)] - sal[R(nxmr, j, i)] : sal[R(nxmr, j, i)];
         }
+    if (!brackets(s)) {          /* salinity.F90:487-488 */
+        to_coarse_general(s, tpdvr, salc, 1.0, 0);
+        return;
+    }
     apply_threads();
 #pragma omp parallel for
     for (int ic = 1; ic <= nzm; ic++) {
-        int icr = s->krangs[ic];    /* krangb = krangs (CreateMgrdStencil.F90:76) */
+        int icr = s->krangb[ic];
         for (int jc = 1; jc <= nym; jc++) {
-            int jcr = s->jrangs[jc];
+            int jcr = s->jrangb[jc];
             for (int kc = 1; kc <= nxm; kc++) {
                 int kcr = s->irangb[kc];
                 double qv3[4][4][4];
@@ -1666,9 +1832,9 @@ void orc_implicit_phase(orc_state *s) {
  * out += scale * v (acc = 1) for kc = 1..nxm. */
 static void pf_to_coarse(orc_state *s, const double *tpdvr, double *out, double scale, int acc) {
     BASE_DECL REF_DECL
-    if (!(s->xmr[1] < s->xm[1] && s->xmr[nxmr] > s->xm[nxm])) {
-        fprintf(stderr, "oracle: refined-to-coarse general path not restated\n");
-        abort();
+    if (!brackets(s)) {          /* phasefield.F90:489-492 and :701-723 */
+        to_coarse_general(s, tpdvr, out, scale, acc);
+        return;
     }
     apply_threads();
 #pragma omp parallel for
@@ -1996,13 +2162,326 @@ void orc_global_quantities(orc_state *s, double out[4]) {
         for (int j = 0; j < nym; j++)
             for (int kc = 1; kc <= nxm; kc++) {
                 double fac2 = s->g3rc[kc];
-                double vxcen = (vx[B(kc, j, i)] + vx[B(kc + 1, j, i)]) * 0.5, t = temp[B(kc, j, i)];
+                double vxcen = g_plant == 5 ? vx[B(kc, j, i)] : (vx[B(kc, j, i)] + vx[B(kc + 1, j, i)]) * 0.5, t = temp[B(kc, j, i)];
                 anusin += (long double)(t * vxcen * fac2);
                 tempm += (long double)(t * fac2);
-                e += (long double)(fac2 * (vz[B(kc, j, i)] * vz[B(kc, j, i)] + vy[B(kc, j, i)] * vy[B(kc, j, i)] + vx[B(kc, j, i)] * vx[B(kc, j, i)]));
+                e += (long double)(fac2 * (vz[B(kc, j, i)] * vz[B(kc, j, i)] + (g_plant == 4 ? 0.0 : vy[B(kc, j, i)] * vy[B(kc, j, i)]) +
+                                           vx[B(kc, j, i)] * vx[B(kc, j, i)]));
             }
     out[0] = 1.0 + sqrt(s->c.prat * fabs(s->c.rayt)) * (double)anusin * vol;
     out[1] = 0.5 * (double)e * vol;
     out[2] = sqrt((double)e * vol) * sqrt(fabs(s->c.rayt) / s->c.prat);
     out[3] = (double)tempm * vol;
 }
+
+/* ------------------------------------------------------------------------ */
+/* restart onto a new grid: ReadFlowInterp.F90:90-141 and                   */
+/* src/multires/IC_interpolation/ (CreateOldGrid, CreateNewInputStencil,    */
+/* CreateNewSalStencil, InterpInputVel, InterpInputSal, InterpInputPhi)     */
+/* ------------------------------------------------------------------------ */
+/* CreateOldGrid.F90:11-115, one wall-normal grid of the old run: the options
+ * of CreateGrid with the old stretching str3o (option 3 with parameter 1.0
+ * for both grids).  The old base grid takes option 5 when the *current*
+ * istr3 is 5 (CreateOldGrid.F90:77); an old option 5 without it leaves the
+ * reference's grid unset, which is refused here.  c(1:Nm+1), m(1:Nm+1). */
+static int old_xgrid(orc_state *s, int istr3o, double str3o, double *c, double *m, int Nm, int refined) {
+    double L = s->c.alx3;
+    int opt = istr3o;
+    if (!refined && s->c.istr3 == 5) opt = 5;
+    else if (!refined && istr3o == 5) return -1;
+    switch (opt) {
+    case 0: uniform_grid(c, m, Nm, L); break;
+    case 1: centre_focus_grid(c, m, Nm, L, str3o); break;
+    case 2: natural_BL_grid(c, m, Nm, L); break;
+    case 3: sym_natural_BL_grid(c, m, Nm, L, 1.0); break;
+    case 4: tanh_grid(c, m, Nm, L, str3o); break;
+    case 5: scallop_grid(c, m, Nm, L, s->c.dpdy, 0.5); break;
+    case 6: cheb_grid(c, m, Nm, L, str3o, s->pi); break;
+    case 7: asym_cheb_grid(c, m, Nm, L, str3o, s->pi); break;
+    case 8: cosine_grid(c, m, Nm, L, s->pi); break;
+    default: return -1;
+    }
+    m[Nm + 1] = 2 * c[Nm + 1] - m[Nm];     /* xmo(nxo) = 2 xco(nxo) - xmo(nxmo) */
+    return 0;
+}
+
+/* old -> new tables along one axis: idx(0:n_old+1) and the weights cx(4, n_new) */
+typedef struct { int *idx; double *w; } axis_tab;
+static axis_tab make_axis(const double *xold, int n_old, const double *xnew, int n_new, double L, char axis) {
+    axis_tab t;
+    t.idx = malloc(sizeof(int) * (size_t)(n_old + 2));
+    t.w = malloc(sizeof(double) * 4 * (size_t)n_new);
+    interpolation_indices(t.idx, xold, n_old, xnew, n_new, L);
+    construct_stencil(t.w, xold, n_old, xnew, n_new, L, t.idx, axis);
+    return t;
+}
+static void free_axis(axis_tab *t) { free(t->idx); free(t->w); }
+
+/* interpolate_xyz_old_to_new(_ref), HermiteInterpolations.F90:314-380: every
+ * old cell (kc = 0..nxo-1, jc = 0..nyo, ic = 0..nzo with the periodic halo
+ * cell 0) spreads its 4x4x4 block ovar(kc-1:kc+2, ...) to the new centres it
+ * owns.  ovar: (nzo, nyo, nxo+3) with x index -1..nxo+1 (ghosts set by the
+ * caller); nvar: (nzn, nyn, ldn), x index 1..nxn at offset 0. */
+static void old_to_new(const double *ovar, int nxo_m, int nyo, int nzo, const axis_tab *tx, const axis_tab *ty,
+                       const axis_tab *tz, double *nvar, int nxn, int nyn, int nzn, int ldn) {
+    int lo = nxo_m + 4;          /* x index -1..nxo+1 with nxo = nxo_m + 1 */
+#define OV(k, j, i) ovar[((size_t)W((i), nzo) * nyo + W((j), nyo)) * lo + (k) + 1]
+    for (int ic = 0; ic <= nzo; ic++) {
+        int ilo = tz->idx[ic] > 1 ? tz->idx[ic] : 1, ihi = tz->idx[ic + 1] - 1 < nzn ? tz->idx[ic + 1] - 1 : nzn;
+        for (int jc = 0; jc <= nyo; jc++) {
+            int jlo = ty->idx[jc] > 1 ? ty->idx[jc] : 1, jhi = ty->idx[jc + 1] - 1 < nyn ? ty->idx[jc + 1] - 1 : nyn;
+            for (int kc = 0; kc <= nxo_m; kc++) {
+                int klo = tx->idx[kc] > 1 ? tx->idx[kc] : 1, khi = tx->idx[kc + 1] - 1 < nxn ? tx->idx[kc + 1] - 1 : nxn;
+                if (ilo > ihi || jlo > jhi || klo > khi) continue;
+                double qv3[4][4][4];
+                for (int c = 0; c < 4; c++)
+                    for (int b = 0; b < 4; b++)
+                        for (int a = 0; a < 4; a++) qv3[c][b][a] = OV(kc - 1 + a, jc - 1 + b, ic - 1 + c);
+                for (int icr = ilo; icr <= ihi; icr++) {
+                    const double *cz = tz->w + (size_t)(icr - 1) * 4;
+                    double qv2[4][4];
+                    for (int b = 0; b < 4; b++)
+                        for (int a = 0; a < 4; a++)
+                            qv2[b][a] = qv3[0][b][a] * cz[0] + qv3[1][b][a] * cz[1] + qv3[2][b][a] * cz[2] + qv3[3][b][a] * cz[3];
+                    for (int jcr = jlo; jcr <= jhi; jcr++) {
+                        const double *cy = ty->w + (size_t)(jcr - 1) * 4;
+                        double qv1[4];
+                        for (int a = 0; a < 4; a++) qv1[a] = qv2[0][a] * cy[0] + qv2[1][a] * cy[1] + qv2[2][a] * cy[2] + qv2[3][a] * cy[3];
+                        for (int kcr = klo; kcr <= khi; kcr++) {
+                            const double *cx = tx->w + (size_t)(kcr - 1) * 4;
+                            nvar[((size_t)(icr - 1) * nyn + (jcr - 1)) * ldn + kcr - 1] =
+                                qv1[0] * cx[0] + qv1[1] * cx[1] + qv1[2] * cx[2] + qv1[3] * cx[3];
+                        }
+                    }
+                }
+            }
+        }
+    }
+#undef OV
+}
+
+/* the old grids (CreateOldGrid) and the old -> new cell-centre tables of the
+ * base grid (CreateNewInputStencil.F90) or of the refined grid
+ * (CreateNewSalStencil.F90) */
+typedef struct {
+    int nxm, nym, nzm;                 /* old cell counts */
+    double *xc, *xm, *yc, *ym, *zm;    /* old grid, 1-based (xm to nxm+1; ym, zm 0..n+1) */
+    axis_tab tx, ty, tz;               /* old centres -> new centres */
+} old_grid;
+
+static void free_old(old_grid *g) {
+    free(g->xc); free(g->xm); free(g->yc); free(g->ym); free(g->zm);
+    free_axis(&g->tx); free_axis(&g->ty); free_axis(&g->tz);
+}
+
+static int make_old(orc_state *s, const orc_old_grid *og, int refined, old_grid *g) {
+    int nx = refined ? og->nxro : og->nxo, ny = refined ? og->nyro : og->nyo, nz = refined ? og->nzro : og->nzo;
+    g->nxm = nx - 1; g->nym = ny - 1; g->nzm = nz - 1;
+    if (g->nxm < 2 || g->nym < 1 || g->nzm < 1) return -1;
+    g->xc = calloc((size_t)nx + 2, sizeof(double)); g->xm = calloc((size_t)nx + 2, sizeof(double));
+    g->yc = calloc((size_t)ny + 2, sizeof(double)); g->ym = calloc((size_t)ny + 2, sizeof(double));
+    g->zm = calloc((size_t)nz + 2, sizeof(double));
+    double *zc = calloc((size_t)nz + 2, sizeof(double));
+    if (old_xgrid(s, refined ? og->istr3ro : og->istr3o, og->str3o, g->xc, g->xm, g->nxm, refined)) { free(zc); return -2; }
+    uniform_grid(g->yc, g->ym, g->nym, s->c.ylen);
+    uniform_grid(zc, g->zm, g->nzm, s->c.zlen);
+    free(zc);
+    const double *xn = refined ? s->xmr : s->xm, *yn = refined ? s->ymr : s->ym, *zn = refined ? s->zmr : s->zm;
+    int nxn = refined ? s->nxmr : s->nxm, nyn = refined ? s->nymr : s->nym, nzn = refined ? s->nzmr : s->nzm;
+    g->tx = make_axis(g->xm, g->nxm, xn, nxn, s->c.alx3, 'x');
+    g->ty = make_axis(g->ym, g->nym, yn, nyn, s->c.ylen, 'y');
+    g->tz = make_axis(g->zm, g->nzm, zn, nzn, s->c.zlen, 'z');
+    return 0;
+}
+
+/* InterpInputVel.F90:1-350.  vxo, vyo, vzo, tempo: the old fields as the
+ * checkpoint holds them, (nzo-1, nyo-1, nxo) x fastest.  The gradients of
+ * the old velocity are interpolated to the new cell centres and integrated
+ * (vx from the wall, vy / vz from a first-guess plane j = 1 / i = 1 made by
+ * 2-D Hermite interpolation of the old plane); T is interpolated with its
+ * wall ghosts (:280-330).  Returns 0, or nonzero for an old grid option the
+ * reference cannot build. */
+int orc_interp_input_vel(orc_state *s, const orc_old_grid *og, const double *vxo, const double *vyo,
+                         const double *vzo, const double *tempo) {
+    BASE_DECL
+    old_grid g;
+    if (make_old(s, og, 0, &g)) return 1;
+    int nxmo = g.nxm, nymo = g.nym, nzmo = g.nzm, nxo = nxmo + 1, lo = nxmo + 4;
+    int sS = 1 - 2 * s->c.inslws, sN = 1 - 2 * s->c.inslwn;
+    double *ov = calloc((size_t)lo * nymo * nzmo, sizeof(double));    /* x -1..nxo+1 */
+    double *tp = calloc((size_t)nx * nym * nzm, sizeof(double));      /* new centres, x 1..nxm */
+#define O3(a, k, j, i) (a)[((size_t)(i) * nymo + (j)) * nxo + (k) - 1]
+#define OE(k, j, i) ov[((size_t)(i) * nymo + (j)) * lo + (k) + 1]
+    double *vx = s->f[ORC_VX], *vy = s->f[ORC_VY], *vz = s->f[ORC_VZ], *temp = s->f[ORC_TEMP];
+    memset(vx, 0, sizeof(double) * nbase(s)); memset(vy, 0, sizeof(double) * nbase(s)); memset(vz, 0, sizeof(double) * nbase(s));
+    double dyo = 1.0 / (g.yc[2] - g.yc[1]), dzo = (double)nzmo / s->c.zlen;   /* dzo = 1/(zco(2)-zco(1)) */
+
+    /* vx: dvx/dx at the old centres (:49-64), interpolated (:68), integrated from the wall (:71-79) */
+    memset(ov, 0, sizeof(double) * (size_t)lo * nymo * nzmo);
+    for (int i = 0; i < nzmo; i++)
+        for (int j = 0; j < nymo; j++) {
+            for (int kc = 1; kc <= nxmo; kc++)
+                OE(kc, j, i) = (O3(vxo, kc + 1, j, i) - O3(vxo, kc, j, i)) / (g.xc[kc + 1] - g.xc[kc]);
+            OE(0, j, i) = sS * OE(1, j, i);
+            OE(nxo, j, i) = sN * OE(nxmo, j, i);
+        }
+    old_to_new(ov, nxmo, nymo, nzmo, &g.tx, &g.ty, &g.tz, tp, nxm, nym, nzm, nx);
+    for (int i = 0; i < nzm; i++)
+        for (int j = 0; j < nym; j++) {
+            vx[B(1, j, i)] = 0.0;
+            for (int kc = 1; kc <= nxm; kc++)
+                vx[B(kc + 1, j, i)] = vx[B(kc, j, i)] + tp[((size_t)i * nym + j) * nx + kc - 1] * (s->xc[kc + 1] - s->xc[kc]);
+        }
+
+    /* vy: dvy/dy (:104-118), interpolated (:122), first-guess plane at the
+     * first y face from the old plane jc0 = 1 (:125-156), integrated along y
+     * (:161-168; the value carried past nym lands in the halo) */
+    memset(ov, 0, sizeof(double) * (size_t)lo * nymo * nzmo);
+    for (int i = 0; i < nzmo; i++)
+        for (int j = 0; j < nymo; j++) {
+            int jp = (j + 1) % nymo;
+            for (int kc = 1; kc <= nxmo; kc++) OE(kc, j, i) = (O3(vyo, kc, jp, i) - O3(vyo, kc, j, i)) * dyo;
+            OE(0, j, i) = sS * OE(1, j, i);
+            OE(nxo, j, i) = sN * OE(nxmo, j, i);
+        }
+    old_to_new(ov, nxmo, nymo, nzmo, &g.tx, &g.ty, &g.tz, tp, nxm, nym, nzm, nx);
+    {
+        /* vyxzc(-1:nxmo+2, z) with the wall ghosts (:132-141) */
+        double *pl = calloc((size_t)lo * nzmo, sizeof(double));
+#define PL(k, i0) pl[(size_t)W((i0), nzmo) * lo + (k) + 1]
+        for (int i = 1; i <= nzmo; i++) {
+            for (int kc = 1; kc <= nxmo; kc++) PL(kc, i) = O3(vyo, kc, 0, i - 1);
+            PL(0, i) = sS * PL(1, i);
+            PL(nxo, i) = sN * PL(nxmo, i);
+        }
+        for (int ic = 0; ic <= nzmo; ic++) {
+            int ilo = g.tz.idx[ic] > 1 ? g.tz.idx[ic] : 1, ihi = g.tz.idx[ic + 1] - 1 < nzm ? g.tz.idx[ic + 1] - 1 : nzm;
+            for (int kc = 0; kc <= nxmo; kc++) {
+                int klo = g.tx.idx[kc] > 1 ? g.tx.idx[kc] : 1, khi = g.tx.idx[kc + 1] - 1 < nxm ? g.tx.idx[kc + 1] - 1 : nxm;
+                double qv2[4][4];   /* [c][a] = vyxzc(kc-1+a, ic-1+c) */
+                for (int c = 0; c < 4; c++) for (int a = 0; a < 4; a++) qv2[c][a] = PL(kc - 1 + a, ic - 1 + c);
+                for (int icr = ilo; icr <= ihi; icr++) {
+                    const double *cz = g.tz.w + (size_t)(icr - 1) * 4;      /* czvy = czvx */
+                    double qv1[4];
+                    for (int a = 0; a < 4; a++) qv1[a] = qv2[0][a] * cz[0] + qv2[1][a] * cz[1] + qv2[2][a] * cz[2] + qv2[3][a] * cz[3];
+                    for (int kcr = klo; kcr <= khi; kcr++) {
+                        const double *cx = g.tx.w + (size_t)(kcr - 1) * 4;  /* cxvy */
+                        vy[B(kcr, 0, icr - 1)] = qv1[0] * cx[0] + qv1[1] * cx[1] + qv1[2] * cx[2] + qv1[3] * cx[3];
+                    }
+                }
+            }
+        }
+#undef PL
+        free(pl);
+        double lya = 1.0 / s->dy;
+        for (int i = 0; i < nzm; i++)
+            for (int j = 1; j < nym; j++)
+                for (int kc = 1; kc <= nxm; kc++)
+                    vy[B(kc, j, i)] = vy[B(kc, j - 1, i)] + tp[((size_t)i * nym + j - 1) * nx + kc - 1] * lya;
+    }
+
+    /* vz: dvz/dz (:196-210), interpolated (:214), first-guess plane at the
+     * first z face from the old plane ic0 = 1 (:217-248), integrated along z (:253-261) */
+    memset(ov, 0, sizeof(double) * (size_t)lo * nymo * nzmo);
+    for (int i = 0; i < nzmo; i++) {
+        int ip = (i + 1) % nzmo;
+        for (int j = 0; j < nymo; j++) {
+            for (int kc = 1; kc <= nxmo; kc++) OE(kc, j, i) = (O3(vzo, kc, j, ip) - O3(vzo, kc, j, i)) * dzo;
+            OE(0, j, i) = sS * OE(1, j, i);
+            OE(nxo, j, i) = sN * OE(nxmo, j, i);
+        }
+    }
+    old_to_new(ov, nxmo, nymo, nzmo, &g.tx, &g.ty, &g.tz, tp, nxm, nym, nzm, nx);
+    {
+        double *pl = calloc((size_t)lo * nymo, sizeof(double));
+#define PL(k, j0) pl[(size_t)W((j0), nymo) * lo + (k) + 1]
+        for (int j = 1; j <= nymo; j++) {
+            for (int kc = 1; kc <= nxmo; kc++) PL(kc, j) = O3(vzo, kc, j - 1, 0);
+            PL(0, j) = sS * PL(1, j);
+            PL(nxo, j) = sN * PL(nxmo, j);
+        }
+        for (int jc = 0; jc <= nymo; jc++) {
+            int jlo = g.ty.idx[jc] > 1 ? g.ty.idx[jc] : 1, jhi = g.ty.idx[jc + 1] - 1 < nym ? g.ty.idx[jc + 1] - 1 : nym;
+            for (int kc = 0; kc <= nxmo; kc++) {
+                int klo = g.tx.idx[kc] > 1 ? g.tx.idx[kc] : 1, khi = g.tx.idx[kc + 1] - 1 < nxm ? g.tx.idx[kc + 1] - 1 : nxm;
+                double qv2[4][4];   /* [b][a] = vzxyc(kc-1+a, jc-1+b) */
+                for (int b = 0; b < 4; b++) for (int a = 0; a < 4; a++) qv2[b][a] = PL(kc - 1 + a, jc - 1 + b);
+                for (int jcr = jlo; jcr <= jhi; jcr++) {
+                    const double *cy = g.ty.w + (size_t)(jcr - 1) * 4;      /* cyvz = cyvx */
+                    double qv1[4];
+                    for (int a = 0; a < 4; a++) qv1[a] = qv2[0][a] * cy[0] + qv2[1][a] * cy[1] + qv2[2][a] * cy[2] + qv2[3][a] * cy[3];
+                    for (int kcr = klo; kcr <= khi; kcr++) {
+                        const double *cx = g.tx.w + (size_t)(kcr - 1) * 4;  /* cxvz */
+                        vz[B(kcr, jcr - 1, 0)] = qv1[0] * cx[0] + qv1[1] * cx[1] + qv1[2] * cx[2] + qv1[3] * cx[3];
+                    }
+                }
+            }
+        }
+#undef PL
+        free(pl);
+        double lza = 1.0 / s->dz;
+        for (int i = 1; i < nzm; i++)
+            for (int j = 0; j < nym; j++)
+                for (int kc = 1; kc <= nxm; kc++)
+                    vz[B(kc, j, i)] = vz[B(kc, j, i - 1)] + tp[((size_t)(i - 1) * nym + j) * nx + kc - 1] * lza;
+    }
+
+    /* T with the wall ghosts of its boundary values (:280-330) */
+    double tup, tlo;
+    if (s->c.rayt > 0) {
+        if (s->c.inslwn == 0) { tup = 0.0; tlo = 1.0; } else { tup = -0.5; tlo = 0.5; }
+    } else { tup = 0.5; tlo = -0.5; }
+    if (s->c.phasefield) {
+        if (s->c.rayt > 0) { tup = 0.0; tlo = 1.0; } else { tup = 1.0; tlo = 0.0; }
+    }
+    memset(ov, 0, sizeof(double) * (size_t)lo * nymo * nzmo);
+    for (int i = 0; i < nzmo; i++)
+        for (int j = 0; j < nymo; j++) {
+            for (int kc = 1; kc <= nxo; kc++) OE(kc, j, i) = O3(tempo, kc, j, i);
+            OE(0, j, i) = s->c.tfixs == 1 ? 2.0 * tlo - OE(1, j, i) : OE(1, j, i);
+            OE(nxo, j, i) = s->c.tfixn == 1 ? 2.0 * tup - OE(nxmo, j, i) : OE(nxmo, j, i);
+        }
+    memset(temp, 0, sizeof(double) * nbase(s));
+    old_to_new(ov, nxmo, nymo, nzmo, &g.tx, &g.ty, &g.tz, temp, nxm, nym, nzm, nx);
+#undef O3
+#undef OE
+    free(ov); free(tp);
+    free_old(&g);
+    return 0;
+}
+
+/* InterpInputSal.F90 (which = 0) and InterpInputPhi.F90 (which = 1): the old
+ * refined scalar (nzro-1, nyro-1, nxro) with its wall ghosts, Dirichlet from
+ * the sign of Ra_S (phase field: 1 below, 0 above) or zero-gradient for phi,
+ * interpolated to the new refined cell centres (CreateNewSalStencil.F90). */
+static int interp_input_ref(orc_state *s, const orc_old_grid *og, const double *fo, int which) {
+    REF_DECL
+    old_grid g;
+    if (make_old(s, og, 1, &g)) return 1;
+    int nxmo = g.nxm, nymo = g.nym, nzmo = g.nzm, nxo = nxmo + 1, lo = nxmo + 4;
+    double *ov = calloc((size_t)lo * nymo * nzmo, sizeof(double));
+    double sup, slo;
+    if (s->c.rays > 0) { sup = 0.5; slo = -0.5; } else { sup = -0.5; slo = 0.5; }
+    if (s->c.phasefield) { sup = 0.0; slo = 1.0; }
+    for (int i = 0; i < nzmo; i++)
+        for (int j = 0; j < nymo; j++) {
+            double *row = ov + ((size_t)i * nymo + j) * lo + 1;
+            const double *src = fo + ((size_t)i * nymo + j) * nxo;
+            for (int kc = 1; kc <= nxo; kc++) row[kc] = src[kc - 1];
+            if (which == 0) {
+                row[0] = s->c.sfixs == 1 ? 2.0 * slo - row[1] : row[1];
+                row[nxo] = s->c.sfixn == 1 ? 2.0 * sup - row[nxmo] : row[nxmo];
+            } else {
+                row[0] = row[1];
+                row[nxo] = row[nxmo];
+            }
+        }
+    double *f = s->f[which == 0 ? ORC_SAL : ORC_PHI];
+    memset(f, 0, sizeof(double) * nref(s));
+    old_to_new(ov, nxmo, nymo, nzmo, &g.tx, &g.ty, &g.tz, f, nxmr, nymr, nzmr, nxr);
+    free(ov);
+    free_old(&g);
+    return 0;
+}
+int orc_interp_input_sal(orc_state *s, const orc_old_grid *og, const double *salo) { return interp_input_ref(s, og, salo, 0); }
+int orc_interp_input_phi(orc_state *s, const orc_old_grid *og, const double *phio) { return interp_input_ref(s, og, phio, 1); }

@@ -110,11 +110,35 @@ void orc_sal_stats(orc_state *s, double *out);
 /* CalcPhiStats: phibar, phirms of nxmr values each */
 void orc_phi_stats(orc_state *s, double *out);
 
+/* Restart onto a new grid (ReadFlowInterp.F90:90-141): the old run's grid
+ * as continua_master.h5 describes it (nx, ny, nz, istr3, str3 and, with
+ * multires, nxr, nyr, nzr, istr3r), and the interpolation of its fields,
+ * (n?o-1, ..., nxo) x fastest as the checkpoints hold them. */
+typedef struct orc_old_grid {
+    int32_t nxo, nyo, nzo, istr3o;
+    double str3o;
+    int32_t nxro, nyro, nzro, istr3ro;
+} orc_old_grid;
+int orc_interp_input_vel(orc_state *s, const orc_old_grid *og, const double *vxo, const double *vyo,
+                         const double *vzo, const double *tempo);   /* InterpInputVel.F90 */
+int orc_interp_input_sal(orc_state *s, const orc_old_grid *og, const double *salo);   /* InterpInputSal.F90 */
+int orc_interp_input_phi(orc_state *s, const orc_old_grid *og, const double *phio);   /* InterpInputPhi.F90 */
+
+/* Immersed boundary (src/ibm/ibm_param.F90): the mask of the solve of field
+ * (ORC_VX, ORC_VY, ORC_VZ, ORC_TEMP on (nzm, nym, nx); ORC_SAL on (nzmr,
+ * nymr, nxr)), values 2 liquid, 0 solid, 1 / -1 upper / lower boundary
+ * point, and the boundary distances in the loop order (ic, jc, kc) (none for
+ * S).  The implicit updates then solve SolveImpEqnUpdate_*_ibm.  Returns 0,
+ * or nonzero for a bad field or too short a list. */
+int orc_set_ibm(orc_state *s, int field, const int32_t *mask, const double *dist, int64_t ndist);
+
 /* free-standing numerics used by the pins */
 void orc_fft(double *re, double *im, int n, int sign);      /* in place, unnormalised */
 void orc_dft_naive(const double *re, const double *im, double *ore, double *oim, int n, int sign);
 void orc_tridiag_solve(int n, const double *a, const double *b, const double *c, double *d);
 void orc_set_threads(int n);
+/* Planted mistake n (0: none); the pin tests show each one is caught. */
+void orc_set_plant(int n);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,11 @@ def lib():
         L.orc_calc_max_cfl.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_check_divergence.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_global_quantities.argtypes = [vp, C.POINTER(C.c_double)]
+        L.orc_set_plant.argtypes = [C.c_int]
+        L.orc_set_ibm.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_int64]
+        L.orc_interp_input_vel.argtypes = [vp, C.POINTER(OldGrid)] + [C.POINTER(C.c_double)] * 4
+        L.orc_interp_input_sal.argtypes = [vp, C.POINTER(OldGrid), C.POINTER(C.c_double)]
+        L.orc_interp_input_phi.argtypes = [vp, C.POINTER(OldGrid), C.POINTER(C.c_double)]
         L.orc_mean_profiles.argtypes = [vp, C.POINTER(C.c_double)]
         L.orc_sal_stats.argtypes = [vp, C.POINTER(C.c_double)]
         L.orc_phi_stats.argtypes = [vp, C.POINTER(C.c_double)]
@@ -154,6 +159,29 @@ class Oracle:
 
     def call(self, name):
         getattr(self.L, "orc_" + name)(self.h)
+
+    def set_ibm(self, field, mask, dist=None):
+        """The immersed-boundary mask (nz, ny, nx) of a solve and its boundary
+        distances in loop order (orc_set_ibm)."""
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        d = np.ascontiguousarray(dist if dist is not None else np.zeros(1), dtype=np.float64)
+        if self.L.orc_set_ibm(self.h, FIELD_ID[field], m.ctypes.data_as(C.POINTER(C.c_int32)), _dp(d),
+                              0 if dist is None else d.size):
+            raise RuntimeError("orc_set_ibm failed")
+
+    def interp_input_vel(self, og, vx, vy, vz, temp):
+        """InterpInputVel: old fields (nzo-1, nyo-1, nxo) onto this grid."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (vx, vy, vz, temp)]
+        rc = self.L.orc_interp_input_vel(self.h, C.byref(og), *[_dp(x) for x in a])
+        if rc:
+            raise RuntimeError("orc_interp_input_vel failed")
+
+    def interp_input_ref(self, og, name, f):
+        """InterpInputSal (name 'sal') / InterpInputPhi ('phi') onto this refined grid."""
+        a = np.ascontiguousarray(f, dtype=np.float64)
+        fn = self.L.orc_interp_input_sal if name == "sal" else self.L.orc_interp_input_phi
+        if fn(self.h, C.byref(og), _dp(a)):
+            raise RuntimeError("orc_interp_input_" + name + " failed")
 
     def time_marcher(self, dt):
         self.L.orc_time_marcher(self.h, dt)
@@ -241,3 +269,34 @@ def tridiag_solve(a, b, c, d):
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+class OldGrid(C.Structure):
+    """The old run's grid of a checkpoint (continua_master.h5 scalars)."""
+    _fields_ = [("nxo", C.c_int32), ("nyo", C.c_int32), ("nzo", C.c_int32), ("istr3o", C.c_int32),
+                ("str3o", C.c_double), ("nxro", C.c_int32), ("nyro", C.c_int32), ("nzro", C.c_int32),
+                ("istr3ro", C.c_int32)]
+
+
+def old_grid(cfg):
+    """OldGrid of a run with config dict cfg (the grid its checkpoint records)."""
+    mr = cfg.get("multires") or cfg.get("salinity") or cfg.get("phasefield")
+    return OldGrid(cfg["nxm"] + 1, cfg["nym"] + 1, cfg["nzm"] + 1, cfg.get("istr3", 0), float(cfg.get("str3", 0.0)),
+                   (cfg["nxmr"] + 1) if mr else 0, (cfg["nymr"] + 1) if mr else 0, (cfg["nzmr"] + 1) if mr else 0,
+                   cfg.get("istr3r", 0) if mr else 0)
+
+
+class planted:
+    """Context manager: the oracle runs with planted mistake n (see
+    orc_set_plant in afid_oracle.c) inside the block."""
+
+    def __init__(self, n):
+        self.n = int(n)
+
+    def __enter__(self):
+        lib().orc_set_plant(self.n)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_plant(0)
+        return False

@@ -1,0 +1,133 @@
+"""Pins of the oracle's restart onto a new grid (ReadFlowInterp.F90:90-141,
+IC_interpolation/InterpInputVel.F90, InterpInputSal.F90, InterpInputPhi.F90):
+
+  * identity: an old grid equal to the new one gives the old fields back
+    (the interpolation injects, t < 1e-12 in construct_stencil, and the
+    integrated gradients telescope);
+  * exactness: T = a + b x with its own wall values is transferred exactly at
+    every new centre (linear wall intervals, Hermite interior), and a
+    quadratic in x exactly away from the two wall intervals;
+  * divergence: the new velocity integrates interpolated gradients, so a
+    discretely divergence-free old velocity gives a discretely
+    divergence-free new one (all three gradients share the cell-centre
+    stencils), periodic seams included up to the closure of the integration;
+  * a planted mistake (the z-gradient scaled by the new spacing) fails.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from helpers import uniform
+
+OLD = dict(nxm=16, nym=8, nzm=12, istr3=4, str3=1.5, rayt=1e5, prat=1.0, ylen=1.0, zlen=1.5)
+
+
+def divergence_free(cfg, seed):
+    """A discretely divergence-free velocity on cfg's grid (the projection of
+    a random field through the oracle's own pressure solve), and a random T."""
+    o = po.Oracle(cfg)
+    for q, n in enumerate(("vx", "vy", "vz", "temp")):
+        a = uniform(seed * 10 + q, o.field(n).shape) - 0.5
+        if n == "vx":
+            a[..., 0] = 0.0
+            a[..., -1] = 0.0
+        o.field(n)[...] = a
+    o.set_substep(1e-2, 1)
+    for r in ("calc_local_divergence", "solve_pressure_correction", "correct_velocity"):
+        o.call(r)
+    assert o.check_divergence()[0] < 1e-12
+    return o
+
+
+def restart(new_cfg, old_cfg, fields):
+    o = po.Oracle(new_cfg)
+    o.interp_input_vel(po.old_grid(old_cfg), fields["vx"], fields["vy"], fields["vz"], fields["temp"])
+    return o
+
+
+def test_identity_same_grid():
+    src = divergence_free(OLD, 3)
+    f = {n: src.field(n).copy() for n in ("vx", "vy", "vz", "temp")}
+    o = restart(OLD, OLD, f)
+    nxm = OLD["nxm"]
+    for n in ("vx", "vy", "vz", "temp"):
+        a, b = o.field(n)[..., :nxm], f[n][..., :nxm]
+        assert np.abs(a - b).max() < 1e-13 * max(np.abs(b).max(), 1.0), n
+    src.close(); o.close()
+
+
+@pytest.mark.parametrize("new", [
+    dict(nxm=24, nym=12, nzm=18, istr3=8),                  # finer, another clustering
+    dict(nxm=12, nym=6, nzm=9, istr3=0),                    # coarser, uniform
+    dict(nxm=20, nym=16, nzm=12, istr3=6, str3=0.0),        # clipped Chebyshev
+])
+def test_divergence_preserved(new):
+    """The discrete divergence (CheckDivergence.F90:29-41, written out here
+    from the new grid's udx3m and spacings) of the restarted velocity vanishes
+    in every cell off the periodic seams.  On the seams the integrated vy / vz
+    meet their periodic image, and the mismatch is the sum of the
+    interpolated gradient over the period: the reference's closure, which
+    the Hermite weights keep only approximately (large for a coarsened
+    white-noise field), so it is not pinned."""
+    cfg = dict(OLD, **new)
+    src = divergence_free(OLD, 5)
+    f = {n: src.field(n).copy() for n in ("vx", "vy", "vz", "temp")}
+    o = restart(cfg, OLD, f)
+    nxm, nym, nzm = cfg["nxm"], cfg["nym"], cfg["nzm"]
+    vx, vy, vz = o.field("vx"), o.field("vy"), o.field("vz")
+    udx3m = o.grid("udx3m")[:nxm]
+    dy, dz = nym / cfg["ylen"], nzm / cfg["zlen"]
+    div = ((vx[..., 1:nxm + 1] - vx[..., :nxm]) * udx3m + (np.roll(vy, -1, axis=1) - vy)[..., :nxm] * dy
+           + (np.roll(vz, -1, axis=0) - vz)[..., :nxm] * dz)
+    scale = max(np.abs(vx).max() * udx3m.max(), np.abs(vy).max() * dy, np.abs(vz).max() * dz)
+    assert np.abs(div[:nzm - 1, :nym - 1]).max() < 1e-13 * scale
+    # the oracle's CheckDivergence agrees with the written-out divergence
+    assert abs(o.check_divergence()[0] - np.abs(div).max()) <= 1e-12 * scale
+    src.close(); o.close()
+
+
+def test_linear_and_quadratic_temperature():
+    new = dict(OLD, nxm=28, nym=8, nzm=12, istr3=8)
+    src = po.Oracle(OLD)
+    xo = src.grid("xm")[:OLD["nxm"]]
+    o = po.Oracle(new)
+    xn = o.grid("xm")[:new["nxm"]]
+    z = {n: np.zeros_like(src.field(n)) for n in ("vx", "vy", "vz")}
+    # RBC walls of T (+0.5 / -0.5, InterpInputVel.F90:286-292): T = 0.5 - x is exact everywhere
+    T = np.zeros_like(src.field("temp"))
+    T[..., :OLD["nxm"]] = 0.5 - xo
+    o.interp_input_vel(po.old_grid(OLD), z["vx"], z["vy"], z["vz"], T)
+    assert np.abs(o.field("temp")[..., :new["nxm"]] - (0.5 - xn)).max() < 1e-14
+    # a quadratic is exact where the new centre's old interval is an interior one
+    T[..., :OLD["nxm"]] = 0.5 - xo + 0.7 * xo * (1 - xo)
+    o.interp_input_vel(po.old_grid(OLD), z["vx"], z["vy"], z["vz"], T)
+    inner = (xn > xo[1]) & (xn < xo[-2])      # 4-point stencils of real points only
+    e = np.abs(o.field("temp")[..., :new["nxm"]] - (0.5 - xn + 0.7 * xn * (1 - xn)))[..., inner]
+    assert e.max() < 1e-13
+    src.close(); o.close()
+
+
+def test_refined_scalars():
+    """InterpInputSal / InterpInputPhi on the refined grids: identity on the
+    same grid; a line through the S wall values (Ra_S < 0: +0.5 below, -0.5
+    above, InterpInputSal.F90:33-40) exact on another grid; phi with
+    zero-gradient ghosts keeps a constant."""
+    old = dict(OLD, multires=1, salinity=1, nxmr=24, nymr=16, nzmr=24, istr3r=4, rays=-1e5, pras=4.0)
+    new = dict(old, nxmr=36, nymr=24, nzmr=36, istr3r=8)
+    src = po.Oracle(old)
+    S = uniform(9, src.field("sal").shape) - 0.5
+    o = po.Oracle(old)
+    o.interp_input_ref(po.old_grid(old), "sal", S)
+    assert np.abs(o.field("sal")[..., :old["nxmr"]] - S[..., :old["nxmr"]]).max() < 1e-14
+    xo = src.grid("xmr")[:old["nxmr"]]
+    S[..., :old["nxmr"]] = 0.5 - xo
+    n = po.Oracle(new)
+    n.interp_input_ref(po.old_grid(old), "sal", S)
+    xn = n.grid("xmr")[:new["nxmr"]]
+    assert np.abs(n.field("sal")[..., :new["nxmr"]] - (0.5 - xn)).max() < 1e-14
+    pf_old = dict(old, salinity=0, phasefield=1, pf_a=1.0, pf_s=1.0, pf_tm=0.0)
+    pf_new = dict(new, salinity=0, phasefield=1, pf_a=1.0, pf_s=1.0, pf_tm=0.0)
+    p = po.Oracle(pf_new)
+    p.interp_input_ref(po.old_grid(pf_old), "phi", np.full(S.shape, 0.25))
+    assert np.abs(p.field("phi")[..., :new["nxmr"]] - 0.25).max() < 1e-15
+    src.close(); o.close(); n.close(); p.close()
