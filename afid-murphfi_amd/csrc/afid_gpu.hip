@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -23,6 +24,8 @@
 #include "afid_poisson.hpp"
 #include "afid_phase.hpp"
 #include "afid_means.hpp"
+#include "afid_restart.hpp"
+#include "afid_ibm.hpp"
 
 using namespace afid;
 
@@ -39,6 +42,18 @@ struct TimerRec {
 };
 
 }  // namespace
+
+// Refined -> coarse tables of one transfer (k_interp_coarse_tile), per axis
+// and coarse index: the first refined source and 4 weights (FP64, and FP32
+// for the mixed form), with the tile's LDS plan.
+struct CoarseTab {
+    const int *x_src = nullptr, *y_src = nullptr, *z_src = nullptr;
+    const double *x_w = nullptr, *y_w = nullptr, *z_w = nullptr;
+    const float *x_wf = nullptr, *y_wf = nullptr, *z_wf = nullptr;
+    int xr = 0, yr = 0;          // largest refined x / y span of a tile
+    size_t lds = 0;
+    bool general = false;        // interpolate_xyz_to_coarse (refined grid does not bracket)
+};
 
 struct afid_ctx {
     afid_config c{};
@@ -73,13 +88,9 @@ struct afid_ctx {
     double pf_d = 0, pf_eps = 0, pf_b = 0, heta = 0, pf_r0 = 0.1;
     std::vector<double> h_ap3spkr, h_ac3spkr, h_am3spkr;
     const double *d_ap3spkr = nullptr, *d_ac3spkr = nullptr, *d_am3spkr = nullptr;
-    const int *d_xp_src = nullptr;
-    const double *d_xp_w = nullptr;
     double *phi_alt = nullptr;
     float *phi_alt_f = nullptr;                 // second phi buffer of the mixed form
-    const float *f_xp_w = nullptr;              // FP32 cxphic weights (mixed form)
-    int phic_xr = 0;
-    size_t lds_phic = 0;
+    CoarseTab tab_sal, tab_phi;                 // InterpSalMultigrid / InterpPhiMultigrid + AddLatentHeat
     // arithmetic form
     bool mixed = false;
     double div_tol = 1e-10;
@@ -95,10 +106,30 @@ struct afid_ctx {
     double *d_mean_part = nullptr, *d_mean_out = nullptr, *d_sal_part = nullptr, *d_sal_out = nullptr;
     double *d_phi_part = nullptr, *d_phi_out = nullptr;
     int mean_groups = 0, sal_groups = 0, phi_groups = 0;
+    int col_tz = 0;                     // z extent of the base column tiles (0: largest that fits)
+    // mixed form, gAxis = 1: the FP32 vx explicit term and its history hold
+    // the buoyancy minus a per-row FP64 reference bref (that of the
+    // conductive profile between the mean wall temperatures), which the FP64
+    // RHS adds back; recomputed at the first substep after a wall upload
+    // immersed boundary: per solve (vx, vy, vz, T, S) the row mask and the
+    // per-cell boundary distances on the device (null: the plain solve)
+    signed char *ibm_mask[5] = {nullptr};
+    double *ibm_dist[5] = {nullptr};
+    std::vector<double> h_bref;
+    double *d_bref = nullptr;
+    double wall_mean[2] = {0.0, 0.0};
+    bool bref_dirty = false;
+    // afid_time_marcher_graph: two RK3 steps captured as one graph (the
+    // ping-pong buffers swap an odd number of times per step, so two steps
+    // bring every pointer back), valid for graph_dt and the pointers it
+    // captured
+    hipGraphExec_t graph = nullptr;
+    double graph_dt = 0.0;
+    double *graph_ptrs[AFID_NFIELDS] = {nullptr};
     const double *d_udx3mr = nullptr, *d_ap3sskr = nullptr, *d_ac3sskr = nullptr, *d_am3sskr = nullptr, *d_lxa = nullptr;
-    const int *d_xs_src = nullptr, *d_ys_src = nullptr, *d_zs_src = nullptr, *d_xb_src = nullptr, *d_yb_src = nullptr, *d_zb_src = nullptr;
-    const double *d_xs_w = nullptr, *d_ys_w = nullptr, *d_zs_w = nullptr, *d_xb_w = nullptr, *d_yb_w = nullptr, *d_zb_w = nullptr;
-    const float *f_xs_w = nullptr, *f_ys_w = nullptr, *f_zs_w = nullptr, *f_xb_w = nullptr, *f_yb_w = nullptr, *f_zb_w = nullptr;
+    const int *d_xs_src = nullptr, *d_ys_src = nullptr, *d_zs_src = nullptr;
+    const double *d_xs_w = nullptr, *d_ys_w = nullptr, *d_zs_w = nullptr;
+    const float *f_xs_w = nullptr, *f_ys_w = nullptr, *f_zs_w = nullptr;
     const float *f_lxa = nullptr;
     // host copies of the 1-D coefficients for the tridiagonal factors
     std::vector<double> h_ap3ck, h_ac3ck, h_am3ck, h_ap3sk, h_ac3sk, h_am3sk, h_ap3ssk, h_ac3ssk, h_am3ssk;
@@ -119,8 +150,6 @@ struct afid_ctx {
     // LDS plan of the transfer kernels
     int vxr_xc = 0, vxr_cj = 0, vxr_ci = 0, vy_cs = 0, vy_co = 0, vz_cs = 0, vz_co = 0;
     size_t lds_vxr = 0, lds_vy = 0, lds_vz = 0;
-    int salc_xr = 0, salc_yr = 0;
-    size_t lds_salc = 0;
     // reductions
     unsigned long long *redmax = nullptr;
     double *redsum = nullptr, *redout = nullptr;
@@ -216,12 +245,10 @@ static KP make_kp(const afid_ctx *c) {
     p.udx3mr = c->d_udx3mr; p.ap3sskr = c->d_ap3sskr; p.ac3sskr = c->d_ac3sskr; p.am3sskr = c->d_am3sskr; p.lxa = c->d_lxa;
     p.xs_src = c->d_xs_src; p.ys_src = c->d_ys_src; p.zs_src = c->d_zs_src;
     p.xs_w = c->d_xs_w; p.ys_w = c->d_ys_w; p.zs_w = c->d_zs_w;
-    p.xb_src = c->d_xb_src; p.yb_src = c->d_yb_src; p.zb_src = c->d_zb_src;
-    p.xb_w = c->d_xb_w; p.yb_w = c->d_yb_w; p.zb_w = c->d_zb_w;
     p.redmax = c->redmax; p.redsum = c->redsum;
     p.salf = c->g[AFID_SAL]; p.rusalf = c->g[AFID_RUSAL]; p.vxrf = c->g[AFID_VXR]; p.vyrf = c->g[AFID_VYR]; p.vzrf = c->g[AFID_VZR];
     p.ruxf = c->g[AFID_RUX]; p.ruyf = c->g[AFID_RUY]; p.ruzf = c->g[AFID_RUZ]; p.rutf = c->g[AFID_RUTEMP]; p.salcf = c->g[AFID_SALC];
-    p.xs_wf = c->f_xs_w; p.ys_wf = c->f_ys_w; p.zs_wf = c->f_zs_w; p.xb_wf = c->f_xb_w; p.yb_wf = c->f_yb_w; p.zb_wf = c->f_zb_w;
+    p.xs_wf = c->f_xs_w; p.ys_wf = c->f_ys_w; p.zs_wf = c->f_zs_w;
     p.lxaf = c->f_lxa;
     p.phi = c->f[AFID_PHI]; p.ruphi = c->f[AFID_RUPHI]; p.hphi = c->f[AFID_HPHI]; p.tempr = c->f[AFID_TEMPR];
     p.phic = c->f[AFID_PHIC]; p.qlat = c->f[AFID_HRO];
@@ -229,8 +256,8 @@ static KP make_kp(const afid_ctx *c) {
     p.dphf = c->g[AFID_DPH];
     p.pf_d = c->pf_d; p.pf_b = c->pf_b; p.pf_a = c->c.pf_a; p.pf_tm = c->c.pf_tm; p.pf_s = c->c.pf_s; p.heta = c->heta;
     p.pf = c->pf ? 1 : 0;
+    p.bref = c->d_bref;
     p.ap3spkr = c->d_ap3spkr; p.ac3spkr = c->d_ac3spkr; p.am3spkr = c->d_am3spkr;
-    p.xp_src = c->d_xp_src; p.xp_w = c->d_xp_w;
     return p;
 }
 
@@ -631,6 +658,11 @@ static int setup(afid_ctx *c) {
     c->npadmax = 64 * chmax;
     if ((rc = dalloc(c, reinterpret_cast<void **>(&c->tri), sizeof(double) * size_t(3 * MAT_NKIND * kTriArrays) * c->npadmax))) return rc;
 
+    // mixed form: the reference buoyancy of the FP32 vx explicit term (zero until the walls are set)
+    if (c->mixed && g.gaxis == 1 && g.active_t != 0) {
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_bref), sizeof(double) * size_t(c->ldx)))) return rc;
+        c->h_bref.assign(size_t(c->ldx), 0.0);
+    }
     // reductions
     if ((rc = dalloc(c, reinterpret_cast<void **>(&c->redmax), sizeof(unsigned long long) * 4))) return rc;
     {
@@ -690,46 +722,61 @@ static int setup(afid_ctx *c) {
     Stencil sy = construct_stencil(ym, c->ny, ymr, nyr, g.ylen, jrangs, 'y');
     IArr krangs = interpolation_indices(zm, c->nz, zmr, nzr, g.zlen);
     Stencil sz = construct_stencil(zm, c->nz, zmr, nzr, g.zlen, krangs, 'z');
-    // refined -> coarse (InterpSalMultigrid fast path)
-    if (!(xmr(1) < xm(1) && xmr(nxmr) > xm(nxm)))
-        return set_err(c, AFID_E_ARG, "InterpSalMultigrid: only the bracketing (fast) path is implemented");
-    Stencil bx, by, bz;
-    std::vector<int> xb_src((size_t)(nx)), yb_src((size_t)(c->ny)), zb_src((size_t)(c->nz));
-    {
-        IArr jrangr = interpolation_indices(ymr, nyr, ym, c->ny, g.ylen);
-        by = construct_stencil(ymr, nyr, ym, c->ny, g.ylen, jrangr, 'y');
-        IArr krangr = interpolation_indices(zmr, nzr, zm, c->nz, g.zlen);
-        bz = construct_stencil(zmr, nzr, zm, c->nz, g.zlen, krangr, 'z');
-        IArr irangb;
+    // refined -> coarse tables (CreateMgrdStencil.F90:21-78).  Each is, per
+    // coarse index, the first of 4 refined sources (1-based) and 4 weights.
+    // The bracketing (fast) path (HermiteInterpolations.F90:256-312) takes
+    // sources irang-2..irang+1 with the coarse -> refined index arrays; the
+    // general path (:218-250, used when xmr(1) >= xm(1) or xmr(nxmr) <=
+    // xm(nxm), salinity.F90:485-489, phasefield.F90:489-492, :701-723) takes
+    // kcr-1..kcr+2 of the refined cell kcr owning the coarse point, with the
+    // cell-centre (phic) weights on every axis.
+    const bool bracket = xmr(1) < xm(1) && xmr(nxmr) > xm(nxm);
+    IArr irangr = interpolation_indices(xmrr, nxmr, xmc, nxm, g.alx3);
+    IArr jrangr = interpolation_indices(ymr, nyr, ym, c->ny, g.ylen);
+    IArr krangr = interpolation_indices(zmr, nzr, zm, c->nz, g.zlen);
+    Stencil bp = construct_stencil(xmrr, nxmr, xmc, nxm, g.alx3, irangr, 'x');   // cxphic
+    Stencil byc = construct_stencil(ymr, nyr, ym, c->ny, g.ylen, jrangr, 'y');    // cyphic
+    Stencil bzc = construct_stencil(zmr, nzr, zm, c->nz, g.zlen, krangr, 'z');    // czphic
+    H["cxphic"] = bp.w;
+    struct HostTab { std::vector<int> xs, ys, zs; std::vector<double> xw, yw, zw; };
+    HostTab hs, hp;     // salinity, phase field
+    if (bracket) {
+        // salinity: salc at the faces along gAxis (cxsalc / cysalc / czsalc, irangb / jrangb / krangb)
+        FArr xcc(1, nx), ycc(1, c->ny), zcc(1, c->nz);
+        for (int k = 1; k <= nx; ++k) xcc(k) = xc(k);
+        for (int j = 1; j <= c->ny; ++j) ycc(j) = yc(j);
+        for (int i = 1; i <= c->nz; ++i) zcc(i) = zc(i);
+        hs.xw = bp.w; hs.yw = byc.w; hs.zw = bzc.w;
+        IArr irangb = irangs, jrangb = jrangs, krangb = krangs;
         if (g.gaxis == 1) {
-            FArr xcc(1, nx);
-            for (int k = 1; k <= nx; ++k) xcc(k) = xc(k);
             IArr irange = interpolation_indices(xmrr, nxmr, xcc, nx, g.alx3);
-            bx = construct_stencil(xmrr, nxmr, xcc, nx, g.alx3, irange, 'x');
+            hs.xw = construct_stencil(xmrr, nxmr, xcc, nx, g.alx3, irange, 'x').w;
             irangb = interpolation_indices(xcc, nx, xmrr, nxmr, g.alx3);
-        } else {
-            IArr irangr = interpolation_indices(xmrr, nxmr, xmc, nxm, g.alx3);
-            bx = construct_stencil(xmrr, nxmr, xmc, nxm, g.alx3, irangr, 'x');
-            irangb = irangs;
+        } else if (g.gaxis == 2) {
+            IArr jrange = interpolation_indices(ymr, nyr, ycc, c->ny, g.ylen);
+            hs.yw = construct_stencil(ymr, nyr, ycc, c->ny, g.ylen, jrange, 'y').w;
+            jrangb = interpolation_indices(ycc, c->ny, ymr, nyr, g.ylen);
+        } else if (g.gaxis == 3) {
+            IArr krange = interpolation_indices(zmr, nzr, zcc, c->nz, g.zlen);
+            hs.zw = construct_stencil(zmr, nzr, zcc, c->nz, g.zlen, krange, 'z').w;
+            krangb = interpolation_indices(zcc, c->nz, zmr, nzr, g.zlen);
         }
-        // first refined point of each coarse target: irang(kc)-2 (HermiteInterpolations.F90:292-298)
-        for (int k = 1; k <= nx; ++k) xb_src[size_t(k - 1)] = k <= nxm ? irangb(k) - 2 : 0;
-        for (int j = 1; j <= c->ny; ++j) yb_src[size_t(j - 1)] = jrangs(j) - 2;
-        for (int i = 1; i <= c->nz; ++i) zb_src[size_t(i - 1)] = krangs(i) - 2;
-        bx.w.resize(size_t(4 * nx), 0.0);
+        for (int k = 1; k <= nxm; ++k) hs.xs.push_back(irangb(k) - 2);
+        for (int j = 1; j <= c->ny; ++j) hs.ys.push_back(jrangb(j) - 2);
+        for (int i = 1; i <= c->nz; ++i) hs.zs.push_back(krangb(i) - 2);
+        // phase field: cxphic with irangs ("phi" branch, :283-290)
+        hp.xw = bp.w; hp.yw = byc.w; hp.zw = bzc.w;
+        for (int k = 1; k <= nxm; ++k) hp.xs.push_back(irangs(k) - 2);
+        for (int j = 1; j <= c->ny; ++j) hp.ys.push_back(jrangs(j) - 2);
+        for (int i = 1; i <= c->nz; ++i) hp.zs.push_back(krangs(i) - 2);
+    } else {
+        // construct_stencil's src: kcr - 1 of the refined cell kcr owning the coarse point
+        hp.xs = bp.src; hp.ys = byc.src; hp.zs = bzc.src;
+        hp.xw = bp.w; hp.yw = byc.w; hp.zw = bzc.w;
+        hs = hp;
     }
-    // phase field: refined -> coarse cell centres with cxphic and irangs
-    // (interpolate_xyz_to_coarse_fast, "phi", HermiteInterpolations.F90:283-290)
-    std::vector<int> xp_src((size_t)(nx), 0);
-    Stencil bp;
-    if (c->pf) {
-        IArr irangr = interpolation_indices(xmrr, nxmr, xmc, nxm, g.alx3);
-        bp = construct_stencil(xmrr, nxmr, xmc, nxm, g.alx3, irangr, 'x');
-        for (int k = 1; k <= nxm; ++k) xp_src[size_t(k - 1)] = irangs(k) - 2;
-        bp.w.resize(size_t(4 * nx), 0.0);
-        H["cxphic"] = bp.w;
-    }
-    H["cxrs"] = sx.w; H["cyrs"] = sy.w; H["czrs"] = sz.w; H["cxsalc"] = bx.w; H["cysalc"] = by.w; H["czsalc"] = bz.w;
+    H["cxsalc"] = hs.xw; H["cysalc"] = hs.yw; H["czsalc"] = hs.zw;
+    H["cxrs"] = sx.w; H["cyrs"] = sy.w; H["czrs"] = sz.w;
 
     const int *pi;
     const double *pd;
@@ -739,12 +786,28 @@ static int setup(afid_ctx *c) {
     if ((rc = upload_vec(c, sx.w, &pd))) return rc; c->d_xs_w = pd;
     if ((rc = upload_vec(c, sy.w, &pd))) return rc; c->d_ys_w = pd;
     if ((rc = upload_vec(c, sz.w, &pd))) return rc; c->d_zs_w = pd;
-    if ((rc = upload_vec(c, xb_src, &pi))) return rc; c->d_xb_src = pi;
-    if ((rc = upload_vec(c, yb_src, &pi))) return rc; c->d_yb_src = pi;
-    if ((rc = upload_vec(c, zb_src, &pi))) return rc; c->d_zb_src = pi;
-    if ((rc = upload_vec(c, bx.w, &pd))) return rc; c->d_xb_w = pd;
-    if ((rc = upload_vec(c, by.w, &pd))) return rc; c->d_yb_w = pd;
-    if ((rc = upload_vec(c, bz.w, &pd))) return rc; c->d_zb_w = pd;
+    // the refined -> coarse tables on the device (FP32 weights too in the mixed form)
+    auto up_tab = [&](const HostTab &h, CoarseTab &t) {
+        const int *q;
+        const double *w;
+        if (rc || (rc = upload_vec(c, h.xs, &q))) return; t.x_src = q;
+        if ((rc = upload_vec(c, h.ys, &q))) return; t.y_src = q;
+        if ((rc = upload_vec(c, h.zs, &q))) return; t.z_src = q;
+        if ((rc = upload_vec(c, h.xw, &w))) return; t.x_w = w;
+        if ((rc = upload_vec(c, h.yw, &w))) return; t.y_w = w;
+        if ((rc = upload_vec(c, h.zw, &w))) return; t.z_w = w;
+        if (c->mixed) {
+            const float *f;
+            std::vector<float> a(h.xw.begin(), h.xw.end()), b(h.yw.begin(), h.yw.end()), d(h.zw.begin(), h.zw.end());
+            if ((rc = upload_vec(c, a, &f))) return; t.x_wf = f;
+            if ((rc = upload_vec(c, b, &f))) return; t.y_wf = f;
+            if ((rc = upload_vec(c, d, &f))) return; t.z_wf = f;
+        }
+        t.general = !bracket;
+    };
+    if (c->sal) up_tab(hs, c->tab_sal);
+    if (c->pf) up_tab(hp, c->tab_phi);
+    if (rc) return rc;
     std::vector<double> udx3mr_v = farr_range(udx3mr, 1, nxmr);
     std::vector<double> xmr_v = farr_range(xmr, 1, nxmr);
     up(xmr_v, &c->d_xmr);
@@ -755,8 +818,6 @@ static int setup(afid_ctx *c) {
     up(lxa, &c->d_lxa);
     if (c->pf) {
         up(c->h_ap3spkr, &c->d_ap3spkr); up(c->h_ac3spkr, &c->d_ac3spkr); up(c->h_am3spkr, &c->d_am3spkr);
-        up(bp.w, &c->d_xp_w);
-        if (!rc) { rc = upload_vec(c, xp_src, &pi); c->d_xp_src = pi; }
     }
     if (rc) return rc;
 
@@ -779,9 +840,7 @@ static int setup(afid_ctx *c) {
             return rc ? rc : (rc = upload_vec(c, w, d));
         };
         upf(sx.w, &c->f_xs_w); upf(sy.w, &c->f_ys_w); upf(sz.w, &c->f_zs_w);
-        upf(bx.w, &c->f_xb_w); upf(by.w, &c->f_yb_w); upf(bz.w, &c->f_zb_w);
         upf(lxa, &c->f_lxa);
-        if (c->pf) upf(bp.w, &c->f_xp_w);
         if (rc) return rc;
     } else if (c->sal) {
         for (int f = AFID_SAL; f <= AFID_VZR; ++f)
@@ -823,20 +882,19 @@ static int setup(afid_ctx *c) {
     };
     c->lds_vy = lds_yz(c->vy_cs, c->vy_co);
     c->lds_vz = lds_yz(c->vz_cs, c->vz_co);
-    {
-        std::vector<int> xb0(xb_src.begin(), xb_src.begin() + nxm);
-        c->salc_xr = span(xb0, nxm, 64);
-        c->salc_yr = span(yb_src, c->ny, SALC_TJ);
-        if (c->salc_yr > 64) return set_err(c, AFID_E_ARG, "refinement ratio too large for the refined -> coarse tiles");
-        c->lds_salc = es * size_t(2 * c->salc_yr * c->salc_xr);
-        if (c->pf) {
-            std::vector<int> xp0(xp_src.begin(), xp_src.begin() + nxm);
-            c->phic_xr = span(xp0, nxm, 64);
-            c->lds_phic = sizeof(double) * size_t(2 * c->salc_yr * c->phic_xr);
-            if (c->lds_phic > 65536) return set_err(c, AFID_E_ARG, "refinement ratio too small for the phic tiles");
-        }
+    // refined -> coarse tiles: 64 coarse x by SALC_TJ coarse y, two refined
+    // planes of the tile in LDS (storage type of the refined field)
+    for (int q = 0; q < 2; ++q) {
+        const HostTab &h = q == 0 ? hs : hp;
+        CoarseTab &t = q == 0 ? c->tab_sal : c->tab_phi;
+        if ((q == 0 && !c->sal) || (q == 1 && !c->pf)) continue;
+        t.xr = span(h.xs, nxm, 64);
+        t.yr = span(h.ys, c->ny, SALC_TJ);
+        if (t.yr > 64) return set_err(c, AFID_E_ARG, "refinement ratio too large for the refined -> coarse tiles");
+        t.lds = es * size_t(2 * t.yr * t.xr);
+        if (t.lds > 65536) return set_err(c, AFID_E_ARG, "refinement ratio too small for the refined -> coarse tiles");
     }
-    if (c->lds_vxr > 65536 || c->lds_vy > 65536 || c->lds_vz > 65536 || c->lds_salc > 65536)
+    if (c->lds_vxr > 65536 || c->lds_vy > 65536 || c->lds_vz > 65536)
         return set_err(c, AFID_E_ARG, "refinement ratio too small for the transfer tiles (LDS plan above 64 KiB)");
     return AFID_OK;
 }
@@ -861,6 +919,7 @@ static int ctx_create(const afid_config *cfg, hipStream_t shared, afid_ctx **out
         return AFID_E_HIP;
     }
     c->device = dev;
+    if (const char *e = std::getenv("AFID_COLUMN_TZ")) c->col_tz = std::atoi(e);
     if (shared) {
         c->stream = shared;
         c->own_stream = false;
@@ -889,6 +948,7 @@ int afid_gpu_init(const afid_config *cfg, afid_ctx **out) { return ctx_create(cf
 int afid_gpu_finalize(afid_ctx *c) {
     if (!c) return AFID_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto &b : c->bufs) (void)hipFree(b.p);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -922,12 +982,22 @@ int afid_upload(afid_ctx *c, int f, const double *host, const int64_t lb[3], con
         for (int j = 1; j <= ny; ++j)
             for (int k = 1; k <= kmax; ++k)
                 slab[size_t(j - 1) * ld + (k - 1)] = host[(k - lb[0]) + ext[0] * ((j - lb[1]) + ext[1] * (int64_t)(i - lb[2]))];
+        if (f == AFID_RUX && c->g[f] && c->d_bref)        // the FP32 history holds H - bref
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < kmax; ++k) slab[size_t(j) * ld + k] -= c->h_bref[size_t(k)];
         if (c->g[f]) {       // FP32 field of the mixed form: rounded to nearest
             for (size_t q = 0; q < slab.size(); ++q) slabf[q] = float(slab[q]);
             HIPCHK(c, hipMemcpy(c->g[f] + size_t(i - 1) * ny * ld, slabf.data(), sizeof(float) * slabf.size(), hipMemcpyHostToDevice));
         } else {
             HIPCHK(c, hipMemcpy(c->f[f] + size_t(i - 1) * ny * ld, slab.data(), sizeof(double) * slab.size(), hipMemcpyHostToDevice));
         }
+    }
+    if (f == AFID_TEMPBP || f == AFID_TEMPTP) {
+        double sum = 0.0;
+        for (int i = 1; i <= nz; ++i)
+            for (int j = 1; j <= ny; ++j) sum += host[(1 - lb[0]) + ext[0] * ((j - lb[1]) + ext[1] * (int64_t)(i - lb[2]))];
+        c->wall_mean[f == AFID_TEMPBP ? 0 : 1] = sum / (double(ny) * nz);
+        c->bref_dirty = true;
     }
     return AFID_OK;
 }
@@ -946,6 +1016,9 @@ int afid_download(afid_ctx *c, int f, double *host, const int64_t lb[3], const i
         if (c->g[f]) {
             HIPCHK(c, hipMemcpy(slabf.data(), c->g[f] + size_t(i) * ny * ld, sizeof(float) * slabf.size(), hipMemcpyDeviceToHost));
             for (size_t q = 0; q < slab.size(); ++q) slab[q] = double(slabf[q]);
+            if (f == AFID_RUX && c->d_bref)
+                for (int j = 0; j < ny; ++j)
+                    for (int k = 0; k < kmax; ++k) slab[size_t(j) * ld + k] += c->h_bref[size_t(k)];
         } else {
             HIPCHK(c, hipMemcpy(slab.data(), c->f[f] + size_t(i) * ny * ld, sizeof(double) * slab.size(), hipMemcpyDeviceToHost));
         }
@@ -964,6 +1037,20 @@ int afid_set_substep(afid_ctx *c, double dt, int ns) {
     c->beta = dt / c->ren * 0.5;
     c->al = c->alm[ns - 1]; c->ga = c->gam[ns - 1]; c->ro = c->rom[ns - 1];
     c->ns = ns;
+    if (c->d_bref && c->bref_dirty && ns == 1) {
+        // the reference buoyancy of the conductive profile T = Tb + (Tt - Tb) x / alx3
+        // at the vx faces: byct active_T (T(xm_k) + T(xm_{k-1})) / 2, k = 1..nxm-1
+        // (0-based); taken at the first substep, whose history weight is zero
+        const std::vector<double> &xm = c->host1d["xm"];
+        const double tb = c->wall_mean[0], tt = c->wall_mean[1];
+        auto tl = [&](double x) { return tb + (tt - tb) * x / c->c.alx3; };
+        c->h_bref.assign(size_t(c->ldx), 0.0);
+        for (int k = 1; k < c->nxm; ++k)
+            c->h_bref[size_t(k)] = c->byct * (c->c.active_t * 0.5 * (tl(xm[size_t(k)]) + tl(xm[size_t(k - 1)])));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(c->d_bref, c->h_bref.data(), sizeof(double) * c->h_bref.size(), hipMemcpyHostToDevice));
+        c->bref_dirty = false;
+    }
     return ensure_factors(c);
 }
 
@@ -994,6 +1081,77 @@ int afid_add_sal_buoyancy(afid_ctx *c) {
 
 }  // extern "C"
 
+// immersed-boundary solve (SolveImpEqnUpdate_*_ibm) of slot 0..4
+template <int CH>
+static void launch_tri_ibm(afid_ctx *c, const TriFactors &f, const IbmJob &jb, const double *rhs, double *q) {
+    const unsigned nb = unsigned(((long long)jb.ny * jb.nz + 3) / 4);
+    hipLaunchKernelGGL((k_tri_update_ibm<CH>), dim3(nb), dim3(256), 0, c->stream, f, jb, rhs, q);
+}
+static int run_tri_ibm(afid_ctx *c, int slot, int kind, const double *rhs, double *q) {
+    TriFactors f = factors(c, kind);
+    const bool ref = slot == 4;
+    IbmJob jb{c->ibm_mask[slot], c->ibm_dist[slot], 1.0, slot, ref ? c->nxmr : c->nxm, ref ? c->ldxr : c->ldx,
+              ref ? c->nyr : c->ny, ref ? c->nzr : c->nz, ref ? (long long)c->nyr * c->ldxr : (long long)c->ny * c->ldx};
+    switch (f.chunk) {
+#define AFID_TI_CASE(CH) case CH: launch_tri_ibm<CH>(c, f, jb, rhs, q); break;
+        AFID_TI_CASE(1) AFID_TI_CASE(2) AFID_TI_CASE(3) AFID_TI_CASE(4) AFID_TI_CASE(5) AFID_TI_CASE(6)
+        AFID_TI_CASE(7) AFID_TI_CASE(8) AFID_TI_CASE(9) AFID_TI_CASE(10) AFID_TI_CASE(11) AFID_TI_CASE(12)
+        AFID_TI_CASE(13) AFID_TI_CASE(14) AFID_TI_CASE(15) AFID_TI_CASE(16)
+#undef AFID_TI_CASE
+    default: return set_err(c, AFID_E_ARG, "wall-normal size above 1024 points is not supported");
+    }
+    return check_launch(c, "k_tri_update_ibm");
+}
+
+extern "C" {
+
+// ibm_param.F90: the mask and distances of one solve (see afid_gpu.h)
+int afid_set_ibm(afid_ctx *c, int field, const int32_t *mask, const double *dist, int64_t ndist) {
+    if (!c) return AFID_E_ARG;
+    const int slot = field == AFID_VX ? 0 : field == AFID_VY ? 1 : field == AFID_VZ ? 2 : field == AFID_TEMP ? 3 : field == AFID_SAL ? 4 : -1;
+    if (slot < 0) return set_err(c, AFID_E_ARG, "afid_set_ibm: field must be AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP or AFID_SAL");
+    if (slot == 4 && !c->sal) return set_err(c, AFID_E_STATE, "salinity is not enabled");
+    if (c->mixed) return set_err(c, AFID_E_STATE, "the immersed-boundary solves belong to the FP64 call tree");
+    int rc;
+    if (!mask) {                       // switch this solve back to the plain form
+        c->ibm_mask[slot] = nullptr;
+        c->ibm_dist[slot] = nullptr;
+        return AFID_OK;
+    }
+    const bool ref = slot == 4;
+    const int nx = ref ? c->nxr : c->nx, ny = ref ? c->nyr : c->ny, nz = ref ? c->nzr : c->nz, ld = ref ? c->ldxr : c->ldx;
+    const int nrow = ref ? c->nxmr : c->nxm;
+    std::vector<signed char> m(size_t(nz) * ny * ld, 2);
+    std::vector<double> d(m.size(), 0.0);
+    int64_t q = 0;
+    // the reference consumes the distances in its loop order: ic, jc, kc = 1..nxm
+    for (int i = 0; i < nz; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nx; ++k) {
+                const int32_t v = mask[(size_t(i) * ny + j) * nx + k];
+                if (v != 2 && v != 0 && v != 1 && v != -1) return set_err(c, AFID_E_ARG, "afid_set_ibm: mask values are 2, 0, 1 or -1");
+                const size_t o = (size_t(i) * ny + j) * ld + k;
+                m[o] = (signed char)v;
+                if (k < nrow && (v == 1 || v == -1)) {
+                    if (!ref) {
+                        if (!dist || q >= ndist) return set_err(c, AFID_E_ARG, "afid_set_ibm: fewer distances than boundary points");
+                        d[o] = dist[q];
+                    }
+                    ++q;
+                }
+            }
+    if (!c->ibm_mask[slot]) {
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->ibm_mask[slot]), m.size()))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->ibm_dist[slot]), sizeof(double) * d.size()))) return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->ibm_mask[slot], m.data(), m.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->ibm_dist[slot], d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
+    return AFID_OK;
+}
+
+}  // extern "C"
+
 // masked velocity solve of the phase field (SolveImpEqnUpdate_X_pf / _YZ_pf)
 template <int CH, int AX>
 static void launch_tri_pf(afid_ctx *c, const TriFactors &f, const double *rhs, double *q) {
@@ -1020,35 +1178,41 @@ extern "C" {
 int afid_implicit_and_update_vx(afid_ctx *c) {
     NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_vx, false, "k_rhs_vx");
-    if (!rc) rc = c->pf ? run_tri_pf<0>(c, MAT_X, c->rhs4[0], c->f[AFID_VX]) : run_tri(c, MAT_X, c->rhs4[0], c->f[AFID_VX]);
+    if (!rc) rc = c->ibm_mask[0] ? run_tri_ibm(c, 0, MAT_X, c->rhs4[0], c->f[AFID_VX])
+                  : c->pf ? run_tri_pf<0>(c, MAT_X, c->rhs4[0], c->f[AFID_VX]) : run_tri(c, MAT_X, c->rhs4[0], c->f[AFID_VX]);
     c->f[AFID_RHS] = c->rhs4[0];
     return rc;
 }
 int afid_implicit_and_update_vy(afid_ctx *c) {
     NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_vy, false, "k_rhs_vy");
-    if (!rc) rc = c->pf ? run_tri_pf<1>(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]) : run_tri(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]);
+    if (!rc) rc = c->ibm_mask[1] ? run_tri_ibm(c, 1, MAT_YZ, c->rhs4[1], c->f[AFID_VY])
+                  : c->pf ? run_tri_pf<1>(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]) : run_tri(c, MAT_YZ, c->rhs4[1], c->f[AFID_VY]);
     c->f[AFID_RHS] = c->rhs4[1];
     return rc;
 }
 int afid_implicit_and_update_vz(afid_ctx *c) {
     NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_vz, false, "k_rhs_vz");
-    if (!rc) rc = c->pf ? run_tri_pf<2>(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]) : run_tri(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]);
+    if (!rc) rc = c->ibm_mask[2] ? run_tri_ibm(c, 2, MAT_YZ, c->rhs4[2], c->f[AFID_VZ])
+                  : c->pf ? run_tri_pf<2>(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]) : run_tri(c, MAT_YZ, c->rhs4[2], c->f[AFID_VZ]);
     c->f[AFID_RHS] = c->rhs4[2];
     return rc;
 }
 int afid_implicit_and_update_temp(afid_ctx *c) {
     NEED_SUBSTEP(c) NEED_FP64(c)
     int rc = launch_point(c, k_rhs_temp, false, "k_rhs_temp");
-    if (!rc) rc = run_tri(c, MAT_T, c->rhs4[3], c->f[AFID_TEMP]);
+    // ImplicitAndUpdateTemp.F90:69-70: the immersed-boundary solve unless the phase field is on
+    if (!rc) rc = c->ibm_mask[3] && !c->pf ? run_tri_ibm(c, 3, MAT_T, c->rhs4[3], c->f[AFID_TEMP])
+                                           : run_tri(c, MAT_T, c->rhs4[3], c->f[AFID_TEMP]);
     c->f[AFID_RHS] = c->rhs4[3];
     return rc;
 }
 int afid_implicit_salinity(afid_ctx *c) {
     NEED_SUBSTEP(c) NEED_FP64(c) NEED_SAL(c)
     int rc = launch_point(c, k_rhs_sal, true, "k_rhs_sal");
-    if (!rc) rc = run_tri(c, MAT_S, c->f[AFID_RHSR], c->f[AFID_SAL]);
+    if (!rc) rc = c->ibm_mask[4] ? run_tri_ibm(c, 4, MAT_S, c->f[AFID_RHSR], c->f[AFID_SAL])     // salinity.F90:403-404
+                                 : run_tri(c, MAT_S, c->f[AFID_RHSR], c->f[AFID_SAL]);
     return rc;
 }
 
@@ -1188,12 +1352,16 @@ static int interp_vel(afid_ctx *c) {
 int afid_interp_vel_mgrd(afid_ctx *c) { NEED_SAL(c) return interp_vel(c); }
 
 // refined -> coarse cell centres (k_interp_coarse_tile)
-static int to_coarse(afid_ctx *c, const CoarseJob &jb, int xr, size_t lds, const char *timer) {
+// (FP32 weights of the table for the mixed form's FP32 refined fields)
+static int to_coarse(afid_ctx *c, CoarseJob jb, const CoarseTab &t, const char *timer) {
     KP p = make_kp(c);
+    jb.x_src = t.x_src; jb.y_src = t.y_src; jb.z_src = t.z_src;
+    if (c->mixed) { jb.x_w = t.x_wf; jb.y_w = t.y_wf; jb.z_w = t.z_wf; }
+    else { jb.x_w = t.x_w; jb.y_w = t.y_w; jb.z_w = t.z_w; }
     const dim3 gr(unsigned((c->nxm + 63) / 64), unsigned((c->ny + SALC_TJ - 1) / SALC_TJ), unsigned((c->nz + SALC_TZ - 1) / SALC_TZ));
     tstart(c, timer);
-    if (c->mixed) hipLaunchKernelGGL(k_interp_coarse_tile<float>, gr, dim3(256), lds, c->stream, p, jb, xr, c->salc_yr);
-    else hipLaunchKernelGGL(k_interp_coarse_tile<double>, gr, dim3(256), lds, c->stream, p, jb, xr, c->salc_yr);
+    if (c->mixed) hipLaunchKernelGGL(k_interp_coarse_tile<float>, gr, dim3(256), t.lds, c->stream, p, jb, t.xr, t.yr);
+    else hipLaunchKernelGGL(k_interp_coarse_tile<double>, gr, dim3(256), t.lds, c->stream, p, jb, t.xr, t.yr);
     tstop(c);
     return check_launch(c, "k_interp_coarse_tile");
 }
@@ -1202,11 +1370,10 @@ int afid_interp_sal_multigrid(afid_ctx *c) {
     NEED_SAL(c)
     CoarseJob jb{};
     jb.mode = CM_SAL;
-    if (c->mixed) { jb.src = c->g[AFID_SAL]; jb.out = c->g[AFID_SALC]; jb.x_w = c->f_xb_w; }
-    else { jb.src = c->f[AFID_SAL]; jb.out = c->f[AFID_SALC]; jb.x_w = c->d_xb_w; }
-    jb.x_src = c->d_xb_src;
+    if (c->mixed) { jb.src = c->g[AFID_SAL]; jb.out = c->g[AFID_SALC]; }
+    else { jb.src = c->f[AFID_SAL]; jb.out = c->f[AFID_SALC]; }
     jb.scale = 1.0;
-    return to_coarse(c, jb, c->salc_xr, c->lds_salc, "mgrd_interp_sal");
+    return to_coarse(c, jb, c->tab_sal, "mgrd_interp_sal");
 }
 
 #define NEED_PF(c) if (!(c)->pf) return set_err((c), AFID_E_STATE, "the phase field is not enabled");
@@ -1221,22 +1388,20 @@ int afid_implicit_phase(afid_ctx *c) {
 }
 static int latent_heat(afid_ctx *c, double *out, int accumulate) {
     CoarseJob jb{};
-    jb.mode = CM_LAT; jb.out = out; jb.x_src = c->d_xp_src;
-    if (c->mixed) { jb.src = c->g[AFID_RHSR]; jb.x_w = c->f_xp_w; }
-    else { jb.src = c->f[AFID_RHSR]; jb.x_w = c->d_xp_w; }
+    jb.mode = CM_LAT; jb.out = out;
+    jb.src = c->mixed ? static_cast<const void *>(c->g[AFID_RHSR]) : static_cast<const void *>(c->f[AFID_RHSR]);
     jb.scale = c->c.pf_s / (c->al * c->dt);          // pf_S d(phi)/(al dt), phasefield.F90:666,701
     jb.accumulate = accumulate;
-    return to_coarse(c, jb, c->phic_xr, c->lds_phic, "pf_latent_heat");
+    return to_coarse(c, jb, c->tab_phi, "pf_latent_heat");
 }
 int afid_add_latent_heat(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) NEED_PF(c) return latent_heat(c, c->f[AFID_HRO], 1); }
 int afid_interp_phi_multigrid(afid_ctx *c) {
     NEED_PF(c)
     CoarseJob jb{};
-    jb.mode = CM_PHI; jb.out = c->f[AFID_PHIC]; jb.x_src = c->d_xp_src;
-    if (c->mixed) { jb.src = c->g[AFID_PHI]; jb.x_w = c->f_xp_w; }
-    else { jb.src = c->f[AFID_PHI]; jb.x_w = c->d_xp_w; }
+    jb.mode = CM_PHI; jb.out = c->f[AFID_PHIC];
+    jb.src = c->mixed ? static_cast<const void *>(c->g[AFID_PHI]) : static_cast<const void *>(c->f[AFID_PHI]);
     jb.scale = 1.0;
-    return to_coarse(c, jb, c->phic_xr, c->lds_phic, "pf_interp_phi");
+    return to_coarse(c, jb, c->tab_phi, "pf_interp_phi");
 }
 int afid_interp_temp_multigrid(afid_ctx *c) {
     NEED_PF(c)
@@ -1252,25 +1417,42 @@ int afid_interp_temp_multigrid(afid_ctx *c) {
 }  // extern "C"
 
 // fused column kernels of the step (one sweep per grid, ping-pong buffers)
+// LDS of a base-column workgroup: 2 (3 with the phase field) column buffers
+// per wavefront, 4 x TZ wavefronts; one workgroup must fit the CU's 160 KiB
+template <int CH, bool PF, int TZ>
+constexpr bool column_tile_fits() { return size_t(PF ? 3 : 2) * 4 * TZ * lds_col_size<CH>() * sizeof(double) <= 160 * 1024; }
+
+template <int CH, typename HT, bool PF, int TZ>
+static void launch_column_base_tz(afid_ctx *c, const KP &p) {
+    if constexpr (column_tile_fits<CH, PF, TZ>()) {
+        BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
+        const long long nblk = strip_blocks<TZ>(c->ny, c->nz);
+        hipLaunchKernelGGL((k_column_base<CH, HT, PF, TZ>), dim3(unsigned((nblk + 7) / 8 * 8)), dim3(256 * TZ), 0, c->stream, p,
+                           factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
+    } else {
+        launch_column_base_tz<CH, HT, PF, TZ / 2>(c, p);      // TZ 1 always fits (CH <= 16)
+    }
+}
+// tile depth: 4 unless the LDS does not fit (then 2, then 1); AFID_COLUMN_TZ overrides it
+template <int CH, typename HT, bool PF>
+static void launch_column_base_form(afid_ctx *c, const KP &p) {
+    const int tz = c->col_tz > 0 ? c->col_tz : 4;
+    if (tz >= 4) launch_column_base_tz<CH, HT, PF, 4>(c, p);
+    else if (tz >= 2) launch_column_base_tz<CH, HT, PF, 2>(c, p);
+    else launch_column_base_tz<CH, HT, PF, 1>(c, p);
+}
 template <int CH>
-static void launch_column_base(afid_ctx *c, const KP &p, long long nblk) {
-    BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
-    const dim3 gr(unsigned((nblk + 7) / 8 * 8));
-    if (c->mixed && c->pf)
-        hipLaunchKernelGGL((k_column_base<CH, float, true>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
-    else if (c->mixed)
-        hipLaunchKernelGGL((k_column_base<CH, float, false>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
-    else if (c->pf)
-        hipLaunchKernelGGL((k_column_base<CH, double, true>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
-    else
-        hipLaunchKernelGGL((k_column_base<CH, double, false>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
+static void launch_column_base(afid_ctx *c, const KP &p) {
+    if (c->mixed && c->pf) launch_column_base_form<CH, float, true>(c, p);
+    else if (c->mixed) launch_column_base_form<CH, float, false>(c, p);
+    else if (c->pf) launch_column_base_form<CH, double, true>(c, p);
+    else launch_column_base_form<CH, double, false>(c, p);
 }
 static int column_base(afid_ctx *c) {
     KP p = make_kp(c);
-    const long long nblk = strip_blocks(c->ny, c->nz);
     const int ch = (c->nx + 63) / 64;
     switch (ch) {
-#define AFID_CB_CASE(CH) case CH: launch_column_base<CH>(c, p, nblk); break;
+#define AFID_CB_CASE(CH) case CH: launch_column_base<CH>(c, p); break;
         AFID_CB_CASE(1) AFID_CB_CASE(2) AFID_CB_CASE(3) AFID_CB_CASE(4) AFID_CB_CASE(5) AFID_CB_CASE(6)
         AFID_CB_CASE(7) AFID_CB_CASE(8) AFID_CB_CASE(9) AFID_CB_CASE(10) AFID_CB_CASE(11) AFID_CB_CASE(12)
         AFID_CB_CASE(13) AFID_CB_CASE(14) AFID_CB_CASE(15) AFID_CB_CASE(16)
@@ -1340,6 +1522,8 @@ extern "C" {
 // TimeMarcher.F90:13-165
 int afid_time_marcher(afid_ctx *c, double dt) {
     if (!c) return AFID_E_ARG;
+    for (int q = 0; q < 5; ++q)
+        if (c->ibm_mask[q]) return set_err(c, AFID_E_STATE, "the immersed-boundary solves run through the per-routine call tree");
     for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     c->timers.clear();
     int rc;
@@ -1378,6 +1562,47 @@ int afid_time_marcher(afid_ctx *c, double dt) {
             if ((rc = afid_interp_phi_multigrid(c))) return rc;
         }
     }
+    return AFID_OK;
+}
+
+// afid_time_marcher replayed from a captured graph.  Two steps are captured
+// (kernel nodes only: the step holds no memset or copy; the Poisson gates and
+// the refinement slots live on the device) and replayed nsteps / 2 times; an
+// odd last step runs eagerly.  A new dt (the factors and the substep
+// scalars are baked into the nodes) or moved buffers recapture.
+int afid_time_marcher_graph(afid_ctx *c, double dt, int nsteps) {
+    if (!c || nsteps < 0) return AFID_E_ARG;
+    int rc;
+    bool same = c->graph && c->graph_dt == dt;
+    for (int f = 0; same && f < AFID_NFIELDS; ++f) same = c->graph_ptrs[f] == c->f[f];
+    if (!same && nsteps >= 2) {
+        if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+        // the factors of dt are computed and copied before the capture
+        if ((rc = afid_set_substep(c, dt, 1))) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const bool timing = c->timing;
+        c->timing = false;
+        for (auto &t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+        c->timers.clear();
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = afid_time_marcher(c, dt);
+        if (!rc) rc = afid_time_marcher(c, dt);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        c->timing = timing;
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (ec != hipSuccess) return set_err(c, AFID_E_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) { c->graph = nullptr; return set_err(c, AFID_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei)); }
+        c->graph_dt = dt;
+        for (int f = 0; f < AFID_NFIELDS; ++f) c->graph_ptrs[f] = c->f[f];
+    }
+    int done = 0;
+    if (c->graph)
+        for (; done + 2 <= nsteps; done += 2) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    for (; done < nsteps; ++done)
+        if ((rc = afid_time_marcher(c, dt))) return rc;
     return AFID_OK;
 }
 
@@ -1434,6 +1659,210 @@ int afid_create_initial_conditions(afid_ctx *c, uint64_t seed, double eps) {
     // salc from the initial S (the reference interpolates at start-up, main.F90:206-213)
     return afid_interp_sal_multigrid(c);
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// restart onto a new grid (ReadFlowInterp.F90:90-141, IC_interpolation/)
+// ---------------------------------------------------------------------------
+// device buffers of one restart call, freed after the stream has drained
+struct RestartScratch {
+    afid_ctx *c;
+    std::vector<void *> p;
+    explicit RestartScratch(afid_ctx *ctx) : c(ctx) {}
+    ~RestartScratch() {
+        (void)hipStreamSynchronize(c->stream);
+        for (void *q : p) (void)hipFree(q);
+    }
+    template <typename T>
+    int alloc(size_t n, T **out) {
+        void *q = nullptr;
+        const hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return set_err(c, AFID_E_MEM, std::string("restart scratch: ") + hipGetErrorString(e));
+        p.push_back(q);
+        *out = static_cast<T *>(q);
+        return AFID_OK;
+    }
+    template <typename T>
+    int upload(const std::vector<T> &v, const T **out) {
+        T *q;
+        int rc = alloc(v.size(), &q);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpyAsync(q, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream));
+        *out = q;
+        return AFID_OK;
+    }
+};
+
+// CreateOldGrid.F90:11-115 (one grid of the old run) and the old -> new
+// cell-centre tables of CreateNewInputStencil.F90 / CreateNewSalStencil.F90.
+// Old cell counts nxmo x nymo x nzmo; returns the old faces xco (0-based).
+static int restart_tables(afid_ctx *c, RestartScratch &sc, int nxmo, int nymo, int nzmo, int istr3o, double str3o,
+                          bool refined, RestartTabs &t, const double **xco_dev) {
+    const afid_config &g = c->c;
+    if (nxmo < 2 || nymo < 1 || nzmo < 1) return set_err(c, AFID_E_ARG, "restart: the old grid is too small");
+    int opt = istr3o;
+    if (!refined && g.istr3 == 5) opt = 5;     // CreateOldGrid.F90:77 tests the current istr3
+    else if (!refined && istr3o == 5) return set_err(c, AFID_E_ARG, "restart: old grid option 5 needs istr3 = 5 (the reference leaves it unset)");
+    FArr xco, xmo;
+    try {
+        // option 3 takes the parameter 1.0 on both old grids (CreateOldGrid.F90:62-67)
+        make_wall_grid(opt, nxmo, g.alx3, str3o, opt == 5 ? g.dpdy : 1.0, xco, xmo);
+    } catch (const std::exception &e) {
+        return set_err(c, AFID_E_ARG, std::string("restart: ") + e.what());
+    }
+    auto uni = [](int n, double L) {
+        FArr m(1, n);
+        for (int i = 1; i <= n; ++i) m(i) = 0.5 * (L * double(i - 1) / n + L * double(i) / n);
+        return m;
+    };
+    const int nxn = refined ? c->nxmr : c->nxm, nyn = refined ? c->nyr : c->ny, nzn = refined ? c->nzr : c->nz;
+    const std::vector<double> &xmv = c->host1d[refined ? "xmr" : "xm"];
+    FArr xn(1, nxn), xo(1, nxmo);
+    for (int k = 1; k <= nxn; ++k) xn(k) = xmv[size_t(k - 1)];
+    for (int k = 1; k <= nxmo; ++k) xo(k) = xmo(k);
+    FArr yo = uni(nymo, g.ylen), zo = uni(nzmo, g.zlen), yn = uni(nyn, g.ylen), zn = uni(nzn, g.zlen);
+    Stencil sx, sy, sz;
+    try {
+        sx = construct_stencil(xo, nxmo, xn, nxn, g.alx3, interpolation_indices(xo, nxmo, xn, nxn, g.alx3), 'x');
+        sy = construct_stencil(yo, nymo, yn, nyn, g.ylen, interpolation_indices(yo, nymo, yn, nyn, g.ylen), 'y');
+        sz = construct_stencil(zo, nzmo, zn, nzn, g.zlen, interpolation_indices(zo, nzmo, zn, nzn, g.zlen), 'z');
+    } catch (const std::exception &e) {
+        return set_err(c, AFID_E_ARG, std::string("restart: ") + e.what());
+    }
+    int rc;
+    if ((rc = sc.upload(sx.src, &t.xs)) || (rc = sc.upload(sy.src, &t.ys)) || (rc = sc.upload(sz.src, &t.zs))) return rc;
+    if ((rc = sc.upload(sx.w, &t.xw)) || (rc = sc.upload(sy.w, &t.yw)) || (rc = sc.upload(sz.w, &t.zw))) return rc;
+    return sc.upload(farr_range(xco, 1, nxmo + 1), xco_dev);
+}
+
+static int check_old_grid(afid_ctx *c, const afid_old_grid *og, bool refined) {
+    const int nx = refined ? og->nxro : og->nxo, ny = refined ? og->nyro : og->nyo, nz = refined ? og->nzro : og->nzo;
+    if (nx < 3 || ny < 2 || nz < 2 || nx > 4097 || ny > 65536 || nz > 65536)
+        return set_err(c, AFID_E_ARG, "restart: old grid sizes out of range");
+    if (std::fabs(og->ylen - c->c.ylen) > 1e-8 || std::fabs(og->zlen - c->c.zlen) > 1e-8)    // ReadFlowInterp.F90:76-82
+        return set_err(c, AFID_E_ARG, "restart: continua domain size does not match the configuration");
+    return AFID_OK;
+}
+
+extern "C" {
+
+// InterpInputVel.F90:1-350 (with CreateOldGrid and CreateNewInputStencil)
+int afid_interp_input_vel(afid_ctx *c, const afid_old_grid *og, const double *vxo, const double *vyo, const double *vzo,
+                          const double *tempo) {
+    if (!c || !og || !vxo || !vyo || !vzo || !tempo) return AFID_E_ARG;
+    int rc;
+    if ((rc = check_old_grid(c, og, false))) return rc;
+    const int nxo = og->nxo, nyo = og->nyo - 1, nzo = og->nzo - 1, nxmo = nxo - 1;
+    RestartScratch sc(c);
+    RestartTabs t{};
+    const double *xco;
+    if ((rc = restart_tables(c, sc, nxmo, nyo, nzo, og->istr3o, og->str3o, false, t, &xco))) return rc;
+    const size_t nold = size_t(nxo) * nyo * nzo, next = size_t(nxo + 3) * nyo * nzo;
+    const size_t nb = size_t(c->nz) * c->ny * c->ldx;
+    double *F, *E, *G;
+    const double *lxa;
+    if ((rc = sc.alloc(nold, &F)) || (rc = sc.alloc(next, &E)) || (rc = sc.alloc(nb, &G))) return rc;
+    {
+        const std::vector<double> &xc = c->host1d["xc"];
+        std::vector<double> l;
+        for (int k = 0; k < c->nxm; ++k) l.push_back(xc[size_t(k + 1)] - xc[size_t(k)]);   // lxa, InterpInputVel.F90:75
+        if ((rc = sc.upload(l, &lxa))) return rc;
+    }
+    // the new state starts from zero (pr and the RK history are not in the checkpoint)
+    for (int f : {AFID_VX, AFID_VY, AFID_VZ, AFID_TEMP, AFID_PR, AFID_RUX, AFID_RUY, AFID_RUZ, AFID_RUTEMP}) {
+        if (c->g[f]) HIPCHK(c, hipMemsetAsync(c->g[f], 0, sizeof(float) * nb, c->stream));
+        else if (c->f[f]) HIPCHK(c, hipMemsetAsync(c->f[f], 0, sizeof(double) * nb, c->stream));
+    }
+    const double dyo = double(nyo) / c->c.ylen, dzo = double(nzo) / c->c.zlen;
+    RestartGhost vel{0, double(1 - 2 * c->c.inslws), double(1 - 2 * c->c.inslwn), 0, 0, 0.0, 0.0};
+    const dim3 sgr(unsigned((nxo + 3 + 255) / 256), unsigned(nyo), unsigned(nzo));
+    const dim3 tgr = point_grid(c->nxm, c->ny, c->nz);
+    const double *old[3] = {vxo, vyo, vzo};
+    double *vel_out[3] = {c->f[AFID_VX], c->f[AFID_VY], c->f[AFID_VZ]};
+    for (int comp = 0; comp < 3; ++comp) {
+        HIPCHK(c, hipMemcpyAsync(F, old[comp], sizeof(double) * nold, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_restart_stage, sgr, dim3(256), 0, c->stream, F, comp, nxo, nyo, nzo, xco, dyo, dzo, vel, E);
+        hipLaunchKernelGGL(k_restart_transfer, tgr, kPointBlock, 0, c->stream, E, nxo, nyo, nzo, t, G, c->nxm, c->ny, c->nz,
+                           c->ldx, (long long)c->ny * c->ldx);
+        if ((rc = check_launch(c, "k_restart_transfer"))) return rc;
+        if (comp == 0) {
+            hipLaunchKernelGGL(k_restart_integrate, dim3(unsigned((c->ny * c->nz + 255) / 256)), dim3(256), 0, c->stream,
+                               vel_out[0], G, 0, c->nxm, c->ny, c->nz, c->ldx, (long long)c->ny * c->ldx, lxa, 0.0);
+        } else {
+            const int nm = comp == 1 ? c->nz : c->ny;
+            hipLaunchKernelGGL(k_restart_plane, dim3(unsigned((c->nxm + 63) / 64), unsigned((nm + 3) / 4)), kPointBlock, 0,
+                               c->stream, F, comp, nxo, nyo, nzo, t, vel, vel_out[comp], c->nxm, nm, c->ldx, (long long)c->ny * c->ldx);
+            const int lines = comp == 1 ? c->nxm * c->nz : c->nxm * c->ny;
+            hipLaunchKernelGGL(k_restart_integrate, dim3(unsigned((lines + 255) / 256)), dim3(256), 0, c->stream, vel_out[comp],
+                               G, comp, c->nxm, c->ny, c->nz, c->ldx, (long long)c->ny * c->ldx, lxa,
+                               comp == 1 ? c->c.ylen / double(c->ny) : c->c.zlen / double(c->nz));
+        }
+        if ((rc = check_launch(c, "k_restart_integrate"))) return rc;
+    }
+    // T with its wall ghosts (InterpInputVel.F90:280-330)
+    double tup, tlo;
+    if (c->c.rayt > 0) {
+        if (c->c.inslwn == 0) { tup = 0.0; tlo = 1.0; } else { tup = -0.5; tlo = 0.5; }
+    } else { tup = 0.5; tlo = -0.5; }
+    if (c->pf) {
+        if (c->c.rayt > 0) { tup = 0.0; tlo = 1.0; } else { tup = 1.0; tlo = 0.0; }
+    }
+    RestartGhost sg{1, 0.0, 0.0, c->c.tfixs, c->c.tfixn, tlo, tup};
+    HIPCHK(c, hipMemcpyAsync(F, tempo, sizeof(double) * nold, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_restart_stage, sgr, dim3(256), 0, c->stream, F, 3, nxo, nyo, nzo, xco, dyo, dzo, sg, E);
+    hipLaunchKernelGGL(k_restart_transfer, tgr, kPointBlock, 0, c->stream, E, nxo, nyo, nzo, t, c->f[AFID_TEMP], c->nxm, c->ny,
+                       c->nz, c->ldx, (long long)c->ny * c->ldx);
+    if ((rc = check_launch(c, "k_restart_transfer"))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AFID_OK;
+}
+
+}  // extern "C"
+
+// InterpInputSal.F90 (phi = false) and InterpInputPhi.F90 (phi = true)
+static int interp_input_refined(afid_ctx *c, const afid_old_grid *og, const double *fo, bool phi) {
+    if (!c || !og || !fo) return AFID_E_ARG;
+    if (phi ? !c->pf : !c->sal) return set_err(c, AFID_E_STATE, phi ? "the phase field is not enabled" : "salinity is not enabled");
+    int rc;
+    if ((rc = check_old_grid(c, og, true))) return rc;
+    const int nxo = og->nxro, nyo = og->nyro - 1, nzo = og->nzro - 1, nxmo = nxo - 1;
+    RestartScratch sc(c);
+    RestartTabs t{};
+    const double *xco;
+    if ((rc = restart_tables(c, sc, nxmo, nyo, nzo, og->istr3ro, og->str3o, true, t, &xco))) return rc;
+    const size_t nold = size_t(nxo) * nyo * nzo, next = size_t(nxo + 3) * nyo * nzo;
+    const size_t nr = size_t(c->nzr) * c->nyr * c->ldxr;
+    const int fid = phi ? AFID_PHI : AFID_SAL, hid = phi ? AFID_RUPHI : AFID_RUSAL;
+    double *F, *E, *out = c->f[fid];
+    if ((rc = sc.alloc(nold, &F)) || (rc = sc.alloc(next, &E))) return rc;
+    if (!out && (rc = sc.alloc(nr, &out))) return rc;         // FP32 storage: staged in FP64
+    HIPCHK(c, hipMemsetAsync(out, 0, sizeof(double) * nr, c->stream));
+    if (c->g[hid]) HIPCHK(c, hipMemsetAsync(c->g[hid], 0, sizeof(float) * nr, c->stream));
+    else if (c->f[hid]) HIPCHK(c, hipMemsetAsync(c->f[hid], 0, sizeof(double) * nr, c->stream));
+    // S: Dirichlet ghosts from the sign of Ra_S, 1 / 0 with the phase field
+    // (InterpInputSal.F90:33-56); phi: zero gradient (InterpInputPhi.F90:28-33)
+    double sup = c->c.rays > 0 ? 0.5 : -0.5, slo = c->c.rays > 0 ? -0.5 : 0.5;
+    if (c->pf) { sup = 0.0; slo = 1.0; }
+    RestartGhost gh{1, 0.0, 0.0, phi ? 0 : c->c.sfixs, phi ? 0 : c->c.sfixn, slo, sup};
+    HIPCHK(c, hipMemcpyAsync(F, fo, sizeof(double) * nold, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_restart_stage, dim3(unsigned((nxo + 3 + 255) / 256), unsigned(nyo), unsigned(nzo)), dim3(256), 0, c->stream,
+                       F, 3, nxo, nyo, nzo, xco, 0.0, 0.0, gh, E);
+    hipLaunchKernelGGL(k_restart_transfer, point_grid(c->nxmr, c->nyr, c->nzr), kPointBlock, 0, c->stream, E, nxo, nyo, nzo, t, out,
+                       c->nxmr, c->nyr, c->nzr, c->ldxr, (long long)c->nyr * c->ldxr);
+    if ((rc = check_launch(c, "k_restart_transfer"))) return rc;
+    if (!c->f[fid]) {
+        hipLaunchKernelGGL(k_restart_to_float, dim3(unsigned((nr + 255) / 256)), dim3(256), 0, c->stream, out, c->g[fid], (long long)nr);
+        if ((rc = check_launch(c, "k_restart_to_float"))) return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AFID_OK;
+}
+
+extern "C" {
+
+int afid_interp_input_sal(afid_ctx *c, const afid_old_grid *og, const double *salo) { return interp_input_refined(c, og, salo, false); }
+int afid_interp_input_phi(afid_ctx *c, const afid_old_grid *og, const double *phio) { return interp_input_refined(c, og, phio, true); }
 
 int afid_synchronize(afid_ctx *c) {
     if (!c) return AFID_E_ARG;

@@ -303,20 +303,24 @@ __device__ __forceinline__ long long xcd_block(long long) {
     const long long per = gridDim.x / 8;
     return (b % 8) * per + b / 8;
 }
-// Column walk: strips of STRIP_BLOCKS*4 columns in y, walked along z, so the
-// y and z neighbour columns a workgroup reads were touched by workgroups that
-// ran just before it on the same XCD (still in its L2).
+// Column walk: a workgroup owns a tile of 4 (y) x TZ (z) columns, one per
+// wavefront; tiles form strips of STRIP_BLOCKS*4 columns in y, walked along
+// z, so the y and z neighbour columns a wavefront reads belong to its own
+// workgroup (inside the tile) or to workgroups that ran just before it on
+// the same XCD (still in its L2).
 constexpr int STRIP_BLOCKS = 4;
+template <int TZ = 1>
 __host__ __device__ __forceinline__ long long strip_blocks(int ny, int nz) {
-    return (long long)((ny + 4 * STRIP_BLOCKS - 1) / (4 * STRIP_BLOCKS)) * STRIP_BLOCKS * nz;
+    return (long long)((ny + 4 * STRIP_BLOCKS - 1) / (4 * STRIP_BLOCKS)) * STRIP_BLOCKS * ((nz + TZ - 1) / TZ);
 }
 // (j, i) of this wavefront's column; false if the slot is idle
+template <int TZ = 1>
 __device__ __forceinline__ bool strip_column(long long blk, int wave, int ny, int nz, int &j, int &i) {
-    const long long per_strip = (long long)STRIP_BLOCKS * nz;
+    const long long per_strip = (long long)STRIP_BLOCKS * ((nz + TZ - 1) / TZ);
     const long long strip = blk / per_strip, rem = blk % per_strip;
-    i = int(rem / STRIP_BLOCKS);
-    j = int(strip * 4 * STRIP_BLOCKS + (rem % STRIP_BLOCKS) * 4 + wave);
-    const bool ok = blk < strip_blocks(ny, nz) && j < ny;
+    i = int(rem / STRIP_BLOCKS) * TZ + (wave >> 2);
+    j = int(strip * 4 * STRIP_BLOCKS + (rem % STRIP_BLOCKS) * 4 + (wave & 3));
+    const bool ok = blk < strip_blocks<TZ>(ny, nz) && j < ny && i < nz;
     if (!ok) { j = 0; i = 0; }
     return ok;
 }
@@ -501,13 +505,23 @@ __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long 
             if (FLD == 0) {
                 old = n.vx[r];
                 if (r >= 1 && r < nxm) {
-                    HT h = ex_vx<HT>(p, n, r);
+                    HT h;
+                    double bref = 0.0;
+                    if (sizeof(HT) == sizeof(float) && p.bref) {
+                        // FP32 terms: the buoyancy enters as its deviation from
+                        // the row's FP64 reference, formed in FP64 and rounded
+                        // once; the RHS adds the reference back in FP64
+                        bref = p.bref[r];
+                        h = ex_vx<HT, false>(p, n, r) + HT(p.byct * (p.active_t * 0.5 * (n.t[r] + n.t[r - 1])) - bref);
+                    } else {
+                        h = ex_vx<HT>(p, n, r);
+                    }
                     if (p.gaxis == 1) h -= sb;
                     if (PF) {
                         h -= HT((n.pc[r] + n.pc[r - 1]) * n.vx[r] * p.heta);
                         solid = 0.5 * (n.pc[r] + n.pc[r + 1]) > 0.9;
                     }
-                    rhs = rhs_vx(p, n, double(h), double(ru[r]), r);
+                    rhs = rhs_vx(p, n, double(h) + bref, double(ru[r]) + bref, r);
                     ru[r] = h;
                 } else if (PF) {
                     solid = true;                    // rows 1 and nx: identity with rhs -vx
@@ -557,13 +571,14 @@ __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long 
     }
 }
 
-template <int CH, typename HT, bool PF>
-__global__ void __launch_bounds__(256) k_column_base(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
-    __shared__ double lds[PF ? 3 : 2][4][lds_col_size<CH>()];
+// TZ: z extent of the workgroup's column tile (4 x TZ wavefronts)
+template <int CH, typename HT, bool PF, int TZ>
+__global__ void __launch_bounds__(256 * TZ) k_column_base(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
+    __shared__ double lds[PF ? 3 : 2][4 * TZ][lds_col_size<CH>()];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
     int j, i;
-    const bool live = strip_column(xcd_block(0), wave, p.ny, p.nz, j, i);
+    const bool live = strip_column<TZ>(xcd_block(0), wave, p.ny, p.nz, j, i);
     double *buf = lds[0][wave], *own = lds[1][wave], *msk = lds[PF ? 2 : 1][wave];
     const Nb n = make_nb(p, j, i);
     const long long c0 = (long long)i * p.plane + (long long)j * p.ldx;
@@ -1113,12 +1128,17 @@ enum CoarseMode {
     CM_PHI = 1,     // phi with zero-gradient ghosts (phasefield.F90:479-487) -> phic (phic(nx) = 0)
     CM_LAT = 2      // d phi with zero-gradient ghosts (:674-683) -> out (+)= scale * value
 };
+// Per axis and coarse index: the first of the 4 refined sources (1-based
+// reference index: x may name the wall ghosts 0 and nxr or the zero entries
+// beyond them, y/z may name periodic images) and its 4 weights (RT).  The
+// bracketing (fast) path of interpolate_xyz_to_coarse_fast and the general
+// path of interpolate_xyz_to_coarse differ only in these tables.
 struct CoarseJob {
     int mode;
     const void *src;            // refined field (RT)
     void *out;                  // coarse field (RT for CM_SAL, double otherwise)
-    const int *x_src;           // first refined x source of each coarse x (reference index)
-    const void *x_w;            // its 4 weights (RT)
+    const int *x_src, *y_src, *z_src;
+    const void *x_w, *y_w, *z_w;
     double scale;
     int accumulate;
 };
@@ -1145,14 +1165,15 @@ __global__ void __launch_bounds__(256) k_interp_coarse_tile(KP p, CoarseJob jb, 
     const RT *src = static_cast<const RT *>(jb.src);
     const int *x_src = jb.x_src;
     const RT *xb_w = static_cast<const RT *>(jb.x_w);
-    const RT *yb_w = sel<RT>(p.yb_w, p.yb_wf), *zb_w = sel<RT>(p.zb_w, p.zb_wf);
+    const RT *yb_w = static_cast<const RT *>(jb.y_w), *zb_w = static_cast<const RT *>(jb.z_w);
+    const int *yb_src = jb.y_src, *zb_src = jb.z_src;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x;
     const int K0 = blockIdx.x * 64, K1 = min(K0 + 64, p.nxm);
     const int J0 = blockIdx.y * SALC_TJ, nj = min(SALC_TJ, p.ny - J0);
     const int I0 = blockIdx.z * SALC_TZ, I1 = min(I0 + SALC_TZ, p.nz);
     const int x0 = x_src[K0], XR = x_src[K1 - 1] + 4 - x0;                  // reference refined x index
-    const int y0 = p.yb_src[J0] - 1, YR = p.yb_src[J0 + nj - 1] - 1 + 4 - y0; // 0-based refined y
-    const int zfirst = p.zb_src[I0] - 1, zlast = p.zb_src[I1 - 1] - 1 + 3;    // 0-based refined z (unwrapped)
+    const int y0 = yb_src[J0] - 1, YR = yb_src[J0 + nj - 1] - 1 + 4 - y0;   // 0-based refined y
+    const int zfirst = zb_src[I0] - 1, zlast = zb_src[I1 - 1] - 1 + 3;      // 0-based refined z (unwrapped)
     const int k = K0 + lane;
     const bool kok = k < K1;
     RT wx[4] = {0, 0, 0, 0};
@@ -1165,7 +1186,7 @@ __global__ void __launch_bounds__(256) k_interp_coarse_tile(KP p, CoarseJob jb, 
     for (int h = 0; h < 2; ++h) {
         const int tj = wave + 4 * h;
         rok[h] = tj < nj;
-        if (rok[h]) { ya[h] = p.yb_src[J0 + tj] - 1 - y0; for (int b = 0; b < 4; ++b) wy[h][b] = yb_w[4 * (J0 + tj) + b]; }
+        if (rok[h]) { ya[h] = yb_src[J0 + tj] - 1 - y0; for (int b = 0; b < 4; ++b) wy[h][b] = yb_w[4 * (J0 + tj) + b]; }
     }
     RT ring[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};   // contracted refined planes zr-3..zr
     const int plane = XR * YR;
@@ -1212,7 +1233,7 @@ __global__ void __launch_bounds__(256) k_interp_coarse_tile(KP p, CoarseJob jb, 
             ring[h][0] = ring[h][1]; ring[h][1] = ring[h][2]; ring[h][2] = ring[h][3]; ring[h][3] = v;
         }
         // every coarse plane whose 4-plane stencil ends at zr is complete
-        while (ic < I1 && p.zb_src[ic] - 1 + 3 == zr) {
+        while (ic < I1 && zb_src[ic] - 1 + 3 == zr) {
             const RT *wz = zb_w + 4 * ic;
             for (int h = 0; h < 2; ++h)
                 if (kok && rok[h]) {

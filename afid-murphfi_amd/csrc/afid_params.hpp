@@ -64,11 +64,8 @@ struct KP {
     // Hermite tables coarse -> refined (per refined index): source + 4 weights
     const int *xs_src, *ys_src, *zs_src;
     const double *xs_w, *ys_w, *zs_w;
-    // refined -> coarse (InterpSalMultigrid fast path), per coarse index
-    const int *xb_src, *yb_src, *zb_src;
-    const double *xb_w, *yb_w, *zb_w;
     // FP32 copies of the Hermite weights and of lxa (mixed form)
-    const float *xs_wf, *ys_wf, *zs_wf, *xb_wf, *yb_wf, *zb_wf, *lxaf;
+    const float *xs_wf, *ys_wf, *zs_wf, *lxaf;
     // phase field (phasefield.F90): refined phi, its RK arrays and the
     // refined temperature; coarse phic and the latent-heat source qlat
     double *phi, *ruphi, *hphi, *tempr, *phic, *qlat;
@@ -77,9 +74,9 @@ struct KP {
     double pf_d, pf_b, pf_a, pf_tm, pf_s, heta;  // pf_D, pf_D/eps^2, A, T_m, S_t, 0.5/eta
     int pf;
     const double *ap3spkr, *ac3spkr, *am3spkr;   // zero-gradient x operator of phi (refined)
-    // refined -> coarse cell-centre stencil of phi (cxphic, irangs), per coarse x
-    const int *xp_src;
-    const double *xp_w;
+    // mixed form, gAxis = 1: per-row FP64 reference buoyancy subtracted from
+    // the FP32 vx explicit term and added back in the RHS (null otherwise)
+    const double *bref;
     // reductions
     unsigned long long *redmax;         // [4] max slots (bit patterns of non-negative doubles)
     double *redsum;                     // per-block partial sums

@@ -50,8 +50,9 @@ __device__ __forceinline__ Nb make_nb(const KP &p, int j, int i) {
     return n;
 }
 
-// ExplicitTermsVX.F90:42-127, valid for k = 1..nxm-1 (kc = 2..nxm)
-template <typename T>
+// ExplicitTermsVX.F90:42-127, valid for k = 1..nxm-1 (kc = 2..nxm); BUOY
+// false leaves out the buoyancy of :117-127 (the caller adds it)
+template <typename T, bool BUOY = true>
 __device__ __forceinline__ T ex_vx(const KP &p, const Nb &n, unsigned k) {
     const T udy = T(p.dy) * T(0.25), udz = T(p.dz) * T(0.25), udyq = T(p.dyq) / T(p.ren), udzq = T(p.dzq) / T(p.ren);
     const T rkm = T(p.rudx3m[k - 1]), rkc = T(p.rudx3m[k]), u3 = T(p.udx3c[k]);
@@ -66,7 +67,7 @@ __device__ __forceinline__ T ex_vx(const KP &p, const Nb &n, unsigned k) {
     T dzzvx = (vxzm - T(2.0) * vxc + vxzp) * udzq;
     T dyyvx = (vxym - T(2.0) * vxc + vxyp) * udyq;
     T q = -(hxx + hxy + hxz) + dyyvx + dzzvx;
-    if (p.gaxis == 1) q += T(p.byct) * (T(p.active_t) * T(0.5) * (T(n.t[k]) + T(n.t[k - 1])));
+    if (BUOY && p.gaxis == 1) q += T(p.byct) * (T(p.active_t) * T(0.5) * (T(n.t[k]) + T(n.t[k - 1])));
     return q;
 }
 

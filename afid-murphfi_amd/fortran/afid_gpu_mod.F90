@@ -44,6 +44,13 @@ module afid_gpu
         real(c_double)     :: ms_per_step
     end type afid_ens_tuning
 
+    !> the old run's grid of a checkpoint (continua_master.h5; sizes count the faces)
+    type, bind(C) :: afid_old_grid
+        integer(c_int32_t) :: nxo, nyo, nzo, istr3o
+        real(c_double)     :: str3o, ylen, zlen
+        integer(c_int32_t) :: nxro, nyro, nzro, istr3ro
+    end type afid_old_grid
+
     type, bind(C) :: afid_run_state
         real(c_double)     :: time, dt, cfl, max_div
         integer(c_int32_t) :: ntime, status, steps
@@ -123,6 +130,31 @@ module afid_gpu
             import; type(c_ptr), value :: ctx
         end function
         !> InterpTempMultigrid / InterpPhiMultigrid (phasefield.F90)
+        ! restart onto a new grid (ReadFlowInterp.F90:90-141): InterpInputVel,
+        ! InterpInputSal, InterpInputPhi; old arrays of the checkpoint's dims
+        integer(c_int) function afid_interp_input_vel(ctx, og, vx, vy, vz, temp) bind(C, name='afid_interp_input_vel')
+            import; type(c_ptr), value :: ctx; type(afid_old_grid), intent(in) :: og
+            real(c_double), intent(in) :: vx(*), vy(*), vz(*), temp(*)
+        end function
+        integer(c_int) function afid_interp_input_sal(ctx, og, sal) bind(C, name='afid_interp_input_sal')
+            import; type(c_ptr), value :: ctx; type(afid_old_grid), intent(in) :: og; real(c_double), intent(in) :: sal(*)
+        end function
+        integer(c_int) function afid_interp_input_phi(ctx, og, phi) bind(C, name='afid_interp_input_phi')
+            import; type(c_ptr), value :: ctx; type(afid_old_grid), intent(in) :: og; real(c_double), intent(in) :: phi(*)
+        end function
+        ! phase field (phasefield.F90:345-725)
+        integer(c_int) function afid_explicit_phase(ctx) bind(C, name='afid_explicit_phase')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_add_volume_penalty(ctx) bind(C, name='afid_add_volume_penalty')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_implicit_phase(ctx) bind(C, name='afid_implicit_phase')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_add_latent_heat(ctx) bind(C, name='afid_add_latent_heat')
+            import; type(c_ptr), value :: ctx
+        end function
         integer(c_int) function afid_interp_temp_multigrid(ctx) bind(C, name='afid_interp_temp_multigrid')
             import; type(c_ptr), value :: ctx
         end function

@@ -68,7 +68,8 @@ subroutine ReadInputFile
     if (mixed) cfg%precision = AFID_MIXED
 end subroutine ReadInputFile
 
-!> SetTempBCs.F90:18-40 and SetSalBCs (salinity.F90:105-122) for the RBC case
+!> SetTempBCs.F90:18-59 (RBC and phase-field wall values) and SetSalBCs
+!! (salinity.F90:105-122)
 subroutine SetBCs
     use afid_host_state
     implicit none
@@ -83,6 +84,14 @@ subroutine SetBCs
         end if
     else
         tt = 0.5d0; tb = -0.5d0
+    end if
+    ! SetTempBCs.F90:43-59: the phase-field case holds T at 1 and 0
+    if (cfg%phasefield /= 0) then
+        if (cfg%rayt >= 0) then
+            tt = 0.d0; tb = 1.d0
+        else
+            tt = 1.d0; tb = 0.d0
+        end if
     end if
     lb = [1_c_int64_t, 1_c_int64_t, 1_c_int64_t]
     ext = [1_c_int64_t, int(cfg%nym, c_int64_t), int(cfg%nzm, c_int64_t)]
@@ -127,6 +136,16 @@ subroutine TimeMarcher
             call afid_check(ctx, afid_explicit_salinity(ctx), 'ExplicitSalinity')
             if (cfg%active_s == 1) call afid_check(ctx, afid_add_sal_buoyancy(ctx), 'AddSalBuoyancy')
         end if
+        ! TimeMarcher.F90:66-77 (the salt terms of the interface belong to the
+        ! salinity + phase-field case, which the library does not support)
+        if (cfg%phasefield /= 0) then
+            call afid_check(ctx, afid_explicit_phase(ctx), 'ExplicitPhase')
+            call afid_check(ctx, afid_add_volume_penalty(ctx), 'AddVolumePenalty')
+            call afid_check(ctx, afid_implicit_phase(ctx), 'ImplicitPhase')
+            call afid_check(ctx, afid_add_latent_heat(ctx), 'AddLatentHeat')
+        end if
+        ! with the phase field on, the velocity solves are the masked
+        ! SolveImpEqnUpdate_X_pf / _YZ_pf (phasefield.F90:821-930)
         call afid_check(ctx, afid_implicit_and_update_vx(ctx), 'ImplicitAndUpdateVX')
         call afid_check(ctx, afid_implicit_and_update_vy(ctx), 'ImplicitAndUpdateVY')
         call afid_check(ctx, afid_implicit_and_update_vz(ctx), 'ImplicitAndUpdateVZ')
@@ -140,8 +159,23 @@ subroutine TimeMarcher
             call afid_check(ctx, afid_interp_vel_mgrd(ctx), 'InterpVelMgrd')
             call afid_check(ctx, afid_interp_sal_multigrid(ctx), 'InterpSalMultigrid')
         end if
+        ! TimeMarcher.F90:147-152
+        if (cfg%phasefield /= 0) then
+            call afid_check(ctx, afid_interp_temp_multigrid(ctx), 'InterpTempMultigrid')
+            call afid_check(ctx, afid_interp_phi_multigrid(ctx), 'InterpPhiMultigrid')
+        end if
     end do
 end subroutine TimeMarcher
+
+!> wall-clock seconds (the reference times its loop with MPI_WTIME, main.F90:282,383)
+function wtime() result(t)
+    use iso_c_binding, only: c_double, c_int64_t
+    implicit none
+    real(c_double) :: t
+    integer(c_int64_t) :: cnt, rate
+    call system_clock(cnt, rate)
+    t = real(cnt, c_double) / real(rate, c_double)
+end function wtime
 
 !> extent (x incl. the top point, y, z) of a base or refined module array
 function field_extent(refined) result(ext)
@@ -225,8 +259,9 @@ subroutine WriteFlowField(final, time)
     end if
 end subroutine WriteFlowField
 
-!> ReadFlowInterp.F90:1-179 on the same grid; another grid stops the run
-!! (the interpolation onto a new grid, :90-141, is not built)
+!> ReadFlowInterp.F90:1-179: the same grid reads the fields (:143-171);
+!! another grid reads them with the old dims and interpolates them onto the
+!! configured grids (:90-141: InterpInputVel, InterpInputSal, InterpInputPhi)
 subroutine ReadFlowInterp(time)
     use afid_host_state
     use afid_io
@@ -234,14 +269,20 @@ subroutine ReadFlowInterp(time)
     real(c_double), intent(out) :: time
     character(len=*), parameter :: master = 'outputdir/continua_master.h5'
     integer(c_int32_t) :: nxo, nyo, nzo, istr3o, nxro, nyro, nzro, istr3ro
-    real(c_double) :: str3o
+    real(c_double) :: str3o, yleno, zleno
     logical :: newgrid, ex
+    type(afid_old_grid) :: og
+    real(c_double), allocatable :: vxo(:), vyo(:), vzo(:), tempo(:), fo(:)
+    integer(c_int64_t) :: n3, m3
     call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('nx'), nxo), 'read nx')
     call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('ny'), nyo), 'read ny')
     call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('nz'), nzo), 'read nz')
     call h5_check(afid_h5_read_real_scalar(cstr(master), cstr('time'), time), 'read time')
     call h5_check(afid_h5_read_int_scalar(cstr(master), cstr('istr3'), istr3o), 'read istr3')
     call h5_check(afid_h5_read_real_scalar(cstr(master), cstr('str3'), str3o), 'read str3')
+    call h5_check(afid_h5_read_real_scalar(cstr(master), cstr('ylen'), yleno), 'read ylen')
+    call h5_check(afid_h5_read_real_scalar(cstr(master), cstr('zlen'), zleno), 'read zlen')
+    nxro = 0; nyro = 0; nzro = 0; istr3ro = 0
     newgrid = (cfg%nxm + 1 /= nxo) .or. (cfg%nym + 1 /= nyo) .or. (cfg%nzm + 1 /= nzo) .or. &
               (cfg%istr3 /= istr3o) .or. (abs(cfg%str3 - str3o) > 1d-8)
     if (cfg%multires /= 0) then
@@ -253,8 +294,37 @@ subroutine ReadFlowInterp(time)
                   (cfg%nzmr + 1 /= nzro) .or. (cfg%istr3r /= istr3ro)
     end if
     if (newgrid) then
-        write(0, '(a)') 'ReadFlowInterp: the checkpoint is on another grid; restart onto a new grid is not built'
-        error stop 2
+        ! :90-141 (the domain lengths are checked by the library, :76-82)
+        og = afid_old_grid(nxo, nyo, nzo, istr3o, str3o, yleno, zleno, nxro, nyro, nzro, istr3ro)
+        n3 = int(nxo, c_int64_t) * (nyo - 1) * (nzo - 1)
+        allocate(vxo(n3), vyo(n3), vzo(n3), tempo(n3))
+        call h5_check(afid_h5_read_field3d(cstr('outputdir/continua_vx.h5'), vxo, int(nxo, c_int64_t), &
+                      int(nyo - 1, c_int64_t), int(nzo - 1, c_int64_t)), 'read vx')
+        call h5_check(afid_h5_read_field3d(cstr('outputdir/continua_vy.h5'), vyo, int(nxo, c_int64_t), &
+                      int(nyo - 1, c_int64_t), int(nzo - 1, c_int64_t)), 'read vy')
+        call h5_check(afid_h5_read_field3d(cstr('outputdir/continua_vz.h5'), vzo, int(nxo, c_int64_t), &
+                      int(nyo - 1, c_int64_t), int(nzo - 1, c_int64_t)), 'read vz')
+        call h5_check(afid_h5_read_field3d(cstr('outputdir/continua_temp.h5'), tempo, int(nxo, c_int64_t), &
+                      int(nyo - 1, c_int64_t), int(nzo - 1, c_int64_t)), 'read temp')
+        call afid_check(ctx, afid_interp_input_vel(ctx, og, vxo, vyo, vzo, tempo), 'InterpInputVel')
+        deallocate(vxo, vyo, vzo, tempo)
+        if (salinity .or. cfg%phasefield /= 0) then
+            m3 = int(nxro, c_int64_t) * (nyro - 1) * (nzro - 1)
+            allocate(fo(m3))
+            if (salinity) then
+                call h5_check(afid_h5_read_field3d(cstr('outputdir/continua_sal.h5'), fo, int(nxro, c_int64_t), &
+                              int(nyro - 1, c_int64_t), int(nzro - 1, c_int64_t)), 'read sal')
+                call afid_check(ctx, afid_interp_input_sal(ctx, og, fo), 'InterpInputSal')
+            end if
+            if (cfg%phasefield /= 0) then
+                call h5_check(afid_h5_read_field3d(cstr('outputdir/continua_phi.h5'), fo, int(nxro, c_int64_t), &
+                              int(nyro - 1, c_int64_t), int(nzro - 1, c_int64_t)), 'read phi')
+                call afid_check(ctx, afid_interp_input_phi(ctx, og, fo), 'InterpInputPhi')
+            end if
+            deallocate(fo)
+        end if
+        if (ireset /= 0) time = 0.d0
+        return
     end if
     call field_file(AFID_VX, .false., 'outputdir/continua_vx.h5', .false.)
     call field_file(AFID_VY, .false., 'outputdir/continua_vy.h5', .false.)
@@ -339,6 +409,7 @@ program afid_host
     real(c_double) :: t0, t1, tin, minwtdt
     character(len=64) :: arg
     integer(c_int) :: rc
+    real(c_double), external :: wtime
 
     seed = 1
     narg = command_argument_count()
@@ -376,13 +447,13 @@ program afid_host
     call WriteMeans(time)
     call WriteFlowField(.false., time)
 
-    call cpu_time(tin)
+    tin = wtime()
     minwtdt = huge(0.d0)
     errorcode = 0
     dmax = 0.d0; dmaxr = 0.d0
     ! main.F90:281-422
     do ntime = 0, ntst
-        call cpu_time(t0)
+        t0 = wtime()
         call afid_check(ctx, afid_calc_max_cfl(ctx, instCFL, CFLmr), 'CalcMaxCFL')
         if (idtv /= 0) then
             if (ntime > 1) then
@@ -412,7 +483,7 @@ program afid_host
         ! main.F90:376-380
         if (mod(time, save_3D) < dt .and. floor(time / tframe) /= 0) call WriteFlowField(.false., time)
         call afid_check(ctx, afid_synchronize(ctx), 'synchronize')
-        call cpu_time(t1)
+        t1 = wtime()
         minwtdt = min(minwtdt, t1 - t0)
         if (t1 - tin > walltimemax) errorcode = 334
         if (ntime == ntst) errorcode = 555

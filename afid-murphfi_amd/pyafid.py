@@ -34,9 +34,10 @@ EXPORTS = [
     "afid_calc_local_divergence", "afid_solve_pressure_correction", "afid_correct_velocity", "afid_correct_pressure",
     "afid_explicit_phase", "afid_add_volume_penalty", "afid_implicit_phase", "afid_add_latent_heat",
     "afid_interp_temp_multigrid", "afid_interp_phi_multigrid",
-    "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher", "afid_refinement_residuals",
+    "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher", "afid_time_marcher_graph", "afid_refinement_residuals",
     "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_calc_sal_stats", "afid_calc_phi_stats",
-    "afid_synchronize",
+    "afid_synchronize", "afid_set_ibm",
+    "afid_interp_input_vel", "afid_interp_input_sal", "afid_interp_input_phi",
     "afid_create_initial_conditions",
     "afid_set_timing", "afid_get_timing", "afid_grid_array",
 ]
@@ -78,6 +79,14 @@ IO_EXPORTS = [
 CONTINUA = {"vx": "vx", "vy": "vy", "vz": "vz", "temp": "temp", "sal": "sal", "phi": "phi", "pr": "pr"}
 # the phase-field profiles of afid_calc_phi_stats (phasefield.F90:788-793)
 PHI_PROFILES = ["phibar", "phirms"]
+
+
+class OldGrid(C.Structure):
+    """Mirror of afid_old_grid: the old run's grid as continua_master.h5
+    records it (sizes count the faces)."""
+    _fields_ = [("nxo", C.c_int32), ("nyo", C.c_int32), ("nzo", C.c_int32), ("istr3o", C.c_int32),
+                ("str3o", C.c_double), ("ylen", C.c_double), ("zlen", C.c_double),
+                ("nxro", C.c_int32), ("nyro", C.c_int32), ("nzro", C.c_int32), ("istr3ro", C.c_int32)]
 
 
 class AfidConfig(C.Structure):
@@ -176,6 +185,7 @@ def lib():
         L.afid_download.argtypes = [vp, C.c_int, dp, i64p, i64p]
         L.afid_set_substep.argtypes = [vp, C.c_double, C.c_int]
         L.afid_time_marcher.argtypes = [vp, C.c_double]
+        L.afid_time_marcher_graph.argtypes = [vp, C.c_double, C.c_int]
         for r in ROUTINES:
             getattr(L, "afid_" + r).argtypes = [vp]
         L.afid_calc_max_cfl.argtypes = [vp, dp, dp]
@@ -185,6 +195,9 @@ def lib():
         L.afid_calc_sal_stats.argtypes = [vp, dp]
         L.afid_calc_phi_stats.argtypes = [vp, dp]
         L.afid_synchronize.argtypes = [vp]
+        L.afid_interp_input_vel.argtypes = [vp, C.POINTER(OldGrid), dp, dp, dp, dp]
+        L.afid_interp_input_sal.argtypes = [vp, C.POINTER(OldGrid), dp]
+        L.afid_interp_input_phi.argtypes = [vp, C.POINTER(OldGrid), dp]
         L.afid_create_initial_conditions.argtypes = [vp, C.c_uint64, C.c_double]
         L.afid_set_timing.argtypes = [vp, C.c_int]
         L.afid_get_timing.argtypes = [vp, C.POINTER(C.c_char_p), dp, C.c_int, C.POINTER(C.c_int)]
@@ -408,6 +421,10 @@ class Solver:
     def time_marcher(self, dt):
         self._check(self.L.afid_time_marcher(self.h, dt), "afid_time_marcher")
 
+    def time_marcher_graph(self, dt, nsteps):
+        """nsteps RK3 steps replayed from a captured HIP graph."""
+        self._check(self.L.afid_time_marcher_graph(self.h, dt, nsteps), "afid_time_marcher_graph")
+
     def create_initial_conditions(self, seed, eps=1e-3):
         self._check(self.L.afid_create_initial_conditions(self.h, seed, eps), "afid_create_initial_conditions")
 
@@ -507,12 +524,45 @@ class Solver:
             out.append(path)
         return out
 
+    def set_ibm(self, field, mask, dist=None):
+        """ibm_param.F90: the immersed-boundary mask (nz, ny, nx) of the solve
+        of `field` (vx, vy, vz, temp, sal) and its boundary distances in the
+        reference's loop order; mask None switches the solve back."""
+        if mask is None:
+            self.L.afid_set_ibm.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_int64]
+            self._check(self.L.afid_set_ibm(self.h, FIELD_ID[field], None, None, 0), "afid_set_ibm")
+            return
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        d = np.ascontiguousarray(dist if dist is not None else np.zeros(1), dtype=np.float64)
+        self.L.afid_set_ibm.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_int64]
+        self._check(self.L.afid_set_ibm(self.h, FIELD_ID[field], m.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        d.ctypes.data_as(C.POINTER(C.c_double)), 0 if dist is None else d.size),
+                    "afid_set_ibm")
+
+    def interp_input(self, old, fields):
+        """The interpolation of ReadFlowInterp.F90:90-141 onto this context's
+        grids: `old` a dict of the old grid (nx, ny, nz, istr3, str3, ylen,
+        zlen, and nxr, nyr, nzr, istr3r with multires; sizes count the faces)
+        and `fields` the old arrays (nz-1, ny-1, nx) by name: vx, vy, vz, temp
+        (InterpInputVel) and sal / phi (InterpInputSal / InterpInputPhi)."""
+        og = OldGrid(int(old["nx"]), int(old["ny"]), int(old["nz"]), int(old["istr3"]), float(old["str3"]),
+                     float(old.get("ylen", self.cfgd["ylen"])), float(old.get("zlen", self.cfgd["zlen"])),
+                     int(old.get("nxr", 0)), int(old.get("nyr", 0)), int(old.get("nzr", 0)), int(old.get("istr3r", 0)))
+        a = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in ("vx", "vy", "vz", "temp")]
+        dpp = [x.ctypes.data_as(C.POINTER(C.c_double)) for x in a]
+        self._check(self.L.afid_interp_input_vel(self.h, C.byref(og), *dpp), "afid_interp_input_vel")
+        for n, fn in (("sal", self.L.afid_interp_input_sal), ("phi", self.L.afid_interp_input_phi)):
+            if n in fields:
+                b = np.ascontiguousarray(fields[n], dtype=np.float64)
+                self._check(fn(self.h, C.byref(og), b.ctypes.data_as(C.POINTER(C.c_double))), "afid_interp_input_" + n)
+
     def read_flow_field(self, outdir, interpolate=True):
-        """ReadFlowInterp (ReadFlowInterp.F90:1-179) on the same grid: reads
-        continua_master.h5, checks the grid against the configured one (:84-89),
+        """ReadFlowInterp (ReadFlowInterp.F90:1-179): reads continua_master.h5
+        and returns the checkpoint's time.  On the configured grid (:84-89) it
         uploads vx, vy, vz, temp (+ sal, phi; + pr if continua_pr.h5 exists,
-        :143-171) and returns the checkpoint's time.  A checkpoint on another
-        grid raises: the interpolation onto a new grid (:90-141) is not built.
+        :143-171).  On another grid it reads the old fields with the old
+        grid's dims and interpolates them (InterpInputVel / InterpInputSal /
+        InterpInputPhi, :90-141); the domain lengths must match (:76-82).
         With `interpolate`, the start-up transfers that main.F90:205-213 runs
         after either initial state follow: InterpVelMgrd and InterpSalMultigrid
         (salinity), InterpTempMultigrid and InterpPhiMultigrid (phase field)."""
@@ -525,13 +575,20 @@ class Solver:
         t = h5_read_real(master, "time")
         same = all(old[k] == g[k] for k in old if k != "str3") and abs(g["str3"] - old["str3"]) <= 1e-8
         if not same:
-            raise AfidError(f"checkpoint grid {old} differs from the configured grid {g}: "
-                            "restart onto a new grid (ReadFlowInterp.F90:90-141) is not built")
-        for n in self._continua_fields(final=False):
-            self.upload(n, h5_read_field(os.path.join(outdir, f"continua_{CONTINUA[n]}.h5"), self.shape(n)))
-        pr = os.path.join(outdir, "continua_pr.h5")
-        if os.path.exists(pr):
-            self.upload("pr", h5_read_field(pr, self.shape("pr")))
+            old["ylen"] = h5_read_real(master, "ylen")
+            old["zlen"] = h5_read_real(master, "zlen")
+            fields = {}
+            for n in self._continua_fields(final=False):
+                ref = n in REFINED
+                dims = (old["nzr"] - 1, old["nyr"] - 1, old["nxr"]) if ref else (old["nz"] - 1, old["ny"] - 1, old["nx"])
+                fields[n] = h5_read_field(os.path.join(outdir, f"continua_{CONTINUA[n]}.h5"), dims)
+            self.interp_input(old, fields)
+        else:
+            for n in self._continua_fields(final=False):
+                self.upload(n, h5_read_field(os.path.join(outdir, f"continua_{CONTINUA[n]}.h5"), self.shape(n)))
+            pr = os.path.join(outdir, "continua_pr.h5")
+            if os.path.exists(pr):
+                self.upload("pr", h5_read_field(pr, self.shape("pr")))
         if interpolate:
             if self.cfgd["salinity"]:
                 self.call("interp_vel_mgrd")

@@ -128,6 +128,21 @@ int afid_add_latent_heat(afid_ctx *ctx);          /* phasefield.F90:651-725 */
 int afid_interp_temp_multigrid(afid_ctx *ctx);    /* phasefield.F90:498-520 */
 int afid_interp_phi_multigrid(afid_ctx *ctx);     /* phasefield.F90:470-494 */
 
+/* Immersed boundary (src/ibm/ibm_param.F90): the mask of one wall-normal
+ * solve, field = AFID_VX (ibmaskx, distx), AFID_VY (ibmasky, disty), AFID_VZ
+ * (ibmaskz, distz), AFID_TEMP (ibmaskt, distbt) on (nx, nym, nzm), or
+ * AFID_SAL (ibmaskr) on (nxr, nymr, nzmr), x fastest: 2 liquid, 0 solid,
+ * 1 / -1 upper / lower boundary point.  dist: the boundary distances in the
+ * reference's loop order (z, y, x over rows 1..nxm), ndist of them (none for
+ * S, whose solve takes 1).  The implicit updates of that field then solve
+ * SolveImpEqnUpdate_X_ibm / _YZ_ibm / _Temp_ibm / _Sal_ibm (src/ibm/): solid
+ * rows force the increment to -q (to Tb - T, Tb = 1, for T), boundary rows
+ * tie the point to its neighbour.  T keeps the plain solve when the phase
+ * field is on (ImplicitAndUpdateTemp.F90:69-70).  mask = NULL switches the
+ * solve back.  FP64 call tree only: afid_time_marcher returns AFID_E_STATE
+ * while a mask is set. */
+int afid_set_ibm(afid_ctx *ctx, int field, const int32_t *mask, const double *dist, int64_t ndist);
+
 /* Multi-resolution transfers. */
 int afid_interp_vel_mgrd(afid_ctx *ctx);       /* InterpVelMgrd.F90:1-243 */
 int afid_interp_sal_multigrid(afid_ctx *ctx);  /* salinity.F90:458-491 (fast path, HermiteInterpolations.F90:256-312) */
@@ -138,6 +153,13 @@ int afid_interp_sal_multigrid(afid_ctx *ctx);  /* salinity.F90:458-491 (fast pat
  * FP64 solve (pressure.F90:165-197) by FP32 sweeps inside FP64-residual
  * iterative refinement. */
 int afid_time_marcher(afid_ctx *ctx, double dt);
+
+/* No counterpart in the reference: nsteps calls of afid_time_marcher(ctx, dt)
+ * replayed from a HIP graph of two captured steps (the small grids of a
+ * 64^3 run are bound by kernel launches).  Bit-identical to the eager
+ * steps.  The graph is captured on the first call, and again when dt changes
+ * or the fields were reallocated; an odd last step runs eagerly. */
+int afid_time_marcher_graph(afid_ctx *ctx, double dt, int nsteps);
 
 /* AFID_MIXED: max|div u| (CheckDivergence units) measured before each
  * refinement sweep 2..max_sweeps of the last substep; 0 where the sweep was
@@ -167,6 +189,34 @@ int afid_calc_sal_stats(afid_ctx *ctx, double *out);
  * phase field, out[q * nxmr + k], q = 0 phibar, 1 phirms (2*nxmr doubles).
  * FP64 sums in both forms.  Needs the phase field.  Waits for the result. */
 int afid_calc_phi_stats(afid_ctx *ctx, double *out);
+
+/* Restart onto a new grid (ReadFlowInterp.F90:90-141): the grid of the old run,
+ * as its continua_master.h5 records it (nx, ny, nz, istr3, str3, ylen, zlen
+ * and, with multires, nxr, nyr, nzr, istr3r; sizes count the faces). */
+typedef struct afid_old_grid {
+    int32_t nxo, nyo, nzo, istr3o;
+    double str3o, ylen, zlen;
+    int32_t nxro, nyro, nzro, istr3ro;
+} afid_old_grid;
+/* CreateOldGrid + CreateNewInputStencil + InterpInputVel
+ * (src/multires/IC_interpolation/): vx, vy, vz and temp of the old run, host
+ * arrays of the checkpoint's dims (nxo, nyo-1, nzo-1), x fastest, moved to
+ * this context's base grid by the 4-point Hermite transfer: T directly (with
+ * the ghosts of its wall values), the velocity through its interpolated
+ * gradients (vx integrated from the wall, vy and vz from first-guess planes
+ * at the first y / z face), so a divergence-free old field stays
+ * divergence-free off the periodic seams.  pr and the RK history restart
+ * from zero, as in the reference.  The domain lengths must match
+ * (ReadFlowInterp.F90:76-82).  Waits for the result; the host then runs the
+ * start-up transfers (InterpVelMgrd, InterpSalMultigrid, ...). */
+int afid_interp_input_vel(afid_ctx *ctx, const afid_old_grid *og, const double *vx, const double *vy,
+                          const double *vz, const double *temp);
+/* CreateNewSalStencil + InterpInputSal: the old refined S, dims (nxro,
+ * nyro-1, nzro-1), with its Dirichlet (sign of Ra_S; 1 / 0 with the phase
+ * field) or zero-gradient wall ghosts, onto the refined grid. */
+int afid_interp_input_sal(afid_ctx *ctx, const afid_old_grid *og, const double *sal);
+/* InterpInputPhi: the old refined phi with zero-gradient ghosts onto the refined grid. */
+int afid_interp_input_phi(afid_ctx *ctx, const afid_old_grid *og, const double *phi);
 
 /* Seeded initial state (CreateInitialConditions.F90:150-201, RBC branch, with
  * u = 0): T = tempbp + (temptp - tempbp) x + noise, the wall-clamped noise of

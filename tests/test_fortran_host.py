@@ -1,6 +1,13 @@
 """The Fortran host (driver loop of main.F90:281-422 and the TimeMarcher call
 tree, bound with iso_c_binding) against the FP64 CPU oracle driven the same
-way: same bou.in, same seeded initial state, same number of steps."""
+way: same bou.in, same seeded initial state, same number of steps.
+
+The field tests read the host's own HDF5 output: the start-up snapshot
+(fields/00000_*.h5, main.F90:241) seeds the oracle, and the checkpoint a
+normal end writes (continua_*.h5, QuitRoutine.F90:34-38) is compared with the
+oracle after the same steps, every field within 1e-10 of its max norm (p
+modulo its mean).  The exit-code tests force each of the driver's stops from
+bou.in alone."""
 import os
 import re
 import shutil
@@ -130,3 +137,167 @@ def test_fortran_host_checkpoint_and_restart(tmp_path, case):
         assert np.isfinite(a).all() and np.array_equal(a, b)
         assert pyafid.h5_read_int(out / "continua_master.h5", "nxr") == 2 * nxm + 1
         assert pyafid.h5_dims(out / "means.h5", "Sbar/00000") == (2 * nxm,)
+
+
+def set_line(txt, header, values):
+    """bou.in is positional: replace the value line under the header line
+    that starts with `header`."""
+    lines = txt.split("\n")
+    k = [i for i, l in enumerate(lines) if l.startswith(header)]
+    assert len(k) == 1, header
+    lines[k[0] + 1] = "    ".join(str(v) for v in values)
+    return "\n".join(lines)
+
+
+def bou(path, nxm, nym, nzm, ntst, dt, multires=None, pf=False, sal=False, idtv=(0, None, 1e-3, 1.0, 1e-8, 1e-2),
+        walltime=7000, tmax=500.0, rayt=1e5):
+    txt = open(EXAMPLE).read()
+    txt = set_line(txt, "NXM", [nxm, nym, nzm, 3])
+    if multires:
+        txt = set_line(txt, "MULTIRES", [1, *multires])
+    txt = set_line(txt, "FLAGSAL", [1 if sal else 0, 1 if pf else 0])
+    txt = set_line(txt, "NTST", [ntst, walltime, tmax])
+    v = list(idtv)
+    v[1] = dt if v[1] is None else v[1]
+    txt = set_line(txt, "IDTV", v)
+    txt = set_line(txt, "RAYT", [rayt, 1.0, -1e5, 4.0, 0])
+    open(path, "w").write(txt)
+
+
+CASES = {
+    # RBC: SetTempBCs.F90:27-33 (inslwN = 1) holds T at +0.5 / -0.5
+    "rbc": dict(nxm=24, nym=16, nzm=12, walls=(0.5, -0.5)),
+    # multires with a passive S on the 2x grid: S walls from Ra_S < 0
+    # (salinity.F90:108-122, +0.5 bottom / -0.5 top)
+    "multires": dict(nxm=16, nym=12, nzm=12, multires=(32, 24, 24), sal=True, walls=(0.5, -0.5)),
+    # phase field: SetTempBCs.F90:43-59 holds T at 1 / 0; the seeded state
+    # is the solid sphere of radius 0.1 (pf_eps = 1/nxmr)
+    "phasefield": dict(nxm=16, nym=16, nzm=16, multires=(32, 32, 32), pf=True, walls=(1.0, 0.0)),
+}
+
+
+def host_case(tmp_path, case, mode, ntst=5, dt=2e-3):
+    c = CASES[case]
+    bou(os.path.join(tmp_path, "bou.in"), c["nxm"], c["nym"], c["nzm"], ntst, dt, multires=c.get("multires"),
+        pf=c.get("pf", False), sal=c.get("sal", False))
+    r = run_host(tmp_path, mode)
+    return c, r
+
+
+def read(out, name, shape):
+    import pyafid
+    return pyafid.h5_read_field(out / name, shape)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+@pytest.mark.parametrize("mode", ["calltree", "fused"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_fortran_host_fields_match_oracle(tmp_path, case, mode):
+    ntst, dt = 5, 2e-3
+    c, r = host_case(tmp_path, case, mode, ntst, dt)
+    assert int(r["steps"]) == ntst + 1 and int(r["errorcode"]) == 555
+    nxm, nym, nzm = c["nxm"], c["nym"], c["nzm"]
+    base = (nzm, nym, nxm + 1)
+    out = tmp_path / "outputdir"
+    cfg = dict(nxm=nxm, nym=nym, nzm=nzm, istr3=8, rayt=1e5, prat=1.0)
+    names = ["temp", "vx", "vy", "vz"]
+    if "multires" in c:
+        nxr, nyr, nzr = c["multires"]
+        cfg.update(multires=1, nxmr=nxr, nymr=nyr, nzmr=nzr, istr3r=8)
+        ref = (nzr, nyr, nxr + 1)
+    if c.get("sal"):
+        cfg.update(salinity=1, rays=-1e5, pras=4.0)
+        names.append("sal")
+    if c.get("pf"):
+        cfg.update(phasefield=1, pf_a=1.0, pf_s=1.0, pf_tm=0.0)
+        names.append("phi")
+    o = pyoracle.Oracle(cfg)
+    o.field("tempbp")[...] = c["walls"][0]
+    o.field("temptp")[...] = c["walls"][1]
+    if c.get("sal"):
+        o.field("salbp")[...] = 0.5
+        o.field("saltp")[...] = -0.5
+    for n in names:
+        o.field(n)[...] = read(out / "fields", f"00000_{n}.h5", ref if n in ("sal", "phi") else base)
+    if c.get("sal"):
+        o.call("interp_vel_mgrd")
+        o.call("interp_sal_multigrid")
+    if c.get("pf"):
+        o.call("interp_temp_multigrid")
+        o.call("interp_phi_multigrid")
+        assert o.field("phi").max() > 0.9          # the solid sphere is there
+    for _ in range(ntst + 1):
+        o.time_marcher(dt)
+    for n in names + ["pr"]:
+        a = read(out, f"continua_{n}.h5", ref if n in ("sal", "phi") else base)
+        b = o.field(n)
+        if n in ("sal", "phi"):
+            a, b = a[..., :cfg["nxmr"]], b[..., :cfg["nxmr"]]
+        elif n != "vx":
+            a, b = a[..., :nxm], b[..., :nxm]
+        if n == "pr":
+            a, b = a - a.mean(), b - b.mean()
+        e = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        assert e <= 1e-10, f"{case} {mode} {n}: {e:.3e}"
+    q = o.global_quantities()
+    assert abs(r["nu"] - q["nu"]) <= 1e-10 * abs(q["nu"])
+    assert abs(r["ke"] - q["ke"]) <= 1e-9 * abs(q["ke"])
+    o.close()
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+@pytest.mark.parametrize("code,steps,edit", [
+    # fixed dt (IDTV = 0) with CFL * dt above CFLMAX at the first check
+    # (main.F90:300-302; CalcMaxCFL floors the CFL at 1e-8)
+    (165, 1, dict(idtv=(0, None, 1e-3, 1e-12, 1e-8, 1e-2))),
+    # IDTV = 1 with DT (the first steps' dt) below DTMIN (main.F90:297)
+    (166, 1, dict(idtv=(1, None, 1e-3, 1.0, 1.0, 1e-2))),
+    # RESID below the divergence the step leaves: checked at ntime = 1 (main.F90:343-355)
+    (169, 2, dict(idtv=(0, None, 1e-300, 1.0, 1e-8, 1e-2))),
+    # WALLTIMEMAX = 0: the wall clock passes it in the first step (main.F90:383)
+    (334, 1, dict(walltime=0)),
+])
+def test_fortran_host_exit_codes(tmp_path, code, steps, edit):
+    bou(os.path.join(tmp_path, "bou.in"), 16, 8, 8, 9, 2e-3, **edit)
+    r = run_host(tmp_path, "calltree")
+    assert int(r["errorcode"]) == code and int(r["steps"]) == steps, r
+    # 334 is a normal end (QuitRoutine(tin, .true., ...)): the checkpoint is written
+    assert os.path.exists(tmp_path / "outputdir" / "continua_master.h5") == (code == 334)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+def test_fortran_host_restart_onto_new_grid(tmp_path):
+    """NREAD = 1 with a checkpoint of another grid (ReadFlowInterp.F90:90-141):
+    a 16x8x8 run's checkpoint read by a 24x12x12 host on another clustering.
+    The restarted host's start-up snapshot (main.F90:241, the interpolated
+    state) equals the oracle's InterpInputVel of the same files, and the run
+    then steps normally."""
+    import pyafid
+    one, two = tmp_path / "one", tmp_path / "two"
+    os.makedirs(one)
+    os.makedirs(two)
+    bou(os.path.join(one, "bou.in"), 16, 8, 8, 3, 2e-3)
+    r = run_host(one, "fused")
+    assert int(r["errorcode"]) == 555
+    shutil.copytree(one / "outputdir", two / "outputdir")
+    bou(os.path.join(two, "bou.in"), 24, 12, 12, 2, 2e-3)
+    txt = open(os.path.join(two, "bou.in")).read()
+    txt = set_line(txt, "NREAD", [1, 0])
+    txt = set_line(txt, "ISTR3", [6, 0.0, 8])
+    open(os.path.join(two, "bou.in"), "w").write(txt)
+    r2 = run_host(two, "calltree")
+    assert int(r2["errorcode"]) == 555 and int(r2["steps"]) == 3
+    assert abs(r2["time"] - 14e-3) < 1e-14
+    old = {n: pyafid.h5_read_field(one / "outputdir" / f"continua_{n}.h5", (8, 8, 17)) for n in ("vx", "vy", "vz", "temp")}
+    cfg = dict(nxm=24, nym=12, nzm=12, istr3=6, str3=0.0, rayt=1e5, prat=1.0)
+    o = pyoracle.Oracle(cfg)
+    o.interp_input_vel(pyoracle.old_grid(dict(nxm=16, nym=8, nzm=8, istr3=8, str3=0.0)), old["vx"], old["vy"], old["vz"],
+                       old["temp"])
+    for n in ("vx", "vy", "vz", "temp"):
+        a = pyafid.h5_read_field(two / "outputdir" / "fields" / f"00000_{n}.h5", (12, 12, 25))
+        b = o.field(n)
+        if n != "vx":
+            a, b = a[..., :24], b[..., :24]
+        e = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        assert e <= 1e-11, f"{n}: {e:.3e}"
+    o.close()

@@ -1,0 +1,444 @@
+"""GPU tests at the workloads the path is measured on, not only on small grids.
+
+  * 64^3 Rayleigh-Benard, Ra 1e5, dt 1e-3, 200 steps: the FP64 GPU step
+    against the FP64 oracle over the whole run (and the mixed form beside it).
+  * 384x768^2 base + 768x1536^2 refined multires RBC with a passive scalar
+    (Ra 1e9, Sc 16, dt 5e-5), FP64 and mixed forms at full size for 50 steps:
+    properties that do not need the oracle (divergence, Nu / KE agreement of
+    the forms, bounded field differences); and both forms against the oracle
+    on the same grids cut by 4 in y and z (same spacing).
+  * 256 uneven 2-D runs in one ensemble (nx 64..128, nz 128..512, Ra 1e6..1e9,
+    Pr 1 / 7, CFL-controlled dt): every run against the oracle driver loop,
+    and the mixed ensemble against the FP64 one.
+  * the melting sphere (base 256x512^2, refined 512x1024^2 phase field, Ra 1e8,
+    dt 1e-4), both forms at full size for 100 steps through properties, and
+    against the oracle on the grids cut by 4 in y and z.
+
+Tolerances are stated beside each check.  FP64 GPU against the FP64 oracle:
+the same arithmetic in another order (Thomas scans, Stockham FFTs), so the
+state agrees to round-off growth, <= 1e-10 relative to each field's max norm
+(pressure modulo its mean).  Mixed against FP64: max|div u| <= 1e-10 and
+Nu / KE within 1e-6 relative, the bounds the mixed form is built for.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyafid
+import pyoracle
+from bench import WORKLOADS, ensemble_runs
+from helpers import relerr, uniform
+
+pytestmark = pytest.mark.gpu
+
+BASE = ["vx", "vy", "vz", "temp", "pr"]
+
+
+def workload(name, cut=1):
+    """Grid and physics of a bench workload (dt separate); cut > 1 divides the
+    periodic extent and the periodic sizes by cut (same spacing)."""
+    wl = dict(WORKLOADS[name])
+    dt = wl.pop("dt")
+    if cut > 1:
+        wl.update(nym=wl["nym"] // cut, nzm=wl["nzm"] // cut, ylen=wl["ylen"] / cut, zlen=wl["zlen"] / cut)
+        if "nymr" in wl:
+            wl.update(nymr=wl["nymr"] // cut, nzmr=wl["nzmr"] // cut)
+    return wl, dt
+
+
+def start(cfg, form, seed):
+    """Seeded initial state on the device: T = 1 - x (+ S = 1 - x) with 1e-3
+    wall-clamped noise, u = 0; with the phase field the solid sphere."""
+    g = pyafid.Solver(dict(cfg, precision=pyafid.MIXED if form == "mixed" else pyafid.FP64))
+    nz, ny = cfg["nzm"], cfg["nym"]
+    g.upload("tempbp", np.ones((nz, ny, 1)))
+    g.upload("temptp", np.zeros((nz, ny, 1)))
+    if cfg.get("salinity"):
+        g.upload("salbp", np.ones((cfg["nzmr"], cfg["nymr"], 1)))
+        g.upload("saltp", np.zeros((cfg["nzmr"], cfg["nymr"], 1)))
+    g.create_initial_conditions(seed, 1e-3)
+    g.synchronize()
+    return g
+
+
+def oracle_from(g, cfg):
+    """An oracle holding the solver's state (the start-up transfers redone on
+    the CPU from the downloaded fields)."""
+    o = pyoracle.Oracle(cfg)
+    names = BASE + ["tempbp", "temptp"]
+    if cfg.get("salinity"):
+        names += ["sal", "salbp", "saltp"]
+    if cfg.get("phasefield"):
+        names += ["phi"]
+    for n in names:
+        o.field(n)[...] = g.download(n)
+    if cfg.get("salinity"):
+        o.call("interp_sal_multigrid")
+    if cfg.get("phasefield"):
+        o.call("interp_temp_multigrid")
+        o.call("interp_phi_multigrid")
+    return o
+
+
+def interior(cfg, name, a):
+    """Cells the step defines: x 1..nxm (vx: 2..nxm), refined 1..nxmr, pr
+    minus its mean."""
+    if name in ("sal", "phi", "tempr"):
+        return a[..., :cfg["nxmr"]]
+    if name == "vx":
+        return a[..., 1:cfg["nxm"]]
+    a = a[..., :cfg["nxm"]]
+    return a - a.mean() if name == "pr" else a
+
+
+def l2rel(a, b):
+    """Relative L2 difference, summed in slabs (fields of 15 GB)."""
+    num = den = 0.0
+    for s in range(0, a.shape[0], 64):
+        d = np.asarray(a[s:s + 64], np.float64) - np.asarray(b[s:s + 64], np.float64)
+        num += float(np.sum(d * d))
+        den += float(np.sum(np.asarray(b[s:s + 64], np.float64) ** 2))
+    return float(np.sqrt(num / max(den, 1e-300)))
+
+
+def steps(g, n, dt):
+    for _ in range(n):
+        g.time_marcher(dt)
+    g.synchronize()
+
+
+def oracle_threads():
+    pyoracle.set_threads(max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)))
+
+
+# ---------------------------------------------------------------------------
+# 64^3 RBC, Ra 1e5, 200 steps
+# ---------------------------------------------------------------------------
+def test_rbc64_initial_state_is_the_closed_form():
+    """The seeded state is T = 1 - x + amp (2 r - 1), r = U[0,1) of
+    splitmix64(seed ^ linear index) over the (nz, ny, nx) array, amp = 1e-3
+    clamped near the walls (CreateInitialConditions.F90:187-201)."""
+    cfg, _ = workload("rbc_64x64x64")
+    g = start(cfg, "fp64", seed=0)
+    nxm = cfg["nxm"]
+    xm = g.grid_array("xm")[:nxm]
+    T = g.download("temp")
+    amp = np.where(np.abs(xm - 0.5) + 1e-3 > 0.5, 0.5 - np.abs(xm - 0.5), 1e-3)
+    ref = 1.0 - xm + amp * (2 * uniform(0, T.shape)[..., :nxm] - 1)
+    assert np.abs(T[..., :nxm] - ref).max() <= 2e-16
+    for n in ("vx", "vy", "vz", "pr"):
+        assert not np.any(g.download(n))
+    g.close()
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_rbc64_200_steps_vs_oracle(form):
+    """The whole 200-step run against the oracle from the same seeded state.
+    FP64: every field within 1e-10 of the oracle (max norm), Nu and KE within
+    1e-10.  Mixed: relative L2 <= 1e-5, Nu / KE within 1e-6, max|div u| <=
+    1e-10."""
+    cfg, dt = workload("rbc_64x64x64")
+    oracle_threads()
+    g = start(cfg, form, seed=0)
+    o = oracle_from(g, cfg)
+    nst = 200
+    print(f"rbc 64^3 {form}: {nst} steps of dt {dt}", flush=True)
+    steps(g, nst, dt)
+    for _ in range(nst):
+        o.time_marcher(dt)
+    div, _ = g.check_divergence()
+    divo, _ = o.check_divergence()
+    qg, qo = g.global_quantities(), o.global_quantities()
+    print(f"max|div u| gpu {div:.3e} oracle {divo:.3e}; Nu {qg['nu']:.15e} / {qo['nu']:.15e}; "
+          f"KE {qg['ke']:.6e} / {qo['ke']:.6e}", flush=True)
+    assert qo["ke"] > 0.0
+    if form == "fp64":
+        assert div <= 1e-12
+        for f in BASE:
+            e = relerr(interior(cfg, f, g.download(f)), interior(cfg, f, o.field(f)))
+            print(f"{f}: max-norm relative error {e:.3e}")
+            assert e <= 1e-10, f"{f}: {e:.3e}"
+        for k in ("nu", "ke", "vrms", "tmean"):
+            assert abs(qg[k] - qo[k]) <= 1e-10 * abs(qo[k]), k
+    else:
+        assert div <= 1e-10
+        for f in BASE:
+            e = l2rel(interior(cfg, f, g.download(f)), interior(cfg, f, o.field(f)))
+            print(f"{f}: relative L2 {e:.3e}")
+            assert e <= 1e-5, f"{f}: {e:.3e}"
+        for k in ("nu", "ke"):
+            assert abs(qg[k] - qo[k]) <= 1e-6 * abs(qo[k]), k
+    g.close()
+    o.close()
+
+
+# ---------------------------------------------------------------------------
+# multires RBC 384x768^2 + 768x1536^2 (Ra 1e9, Sc 16, dt 5e-5)
+# ---------------------------------------------------------------------------
+MR = "rbc_multires_384x768x768_768x1536x1536"
+
+
+@pytest.fixture(scope="module")
+def multires_full():
+    """Both forms at full size, 50 steps each from the same seeded state, one
+    context at a time (the FP64 context holds 150 GiB).  Returns the scalars
+    and the FP64 form's interior fields."""
+    cfg, dt = workload(MR)
+    out = {}
+    ref = None
+    for form in ("fp64", "mixed"):
+        print(f"multires full size, {form}: 50 steps", flush=True)
+        g = start(cfg, form, seed=1)
+        steps(g, 50, dt)
+        r = {"div": g.check_divergence(), "gq": g.global_quantities(), "cfl": g.calc_max_cfl()}
+        fields = {}
+        for n in BASE + ["sal"]:
+            a = interior(cfg, n, g.download(n))
+            fields[n] = a.astype(np.float32) if n == "sal" else a
+        if form == "fp64":
+            ref = fields
+        else:
+            r["l2"] = {n: l2rel(fields[n], ref[n]) for n in fields}
+            r["umax"] = max(float(np.abs(g.download(n)).max()) for n in ("vxr", "vyr", "vzr"))
+            r["invh"] = max(float(g.grid_array("udx3mr").max()), cfg["nymr"] / cfg["ylen"], cfg["nzmr"] / cfg["zlen"])
+        del fields
+        out[form] = r
+        print(f"{form}: div {r['div']}, Nu {r['gq']['nu']:.15e}, KE {r['gq']['ke']:.6e}", flush=True)
+        g.close()
+    return out
+
+
+def test_multires_full_size_fp64_divergence_free(multires_full):
+    r = multires_full["fp64"]
+    assert r["div"][0] <= 1e-12, r["div"]        # base grid, CheckDivergence units
+    assert r["div"][1] <= 1e-12, r["div"]        # refined grid (InterpVelMgrd keeps it divergence-free)
+    assert r["gq"]["ke"] > 0.0 and np.isfinite(r["gq"]["nu"])
+    assert r["cfl"][0] * 5e-5 < 0.5
+
+
+def test_multires_full_size_mixed_vs_fp64(multires_full):
+    """Mixed form after 50 steps: max|div u| <= 1e-10; Nu and KE within 1e-6
+    of the FP64 form; relative L2 of T, p and S <= 1e-6 and of u, v, w <=
+    1e-4 (the velocity is ~1e-7 after 50 steps from rest, so its relative
+    error measures FP32 rounding of the explicit terms against that signal);
+    the refined divergence within the FP32 round-off bound of the refined
+    velocities (unit round-off 6e-8 x max|u_r| x the largest inverse refined
+    spacing x 100)."""
+    a, b = multires_full["fp64"], multires_full["mixed"]
+    print(f"mixed: div {b['div']}, l2 {b['l2']}")
+    assert b["div"][0] <= 1e-10
+    for k in ("nu", "ke"):
+        e = abs(b["gq"][k] - a["gq"][k]) / abs(a["gq"][k])
+        print(f"{k}: relative error {e:.3e}")
+        assert e <= 1e-6, k
+    for n in ("temp", "pr", "sal"):
+        assert b["l2"][n] <= 1e-6, (n, b["l2"][n])
+    for n in ("vx", "vy", "vz"):
+        assert b["l2"][n] <= 1e-4, (n, b["l2"][n])
+    bound = 100 * 6e-8 * b["umax"] * b["invh"]
+    assert b["div"][1] <= bound, (b["div"][1], bound)
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_multires_cut_extent_vs_oracle(form):
+    """384x192^2 + 768x384^2 (the full grids cut by 4 in y and z, same spacing),
+    6 steps against the oracle.  FP64 within 1e-10 (max norm, p modulo its
+    mean, S included); mixed: relative L2 <= 1e-4 for u, v, w and 1e-6 for T,
+    p, S, Nu within 1e-6."""
+    cfg, dt = workload(MR, cut=4)
+    oracle_threads()
+    g = start(cfg, form, seed=1)
+    o = oracle_from(g, cfg)
+    print(f"multires cut extent, {form}: 6 steps against the oracle", flush=True)
+    steps(g, 6, dt)
+    for _ in range(6):
+        o.time_marcher(dt)
+    div, divr = g.check_divergence()
+    qg, qo = g.global_quantities(), o.global_quantities()
+    for f in BASE + ["sal"]:
+        a, b = interior(cfg, f, g.download(f)), interior(cfg, f, o.field(f))
+        if form == "fp64":
+            e = relerr(a, b)
+            print(f"{f}: max-norm relative error {e:.3e}")
+            assert e <= 1e-10, (f, e)
+        else:
+            e = l2rel(a, b)
+            print(f"{f}: relative L2 {e:.3e}")
+            assert e <= (1e-4 if f in ("vx", "vy", "vz") else 1e-6), (f, e)
+    if form == "fp64":
+        assert div <= 1e-12 and divr <= 1e-12
+        assert abs(qg["nu"] - qo["nu"]) <= 1e-10 * abs(qo["nu"])
+    else:
+        assert div <= 1e-10
+        assert abs(qg["nu"] - qo["nu"]) <= 1e-6 * abs(qo["nu"])
+    g.close()
+    o.close()
+
+
+# ---------------------------------------------------------------------------
+# 256-run 2-D ensemble
+# ---------------------------------------------------------------------------
+def _ensemble(form):
+    cfgs, ctls = ensemble_runs()
+    prec = pyafid.MIXED if form == "mixed" else pyafid.FP64
+    ens = pyafid.Ensemble([dict(c, precision=prec) for c in cfgs], ctls)
+    for r, c in enumerate(cfgs):
+        ens.upload(r, "tempbp", np.ones((c["nzm"], 1, 1)))
+        ens.upload(r, "temptp", np.zeros((c["nzm"], 1, 1)))
+    ens.create_initial_conditions(3000, 1e-3)
+    return ens, cfgs, ctls
+
+
+def test_ensemble_256_runs_vs_oracle():
+    """Every one of the 256 runs after 4 steps (the CFL-chosen dt takes over
+    at the third) against the oracle's driver loop from the run's own seeded
+    state: fields within 1e-10 (max norm; p modulo its mean), time and the
+    next dt within 1e-12 / 1e-10 relative, Nu within 1e-10."""
+    from test_gpu_ensemble import oracle_driver
+    ens, cfgs, ctls = _ensemble("fp64")
+    orcs = []
+    pyoracle.set_threads(1)
+    for r, c in enumerate(cfgs):
+        o = pyoracle.Oracle(c)
+        for f in ("vx", "vz", "temp", "pr", "tempbp", "temptp"):
+            o.field(f)[...] = ens.download(r, f)
+        orcs.append(o)
+    print("ensemble: 256 runs, 4 steps", flush=True)
+    ens.step(4, graph=True)
+    worst = 0.0
+    for r, o in enumerate(orcs):
+        ref = oracle_driver(o, ctls[r], 4)
+        s = ens.run_state(r)
+        assert s["status"] == 0 and s["steps"] == 4, (r, s)
+        assert abs(s["time"] - ref["time"]) <= 1e-12 * ref["time"], r
+        assert abs(s["dt"] - ref["dt"]) <= 1e-10 * ref["dt"], r
+        nxm = cfgs[r]["nxm"]
+        for f in ("vx", "vz", "temp"):
+            sl = np.s_[...] if f == "vx" else np.s_[..., :nxm]
+            e = relerr(ens.download(r, f)[sl], o.field(f)[sl])
+            worst = max(worst, e)
+            assert e <= 1e-10, (r, f, e)
+        pg, po = ens.download(r, "pr")[..., :nxm], o.field("pr")[..., :nxm]
+        assert relerr(pg - pg.mean(), po - po.mean()) <= 1e-10, r
+        gq, go = ens.global_quantities(r), o.global_quantities()
+        assert abs(gq["nu"] - go["nu"]) <= 1e-10 * abs(go["nu"]), r
+        o.close()
+    print(f"ensemble: worst field error {worst:.3e}")
+    ens.close()
+
+
+def test_ensemble_256_runs_mixed_vs_fp64():
+    """The mixed ensemble against the FP64 ensemble after 20 steps of every
+    run: max|div u| <= 1e-10 in each run, Nu within 1e-6, the same CFL-chosen
+    dt within 1e-6, no run stopped."""
+    out = {}
+    for form in ("fp64", "mixed"):
+        ens, cfgs, _ = _ensemble(form)
+        ens.step(20, graph=True)
+        out[form] = ([ens.run_state(r) for r in range(len(cfgs))], [ens.global_quantities(r)["nu"] for r in range(len(cfgs))])
+        ens.close()
+    (sa, na), (sb, nb) = out["fp64"], out["mixed"]
+    worst = 0.0
+    for r in range(len(sa)):
+        assert sa[r]["status"] == 0 and sb[r]["status"] == 0, r
+        assert sa[r]["max_div"] <= 1e-12 and sb[r]["max_div"] <= 1e-10, (r, sa[r]["max_div"], sb[r]["max_div"])
+        assert abs(sb[r]["dt"] - sa[r]["dt"]) <= 1e-6 * sa[r]["dt"], r
+        e = abs(nb[r] - na[r]) / abs(na[r])
+        worst = max(worst, e)
+        assert e <= 1e-6, (r, e)
+    print(f"ensemble mixed vs fp64: worst Nu relative error {worst:.3e}")
+
+
+# ---------------------------------------------------------------------------
+# melting sphere, base 256x512^2, refined 512x1024^2
+# ---------------------------------------------------------------------------
+PF = "pf_sphere_256x512x512_512x1024x1024"
+
+
+def phi_volume(g, cfg):
+    """Solid fraction: the mean of the refined phi."""
+    return float(np.mean(interior(cfg, "phi", g.download("phi")), dtype=np.float64))
+
+
+@pytest.fixture(scope="module")
+def pf_full():
+    cfg, dt = workload(PF)
+    out = {}
+    ref = None
+    for form in ("fp64", "mixed"):
+        print(f"phase field full size, {form}: 100 steps", flush=True)
+        g = start(cfg, form, seed=4)
+        v0 = phi_volume(g, cfg)
+        steps(g, 50, dt)
+        v50 = phi_volume(g, cfg)
+        steps(g, 50, dt)
+        r = {"div": g.check_divergence()[0], "gq": g.global_quantities(), "vol": (v0, v50, phi_volume(g, cfg))}
+        fields = {n: interior(cfg, n, g.download(n)) for n in BASE + ["phi"]}
+        if form == "fp64":
+            ref = fields
+        else:
+            r["l2"] = {n: l2rel(fields[n], ref[n]) for n in fields}
+        del fields
+        out[form] = r
+        print(f"{form}: div {r['div']:.3e}, Nu {r['gq']['nu']:.15e}, solid fraction {r['vol']}", flush=True)
+        g.close()
+    return out
+
+
+def test_pf_full_size_properties(pf_full):
+    """Both forms after 100 steps: the sphere melts (its phi volume falls
+    monotonically: liquid at T = 1 - x > T_m = 0 around it), max|div u| <=
+    1e-12 (FP64) / 1e-10 (mixed), Nu and KE within 1e-6 between the forms,
+    the solid fraction within 1e-7 of each other, relative L2 of phi, T, p
+    <= 1e-5 and of u <= 1e-4."""
+    a, b = pf_full["fp64"], pf_full["mixed"]
+    for r in (a, b):
+        v0, v50, v100 = r["vol"]
+        assert v0 > v50 > v100 > 0.0, r["vol"]
+    assert a["div"] <= 1e-12
+    assert b["div"] <= 1e-10
+    for k in ("nu", "ke"):
+        e = abs(b["gq"][k] - a["gq"][k]) / abs(a["gq"][k])
+        print(f"{k}: relative error {e:.3e}")
+        assert e <= 1e-6, k
+    assert abs(b["vol"][2] - a["vol"][2]) <= 1e-7 * a["vol"][2]
+    print(f"mixed l2 {b['l2']}")
+    for n in ("phi", "temp", "pr"):
+        assert b["l2"][n] <= 1e-5, (n, b["l2"][n])
+    for n in ("vx", "vy", "vz"):
+        assert b["l2"][n] <= 1e-4, (n, b["l2"][n])
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_pf_cut_extent_vs_oracle(form):
+    """256x128^2 + 512x256^2 (cut by 4 in y and z, same spacing; the sphere of
+    radius 0.15 fits the 0.5-wide box), 4 steps against the oracle.  FP64
+    within 1e-10 (phi, tempr, phic, u, T, p); mixed relative L2 <= 1e-5 for
+    phi, T, p and 1e-4 for u, Nu within 1e-6."""
+    cfg, dt = workload(PF, cut=4)
+    oracle_threads()
+    g = start(cfg, form, seed=4)
+    o = oracle_from(g, cfg)
+    print(f"phase field cut extent, {form}: 4 steps against the oracle", flush=True)
+    steps(g, 4, dt)
+    for _ in range(4):
+        o.time_marcher(dt)
+    names = BASE + ["phi", "tempr"]
+    for f in names:
+        a, b = interior(cfg, f, g.download(f)), interior(cfg, f, o.field(f))
+        if form == "fp64":
+            e = relerr(a, b)
+            print(f"{f}: max-norm relative error {e:.3e}")
+            assert e <= 1e-10, (f, e)
+        else:
+            e = l2rel(a, b)
+            print(f"{f}: relative L2 {e:.3e}")
+            assert e <= (1e-4 if f in ("vx", "vy", "vz") else 1e-5), (f, e)
+    if form == "fp64":
+        e = relerr(g.download("phic")[..., :cfg["nxm"]], o.field("phic")[..., :cfg["nxm"]])
+        assert e <= 1e-10, e
+    qg, qo = g.global_quantities(), o.global_quantities()
+    assert abs(qg["nu"] - qo["nu"]) <= (1e-10 if form == "fp64" else 1e-6) * abs(qo["nu"])
+    g.close()
+    o.close()

@@ -260,3 +260,40 @@ def test_time_marcher_shapes(cfg):
     nxm = cfg["nxm"]
     for f in ("vx", "vy", "vz", "temp", "pr"):
         cmp(o, g, f, np.s_[..., :nxm] if f != "vx" else np.s_[...], tol=1e-10)
+
+
+# refined grid that does not bracket the base one (uniform refined grid over
+# a wall-clustered base grid): the general refined -> coarse path
+# (HermiteInterpolations.F90:218-250, salinity.F90:485-489)
+MR_GEN = dict(MR, nxm=24, nym=8, nzm=12, nxmr=40, nymr=16, nzmr=24, istr3=8, istr3r=0)
+
+
+def test_general_path_grid():
+    o = pyoracle.Oracle(MR_GEN)
+    assert o.grid("xmr")[0] >= o.grid("xm")[0]
+    o.close()
+
+
+@pytest.mark.parametrize("gaxis", [1, 2, 3])
+@pytest.mark.parametrize("base", ["MR", "MR_GEN"])
+def test_interp_sal_multigrid_paths(base, gaxis):
+    """Both refined -> coarse paths for every gravity axis: the fast path puts
+    salc on the faces along gAxis (cxsalc / cysalc / czsalc), the general path
+    on the cell centres."""
+    cfg = dict({"MR": MR, "MR_GEN": MR_GEN}[base], gaxis=gaxis)
+    o, g, _ = make_mr_pair(cfg)
+    o.call("interp_sal_multigrid"); g.call("interp_sal_multigrid")
+    cmp(o, g, "salc", np.s_[..., :cfg["nxm"]], tol=1e-13)
+
+
+@pytest.mark.parametrize("gaxis", [1, 2])
+def test_time_marcher_general_path(gaxis):
+    cfg = dict(MR_GEN, active_s=1, gaxis=gaxis)
+    o, g, _ = make_mr_pair(cfg)
+    for _ in range(2):
+        o.time_marcher(1e-3)
+        g.time_marcher(1e-3)
+    for f in ("vx", "vy", "vz", "temp", "pr", "salc"):
+        cmp(o, g, f, np.s_[..., :cfg["nxm"]] if f != "vx" else np.s_[...], tol=1e-10)
+    for f in ("sal", "vxr", "vyr", "vzr"):
+        cmp(o, g, f, tol=1e-10)

@@ -19,6 +19,10 @@ PF = dict(nxm=16, nym=12, nzm=12, istr3=8, multires=1, phasefield=1, nxmr=32, ny
 PF_ODD = dict(PF, nxm=12, nym=8, nzm=6, nxmr=30, nymr=20, nzmr=18, istr3=4, str3=1.4, istr3r=6, pf_a=0.7, pf_s=1.3,
               pf_tm=0.1, pf_eps=0.05)
 PF_BIG = dict(PF, nxm=40, nym=24, nzm=36, nxmr=80, nymr=72, nzmr=72)
+# uniform refined grid over a wall-clustered base grid: the refined cell
+# centres do not bracket the coarse ones, so phic and the latent heat take
+# the general path (phasefield.F90:489-492, :701-723)
+PF_GEN = dict(PF, nxm=24, nxmr=40, istr3=8, istr3r=0)
 TOL = 1e-12
 
 
@@ -90,7 +94,7 @@ def test_volume_penalty():
     cmp(o, g, "dq", np.s_[..., :nxm])
 
 
-@pytest.mark.parametrize("cfg", [PF, PF_ODD, PF_BIG], ids=["pf", "pf_odd", "pf_big"])
+@pytest.mark.parametrize("cfg", [PF, PF_ODD, PF_BIG, PF_GEN], ids=["pf", "pf_odd", "pf_big", "pf_general"])
 def test_latent_heat(cfg):
     o, g, st = make_pair(cfg)
     d = uniform(61, st["phi"].shape) - 0.5
@@ -102,7 +106,7 @@ def test_latent_heat(cfg):
     cmp(o, g, "hro", np.s_[..., : cfg["nxm"]])
 
 
-@pytest.mark.parametrize("cfg", [PF, PF_ODD, PF_BIG], ids=["pf", "pf_odd", "pf_big"])
+@pytest.mark.parametrize("cfg", [PF, PF_ODD, PF_BIG, PF_GEN], ids=["pf", "pf_odd", "pf_big", "pf_general"])
 def test_interp_phi_and_temp(cfg):
     o, g, _ = make_pair(cfg)
     o.call("interp_phi_multigrid"); g.call("interp_phi_multigrid")
@@ -131,7 +135,7 @@ def test_masked_velocity_solves(routine, field, h, cfg):
         assert solid.any() and np.all(v[..., :nxm][solid] == 0.0)
 
 
-@pytest.mark.parametrize("cfg", [PF, PF_ODD], ids=["pf", "pf_odd"])
+@pytest.mark.parametrize("cfg", [PF, PF_ODD, PF_GEN], ids=["pf", "pf_odd", "pf_general"])
 def test_time_marcher_phase(cfg):
     """The fused step (phi column, latent heat, masked base column, Poisson,
     transfers) against the oracle's TimeMarcher call tree."""

@@ -142,6 +142,10 @@ class HostSolver(pyafid.Solver):
     def call(self, routine):
         self.calls.append(routine)
 
+    def interp_input(self, old, fields):
+        self.calls.append("interp_input")
+        self.interp_args = (dict(old), {n: np.array(a) for n, a in fields.items()})
+
     def close(self):
         pass
 
@@ -218,12 +222,22 @@ def test_read_flow_field_without_pressure_keeps_it(tmp_path):
 
 
 @pytest.mark.parametrize("change", [dict(nym=8), dict(istr3=4), dict(str3=1.3), dict(nxmr=20), dict(istr3r=6)])
-def test_read_flow_field_rejects_another_grid(tmp_path, change):
+def test_read_flow_field_another_grid_interpolates(tmp_path, change):
+    """A checkpoint of another grid (ReadFlowInterp.F90:84-89) is read with
+    the old grid's dims and handed to the interpolation (:90-141) with the
+    old grid of the master file; pr is not read; the start-up transfers
+    follow."""
     a = filled(MR, 1)
     a.write_flow_field(tmp_path, time=1.0, final=True)
     b = filled(dict(MR, **change), 2)
-    with pytest.raises(pyafid.AfidError, match="new grid"):
-        b.read_flow_field(tmp_path)
+    t = b.read_flow_field(tmp_path)
+    assert t == 1.0
+    assert b.calls == ["interp_input", "interp_vel_mgrd", "interp_sal_multigrid"]
+    old, fields = b.interp_args
+    assert old == dict(nx=9, ny=7, nz=5, istr3=6, str3=1.2, nxr=17, nyr=13, nzr=9, istr3r=4, ylen=2.0, zlen=1.5)
+    assert sorted(fields) == ["sal", "temp", "vx", "vy", "vz"]
+    for n in fields:
+        assert np.array_equal(fields[n], a.arrays[n]), n
 
 
 def test_read_flow_field_str3_tolerance(tmp_path):
