@@ -10,13 +10,16 @@ generated on the device.
 
 Both forms start from the same seeded state and run the same steps; the
 mixed form's error is then reported against the FP64 form: max|div u|,
-Nu_vol and KE relative errors and relative L2 errors of the fields.
+Nu_vol and KE relative errors and relative L2 errors of the fields.  The
+error leg then runs both forms and the FP64 CPU oracle for 50 steps on the
+same grids cut by 4 in y and z (same spacing) and reports each form against
+the oracle: relative L2 of the fields, max|div u|, Nu and KE relative error.
 
 Prints ONE JSON line: value = Mcell-steps/s of the mixed form =
 (base + refined cells) x steps / s, with the HBM roofline of its dominant
 kernel group (HIP-event timing on the library's stream inside the timed
-region), the FP64 form's numbers beside it, and a CPU baseline (the FP64 CPU
-oracle, OpenMP, on a bounded sample of the same workload).
+region), the FP64 form's numbers beside it, and a CPU baseline: the FP64 CPU
+oracle (OpenMP) on the cut grids, its minimum step time over the error leg.
 """
 import argparse
 import json
@@ -279,6 +282,97 @@ def kernel_bytes(cfg, form, sweeps=1, checks=0):
     return b
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the baseline's label."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cut_workload(name, f):
+    """The workload with the periodic extent and sizes divided by f (same spacing)."""
+    wl = dict(WORKLOADS[name])
+    dt = wl.pop("dt")
+    wl.update(nym=max(wl["nym"] // f, 1), nzm=max(wl["nzm"] // f, 1), ylen=wl["ylen"] / f, zlen=wl["zlen"] / f)
+    if wl.get("salinity") or wl.get("phasefield"):
+        wl.update(nymr=wl["nymr"] // f, nzmr=wl["nzmr"] // f)
+    return wl, dt
+
+
+def error_leg(args):
+    """Both forms and the FP64 CPU oracle for args.error_steps steps on the
+    cut grids, from the same seeded state (made on the device, downloaded
+    into the oracle): each form against the oracle.  The oracle's per-step
+    times give the CPU baseline."""
+    import pyafid
+    import pyoracle
+    wl, dt = cut_workload(args.workload, args.cpu_shrink)
+    threads = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16))
+    pyoracle.set_threads(threads)
+    names = FIELDS_CMP + (["sal"] if wl.get("salinity") else []) + (["phi"] if wl.get("phasefield") else [])
+    gpu = {}
+    o = None
+    for form in ("fp64", "mixed"):
+        g = start_solver(wl, form, args.seed)
+        if o is None:
+            o = pyoracle.Oracle(wl)
+            state = FIELDS_CMP[:4] + ["tempbp", "temptp"] + (["sal", "salbp", "saltp"] if wl.get("salinity") else []) + \
+                (["phi"] if wl.get("phasefield") else [])
+            for n in state:
+                o.field(n)[...] = g.download(n)
+            if wl.get("salinity"):
+                o.call("interp_sal_multigrid")
+            if wl.get("phasefield"):
+                o.call("interp_temp_multigrid")
+                o.call("interp_phi_multigrid")
+        for _ in range(args.error_steps):
+            g.time_marcher(dt)
+        g.synchronize()
+        gpu[form] = {"div": g.check_divergence(), "gq": g.global_quantities(),
+                     "fields": {n: interior(wl, n, g.download(n)) for n in names}}
+        g.close()
+    log(f"error leg: oracle, {args.error_steps} steps on {wl['nxm']}x{wl['nym']}x{wl['nzm']}"
+        + (f" + {wl['nxmr']}x{wl['nymr']}x{wl['nzmr']}" if "nxmr" in wl else ""))
+    times = []
+    for _ in range(args.error_steps):
+        t0 = time.perf_counter()
+        o.time_marcher(dt)
+        times.append(time.perf_counter() - t0)
+    od = o.check_divergence()
+    oq = o.global_quantities()
+    ref = {n: interior(wl, n, o.field(n).copy()) for n in names}
+    o.close()
+    report = {"grid": {"base": [wl["nxm"], wl["nym"], wl["nzm"]],
+                       "refined": [wl["nxmr"], wl["nymr"], wl["nzmr"]] if "nxmr" in wl else None,
+                       "cut": args.cpu_shrink},
+              "steps": args.error_steps, "oracle": {"max_div": od[0], "max_div_refined": od[1], "nu_vol": oq["nu"], "ke": oq["ke"]}}
+    for form, r in gpu.items():
+        report[form + "_vs_oracle"] = {
+            "max_div": r["div"][0], "max_div_refined": r["div"][1],
+            "nu_rel": abs(r["gq"]["nu"] - oq["nu"]) / abs(oq["nu"]),
+            "ke_rel": abs(r["gq"]["ke"] - oq["ke"]) / max(abs(oq["ke"]), 1e-300),
+            "l2_rel": {n: l2_rel(r["fields"][n], ref[n]) for n in names}}
+    n = cells(wl)
+    best = min(times)
+    cpu = {"value": round(n / best / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+           "cpu": cpu_model(),
+           "sample": (f"FP64 CPU oracle (C, OpenMP {threads} threads on {cpu_model()}), the error leg's "
+                      f"{len(times)} RK3 steps of the same workload with the periodic extent cut by {args.cpu_shrink} "
+                      f"in y and z ({wl['nxm']}x{wl['nym']}x{wl['nzm']}" +
+                      (f" + {wl['nxmr']}x{wl['nymr']}x{wl['nzmr']}" if 'nxmr' in wl else "") +
+                      f" cells, same spacing); minimum step time {best:.3f} s, median {float(np.median(times)):.3f} s")}
+    return report, cpu
+
+
+def l2_rel(a, b):
+    return float(np.linalg.norm((np.asarray(a, np.float64) - np.asarray(b, np.float64)).ravel()) /
+                 max(np.linalg.norm(np.asarray(b, np.float64).ravel()), 1e-300))
+
+
 def cpu_baseline(args):
     """Time the FP64 CPU oracle on a bounded sample of the workload: the same
     wall-normal grids and grid spacings, periodic extent cut by 4 in y and z."""
@@ -337,7 +431,8 @@ def cpu_baseline(args):
     n = cells(wl)
     best = min(times)
     return {"value": round(n / best / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
-            "sample": (f"FP64 CPU oracle (C, OpenMP {threads} threads), {len(times)} RK3 step(s) of the same "
+            "cpu": cpu_model(),
+            "sample": (f"FP64 CPU oracle (C, OpenMP {threads} threads on {cpu_model()}), {len(times)} RK3 step(s) of the same "
                        f"workload with the periodic extent cut by {f} in y and z "
                        f"({wl['nxm']}x{wl['nym']}x{wl['nzm']}" +
                        (f" + {wl['nxmr']}x{wl['nymr']}x{wl['nzmr']}" if 'nxmr' in wl else "") +
@@ -348,7 +443,7 @@ FIELDS_CMP = ["vx", "vy", "vz", "temp", "pr"]
 
 
 def interior(wl, name, a):
-    if name in ("sal", "phi"):
+    if name in ("sal", "phi", "tempr"):
         return a[..., :wl["nxmr"]]
     if name == "vx":
         return a[..., 1:wl["nxm"]]
@@ -356,9 +451,10 @@ def interior(wl, name, a):
     return a - a.mean() if name == "pr" else a
 
 
-def run_form(wl, dt, form, args):
-    """Initialise, warm up and time one form; returns its measurements and
-    its final fields (for the error report)."""
+def start_solver(wl, form, seed):
+    """A context of the given form holding the seeded initial state: T = 1 -
+    x (and S = 1 - x) plus 1e-3 wall-clamped noise, u = 0; with the phase
+    field the solid sphere."""
     import pyafid
     g = pyafid.Solver(dict(wl, precision=pyafid.MIXED if form == "mixed" else pyafid.FP64))
     nz, ny = wl["nzm"], wl["nym"]
@@ -367,8 +463,15 @@ def run_form(wl, dt, form, args):
     if wl.get("salinity"):
         g.upload("salbp", np.ones((wl["nzmr"], wl["nymr"], 1)))
         g.upload("saltp", np.zeros((wl["nzmr"], wl["nymr"], 1)))
-    g.create_initial_conditions(args.seed, 1e-3)
+    g.create_initial_conditions(seed, 1e-3)
     g.synchronize()
+    return g
+
+
+def run_form(wl, dt, form, args):
+    """Initialise, warm up and time one form; returns its measurements and
+    its final fields (for the error report)."""
+    g = start_solver(wl, form, args.seed)
     log(f"{form}: {args.workload}, device memory {g.device_bytes() / 2**30:.1f} GiB")
     for _ in range(args.warmup):
         g.time_marcher(dt)
@@ -429,15 +532,37 @@ def run_form(wl, dt, form, args):
     return res
 
 
-# HBM bytes per launch of a dominant kernel group, measured with rocprofv3
-# FETCH_SIZE and WRITE_SIZE (one pass each, bench.py --form F --steps 1),
-# scaled by the same counters on a 2 GiB copy (fetch x2.0, write x1.0, the
-# --copy-calibration run).  Only valid for the kernels as measured: re-measure
-# after changing them.
-MEASURED_TRAFFIC = {
-    ("rbc_multires_384x768x768_768x1536x1536", "mixed", "column_sal"): 52.97e9,
-    ("rbc_multires_384x768x768_768x1536x1536", "mixed", "column_base"): 50.14e9,
-}
+def source_digest():
+    """sha256 (16 hex digits) of the kernel sources (afid-murphfi_amd/csrc/*.hip,
+    *.hpp), the build a traffic measurement belongs to."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    for f in sorted(glob.glob(os.path.join(ROOT, "afid-murphfi_amd", "csrc", "*.hip")) +
+                    glob.glob(os.path.join(ROOT, "afid-murphfi_amd", "csrc", "*.hpp"))):
+        h.update(os.path.basename(f).encode())
+        h.update(open(f, "rb").read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(workload, form, group):
+    """HBM bytes per launch of a kernel group, measured with rocprofv3
+    FETCH_SIZE and WRITE_SIZE (one pass each) and scaled by the same counters
+    on a plain copy (--copy-calibration).  Read from the lines
+    `traffic-per-launch: <workload> <form> <group> <bytes> sources=<digest>` of
+    the summaries in profiles/; a figure measured on other kernel sources than
+    these (another digest) is not used, so the result is None until the
+    counters are taken again."""
+    import glob
+    digest = source_digest()
+    for f in glob.glob(os.path.join(ROOT, "profiles", "*.md")):
+        for line in open(f):
+            if not line.startswith("traffic-per-launch:"):
+                continue
+            w = line.split()
+            if len(w) == 6 and w[1:4] == [workload, form, group] and w[5] == "sources=" + digest:
+                return float(w[4])
+    return None
 
 
 def roofline(wl, res, form, workload=None):
@@ -457,7 +582,7 @@ def roofline(wl, res, form, workload=None):
     per_launch = kb[dom] / nl
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     step_bytes = sum(kb.values()) * nsst
-    traffic = MEASURED_TRAFFIC.get((workload, form, dom))
+    traffic = measured_traffic(workload, form, dom)
     return ({"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
              "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": int(per_launch)},
@@ -479,6 +604,9 @@ def main():
     ap.add_argument("--cpu-shrink", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--error-steps", type=int, default=50, help="steps of the error leg on the cut grids")
+    ap.add_argument("--no-error-leg", action="store_true",
+                    help="skip the error leg (the CPU baseline then times a few oracle steps on its own)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
@@ -551,12 +679,21 @@ def main():
         summary["speedup_mixed_over_fp64"] = round(a["total_s"] / b["total_s"], 3)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    leg = None
+    if not args.no_error_leg:
+        log("error leg: both forms and the oracle on the cut grids")
+        try:
+            leg, cpu = error_leg(args)
+        except Exception as e:   # reported, never replaces the GPU number
+            leg = {"error": str(e)}
+    elif not args.no_cpu_baseline:
         log("CPU baseline")
         try:
             cpu = cpu_baseline(args)
         except Exception as e:   # reported, never replaces the GPU number
             cpu = {"error": str(e)}
+    if args.no_cpu_baseline:
+        cpu = None
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
@@ -573,7 +710,11 @@ def main():
         "step_hbm_frac": step_frac,
         "forms": summary,
         "mixed_vs_fp64": errors,
+        "mixed_vs_oracle": leg.get("mixed_vs_oracle") if leg else None,
+        "fp64_vs_oracle": leg.get("fp64_vs_oracle") if leg else None,
+        "error_leg": {k: v for k, v in leg.items() if not k.endswith("_vs_oracle")} if leg else None,
         "cpu_baseline": cpu,
+        "kernel_sources": source_digest(),
     }
     print(json.dumps(line))
 
