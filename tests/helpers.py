@@ -21,8 +21,11 @@ def uniform(seed, shape):
 
 
 def relerr(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    """Max-norm difference relative to max|b|; complex inputs are compared as
+    complex numbers (both parts)."""
+    a, b = np.asarray(a), np.asarray(b)
+    dt = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a, b = a.astype(dt), b.astype(dt)
     den = max(np.abs(b).max(), 1e-300)
     return float(np.abs(a - b).max() / den)
 
