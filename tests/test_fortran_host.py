@@ -225,7 +225,10 @@ def test_fortran_host_fields_match_oracle(tmp_path, case, mode):
     if c.get("pf"):
         o.call("interp_temp_multigrid")
         o.call("interp_phi_multigrid")
-        assert o.field("phi").max() > 0.9          # the solid sphere is there
+        # the solid sphere is there: the refined cell centre nearest the
+        # sphere's centre lies 0.033 from it (x cosine-clustered, y/z
+        # uniform 1/32), where 0.5(1 - tanh((r - 0.1)/(2/32))) = 0.895
+        assert o.field("phi").max() > 0.85
     for _ in range(ntst + 1):
         o.time_marcher(dt)
     for n in names + ["pr"]:
