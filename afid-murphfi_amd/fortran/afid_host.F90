@@ -282,6 +282,14 @@ subroutine ReadFlowInterp(time)
     call h5_check(afid_h5_read_real_scalar(cstr(master), cstr('str3'), str3o), 'read str3')
     call h5_check(afid_h5_read_real_scalar(cstr(master), cstr('ylen'), yleno), 'read ylen')
     call h5_check(afid_h5_read_real_scalar(cstr(master), cstr('zlen'), zleno), 'read zlen')
+    ! ReadFlowInterp.F90:76-81: the domain lengths are checked first, for
+    ! the same grid as for a new one
+    if (abs(cfg%ylen - yleno) > 1d-8 .or. abs(cfg%zlen - zleno) > 1d-8) then
+        write(0, '(a)') 'Continua domain size does not match bou.in'
+        write(0, '(a,2es16.8)') 'old Ly, Lz: ', yleno, zleno
+        write(0, '(a,2es16.8)') 'current Ly, Lz: ', cfg%ylen, cfg%zlen
+        error stop 1
+    end if
     nxro = 0; nyro = 0; nzro = 0; istr3ro = 0
     newgrid = (cfg%nxm + 1 /= nxo) .or. (cfg%nym + 1 /= nyo) .or. (cfg%nzm + 1 /= nzo) .or. &
               (cfg%istr3 /= istr3o) .or. (abs(cfg%str3 - str3o) > 1d-8)
@@ -294,7 +302,7 @@ subroutine ReadFlowInterp(time)
                   (cfg%nzmr + 1 /= nzro) .or. (cfg%istr3r /= istr3ro)
     end if
     if (newgrid) then
-        ! :90-141 (the domain lengths are checked by the library, :76-82)
+        ! :90-141
         og = afid_old_grid(nxo, nyo, nzo, istr3o, str3o, yleno, zleno, nxro, nyro, nzro, istr3ro)
         n3 = int(nxo, c_int64_t) * (nyo - 1) * (nzo - 1)
         allocate(vxo(n3), vyo(n3), vzo(n3), tempo(n3))

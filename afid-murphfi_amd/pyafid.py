@@ -573,10 +573,14 @@ class Solver:
         if self.cfgd["multires"]:
             old.update({k: h5_read_int(master, k) for k in ["nxr", "nyr", "nzr", "istr3r"]})
         t = h5_read_real(master, "time")
+        # :76-81: the domain lengths must match, on the same grid as on a new one
+        yleno, zleno = h5_read_real(master, "ylen"), h5_read_real(master, "zlen")
+        if abs(g["ylen"] - yleno) > 1e-8 or abs(g["zlen"] - zleno) > 1e-8:
+            raise AfidError(f"continua domain size does not match the configuration: old Ly, Lz = {yleno}, {zleno}; "
+                            f"current {g['ylen']}, {g['zlen']}")
         same = all(old[k] == g[k] for k in old if k != "str3") and abs(g["str3"] - old["str3"]) <= 1e-8
         if not same:
-            old["ylen"] = h5_read_real(master, "ylen")
-            old["zlen"] = h5_read_real(master, "zlen")
+            old["ylen"], old["zlen"] = yleno, zleno
             fields = {}
             for n in self._continua_fields(final=False):
                 ref = n in REFINED

@@ -304,3 +304,24 @@ def test_fortran_host_restart_onto_new_grid(tmp_path):
         e = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
         assert e <= 1e-11, f"{n}: {e:.3e}"
     o.close()
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+def test_fortran_host_restart_rejects_another_domain(tmp_path):
+    """NREAD = 1 with a checkpoint of another Ly on the same grid: the host
+    stops before reading any field, as ReadFlowInterp.F90:76-81 does
+    (MpiAbort with "Continua domain size does not match bou.in")."""
+    one, two = tmp_path / "one", tmp_path / "two"
+    os.makedirs(one)
+    os.makedirs(two)
+    bou(os.path.join(one, "bou.in"), 16, 8, 8, 2, 2e-3)
+    assert int(run_host(one, "fused")["errorcode"]) == 555
+    shutil.copytree(one / "outputdir", two / "outputdir")
+    bou(os.path.join(two, "bou.in"), 16, 8, 8, 2, 2e-3)
+    txt = open(os.path.join(two, "bou.in")).read()
+    txt = set_line(txt, "NREAD", [1, 0])
+    txt = set_line(txt, "ALX3", [1.0, 1.5, 1.0])
+    open(os.path.join(two, "bou.in"), "w").write(txt)
+    out = subprocess.run([EXE, "fused", "seed=7"], cwd=two, capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0
+    assert "domain size does not match" in out.stderr

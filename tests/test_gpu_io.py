@@ -61,15 +61,30 @@ def test_restart_is_exact(tmp_path, case, form):
         assert np.array_equal(got[n], v), f"{n}: max diff {np.abs(got[n] - v).max():.3e}"
 
 
-def test_restart_rejects_another_grid(tmp_path):
+def test_restart_on_another_grid_interpolates(tmp_path):
+    """A checkpoint of another refined grid is read through the restart
+    transfer (ReadFlowInterp.F90:90-141, tests/test_gpu_restart.py checks its
+    values): S lands on the new grid and the base fields, whose grid did not
+    change, come back unchanged.  Another domain length is refused
+    (ReadFlowInterp.F90:76-82)."""
     a = pyafid.Solver(MR)
     a.create_initial_conditions(3)
     a.write_flow_field(tmp_path, time=0.0, final=True)
+    base = {n: a.download(n) for n in ("temp", "vy")}
     a.close()
     b = pyafid.Solver(dict(MR, nxmr=48))
-    with pytest.raises(pyafid.AfidError, match="new grid"):
-        b.read_flow_field(tmp_path)
+    b.read_flow_field(tmp_path)
+    s = b.download("sal")
+    assert s.shape[-1] == 49 and np.isfinite(s).all()
+    assert np.abs(s[..., :48]).max() > 0.0
+    nxm = MR["nxm"]
+    for n, v in base.items():
+        assert np.allclose(b.download(n)[..., :nxm], v[..., :nxm], rtol=0, atol=1e-12), n
     b.close()
+    c = pyafid.Solver(dict(MR, ylen=2.0 * MR.get("ylen", 1.0)))
+    with pytest.raises(pyafid.AfidError, match="domain size"):
+        c.read_flow_field(tmp_path)
+    c.close()
 
 
 def test_means_file_matches_profiles(tmp_path):

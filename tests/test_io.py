@@ -240,6 +240,23 @@ def test_read_flow_field_another_grid_interpolates(tmp_path, change):
         assert np.array_equal(fields[n], a.arrays[n]), n
 
 
+@pytest.mark.parametrize("change", [dict(ylen=2.5), dict(zlen=1.0)])
+def test_read_flow_field_rejects_another_domain(tmp_path, change):
+    """ReadFlowInterp.F90:76-81 compares the domain lengths before anything
+    else: another Ly or Lz stops the restart on the same grid as on a new
+    one, and nothing is uploaded or interpolated."""
+    a = filled(MR, 1)
+    a.write_flow_field(tmp_path, time=1.0, final=True)
+    for grid in ({}, dict(nym=8)):
+        b = filled(dict(MR, **change, **grid), 2)
+        before = {n: v.copy() for n, v in b.arrays.items()}
+        with pytest.raises(pyafid.AfidError, match="domain size"):
+            b.read_flow_field(tmp_path)
+        assert b.calls == []
+        for n, v in before.items():
+            assert np.array_equal(b.arrays[n], v), n
+
+
 def test_read_flow_field_str3_tolerance(tmp_path):
     a = filled(MR, 1)
     a.write_flow_field(tmp_path, time=1.0, final=True)
