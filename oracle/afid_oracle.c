@@ -51,6 +51,9 @@ struct orc_state {
     double pf_a, pf_s, pf_tm, pf_d, pf_eps;
     double *ap3spkr, *ac3spkr, *am3spkr;                     /* (1:nxr), Neumann (CreateMgrdGrid.F90:163) */
     double *cxphic, *cyphic, *czphic;                        /* refined -> coarse centres, (4, n) */
+    /* moisture (moisture.F90:15-23, ReadMoistParameters :64-117) */
+    double q_alpha, q_beta, q_gamma, q_tau, pecq, kappa_q;
+    int q_fixn, q_fixs;
     int *irangr, *jrangr, *krangr;                           /* refined -> coarse centre indices (0:n_ref+1) */
     /* 3-D fields (x fastest, no halo) */
     double *f[ORC_NFIELDS];
@@ -74,7 +77,8 @@ void orc_set_threads(int n) { g_threads = n; }
  * CorrectPressure; 3 drops the 1/2 of the vx buoyancy average; 4 leaves vy
  * out of the kinetic energy; 5 takes vx at the lower face instead of the
  * cell-centre average in the Nusselt sum; 6 drops alpha from the x diffusion
- * of the vz RHS. */
+ * of the vz RHS; 7 flips the sign of the condensation heating; 8 takes the
+ * humidity x-diffusion with pect instead of pecq. */
 static int g_plant = 0;
 void orc_set_plant(int n) { g_plant = n; }
 static void apply_threads(void) {
@@ -340,6 +344,20 @@ orc_state *orc_create(const orc_config *cfg) {
         s->pf_d = 1.2 / s->pect / cfg->pf_s / cfg->pf_a;
         s->pf_eps = cfg->pf_eps > 0.0 ? cfg->pf_eps : 1.0 / s->nxmr;
     }
+    /* moisture.F90:99-110: pecq = pect / Sm, kappa_q = 1 / pecq, tau_q in
+     * diffusive units when ifdiff, and gamma_q < 0 chosen so that the
+     * moist-static jump is 1 (beta_q with the unsaturated top, qfixN = 2) */
+    if (cfg->moist) {
+        s->q_alpha = cfg->q_alpha; s->q_beta = cfg->q_beta;
+        s->pecq = s->pect / cfg->q_sm;
+        s->kappa_q = 1.0 / s->pecq;
+        s->q_tau = cfg->q_ifdiff ? s->pect * cfg->q_tau : cfg->q_tau;
+        s->q_gamma = cfg->q_gamma;
+        if (s->q_gamma < 0.0) s->q_gamma = cfg->q_fixn == 2 ? s->q_beta : s->q_beta / (1.0 - exp(-s->q_alpha));
+        /* SetHumidityBCs (:139-147) turns qfixN = 2 into a zero top value with qfixN = 1 */
+        s->q_fixn = cfg->q_fixn == 2 ? 1 : cfg->q_fixn;
+        s->q_fixs = cfg->q_fixs;
+    }
 
     /* InitTimeMarchScheme.F90:17-49 */
     if (cfg->nsst > 1) {
@@ -427,6 +445,11 @@ orc_state *orc_create(const orc_config *cfg) {
     s->f[ORC_TEMPBP] = track(s, sizeof(double) * (size_t)nym * nzm);
     s->f[ORC_TEMPTP] = track(s, sizeof(double) * (size_t)nym * nzm);
     s->tpdv = track(s, sizeof(double) * (size_t)(nx + 3) * nym * nzm);
+    if (cfg->moist) {
+        for (int id = ORC_HUMID; id <= ORC_QSAT; id++) s->f[id] = track(s, sizeof(double) * nbase(s));
+        s->f[ORC_HUMBP] = track(s, sizeof(double) * (size_t)nym * nzm);
+        s->f[ORC_HUMTP] = track(s, sizeof(double) * (size_t)nym * nzm);
+    }
 
     if (cfg->salinity || cfg->phasefield) {
         int nxr = s->nxr, nyr = s->nyr, nzr = s->nzr, nxmr = s->nxmr, nymr = s->nymr, nzmr = s->nzmr;
@@ -546,7 +569,8 @@ double *orc_field(orc_state *s, int id, int64_t dims[3]) {
     if (id < 0 || id >= ORC_NFIELDS || !s->f[id]) return NULL;
     int refined = (id >= ORC_SAL && id <= ORC_VZR) || id == ORC_SALBP || id == ORC_SALTP ||
                   id == ORC_PHI || id == ORC_RUPHI || id == ORC_HPHI || id == ORC_TEMPR;
-    int plane = id == ORC_TEMPBP || id == ORC_TEMPTP || id == ORC_SALBP || id == ORC_SALTP;
+    int plane = id == ORC_TEMPBP || id == ORC_TEMPTP || id == ORC_SALBP || id == ORC_SALTP ||
+                id == ORC_HUMBP || id == ORC_HUMTP;
     dims[0] = refined ? s->nzmr : s->nzm;
     dims[1] = refined ? s->nymr : s->nym;
     dims[2] = plane ? 1 : (refined ? s->nxr : s->nx);
@@ -1911,7 +1935,129 @@ void orc_interp_temp_multigrid(orc_state *s) {
             for (int k = 1; k <= nxmr; k++) tempr[R(k, j, i)] = tpdvr[TR(k, j, i)];
 }
 
-/* TimeMarcher.F90:13-165 (no IBM or moisture) */
+/* ------------------------------------------------------------------------ */
+/* moisture (src/moisture.F90): specific humidity on the base grid          */
+/* ------------------------------------------------------------------------ */
+void orc_moist_scalars(orc_state *s, double out[5]) {
+    out[0] = s->q_alpha; out[1] = s->q_beta; out[2] = s->q_gamma; out[3] = s->q_tau; out[4] = s->pecq;
+}
+
+/* SetHumidityBCs, moisture.F90:129-148: saturated walls, humbp =
+ * exp(alpha_q tempbp), humtp = exp(alpha_q (temptp - beta_q)); zero at the
+ * top with the unsaturated-top option qfixN = 2 */
+void orc_set_humidity_bcs(orc_state *s) {
+    BASE_DECL
+    const double *tbp = s->f[ORC_TEMPBP], *ttp = s->f[ORC_TEMPTP];
+    double *hbp = s->f[ORC_HUMBP], *htp = s->f[ORC_HUMTP];
+    for (int i0 = 0; i0 < nzm; i0++)
+        for (int j0 = 0; j0 < nym; j0++) {
+            hbp[P2(j0, i0)] = exp(s->q_alpha * tbp[P2(j0, i0)]);
+            htp[P2(j0, i0)] = s->c.q_fixn == 2 ? 0.0 : exp(s->q_alpha * (ttp[P2(j0, i0)] - s->q_beta));
+        }
+}
+
+/* ExplicitHumidity, moisture.F90:220-292: advection in flux form with the
+ * wall values, kappa_q times the y/z second differences, and the
+ * condensation sink -(q - qsat)/tau_q where q > qsat */
+void orc_explicit_humidity(orc_state *s) {
+    BASE_DECL
+    const double *vx = s->f[ORC_VX], *vy = s->f[ORC_VY], *vz = s->f[ORC_VZ], *q = s->f[ORC_HUMID];
+    const double *qs = s->f[ORC_QSAT], *hbp = s->f[ORC_HUMBP], *htp = s->f[ORC_HUMTP];
+    double *rk = s->f[ORC_HHUMID];
+    const double udy = 0.5 * s->dy, udz = 0.5 * s->dz, itau = 1.0 / s->q_tau;
+    apply_threads();
+#pragma omp parallel for
+    for (int ic = 1; ic <= nzm; ic++) {
+        int i0 = W(ic, nzm), im = W(ic - 1, nzm), ip = W(ic + 1, nzm);
+        for (int jc = 1; jc <= nym; jc++) {
+            int j0 = W(jc, nym), jm = W(jc - 1, nym), jp = W(jc + 1, nym);
+            for (int kc = 1; kc <= nxm; kc++) {
+                double hqx;
+                if (kc == 1)
+                    hqx = (vx[B(kc + 1, j0, i0)] * (q[B(kc + 1, j0, i0)] + q[B(kc, j0, i0)])
+                         - vx[B(kc, j0, i0)] * 2.0 * hbp[P2(j0, i0)]) * 0.5 * s->udx3m[kc];
+                else if (kc == nxm)
+                    hqx = (vx[B(kc + 1, j0, i0)] * 2.0 * htp[P2(j0, i0)]
+                         - vx[B(kc, j0, i0)] * (q[B(kc, j0, i0)] + q[B(kc - 1, j0, i0)])) * 0.5 * s->udx3m[kc];
+                else
+                    hqx = (vx[B(kc + 1, j0, i0)] * (q[B(kc + 1, j0, i0)] + q[B(kc, j0, i0)])
+                         - vx[B(kc, j0, i0)] * (q[B(kc, j0, i0)] + q[B(kc - 1, j0, i0)])) * 0.5 * s->udx3m[kc];
+                double hqy = (vy[B(kc, jp, i0)] * (q[B(kc, jp, i0)] + q[B(kc, j0, i0)])
+                            - vy[B(kc, j0, i0)] * (q[B(kc, j0, i0)] + q[B(kc, jm, i0)])) * udy;
+                double hqz = (vz[B(kc, j0, ip)] * (q[B(kc, j0, ip)] + q[B(kc, j0, i0)])
+                            - vz[B(kc, j0, i0)] * (q[B(kc, j0, i0)] + q[B(kc, j0, im)])) * udz;
+                double dyyq = (q[B(kc, jp, i0)] - 2.0 * q[B(kc, j0, i0)] + q[B(kc, jm, i0)]) * s->dyq;
+                double dzzq = (q[B(kc, j0, ip)] - 2.0 * q[B(kc, j0, i0)] + q[B(kc, j0, im)]) * s->dzq;
+                double r = s->kappa_q * (dyyq + dzzq) - (hqx + hqy + hqz);
+                if (q[B(kc, j0, i0)] > qs[B(kc, j0, i0)]) r = r + (qs[B(kc, j0, i0)] - q[B(kc, j0, i0)]) * itau;
+                rk[B(kc, j0, i0)] = r;
+            }
+        }
+    }
+}
+
+/* AddCondensation, moisture.F90:296-318: the latent heat of condensing
+ * vapour, gamma_q/tau_q (q - qsat) where q > qsat, into the T explicit term */
+void orc_add_condensation(orc_state *s) {
+    BASE_DECL
+    const double *q = s->f[ORC_HUMID], *qs = s->f[ORC_QSAT];
+    double *hro = s->f[ORC_HRO];
+    const double gtau = s->q_gamma / s->q_tau;
+    for (int i0 = 0; i0 < nzm; i0++)
+        for (int j0 = 0; j0 < nym; j0++)
+            for (int kc = 1; kc <= nxm; kc++) {
+                const size_t c = B(kc, j0, i0);
+                if (q[c] > qs[c]) hro[c] = hro[c] + (g_plant == 7 ? -gtau : gtau) * (q[c] - qs[c]);
+            }
+}
+
+/* UpdateSaturation, moisture.F90:321-333: qsat = exp(alpha_q (T - beta_q x)),
+ * T being the buoyancy field of this model */
+void orc_update_saturation(orc_state *s) {
+    BASE_DECL
+    const double *temp = s->f[ORC_TEMP];
+    double *qs = s->f[ORC_QSAT];
+    for (int i0 = 0; i0 < nzm; i0++)
+        for (int j0 = 0; j0 < nym; j0++)
+            for (int kc = 1; kc <= nxm; kc++)
+                qs[B(kc, j0, i0)] = exp(s->q_alpha * (temp[B(kc, j0, i0)] - s->q_beta * s->xm[kc]));
+}
+
+/* ImplicitHumidity, moisture.F90:336-373, and SolveImpEqnUpdate_Humidity,
+ * :377-418: the x-diffusion with the wall values (qfixS / qfixN), then the
+ * Crank-Nicolson solve with the T coefficients and pecq */
+void orc_implicit_humidity(orc_state *s) {
+    BASE_DECL
+    double *q = s->f[ORC_HUMID], *rhs = s->f[ORC_RHS], *hk = s->f[ORC_RUHUMID];
+    const double *rk = s->f[ORC_HHUMID], *hbp = s->f[ORC_HUMBP], *htp = s->f[ORC_HUMTP];
+    const double alpec = s->al / s->pecq;
+    apply_threads();
+#pragma omp parallel for
+    for (int ic = 1; ic <= nzm; ic++) {
+        int i0 = W(ic, nzm);
+        for (int jc = 1; jc <= nym; jc++) {
+            int j0 = W(jc, nym);
+            for (int kc = 1; kc <= nxm; kc++) {
+                double dxxq;
+                if (kc == 1)
+                    dxxq = q[B(kc + 1, j0, i0)] * s->ap3ssk[kc] + q[B(kc, j0, i0)] * s->ac3ssk[kc]
+                         - (s->ap3ssk[kc] + s->ac3ssk[kc]) * hbp[P2(j0, i0)] * s->q_fixs;
+                else if (kc == nxm)
+                    dxxq = -(s->am3ssk[kc] + s->ac3ssk[kc]) * htp[P2(j0, i0)] * s->q_fixn
+                         + q[B(kc, j0, i0)] * s->ac3ssk[kc] + q[B(kc - 1, j0, i0)] * s->am3ssk[kc];
+                else
+                    dxxq = q[B(kc + 1, j0, i0)] * s->ap3ssk[kc] + q[B(kc, j0, i0)] * s->ac3ssk[kc]
+                         + q[B(kc - 1, j0, i0)] * s->am3ssk[kc];
+                const double a = g_plant == 8 ? s->al / s->pect : alpec;
+                rhs[B(kc, j0, i0)] = (s->ga * rk[B(kc, j0, i0)] + s->ro * hk[B(kc, j0, i0)] + a * dxxq) * s->dt;
+                hk[B(kc, j0, i0)] = rk[B(kc, j0, i0)];
+            }
+        }
+    }
+    solve_columns(s, q, rhs, nx, nym, nzm, nxm, s->am3ssk, s->ac3ssk, s->ap3ssk, 0.5 * s->al * s->dt / s->pecq, 1, nxm, 1, nxm);
+}
+
+/* TimeMarcher.F90:13-165 (no IBM) */
 void orc_time_marcher(orc_state *s, double dt) {
     for (int ns = 1; ns <= s->c.nsst; ns++) {
         orc_set_substep(s, dt, ns);
@@ -1929,11 +2075,16 @@ void orc_time_marcher(orc_state *s, double dt) {
             orc_implicit_phase(s);
             orc_add_latent_heat(s);
         }
+        if (s->c.moist) {                       /* :79-81 */
+            orc_explicit_humidity(s);
+            orc_add_condensation(s);
+        }
         orc_implicit_and_update_vx(s);
         orc_implicit_and_update_vy(s);
         orc_implicit_and_update_vz(s);
         orc_implicit_and_update_temp(s);
         if (s->c.salinity) orc_implicit_salinity(s);
+        if (s->c.moist) orc_implicit_humidity(s);     /* :97 */
         orc_calc_local_divergence(s);
         orc_solve_pressure_correction(s);
         orc_correct_velocity(s);
@@ -1946,6 +2097,7 @@ void orc_time_marcher(orc_state *s, double dt) {
             orc_interp_temp_multigrid(s);
             orc_interp_phi_multigrid(s);
         }
+        if (s->c.moist) orc_update_saturation(s);     /* :154 */
     }
 }
 

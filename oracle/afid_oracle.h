@@ -34,6 +34,14 @@ typedef struct orc_config {
     /* phase field: pf_A, pf_S, pf_Tm of the bou.in PF line; interface width
        pf_eps (0 selects the reference's 1/nxmr, ReadInputFile.F90:126) */
     double pf_a, pf_s, pf_tm, pf_eps;
+    /* moisture (src/moisture.F90, humid.in: ReadMoistParameters :64-117):
+       moist on/off, alpha_q, beta_q, gamma_q (< 0: derived, :104-110),
+       tau_q (times pect when q_ifdiff), qfixN, qfixS, and Sm, the
+       humidity-to-thermal diffusivity ratio (pecq = pect / Sm) */
+    int32_t moist;
+    double q_alpha, q_beta, q_gamma, q_tau;
+    int32_t q_ifdiff, q_fixn, q_fixs;
+    double q_sm;
 } orc_config;
 
 typedef struct orc_state orc_state;
@@ -46,6 +54,9 @@ enum {
     ORC_SAL, ORC_RUSAL, ORC_HSAL, ORC_RHSR, ORC_VXR, ORC_VYR, ORC_VZR, ORC_SALC,
     ORC_TEMPBP, ORC_TEMPTP, ORC_SALBP, ORC_SALTP,
     ORC_PHI, ORC_RUPHI, ORC_HPHI, ORC_PHIC, ORC_TEMPR,
+    /* moisture.F90:10-13 (base grid): humid, hkhumid (RK history), rkhumid
+       (this substep's explicit term), qsat, and the wall planes humbp/humtp */
+    ORC_HUMID, ORC_RUHUMID, ORC_HHUMID, ORC_QSAT, ORC_HUMBP, ORC_HUMTP,
     ORC_NFIELDS
 };
 
@@ -97,6 +108,13 @@ void orc_add_latent_heat(orc_state *s);
 void orc_interp_temp_multigrid(orc_state *s);
 void orc_interp_phi_multigrid(orc_state *s);
 void orc_pf_scalars(orc_state *s, double out[4]);   /* pf_D, pf_eps, pf_eta, pf_B */
+/* moisture (moisture.F90) */
+void orc_set_humidity_bcs(orc_state *s);        /* SetHumidityBCs :129-148 */
+void orc_explicit_humidity(orc_state *s);       /* ExplicitHumidity :220-292 */
+void orc_add_condensation(orc_state *s);        /* AddCondensation :296-318 */
+void orc_implicit_humidity(orc_state *s);       /* ImplicitHumidity + SolveImpEqnUpdate_Humidity :336-418 */
+void orc_update_saturation(orc_state *s);       /* UpdateSaturation :321-333 */
+void orc_moist_scalars(orc_state *s, double out[5]);   /* alpha_q, beta_q, gamma_q, tau_q, pecq */
 void orc_time_marcher(orc_state *s, double dt);
 
 void orc_calc_max_cfl(orc_state *s, double *cflm, double *cflmr);

@@ -15,7 +15,8 @@ FIELDS = ["vx", "vy", "vz", "temp", "pr", "rux", "ruy", "ruz", "rutemp",
           "qcap", "dph", "dq", "hro", "rhs",
           "sal", "rusal", "hsal", "rhsr", "vxr", "vyr", "vzr", "salc",
           "tempbp", "temptp", "salbp", "saltp",
-          "phi", "ruphi", "hphi", "phic", "tempr"]
+          "phi", "ruphi", "hphi", "phic", "tempr",
+          "humid", "ruhumid", "hhumid", "qsat", "humbp", "humtp"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 
 GRID = ["xc", "xm", "g3rc", "g3rm", "d3xc", "d3xm", "udx3c", "udx3m",
@@ -42,6 +43,9 @@ class OrcConfig(C.Structure):
         ("active_t", C.c_int32), ("active_s", C.c_int32), ("gaxis", C.c_int32),
         ("xplusu", C.c_double), ("xminusu", C.c_double), ("dpdy", C.c_double), ("dpdz", C.c_double),
         ("pf_a", C.c_double), ("pf_s", C.c_double), ("pf_tm", C.c_double), ("pf_eps", C.c_double),
+        ("moist", C.c_int32), ("q_alpha", C.c_double), ("q_beta", C.c_double), ("q_gamma", C.c_double),
+        ("q_tau", C.c_double), ("q_ifdiff", C.c_int32), ("q_fixn", C.c_int32), ("q_fixs", C.c_int32),
+        ("q_sm", C.c_double),
     ]
 
 
@@ -73,7 +77,9 @@ def lib():
                      "correct_velocity", "correct_pressure", "explicit_salinity", "add_sal_buoyancy",
                      "implicit_salinity", "interp_vel_mgrd", "interp_sal_multigrid",
                      "explicit_phase", "add_volume_penalty", "implicit_phase", "add_latent_heat",
-                     "interp_temp_multigrid", "interp_phi_multigrid"]:
+                     "interp_temp_multigrid", "interp_phi_multigrid",
+                     "set_humidity_bcs", "explicit_humidity", "add_condensation", "implicit_humidity",
+                     "update_saturation"]:
             getattr(L, "orc_" + name).argtypes = [vp]
         L.orc_calc_max_cfl.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_check_divergence.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -92,6 +98,7 @@ def lib():
         L.orc_tridiag_solve.argtypes = [C.c_int, dp, dp, dp, dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_pf_scalars.argtypes = [vp, C.POINTER(C.c_double)]
+        L.orc_moist_scalars.argtypes = [vp, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -153,6 +160,11 @@ class Oracle:
         out = (C.c_double * 4)()
         self.L.orc_pf_scalars(self.h, out)
         return dict(zip(["pf_d", "pf_eps", "pf_eta", "pf_b"], list(out)))
+
+    def moist_scalars(self):
+        out = (C.c_double * 5)()
+        self.L.orc_moist_scalars(self.h, out)
+        return dict(zip(["alpha_q", "beta_q", "gamma_q", "tau_q", "pecq"], list(out)))
 
     def set_substep(self, dt, ns):
         self.L.orc_set_substep(self.h, dt, ns)
@@ -233,7 +245,8 @@ DEFAULTS = dict(nxm=16, nym=16, nzm=16, nsst=3, multires=0, nxmr=0, nymr=0, nzmr
                 istr3=8, str3=0.0, istr3r=8, rayt=1e5, prat=1.0, rays=-1e5, pras=1.0, ffscales=0,
                 inslws=1, inslwn=1, tfixs=1, tfixn=1, sfixs=1, sfixn=1,
                 active_t=1, active_s=0, gaxis=1, xplusu=0.0, xminusu=0.0, dpdy=0.0, dpdz=0.0,
-                pf_a=1.0, pf_s=1.0, pf_tm=0.0, pf_eps=0.0)
+                pf_a=1.0, pf_s=1.0, pf_tm=0.0, pf_eps=0.0,
+                moist=0, q_alpha=3.0, q_beta=1.2, q_gamma=0.19, q_tau=5e-5, q_ifdiff=1, q_fixn=1, q_fixs=1, q_sm=1.0)
 
 
 def make_config(d):
