@@ -360,7 +360,8 @@ int afid_ens_init(const afid_config *cfgs, const afid_run_control *ctl, int32_t 
         const afid_config &g = cfgs[r];
         const std::string who = "ensemble run " + std::to_string(r) + ": ";
         if (g.nym != 1) return ens_err(e, AFID_E_ARG, who + "the ensemble holds 2-D runs (nym = 1)");
-        if (g.multires || g.salinity || g.phasefield) return ens_err(e, AFID_E_ARG, who + "no refined grid, salinity or phase field");
+        if (g.multires || g.salinity || g.phasefield || g.moist)
+            return ens_err(e, AFID_E_ARG, who + "no refined grid, salinity, phase field or moisture");
         if (g.precision != cfgs[0].precision) return ens_err(e, AFID_E_ARG, who + "every run needs the same precision");
         if (g.nsst != e->nsst) return ens_err(e, AFID_E_ARG, who + "every run needs the same nsst");
         if (g.gaxis == 2) return ens_err(e, AFID_E_ARG, who + "gravity along y has no effect in a 2-D run (gaxis 1 or 3)");

@@ -26,6 +26,7 @@
 #include "afid_means.hpp"
 #include "afid_restart.hpp"
 #include "afid_ibm.hpp"
+#include "afid_moist.hpp"
 
 using namespace afid;
 
@@ -91,6 +92,13 @@ struct afid_ctx {
     double *phi_alt = nullptr;
     float *phi_alt_f = nullptr;                 // second phi buffer of the mixed form
     CoarseTab tab_sal, tab_phi;                 // InterpSalMultigrid / InterpPhiMultigrid + AddLatentHeat
+    // moisture (moisture.F90:15-23): the parameters after ReadMoistParameters'
+    // derivations, the wall flags after SetHumidityBCs, the second humidity
+    // buffer of the fused column kernel
+    bool moist = false;
+    double q_alpha = 0, q_beta = 0, q_gamma = 0, q_tau = 0, pecq = 0, kappa_q = 0;
+    int q_fixn = 1, q_fixs = 1;
+    double *humid_alt = nullptr;
     // arithmetic form
     bool mixed = false;
     double div_tol = 1e-10;
@@ -107,6 +115,7 @@ struct afid_ctx {
     double *d_phi_part = nullptr, *d_phi_out = nullptr;
     int mean_groups = 0, sal_groups = 0, phi_groups = 0;
     int col_tz = 0;                     // z extent of the base column tiles (0: largest that fits)
+    int col_nw = 0;                     // >0: single-pass base column kernel with this many wavefronts per workgroup
     // mixed form, gAxis = 1: the FP32 vx explicit term and its history hold
     // the buoyancy minus a per-row FP64 reference bref (that of the
     // conductive profile between the mean wall temperatures), which the FP64
@@ -212,8 +221,9 @@ static bool is_refined(int f) {
            f == AFID_PHI || f == AFID_RUPHI || f == AFID_HPHI || f == AFID_TEMPR;
 }
 static bool is_plane(int f) {
-    return f == AFID_TEMPBP || f == AFID_TEMPTP || f == AFID_SALBP || f == AFID_SALTP;
+    return f == AFID_TEMPBP || f == AFID_TEMPTP || f == AFID_SALBP || f == AFID_SALTP || f == AFID_HUMBP || f == AFID_HUMTP;
 }
+static bool is_moist_field(int f) { return f >= AFID_HUMID && f <= AFID_HUMTP; }
 
 static KP make_kp(const afid_ctx *c) {
     KP p{};
@@ -258,6 +268,13 @@ static KP make_kp(const afid_ctx *c) {
     p.pf = c->pf ? 1 : 0;
     p.bref = c->d_bref;
     p.ap3spkr = c->d_ap3spkr; p.ac3spkr = c->d_ac3spkr; p.am3spkr = c->d_am3spkr;
+    p.humid = c->f[AFID_HUMID]; p.ruhumid = c->f[AFID_RUHUMID]; p.hhumid = c->f[AFID_HHUMID]; p.qsat = c->f[AFID_QSAT];
+    p.humbp = c->f[AFID_HUMBP]; p.humtp = c->f[AFID_HUMTP]; p.xm = c->d_xm;
+    p.moist = c->moist ? 1 : 0;
+    if (c->moist) {
+        p.q_alpha = c->q_alpha; p.q_beta = c->q_beta; p.q_gtau = c->q_gamma / c->q_tau; p.q_itau = 1.0 / c->q_tau;
+        p.kappa_q = c->kappa_q; p.pecq = c->pecq; p.q_fixn = c->q_fixn; p.q_fixs = c->q_fixs;
+    }
     return p;
 }
 
@@ -328,6 +345,7 @@ static int ensure_factors(afid_ctx *c) {
         for (int kind = 0; kind < MAT_NKIND; ++kind) {
             if (kind == MAT_S && !c->sal) continue;
             if (kind == MAT_P && !c->pf) continue;
+            if (kind == MAT_Q && !c->moist) continue;
             double *base = c->tri_host.data() + size_t((s * MAT_NKIND + kind) * kTriArrays * np);
             double *L = base, *E = base + np, *G = base + 2 * np, *S = base + 3 * np, *A = base + 4 * np, *C = base + 5 * np;
             switch (kind) {
@@ -345,6 +363,9 @@ static int ensure_factors(afid_ctx *c) {
                 break;
             case MAT_P:   // SolveImpEqnUpdate_Phi, phasefield.F90:427-443
                 lu_rows(c->h_am3spkr, c->h_ac3spkr, c->h_ap3spkr, 0.5 * al * c->dt * c->pf_d, c->nxmr, 0, c->nxmr, np, L, E, G, S, A, C);
+                break;
+            case MAT_Q:   // SolveImpEqnUpdate_Humidity, moisture.F90:382-396
+                lu_rows(c->h_am3ssk, c->h_ac3ssk, c->h_ap3ssk, 0.5 * al * c->dt / c->pecq, c->nxm, 0, c->nxm, np, L, E, G, S, A, C);
                 break;
             }
         }
@@ -462,6 +483,20 @@ static int setup(afid_ctx *c) {
         c->heta = 0.5 / (c->ren * std::pow(1.51044285 * c->pf_eps, 2));
         c->pf_r0 = g.pf_r0 > 0.0 ? g.pf_r0 : 0.1;
     }
+    // moisture.F90:44-56, :99-110 (the humid.in values; pecq = pect / Sm)
+    c->moist = g.moist != 0;
+    if (c->moist) {
+        if (!(g.q_sm > 0.0) || !(g.q_tau > 0.0)) return set_err(c, AFID_E_ARG, "moisture needs q_sm > 0 and q_tau > 0");
+        if (g.q_fixn < 0 || g.q_fixn > 2 || g.q_fixs < 0 || g.q_fixs > 1) return set_err(c, AFID_E_ARG, "qfixN must be 0, 1 or 2 and qfixS 0 or 1");
+        c->q_alpha = g.q_alpha; c->q_beta = g.q_beta;
+        c->pecq = c->pect / g.q_sm;
+        c->kappa_q = 1.0 / c->pecq;
+        c->q_tau = g.q_ifdiff ? c->pect * g.q_tau : g.q_tau;
+        c->q_gamma = g.q_gamma;
+        if (c->q_gamma < 0.0) c->q_gamma = g.q_fixn == 2 ? c->q_beta : c->q_beta / (1.0 - std::exp(-c->q_alpha));
+        c->q_fixn = g.q_fixn == 2 ? 1 : g.q_fixn;      // SetHumidityBCs, :139-146
+        c->q_fixs = g.q_fixs;
+    }
     // ExplicitTermsVY.F90:30-34
     c->Gy = (g.inslwn == 0 || g.inslws == 0) ? g.dpdy * g.dpdy / (c->ren * c->ren) : 8.0 * g.dpdy * g.dpdy / (c->ren * c->ren);
     // InitTimeMarchScheme.F90:17-49
@@ -560,6 +595,7 @@ static int setup(afid_ctx *c) {
     const size_t nb = size_t(c->nz) * c->ny * c->ldx;
     for (int f = 0; f < AFID_NFIELDS; ++f) {
         if (is_refined(f) || f == AFID_RHS || is_plane(f) || f == AFID_PHIC) continue;
+        if (is_moist_field(f) && !c->moist) continue;      // humidity is FP64 in both forms
         // (hro carries the latent heat of the phase field in both forms)
         if (c->mixed && (f == AFID_QCAP || f == AFID_DQ || (f == AFID_HRO && !c->pf))) continue;
         if (c->mixed && ((f >= AFID_RUX && f <= AFID_RUTEMP) || f == AFID_SALC || f == AFID_DPH)) {
@@ -578,6 +614,11 @@ static int setup(afid_ctx *c) {
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->alt[q]), sizeof(double) * nb))) return rc;
     for (int f : {AFID_TEMPBP, AFID_TEMPTP})
         if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * size_t(c->ny) * c->nz))) return rc;
+    if (c->moist) {
+        for (int f : {AFID_HUMBP, AFID_HUMTP})
+            if ((rc = dalloc(c, reinterpret_cast<void **>(&c->f[f]), sizeof(double) * size_t(c->ny) * c->nz))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->humid_alt), sizeof(double) * nb))) return rc;
+    }
     const int csize = c->mixed ? int(sizeof(cpx<float>)) : int(sizeof(cpx<double>));
     if ((rc = dalloc(c, &c->spec, size_t(csize) * size_t(c->nz) * c->nymh * c->nxm))) return rc;
     if ((rc = dalloc(c, reinterpret_cast<void **>(&c->slots), sizeof(unsigned long long) * 4))) return rc;
@@ -652,6 +693,7 @@ static int setup(afid_ctx *c) {
     // tridiagonal factor storage
     c->nrows[MAT_X] = nx; c->nrows[MAT_YZ] = nxm; c->nrows[MAT_T] = nxm; c->nrows[MAT_S] = c->sal ? c->nxmr : 0;
     c->nrows[MAT_P] = c->pf ? c->nxmr : 0;
+    c->nrows[MAT_Q] = c->moist ? nxm : 0;
     int chmax = 1;
     for (int k = 0; k < MAT_NKIND; ++k) { c->chunk[k] = std::max(1, (c->nrows[k] + 63) / 64); chmax = std::max(chmax, c->chunk[k]); }
     if (chmax > 16) return set_err(c, AFID_E_ARG, "wall-normal size above 1024 points is not supported");
@@ -920,6 +962,7 @@ static int ctx_create(const afid_config *cfg, hipStream_t shared, afid_ctx **out
     }
     c->device = dev;
     if (const char *e = std::getenv("AFID_COLUMN_TZ")) c->col_tz = std::atoi(e);
+    if (const char *e = std::getenv("AFID_COLUMN_NW")) c->col_nw = std::atoi(e);
     if (shared) {
         c->stream = shared;
         c->own_stream = false;
@@ -1057,6 +1100,7 @@ int afid_set_substep(afid_ctx *c, double dt, int ns) {
 #define NEED_SUBSTEP(c) if (!(c) || (c)->ns == 0) return set_err((c), AFID_E_STATE, "no substep set (afid_set_substep)");
 #define NEED_SAL(c) if (!(c)->sal) return set_err((c), AFID_E_STATE, "salinity is not enabled");
 #define NEED_FP64(c) if ((c)->mixed) return set_err((c), AFID_E_STATE, "the per-routine calls belong to the FP64 form; the mixed form runs through afid_time_marcher");
+#define NEED_MOIST(c) if (!(c) || !(c)->moist) return set_err((c), AFID_E_STATE, "moisture is not enabled (moist = 0)");
 
 static int launch_point(afid_ctx *c, void (*kern)(KP), bool refined, const char *name) {
     KP p = make_kp(c);
@@ -1217,6 +1261,37 @@ int afid_implicit_salinity(afid_ctx *c) {
 }
 
 int afid_calc_local_divergence(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return launch_point(c, k_local_divergence, false, "k_local_divergence"); }
+
+// moisture.F90 (the per-routine call tree; afid_time_marcher fuses them)
+int afid_set_humidity_bcs(afid_ctx *c) {
+    NEED_MOIST(c)
+    KP p = make_kp(c);
+    const long long n = (long long)c->ny * c->nz;
+    hipLaunchKernelGGL(k_humidity_bcs, dim3(unsigned((n + 255) / 256)), dim3(256), 0, c->stream, p, c->f[AFID_HUMBP],
+                       c->f[AFID_HUMTP], c->c.q_fixn == 2 ? 1 : 0);
+    return check_launch(c, "k_humidity_bcs");
+}
+int afid_explicit_humidity(afid_ctx *c) {
+    NEED_MOIST(c) NEED_SUBSTEP(c) NEED_FP64(c)
+    return launch_point(c, k_explicit_humid, false, "k_explicit_humid");
+}
+int afid_add_condensation(afid_ctx *c) {
+    NEED_MOIST(c) NEED_SUBSTEP(c) NEED_FP64(c)
+    return launch_point(c, k_add_condensation, false, "k_add_condensation");
+}
+int afid_implicit_humidity(afid_ctx *c) {
+    NEED_MOIST(c) NEED_SUBSTEP(c) NEED_FP64(c)
+    KP p = make_kp(c);
+    hipLaunchKernelGGL(k_rhs_humid, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, p, c->rhs4[3]);
+    int rc = check_launch(c, "k_rhs_humid");
+    if (!rc) rc = run_tri(c, MAT_Q, c->rhs4[3], c->f[AFID_HUMID]);
+    c->f[AFID_RHS] = c->rhs4[3];            // the reference's rhs holds the last solve's increment
+    return rc;
+}
+int afid_update_saturation(afid_ctx *c) {
+    NEED_MOIST(c)
+    return launch_point(c, k_update_saturation, false, "k_update_saturation");
+}
 
 }  // extern "C"
 
@@ -1441,12 +1516,51 @@ static void launch_column_base_form(afid_ctx *c, const KP &p) {
     else if (tz >= 2) launch_column_base_tz<CH, HT, PF, 2>(c, p);
     else launch_column_base_tz<CH, HT, PF, 1>(c, p);
 }
+// single-pass kernel (no phase field): NW wavefronts per workgroup
+template <int CH, typename HT, int NW>
+static void launch_column_base1(afid_ctx *c, const KP &p) {
+    BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
+    const long long nblk = strip_blocks_w<NW>(c->ny, c->nz);
+    hipLaunchKernelGGL((k_column_base1<CH, HT, NW>), dim3(unsigned((nblk + 7) / 8 * 8)), dim3(64 * NW), 0, c->stream, p,
+                       factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
+}
 template <int CH>
 static void launch_column_base(afid_ctx *c, const KP &p) {
+    if (!c->pf && c->col_nw > 0) {
+        if (c->mixed) {
+            if (c->col_nw >= 2) launch_column_base1<CH, float, 2>(c, p);
+            else launch_column_base1<CH, float, 1>(c, p);
+        } else {
+            if (c->col_nw >= 2) launch_column_base1<CH, double, 2>(c, p);
+            else launch_column_base1<CH, double, 1>(c, p);
+        }
+        return;
+    }
     if (c->mixed && c->pf) launch_column_base_form<CH, float, true>(c, p);
     else if (c->mixed) launch_column_base_form<CH, float, false>(c, p);
     else if (c->pf) launch_column_base_form<CH, double, true>(c, p);
     else launch_column_base_form<CH, double, false>(c, p);
+}
+// ExplicitHumidity + ImplicitHumidity + SolveImpEqnUpdate_Humidity of every
+// column into humid_alt; the caller swaps the buffers once the base column
+// kernel (which reads the old humidity for the condensation heat) has run
+template <int CH>
+static void launch_column_humid(afid_ctx *c, const KP &p) {
+    const long long nblk = strip_blocks(c->ny, c->nz);
+    hipLaunchKernelGGL((k_column_humid<CH>), dim3(unsigned((nblk + 7) / 8 * 8)), dim3(256), 0, c->stream, p, factors(c, MAT_Q),
+                       c->humid_alt);
+}
+static int column_humid(afid_ctx *c) {
+    KP p = make_kp(c);
+    switch (c->chunk[MAT_Q]) {
+#define AFID_CQ_CASE(CH) case CH: launch_column_humid<CH>(c, p); break;
+        AFID_CQ_CASE(1) AFID_CQ_CASE(2) AFID_CQ_CASE(3) AFID_CQ_CASE(4) AFID_CQ_CASE(5) AFID_CQ_CASE(6)
+        AFID_CQ_CASE(7) AFID_CQ_CASE(8) AFID_CQ_CASE(9) AFID_CQ_CASE(10) AFID_CQ_CASE(11) AFID_CQ_CASE(12)
+        AFID_CQ_CASE(13) AFID_CQ_CASE(14) AFID_CQ_CASE(15) AFID_CQ_CASE(16)
+#undef AFID_CQ_CASE
+    default: return set_err(c, AFID_E_ARG, "nxm too large");
+    }
+    return check_launch(c, "k_column_humid");
 }
 static int column_base(afid_ctx *c) {
     KP p = make_kp(c);
@@ -1537,9 +1651,18 @@ int afid_time_marcher(afid_ctx *c, double dt) {
             tstop(c);
             if ((rc = latent_heat(c, c->f[AFID_HRO], 0))) return rc;
         }
+        if (c->moist) {
+            // TimeMarcher.F90:79-81, :97: the humidity column reads the old
+            // velocity and humidity, the base column reads the old humidity
+            // for the condensation heat; both write second buffers
+            tstart(c, "moist_column_humid");
+            if ((rc = column_humid(c))) return rc;
+            tstop(c);
+        }
         tstart(c, "column_base");
         if ((rc = column_base(c))) return rc;
         tstop(c);
+        if (c->moist) std::swap(c->f[AFID_HUMID], c->humid_alt);
         if (c->sal) {
             tstart(c, "column_sal");
             if ((rc = column_sal(c))) return rc;
@@ -1560,6 +1683,11 @@ int afid_time_marcher(afid_ctx *c, double dt) {
         if (c->pf) {
             if ((rc = afid_interp_temp_multigrid(c))) return rc;
             if ((rc = afid_interp_phi_multigrid(c))) return rc;
+        }
+        if (c->moist) {                           // TimeMarcher.F90:154
+            tstart(c, "moist_saturation");
+            if ((rc = afid_update_saturation(c))) return rc;
+            tstop(c);
         }
     }
     return AFID_OK;

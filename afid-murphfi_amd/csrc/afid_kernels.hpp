@@ -330,20 +330,24 @@ __device__ __forceinline__ bool strip_column(long long blk, int wave, int ny, in
 // (lane * CHP) hit distinct bank pairs.
 template <int CH>
 constexpr int lds_chunk() { return (CH + 1) % 2 ? CH + 1 : CH + 2; }
+// Tight form: CH itself when odd (lane strides of 2*CH dwords reach 16
+// distinct bank pairs per 16 lanes), else CH+1.
 template <int CH>
-__device__ __forceinline__ unsigned lds_row(unsigned r) { return (r / CH) * lds_chunk<CH>() + r % CH; }
-template <int CH>
-constexpr int lds_col_size() { return 64 * lds_chunk<CH>(); }
+constexpr int lds_chunk_tight() { return CH % 2 ? CH : CH + 1; }
+template <int CH, int CHP = lds_chunk<CH>()>
+__device__ __forceinline__ unsigned lds_row(unsigned r) { return (r / CH) * CHP + r % CH; }
+template <int CH, int CHP = lds_chunk<CH>()>
+constexpr int lds_col_size() { return 64 * CHP; }
 
-template <int CH>
+template <int CH, int CHP = lds_chunk<CH>()>
 __device__ __forceinline__ void column_solve_lds(double *buf, const TriFactors &f) {
     const int lane = threadIdx.x & 63;
     double v[CH];
 #pragma unroll
-    for (int m = 0; m < CH; ++m) v[m] = buf[lane * lds_chunk<CH>() + m];
+    for (int m = 0; m < CH; ++m) v[m] = buf[lane * CHP + m];
     wave_tri_solve<CH>(v, f);
 #pragma unroll
-    for (int m = 0; m < CH; ++m) buf[lane * lds_chunk<CH>() + m] = v[m];
+    for (int m = 0; m < CH; ++m) buf[lane * CHP + m] = v[m];
 }
 
 // two-element vector of the storage type (64-bit or 128-bit lane accesses)
@@ -557,6 +561,7 @@ __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long 
                 if (r < nxm) {
                     HT h = ex_temp<HT>(p, n, r);
                     if (PF) h += HT(p.qlat[c0 + r]);
+                    if (p.moist) h = HT(double(h) + condensation_heat(p, c0, r));
                     rhs = rhs_temp(p, n, double(h), double(ru[r]), r);
                     ru[r] = h;
                 }
@@ -607,6 +612,113 @@ __global__ void __launch_bounds__(256 * TZ) k_column_base(KP p, TriFactors fx, T
             }
         }
         __syncthreads();
+    }
+}
+
+// Column walk of the single-pass kernel: workgroups of NW wavefronts cover NW
+// consecutive y columns of a strip 16 columns wide; the strips are walked
+// along z as in strip_column.
+constexpr int STRIP_W = 16;
+template <int NW>
+__host__ __device__ __forceinline__ long long strip_blocks_w(int ny, int nz) {
+    return (long long)((ny + STRIP_W - 1) / STRIP_W) * (STRIP_W / NW) * nz;
+}
+template <int NW>
+__device__ __forceinline__ bool strip_column_w(long long blk, int wave, int ny, int nz, int &j, int &i) {
+    constexpr int SB = STRIP_W / NW;
+    const long long per_strip = (long long)SB * nz;
+    const long long strip = blk / per_strip, rem = blk % per_strip;
+    i = int(rem / SB);
+    j = int(strip * STRIP_W + (rem % SB) * NW + wave);
+    const bool ok = blk < strip_blocks_w<NW>(ny, nz) && j < ny && i < nz;
+    if (!ok) { j = 0; i = 0; }
+    return ok;
+}
+
+// Right-hand sides of vx, vy, vz and T at row r < nx of one column (the
+// stencil phase of column_base_rows for the four fields at once, no phase
+// field): every neighbour value is loaded once for the four explicit terms,
+// and the explicit terms are stored to their history arrays after all
+// loads.  Rows without an equation (vx rows 0 and nxm, the unused index
+// nxm of the others) give 0.
+template <typename HT>
+__device__ __forceinline__ void base_rhs4(const KP &p, const Nb &n, long long c0, unsigned r, double (&rhs)[4]) {
+    const unsigned nxm = p.nxm;
+    HT *rux = sel<HT>(p.rux, p.ruxf) + c0, *ruy = sel<HT>(p.ruy, p.ruyf) + c0;
+    HT *ruz = sel<HT>(p.ruz, p.ruzf) + c0, *rut = sel<HT>(p.rut, p.rutf) + c0;
+    const bool wx = r >= 1 && r < nxm, wc = r < nxm;
+    const HT sb = (p.active_s == 1 && wc) ? HT(p.bycs) * sel<HT>(p.salc, p.salcf)[c0 + r] : HT(0);
+    HT h0 = HT(0), h1 = HT(0), h2 = HT(0), h3 = HT(0);
+    double bref = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    if (wx) {
+        if (sizeof(HT) == sizeof(float) && p.bref) {
+            // FP32 terms: buoyancy as its deviation from the row's FP64
+            // reference, which the RHS adds back in FP64
+            bref = p.bref[r];
+            h0 = ex_vx<HT, false>(p, n, r) + HT(p.byct * (p.active_t * 0.5 * (n.t[r] + n.t[r - 1])) - bref);
+        } else {
+            h0 = ex_vx<HT>(p, n, r);
+        }
+        if (p.gaxis == 1) h0 -= sb;
+        r0 = double(rux[r]);
+    }
+    if (wc) {
+        h1 = ex_vy<HT>(p, n, r);
+        if (p.gaxis == 2) h1 -= sb;
+        h2 = ex_vz<HT>(p, n, r);
+        if (p.gaxis == 3) h2 -= sb;
+        h3 = ex_temp<HT>(p, n, r);
+        if (p.moist) h3 = HT(double(h3) + condensation_heat(p, c0, r));
+        r1 = double(ruy[r]); r2 = double(ruz[r]); r3 = double(rut[r]);
+    }
+    rhs[0] = wx ? rhs_vx(p, n, double(h0) + bref, r0 + bref, r) : 0.0;
+    rhs[1] = wc ? rhs_vy(p, n, double(h1), r1, r) : 0.0;
+    rhs[2] = wc ? rhs_vz(p, n, double(h2), r2, r) : 0.0;
+    rhs[3] = wc ? rhs_temp(p, n, double(h3), r3, r) : 0.0;
+    if (wx) rux[r] = h0;
+    if (wc) { ruy[r] = h1; ruz[r] = h2; rut[r] = h3; }
+}
+
+// Single-pass form of k_column_base (no phase field): one sweep over the rows
+// forms the four right-hand sides into four LDS columns per wavefront, the
+// four tridiagonal solves follow, and the update adds each solution to the
+// old value of the own column, read again from global memory (in cache).
+// LDS per wavefront: 4 columns of 64 x lds_chunk_tight<CH>() doubles.
+template <int CH, typename HT, int NW>
+__global__ void __launch_bounds__(64 * NW) k_column_base1(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
+    constexpr int CHP = lds_chunk_tight<CH>();
+    __shared__ double lds[NW][4][lds_col_size<CH, CHP>()];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const unsigned lane = threadIdx.x & 63;
+    int j, i;
+    const bool live = strip_column_w<NW>(xcd_block(0), wave, p.ny, p.nz, j, i);
+    const Nb n = make_nb(p, j, i);
+    const long long c0 = (long long)i * p.plane + (long long)j * p.ldx;
+    const unsigned nx = p.nx, nxm = p.nxm;
+    double *b0 = lds[wave][0], *b1 = lds[wave][1], *b2 = lds[wave][2], *b3 = lds[wave][3];
+#pragma unroll 1
+    for (unsigned r = lane; r < 64u * CH; r += 64) {
+        double rhs[4] = {0.0, 0.0, 0.0, 0.0};
+        if (live && r < nx) base_rhs4<HT>(p, n, c0, r, rhs);
+        const unsigned q = lds_row<CH, CHP>(r);
+        b0[q] = rhs[0]; b1[q] = rhs[1]; b2[q] = rhs[2]; b3[q] = rhs[3];
+    }
+    __syncthreads();
+    column_solve_lds<CH, CHP>(b0, fx);
+    column_solve_lds<CH, CHP>(b1, fyz);
+    column_solve_lds<CH, CHP>(b2, fyz);
+    column_solve_lds<CH, CHP>(b3, ft);
+    __syncthreads();
+    if (!live) return;
+    double *ox = o.vx + c0, *oy = o.vy + c0, *oz = o.vz + c0, *ot = o.temp + c0;
+#pragma unroll 1
+    for (unsigned r = lane; r < nx; r += 64) {
+        const unsigned q = lds_row<CH, CHP>(r);
+        const double ux = n.vx[r], uy = n.vy[r], uz = n.vz[r], ut = n.t[r];
+        ox[r] = (r >= 1 && r < nxm) ? ux + b0[q] : 0.0;     // walls, ImplicitAndUpdateVX.F90:86-87
+        oy[r] = r < nxm ? uy + b1[q] : uy;
+        oz[r] = r < nxm ? uz + b2[q] : uz;
+        ot[r] = r < nxm ? ut + b3[q] : ut;
     }
 }
 

@@ -11,7 +11,7 @@
 
 namespace afid {
 
-enum MatKind { MAT_X = 0, MAT_YZ = 1, MAT_T = 2, MAT_S = 3, MAT_P = 4, MAT_NKIND = 5 };
+enum MatKind { MAT_X = 0, MAT_YZ = 1, MAT_T = 2, MAT_S = 3, MAT_P = 4, MAT_Q = 5, MAT_NKIND = 6 };
 
 // LU factors of one constant wall-normal tridiagonal system, in the form the
 // wave solver consumes (row r, 0-based, padded with zeros to 64*chunk rows):
@@ -74,6 +74,13 @@ struct KP {
     double pf_d, pf_b, pf_a, pf_tm, pf_s, heta;  // pf_D, pf_D/eps^2, A, T_m, S_t, 0.5/eta
     int pf;
     const double *ap3spkr, *ac3spkr, *am3spkr;   // zero-gradient x operator of phi (refined)
+    // moisture (moisture.F90): humidity, its RK history (hkhumid), this
+    // substep's explicit term (rkhumid), saturation, wall values; alpha_q,
+    // beta_q, gamma_q/tau_q, 1/tau_q, kappa_q = 1/pecq, and the wall flags
+    double *humid, *ruhumid, *hhumid, *qsat;
+    const double *humbp, *humtp, *xm;
+    double q_alpha, q_beta, q_gtau, q_itau, kappa_q, pecq;
+    int q_fixn, q_fixs, moist;
     // mixed form, gAxis = 1: per-row FP64 reference buoyancy subtracted from
     // the FP32 vx explicit term and added back in the RHS (null otherwise)
     const double *bref;

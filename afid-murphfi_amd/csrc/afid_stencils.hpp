@@ -169,6 +169,14 @@ __device__ __forceinline__ double rhs_temp(const KP &p, const Nb &n, double h, d
     return (p.ga * h + p.ro * ru + (p.al / p.pect) * dxxt) * p.dt;
 }
 
+// latent heat of condensing vapour added to the T explicit term
+// (AddCondensation, moisture.F90:304-311) at row k of the base column that
+// starts at c0: gamma_q/tau_q (q - qsat) where q > qsat
+__device__ __forceinline__ double condensation_heat(const KP &p, long long c0, unsigned k) {
+    const double q = p.humid[c0 + k], qs = p.qsat[c0 + k];
+    return q > qs ? p.q_gtau * (q - qs) : 0.0;
+}
+
 // storage-type selection of a field that exists in FP64 and FP32 forms
 template <typename RT> __device__ __forceinline__ RT *sel(double *d, float *f);
 template <> __device__ __forceinline__ double *sel<double>(double *d, float *) { return d; }

@@ -9,6 +9,8 @@ module afid_gpu
     integer(c_int), parameter :: AFID_SAL = 14, AFID_VXR = 18, AFID_VYR = 19, AFID_VZR = 20, AFID_SALC = 21
     integer(c_int), parameter :: AFID_TEMPBP = 22, AFID_TEMPTP = 23, AFID_SALBP = 24, AFID_SALTP = 25
     integer(c_int), parameter :: AFID_PHI = 26
+    integer(c_int), parameter :: AFID_HUMID = 31, AFID_RUHUMID = 32, AFID_HHUMID = 33, AFID_QSAT = 34
+    integer(c_int), parameter :: AFID_HUMBP = 35, AFID_HUMTP = 36
 
     !> the bou.in value lines (ReadInputFile.F90:20-76) + arithmetic form
     type, bind(C) :: afid_config
@@ -28,6 +30,12 @@ module afid_gpu
         real(c_double)     :: div_tol
         integer(c_int32_t) :: max_sweeps
         real(c_double)     :: pf_a, pf_s, pf_tm, pf_eps, pf_r0
+        !> moisture (moisture.F90, humid.in): on/off, alpha_q, beta_q,
+        !! gamma_q, tau_q, ifdiff, qfixN, qfixS, Sm
+        integer(c_int32_t) :: moist
+        real(c_double)     :: q_alpha, q_beta, q_gamma, q_tau
+        integer(c_int32_t) :: q_ifdiff, q_fixn, q_fixs
+        real(c_double)     :: q_sm
     end type afid_config
 
     !> batched 2-D ensemble (include/afid_ensemble.h): the time-stepping line
@@ -159,6 +167,22 @@ module afid_gpu
             import; type(c_ptr), value :: ctx
         end function
         integer(c_int) function afid_interp_phi_multigrid(ctx) bind(C, name='afid_interp_phi_multigrid')
+            import; type(c_ptr), value :: ctx
+        end function
+        ! moisture.F90
+        integer(c_int) function afid_set_humidity_bcs(ctx) bind(C, name='afid_set_humidity_bcs')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_explicit_humidity(ctx) bind(C, name='afid_explicit_humidity')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_add_condensation(ctx) bind(C, name='afid_add_condensation')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_implicit_humidity(ctx) bind(C, name='afid_implicit_humidity')
+            import; type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function afid_update_saturation(ctx) bind(C, name='afid_update_saturation')
             import; type(c_ptr), value :: ctx
         end function
         integer(c_int) function afid_time_marcher(ctx, dt) bind(C, name='afid_time_marcher')

@@ -64,7 +64,8 @@ subroutine ReadInputFile
                       alx3, ylen, zlen, istr3, str3, istr3r, rayt, prat, rays, pras, FFscaleS, &
                       inslws, inslwn, TfixS, TfixN, SfixS, SfixN, active_T, active_S, gAxis, &
                       xplusU, xminusU, dPdy, dPdz, AFID_FP64, 0.d0, 0, &
-                      pf_A, pf_S, pf_Tm, 0.d0, 0.d0)
+                      pf_A, pf_S, pf_Tm, 0.d0, 0.d0, &
+                      0, 0.d0, 0.d0, 0.d0, 0.d0, 0, 1, 1, 1.d0)
     if (mixed) cfg%precision = AFID_MIXED
 end subroutine ReadInputFile
 

@@ -18,10 +18,11 @@ FIELDS = ["vx", "vy", "vz", "temp", "pr", "rux", "ruy", "ruz", "rutemp",
           "qcap", "dph", "dq", "hro", "rhs",
           "sal", "rusal", "hsal", "rhsr", "vxr", "vyr", "vzr", "salc",
           "tempbp", "temptp", "salbp", "saltp",
-          "phi", "ruphi", "hphi", "phic", "tempr"]
+          "phi", "ruphi", "hphi", "phic", "tempr",
+          "humid", "ruhumid", "hhumid", "qsat", "humbp", "humtp"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 REFINED = {"sal", "rusal", "hsal", "rhsr", "vxr", "vyr", "vzr", "salbp", "saltp", "phi", "ruphi", "hphi", "tempr"}
-PLANES = {"tempbp", "temptp", "salbp", "saltp"}
+PLANES = {"tempbp", "temptp", "salbp", "saltp", "humbp", "humtp"}
 
 # every entry point declared in include/afid_gpu.h
 EXPORTS = [
@@ -37,6 +38,8 @@ EXPORTS = [
     "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher", "afid_time_marcher_graph", "afid_refinement_residuals",
     "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_calc_sal_stats", "afid_calc_phi_stats",
     "afid_synchronize", "afid_set_ibm",
+    "afid_set_humidity_bcs", "afid_explicit_humidity", "afid_add_condensation", "afid_implicit_humidity",
+    "afid_update_saturation",
     "afid_interp_input_vel", "afid_interp_input_sal", "afid_interp_input_phi",
     "afid_create_initial_conditions",
     "afid_set_timing", "afid_get_timing", "afid_grid_array",
@@ -63,7 +66,8 @@ ROUTINES = ["explicit_terms_vx", "explicit_terms_vy", "explicit_terms_vz", "expl
             "calc_local_divergence", "solve_pressure_correction", "correct_velocity", "correct_pressure",
             "interp_vel_mgrd", "interp_sal_multigrid",
             "explicit_phase", "add_volume_penalty", "implicit_phase", "add_latent_heat",
-            "interp_temp_multigrid", "interp_phi_multigrid"]
+            "interp_temp_multigrid", "interp_phi_multigrid",
+            "set_humidity_bcs", "explicit_humidity", "add_condensation", "implicit_humidity", "update_saturation"]
 
 FP64, MIXED = 0, 1
 
@@ -107,6 +111,9 @@ class AfidConfig(C.Structure):
         ("div_tol", C.c_double), ("max_sweeps", C.c_int32),
         ("pf_a", C.c_double), ("pf_s", C.c_double), ("pf_tm", C.c_double), ("pf_eps", C.c_double),
         ("pf_r0", C.c_double),
+        ("moist", C.c_int32), ("q_alpha", C.c_double), ("q_beta", C.c_double), ("q_gamma", C.c_double),
+        ("q_tau", C.c_double), ("q_ifdiff", C.c_int32), ("q_fixn", C.c_int32), ("q_fixs", C.c_int32),
+        ("q_sm", C.c_double),
     ]
 
 
@@ -116,7 +123,8 @@ DEFAULTS = dict(nxm=16, nym=16, nzm=16, nsst=3, multires=0, nxmr=0, nymr=0, nzmr
                 inslws=1, inslwn=1, tfixs=1, tfixn=1, sfixs=1, sfixn=1,
                 active_t=1, active_s=0, gaxis=1, xplusu=0.0, xminusu=0.0, dpdy=0.0, dpdz=0.0,
                 precision=FP64, div_tol=0.0, max_sweeps=0,
-                pf_a=1.0, pf_s=1.0, pf_tm=0.0, pf_eps=0.0, pf_r0=0.0)
+                pf_a=1.0, pf_s=1.0, pf_tm=0.0, pf_eps=0.0, pf_r0=0.0,
+                moist=0, q_alpha=3.0, q_beta=1.2, q_gamma=0.19, q_tau=5e-5, q_ifdiff=1, q_fixn=1, q_fixs=1, q_sm=1.0)
 
 
 def make_config(d):

@@ -61,6 +61,16 @@ typedef struct afid_config {
        radius pf_r0 of the solid sphere of the seeded initial state (0: 0.1,
        phasefield.F90:109-110) */
     double pf_a, pf_s, pf_tm, pf_eps, pf_r0;
+    /* moisture (src/moisture.F90, switched on when humid.in exists,
+       main.F90:38-39): the humid.in values of ReadMoistParameters
+       (moisture.F90:64-117): alpha_q, beta_q, gamma_q (< 0: the value
+       that makes the moist-static jump 1, :104-110), tau_q (times pect when
+       q_ifdiff), qfixN (2: unsaturated top), qfixS, and Sm, the
+       humidity-to-thermal diffusivity ratio (pecq = pect / Sm) */
+    int32_t moist;
+    double q_alpha, q_beta, q_gamma, q_tau;
+    int32_t q_ifdiff, q_fixn, q_fixs;
+    double q_sm;
 } afid_config;
 
 typedef struct afid_ctx afid_ctx;
@@ -73,6 +83,9 @@ enum afid_field {
     AFID_SAL, AFID_RUSAL, AFID_HSAL, AFID_RHSR, AFID_VXR, AFID_VYR, AFID_VZR, AFID_SALC,
     AFID_TEMPBP, AFID_TEMPTP, AFID_SALBP, AFID_SALTP,
     AFID_PHI, AFID_RUPHI, AFID_HPHI, AFID_PHIC, AFID_TEMPR,   /* phasefield.F90:12-16 */
+    /* moisture.F90:10-13, base grid: humid, hkhumid (RK history), rkhumid
+       (explicit term of the substep), qsat, and the wall planes humbp/humtp */
+    AFID_HUMID, AFID_RUHUMID, AFID_HHUMID, AFID_QSAT, AFID_HUMBP, AFID_HUMTP,
     AFID_NFIELDS
 };
 
@@ -142,6 +155,15 @@ int afid_interp_phi_multigrid(afid_ctx *ctx);     /* phasefield.F90:470-494 */
  * solve back.  FP64 call tree only: afid_time_marcher returns AFID_E_STATE
  * while a mask is set. */
 int afid_set_ibm(afid_ctx *ctx, int field, const int32_t *mask, const double *dist, int64_t ndist);
+
+/* Moisture (src/moisture.F90), for a context with moist = 1.  The
+ * per-routine calls follow the call tree (TimeMarcher.F90:79-81, :97, :154);
+ * afid_time_marcher runs them fused.  FP64 arithmetic in both forms. */
+int afid_set_humidity_bcs(afid_ctx *ctx);     /* SetHumidityBCs, moisture.F90:129-148 (from tempbp / temptp) */
+int afid_explicit_humidity(afid_ctx *ctx);    /* ExplicitHumidity, :220-292 (into hhumid = rkhumid) */
+int afid_add_condensation(afid_ctx *ctx);     /* AddCondensation, :296-318 (into hro) */
+int afid_implicit_humidity(afid_ctx *ctx);    /* ImplicitHumidity + SolveImpEqnUpdate_Humidity, :336-418 */
+int afid_update_saturation(afid_ctx *ctx);    /* UpdateSaturation, :321-333 */
 
 /* Multi-resolution transfers. */
 int afid_interp_vel_mgrd(afid_ctx *ctx);       /* InterpVelMgrd.F90:1-243 */

@@ -57,3 +57,15 @@ def test_ensemble_struct_layouts():
     assert _struct_fields(ENS_HEADER, "afid_run_control") == [f for f, _ in pyafid.RunControl._fields_]
     assert _struct_fields(ENS_HEADER, "afid_ens_tuning") == [f for f, _ in pyafid.EnsTuning._fields_]
     assert _struct_fields(ENS_HEADER, "afid_run_state") == [f for f, _ in pyafid.RunState._fields_]
+
+
+def test_fortran_config_type_mirrors_the_struct():
+    """type(afid_config) of the Fortran binding lists the C struct's members in
+    the same order (afid_gpu_init reads sizeof(afid_config) bytes from it)."""
+    src = open(os.path.join(ROOT, "afid-murphfi_amd", "fortran", "afid_gpu_mod.F90")).read()
+    start = src.index("type, bind(C) :: afid_config")
+    body = src[src.index("\n", start):src.index("end type afid_config")]
+    body = re.sub(r"!.*", "", body)
+    names = re.findall(r"::\s*([^\n]+)", body)
+    flat = [n.strip() for group in names for n in group.split(",")]
+    assert flat == [f for f, _ in pyafid.AfidConfig._fields_]
