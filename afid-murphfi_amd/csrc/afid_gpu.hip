@@ -99,6 +99,7 @@ struct afid_ctx {
     double q_alpha = 0, q_beta = 0, q_gamma = 0, q_tau = 0, pecq = 0, kappa_q = 0;
     int q_fixn = 1, q_fixs = 1;
     double *humid_alt = nullptr;
+    double *d_moist_part = nullptr, *d_moist_out = nullptr;    // CalcMoistStats partial sums and profiles
     // arithmetic form
     bool mixed = false;
     double div_tol = 1e-10;
@@ -1992,6 +1993,42 @@ extern "C" {
 int afid_interp_input_sal(afid_ctx *c, const afid_old_grid *og, const double *salo) { return interp_input_refined(c, og, salo, false); }
 int afid_interp_input_phi(afid_ctx *c, const afid_old_grid *og, const double *phio) { return interp_input_refined(c, og, phio, true); }
 
+// InterpInputHum, moisture.F90:539-582: the old humidity onto the base grid
+// with Dirichlet (qfixS / qfixN) or zero-gradient ghosts, the wall values
+// being those of the first column of humbp / humtp (the reference assumes
+// them uniform, :557-559; SetHumidityBCs runs first); the RK history restarts
+// from zero
+int afid_interp_input_humid(afid_ctx *c, const afid_old_grid *og, const double *qo) {
+    if (!c || !og || !qo) return AFID_E_ARG;
+    NEED_MOIST(c)
+    int rc;
+    if ((rc = check_old_grid(c, og, false))) return rc;
+    const int nxo = og->nxo, nyo = og->nyo - 1, nzo = og->nzo - 1, nxmo = nxo - 1;
+    RestartScratch sc(c);
+    RestartTabs t{};
+    const double *xco;
+    if ((rc = restart_tables(c, sc, nxmo, nyo, nzo, og->istr3o, og->str3o, false, t, &xco))) return rc;
+    const size_t nold = size_t(nxo) * nyo * nzo, next = size_t(nxo + 3) * nyo * nzo;
+    const size_t nb = size_t(c->nz) * c->ny * c->ldx;
+    double *F, *E;
+    if ((rc = sc.alloc(nold, &F)) || (rc = sc.alloc(next, &E))) return rc;
+    double walls[2] = {0.0, 0.0};
+    HIPCHK(c, hipMemcpyAsync(&walls[0], c->f[AFID_HUMBP], sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&walls[1], c->f[AFID_HUMTP], sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemsetAsync(c->f[AFID_HUMID], 0, sizeof(double) * nb, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->f[AFID_RUHUMID], 0, sizeof(double) * nb, c->stream));
+    RestartGhost gh{1, 0.0, 0.0, c->q_fixs, c->q_fixn, walls[0], walls[1]};
+    HIPCHK(c, hipMemcpyAsync(F, qo, sizeof(double) * nold, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_restart_stage, dim3(unsigned((nxo + 3 + 255) / 256), unsigned(nyo), unsigned(nzo)), dim3(256), 0, c->stream,
+                       F, 3, nxo, nyo, nzo, xco, 0.0, 0.0, gh, E);
+    hipLaunchKernelGGL(k_restart_transfer, point_grid(c->nxm, c->ny, c->nz), kPointBlock, 0, c->stream, E, nxo, nyo, nzo, t,
+                       c->f[AFID_HUMID], c->nxm, c->ny, c->nz, c->ldx, (long long)c->ny * c->ldx);
+    if ((rc = check_launch(c, "k_restart_transfer"))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AFID_OK;
+}
+
 int afid_synchronize(afid_ctx *c) {
     if (!c) return AFID_E_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2085,6 +2122,26 @@ int afid_calc_mean_profiles(afid_ctx *c, double *out) {
                        c->mean_groups, c->d_mean_out);
     if ((rc = check_launch(c, "k_mean_finish"))) return rc;
     HIPCHK(c, hipMemcpyAsync(out, c->d_mean_out, sizeof(double) * size_t(MEAN_NQ) * nxm, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AFID_OK;
+}
+
+int afid_calc_moist_stats(afid_ctx *c, double *out) {
+    if (!c || !out) return AFID_E_ARG;
+    NEED_MOIST(c)
+    int rc;
+    const int nxm = c->nxm, groups = mean_ygroups(c->ny);
+    if (!c->d_moist_part) {
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_moist_part), sizeof(double) * size_t(groups) * MOIST_NQ * nxm))) return rc;
+        if ((rc = dalloc(c, reinterpret_cast<void **>(&c->d_moist_out), sizeof(double) * size_t(MOIST_NQ) * nxm))) return rc;
+    }
+    KP p = make_kp(c);
+    hipLaunchKernelGGL(k_moist_partials, dim3(unsigned((nxm + 63) / 64), unsigned(groups)), dim3(256), 0, c->stream, p, c->d_moist_part);
+    if ((rc = check_launch(c, "k_moist_partials"))) return rc;
+    hipLaunchKernelGGL(k_moist_finish, dim3(unsigned((MOIST_NQ * nxm + 255) / 256)), dim3(256), 0, c->stream, p, c->d_moist_part,
+                       groups, c->d_moist_out);
+    if ((rc = check_launch(c, "k_moist_finish"))) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->d_moist_out, sizeof(double) * size_t(MOIST_NQ) * nxm, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return AFID_OK;
 }

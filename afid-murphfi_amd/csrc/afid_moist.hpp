@@ -148,4 +148,55 @@ __global__ void __launch_bounds__(256) k_column_humid(KP p, TriFactors f, double
     }
 }
 
+// ---------------------------------------------------------------------------
+// CalcMoistStats, moisture.F90:422-505: qbar, qsbar, qrms, qrel, vxq, vyq,
+// vzq (nxm values each, the order of means.h5).  Same two-kernel structure
+// as the mean profiles: a workgroup takes 64 x points and 4 y columns, each
+// wavefront marches along z summing in FP64; the groups meet in LDS, then a
+// second kernel sums the groups in a fixed order and normalises.
+// ---------------------------------------------------------------------------
+constexpr int MOIST_NQ = 7;
+__global__ void __launch_bounds__(256) k_moist_partials(KP p, double *__restrict__ part) {
+    __shared__ double red[4][MOIST_NQ][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane, nxm = p.nxm;
+    const int j = blockIdx.y * 4 + wave;
+    double a[MOIST_NQ];
+#pragma unroll
+    for (int q = 0; q < MOIST_NQ; ++q) a[q] = 0.0;
+    if (k < nxm && j < p.ny) {
+        const int jp = wrapp(j, p.ny);
+        for (int i = 0; i < p.nz; ++i) {
+            const int ip = wrapp(i, p.nz);
+            const long long c = bidx(p, k, j, i);
+            const double h = p.humid[c], qs = p.qsat[c];
+            a[0] += h;
+            a[1] += qs;
+            a[2] += h * h;
+            a[3] += h / qs;
+            a[4] += 0.5 * (p.vx[c] + p.vx[c + 1]) * h;
+            a[5] += 0.5 * (p.vy[c] + p.vy[bidx(p, k, jp, i)]) * h;
+            a[6] += 0.5 * (p.vz[c] + p.vz[bidx(p, k, j, ip)]) * h;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < MOIST_NQ; ++q) red[wave][q][lane] = a[q];
+    __syncthreads();
+    if (wave == 0 && k < nxm) {
+#pragma unroll
+        for (int q = 0; q < MOIST_NQ; ++q)
+            part[((long long)blockIdx.y * MOIST_NQ + q) * nxm + k] =
+                ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+    }
+}
+__global__ void __launch_bounds__(256) k_moist_finish(KP p, const double *__restrict__ part, int groups, double *__restrict__ out) {
+    const int t = blockIdx.x * 256 + threadIdx.x, nxm = p.nxm;
+    if (t >= MOIST_NQ * nxm) return;
+    const int q = t / nxm, k = t - q * nxm;
+    double s = 0.0;
+    for (int g = 0; g < groups; ++g) s += part[((long long)g * MOIST_NQ + q) * nxm + k];
+    const double inyzm = 1.0 / double(p.ny) / double(p.nz);       // :441
+    out[t] = q == 2 ? sqrt(s * inyzm) : s * inyzm;                  // :465-473
+}
+
 }  // namespace afid

@@ -185,6 +185,16 @@ module afid_gpu
         integer(c_int) function afid_update_saturation(ctx) bind(C, name='afid_update_saturation')
             import; type(c_ptr), value :: ctx
         end function
+        integer(c_int) function afid_calc_moist_stats(ctx, out) bind(C, name='afid_calc_moist_stats')
+            import; type(c_ptr), value :: ctx; real(c_double) :: out(*)
+        end function
+        integer(c_int) function afid_interp_input_humid(ctx, og, humid) bind(C, name='afid_interp_input_humid')
+            import; type(c_ptr), value :: ctx; type(afid_old_grid), intent(in) :: og; real(c_double), intent(in) :: humid(*)
+        end function
+        integer(c_int) function afid_grid_array(ctx, name, out, n, count) bind(C, name='afid_grid_array')
+            import; type(c_ptr), value :: ctx; character(kind=c_char), intent(in) :: name(*)
+            real(c_double) :: out(*); integer(c_int64_t), value :: n; integer(c_int64_t) :: count
+        end function
         integer(c_int) function afid_time_marcher(ctx, dt) bind(C, name='afid_time_marcher')
             import; type(c_ptr), value :: ctx; real(c_double), value :: dt
         end function

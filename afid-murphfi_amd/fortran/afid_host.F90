@@ -8,6 +8,7 @@
 !!   outputdir/means.h5 every TOUT; NREAD = 1 restarts from outputdir/continua_*.h5
 !!   fused: one afid_time_marcher call per step instead of the call tree
 !!   mixed: the mixed FP32/FP64 form (runs through afid_time_marcher)
+!!   moisture (src/moisture.F90) is on when humid.in exists, as in main.F90:38-39
 module afid_host_state
     use iso_c_binding
     use afid_gpu
@@ -17,7 +18,7 @@ module afid_host_state
     integer :: nsst, ntst, idtv, nread, ireset, flagMR, flagsal, flagPF, flagmelt, solidtype, pf_IC
     real(c_double) :: walltimemax, tmax, tout, tframe, save_3D, dt, resid, limitCFL, dtmin, dtmax
     real(c_double) :: pf_D, pf_A, pf_S, pf_Tm
-    logical :: salinity = .false., fused = .false., mixed = .false.
+    logical :: salinity = .false., fused = .false., mixed = .false., moist = .false.
 end module afid_host_state
 
 !> ReadInputFile.F90:20-76 (same line layout)
@@ -67,7 +68,94 @@ subroutine ReadInputFile
                       pf_A, pf_S, pf_Tm, 0.d0, 0.d0, &
                       0, 0.d0, 0.d0, 0.d0, 0.d0, 0, 1, 1, 1.d0)
     if (mixed) cfg%precision = AFID_MIXED
+    ! main.F90:38-39 and InitMoistVariables (moisture.F90:31-61)
+    inquire(file='humid.in', exist=moist)
+    if (moist) call ReadMoistParameters
 end subroutine ReadInputFile
+
+!> ReadMoistParameters, moisture.F90:64-117: the humid.in lines; dtmax is
+!! capped at a tenth of the condensation time (InitMoistVariables, :57)
+subroutine ReadMoistParameters
+    use afid_host_state
+    implicit none
+    integer :: io, i
+    logical :: ifdiff
+    real(c_double) :: alpha_q, beta_q, gamma_q, tau_q, sm, pect
+    integer :: qfixn, qfixs
+    open(newunit=io, file='humid.in', status='old', action='read')
+    do i = 1, 4
+        read(io,*)
+    end do
+    read(io,*) alpha_q
+    read(io,*); read(io,*)
+    read(io,*) beta_q
+    read(io,*); read(io,*)
+    read(io,*) gamma_q
+    read(io,*); read(io,*)
+    read(io,*) tau_q
+    read(io,*); read(io,*)
+    read(io,*) ifdiff
+    read(io,*); read(io,*); read(io,*)
+    read(io,*) qfixn, qfixs
+    read(io,*); read(io,*)
+    read(io,*) sm
+    close(io)
+    cfg%moist = 1
+    cfg%q_alpha = alpha_q; cfg%q_beta = beta_q; cfg%q_gamma = gamma_q; cfg%q_tau = tau_q
+    cfg%q_ifdiff = merge(1, 0, ifdiff); cfg%q_fixn = qfixn; cfg%q_fixs = qfixs; cfg%q_sm = sm
+    ! ReadInputFile.F90:101-122 (pect), moisture.F90:99 (tau in diffusive units)
+    pect = sqrt(abs(cfg%rayt) / cfg%prat) * cfg%prat
+    if (salinity .and. cfg%ffscales == 1) pect = sqrt(abs(cfg%rays) / cfg%pras) * cfg%prat
+    if (ifdiff) tau_q = pect * tau_q
+    dtmax = min(dtmax, 0.1d0 * tau_q)
+end subroutine ReadMoistParameters
+
+!> CreateInitialHumidity, moisture.F90:150-215 (no drizzle.h5): T = x (1-x)
+!! times 1e-3 noise and a humid blob q = 5 exp(-r^2/0.005) around (0.1, Ly/2,
+!! Lz/2) plus 1e-3 noise, then qsat (main.F90:202).  The reference draws the
+!! noise from the system-seeded random_number; here it is seeded with seed.
+subroutine CreateInitialHumidity(seed)
+    use afid_host_state
+    implicit none
+    integer(c_int64_t), intent(in) :: seed
+    real(c_double), allocatable :: t(:,:,:), q(:,:,:), xm(:)
+    integer(c_int64_t) :: lb(3), ext(3), n
+    integer :: ic, jc, kc, ns, i_
+    integer, allocatable :: put(:)
+    real(c_double) :: rnum, r2, ym, zm
+    allocate(t(cfg%nxm + 1, cfg%nym, cfg%nzm), q(cfg%nxm + 1, cfg%nym, cfg%nzm), xm(cfg%nxm))
+    n = int(cfg%nxm, c_int64_t)
+    call afid_check(ctx, afid_grid_array(ctx, 'xm'//c_null_char, xm, n, n), 'grid xm')
+    call random_seed(size=ns)
+    allocate(put(ns))
+    put = int(seed) + 7919 * [(i_, i_ = 1, ns)]
+    call random_seed(put=put)
+    t = 0.d0; q = 0.d0
+    do ic = 1, cfg%nzm
+        do jc = 1, cfg%nym
+            do kc = 1, cfg%nxm
+                call random_number(rnum)
+                t(kc, jc, ic) = (1.d0 - xm(kc)) * xm(kc) * rnum * 1.d-3
+            end do
+        end do
+    end do
+    do ic = 1, cfg%nzm
+        zm = (ic - 0.5d0) * cfg%zlen / cfg%nzm
+        do jc = 1, cfg%nym
+            ym = (jc - 0.5d0) * cfg%ylen / cfg%nym
+            do kc = 1, cfg%nxm
+                call random_number(rnum)
+                r2 = (ym - 0.5d0 * cfg%ylen)**2 + (xm(kc) - 0.1d0)**2 + (zm - 0.5d0 * cfg%zlen)**2
+                q(kc, jc, ic) = 5.d0 * exp(-r2 / 0.005d0) + 1.d-3 * rnum
+            end do
+        end do
+    end do
+    lb = [1_c_int64_t, 1_c_int64_t, 1_c_int64_t]
+    ext = [int(cfg%nxm + 1, c_int64_t), int(cfg%nym, c_int64_t), int(cfg%nzm, c_int64_t)]
+    call afid_check(ctx, afid_upload(ctx, AFID_TEMP, t, lb, ext), 'upload temp')
+    call afid_check(ctx, afid_upload(ctx, AFID_HUMID, q, lb, ext), 'upload humid')
+    call afid_check(ctx, afid_update_saturation(ctx), 'UpdateSaturation')
+end subroutine CreateInitialHumidity
 
 !> SetTempBCs.F90:18-59 (RBC and phase-field wall values) and SetSalBCs
 !! (salinity.F90:105-122)
@@ -115,6 +203,8 @@ subroutine SetBCs
         plane = st
         call afid_check(ctx, afid_upload(ctx, AFID_SALTP, plane, lb, ext), 'upload saltp')
     end if
+    ! main.F90:155
+    if (moist) call afid_check(ctx, afid_set_humidity_bcs(ctx), 'SetHumidityBCs')
 end subroutine SetBCs
 
 !> TimeMarcher.F90:13-165: the call tree of one RK3 step, one C-ABI call per
@@ -145,6 +235,11 @@ subroutine TimeMarcher
             call afid_check(ctx, afid_implicit_phase(ctx), 'ImplicitPhase')
             call afid_check(ctx, afid_add_latent_heat(ctx), 'AddLatentHeat')
         end if
+        ! TimeMarcher.F90:79-81
+        if (moist) then
+            call afid_check(ctx, afid_explicit_humidity(ctx), 'ExplicitHumidity')
+            call afid_check(ctx, afid_add_condensation(ctx), 'AddCondensation')
+        end if
         ! with the phase field on, the velocity solves are the masked
         ! SolveImpEqnUpdate_X_pf / _YZ_pf (phasefield.F90:821-930)
         call afid_check(ctx, afid_implicit_and_update_vx(ctx), 'ImplicitAndUpdateVX')
@@ -152,6 +247,7 @@ subroutine TimeMarcher
         call afid_check(ctx, afid_implicit_and_update_vz(ctx), 'ImplicitAndUpdateVZ')
         call afid_check(ctx, afid_implicit_and_update_temp(ctx), 'ImplicitAndUpdateTemp')
         if (salinity) call afid_check(ctx, afid_implicit_salinity(ctx), 'ImplicitSalinity')
+        if (moist) call afid_check(ctx, afid_implicit_humidity(ctx), 'ImplicitHumidity')     ! :97
         call afid_check(ctx, afid_calc_local_divergence(ctx), 'CalcLocalDivergence')
         call afid_check(ctx, afid_solve_pressure_correction(ctx), 'SolvePressureCorrection')
         call afid_check(ctx, afid_correct_velocity(ctx), 'CorrectVelocity')
@@ -165,6 +261,7 @@ subroutine TimeMarcher
             call afid_check(ctx, afid_interp_temp_multigrid(ctx), 'InterpTempMultigrid')
             call afid_check(ctx, afid_interp_phi_multigrid(ctx), 'InterpPhiMultigrid')
         end if
+        if (moist) call afid_check(ctx, afid_update_saturation(ctx), 'UpdateSaturation')       ! :154
     end do
 end subroutine TimeMarcher
 
@@ -238,6 +335,7 @@ subroutine WriteFlowField(final, time)
     call field_file(AFID_VX, .false., trim(base)//'_vx.h5', .true.)
     call field_file(AFID_VY, .false., trim(base)//'_vy.h5', .true.)
     call field_file(AFID_VZ, .false., trim(base)//'_vz.h5', .true.)
+    if (moist) call field_file(AFID_HUMID, .false., trim(base)//'_humid.h5', .true.)
     if (salinity) call field_file(AFID_SAL, .true., trim(base)//'_sal.h5', .true.)
     if (cfg%phasefield /= 0) call field_file(AFID_PHI, .true., trim(base)//'_phi.h5', .true.)
     if (.not. final) return
@@ -316,6 +414,11 @@ subroutine ReadFlowInterp(time)
         call h5_check(afid_h5_read_field3d(cstr('outputdir/continua_temp.h5'), tempo, int(nxo, c_int64_t), &
                       int(nyo - 1, c_int64_t), int(nzo - 1, c_int64_t)), 'read temp')
         call afid_check(ctx, afid_interp_input_vel(ctx, og, vxo, vyo, vzo, tempo), 'InterpInputVel')
+        if (moist) then                 ! ReadFlowInterp.F90:133-135, InterpInputHum
+            call h5_check(afid_h5_read_field3d(cstr('outputdir/continua_humid.h5'), tempo, int(nxo, c_int64_t), &
+                          int(nyo - 1, c_int64_t), int(nzo - 1, c_int64_t)), 'read humid')
+            call afid_check(ctx, afid_interp_input_humid(ctx, og, tempo), 'InterpInputHum')
+        end if
         deallocate(vxo, vyo, vzo, tempo)
         if (salinity .or. cfg%phasefield /= 0) then
             m3 = int(nxro, c_int64_t) * (nyro - 1) * (nzro - 1)
@@ -339,6 +442,7 @@ subroutine ReadFlowInterp(time)
     call field_file(AFID_VY, .false., 'outputdir/continua_vy.h5', .false.)
     call field_file(AFID_VZ, .false., 'outputdir/continua_vz.h5', .false.)
     call field_file(AFID_TEMP, .false., 'outputdir/continua_temp.h5', .false.)
+    if (moist) call field_file(AFID_HUMID, .false., 'outputdir/continua_humid.h5', .false.)
     if (salinity) call field_file(AFID_SAL, .true., 'outputdir/continua_sal.h5', .false.)
     if (cfg%phasefield /= 0) call field_file(AFID_PHI, .true., 'outputdir/continua_phi.h5', .false.)
     inquire(file='outputdir/continua_pr.h5', exist=ex)
@@ -358,6 +462,7 @@ subroutine WriteMeans(time)
         'vzT', 'Trms', 'vxrms', 'vyrms', 'vzrms', 'vxvy', 'vxvz', 'vyvz', 'chiT', 'epsilon']
     character(len=7), parameter :: snames(6) = [character(len=7) :: 'Sbar', 'Srms', 'vxS', 'vyS', 'vzS', 'chiS']
     character(len=7), parameter :: pnames(2) = [character(len=7) :: 'phibar', 'phirms']
+    character(len=7), parameter :: qnames(7) = [character(len=7) :: 'qbar', 'qsbar', 'qrms', 'qrel', 'vxq', 'vyq', 'vzq']
     real(c_double), allocatable :: buf(:)
     character(len=5) :: nstat
     logical :: ex
@@ -379,6 +484,11 @@ subroutine WriteMeans(time)
                 call h5_check(afid_h5_create_group(cstr(path), cstr(pnames(q))), 'group')
             end do
         end if
+        if (moist) then                   ! CreateMoistH5Groups, moisture.F90:509-535
+            do q = 1, 7
+                call h5_check(afid_h5_create_group(cstr(path), cstr(qnames(q))), 'group')
+            end do
+        end if
     end if
     write(nstat, '(i5.5)') nint(time / tout)
     n = cfg%nxm
@@ -389,6 +499,15 @@ subroutine WriteMeans(time)
                       'write '//mnames(q))
     end do
     deallocate(buf)
+    if (moist) then                       ! CalcMoistStats, moisture.F90:485-500
+        allocate(buf(7 * n))
+        call afid_check(ctx, afid_calc_moist_stats(ctx, buf), 'CalcMoistStats')
+        do q = 1, 7
+            call h5_check(afid_h5_write_real1d(cstr(path), cstr(trim(qnames(q))//'/'//nstat), buf((q - 1) * n + 1), n), &
+                          'write '//qnames(q))
+        end do
+        deallocate(buf)
+    end if
     n = cfg%nxmr
     if (salinity) then
         allocate(buf(6 * n))
@@ -448,9 +567,11 @@ program afid_host
             call afid_check(ctx, afid_interp_temp_multigrid(ctx), 'InterpTempMultigrid')
             call afid_check(ctx, afid_interp_phi_multigrid(ctx), 'InterpPhiMultigrid')
         end if
+        if (moist) call afid_check(ctx, afid_update_saturation(ctx), 'UpdateSaturation')     ! main.F90:202
     else
         time = 0.d0
         call afid_check(ctx, afid_create_initial_conditions(ctx, seed, 1.d-3), 'CreateInitialConditions')
+        if (moist) call CreateInitialHumidity(seed)        ! main.F90:174
     end if
     ! main.F90:227 and :241
     call WriteMeans(time)

@@ -36,11 +36,11 @@ EXPORTS = [
     "afid_explicit_phase", "afid_add_volume_penalty", "afid_implicit_phase", "afid_add_latent_heat",
     "afid_interp_temp_multigrid", "afid_interp_phi_multigrid",
     "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher", "afid_time_marcher_graph", "afid_refinement_residuals",
-    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_calc_sal_stats", "afid_calc_phi_stats",
+    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_calc_sal_stats", "afid_calc_phi_stats", "afid_calc_moist_stats",
     "afid_synchronize", "afid_set_ibm",
     "afid_set_humidity_bcs", "afid_explicit_humidity", "afid_add_condensation", "afid_implicit_humidity",
     "afid_update_saturation",
-    "afid_interp_input_vel", "afid_interp_input_sal", "afid_interp_input_phi",
+    "afid_interp_input_vel", "afid_interp_input_sal", "afid_interp_input_phi", "afid_interp_input_humid",
     "afid_create_initial_conditions",
     "afid_set_timing", "afid_get_timing", "afid_grid_array",
 ]
@@ -80,9 +80,11 @@ IO_EXPORTS = [
     "afid_h5_write_real1d", "afid_h5_read_real1d",
 ]
 # checkpoint file of each field (HdfReadContinua.F90:44-61, WriteFlowField.F90:26-50)
-CONTINUA = {"vx": "vx", "vy": "vy", "vz": "vz", "temp": "temp", "sal": "sal", "phi": "phi", "pr": "pr"}
+CONTINUA = {"vx": "vx", "vy": "vy", "vz": "vz", "temp": "temp", "sal": "sal", "phi": "phi", "humid": "humid", "pr": "pr"}
 # the phase-field profiles of afid_calc_phi_stats (phasefield.F90:788-793)
 PHI_PROFILES = ["phibar", "phirms"]
+# the moisture profiles of afid_calc_moist_stats (moisture.F90:485-500)
+MOIST_PROFILES = ["qbar", "qsbar", "qrms", "qrel", "vxq", "vyq", "vzq"]
 
 
 class OldGrid(C.Structure):
@@ -202,10 +204,12 @@ def lib():
         L.afid_calc_mean_profiles.argtypes = [vp, dp]
         L.afid_calc_sal_stats.argtypes = [vp, dp]
         L.afid_calc_phi_stats.argtypes = [vp, dp]
+        L.afid_calc_moist_stats.argtypes = [vp, dp]
         L.afid_synchronize.argtypes = [vp]
         L.afid_interp_input_vel.argtypes = [vp, C.POINTER(OldGrid), dp, dp, dp, dp]
         L.afid_interp_input_sal.argtypes = [vp, C.POINTER(OldGrid), dp]
         L.afid_interp_input_phi.argtypes = [vp, C.POINTER(OldGrid), dp]
+        L.afid_interp_input_humid.argtypes = [vp, C.POINTER(OldGrid), dp]
         L.afid_create_initial_conditions.argtypes = [vp, C.c_uint64, C.c_double]
         L.afid_set_timing.argtypes = [vp, C.c_int]
         L.afid_get_timing.argtypes = [vp, C.POINTER(C.c_char_p), dp, C.c_int, C.POINTER(C.c_int)]
@@ -463,6 +467,14 @@ class Solver:
         self._check(self.L.afid_calc_mean_profiles(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "afid_calc_mean_profiles")
         return {n: out[q * nxm:(q + 1) * nxm].copy() for q, n in enumerate(MEAN_PROFILES)}
 
+    def moist_stats(self):
+        """CalcMoistStats (moisture.F90:422-505): dict of the 7 profiles (nxm
+        values each) in the reference's means.h5 order."""
+        n = int(self.cfg.nxm)
+        out = np.zeros(len(MOIST_PROFILES) * n)
+        self._check(self.L.afid_calc_moist_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "afid_calc_moist_stats")
+        return {m: out[q * n:(q + 1) * n].copy() for q, m in enumerate(MOIST_PROFILES)}
+
     def sal_stats(self):
         """CalcSalStats (salinity.F90:510-690): dict of the 6 refined profiles
         (each nxmr values) in the reference's means.h5 order."""
@@ -494,6 +506,8 @@ class Solver:
             names.append("sal")
         if c["phasefield"]:
             names.append("phi")
+        if c["moist"]:
+            names.append("humid")            # WriteFlowField.F90:34-38
         if final:
             names.append("pr")
         return names
@@ -552,14 +566,16 @@ class Solver:
         grids: `old` a dict of the old grid (nx, ny, nz, istr3, str3, ylen,
         zlen, and nxr, nyr, nzr, istr3r with multires; sizes count the faces)
         and `fields` the old arrays (nz-1, ny-1, nx) by name: vx, vy, vz, temp
-        (InterpInputVel) and sal / phi (InterpInputSal / InterpInputPhi)."""
+        (InterpInputVel), sal / phi (InterpInputSal / InterpInputPhi) and humid
+        (InterpInputHum, after set_humidity_bcs)."""
         og = OldGrid(int(old["nx"]), int(old["ny"]), int(old["nz"]), int(old["istr3"]), float(old["str3"]),
                      float(old.get("ylen", self.cfgd["ylen"])), float(old.get("zlen", self.cfgd["zlen"])),
                      int(old.get("nxr", 0)), int(old.get("nyr", 0)), int(old.get("nzr", 0)), int(old.get("istr3r", 0)))
         a = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in ("vx", "vy", "vz", "temp")]
         dpp = [x.ctypes.data_as(C.POINTER(C.c_double)) for x in a]
         self._check(self.L.afid_interp_input_vel(self.h, C.byref(og), *dpp), "afid_interp_input_vel")
-        for n, fn in (("sal", self.L.afid_interp_input_sal), ("phi", self.L.afid_interp_input_phi)):
+        for n, fn in (("sal", self.L.afid_interp_input_sal), ("phi", self.L.afid_interp_input_phi),
+                      ("humid", self.L.afid_interp_input_humid)):
             if n in fields:
                 b = np.ascontiguousarray(fields[n], dtype=np.float64)
                 self._check(fn(self.h, C.byref(og), b.ctypes.data_as(C.POINTER(C.c_double))), "afid_interp_input_" + n)
@@ -608,6 +624,8 @@ class Solver:
             if self.cfgd["phasefield"]:
                 self.call("interp_temp_multigrid")
                 self.call("interp_phi_multigrid")
+            if self.cfgd["moist"]:
+                self.call("update_saturation")          # main.F90:202
         return t
 
     def write_means(self, outdir, time, tout):
@@ -624,6 +642,8 @@ class Solver:
             groups += SAL_PROFILES
         if c["phasefield"]:
             groups += PHI_PROFILES
+        if c["moist"]:
+            groups += MOIST_PROFILES
         if not os.path.exists(path):
             h5_create_file(path)
             for gname in groups:
@@ -633,6 +653,8 @@ class Solver:
             prof.update(self.sal_stats())
         if c["phasefield"]:
             prof.update(self.phi_stats())
+        if c["moist"]:
+            prof.update(self.moist_stats())
         nstat = "%05d" % fortran_nint(time / tout)
         for gname in groups:
             h5_write_1d(path, f"{gname}/{nstat}", prof[gname])

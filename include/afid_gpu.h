@@ -211,6 +211,10 @@ int afid_calc_sal_stats(afid_ctx *ctx, double *out);
  * phase field, out[q * nxmr + k], q = 0 phibar, 1 phirms (2*nxmr doubles).
  * FP64 sums in both forms.  Needs the phase field.  Waits for the result. */
 int afid_calc_phi_stats(afid_ctx *ctx, double *out);
+/* CalcMoistStats (moisture.F90:422-505): qbar, qsbar, qrms, qrel, vxq, vyq,
+ * vzq, out[q * nxm + k] (7*nxm doubles, the order of means.h5).  FP64 sums.
+ * Needs moisture.  Waits for the result. */
+int afid_calc_moist_stats(afid_ctx *ctx, double *out);
 
 /* Restart onto a new grid (ReadFlowInterp.F90:90-141): the grid of the old run,
  * as its continua_master.h5 records it (nx, ny, nz, istr3, str3, ylen, zlen
@@ -239,6 +243,10 @@ int afid_interp_input_vel(afid_ctx *ctx, const afid_old_grid *og, const double *
 int afid_interp_input_sal(afid_ctx *ctx, const afid_old_grid *og, const double *sal);
 /* InterpInputPhi: the old refined phi with zero-gradient ghosts onto the refined grid. */
 int afid_interp_input_phi(afid_ctx *ctx, const afid_old_grid *og, const double *phi);
+/* InterpInputHum (moisture.F90:539-582): the old humidity, dims (nxo, nyo-1,
+ * nzo-1), onto the base grid, with the ghosts of its wall values (run
+ * afid_set_humidity_bcs first); the history restarts from zero. */
+int afid_interp_input_humid(afid_ctx *ctx, const afid_old_grid *og, const double *humid);
 
 /* Seeded initial state (CreateInitialConditions.F90:150-201, RBC branch, with
  * u = 0): T = tempbp + (temptp - tempbp) x + noise, the wall-clamped noise of

@@ -2057,6 +2057,39 @@ void orc_implicit_humidity(orc_state *s) {
     solve_columns(s, q, rhs, nx, nym, nzm, nxm, s->am3ssk, s->ac3ssk, s->ap3ssk, 0.5 * s->al * s->dt / s->pecq, 1, nxm, 1, nxm);
 }
 
+/* CalcMoistStats, moisture.F90:422-505: the 7 wall-parallel profiles qbar,
+ * qsbar, qrms, qrel, vxq, vyq, vzq (nxm each, in the order of means.h5),
+ * summed in the reference's loop order */
+void orc_moist_stats(orc_state *s, double *out) {
+    BASE_DECL
+    const double *q = s->f[ORC_HUMID], *qs = s->f[ORC_QSAT];
+    const double *vx = s->f[ORC_VX], *vy = s->f[ORC_VY], *vz = s->f[ORC_VZ];
+    double *qbar = out, *qsbar = out + nxm, *qrms = out + 2 * nxm, *qrel = out + 3 * nxm;
+    double *vxq = out + 4 * nxm, *vyq = out + 5 * nxm, *vzq = out + 6 * nxm;
+    for (int a = 0; a < 7 * nxm; a++) out[a] = 0.0;
+    const double inyzm = 1.0 / nym / nzm;
+    for (int ic = 1; ic <= nzm; ic++) {
+        int i0 = W(ic, nzm), ip = W(ic + 1, nzm);
+        for (int jc = 1; jc <= nym; jc++) {
+            int j0 = W(jc, nym), jp = W(jc + 1, nym);
+            for (int k = 1; k <= nxm; k++) {
+                const double h = q[B(k, j0, i0)];
+                qbar[k - 1] = qbar[k - 1] + h;
+                qsbar[k - 1] = qsbar[k - 1] + qs[B(k, j0, i0)];
+                qrms[k - 1] = qrms[k - 1] + h * h;
+                qrel[k - 1] = qrel[k - 1] + h / qs[B(k, j0, i0)];
+                vxq[k - 1] = vxq[k - 1] + 0.5 * (vx[B(k, j0, i0)] + vx[B(k + 1, j0, i0)]) * h;
+                vyq[k - 1] = vyq[k - 1] + 0.5 * (vy[B(k, j0, i0)] + vy[B(k, jp, i0)]) * h;
+                vzq[k - 1] = vzq[k - 1] + 0.5 * (vz[B(k, j0, i0)] + vz[B(k, j0, ip)]) * h;
+            }
+        }
+    }
+    for (int k = 0; k < nxm; k++) {
+        qbar[k] = qbar[k] * inyzm; qsbar[k] = qsbar[k] * inyzm; qrms[k] = sqrt(qrms[k] * inyzm);
+        qrel[k] = qrel[k] * inyzm; vxq[k] = vxq[k] * inyzm; vyq[k] = vyq[k] * inyzm; vzq[k] = vzq[k] * inyzm;
+    }
+}
+
 /* TimeMarcher.F90:13-165 (no IBM) */
 void orc_time_marcher(orc_state *s, double dt) {
     for (int ns = 1; ns <= s->c.nsst; ns++) {
@@ -2631,6 +2664,34 @@ static int interp_input_ref(orc_state *s, const orc_old_grid *og, const double *
     double *f = s->f[which == 0 ? ORC_SAL : ORC_PHI];
     memset(f, 0, sizeof(double) * nref(s));
     old_to_new(ov, nxmo, nymo, nzmo, &g.tx, &g.ty, &g.tz, f, nxmr, nymr, nzmr, nxr);
+    free(ov);
+    free_old(&g);
+    return 0;
+}
+/* InterpInputHum, moisture.F90:539-582: the old humidity (nzo-1, nyo-1, nxo)
+ * on the base grid with Dirichlet (qfixS / qfixN) or zero-gradient ghosts;
+ * the wall values are those of the first column of humbp / humtp (the
+ * routine assumes them uniform, :557-559, after SetHumidityBCs). */
+int orc_interp_input_humid(orc_state *s, const orc_old_grid *og, const double *qo) {
+    BASE_DECL
+    if (!s->c.moist) return 1;
+    old_grid g;
+    if (make_old(s, og, 0, &g)) return 1;
+    int nxmo = g.nxm, nymo = g.nym, nzmo = g.nzm, nxo = nxmo + 1, lo = nxmo + 4;
+    double *ov = calloc((size_t)lo * nymo * nzmo, sizeof(double));
+    const double qup = s->f[ORC_HUMTP][0], qlo = s->f[ORC_HUMBP][0];
+    for (int i = 0; i < nzmo; i++)
+        for (int j = 0; j < nymo; j++) {
+            double *row = ov + ((size_t)i * nymo + j) * lo + 1;
+            const double *src = qo + ((size_t)i * nymo + j) * nxo;
+            for (int kc = 1; kc <= nxo; kc++) row[kc] = src[kc - 1];
+            row[0] = s->q_fixs == 1 ? 2.0 * qlo - row[1] : row[1];
+            row[nxo] = s->q_fixn == 1 ? 2.0 * qup - row[nxmo] : row[nxmo];
+        }
+    double *q = s->f[ORC_HUMID];
+    memset(q, 0, sizeof(double) * nbase(s));
+    old_to_new(ov, nxmo, nymo, nzmo, &g.tx, &g.ty, &g.tz, q, nxm, nym, nzm, nx);
+    memset(s->f[ORC_RUHUMID], 0, sizeof(double) * nbase(s));
     free(ov);
     free_old(&g);
     return 0;

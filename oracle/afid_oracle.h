@@ -127,6 +127,8 @@ void orc_mean_profiles(orc_state *s, double *out);
 void orc_sal_stats(orc_state *s, double *out);
 /* CalcPhiStats: phibar, phirms of nxmr values each */
 void orc_phi_stats(orc_state *s, double *out);
+/* CalcMoistStats: qbar, qsbar, qrms, qrel, vxq, vyq, vzq of nxm values each */
+void orc_moist_stats(orc_state *s, double *out);
 
 /* Restart onto a new grid (ReadFlowInterp.F90:90-141): the old run's grid
  * as continua_master.h5 describes it (nx, ny, nz, istr3, str3 and, with
@@ -141,6 +143,7 @@ int orc_interp_input_vel(orc_state *s, const orc_old_grid *og, const double *vxo
                          const double *vzo, const double *tempo);   /* InterpInputVel.F90 */
 int orc_interp_input_sal(orc_state *s, const orc_old_grid *og, const double *salo);   /* InterpInputSal.F90 */
 int orc_interp_input_phi(orc_state *s, const orc_old_grid *og, const double *phio);   /* InterpInputPhi.F90 */
+int orc_interp_input_humid(orc_state *s, const orc_old_grid *og, const double *qo);   /* InterpInputHum, moisture.F90:539-582 */
 
 /* Immersed boundary (src/ibm/ibm_param.F90): the mask of the solve of field
  * (ORC_VX, ORC_VY, ORC_VZ, ORC_TEMP on (nzm, nym, nx); ORC_SAL on (nzmr,

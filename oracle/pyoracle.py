@@ -89,9 +89,11 @@ def lib():
         L.orc_interp_input_vel.argtypes = [vp, C.POINTER(OldGrid)] + [C.POINTER(C.c_double)] * 4
         L.orc_interp_input_sal.argtypes = [vp, C.POINTER(OldGrid), C.POINTER(C.c_double)]
         L.orc_interp_input_phi.argtypes = [vp, C.POINTER(OldGrid), C.POINTER(C.c_double)]
+        L.orc_interp_input_humid.argtypes = [vp, C.POINTER(OldGrid), C.POINTER(C.c_double)]
         L.orc_mean_profiles.argtypes = [vp, C.POINTER(C.c_double)]
         L.orc_sal_stats.argtypes = [vp, C.POINTER(C.c_double)]
         L.orc_phi_stats.argtypes = [vp, C.POINTER(C.c_double)]
+        L.orc_moist_stats.argtypes = [vp, C.POINTER(C.c_double)]
         dp = C.POINTER(C.c_double)
         L.orc_fft.argtypes = [dp, dp, C.c_int, C.c_int]
         L.orc_dft_naive.argtypes = [dp, dp, dp, dp, C.c_int, C.c_int]
@@ -195,6 +197,12 @@ class Oracle:
         if fn(self.h, C.byref(og), _dp(a)):
             raise RuntimeError("orc_interp_input_" + name + " failed")
 
+    def interp_input_humid(self, og, q):
+        """InterpInputHum: the old humidity (nzo-1, nyo-1, nxo) onto this base grid."""
+        a = np.ascontiguousarray(q, dtype=np.float64)
+        if self.L.orc_interp_input_humid(self.h, C.byref(og), _dp(a)):
+            raise RuntimeError("orc_interp_input_humid failed")
+
     def time_marcher(self, dt):
         self.L.orc_time_marcher(self.h, dt)
 
@@ -231,6 +239,14 @@ class Oracle:
         out = np.zeros(2 * n)
         self.L.orc_phi_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return {"phibar": out[:n].copy(), "phirms": out[n:].copy()}
+
+    def moist_stats(self):
+        """CalcMoistStats (moisture.F90:422-505): dict of the 7 profiles (nxm values each)."""
+        n = int(self.cfg.nxm)
+        out = np.zeros(7 * n)
+        self.L.orc_moist_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        names = ["qbar", "qsbar", "qrms", "qrel", "vxq", "vyq", "vzq"]
+        return {m: out[q * n:(q + 1) * n].copy() for q, m in enumerate(names)}
 
     def sal_stats(self):
         """CalcSalStats (salinity.F90:510-690): dict of the 6 refined profiles (nxmr values each)."""

@@ -325,3 +325,76 @@ def test_fortran_host_restart_rejects_another_domain(tmp_path):
     out = subprocess.run([EXE, "fused", "seed=7"], cwd=two, capture_output=True, text=True, timeout=120)
     assert out.returncode != 0
     assert "domain size does not match" in out.stderr
+
+
+# humid.in of the reference's RainyBenard example (moisture.F90:64-96 reads it positionally)
+HUMID_IN = """An input file for parameters used in moist convection
+---
+
+alpha (Dimensionless saturation parameter)
+3.0
+
+beta (Dimensionless lapse rate)
+1.2
+
+gamma (Dimensioness latent heat parameter)
+0.19
+
+tau (Condensation time scale)
+0.375
+
+Diffusive scale for tau? (.true. / .false.)
+.false.
+
+Fixed humidity boundary conditions (1 / 0)
+top     bottom
+1       1
+
+Sm (Humid-to-thermal diffusivity ratio)
+1.0
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
+@pytest.mark.parametrize("mode", ["calltree", "fused"])
+def test_fortran_host_moisture(tmp_path, mode):
+    """humid.in switches moisture on (main.F90:38-39): the host reads it,
+    sets the saturated humidity walls, builds the humid-blob initial state
+    (CreateInitialHumidity) and runs ExplicitHumidity, AddCondensation,
+    ImplicitHumidity and UpdateSaturation in its call tree (or fused).  From
+    its start-up snapshot the oracle reaches the same checkpoint, humidity
+    included, within 1e-10; means.h5 holds the CalcMoistStats profiles."""
+    import pyafid
+    nxm, nym, nzm, ntst, dt = 24, 16, 12, 5, 1e-3
+    bou(os.path.join(tmp_path, "bou.in"), nxm, nym, nzm, ntst, dt)
+    open(os.path.join(tmp_path, "humid.in"), "w").write(HUMID_IN)
+    r = run_host(tmp_path, mode)
+    assert int(r["steps"]) == ntst + 1 and int(r["errorcode"]) == 555, r
+    cfg = dict(nxm=nxm, nym=nym, nzm=nzm, istr3=8, rayt=1e5, prat=1.0, moist=1, q_alpha=3.0, q_beta=1.2, q_gamma=0.19,
+               q_tau=0.375, q_ifdiff=0, q_fixn=1, q_fixs=1, q_sm=1.0)
+    o = pyoracle.Oracle(cfg)
+    o.field("tempbp")[...] = 0.5            # SetTempBCs.F90:27-33 (RBC, inslwN = 1)
+    o.field("temptp")[...] = -0.5
+    o.call("set_humidity_bcs")
+    out = tmp_path / "outputdir"
+    base = (nzm, nym, nxm + 1)
+    names = ["temp", "vx", "vy", "vz", "humid"]
+    for n in names:
+        o.field(n)[...] = pyafid.h5_read_field(out / "fields" / f"00000_{n}.h5", base)
+    q0 = o.field("humid")[..., :nxm]
+    assert q0.max() > 4.0                       # the humid blob 5 exp(-r^2/0.005) is there
+    o.call("update_saturation")
+    assert np.any(q0 > o.field("qsat")[..., :nxm])     # and condenses
+    for _ in range(ntst + 1):
+        o.time_marcher(dt)
+    for n in names + ["pr"]:
+        a = pyafid.h5_read_field(out / f"continua_{n}.h5", base)
+        b = o.field(n)
+        if n != "vx":
+            a, b = a[..., :nxm], b[..., :nxm]
+        if n == "pr":
+            a, b = a - a.mean(), b - b.mean()
+        e = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        assert e <= 1e-10, f"{mode} {n}: {e:.3e}"
+    assert pyafid.h5_dims(out / "means.h5", "qbar/00000") == (nxm,)
+    o.close()

@@ -140,3 +140,36 @@ def test_moisture_calls_need_moist():
         with pytest.raises(pyafid.AfidError, match="moisture is not enabled"):
             g.call(r)
     g.close()
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_moist_stats(form):
+    """CalcMoistStats against the oracle on the same state (FP64 sums in a
+    fixed order on both sides: 1e-12 of each profile's max norm)."""
+    g, o = moist_pair(ODD, 19, form)
+    a, b = g.moist_stats(), o.moist_stats()
+    for n in pyafid.MOIST_PROFILES:
+        e = relerr(a[n], b[n])
+        assert e <= 1e-12, (n, e)
+    g.close(); o.close()
+
+
+def test_humidity_restart_onto_new_grid():
+    """InterpInputHum (moisture.F90:539-582) onto a new grid: the library
+    against the oracle from the same old field and wall values."""
+    new = dict(CFG, nxm=40, nym=12, nzm=24, istr3=4, str3=1.5)
+    old = dict(nx=CFG["nxm"] + 1, ny=CFG["nym"] + 1, nz=CFG["nzm"] + 1, istr3=8, str3=0.0)
+    Q = 1.0 + uniform(23, (CFG["nzm"], CFG["nym"], CFG["nxm"] + 1))
+    zero = np.zeros_like(Q)
+    g = pyafid.Solver(new)
+    o = pyoracle.Oracle(new)
+    for obj, up in ((g, True), (o, False)):
+        planes = {"tempbp": np.full((24, 12, 1), 0.1), "temptp": np.full((24, 12, 1), 0.4)}
+        load_state(obj, planes, upload=up)
+        obj.call("set_humidity_bcs")
+    g.interp_input(old, dict(vx=zero, vy=zero, vz=zero, temp=zero, humid=Q))
+    o.interp_input_humid(pyoracle.old_grid(dict(nxm=CFG["nxm"], nym=CFG["nym"], nzm=CFG["nzm"], istr3=8, str3=0.0)), Q)
+    e = relerr(g.download("humid")[..., :40], o.field("humid")[..., :40])
+    assert e <= 1e-13, e
+    assert not np.any(g.download("ruhumid"))
+    g.close(); o.close()

@@ -168,3 +168,32 @@ def test_humidity_diffusion_eigenmode():
 def test_humidity_pin_catches_planted_mistake():
     with po.planted(8):                               # x-diffusion with pect instead of pecq
         assert humidity_decay_error() > 1e-3
+
+
+def test_moist_stats_closed_forms():
+    """CalcMoistStats (moisture.F90:422-505) on fields with closed-form
+    averages: q = f(x) + A sin(2 pi y / Ly), qsat = c(x) uniform in y and z,
+    vx = U on the interior faces (0 on the walls), vy = V, vz = W uniform:
+    qbar = f, qsbar = c, qrms = sqrt(f^2 + A^2 / 2), qrel = f / c, vxq =
+    U f (U / 2 f in the two wall cells), vyq = V f, vzq = W f."""
+    nxm, ny, nz = 10, 8, 6
+    o = po.Oracle(dict(nxm=nxm, nym=ny, nzm=nz, istr3=4, str3=1.5, ylen=1.5, zlen=0.75, **MOIST))
+    xm = o.grid("xm")[:nxm]
+    f, c = 1.0 + 0.5 * xm, 2.0 - xm
+    A, U, V, W = 0.3, 0.2, -0.7, 0.45
+    y = np.arange(ny)
+    q = np.zeros_like(o.field("humid"))
+    q[..., :nxm] = f[None, None, :] + A * np.sin(2 * np.pi * y / ny)[None, :, None]
+    o.field("humid")[...] = q
+    o.field("qsat")[...] = 1.0
+    o.field("qsat")[..., :nxm] = c
+    o.field("vx")[...] = 0.0
+    o.field("vx")[..., 1:nxm] = U
+    o.field("vy")[...] = V
+    o.field("vz")[...] = W
+    m = o.moist_stats()
+    wall = np.ones(nxm); wall[0] = wall[-1] = 0.5
+    expect = dict(qbar=f, qsbar=c, qrms=np.sqrt(f * f + A * A / 2), qrel=f / c, vxq=U * wall * f, vyq=V * f, vzq=W * f)
+    for k, v in expect.items():
+        assert np.allclose(m[k], v, rtol=1e-13, atol=1e-14), k
+    o.close()

@@ -131,3 +131,37 @@ def test_refined_scalars():
     p.interp_input_ref(po.old_grid(pf_old), "phi", np.full(S.shape, 0.25))
     assert np.abs(p.field("phi")[..., :new["nxmr"]] - 0.25).max() < 1e-15
     src.close(); o.close(); n.close(); p.close()
+
+
+def test_humidity():
+    """InterpInputHum (moisture.F90:539-582) on the base grid: identity on
+    the same grid; with the saturated walls of SetHumidityBCs (tempbp = 0,
+    temptp = 0.5: humbp = 1, humtp = exp(alpha (0.5 - beta))) the line
+    through the two wall values is exact at every new centre; with
+    zero-gradient walls (qfixS = qfixN = 0) a constant stays constant."""
+    moist = dict(moist=1, q_alpha=3.0, q_beta=1.2, q_gamma=0.19, q_tau=5e-5, q_ifdiff=1, q_fixn=1, q_fixs=1, q_sm=1.0)
+    old = dict(OLD, **moist)
+    new = dict(old, nxm=28, nym=12, nzm=8, istr3=8)
+    src = po.Oracle(old)
+    Q = uniform(21, src.field("humid").shape)
+    o = po.Oracle(old)
+    o.field("temptp")[...] = 0.5
+    o.call("set_humidity_bcs")
+    o.interp_input_humid(po.old_grid(old), Q)
+    assert np.abs(o.field("humid")[..., :old["nxm"]] - Q[..., :old["nxm"]]).max() < 1e-14
+    qlo, qup = 1.0, np.exp(3.0 * (0.5 - 1.2))
+    xo = src.grid("xm")[:old["nxm"]]
+    Q[..., :old["nxm"]] = qlo + (qup - qlo) * xo
+    n = po.Oracle(new)
+    n.field("temptp")[...] = 0.5
+    n.call("set_humidity_bcs")
+    n.field("ruhumid")[...] = 1.0
+    n.interp_input_humid(po.old_grid(old), Q)
+    xn = n.grid("xm")[:new["nxm"]]
+    assert np.abs(n.field("humid")[..., :new["nxm"]] - (qlo + (qup - qlo) * xn)).max() < 1e-14
+    assert not np.any(n.field("ruhumid"))            # the RK history restarts from zero
+    z = po.Oracle(dict(new, q_fixn=0, q_fixs=0))
+    z.call("set_humidity_bcs")
+    z.interp_input_humid(po.old_grid(old), np.full(Q.shape, 0.625))
+    assert np.abs(z.field("humid")[..., :new["nxm"]] - 0.625).max() < 1e-15
+    src.close(); o.close(); n.close(); z.close()
