@@ -1509,10 +1509,13 @@ static void launch_column_base_tz(afid_ctx *c, const KP &p) {
         launch_column_base_tz<CH, HT, PF, TZ / 2>(c, p);      // TZ 1 always fits (CH <= 16)
     }
 }
-// tile depth: 4 unless the LDS does not fit (then 2, then 1); AFID_COLUMN_TZ overrides it
+// tile depth in z: 1 (one workgroup of 4 wavefronts per 4 y columns); deeper
+// tiles (AFID_COLUMN_TZ = 2 or 4, as far as the LDS fits) hold the whole
+// workgroup at each field's barriers and measured slower on the multires
+// workload (25.2 against 15.6 ms per launch with TZ = 4, mixed form)
 template <int CH, typename HT, bool PF>
 static void launch_column_base_form(afid_ctx *c, const KP &p) {
-    const int tz = c->col_tz > 0 ? c->col_tz : 4;
+    const int tz = c->col_tz > 0 ? c->col_tz : 1;
     if (tz >= 4) launch_column_base_tz<CH, HT, PF, 4>(c, p);
     else if (tz >= 2) launch_column_base_tz<CH, HT, PF, 2>(c, p);
     else launch_column_base_tz<CH, HT, PF, 1>(c, p);

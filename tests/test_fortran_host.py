@@ -382,7 +382,10 @@ def test_fortran_host_moisture(tmp_path, mode):
     for n in names:
         o.field(n)[...] = pyafid.h5_read_field(out / "fields" / f"00000_{n}.h5", base)
     q0 = o.field("humid")[..., :nxm]
-    assert q0.max() > 4.0                       # the humid blob 5 exp(-r^2/0.005) is there
+    # the humid blob 5 exp(-r^2/0.005) is there: the nearest cell centre lies
+    # 1/32 in y and 1/24 in z from its centre (r^2 >= 0.0027, q >= 2.9 there
+    # less the x offset of the cosine grid)
+    assert q0.max() > 2.0
     o.call("update_saturation")
     assert np.any(q0 > o.field("qsat")[..., :nxm])     # and condenses
     for _ in range(ntst + 1):
