@@ -115,8 +115,7 @@ struct afid_ctx {
     double *d_mean_part = nullptr, *d_mean_out = nullptr, *d_sal_part = nullptr, *d_sal_out = nullptr;
     double *d_phi_part = nullptr, *d_phi_out = nullptr;
     int mean_groups = 0, sal_groups = 0, phi_groups = 0;
-    int col_tz = 0;                     // z extent of the base column tiles (0: largest that fits)
-    int col_nw = 0;                     // >0: single-pass base column kernel with this many wavefronts per workgroup
+    bool col_four_pass = false;         // base columns through the four-pass kernel (AFID_COLUMN_FOUR_PASS=1)
     // mixed form, gAxis = 1: the FP32 vx explicit term and its history hold
     // the buoyancy minus a per-row FP64 reference bref (that of the
     // conductive profile between the mean wall temperatures), which the FP64
@@ -962,8 +961,7 @@ static int ctx_create(const afid_config *cfg, hipStream_t shared, afid_ctx **out
         return AFID_E_HIP;
     }
     c->device = dev;
-    if (const char *e = std::getenv("AFID_COLUMN_TZ")) c->col_tz = std::atoi(e);
-    if (const char *e = std::getenv("AFID_COLUMN_NW")) c->col_nw = std::atoi(e);
+    if (const char *e = std::getenv("AFID_COLUMN_FOUR_PASS")) c->col_four_pass = std::atoi(e) != 0;
     if (shared) {
         c->stream = shared;
         c->own_stream = false;
@@ -1493,57 +1491,32 @@ int afid_interp_temp_multigrid(afid_ctx *c) {
 }  // extern "C"
 
 // fused column kernels of the step (one sweep per grid, ping-pong buffers)
-// LDS of a base-column workgroup: 2 (3 with the phase field) column buffers
-// per wavefront, 4 x TZ wavefronts; one workgroup must fit the CU's 160 KiB
-template <int CH, bool PF, int TZ>
-constexpr bool column_tile_fits() { return size_t(PF ? 3 : 2) * 4 * TZ * lds_col_size<CH>() * sizeof(double) <= 160 * 1024; }
-
-template <int CH, typename HT, bool PF, int TZ>
-static void launch_column_base_tz(afid_ctx *c, const KP &p) {
-    if constexpr (column_tile_fits<CH, PF, TZ>()) {
-        BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
-        const long long nblk = strip_blocks<TZ>(c->ny, c->nz);
-        hipLaunchKernelGGL((k_column_base<CH, HT, PF, TZ>), dim3(unsigned((nblk + 7) / 8 * 8)), dim3(256 * TZ), 0, c->stream, p,
-                           factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
-    } else {
-        launch_column_base_tz<CH, HT, PF, TZ / 2>(c, p);      // TZ 1 always fits (CH <= 16)
-    }
-}
-// tile depth in z: 1 (one workgroup of 4 wavefronts per 4 y columns); deeper
-// tiles (AFID_COLUMN_TZ = 2 or 4, as far as the LDS fits) hold the whole
-// workgroup at each field's barriers and measured slower on the multires
-// workload (25.2 against 15.6 ms per launch with TZ = 4, mixed form)
 template <int CH, typename HT, bool PF>
 static void launch_column_base_form(afid_ctx *c, const KP &p) {
-    const int tz = c->col_tz > 0 ? c->col_tz : 1;
-    if (tz >= 4) launch_column_base_tz<CH, HT, PF, 4>(c, p);
-    else if (tz >= 2) launch_column_base_tz<CH, HT, PF, 2>(c, p);
-    else launch_column_base_tz<CH, HT, PF, 1>(c, p);
-}
-// single-pass kernel (no phase field): NW wavefronts per workgroup
-template <int CH, typename HT, int NW>
-static void launch_column_base1(afid_ctx *c, const KP &p) {
     BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
-    const long long nblk = strip_blocks_w<NW>(c->ny, c->nz);
-    hipLaunchKernelGGL((k_column_base1<CH, HT, NW>), dim3(unsigned((nblk + 7) / 8 * 8)), dim3(64 * NW), 0, c->stream, p,
+    const long long nblk = strip_blocks(c->ny, c->nz);
+    hipLaunchKernelGGL((k_column_base<CH, HT, PF>), dim3(unsigned((nblk + 7) / 8 * 8)), dim3(256), 0, c->stream, p,
                        factors(c, MAT_X), factors(c, MAT_YZ), factors(c, MAT_T), o);
 }
 template <int CH>
 static void launch_column_base(afid_ctx *c, const KP &p) {
-    if (!c->pf && c->col_nw > 0) {
-        if (c->mixed) {
-            if (c->col_nw >= 2) launch_column_base1<CH, float, 2>(c, p);
-            else launch_column_base1<CH, float, 1>(c, p);
-        } else {
-            if (c->col_nw >= 2) launch_column_base1<CH, double, 2>(c, p);
-            else launch_column_base1<CH, double, 1>(c, p);
-        }
-        return;
-    }
     if (c->mixed && c->pf) launch_column_base_form<CH, float, true>(c, p);
     else if (c->mixed) launch_column_base_form<CH, float, false>(c, p);
     else if (c->pf) launch_column_base_form<CH, double, true>(c, p);
     else launch_column_base_form<CH, double, false>(c, p);
+}
+// single-pass kernel (no phase field); CH here is the chunk of the nxm-row
+// systems
+template <int CH>
+static void launch_column_base1(afid_ctx *c, const KP &p) {
+    BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
+    const dim3 gr(unsigned((strip_blocks(c->ny, c->nz) + 7) / 8 * 8));
+    if (c->mixed)
+        hipLaunchKernelGGL((k_column_base1<CH, float>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ),
+                           factors(c, MAT_T), o);
+    else
+        hipLaunchKernelGGL((k_column_base1<CH, double>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ),
+                           factors(c, MAT_T), o);
 }
 // ExplicitHumidity + ImplicitHumidity + SolveImpEqnUpdate_Humidity of every
 // column into humid_alt; the caller swaps the buffers once the base column
@@ -1568,14 +1541,24 @@ static int column_humid(afid_ctx *c) {
 }
 static int column_base(afid_ctx *c) {
     KP p = make_kp(c);
-    const int ch = (c->nx + 63) / 64;
-    switch (ch) {
+    // single pass up to 512 wall-normal cells (its registers hold two
+    // right-hand sides of CH rows), four passes with the phase field
+    if (!c->pf && !c->col_four_pass && c->chunk[MAT_YZ] <= 8) {
+        switch (c->chunk[MAT_YZ]) {
+#define AFID_CB1_CASE(CH) case CH: launch_column_base1<CH>(c, p); break;
+            AFID_CB1_CASE(1) AFID_CB1_CASE(2) AFID_CB1_CASE(3) AFID_CB1_CASE(4) AFID_CB1_CASE(5) AFID_CB1_CASE(6)
+            AFID_CB1_CASE(7) AFID_CB1_CASE(8)
+#undef AFID_CB1_CASE
+        }
+    } else {
+        switch ((c->nx + 63) / 64) {
 #define AFID_CB_CASE(CH) case CH: launch_column_base<CH>(c, p); break;
-        AFID_CB_CASE(1) AFID_CB_CASE(2) AFID_CB_CASE(3) AFID_CB_CASE(4) AFID_CB_CASE(5) AFID_CB_CASE(6)
-        AFID_CB_CASE(7) AFID_CB_CASE(8) AFID_CB_CASE(9) AFID_CB_CASE(10) AFID_CB_CASE(11) AFID_CB_CASE(12)
-        AFID_CB_CASE(13) AFID_CB_CASE(14) AFID_CB_CASE(15) AFID_CB_CASE(16)
+            AFID_CB_CASE(1) AFID_CB_CASE(2) AFID_CB_CASE(3) AFID_CB_CASE(4) AFID_CB_CASE(5) AFID_CB_CASE(6)
+            AFID_CB_CASE(7) AFID_CB_CASE(8) AFID_CB_CASE(9) AFID_CB_CASE(10) AFID_CB_CASE(11) AFID_CB_CASE(12)
+            AFID_CB_CASE(13) AFID_CB_CASE(14) AFID_CB_CASE(15) AFID_CB_CASE(16)
 #undef AFID_CB_CASE
-    default: return set_err(c, AFID_E_ARG, "nx too large");
+        default: return set_err(c, AFID_E_ARG, "nx too large");
+        }
     }
     int rc = check_launch(c, "k_column_base");
     if (rc) return rc;

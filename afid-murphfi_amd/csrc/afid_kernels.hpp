@@ -303,24 +303,22 @@ __device__ __forceinline__ long long xcd_block(long long) {
     const long long per = gridDim.x / 8;
     return (b % 8) * per + b / 8;
 }
-// Column walk: a workgroup owns a tile of 4 (y) x TZ (z) columns, one per
-// wavefront; tiles form strips of STRIP_BLOCKS*4 columns in y, walked along
-// z, so the y and z neighbour columns a wavefront reads belong to its own
-// workgroup (inside the tile) or to workgroups that ran just before it on
-// the same XCD (still in its L2).
+// Column walk: a workgroup owns 4 consecutive y columns, one per
+// wavefront; workgroups form strips of STRIP_BLOCKS*4 columns in y, walked
+// along z, so the y and z neighbour columns a wavefront reads belong to its
+// own workgroup or to workgroups that ran just before it on the same XCD
+// (still in its L2).
 constexpr int STRIP_BLOCKS = 4;
-template <int TZ = 1>
 __host__ __device__ __forceinline__ long long strip_blocks(int ny, int nz) {
-    return (long long)((ny + 4 * STRIP_BLOCKS - 1) / (4 * STRIP_BLOCKS)) * STRIP_BLOCKS * ((nz + TZ - 1) / TZ);
+    return (long long)((ny + 4 * STRIP_BLOCKS - 1) / (4 * STRIP_BLOCKS)) * STRIP_BLOCKS * nz;
 }
 // (j, i) of this wavefront's column; false if the slot is idle
-template <int TZ = 1>
 __device__ __forceinline__ bool strip_column(long long blk, int wave, int ny, int nz, int &j, int &i) {
-    const long long per_strip = (long long)STRIP_BLOCKS * ((nz + TZ - 1) / TZ);
+    const long long per_strip = (long long)STRIP_BLOCKS * nz;
     const long long strip = blk / per_strip, rem = blk % per_strip;
-    i = int(rem / STRIP_BLOCKS) * TZ + (wave >> 2);
-    j = int(strip * 4 * STRIP_BLOCKS + (rem % STRIP_BLOCKS) * 4 + (wave & 3));
-    const bool ok = blk < strip_blocks<TZ>(ny, nz) && j < ny && i < nz;
+    i = int(rem / STRIP_BLOCKS);
+    j = int(strip * 4 * STRIP_BLOCKS + (rem % STRIP_BLOCKS) * 4 + wave);
+    const bool ok = blk < strip_blocks(ny, nz) && j < ny && i < nz;
     if (!ok) { j = 0; i = 0; }
     return ok;
 }
@@ -576,14 +574,15 @@ __device__ __forceinline__ void column_base_rows(const KP &p, const Nb &n, long 
     }
 }
 
-// TZ: z extent of the workgroup's column tile (4 x TZ wavefronts)
-template <int CH, typename HT, bool PF, int TZ>
-__global__ void __launch_bounds__(256 * TZ) k_column_base(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
-    __shared__ double lds[PF ? 3 : 2][4 * TZ][lds_col_size<CH>()];
+// Four passes, one field each (the phase-field form, and wall-normal sizes
+// above 512 cells).  CH = ceil(nx / 64).
+template <int CH, typename HT, bool PF>
+__global__ void __launch_bounds__(256) k_column_base(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
+    __shared__ double lds[PF ? 3 : 2][4][lds_col_size<CH>()];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
     int j, i;
-    const bool live = strip_column<TZ>(xcd_block(0), wave, p.ny, p.nz, j, i);
+    const bool live = strip_column(xcd_block(0), wave, p.ny, p.nz, j, i);
     double *buf = lds[0][wave], *own = lds[1][wave], *msk = lds[PF ? 2 : 1][wave];
     const Nb n = make_nb(p, j, i);
     const long long c0 = (long long)i * p.plane + (long long)j * p.ldx;
@@ -613,26 +612,6 @@ __global__ void __launch_bounds__(256 * TZ) k_column_base(KP p, TriFactors fx, T
         }
         __syncthreads();
     }
-}
-
-// Column walk of the single-pass kernel: workgroups of NW wavefronts cover NW
-// consecutive y columns of a strip 16 columns wide; the strips are walked
-// along z as in strip_column.
-constexpr int STRIP_W = 16;
-template <int NW>
-__host__ __device__ __forceinline__ long long strip_blocks_w(int ny, int nz) {
-    return (long long)((ny + STRIP_W - 1) / STRIP_W) * (STRIP_W / NW) * nz;
-}
-template <int NW>
-__device__ __forceinline__ bool strip_column_w(long long blk, int wave, int ny, int nz, int &j, int &i) {
-    constexpr int SB = STRIP_W / NW;
-    const long long per_strip = (long long)SB * nz;
-    const long long strip = blk / per_strip, rem = blk % per_strip;
-    i = int(rem / SB);
-    j = int(strip * STRIP_W + (rem % SB) * NW + wave);
-    const bool ok = blk < strip_blocks_w<NW>(ny, nz) && j < ny && i < nz;
-    if (!ok) { j = 0; i = 0; }
-    return ok;
 }
 
 // Right-hand sides of vx, vy, vz and T at row r < nx of one column (the
@@ -679,46 +658,74 @@ __device__ __forceinline__ void base_rhs4(const KP &p, const Nb &n, long long c0
     if (wc) { ruy[r] = h1; ruz[r] = h2; rut[r] = h3; }
 }
 
-// Single-pass form of k_column_base (no phase field): one sweep over the rows
-// forms the four right-hand sides into four LDS columns per wavefront, the
-// four tridiagonal solves follow, and the update adds each solution to the
-// old value of the own column, read again from global memory (in cache).
-// LDS per wavefront: 4 columns of 64 x lds_chunk_tight<CH>() doubles.
-template <int CH, typename HT, int NW>
-__global__ void __launch_bounds__(64 * NW) k_column_base1(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
+// Single-pass form of k_column_base (no phase field).  One sweep over the
+// rows forms the four right-hand sides, loading every neighbour value once
+// for all four explicit terms.  Those of vx and vy go to two LDS columns of
+// the wavefront, those of vz and T stay in registers.  The vx and vy solves
+// and updates follow, then the vz and T right-hand sides move into the same
+// two LDS columns for their solves and updates.  Two LDS columns per
+// wavefront instead of four keep 4 wavefronts per SIMD; the updates read the
+// old value of the own column again (from cache).
+// CH = ceil(nxm / 64) is the size of the largest system (nxm rows); when
+// nx = 64 CH + 1 the last face has no equation (the wall of vx, the unused
+// index of the others) and only the update touches it.
+template <int CH, typename HT>
+__global__ void __launch_bounds__(256) k_column_base1(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
     constexpr int CHP = lds_chunk_tight<CH>();
-    __shared__ double lds[NW][4][lds_col_size<CH, CHP>()];
+    constexpr unsigned NROW = 64u * CH;
+    __shared__ double lds[4][2][lds_col_size<CH, CHP>()];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
     int j, i;
-    const bool live = strip_column_w<NW>(xcd_block(0), wave, p.ny, p.nz, j, i);
+    const bool live = strip_column(xcd_block(0), wave, p.ny, p.nz, j, i);
     const Nb n = make_nb(p, j, i);
     const long long c0 = (long long)i * p.plane + (long long)j * p.ldx;
     const unsigned nx = p.nx, nxm = p.nxm;
-    double *b0 = lds[wave][0], *b1 = lds[wave][1], *b2 = lds[wave][2], *b3 = lds[wave][3];
-#pragma unroll 1
-    for (unsigned r = lane; r < 64u * CH; r += 64) {
+    double *b0 = lds[wave][0], *b1 = lds[wave][1];
+    double keep[2][CH];                          // right-hand sides of vz and T
+#pragma unroll
+    for (int m = 0; m < CH; ++m) {
+        const unsigned r = lane + 64u * m;
         double rhs[4] = {0.0, 0.0, 0.0, 0.0};
-        if (live && r < nx) base_rhs4<HT>(p, n, c0, r, rhs);
+        if (live && r < nxm) base_rhs4<HT>(p, n, c0, r, rhs);
         const unsigned q = lds_row<CH, CHP>(r);
-        b0[q] = rhs[0]; b1[q] = rhs[1]; b2[q] = rhs[2]; b3[q] = rhs[3];
+        b0[q] = rhs[0];
+        b1[q] = rhs[1];
+        keep[0][m] = rhs[2];
+        keep[1][m] = rhs[3];
     }
     __syncthreads();
-    column_solve_lds<CH, CHP>(b0, fx);
-    column_solve_lds<CH, CHP>(b1, fyz);
-    column_solve_lds<CH, CHP>(b2, fyz);
-    column_solve_lds<CH, CHP>(b3, ft);
-    __syncthreads();
-    if (!live) return;
-    double *ox = o.vx + c0, *oy = o.vy + c0, *oz = o.vz + c0, *ot = o.temp + c0;
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+        if (g == 1) {
+#pragma unroll
+            for (int m = 0; m < CH; ++m) {
+                const unsigned q = lds_row<CH, CHP>(lane + 64u * m);
+                b0[q] = keep[0][m];
+                b1[q] = keep[1][m];
+            }
+            __syncthreads();
+        }
+        column_solve_lds<CH, CHP>(b0, g == 0 ? fx : fyz);
+        column_solve_lds<CH, CHP>(b1, g == 0 ? fyz : ft);
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                const int fld = 2 * g + f;
+                const double *old = fld == 0 ? n.vx : fld == 1 ? n.vy : fld == 2 ? n.vz : n.t;
+                double *out = (fld == 0 ? o.vx : fld == 1 ? o.vy : fld == 2 ? o.vz : o.temp) + c0;
+                const double *d = f == 0 ? b0 : b1;
 #pragma unroll 1
-    for (unsigned r = lane; r < nx; r += 64) {
-        const unsigned q = lds_row<CH, CHP>(r);
-        const double ux = n.vx[r], uy = n.vy[r], uz = n.vz[r], ut = n.t[r];
-        ox[r] = (r >= 1 && r < nxm) ? ux + b0[q] : 0.0;     // walls, ImplicitAndUpdateVX.F90:86-87
-        oy[r] = r < nxm ? uy + b1[q] : uy;
-        oz[r] = r < nxm ? uz + b2[q] : uz;
-        ot[r] = r < nxm ? ut + b3[q] : ut;
+                for (unsigned r = lane; r < nx; r += 64) {
+                    const double u = old[r];
+                    const double x = d[lds_row<CH, CHP>(r < NROW ? r : NROW - 1)];
+                    if (fld == 0) out[r] = (r >= 1 && r < nxm) ? u + x : 0.0;     // walls, ImplicitAndUpdateVX.F90:86-87
+                    else out[r] = r < nxm ? u + x : u;
+                }
+            }
+        }
+        if (g == 0) __syncthreads();
     }
 }
 
