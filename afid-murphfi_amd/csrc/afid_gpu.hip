@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -948,6 +949,8 @@ extern "C" {
 
 }  // extern "C"
 
+static int scratch_ptr_for_device(afid_ctx *c, int dev);
+
 // A context on the caller's stream (shared) or on a stream of its own.
 static int ctx_create(const afid_config *cfg, hipStream_t shared, afid_ctx **out) {
     if (!cfg || !out) return AFID_E_ARG;
@@ -975,6 +978,9 @@ static int ctx_create(const afid_config *cfg, hipStream_t shared, afid_ctx **out
     } catch (const std::exception &e) {
         rc = set_err(c, AFID_E_ARG, e.what());
     }
+    // the static scratch of afid_calc_mean_profiles_noalloc: its address on
+    // this device, once per device
+    if (rc == AFID_OK) rc = scratch_ptr_for_device(c, dev);
     if (rc == AFID_OK) {
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = set_err(c, AFID_E_HIP, std::string("set-up: ") + hipGetErrorString(e));
@@ -2110,6 +2116,64 @@ int afid_calc_mean_profiles(afid_ctx *c, double *out) {
     HIPCHK(c, hipMemcpyAsync(out, c->d_mean_out, sizeof(double) * size_t(MEAN_NQ) * nxm, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return AFID_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// CalcMeanProfiles without allocation: the partial sums live in the
+// file-scope device array `scratch`, whose address on each device is taken
+// once at context set-up (scratch_ptr_for_device) and used from then on.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[AFID_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_ptr;     // device ordinal -> address of `scratch` there
+}  // namespace
+
+// Address of `scratch` on the current device (`dev`), looked up once per
+// device.  Called from context set-up with that device current.
+static int scratch_ptr_for_device(afid_ctx *c, int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(dev)) return AFID_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_err(c, AFID_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+    return AFID_OK;
+}
+
+static double *scratch_ptr(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    return it == g_scratch_ptr.end() ? nullptr : it->second;
+}
+
+extern "C" {
+
+int afid_calc_mean_profiles_noalloc(afid_ctx *c, double *out_dev) {
+    if (!c || !out_dev) return AFID_E_ARG;
+    if (c->nxm < 2) return set_err(c, AFID_E_ARG, "CalcMeanProfiles needs nxm >= 2");
+    const int nxm = c->nxm, groups = mean_ygroups(c->ny);
+    const long long need = (long long)groups * MEAN_NQ * nxm;
+    if (need > AFID_NOALLOC_SCRATCH_DOUBLES)
+        return set_err(c, AFID_E_ARG, "afid_calc_mean_profiles_noalloc: the partial sums need " + std::to_string(need) +
+                                          " doubles (15 x nxm x ceil(ny/4)); the static scratch holds " +
+                                          std::to_string((long long)AFID_NOALLOC_SCRATCH_DOUBLES));
+    double *part = scratch_ptr(c->device);
+    if (!part) return set_err(c, AFID_E_STATE, "afid_calc_mean_profiles_noalloc: no scratch address for this device");
+    int rc;
+    KP p = make_kp(c);
+    // every (group, profile, x) partial is written before k_mean_finish reads
+    // it, so the scratch needs no clearing
+    hipLaunchKernelGGL(k_mean_partials, dim3(unsigned((nxm + 63) / 64), unsigned(groups)), dim3(256), 0, c->stream, p,
+                       c->d_sdx, part);
+    if ((rc = check_launch(c, "k_mean_partials"))) return rc;
+    hipLaunchKernelGGL(k_mean_finish, dim3(unsigned((MEAN_NQ * nxm + 255) / 256)), dim3(256), 0, c->stream, p, part, groups,
+                       out_dev);
+    return check_launch(c, "k_mean_finish");
 }
 
 int afid_calc_moist_stats(afid_ctx *c, double *out) {

@@ -36,7 +36,7 @@ EXPORTS = [
     "afid_explicit_phase", "afid_add_volume_penalty", "afid_implicit_phase", "afid_add_latent_heat",
     "afid_interp_temp_multigrid", "afid_interp_phi_multigrid",
     "afid_interp_vel_mgrd", "afid_interp_sal_multigrid", "afid_time_marcher", "afid_time_marcher_graph", "afid_refinement_residuals",
-    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_calc_sal_stats", "afid_calc_phi_stats", "afid_calc_moist_stats",
+    "afid_calc_max_cfl", "afid_check_divergence", "afid_global_quantities", "afid_calc_mean_profiles", "afid_calc_mean_profiles_noalloc", "afid_calc_sal_stats", "afid_calc_phi_stats", "afid_calc_moist_stats",
     "afid_synchronize", "afid_set_ibm",
     "afid_set_humidity_bcs", "afid_explicit_humidity", "afid_add_condensation", "afid_implicit_humidity",
     "afid_update_saturation",
@@ -202,6 +202,8 @@ def lib():
         L.afid_check_divergence.argtypes = [vp, dp, dp]
         L.afid_global_quantities.argtypes = [vp, dp]
         L.afid_calc_mean_profiles.argtypes = [vp, dp]
+        # out is a device address (an integer), not a host array
+        L.afid_calc_mean_profiles_noalloc.argtypes = [vp, C.c_void_p]
         L.afid_calc_sal_stats.argtypes = [vp, dp]
         L.afid_calc_phi_stats.argtypes = [vp, dp]
         L.afid_calc_moist_stats.argtypes = [vp, dp]

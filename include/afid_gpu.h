@@ -201,6 +201,19 @@ int afid_global_quantities(afid_ctx *ctx, double out[4]);
  * doubles; the host writes the file (HdfSerialWriteReal1D).  FP64 in both
  * forms.  Needs nxm >= 2.  Waits for the result. */
 int afid_calc_mean_profiles(afid_ctx *ctx, double *out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight diagnostic loops, hosts that
+ * forbid hipMalloc): afid_calc_mean_profiles with the partial sums in a
+ * static device array of AFID_NOALLOC_SCRATCH_DOUBLES doubles (16 MiB, held
+ * on every GPU the library runs on until the process ends) instead of the
+ * buffers afid_calc_mean_profiles allocates on its first call.  out_dev is
+ * DEVICE memory of 15*nxm doubles, written in the same order; the call only
+ * enqueues work on the context's stream (no allocation, no copy, no wait:
+ * afid_synchronize before reading out_dev).  Grids whose partials exceed the
+ * array (15 * nxm * ceil(ny/4) doubles) get AFID_E_ARG.  All contexts on one
+ * GPU share the array: calls that overlap in time on one GPU race. */
+#define AFID_NOALLOC_SCRATCH_DOUBLES (2 * 1024 * 1024)
+int afid_calc_mean_profiles_noalloc(afid_ctx *ctx, double *out_dev);
 /* CalcSalStats + CalcDissipationSal (salinity.F90:510-690, non-IBM): the 6
  * refined-grid profiles Sbar, Srms, vxS, vyS, vzS, chiS, out[q * nxmr + k],
  * the order of means.h5 (:583-597); out holds 6*nxmr doubles.  Sums in FP64
