@@ -343,7 +343,7 @@ __device__ __forceinline__ void ens_column(const EnsRun &R, const KP &p, int cur
         wave_sync();
         const double *f = fac + fi * 4 * NP;
         TriFactors tf;
-        tf.l = f; tf.e = f + NP; tf.g = f + 2 * NP; tf.s = f + 3 * NP; tf.a = nullptr; tf.c = nullptr;
+        tf.l = f; tf.e = f + NP; tf.g = f + 2 * NP; tf.s = f + 3 * NP; tf.a = nullptr; tf.c = nullptr; tf.ln = nullptr; tf.sn = nullptr;
         tf.n = fi == 0 ? nx : nxm; tf.chunk = CH;
         column_solve_lds<CH>(buf, tf);
         wave_sync();

@@ -309,7 +309,7 @@ static int check_launch(afid_ctx *c, const char *what) {
 
 // arrays per tridiagonal kind and substep: L, E, G, S (factors) and A, C
 // (normalised off-diagonals, for masked solves)
-constexpr int kTriArrays = 6;
+constexpr int kTriArrays = 8;     // l, e, g, s, a, c, ln, sn
 
 // ---------------------------------------------------------------------------
 // tridiagonal factors (SolveImpEqnUpdate_X/YZ/Temp/Sal/Phi matrices, computed
@@ -317,7 +317,7 @@ constexpr int kTriArrays = 6;
 // ---------------------------------------------------------------------------
 static void lu_rows(const std::vector<double> &am, const std::vector<double> &ac, const std::vector<double> &ap,
                     double bd, int n, int r0, int r1, int npad, double *L, double *E, double *G, double *S,
-                    double *A, double *C) {
+                    double *A, double *C, double *LN, double *SN) {
     // rows r (0-based) in [r0, r1) carry coefficients; others are identity rows with zero RHS
     std::vector<double> a(size_t(npad), 0.0), cc(size_t(npad), 0.0), s(size_t(npad), 0.0);
     for (int r = r0; r < r1; ++r) {
@@ -328,11 +328,13 @@ static void lu_rows(const std::vector<double> &am, const std::vector<double> &ac
     }
     double u_prev = 1.0;
     for (int r = 0; r < npad; ++r) {
-        if (r >= n) { L[r] = 0; E[r] = 0; G[r] = 0; S[r] = 0; A[r] = 0; C[r] = 0; continue; }
+        if (r >= n) { L[r] = 0; E[r] = 0; G[r] = 0; S[r] = 0; A[r] = 0; C[r] = 0; LN[r] = 0; SN[r] = 0; continue; }
         const double l = r > 0 ? a[size_t(r)] / u_prev : 0.0;
         const double u = 1.0 - l * (r > 0 ? cc[size_t(r - 1)] : 0.0);
         L[r] = l; E[r] = cc[size_t(r)] / u; G[r] = 1.0 / u; S[r] = s[size_t(r)];
         A[r] = a[size_t(r)]; C[r] = cc[size_t(r)];
+        LN[r] = l * u_prev / u;             // g_r l_r / g_{r-1}
+        SN[r] = s[size_t(r)] / u;           // g_r s_r
         u_prev = u;
     }
 }
@@ -349,24 +351,25 @@ static int ensure_factors(afid_ctx *c) {
             if (kind == MAT_Q && !c->moist) continue;
             double *base = c->tri_host.data() + size_t((s * MAT_NKIND + kind) * kTriArrays * np);
             double *L = base, *E = base + np, *G = base + 2 * np, *S = base + 3 * np, *A = base + 4 * np, *C = base + 5 * np;
+            double *LN = base + 6 * np, *SN = base + 7 * np;
             switch (kind) {
             case MAT_X:   // SolveImpEqnUpdate_X.F90:22-54: identity rows 1 and nx
-                lu_rows(c->h_am3ck, c->h_ac3ck, c->h_ap3ck, beta * al, c->nx, 1, c->nxm, np, L, E, G, S, A, C);
+                lu_rows(c->h_am3ck, c->h_ac3ck, c->h_ap3ck, beta * al, c->nx, 1, c->nxm, np, L, E, G, S, A, C, LN, SN);
                 break;
             case MAT_YZ:  // SolveImpEqnUpdate_YZ.F90:25-44
-                lu_rows(c->h_am3sk, c->h_ac3sk, c->h_ap3sk, beta * al, c->nxm, 0, c->nxm, np, L, E, G, S, A, C);
+                lu_rows(c->h_am3sk, c->h_ac3sk, c->h_ap3sk, beta * al, c->nxm, 0, c->nxm, np, L, E, G, S, A, C, LN, SN);
                 break;
             case MAT_T:   // SolveImpEqnUpdate_Temp.F90:25-46
-                lu_rows(c->h_am3ssk, c->h_ac3ssk, c->h_ap3ssk, 0.5 * al * c->dt / c->pect, c->nxm, 0, c->nxm, np, L, E, G, S, A, C);
+                lu_rows(c->h_am3ssk, c->h_ac3ssk, c->h_ap3ssk, 0.5 * al * c->dt / c->pect, c->nxm, 0, c->nxm, np, L, E, G, S, A, C, LN, SN);
                 break;
             case MAT_S:   // salinity.F90:418-440
-                lu_rows(c->h_am3sskr, c->h_ac3sskr, c->h_ap3sskr, 0.5 * al * c->dt / c->pecs, c->nxmr, 0, c->nxmr, np, L, E, G, S, A, C);
+                lu_rows(c->h_am3sskr, c->h_ac3sskr, c->h_ap3sskr, 0.5 * al * c->dt / c->pecs, c->nxmr, 0, c->nxmr, np, L, E, G, S, A, C, LN, SN);
                 break;
             case MAT_P:   // SolveImpEqnUpdate_Phi, phasefield.F90:427-443
-                lu_rows(c->h_am3spkr, c->h_ac3spkr, c->h_ap3spkr, 0.5 * al * c->dt * c->pf_d, c->nxmr, 0, c->nxmr, np, L, E, G, S, A, C);
+                lu_rows(c->h_am3spkr, c->h_ac3spkr, c->h_ap3spkr, 0.5 * al * c->dt * c->pf_d, c->nxmr, 0, c->nxmr, np, L, E, G, S, A, C, LN, SN);
                 break;
             case MAT_Q:   // SolveImpEqnUpdate_Humidity, moisture.F90:382-396
-                lu_rows(c->h_am3ssk, c->h_ac3ssk, c->h_ap3ssk, 0.5 * al * c->dt / c->pecq, c->nxm, 0, c->nxm, np, L, E, G, S, A, C);
+                lu_rows(c->h_am3ssk, c->h_ac3ssk, c->h_ap3ssk, 0.5 * al * c->dt / c->pecq, c->nxm, 0, c->nxm, np, L, E, G, S, A, C, LN, SN);
                 break;
             }
         }
@@ -382,6 +385,7 @@ static TriFactors factors(const afid_ctx *c, int kind) {
     const double *base = c->tri + size_t(((c->ns - 1) * MAT_NKIND + kind) * kTriArrays * np);
     TriFactors f;
     f.l = base; f.e = base + np; f.g = base + 2 * np; f.s = base + 3 * np; f.a = base + 4 * np; f.c = base + 5 * np;
+    f.ln = base + 6 * np; f.sn = base + 7 * np;
     f.n = c->nrows[kind]; f.chunk = c->chunk[kind];
     return f;
 }

@@ -192,6 +192,32 @@ __device__ __forceinline__ void wave_tri_solve(double (&v)[CH], const TriFactors
     for (int m = CH - 1; m >= 0; --m) { x = -e[m] * x + g[m] * v[m]; v[m] = x; }
 }
 
+// The same solve with the folded factors (TriFactors::ln, e): v in is the
+// right-hand side already scaled by sn, v out the solution.
+template <int CH>
+__device__ __forceinline__ void wave_tri_solve_folded(double (&v)[CH], const double *ln, const double *e) {
+    const int lane = threadIdx.x & 63;
+    const double *l = ln + lane * CH, *ee = e + lane * CH;
+    // forward: y'_r = v_r - ln_r y'_{r-1}
+    double A = 1.0, B = 0.0;
+#pragma unroll
+    for (int m = 0; m < CH; ++m) { const double lm = l[m]; B = -lm * B + v[m]; A = -lm * A; }
+    wave_affine_scan<true>(A, B);
+    double y = __shfl_up(B, 1, 64);
+    if (lane == 0) y = 0.0;
+#pragma unroll
+    for (int m = 0; m < CH; ++m) { y = -l[m] * y + v[m]; v[m] = y; }
+    // backward: x_r = y'_r - e_r x_{r+1}
+    A = 1.0; B = 0.0;
+#pragma unroll
+    for (int m = CH - 1; m >= 0; --m) { const double em = ee[m]; B = -em * B + v[m]; A = -em * A; }
+    wave_affine_scan<false>(A, B);
+    double x = __shfl_down(B, 1, 64);
+    if (lane == 63) x = 0.0;
+#pragma unroll
+    for (int m = CH - 1; m >= 0; --m) { x = -ee[m] * x + v[m]; v[m] = x; }
+}
+
 // 2x2 projective matrix (normalised so its largest entry is 1)
 struct Mob { double m00, m01, m10, m11; };
 __device__ __forceinline__ Mob mob_mul(const Mob &x, const Mob &y) {   // x * y
@@ -348,6 +374,18 @@ __device__ __forceinline__ void column_solve_lds(double *buf, const TriFactors &
     for (int m = 0; m < CH; ++m) buf[lane * CHP + m] = v[m];
 }
 
+// column_solve_lds with the folded factors (two tables, e.g. staged in LDS)
+template <int CH, int CHP = lds_chunk<CH>()>
+__device__ __forceinline__ void column_solve_lds_folded(double *buf, const double *ln, const double *e) {
+    const int lane = threadIdx.x & 63;
+    double v[CH];
+#pragma unroll
+    for (int m = 0; m < CH; ++m) v[m] = buf[lane * CHP + m];
+    wave_tri_solve_folded<CH>(v, ln, e);
+#pragma unroll
+    for (int m = 0; m < CH; ++m) buf[lane * CHP + m] = v[m];
+}
+
 // two-element vector of the storage type (64-bit or 128-bit lane accesses)
 // Masked solve of one column held in LDS (SolveImpEqnUpdate_X_pf /
 // _YZ_pf, phasefield.F90:821-930): rows flagged in msk are the identity with
@@ -395,13 +433,12 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_
     constexpr int SUB = Vec16<RT>::n;
     constexpr int MG = (64 * CH + 64 * SUB - 1) / (64 * SUB);   // group iterations covering the solver's 64*CH rows
     __shared__ double lds[4][lds_col_size<CH>()];
-    // the solver's factors, read by every column of the workgroup
-    __shared__ double fac[4][64 * CH];
+    // the solver's folded factors, read by every column of the workgroup
+    // (two tables: 39 KB of LDS in all at CH = 12, four workgroups per CU)
+    __shared__ double fac[2][64 * CH];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
-    for (int q = threadIdx.x; q < 64 * CH; q += 256) { fac[0][q] = f.l[q]; fac[1][q] = f.e[q]; fac[2][q] = f.g[q]; fac[3][q] = f.s[q]; }
-    TriFactors fl = f;
-    fl.l = fac[0]; fl.e = fac[1]; fl.g = fac[2]; fl.s = fac[3];
+    for (int q = threadIdx.x; q < 64 * CH; q += 256) { fac[0][q] = f.ln[q]; fac[1][q] = f.e[q]; }
     int j, i;
     const bool live = strip_column(xcd_block(0), wave, p.nyr, p.nzr, j, i);
     double *buf = lds[wave];
@@ -455,10 +492,10 @@ __global__ void __launch_bounds__(256) k_column_sal(KP p, TriFactors f, RT *sal_
         }
 #pragma unroll
         for (int q = 0; q < SUB; ++q)
-            if (r0 + q < 64u * CH) buf[lds_row<CH>(r0 + q)] = rhs[q];
+            if (r0 + q < 64u * CH) buf[lds_row<CH>(r0 + q)] = rhs[q] * f.sn[r0 + q];
     }
     __syncthreads();
-    column_solve_lds<CH>(buf, fl);
+    column_solve_lds_folded<CH>(buf, fac[0], fac[1]);
     __syncthreads();
     if (!live) return;
     RT *out = sal_out + c0;

@@ -19,8 +19,13 @@ enum MatKind { MAT_X = 0, MAT_YZ = 1, MAT_T = 2, MAT_S = 3, MAT_P = 4, MAT_Q = 5
 //   backward  x_r = g_r y_r - e_r x_{r+1}
 // a, c: the normalised off-diagonals of each row (row r: a_r x_{r-1} + x_r +
 // c_r x_{r+1} = s_r d_r), for solves whose rows a mask may replace.
+// Folded form of the same factors (the pivot g moved into the forward sweep,
+// the row scale s into the right-hand side), two tables instead of four:
+//   forward   y'_r = sn_r d_r - ln_r y'_{r-1}   (sn = g s, ln_r = g_r l_r / g_{r-1})
+//   backward  x_r  = y'_r - e_r x_{r+1}
 struct TriFactors {
     const double *l, *e, *g, *s, *a, *c;
+    const double *ln, *sn;
     int n, chunk;
 };
 

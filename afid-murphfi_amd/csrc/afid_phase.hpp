@@ -68,13 +68,11 @@ __global__ void __launch_bounds__(256) k_column_phi(KP p, TriFactors f, RT *phi_
     using V = typename Pair<RT>::T;
     constexpr int M2 = (64 * CH + 127) / 128;
     __shared__ double lds[4][lds_col_size<CH>()];
-    // the solver's factors, read by every column of the workgroup
-    __shared__ double fac[4][64 * CH];
+    // the solver's folded factors, read by every column of the workgroup
+    __shared__ double fac[2][64 * CH];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
-    for (int q = threadIdx.x; q < 64 * CH; q += 256) { fac[0][q] = f.l[q]; fac[1][q] = f.e[q]; fac[2][q] = f.g[q]; fac[3][q] = f.s[q]; }
-    TriFactors fl = f;
-    fl.l = fac[0]; fl.e = fac[1]; fl.g = fac[2]; fl.s = fac[3];
+    for (int q = threadIdx.x; q < 64 * CH; q += 256) { fac[0][q] = f.ln[q]; fac[1][q] = f.e[q]; }
     int j, i;
     const bool live = strip_column(xcd_block(0), wave, p.nyr, p.nzr, j, i);
     double *buf = lds[wave];
@@ -111,11 +109,11 @@ __global__ void __launch_bounds__(256) k_column_phi(KP p, TriFactors f, RT *phi_
                 ru[r0] = h0;
             }
         }
-        if (r0 < 64u * CH) buf[lds_row<CH>(r0)] = rhs0;
-        if (r1 < 64u * CH) buf[lds_row<CH>(r1)] = rhs1;
+        if (r0 < 64u * CH) buf[lds_row<CH>(r0)] = rhs0 * f.sn[r0];
+        if (r1 < 64u * CH) buf[lds_row<CH>(r1)] = rhs1 * f.sn[r1];
     }
     __syncthreads();
-    column_solve_lds<CH>(buf, fl);
+    column_solve_lds_folded<CH>(buf, fac[0], fac[1]);
     __syncthreads();
     if (!live) return;
     RT *out = phi_out + c0, *dphi = sel<RT>(p.rhsr, p.rhsrf) + c0;
