@@ -117,6 +117,7 @@ struct afid_ctx {
     double *d_phi_part = nullptr, *d_phi_out = nullptr;
     int mean_groups = 0, sal_groups = 0, phi_groups = 0;
     bool col_four_pass = false;         // base columns through the four-pass kernel (AFID_COLUMN_FOUR_PASS=1)
+    bool vyz_tile = false;              // vyr / vzr through the LDS tile kernel (AFID_VYZ_TILE=1)
     // mixed form, gAxis = 1: the FP32 vx explicit term and its history hold
     // the buoyancy minus a per-row FP64 reference bref (that of the
     // conductive profile between the mean wall temperatures), which the FP64
@@ -914,6 +915,7 @@ static int setup(afid_ctx *c) {
         return m;
     };
     c->vxr_xc = span(sx.src, nxmr, 64);
+    if (c->vxr_xc > 128) return set_err(c, AFID_E_ARG, "refined x coarser than the base x is not supported");
     c->vxr_cj = span(sy.src, nyr, VXR_TJ);
     c->vxr_ci = span(sz.src, nzr, VXR_TI);
     if (c->vxr_cj * c->vxr_ci > 256) return set_err(c, AFID_E_ARG, "refinement ratio too small for the vxr tiles");
@@ -969,6 +971,7 @@ static int ctx_create(const afid_config *cfg, hipStream_t shared, afid_ctx **out
     }
     c->device = dev;
     if (const char *e = std::getenv("AFID_COLUMN_FOUR_PASS")) c->col_four_pass = std::atoi(e) != 0;
+    if (const char *e = std::getenv("AFID_VYZ_TILE")) c->vyz_tile = std::atoi(e) != 0;
     if (shared) {
         c->stream = shared;
         c->own_stream = false;
@@ -1404,6 +1407,13 @@ static int poisson_mixed(afid_ctx *c) {
     return AFID_OK;
 }
 
+// vyr / vzr, one wavefront per refined line (k_vyz_wave)
+template <int AX, bool TWO>
+static void launch_vyz_wave(afid_ctx *c, const KP &p, dim3 grid, int nxc) {
+    if (c->mixed) hipLaunchKernelGGL((k_vyz_wave<AX, float, TWO>), grid, dim3(256), 0, c->stream, p, nxc);
+    else hipLaunchKernelGGL((k_vyz_wave<AX, double, TWO>), grid, dim3(256), 0, c->stream, p, nxc);
+}
+
 extern "C" {
 
 int afid_solve_pressure_correction(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return poisson(c, 1); }
@@ -1422,16 +1432,35 @@ static int interp_vel(afid_ctx *c) {
     else hipLaunchKernelGGL(k_vxr_tile<double>, gx, dim3(256), c->lds_vxr, c->stream, p, c->vxr_xc, c->vxr_cj, c->vxr_ci);
     tstop(c);
     if ((rc = check_launch(c, "k_vxr_tile"))) return rc;
+    if (c->vyz_tile) {
+        tstart(c, "mgrd_vyr");
+        if (c->mixed) hipLaunchKernelGGL((k_vyz_tile<1, float>), gy, dim3(64 * VYZ_LINES), c->lds_vy, c->stream, p, c->vxr_xc, c->vy_cs, c->vy_co);
+        else hipLaunchKernelGGL((k_vyz_tile<1, double>), gy, dim3(64 * VYZ_LINES), c->lds_vy, c->stream, p, c->vxr_xc, c->vy_cs, c->vy_co);
+        tstop(c);
+        if ((rc = check_launch(c, "k_vyz_tile<1>"))) return rc;
+        tstart(c, "mgrd_vzr");
+        if (c->mixed) hipLaunchKernelGGL((k_vyz_tile<2, float>), gz, dim3(64 * VYZ_LINES), c->lds_vz, c->stream, p, c->vxr_xc, c->vz_cs, c->vz_co);
+        else hipLaunchKernelGGL((k_vyz_tile<2, double>), gz, dim3(64 * VYZ_LINES), c->lds_vz, c->stream, p, c->vxr_xc, c->vz_cs, c->vz_co);
+        tstop(c);
+        return check_launch(c, "k_vyz_tile<2>");
+    }
+    // one wavefront per refined line: (x chunks) x (groups of 4 lines)
+    // workgroups, padded to a multiple of 8 for the XCD map
+    // (a chunk of 64 refined x spanning more than 64 coarse points, refinement
+    // ratio below 1.05 in x, takes the second register per lane)
+    const int nxc = (c->nxmr + 63) / 64;
+    const dim3 wy(unsigned(((long long)nxc * ((c->nzr + 3) / 4) + 7) / 8 * 8));
+    const dim3 wz(unsigned(((long long)nxc * ((c->nyr + 3) / 4) + 7) / 8 * 8));
     tstart(c, "mgrd_vyr");
-    if (c->mixed) hipLaunchKernelGGL((k_vyz_tile<1, float>), gy, dim3(64 * VYZ_LINES), c->lds_vy, c->stream, p, c->vxr_xc, c->vy_cs, c->vy_co);
-    else hipLaunchKernelGGL((k_vyz_tile<1, double>), gy, dim3(64 * VYZ_LINES), c->lds_vy, c->stream, p, c->vxr_xc, c->vy_cs, c->vy_co);
+    if (c->vxr_xc > 64) launch_vyz_wave<1, true>(c, p, wy, nxc);
+    else launch_vyz_wave<1, false>(c, p, wy, nxc);
     tstop(c);
-    if ((rc = check_launch(c, "k_vyz_tile<1>"))) return rc;
+    if ((rc = check_launch(c, "k_vyz_wave<1>"))) return rc;
     tstart(c, "mgrd_vzr");
-    if (c->mixed) hipLaunchKernelGGL((k_vyz_tile<2, float>), gz, dim3(64 * VYZ_LINES), c->lds_vz, c->stream, p, c->vxr_xc, c->vz_cs, c->vz_co);
-    else hipLaunchKernelGGL((k_vyz_tile<2, double>), gz, dim3(64 * VYZ_LINES), c->lds_vz, c->stream, p, c->vxr_xc, c->vz_cs, c->vz_co);
+    if (c->vxr_xc > 64) launch_vyz_wave<2, true>(c, p, wz, nxc);
+    else launch_vyz_wave<2, false>(c, p, wz, nxc);
     tstop(c);
-    return check_launch(c, "k_vyz_tile<2>");
+    return check_launch(c, "k_vyz_wave<2>");
 }
 int afid_interp_vel_mgrd(afid_ctx *c) { NEED_SAL(c) return interp_vel(c); }
 
@@ -1515,18 +1544,20 @@ static void launch_column_base(afid_ctx *c, const KP &p) {
     else if (c->pf) launch_column_base_form<CH, double, true>(c, p);
     else launch_column_base_form<CH, double, false>(c, p);
 }
-// single-pass kernel (no phase field); CH here is the chunk of the nxm-row
-// systems
-template <int CH>
-static void launch_column_base1(afid_ctx *c, const KP &p) {
+// single-pass kernel; CH here is the chunk of the nxm-row systems
+template <int CH, typename HT, bool PF>
+static void launch_column_base1_form(afid_ctx *c, const KP &p) {
     BaseOut o{c->alt[0], c->alt[1], c->alt[2], c->alt[3]};
     const dim3 gr(unsigned((strip_blocks(c->ny, c->nz) + 7) / 8 * 8));
-    if (c->mixed)
-        hipLaunchKernelGGL((k_column_base1<CH, float>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ),
-                           factors(c, MAT_T), o);
-    else
-        hipLaunchKernelGGL((k_column_base1<CH, double>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ),
-                           factors(c, MAT_T), o);
+    hipLaunchKernelGGL((k_column_base1<CH, HT, PF>), gr, dim3(256), 0, c->stream, p, factors(c, MAT_X), factors(c, MAT_YZ),
+                       factors(c, MAT_T), o);
+}
+template <int CH>
+static void launch_column_base1(afid_ctx *c, const KP &p) {
+    if (c->mixed && c->pf) launch_column_base1_form<CH, float, true>(c, p);
+    else if (c->mixed) launch_column_base1_form<CH, float, false>(c, p);
+    else if (c->pf) launch_column_base1_form<CH, double, true>(c, p);
+    else launch_column_base1_form<CH, double, false>(c, p);
 }
 // ExplicitHumidity + ImplicitHumidity + SolveImpEqnUpdate_Humidity of every
 // column into humid_alt; the caller swaps the buffers once the base column
@@ -1552,8 +1583,8 @@ static int column_humid(afid_ctx *c) {
 static int column_base(afid_ctx *c) {
     KP p = make_kp(c);
     // single pass up to 512 wall-normal cells (its registers hold two
-    // right-hand sides of CH rows), four passes with the phase field
-    if (!c->pf && !c->col_four_pass && c->chunk[MAT_YZ] <= 8) {
+    // right-hand sides of CH rows), four passes above
+    if (!c->col_four_pass && c->chunk[MAT_YZ] <= 8) {
         switch (c->chunk[MAT_YZ]) {
 #define AFID_CB1_CASE(CH) case CH: launch_column_base1<CH>(c, p); break;
             AFID_CB1_CASE(1) AFID_CB1_CASE(2) AFID_CB1_CASE(3) AFID_CB1_CASE(4) AFID_CB1_CASE(5) AFID_CB1_CASE(6)

@@ -657,8 +657,9 @@ __global__ void __launch_bounds__(256) k_column_base(KP p, TriFactors fx, TriFac
 // and the explicit terms are stored to their history arrays after all
 // loads.  Rows without an equation (vx rows 0 and nxm, the unused index
 // nxm of the others) give 0.
-template <typename HT>
-__device__ __forceinline__ void base_rhs4(const KP &p, const Nb &n, long long c0, unsigned r, double (&rhs)[4]) {
+template <typename HT, bool PF = false>
+__device__ __forceinline__ void base_rhs4(const KP &p, const Nb &n, long long c0, unsigned r, double (&rhs)[4],
+                                          unsigned &solid) {
     const unsigned nxm = p.nxm;
     HT *rux = sel<HT>(p.rux, p.ruxf) + c0, *ruy = sel<HT>(p.ruy, p.ruyf) + c0;
     HT *ruz = sel<HT>(p.ruz, p.ruzf) + c0, *rut = sel<HT>(p.rut, p.rutf) + c0;
@@ -666,6 +667,9 @@ __device__ __forceinline__ void base_rhs4(const KP &p, const Nb &n, long long c0
     const HT sb = (p.active_s == 1 && wc) ? HT(p.bycs) * sel<HT>(p.salc, p.salcf)[c0 + r] : HT(0);
     HT h0 = HT(0), h1 = HT(0), h2 = HT(0), h3 = HT(0);
     double bref = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    // phase field: solid rows of vx, vy, vz (bits 0..2); vx rows 0 and nxm
+    // are identity rows (SolveImpEqnUpdate_X_pf)
+    bool sx = PF && !wx, sy = false, sz = false;
     if (wx) {
         if (sizeof(HT) == sizeof(float) && p.bref) {
             // FP32 terms: buoyancy as its deviation from the row's FP64
@@ -676,6 +680,12 @@ __device__ __forceinline__ void base_rhs4(const KP &p, const Nb &n, long long c0
             h0 = ex_vx<HT>(p, n, r);
         }
         if (p.gaxis == 1) h0 -= sb;
+        if (PF) {
+            // AddVolumePenalty (phasefield.F90:525-566); the row is solid where
+            // the averaged phic exceeds pf_direct_force = 0.9 (:29, :821-930)
+            h0 -= HT((n.pc[r] + n.pc[r - 1]) * n.vx[r] * p.heta);
+            sx = 0.5 * (n.pc[r] + n.pc[r + 1]) > 0.9;
+        }
         r0 = double(rux[r]);
     }
     if (wc) {
@@ -684,6 +694,14 @@ __device__ __forceinline__ void base_rhs4(const KP &p, const Nb &n, long long c0
         h2 = ex_vz<HT>(p, n, r);
         if (p.gaxis == 3) h2 -= sb;
         h3 = ex_temp<HT>(p, n, r);
+        if (PF) {
+            const double py = n.pc[r] + n.pc_jm[r], pz = n.pc[r] + n.pc_im[r];
+            h1 -= HT(py * n.vy[r] * p.heta);
+            h2 -= HT(pz * n.vz[r] * p.heta);
+            sy = 0.5 * py > 0.9;
+            sz = 0.5 * pz > 0.9;
+            h3 += HT(p.qlat[c0 + r]);            // AddLatentHeat (:651-725)
+        }
         if (p.moist) h3 = HT(double(h3) + condensation_heat(p, c0, r));
         r1 = double(ruy[r]); r2 = double(ruz[r]); r3 = double(rut[r]);
     }
@@ -691,26 +709,58 @@ __device__ __forceinline__ void base_rhs4(const KP &p, const Nb &n, long long c0
     rhs[1] = wc ? rhs_vy(p, n, double(h1), r1, r) : 0.0;
     rhs[2] = wc ? rhs_vz(p, n, double(h2), r2, r) : 0.0;
     rhs[3] = wc ? rhs_temp(p, n, double(h3), r3, r) : 0.0;
+    if (PF) {
+        // a solid row is the identity with right-hand side -u
+        if (sx) rhs[0] = -n.vx[r];
+        if (sy) rhs[1] = -n.vy[r];
+        if (sz) rhs[2] = -n.vz[r];
+    }
+    solid = unsigned(sx) | unsigned(sy) << 1 | unsigned(sz) << 2;
     if (wx) rux[r] = h0;
     if (wc) { ruy[r] = h1; ruz[r] = h2; rut[r] = h3; }
 }
 
-// Single-pass form of k_column_base (no phase field).  One sweep over the
-// rows forms the four right-hand sides, loading every neighbour value once
-// for all four explicit terms.  Those of vx and vy go to two LDS columns of
-// the wavefront, those of vz and T stay in registers.  The vx and vy solves
-// and updates follow, then the vz and T right-hand sides move into the same
-// two LDS columns for their solves and updates.  Two LDS columns per
-// wavefront instead of four keep 4 wavefronts per SIMD; the updates read the
-// old value of the own column again (from cache).
+// Masked solve of one column in LDS (tight layout CHP) with the solid flags
+// of its rows as bytes in natural row order (mskb[r] != 0: identity row,
+// right-hand side already in buf); the other rows are those of f
+template <int CH, int CHP>
+__device__ __forceinline__ void column_solve_masked_bytes(double *buf, const unsigned char *mskb, const TriFactors &f) {
+    const int lane = threadIdx.x & 63;
+    double v[CH], a[CH], c[CH];
+#pragma unroll
+    for (int m = 0; m < CH; ++m) {
+        const int r = lane * CH + m;
+        const bool solid = mskb[r] != 0;
+        v[m] = buf[lane * CHP + m];
+        if (!solid) v[m] *= f.s[r];
+        a[m] = solid ? 0.0 : f.a[r];
+        c[m] = solid ? 0.0 : f.c[r];
+    }
+    wave_tri_solve_general<CH>(v, a, c);
+#pragma unroll
+    for (int m = 0; m < CH; ++m) buf[lane * CHP + m] = v[m];
+}
+
+// Single-pass form of k_column_base.  One sweep over the rows forms the
+// four right-hand sides, loading every neighbour value once for all four
+// explicit terms.  Those of vx and vy go to two LDS columns of the
+// wavefront, those of vz and T stay in registers.  The vx and vy solves and
+// updates follow, then the vz and T right-hand sides move into the same two
+// LDS columns for their solves and updates.  Two LDS columns per wavefront
+// instead of four keep 4 wavefronts per SIMD; the updates read the old value
+// of the own column again (from cache).
+// With PF the velocity terms carry the volume penalty, T the latent heat,
+// and solid velocity rows are identity rows: their flags go to LDS as bytes
+// (vx, vy; vz's flags wait in a register bit mask) for the masked solves.
 // CH = ceil(nxm / 64) is the size of the largest system (nxm rows); when
 // nx = 64 CH + 1 the last face has no equation (the wall of vx, the unused
 // index of the others) and only the update touches it.
-template <int CH, typename HT>
+template <int CH, typename HT, bool PF>
 __global__ void __launch_bounds__(256) k_column_base1(KP p, TriFactors fx, TriFactors fyz, TriFactors ft, BaseOut o) {
     constexpr int CHP = lds_chunk_tight<CH>();
     constexpr unsigned NROW = 64u * CH;
     __shared__ double lds[4][2][lds_col_size<CH, CHP>()];
+    __shared__ unsigned char mskb[PF ? 4 : 1][2][PF ? NROW : 1];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned lane = threadIdx.x & 63;
     int j, i;
@@ -719,17 +769,25 @@ __global__ void __launch_bounds__(256) k_column_base1(KP p, TriFactors fx, TriFa
     const long long c0 = (long long)i * p.plane + (long long)j * p.ldx;
     const unsigned nx = p.nx, nxm = p.nxm;
     double *b0 = lds[wave][0], *b1 = lds[wave][1];
+    unsigned char *m0 = mskb[PF ? wave : 0][0], *m1 = mskb[PF ? wave : 0][1];
     double keep[2][CH];                          // right-hand sides of vz and T
+    unsigned zsolid = 0;                         // vz's solid flags, bit m for row lane + 64 m
 #pragma unroll
     for (int m = 0; m < CH; ++m) {
         const unsigned r = lane + 64u * m;
         double rhs[4] = {0.0, 0.0, 0.0, 0.0};
-        if (live && r < nxm) base_rhs4<HT>(p, n, c0, r, rhs);
+        unsigned solid = 0;
+        if (live && r < nxm) base_rhs4<HT, PF>(p, n, c0, r, rhs, solid);
         const unsigned q = lds_row<CH, CHP>(r);
         b0[q] = rhs[0];
         b1[q] = rhs[1];
         keep[0][m] = rhs[2];
         keep[1][m] = rhs[3];
+        if (PF) {
+            m0[r] = (unsigned char)(solid & 1u);
+            m1[r] = (unsigned char)((solid >> 1) & 1u);
+            zsolid |= ((solid >> 2) & 1u) << m;
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -740,11 +798,18 @@ __global__ void __launch_bounds__(256) k_column_base1(KP p, TriFactors fx, TriFa
                 const unsigned q = lds_row<CH, CHP>(lane + 64u * m);
                 b0[q] = keep[0][m];
                 b1[q] = keep[1][m];
+                if (PF) m0[lane + 64u * m] = (unsigned char)((zsolid >> m) & 1u);
             }
             __syncthreads();
         }
-        column_solve_lds<CH, CHP>(b0, g == 0 ? fx : fyz);
-        column_solve_lds<CH, CHP>(b1, g == 0 ? fyz : ft);
+        if (PF) {
+            column_solve_masked_bytes<CH, CHP>(b0, m0, g == 0 ? fx : fyz);
+            if (g == 0) column_solve_masked_bytes<CH, CHP>(b1, m1, fyz);
+            else column_solve_lds<CH, CHP>(b1, ft);
+        } else {
+            column_solve_lds<CH, CHP>(b0, g == 0 ? fx : fyz);
+            column_solve_lds<CH, CHP>(b1, g == 0 ? fyz : ft);
+        }
         __syncthreads();
         if (live) {
 #pragma unroll
@@ -951,6 +1016,7 @@ constexpr int VXR_TJ = 16, VXR_TI = 4;      // refined columns per vxr tile
 constexpr int VYZ_LINES = 4;                // refined lines (waves) per vyr/vzr tile
 constexpr int VYZ_CHUNK = 16;               // refined points per scan chunk
 constexpr int VXR_BU = 4, VYZ_BU = 4;       // coarse-block items loaded per batch and thread
+constexpr int VYZ_NB = 4;                   // coarse planes per load batch of k_vyz_wave (8: slower)
 
 // Inclusive prefix sum over the 64 lanes of a wavefront with DPP lane
 // moves: Kogge-Stone within each row of 16 lanes (row_shr 1, 2, 4, 8),
@@ -1263,6 +1329,166 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
                 v = v + tv * dl;
                 outp[obase + (s + 1) * sstride + kr] = v;
             }
+    }
+}
+
+// vyr (AX = 1) / vzr (AX = 2) with one wavefront per refined line of the
+// other periodic axis and no LDS: lanes = 64 refined x, the scan axis walked
+// point by point (InterpVelMgrd.F90:68-237).  Per coarse scan index cs the
+// wavefront forms, for each coarse x of its chunk (one per lane; a second
+// register when the chunk spans more than 64 coarse points), P(xc) = sum_c
+// wo_c G(xc, cs, o_c), the coarse gradient G with the wall ghosts contracted
+// over the other axis; then X(kr) = sum_a wx_a P(xa + a) by lane shuffles.
+// The last four X stay in registers for the scan-axis stencil of each
+// refined point, and the velocity row of plane cs + 1 is carried to the
+// next step, so a step loads one row per o_c.  The arithmetic is that of
+// k_vyz_tile term by term.  Workgroups of 4 wavefronts take 4 consecutive
+// lines of one x chunk; consecutive workgroups on an XCD take consecutive
+// line groups (shared coarse rows in its L2).
+template <int AX, typename RT, bool TWO>
+__global__ void __launch_bounds__(256) k_vyz_wave(KP p, int nxchunk) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nS = AX == 1 ? p.nyr : p.nzr, nO = AX == 1 ? p.nzr : p.nyr;          // refined extents
+    const int cS = AX == 1 ? p.ny : p.nz, cO = AX == 1 ? p.nz : p.ny;              // coarse extents
+    const int *sS = AX == 1 ? p.ys_src : p.zs_src, *sO = AX == 1 ? p.zs_src : p.ys_src;
+    const RT *ysw = sel<RT>(p.ys_w, p.ys_wf), *zsw = sel<RT>(p.zs_w, p.zs_wf), *xs_w = sel<RT>(p.xs_w, p.xs_wf);
+    const RT *wS = AX == 1 ? ysw : zsw, *wO = AX == 1 ? zsw : ysw;
+    const int ngroups = (nO + 3) / 4;
+    const long long blk = xcd_block(0);
+    if (blk >= (long long)nxchunk * ngroups) return;
+    const int xchunk = int(blk / ngroups), line = int(blk % ngroups) * 4 + wave;
+    if (line >= nO) return;
+    const int R0 = xchunk * 64, R1 = min(R0 + 64, p.nxmr);
+    const int kr = R0 + lane;
+    const bool in = kr < R1;
+    RT *outp = AX == 1 ? sel<RT>(p.vyr, p.vyrf) : sel<RT>(p.vzr, p.vzrf);
+    const long long sstride = AX == 1 ? (long long)p.ldxr : p.planer;
+    const long long obase = AX == 1 ? (long long)line * p.planer : (long long)line * p.ldxr;
+    // the unused x index nxr of vyr / vzr stays zero (InterpVelMgrd.F90:29-31)
+    if (xchunk == nxchunk - 1 && lane == 0)
+        for (int s = 0; s < nS; ++s) outp[obase + s * sstride + p.nxmr] = RT(0);
+    const int x0 = p.xs_src[R0];
+    RT wx[4] = {0, 0, 0, 0};
+    int xa = 0;
+    if (in) { xa = p.xs_src[kr] - x0; for (int a = 0; a < 4; ++a) wx[a] = xs_w[4 * kr + a]; }
+    // the line's stencil over the other axis (wavefront-uniform)
+    const long long ostride = AX == 1 ? p.plane : (long long)p.ldx, sstr = AX == 1 ? (long long)p.ldx : p.plane;
+    const int o0 = sO[line] - 1;
+    RT wo[4];
+    long long ooff[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) { wo[c] = wO[4 * line + c]; ooff[c] = (long long)wrapn(o0 + c, cO) * ostride; }
+    const int sSg = 1 - 2 * p.inslws, sNg = 1 - 2 * p.inslwn;
+    const double *vel = AX == 1 ? p.vy : p.vz;
+    // first guess on the plane s = 0 (InterpVelMgrd.F90:93-123 / :179-209):
+    // velocity plane with wall ghosts, contracted over the other axis, then x
+    RT v = RT(0);
+    if (in && (AX == 1 || p.nz > 1)) {
+        for (int a = 0; a < 4; ++a) {
+            const int kc = x0 + xa + a;
+            RT q1 = RT(0);
+            for (int c = 0; c < 4; ++c) {
+                const double *col = vel + ooff[c];
+                RT f;
+                if (kc >= 1 && kc <= p.nxm) f = RT(col[kc - 1]);
+                else if (kc == 0) f = RT(sSg * col[0]);
+                else if (kc == p.nx) f = RT(sNg * col[p.nxm - 1]);
+                else f = RT(0);
+                q1 = c == 0 ? f * wo[0] : q1 + f * wo[c];
+            }
+            v = a == 0 ? q1 * wx[0] : v + q1 * wx[a];
+        }
+    }
+    if (in) outp[obase + kr] = v;
+    if (nS < 2) return;
+    // this lane's coarse x rows (xc = x0 + lane, and + 64): row, ghost sign
+    // (0 outside -1..nx+1 reads row 0)
+    auto xrow = [&](int xc, int &k, double &sg) {
+        const bool ok = xc >= 0 && xc <= p.nx;
+        k = !ok ? 0 : (xc == 0 ? 0 : (xc == p.nx ? p.nxm - 1 : xc - 1));
+        sg = !ok ? 0.0 : (xc == 0 ? double(sSg) : (xc == p.nx ? double(sNg) : 1.0));
+    };
+    int k0, k1;
+    double g0, g1;
+    xrow(x0 + lane, k0, g0);
+    xrow(x0 + 64 + lane, k1, g1);
+    const double dgrid = AX == 1 ? p.dy : p.dz;
+    const RT dl = AX == 1 ? RT(1.0 / p.dyr) : RT(1.0 / p.dzr);
+    int cs = sS[0] - 1;                       // unwrapped coarse scan index of the next X
+    double va[4], vb[4];                      // velocity rows of plane cs (second register: vb)
+    {
+        const long long pl = (long long)wrapf(cs, cS) * sstr;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { va[c] = vel[ooff[c] + pl + k0]; vb[c] = TWO ? vel[ooff[c] + pl + k1] : 0.0; }
+    }
+    // X at the VYZ_NB coarse indices cs .. cs + VYZ_NB - 1 into W[3 ..]; the
+    // rows of all VYZ_NB planes are loaded before any is used, so a wavefront
+    // keeps 4 VYZ_NB loads in flight instead of waiting on each plane
+    RT W[VYZ_NB + 3];
+    auto batch = [&]() {
+        double na[VYZ_NB][4], nb[VYZ_NB][4];
+#pragma unroll
+        for (int q = 0; q < VYZ_NB; ++q) {
+            const long long pn = (long long)wrapf(cs + 1 + q, cS) * sstr;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                na[q][c] = vel[ooff[c] + pn + k0];
+                nb[q][c] = TWO ? vel[ooff[c] + pn + k1] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < VYZ_NB; ++q) {
+            RT b[4], b2[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double p0 = q == 0 ? va[c] : na[q - 1][c];
+                b[c] = RT(g0 * ((na[q][c] - p0) * dgrid));
+                if (TWO) {
+                    const double p1 = q == 0 ? vb[c] : nb[q - 1][c];
+                    b2[c] = RT(g1 * ((nb[q][c] - p1) * dgrid));
+                } else {
+                    b2[c] = RT(0);
+                }
+            }
+            const RT P = b[0] * wo[0] + b[1] * wo[1] + b[2] * wo[2] + b[3] * wo[3];
+            const RT P2 = b2[0] * wo[0] + b2[1] * wo[1] + b2[2] * wo[2] + b2[3] * wo[3];
+            RT o[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int idx = xa + a;
+                o[a] = __shfl(P, idx & 63, 64);
+                if (TWO) {
+                    const RT o2 = __shfl(P2, idx & 63, 64);
+                    if (idx >= 64) o[a] = o2;
+                }
+            }
+            W[3 + q] = o[0] * wx[0] + o[1] * wx[1] + o[2] * wx[2] + o[3] * wx[3];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { va[c] = na[VYZ_NB - 1][c]; vb[c] = nb[VYZ_NB - 1][c]; }
+        cs += VYZ_NB;
+    };
+    // W[0 .. nv-1] hold X at coarse indices lo .. lo + nv - 1; the window of
+    // the refined point s is W[0 .. 3] once lo has reached its first source
+    batch();
+#pragma unroll
+    for (int q = 0; q < 3; ++q) W[q] = W[3 + q];
+#pragma unroll
+    for (int q = 3; q < VYZ_NB; ++q) W[q] = W[3 + q];
+    int lo = sS[0] - 1, nv = VYZ_NB;
+#pragma unroll 1
+    for (int s = 0; s < nS - 1; ++s) {
+        const int b0 = sS[s] - 1;
+        while (lo < b0) {
+#pragma unroll
+            for (int q = 0; q < VYZ_NB + 2; ++q) W[q] = W[q + 1];
+            ++lo;
+            if (--nv == 3) { batch(); nv += VYZ_NB; }
+        }
+        const RT *w = wS + 4 * s;
+        const RT tv = W[0] * w[0] + W[1] * w[1] + W[2] * w[2] + W[3] * w[3];
+        v = v + tv * dl;
+        if (in) outp[obase + (s + 1) * sstride + kr] = v;
     }
 }
 

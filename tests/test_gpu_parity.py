@@ -181,6 +181,38 @@ def test_interp_vel_mgrd(cfg):
         cmp(o, g, f, tol=1e-12)
 
 
+# refined x barely finer than the base (84 over 80 cells): a chunk of 64
+# refined x points spans more than 64 coarse points, so vyr / vzr take the
+# second per-lane register of their wavefront kernel.  In FP64 the vxr tile
+# of such a grid exceeds 64 KiB of LDS and set-up refuses it; the mixed form
+# (FP32 transfers) runs it.
+MR_FLATX = dict(MR, nxm=80, nym=8, nzm=12, nxmr=84, nymr=16, nzmr=24)
+
+
+def test_interp_vel_mgrd_refined_x_near_base(monkeypatch):
+    """FP32 transfers of FP64 velocities against the FP64 oracle: relative
+    max-norm error <= 1e-5 (FP32 rounding of a 4 x 4 x 4 contraction and a
+    16-step running sum).  The LDS tile form of vyr / vzr (AFID_VYZ_TILE=1)
+    forms the same sums in the same order: within 1e-6."""
+    st = random_state(MR_FLATX, 5)
+    o = pyoracle.Oracle(MR_FLATX)
+    load_state(o, st, upload=False)
+    o.call("interp_vel_mgrd")
+    got = {}
+    for tile in ("0", "1"):
+        monkeypatch.setenv("AFID_VYZ_TILE", tile)
+        g = pyafid.Solver(dict(MR_FLATX, precision=pyafid.MIXED))
+        load_state(g, st, upload=True)
+        g.call("interp_vel_mgrd")
+        got[tile] = {f: g.download(f) for f in ("vxr", "vyr", "vzr")}
+        g.close()
+    for f in ("vxr", "vyr", "vzr"):
+        e = relerr(got["0"][f], o.field(f))
+        assert e <= 1e-5, (f, e)
+        assert relerr(got["0"][f], got["1"][f]) <= 1e-6, f
+    o.close()
+
+
 def test_interp_vel_mgrd_refined_divergence():
     """InterpVelMgrd integrates transferred gradients, so a discretely
     divergence-free coarse velocity (projected on the GPU) gives a refined
