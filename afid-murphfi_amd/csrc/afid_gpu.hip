@@ -155,6 +155,8 @@ struct afid_ctx {
     FftPlan<double> plan_y, plan_z;
     FftPlan<float> plan_yf, plan_zf;
     int Ly = 1, Lz = 1, lds_y = 0, lds_z = 0;
+    int fftz_maxv = FFT_MAXV;           // values per thread and stage of k_fft_z
+    bool fft_z16 = true;                // FP32 z transform may take 16 values per thread (AFID_FFT_Z16=0: no)
     PoissonFactors<double> pois{};
     PoissonFactors<float> pff{};
     int pchunk = 1;
@@ -445,9 +447,9 @@ static int build_plan(afid_ctx *c, int n, FftPlan<R> &pl) {
 // values within the LDS budget and the per-thread register budget
 // (L*N <= FFT_MAXV*FFT_THREADS), at most cap
 template <typename R>
-static int pick_lines(int N, int maxlds, int cap) {
+static int pick_lines(int N, int maxlds, int cap, int maxv = FFT_MAXV) {
     int L = 1;
-    while (2 * L * line_stride<R>(N) * int(sizeof(cpx<R>)) <= maxlds && 2 * L * N <= FFT_MAXV * FFT_THREADS && 2 * L <= cap)
+    while (2 * L * line_stride<R>(N) * int(sizeof(cpx<R>)) <= maxlds && 2 * L * N <= maxv * FFT_THREADS && 2 * L <= cap)
         L *= 2;
     return L;
 }
@@ -647,7 +649,13 @@ static int setup(afid_ctx *c) {
     // values = 128 bytes per row); z: Lz x-adjacent lines (128-byte rows)
     if (c->mixed) {
         c->Ly = pick_lines<float>(c->ny, maxlds, 8);
+        // 16 values per thread and stage (k_fft_z<float, 16>) when 8 would
+        // leave rows shorter than 16 lines (128 bytes)
         c->Lz = pick_lines<float>(c->nz, maxlds, 16);
+        if (c->Lz < 16 && c->fft_z16) {
+            c->Lz = pick_lines<float>(c->nz, maxlds, 16, 16);
+            c->fftz_maxv = 16;
+        }
         c->lds_y = c->Ly * line_stride<float>(c->ny) * csize;
         c->lds_z = c->Lz * line_stride<float>(c->nz) * csize;
     } else {
@@ -665,6 +673,7 @@ static int setup(afid_ctx *c) {
     if (c->lds_z > 65536) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_fft_z<double>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_z);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_fft_z<float>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_z);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_fft_z<float, 16>), hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_z);
     }
     if (c->lds_y > maxlds || c->lds_z > maxlds || c->ny > FFT_MAXV * FFT_THREADS || c->nz > FFT_MAXV * FFT_THREADS)
         return set_err(c, AFID_E_ARG, "periodic size too large for one workgroup transform");
@@ -972,6 +981,7 @@ static int ctx_create(const afid_config *cfg, hipStream_t shared, afid_ctx **out
     c->device = dev;
     if (const char *e = std::getenv("AFID_COLUMN_FOUR_PASS")) c->col_four_pass = std::atoi(e) != 0;
     if (const char *e = std::getenv("AFID_VYZ_TILE")) c->vyz_tile = std::atoi(e) != 0;
+    if (const char *e = std::getenv("AFID_FFT_Z16")) c->fft_z16 = std::atoi(e) != 0;
     if (shared) {
         c->stream = shared;
         c->own_stream = false;
@@ -1310,6 +1320,19 @@ int afid_update_saturation(afid_ctx *c) {
 // One Poisson sweep: forward transforms of the source, per-mode tridiagonals,
 // inverse transforms into dph.  fwd_gate gates the forward y transform, gate
 // the rest; slot_out receives max|div u| of a velocity source.
+// z transform of the spectrum (k_fft_z; the FP32 form may take 16 values per
+// thread and stage, see setup)
+template <typename R>
+static void launch_fft_z(afid_ctx *c, const KP &p, const FftPlan<R> &pz, cpx<R> *spec, int sign, Gate gate) {
+    const dim3 gr(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh));
+    if constexpr (sizeof(R) == sizeof(float)) {
+        if (c->fftz_maxv == 16) {
+            hipLaunchKernelGGL((k_fft_z<R, 16>), gr, dim3(FFT_THREADS), c->lds_z, c->stream, p, pz, spec, c->Lz, sign, gate);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_fft_z<R>), gr, dim3(FFT_THREADS), c->lds_z, c->stream, p, pz, spec, c->Lz, sign, gate);
+}
 template <typename R>
 static int poisson_sweep(afid_ctx *c, const FftPlan<R> &py, const FftPlan<R> &pz, const PoissonFactors<R> &pf, int src,
                          Gate fwd_gate, Gate gate, unsigned long long *slot_out, unsigned long long *clear) {
@@ -1323,8 +1346,7 @@ static int poisson_sweep(afid_ctx *c, const FftPlan<R> &py, const FftPlan<R> &pz
     int rc = check_launch(c, "k_div_fft_y");
     if (rc) return rc;
     tstart(c, "poisson_fft_z");
-    hipLaunchKernelGGL(k_fft_z<R>, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(FFT_THREADS), c->lds_z,
-                       c->stream, p, pz, spec, c->Lz, -1, gate);
+    launch_fft_z<R>(c, p, pz, spec, -1, gate);
     tstop(c);
     if ((rc = check_launch(c, "k_fft_z"))) return rc;
     tstart(c, "poisson_tridiag");
@@ -1338,8 +1360,7 @@ static int poisson_sweep(afid_ctx *c, const FftPlan<R> &py, const FftPlan<R> &pz
     tstop(c);
     if ((rc = check_launch(c, "k_poisson_tri"))) return rc;
     tstart(c, "poisson_fft_z");
-    hipLaunchKernelGGL(k_fft_z<R>, dim3(unsigned((c->nxm + c->Lz - 1) / c->Lz), unsigned(c->nymh)), dim3(FFT_THREADS), c->lds_z,
-                       c->stream, p, pz, spec, c->Lz, +1, gate);
+    launch_fft_z<R>(c, p, pz, spec, +1, gate);
     tstop(c);
     if ((rc = check_launch(c, "k_fft_z"))) return rc;
     tstart(c, "poisson_inv_y");

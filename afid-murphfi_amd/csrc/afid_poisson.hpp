@@ -230,7 +230,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_div_fft_y(KP p, FftPlan<R> pl, 
 
 // Complex transform along z (pressure.F90:176 forward, :189 backward), in
 // place on the spectral array.  Block: L x-adjacent lines at one y-mode m.
-template <typename R>
+// MAXV: complex values per thread and stage (L*N <= MAXV*FFT_THREADS); 16
+// lets the FP32 form take 16 x-adjacent lines (128-byte rows) at N = 768.
+template <typename R, int MAXV = FFT_MAXV>
 __global__ void __launch_bounds__(FFT_THREADS) k_fft_z(KP p, FftPlan<R> pl, cpx<R> *spec, int L, int sign, Gate gate) {
     if (gate_closed(gate)) return;
     extern __shared__ __align__(16) unsigned char smraw[];
@@ -244,7 +246,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fft_z(KP p, FftPlan<R> pl, cpx<
         lb[xo * S + lpad<R>(i)] = k < p.nxm ? g[i * sz + k] : mk<R>(R(0), R(0));
     }
     __syncthreads();
-    lds_fft<R>(lb, L, S, pl, sign);
+    lds_fft<R, FFT_THREADS, MAXV>(lb, L, S, pl, sign);
     for (int t = threadIdx.x; t < L * N; t += FFT_THREADS) {
         const int i = dl.div(t), xo = t - i * L, k = k0 + xo;
         if (k < p.nxm) g[i * sz + k] = lb[xo * S + lpad<R>(i)];
