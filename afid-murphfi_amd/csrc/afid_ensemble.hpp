@@ -271,92 +271,100 @@ __device__ __forceinline__ T ex_temp_2d(const KP &p, const Nb2 &n, unsigned k) {
     return -(htx + htz) + dzzt;
 }
 
-// stencil phase of field FLD (0 vx, 2 vz, 3 T) of a 2-D column: RHS into buf
-// and the old value into own (rows in the solver's LDS layout), new explicit
-// terms into the history (ImplicitAndUpdateVX/VZ/Temp.F90 RHS)
-template <int FLD, int CH, typename HT>
-__device__ __forceinline__ void column_2d_rows(const KP &p, const Nb2 &n, long long c0, double *buf, double *own,
-                                               unsigned lane) {
-    HT *ru = (FLD == 0 ? sel<HT>(p.rux, p.ruxf) : FLD == 2 ? sel<HT>(p.ruz, p.ruzf) : sel<HT>(p.rut, p.rutf)) + c0;
-    const unsigned nx = p.nx, nxm = p.nxm;
-#pragma unroll 1
-    for (unsigned r = lane; r < 64u * CH; r += 64) {
-        double rhs = 0.0, old = 0.0;
-        if (r < nx) {
-            if (FLD == 0) {
-                old = n.vx[r];
-                if (r >= 1 && r < nxm) {
-                    const HT h = ex_vx_2d<HT>(p, n, r);
-                    const double dxx = n.vx[r + 1] * p.ap3ck[r] + n.vx[r] * p.ac3ck[r] + n.vx[r - 1] * p.am3ck[r];
-                    const double dxp = (n.pr[r] - n.pr[r - 1]) * (p.al * p.udx3c[r]);
-                    rhs = (p.ga * double(h) + p.ro * double(ru[r]) + (p.al / p.ren) * dxx - dxp) * p.dt;
-                    ru[r] = h;
-                }
-            } else if (FLD == 2) {
-                old = n.vz[r];
-                if (r < nxm) {
-                    const HT h = ex_vz_2d<HT>(p, n, r);
-                    const unsigned kmm = r == 0 ? 0 : r - 1, kpp = r == nxm - 1 ? r : r + 1;
-                    const double dxx = n.vz[kpp] * p.ap3sk[r] + n.vz[r] * p.ac3sk[r] + n.vz[kmm] * p.am3sk[r];
-                    const double dzp = (n.pr[r] - n.pr_m[r]) * p.dz * p.al;
-                    rhs = (p.ga * double(h) + p.ro * double(ru[r]) + (p.al / p.ren) * dxx - dzp) * p.dt;
-                    ru[r] = h;
-                }
-            } else {
-                old = n.t[r];
-                if (r < nxm) {
-                    const HT h = ex_temp_2d<HT>(p, n, r);
-                    double dxx;
-                    if (r == 0)
-                        dxx = n.t[r + 1] * p.ap3ssk[r] + n.t[r] * p.ac3ssk[r] - (p.ap3ssk[r] + p.ac3ssk[r]) * n.tbp * p.tfixs;
-                    else if (r == nxm - 1)
-                        dxx = n.t[r] * p.ac3ssk[r] + n.t[r - 1] * p.am3ssk[r] - (p.am3ssk[r] + p.ac3ssk[r]) * n.ttp * p.tfixn;
-                    else
-                        dxx = n.t[r + 1] * p.ap3ssk[r] + n.t[r] * p.ac3ssk[r] + n.t[r - 1] * p.am3ssk[r];
-                    rhs = (p.ga * double(h) + p.ro * double(ru[r]) + (p.al / p.pect) * dxx) * p.dt;
-                    ru[r] = h;
-                }
-            }
-        }
-        buf[lds_row<CH>(r)] = rhs;
-        own[lds_row<CH>(r)] = old;
+// Right-hand sides of vx, vz and T at row r of a 2-D column in one pass
+// (ImplicitAndUpdateVX / VZ / Temp with the 2-D explicit terms above; rows
+// without an equation give 0); the new explicit terms go to the histories.
+template <typename HT>
+__device__ __forceinline__ void ens_rhs3(const KP &p, const Nb2 &n, long long c0, unsigned r, double &rx, double &rz,
+                                         double &rt) {
+    const unsigned nxm = p.nxm;
+    rx = 0.0; rz = 0.0; rt = 0.0;
+    if (r >= nxm) return;
+    HT *rux = sel<HT>(p.rux, p.ruxf) + c0, *ruz = sel<HT>(p.ruz, p.ruzf) + c0, *rut = sel<HT>(p.rut, p.rutf) + c0;
+    if (r >= 1) {
+        const HT h = ex_vx_2d<HT>(p, n, r);
+        const double dxx = n.vx[r + 1] * p.ap3ck[r] + n.vx[r] * p.ac3ck[r] + n.vx[r - 1] * p.am3ck[r];
+        const double dxp = (n.pr[r] - n.pr[r - 1]) * (p.al * p.udx3c[r]);
+        rx = (p.ga * double(h) + p.ro * double(rux[r]) + (p.al / p.ren) * dxx - dxp) * p.dt;
+        rux[r] = h;
+    }
+    {
+        const HT h = ex_vz_2d<HT>(p, n, r);
+        const unsigned kmm = r == 0 ? 0 : r - 1, kpp = r == nxm - 1 ? r : r + 1;
+        const double dxx = n.vz[kpp] * p.ap3sk[r] + n.vz[r] * p.ac3sk[r] + n.vz[kmm] * p.am3sk[r];
+        const double dzp = (n.pr[r] - n.pr_m[r]) * p.dz * p.al;
+        rz = (p.ga * double(h) + p.ro * double(ruz[r]) + (p.al / p.ren) * dxx - dzp) * p.dt;
+        ruz[r] = h;
+    }
+    {
+        const HT h = ex_temp_2d<HT>(p, n, r);
+        double dxx;
+        if (r == 0)
+            dxx = n.t[r + 1] * p.ap3ssk[r] + n.t[r] * p.ac3ssk[r] - (p.ap3ssk[r] + p.ac3ssk[r]) * n.tbp * p.tfixs;
+        else if (r == nxm - 1)
+            dxx = n.t[r] * p.ac3ssk[r] + n.t[r - 1] * p.am3ssk[r] - (p.am3ssk[r] + p.ac3ssk[r]) * n.ttp * p.tfixn;
+        else
+            dxx = n.t[r + 1] * p.ap3ssk[r] + n.t[r] * p.ac3ssk[r] + n.t[r - 1] * p.am3ssk[r];
+        rt = (p.ga * double(h) + p.ro * double(rut[r]) + (p.al / p.pect) * dxx) * p.dt;
+        rut[r] = h;
     }
 }
 
-// column i of the run by one wavefront: vx, vz, T in turn, new values into
-// buffer set cur ^ 1; fac holds the three factor sets, buf/own this
-// wavefront's LDS columns
+// column i of the run by one wavefront, new values into buffer set cur ^ 1.
+// One sweep over the rows forms the three right-hand sides (each neighbour
+// value loaded once): vx's and vz's into this wavefront's two LDS columns
+// b0 / b1, T's in registers; the vx and vz solves and updates follow, then
+// T's in b0.  The updates read the old values again (from cache).  fac holds
+// the three factor sets.  Each field's arithmetic is that of a pass over one
+// field at a time, so the two orders give the same results bit for bit.
 template <int CH, typename HT = double>
-__device__ __forceinline__ void ens_column(const EnsRun &R, const KP &p, int cur, int i, const double *fac, double *buf,
-                                           double *own, unsigned lane) {
+__device__ __forceinline__ void ens_column(const EnsRun &R, const KP &p, int cur, int i, const double *fac, double *b0,
+                                           double *b1, unsigned lane) {
     constexpr int NP = 64 * CH;
     const int nx = p.nx, nxm = p.nxm;
     const Nb2 n = make_nb2(p, i);
     const long long c0 = (long long)i * p.plane;
-#pragma unroll 1
-    for (int fi = 0; fi < 3; ++fi) {
-        switch (fi) {
-        case 0: column_2d_rows<0, CH, HT>(p, n, c0, buf, own, lane); break;
-        case 1: column_2d_rows<2, CH, HT>(p, n, c0, buf, own, lane); break;
-        default: column_2d_rows<3, CH, HT>(p, n, c0, buf, own, lane); break;
-        }
-        wave_sync();
+    double keep[CH];
+#pragma unroll
+    for (int m = 0; m < CH; ++m) {
+        const unsigned r = lane + 64u * m;
+        double rx, rz, rt;
+        ens_rhs3<HT>(p, n, c0, r, rx, rz, rt);
+        b0[lds_row<CH>(r)] = rx;
+        b1[lds_row<CH>(r)] = rz;
+        keep[m] = rt;
+    }
+    wave_sync();
+    auto factors_of = [&](int fi) {
         const double *f = fac + fi * 4 * NP;
         TriFactors tf;
         tf.l = f; tf.e = f + NP; tf.g = f + 2 * NP; tf.s = f + 3 * NP; tf.a = nullptr; tf.c = nullptr; tf.ln = nullptr; tf.sn = nullptr;
         tf.n = fi == 0 ? nx : nxm; tf.chunk = CH;
-        column_solve_lds<CH>(buf, tf);
-        wave_sync();
-        double *out = (fi == 0 ? R.vxb[cur ^ 1] : fi == 1 ? R.vzb[cur ^ 1] : R.tb[cur ^ 1]) + c0;
+        return tf;
+    };
+    column_solve_lds<CH>(b0, factors_of(0));
+    column_solve_lds<CH>(b1, factors_of(1));
+    wave_sync();
+    {
+        double *ox = R.vxb[cur ^ 1] + c0, *oz = R.vzb[cur ^ 1] + c0;
         for (unsigned rr = lane; rr < unsigned(nx); rr += 64) {
             const unsigned qq = lds_row<CH>(rr);
-            double v = own[qq];
-            if (fi == 0) v = (rr >= 1 && rr < unsigned(nxm)) ? v + buf[qq] : 0.0;   // walls, ImplicitAndUpdateVX.F90:86-87
-            else if (rr < unsigned(nxm)) v += buf[qq];
-            out[rr] = v;
+            ox[rr] = (rr >= 1 && rr < unsigned(nxm)) ? n.vx[rr] + b0[qq] : 0.0;   // walls, ImplicitAndUpdateVX.F90:86-87
+            oz[rr] = rr < unsigned(nxm) ? n.vz[rr] + b1[qq] : n.vz[rr];
         }
-        wave_sync();
     }
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < CH; ++m) b0[lds_row<CH>(lane + 64u * m)] = keep[m];
+    wave_sync();
+    column_solve_lds<CH>(b0, factors_of(2));
+    wave_sync();
+    {
+        double *ot = R.tb[cur ^ 1] + c0;
+        for (unsigned rr = lane; rr < unsigned(nx); rr += 64)
+            ot[rr] = rr < unsigned(nxm) ? n.t[rr] + b0[lds_row<CH>(rr)] : n.t[rr];
+    }
+    wave_sync();
 }
 
 // the run's spectrum, transform and mode factors in arithmetic R
