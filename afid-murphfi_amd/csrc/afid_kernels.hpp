@@ -1060,6 +1060,10 @@ template <typename RT>
 __device__ __forceinline__ void vxr_tile(const KP &p, int xcmax, int cjmax, int cimax) {
     extern __shared__ __align__(16) unsigned char smraw[];
     __shared__ long long coff[16 * 16];                 // coarse column offsets [ci][cj] (cimax, cjmax <= 16)
+    // per coarse x of the chunk: the row of its difference and the metric
+    // with the wall-ghost sign (0 outside -1..nx+1)
+    __shared__ int krow[128];
+    __shared__ double kfac[128];
     __shared__ RT yw[VXR_TJ][4], zw[VXR_TI][4];
     __shared__ int yo[VXR_TJ], zo[VXR_TI];
     RT *sm = reinterpret_cast<RT *>(smraw);
@@ -1102,35 +1106,40 @@ __device__ __forceinline__ void vxr_tile(const KP &p, int xcmax, int cjmax, int 
         const int R1 = min(R0 + 64, p.nxmr);
         const int x0 = p.xs_src[R0], XC = p.xs_src[R1 - 1] + 4 - x0;
         __syncthreads();
+        for (int xo = t; xo < XC; xo += 256) {
+            const int xc = x0 + xo;
+            const bool ok = xc >= 0 && xc <= p.nx;
+            const int k = !ok ? 0 : (xc == 0 ? 0 : (xc == p.nx ? p.nxm - 1 : xc - 1));
+            krow[xo] = k;
+            kfac[xo] = !ok ? 0.0 : (xc == 0 ? sS * p.udx3m[k] : (xc == p.nx ? sN * p.udx3m[k] : p.udx3m[k]));
+        }
+        __syncthreads();
         {
             // coarse gradient dvx/dx at reference x index xc with the wall
-            // ghosts (InterpVelMgrd.F90:37-51); zero outside -1..nx+1.
-            // The loads of VXR_BU items are issued before any is used: an
-            // item outside the block or the grid reads a valid address.
+            // ghosts (InterpVelMgrd.F90:37-51); zero outside -1..nx+1
+            // (kfac = sign * udx3m: the sign is +-1, so the product is
+            // exactly the signed difference quotient).  The loads of VXR_BU
+            // items are issued before any is used: an item outside the block
+            // reads a valid address.
             const int nB = XC * CJ * CI;
             Walk3 w(t, 256, XC, CJ);
             for (int q0 = t; q0 < nB; q0 += 256 * VXR_BU) {
-                double a0[VXR_BU], a1[VXR_BU], u[VXR_BU];
-                int dst[VXR_BU], xs[VXR_BU];
+                double a0[VXR_BU], a1[VXR_BU], f[VXR_BU];
+                int dst[VXR_BU];
 #pragma unroll
                 for (int r = 0; r < VXR_BU; ++r) {
-                    const int xc = x0 + w.a;
-                    const bool live = q0 + 256 * r < nB, ok = live && xc >= 0 && xc <= p.nx;
-                    const int k = !ok ? 0 : (xc == 0 ? 0 : (xc == p.nx ? p.nxm - 1 : xc - 1));
-                    const long long off = coff[min(w.c, CI - 1) * CJ + min(w.b, CJ - 1)];
-                    const double *col = p.vx + (ok ? off : 0);
-                    a0[r] = col[k]; a1[r] = col[k + 1]; u[r] = p.udx3m[k];
+                    const bool live = q0 + 256 * r < nB;
+                    const int xo = live ? w.a : 0;
+                    const int k = krow[xo];
+                    f[r] = kfac[xo];
+                    const double *col = p.vx + coff[min(w.c, CI - 1) * CJ + min(w.b, CJ - 1)];
+                    a0[r] = col[k]; a1[r] = col[k + 1];
                     dst[r] = live ? ix3(w.c, cjmax, w.b, xcmax, w.a) : -1;
-                    xs[r] = ok ? xc : -1;
                     w.next();
                 }
 #pragma unroll
                 for (int r = 0; r < VXR_BU; ++r)
-                    if (dst[r] >= 0) {
-                        const int xc = xs[r];
-                        const double d = (a1[r] - a0[r]) * u[r];
-                        B[dst[r]] = xc < 0 ? RT(0) : RT(xc == 0 ? sS * d : (xc == p.nx ? sN * d : d));
-                    }
+                    if (dst[r] >= 0) B[dst[r]] = RT(f[r] * (a1[r] - a0[r]));
             }
         }
         __syncthreads();
