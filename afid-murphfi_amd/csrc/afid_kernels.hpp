@@ -1354,8 +1354,10 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
 // k_vyz_tile term by term.  Workgroups of 4 wavefronts take 4 consecutive
 // lines of one x chunk; consecutive workgroups on an XCD take consecutive
 // line groups (shared coarse rows in its L2).
+// (FP32: at least 6 wavefronts per SIMD, 71 VGPRs; FP64 keeps its 94 VGPRs,
+// since the 80 that 6 wavefronts allow spill and cost 30% on C1)
 template <int AX, typename RT, bool TWO>
-__global__ void __launch_bounds__(256) k_vyz_wave(KP p, int nxchunk) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(RT) == 4 ? 6 : 1, 8))) k_vyz_wave(KP p, int nxchunk) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nS = AX == 1 ? p.nyr : p.nzr, nO = AX == 1 ? p.nzr : p.nyr;          // refined extents
     const int cS = AX == 1 ? p.ny : p.nz, cO = AX == 1 ? p.nz : p.ny;              // coarse extents
