@@ -1419,10 +1419,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
         k = !ok ? 0 : (xc == 0 ? 0 : (xc == p.nx ? p.nxm - 1 : xc - 1));
         sg = !ok ? 0.0 : (xc == 0 ? double(sSg) : (xc == p.nx ? double(sNg) : 1.0));
     };
+    // lanes past the chunk's XC coarse points are never gathered: they take
+    // the last point's row, so their loads stay on cache lines in use
+    const int XC = p.xs_src[R1 - 1] + 4 - x0;
     int k0, k1;
     double g0, g1;
-    xrow(x0 + lane, k0, g0);
-    xrow(x0 + 64 + lane, k1, g1);
+    xrow(x0 + min(lane, XC - 1), k0, g0);
+    xrow(x0 + min(64 + lane, XC - 1), k1, g1);
     const double dgrid = AX == 1 ? p.dy : p.dz;
     const RT dl = AX == 1 ? RT(1.0 / p.dyr) : RT(1.0 / p.dzr);
     int cs = sS[0] - 1;                       // unwrapped coarse scan index of the next X
