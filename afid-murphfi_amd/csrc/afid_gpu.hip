@@ -1492,7 +1492,7 @@ static int to_coarse(afid_ctx *c, CoarseJob jb, const CoarseTab &t, const char *
     jb.x_src = t.x_src; jb.y_src = t.y_src; jb.z_src = t.z_src;
     if (c->mixed) { jb.x_w = t.x_wf; jb.y_w = t.y_wf; jb.z_w = t.z_wf; }
     else { jb.x_w = t.x_w; jb.y_w = t.y_w; jb.z_w = t.z_w; }
-    const dim3 gr(unsigned((c->nxm + 63) / 64), unsigned((c->ny + SALC_TJ - 1) / SALC_TJ), unsigned((c->nz + SALC_TZ - 1) / SALC_TZ));
+    const dim3 gr(tile_grid((long long)((c->nxm + 63) / 64) * ((c->ny + SALC_TJ - 1) / SALC_TJ) * ((c->nz + SALC_TZ - 1) / SALC_TZ)));
     tstart(c, timer);
     if (c->mixed) hipLaunchKernelGGL(k_interp_coarse_tile<float>, gr, dim3(256), t.lds, c->stream, p, jb, t.xr, t.yr);
     else hipLaunchKernelGGL(k_interp_coarse_tile<double>, gr, dim3(256), t.lds, c->stream, p, jb, t.xr, t.yr);

@@ -329,6 +329,25 @@ __device__ __forceinline__ long long xcd_block(long long) {
     const long long per = gridDim.x / 8;
     return (b % 8) * per + b / 8;
 }
+// Tile grids: a 1-D grid over the tiles (tile = tx + nx_tiles * (ty +
+// ny_tiles * tz)), padded to a multiple of 8 workgroups.  Through
+// xcd_block each XCD takes a contiguous range of tiles, so neighbouring
+// tiles, which read the same halo cache lines, run on one L2 at about the
+// same time.  False for the padding workgroups.
+// (The vxr and tempr tiles keep a 2-D grid: with 1536 refined y columns
+// their rows of 96 tiles, a multiple of 8, already put z-neighbour tiles,
+// which share most of their coarse block, on one XCD; the 1-D map raised
+// their fetch.)
+__host__ __device__ __forceinline__ unsigned tile_grid(long long ntiles) { return unsigned((ntiles + 7) / 8 * 8); }
+__device__ __forceinline__ bool tile_3d(int nxt, int nyt, int nzt, int &tx, int &ty, int &tz) {
+    const long long t = xcd_block(0);
+    if (t >= (long long)nxt * nyt * nzt) return false;
+    tx = int(t % nxt);
+    const long long r = t / nxt;
+    ty = int(r % nyt);
+    tz = int(r / nyt);
+    return true;
+}
 // Column walk: a workgroup owns 4 consecutive y columns, one per
 // wavefront; workgroups form strips of STRIP_BLOCKS*4 columns in y, walked
 // along z, so the y and z neighbour columns a wavefront reads belong to its
@@ -1564,9 +1583,12 @@ __global__ void __launch_bounds__(256) k_interp_coarse_tile(KP p, CoarseJob jb, 
     const RT *yb_w = static_cast<const RT *>(jb.y_w), *zb_w = static_cast<const RT *>(jb.z_w);
     const int *yb_src = jb.y_src, *zb_src = jb.z_src;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, t = threadIdx.x;
-    const int K0 = blockIdx.x * 64, K1 = min(K0 + 64, p.nxm);
-    const int J0 = blockIdx.y * SALC_TJ, nj = min(SALC_TJ, p.ny - J0);
-    const int I0 = blockIdx.z * SALC_TZ, I1 = min(I0 + SALC_TZ, p.nz);
+    const int nxt = (p.nxm + 63) / 64;
+    int tx, ty, tz;
+    if (!tile_3d(nxt, (p.ny + SALC_TJ - 1) / SALC_TJ, (p.nz + SALC_TZ - 1) / SALC_TZ, tx, ty, tz)) return;
+    const int K0 = tx * 64, K1 = min(K0 + 64, p.nxm);
+    const int J0 = ty * SALC_TJ, nj = min(SALC_TJ, p.ny - J0);
+    const int I0 = tz * SALC_TZ, I1 = min(I0 + SALC_TZ, p.nz);
     const int x0 = x_src[K0], XR = x_src[K1 - 1] + 4 - x0;                  // reference refined x index
     const int y0 = yb_src[J0] - 1, YR = yb_src[J0 + nj - 1] - 1 + 4 - y0;   // 0-based refined y
     const int zfirst = zb_src[I0] - 1, zlast = zb_src[I1 - 1] - 1 + 3;      // 0-based refined z (unwrapped)
@@ -1646,7 +1668,7 @@ __global__ void __launch_bounds__(256) k_interp_coarse_tile(KP p, CoarseJob jb, 
         }
     }
     // salc(nx), phic(nx) stay zero (salinity.F90:463, phasefield.F90:476): written by the last x tile
-    if (jb.mode != CM_LAT && blockIdx.x == gridDim.x - 1 && lane == 0)
+    if (jb.mode != CM_LAT && tx == nxt - 1 && lane == 0)
         for (int h = 0; h < 2; ++h)
             if (rok[h])
                 for (int iz = I0; iz < I1; ++iz) {
