@@ -1445,7 +1445,7 @@ int afid_correct_pressure(afid_ctx *c) { NEED_SUBSTEP(c) NEED_FP64(c) return pro
 static int interp_vel(afid_ctx *c) {
     KP p = make_kp(c);
     int rc;
-    const dim3 gx(unsigned((c->nyr + VXR_TJ - 1) / VXR_TJ), unsigned((c->nzr + VXR_TI - 1) / VXR_TI));
+    const dim3 gx(tile_grid((long long)((c->nyr + VXR_TJ - 1) / VXR_TJ) * ((c->nzr + VXR_TI - 1) / VXR_TI)));
     const dim3 gy(unsigned((c->nxmr + 63) / 64), unsigned((c->nzr + VYZ_LINES - 1) / VYZ_LINES));
     const dim3 gz(unsigned((c->nxmr + 63) / 64), unsigned((c->nyr + VYZ_LINES - 1) / VYZ_LINES));
     tstart(c, "mgrd_vxr");

@@ -334,11 +334,22 @@ __device__ __forceinline__ long long xcd_block(long long) {
 // xcd_block each XCD takes a contiguous range of tiles, so neighbouring
 // tiles, which read the same halo cache lines, run on one L2 at about the
 // same time.  False for the padding workgroups.
-// (The vxr and tempr tiles keep a 2-D grid: with 1536 refined y columns
-// their rows of 96 tiles, a multiple of 8, already put z-neighbour tiles,
-// which share most of their coarse block, on one XCD; the 1-D map raised
-// their fetch.)
 __host__ __device__ __forceinline__ unsigned tile_grid(long long ntiles) { return unsigned((ntiles + 7) / 8 * 8); }
+// The vxr tiles (ny_tiles x nz_tiles) in groups of TILE_GY y tiles walked
+// z-fastest: y-neighbour tiles are 1 apart and z-neighbour tiles, which
+// share 3 of the 5 coarse planes of their block, TILE_GY apart, so both are
+// in flight on one XCD together.
+constexpr int TILE_GY = 4;
+__device__ __forceinline__ bool tile_yz_grouped(int nyt, int nzt, int &ty, int &tz) {
+    const long long t = xcd_block(0);
+    if (t >= (long long)nyt * nzt) return false;
+    const long long per = (long long)TILE_GY * nzt;
+    const int g = int(t / per), r = int(t - g * per);
+    const int w = min(TILE_GY, nyt - g * TILE_GY);      // the last group may be narrower
+    ty = g * TILE_GY + r % w;
+    tz = r / w;
+    return true;
+}
 __device__ __forceinline__ bool tile_3d(int nxt, int nyt, int nzt, int &tx, int &ty, int &tz) {
     const long long t = xcd_block(0);
     if (t >= (long long)nxt * nyt * nzt) return false;
@@ -1089,7 +1100,9 @@ __device__ __forceinline__ void vxr_tile(const KP &p, int xcmax, int cjmax, int 
     RT *vxr = sel<RT>(p.vxr, p.vxrf);
     const RT *xs_w = sel<RT>(p.xs_w, p.xs_wf), *ys_w = sel<RT>(p.ys_w, p.ys_wf), *zs_w = sel<RT>(p.zs_w, p.zs_wf);
     const RT *lxa = sel<RT>(p.lxa, p.lxaf);
-    const int J0 = blockIdx.x * VXR_TJ, I0 = blockIdx.y * VXR_TI;
+    int tj0, ti0;
+    if (!tile_yz_grouped((p.nyr + VXR_TJ - 1) / VXR_TJ, (p.nzr + VXR_TI - 1) / VXR_TI, tj0, ti0)) return;
+    const int J0 = tj0 * VXR_TJ, I0 = ti0 * VXR_TI;
     const int nj = min(VXR_TJ, p.nyr - J0), ni = min(VXR_TI, p.nzr - I0);
     const int cj0 = p.ys_src[J0] - 1, CJ = p.ys_src[J0 + nj - 1] - 1 + 4 - cj0;
     const int ci0 = p.zs_src[I0] - 1, CI = p.zs_src[I0 + ni - 1] - 1 + 4 - ci0;
