@@ -1386,8 +1386,9 @@ __global__ void __launch_bounds__(256) k_vyz_tile(KP p, int xcmax, int csmax, in
 // k_vyz_tile term by term.  Workgroups of 4 wavefronts take 4 consecutive
 // lines of one x chunk; consecutive workgroups on an XCD take consecutive
 // line groups (shared coarse rows in its L2).
-// (FP32: at least 6 wavefronts per SIMD, 71 VGPRs; FP64 keeps its 94 VGPRs,
-// since the 80 that 6 wavefronts allow spill and cost 30% on C1)
+// (FP32: at least 6 wavefronts per SIMD, 71 VGPRs, which make 7; FP64 keeps
+// its 94 VGPRs, since the 80 that 6 wavefronts allow spill and cost 30% on a
+// 768 x 1536^2 refined grid; 8 wavefronts spill in FP32 too)
 template <int AX, typename RT, bool TWO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(RT) == 4 ? 6 : 1, 8))) k_vyz_wave(KP p, int nxchunk) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
